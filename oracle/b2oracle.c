@@ -1,0 +1,1829 @@
+/*
+ * b2oracle.c -- FP64 CPU restatement of pypybox2d's per-step contact pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle the GPU library is
+ * checked against; it is loaded by tests/, by __graft_entry__.smoke() as the
+ * checker, and by bench.py's cpu_baseline leg.  No product code path links,
+ * loads or calls it.
+ *
+ * It follows the pure-Python flavour of the reference (pypybox2d/common.py
+ * PyVec2/PyMat22/PyTransform/PyAABB) operation by operation, so that with the
+ * same glibc it reproduces CPython's binary64 results bit for bit:
+ *   - `x ** 2` is CPython float_pow -> libm pow(|x|, 2.0) (not x*x),
+ *   - `min(a, b)` / `max(a, b)` keep the first argument on ties,
+ *   - expressions are evaluated left to right with no FMA contraction
+ *     (build with -ffp-contract=off -fno-builtin).
+ * Reference quirks reproduced (see the cited lines):
+ *   Q2  Polygon._centroid stays as given (it is (0,0) for every polygon the
+ *       reference builds, shapes.py:244,482).
+ *   Q4  Transform.mul_t(Transform) does not rotate the translation
+ *       (common.py:671-673), used by the edge/polygon collider.
+ *   Q6  ill-conditioned 2-point block: the velocity constraint keeps only its
+ *       first point; the position constraint keeps both (contact.py:487-496).
+ *   Q7  sleep test reads the stale loop variable (island.py:163-179).
+ *   Q9  edge/polygon FACE_B ids are (type_b, type_a, index_b, index_a)
+ *       (collision.py:532).
+ * Contact-creation order (Q1): the reference iterates an identity-hashed set
+ * (broadphase.py:98-109); the oracle emits new pairs in move-buffer order with
+ * ascending partner index, or, when a hint is installed, in the hint's order
+ * (used to replay a recorded reference run).
+ *
+ * Not supported (reported through the error word): sensors, loop shapes,
+ * joints, controllers, listeners, custom filters, continuous physics.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/b2g_layout.h"
+
+#define ORC_EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------ */
+/* settings.py:30-123                                                  */
+/* ------------------------------------------------------------------ */
+static const double PI_ = 3.141592653589793;
+#define EPSILON DBL_EPSILON
+#define MAX_FLOAT DBL_MAX
+static const double AABB_EXTENSION = 0.1;   /* settings.py:52 */
+static const double AABB_MULTIPLIER = 2.0;  /* settings.py:57 */
+static const double LINEAR_SLOP = 0.005;    /* settings.py:61 */
+static const double VELOCITY_THRESHOLD = 1.0;
+static const double MAX_LINEAR_CORRECTION = 0.2;
+static const double MAX_TRANSLATION = 2.0;
+static const double BAUMGARTE = 0.2;
+static const double TIME_TO_SLEEP = 0.5;
+static double ANGULAR_SLOP, POLYGON_RADIUS, MAX_ROTATION, MAX_ROTATION_SQR,
+    MAX_TRANSLATION_SQR, LINEAR_SLEEP_TOLERANCE_SQR, ANGULAR_SLEEP_TOLERANCE_SQR,
+    EPSILON_SQR;
+
+/* CPython float_pow(x, 2) (Objects/floatobject.c): 0 -> 0.0, |x| -> pow */
+static double (*volatile pow_fn)(double, double) = pow;
+static double pysq(double x) {
+  if (x != x) return x;
+  if (x == 0.0) return 0.0;
+  if (x < 0.0) x = -x;
+  if (x == 1.0) return 1.0;
+  return pow_fn(x, 2.0);
+}
+
+static int g_init = 0;
+static double g_norm_eps = EPSILON; /* common.py:216-222 (C flavour: FLT_EPSILON) */
+static void init_consts(void) {
+  if (g_init) return;
+  ANGULAR_SLOP = 2.0 / 180.0 * PI_;        /* settings.py:65 */
+  POLYGON_RADIUS = 2.0 * LINEAR_SLOP;      /* settings.py:70 */
+  MAX_TRANSLATION_SQR = pysq(MAX_TRANSLATION);
+  MAX_ROTATION = 0.5 * PI_;                /* settings.py:103 */
+  MAX_ROTATION_SQR = pysq(MAX_ROTATION);
+  LINEAR_SLEEP_TOLERANCE_SQR = pysq(0.01); /* settings.py:118-119 */
+  ANGULAR_SLEEP_TOLERANCE_SQR = pysq(2.0 / 180.0 * PI_);
+  EPSILON_SQR = pysq(EPSILON);             /* settings.py:39 */
+  g_init = 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* common.py PyVec2 / PyMat22 / PyTransform / PyAABB                   */
+/* ------------------------------------------------------------------ */
+typedef struct { double x, y; } V2;
+typedef struct { double c1x, c1y, c2x, c2y; } M22; /* column-major, common.py:360 */
+typedef struct { V2 p; M22 R; } XF;
+typedef struct { V2 lo, hi; } AABB;
+
+static inline V2 v2(double x, double y) { V2 r; r.x = x; r.y = y; return r; }
+static inline V2 add(V2 a, V2 b) { return v2(a.x + b.x, a.y + b.y); }
+static inline V2 sub(V2 a, V2 b) { return v2(a.x - b.x, a.y - b.y); }
+static inline V2 neg(V2 a) { return v2(-a.x, -a.y); }
+/* Vec2.__mul__(scalar) = Vec2(value*x, value*y) (common.py:113-117) */
+static inline V2 scale(double s, V2 a) { return v2(s * a.x, s * a.y); }
+static inline double dot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+static inline double cross(V2 a, V2 b) { return a.x * b.y - a.y * b.x; }
+/* Vec2.cross(scalar) = (s*y, -s*x) (common.py:200-204) */
+static inline V2 cross_vs(V2 a, double s) { return v2(s * a.y, -s * a.x); }
+/* scalar_cross(s, v) = v._scalar_cross(s) = (-s*y, s*x) (common.py:188-195) */
+static inline V2 cross_sv(double s, V2 a) { return v2(-s * a.y, s * a.x); }
+static inline double length_(V2 a) { return sqrt(pysq(a.x) + pysq(a.y)); }
+static inline double length_sq(V2 a) { return pysq(a.x) + pysq(a.y); }
+static inline V2 normalized(V2 a) {
+  double l = length_(a);
+  if (l < g_norm_eps) return a;
+  double il = 1.0 / l;
+  return v2(a.x * il, a.y * il);
+}
+static inline double pmin(double a, double b) { return (b < a) ? b : a; } /* min(a, b) */
+static inline double pmax(double a, double b) { return (b > a) ? b : a; } /* max(a, b) */
+static inline double pclamp(double v, double lo, double hi) { return pmax(lo, pmin(v, hi)); }
+static inline V2 vmin(V2 a, V2 b) { return v2(pmin(a.x, b.x), pmin(a.y, b.y)); }
+static inline V2 vmax(V2 a, V2 b) { return v2(pmax(a.x, b.x), pmax(a.y, b.y)); }
+
+/* PyMat22.angle setter (common.py:492-500) */
+static inline M22 rot(double angle) {
+  double c = cos(angle), s = sin(angle);
+  M22 m; m.c1x = c; m.c2x = -s; m.c1y = s; m.c2y = c;
+  return m;
+}
+static inline V2 mulmv(M22 m, V2 v) { /* common.py:405-412 */
+  return v2(m.c1x * v.x + m.c2x * v.y, m.c1y * v.x + m.c2y * v.y);
+}
+static inline V2 mulmtv(M22 m, V2 v) { /* common.py:463-468: (col1.dot(v), col2.dot(v)) */
+  return v2(m.c1x * v.x + m.c1y * v.y, m.c2x * v.x + m.c2y * v.y);
+}
+static inline M22 mulmtm(M22 a, M22 b) { /* common.py:478-482 */
+  M22 r;
+  r.c1x = a.c1x * b.c1x + a.c1y * b.c1y;
+  r.c1y = a.c2x * b.c1x + a.c2y * b.c1y;
+  r.c2x = a.c1x * b.c2x + a.c1y * b.c2y;
+  r.c2y = a.c2x * b.c2x + a.c2y * b.c2y;
+  return r;
+}
+/* PyTransform.__mul__ (common.py:677-685): (p.x + c1.x*o0) + c2.x*o1 */
+static inline V2 xmul(XF t, V2 o) {
+  return v2(t.p.x + t.R.c1x * o.x + t.R.c2x * o.y, t.p.y + t.R.c1y * o.x + t.R.c2y * o.y);
+}
+/* PyTransform.mul_t(vector) = R.mul_t(v - p) (common.py:666-669) */
+static inline V2 xmult(XF t, V2 v) { return mulmtv(t.R, sub(v, t.p)); }
+/* PyTransform.mul_t(transform) (common.py:670-673): quirk Q4 */
+static inline XF xmult_xf(XF a, XF b) {
+  XF r; r.p = sub(b.p, a.p); r.R = mulmtm(a.R, b.R); return r;
+}
+/* Transform(angle=a); position = c - R*lc  (body.py:906-908, contact.py:437-440) */
+static inline XF xf_from(V2 c, double a, V2 lc) {
+  XF t; t.R = rot(a); t.p = sub(c, mulmv(t.R, lc)); return t;
+}
+/* PyAABB.test_overlap (common.py:830-839) */
+static inline int aabb_overlap(const AABB* a, const AABB* b) {
+  V2 d1 = sub(b->lo, a->hi), d2 = sub(a->lo, b->hi);
+  if (d1.x > 0.0 || d1.y > 0.0) return 0;
+  if (d2.x > 0.0 || d2.y > 0.0) return 0;
+  return 1;
+}
+/* PyAABB.contains (common.py:863-868) */
+static inline int aabb_contains(const AABB* a, const AABB* b) {
+  return (a->lo.x <= b->lo.x) && (a->lo.y <= b->lo.y) && (b->hi.x <= a->hi.x) && (b->hi.y <= a->hi.y);
+}
+static inline AABB aabb_union(AABB a, AABB b) { /* PyAABB.__add__ (common.py:857-861) */
+  AABB r; r.lo = vmin(a.lo, b.lo); r.hi = vmax(a.hi, b.hi); return r;
+}
+/* Python list.index(min(xs)) / list.index(max(xs)): first index of the extreme */
+static int argmin_first(const double* xs, int n) {
+  double m = xs[0];
+  for (int i = 1; i < n; ++i) if (xs[i] < m) m = xs[i];
+  for (int i = 0; i < n; ++i) if (xs[i] == m) return i;
+  return 0;
+}
+static int argmax_first(const double* xs, int n) {
+  double m = xs[0];
+  for (int i = 1; i < n; ++i) if (xs[i] > m) m = xs[i];
+  for (int i = 0; i < n; ++i) if (xs[i] == m) return i;
+  return 0;
+}
+static inline int pymod(int a, int n) { int r = a % n; return r < 0 ? r + n : r; }
+
+/* ------------------------------------------------------------------ */
+/* world state                                                         */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int type;
+  double radius;
+  int n;
+  V2 v[B2G_MAX_POLY_VERTS], nrm[B2G_MAX_POLY_VERTS];
+  V2 centroid;
+  V2 p;                  /* circle position */
+  V2 e0, e1, e2, e3;     /* edge vertex0..vertex3 */
+  int has0, has3;
+} Shape;
+
+typedef struct {
+  int body;
+  Shape sh;
+  double friction, restitution, density;
+  int category, mask, group, sensor;
+  AABB fat;
+} Fix;
+
+typedef struct {
+  int type, awake, allow_sleep, fixed_rot, active, bullet;
+  int island_flag, island_index, island_id;
+  XF xf;
+  V2 lc, c0, c;
+  double a0, a;
+  V2 v; double w;
+  V2 force; double torque;
+  double mass, inv_mass, I, invI;
+  double gscale, ldamp, adamp, sleep_time;
+} Body;
+
+typedef struct { V2 lp; double ni, ti; int id[4]; } MPoint;
+typedef struct { MPoint pts[2]; V2 ln, lp; int type, pc; } Manifold;
+
+typedef struct {
+  int fa, fb;
+  double friction, restitution;
+  Manifold m;
+  int touching, enabled, filter, island_flag;
+} Contact;
+
+typedef struct {
+  V2 gravity;
+  double inv_dt0;
+  int allow_sleep, warm_starting, new_fixture, keep_forces;
+  int nb, nf, nc, ccap, nmove, mcap;
+  Body* b;
+  Fix* f;
+  Contact* c;
+  int* move;
+  int* bfix_off; /* per-body fixture CSR (fixtures never change during stepping) */
+  int* bfix;
+  /* replay hint: unordered fixture-pair keys in reference creation order */
+  int hint_n;
+  int64_t* hint;
+  int err;
+} World;
+
+#define ORC_E_UNSUPPORTED 1
+#define ORC_E_HINT 2
+#define ORC_E_ALLOC 4
+
+typedef struct { int nw; World* w; } Batch;
+
+static void mp_id_unpack(int32_t packed, int id[4]) {
+  uint32_t u = (uint32_t)packed;
+  id[0] = (int)(u & 255u); id[1] = (int)((u >> 8) & 255u);
+  id[2] = (int)((u >> 16) & 255u); id[3] = (int)((u >> 24) & 255u);
+}
+static int32_t mp_id_pack(const int id[4]) {
+  uint32_t u = (uint32_t)(id[0] & 255) | ((uint32_t)(id[1] & 255) << 8) |
+               ((uint32_t)(id[2] & 255) << 16) | ((uint32_t)(id[3] & 255) << 24);
+  return (int32_t)u;
+}
+static inline void set_id(int id[4], int a, int b, int c, int d) { id[0] = a; id[1] = b; id[2] = c; id[3] = d; }
+
+/* ------------------------------------------------------------------ */
+/* shapes.py compute_aabb                                              */
+/* ------------------------------------------------------------------ */
+static AABB compute_aabb(const Shape* s, XF xf) {
+  AABB r;
+  if (s->type == B2G_SHAPE_CIRCLE) { /* shapes.py:178-189: p = xf.p + R*position */
+    V2 p = add(xf.p, mulmv(xf.R, s->p));
+    V2 rad = v2(s->radius, s->radius);
+    r.lo = sub(p, rad); r.hi = add(p, rad);
+  } else if (s->type == B2G_SHAPE_POLYGON) { /* shapes.py:316-334 */
+    V2 lower = xmul(xf, s->v[0]);
+    V2 upper = lower;
+    for (int i = 0; i < s->n; ++i) {
+      V2 v = xmul(xf, s->v[i]);
+      lower = vmin(lower, v);
+      upper = vmax(upper, v);
+    }
+    V2 rr = v2(s->radius, s->radius);
+    r.lo = sub(lower, rr); r.hi = add(upper, rr);
+  } else { /* edge, shapes.py:702-718 */
+    V2 a = xmul(xf, s->e1), b = xmul(xf, s->e2);
+    V2 rr = v2(s->radius, s->radius);
+    r.lo = sub(vmin(a, b), rr); r.hi = add(vmax(a, b), rr);
+  }
+  return r;
+}
+
+/* ------------------------------------------------------------------ */
+/* collision.py / collision_util.py                                    */
+/* ------------------------------------------------------------------ */
+typedef struct { V2 v; int id[4]; } ClipV; /* contact_util.py:97-118 */
+
+static void collide_circles(Manifold* m, const Shape* A, XF xfa, const Shape* B, XF xfb) {
+  /* collision.py:46-66 */
+  m->pc = 0;
+  V2 pa = xmul(xfa, A->p), pb = xmul(xfb, B->p);
+  V2 d = sub(pb, pa);
+  double dist_sqr = length_sq(d);
+  double radius = A->radius + B->radius;
+  if (dist_sqr > pysq(radius)) return;
+  m->type = B2G_M_CIRCLES;
+  m->lp = A->p;
+  m->ln = v2(0.0, 0.0);
+  m->pc = 1;
+  m->pts[0].lp = B->p;
+  set_id(m->pts[0].id, 0, 0, 0, 0);
+}
+
+static void collide_polygon_circle(Manifold* m, const Shape* P, XF xfa, const Shape* C, XF xfb) {
+  /* collision.py:68-141 */
+  m->pc = 0;
+  V2 c = xmul(xfb, C->p);
+  V2 cl = xmult(xfa, c);
+  int normal_index = 0;
+  double separation = -MAX_FLOAT;
+  double radius = P->radius + C->radius;
+  int n = P->n;
+  for (int i = 0; i < n; ++i) {
+    double s = dot(P->nrm[i], sub(cl, P->v[i]));
+    if (s > radius) return;
+    if (s > separation) { separation = s; normal_index = i; }
+  }
+  int i1 = normal_index, i2 = (i1 + 1) % n;
+  V2 v1 = P->v[i1], v2_ = P->v[i2];
+  if (separation < EPSILON) {
+    m->pc = 1;
+    m->type = B2G_M_FACE_A;
+    m->ln = P->nrm[normal_index];
+    m->lp = scale(0.5, add(v1, v2_));
+    m->pts[0].lp = C->p;
+    set_id(m->pts[0].id, 0, 0, 0, 0);
+    return;
+  }
+  double u1 = dot(sub(cl, v1), sub(v2_, v1));
+  double u2 = dot(sub(cl, v2_), sub(v1, v2_));
+  if (u1 < 0.0) {
+    V2 d = sub(cl, v1);
+    if (dot(d, d) > pysq(radius)) return;
+    m->ln = normalized(sub(cl, v1));
+    m->lp = v1;
+  } else if (u2 < 0.0) {
+    V2 d = sub(cl, v2_);
+    if (dot(d, d) > pysq(radius)) return;
+    m->ln = normalized(sub(cl, v2_));
+    m->lp = v2_;
+  } else {
+    V2 fc = scale(0.5, add(v1, v2_));
+    separation = dot(sub(cl, fc), P->nrm[i1]);
+    if (separation > radius) return;
+    m->ln = P->nrm[i1];
+    m->lp = fc;
+  }
+  m->pc = 1;
+  m->type = B2G_M_FACE_A;
+  m->pts[0].lp = C->p;
+  set_id(m->pts[0].id, 0, 0, 0, 0);
+}
+
+/* collision_util.py:31-52 */
+static double edge_separation(const Shape* p1, XF xf1, int edge1, const Shape* p2, XF xf2) {
+  V2 n1w = mulmv(xf1.R, p1->nrm[edge1]);
+  V2 n1 = mulmtv(xf2.R, n1w);
+  double dots[B2G_MAX_POLY_VERTS];
+  for (int i = 0; i < p2->n; ++i) dots[i] = dot(p2->v[i], n1);
+  int index = argmin_first(dots, p2->n);
+  V2 v1 = xmul(xf1, p1->v[edge1]);
+  V2 v2_ = xmul(xf2, p2->v[index]);
+  return dot(sub(v2_, v1), n1w);
+}
+
+/* collision_util.py:54-109 (hill climbing, quirk Q2/Q3) */
+static double max_separation(const Shape* p1, XF xf1, const Shape* p2, XF xf2, int* edge_out) {
+  int count1 = p1->n;
+  V2 d = sub(xmul(xf2, p2->centroid), xmul(xf1, p1->centroid));
+  V2 dl = mulmtv(xf1.R, d);
+  double dots[B2G_MAX_POLY_VERTS];
+  for (int i = 0; i < count1; ++i) dots[i] = dot(p1->nrm[i], dl);
+  int edge = argmax_first(dots, count1);
+  double s = edge_separation(p1, xf1, edge, p2, xf2);
+  int prev_edge = pymod(edge - 1, count1);
+  double s_prev = edge_separation(p1, xf1, prev_edge, p2, xf2);
+  int next_edge = pymod(edge + 1, count1);
+  double s_next = edge_separation(p1, xf1, next_edge, p2, xf2);
+  int increment, best_edge;
+  double best_sep;
+  if (s_prev > s && s_prev > s_next) {
+    increment = -1; best_edge = prev_edge; best_sep = s_prev;
+  } else if (s_next > s) {
+    increment = 1; best_edge = next_edge; best_sep = s_next;
+  } else {
+    *edge_out = edge;
+    return s;
+  }
+  for (;;) {
+    edge = increment == -1 ? pymod(best_edge - 1, count1) : pymod(best_edge + 1, count1);
+    s = edge_separation(p1, xf1, edge, p2, xf2);
+    if (s > best_sep) { best_edge = edge; best_sep = s; }
+    else break;
+  }
+  *edge_out = best_edge;
+  return best_sep;
+}
+
+/* collision_util.py:111-133 */
+static void find_incident_edge(ClipV c[2], const Shape* p1, XF xf1, int edge1, const Shape* p2, XF xf2) {
+  V2 n1 = mulmtv(xf2.R, mulmv(xf1.R, p1->nrm[edge1]));
+  double dots[B2G_MAX_POLY_VERTS];
+  for (int i = 0; i < p2->n; ++i) dots[i] = dot(n1, p2->nrm[i]);
+  int i1 = argmin_first(dots, p2->n);
+  int i2 = (i1 + 1) % p2->n;
+  c[0].v = xmul(xf2, p2->v[i1]); set_id(c[0].id, edge1, i1, 1, 0);
+  c[1].v = xmul(xf2, p2->v[i2]); set_id(c[1].id, edge1, i2, 1, 0);
+}
+
+/* collision_util.py:135-164 (Sutherland-Hodgman) */
+static int clip_segment_to_line(ClipV out[3], const ClipV in[2], V2 normal, double offset, int via) {
+  int n = 0;
+  double d0 = dot(normal, in[0].v) - offset;
+  double d1 = dot(normal, in[1].v) - offset;
+  if (d0 <= 0.0) out[n++] = in[0];
+  if (d1 <= 0.0) out[n++] = in[1];
+  if (d0 * d1 < 0.0) {
+    double interp = d0 / (d0 - d1);
+    out[n].v = add(in[0].v, scale(interp, sub(in[1].v, in[0].v)));
+    set_id(out[n].id, via, in[0].id[1], 0, 1);
+    ++n;
+  }
+  return n;
+}
+
+static const double RELATIVE_TOL = 0.98, ABSOLUTE_TOL = 0.001; /* collision.py:36-37 */
+
+static void collide_polygons(Manifold* m, const Shape* A, XF xfa, const Shape* B, XF xfb) {
+  /* collision.py:144-241 */
+  m->pc = 0;
+  double total_radius = A->radius + B->radius;
+  int edge_a, edge_b;
+  double sa = max_separation(A, xfa, B, xfb, &edge_a);
+  if (sa > total_radius) return;
+  double sb = max_separation(B, xfb, A, xfa, &edge_b);
+  if (sb > total_radius) return;
+  const Shape *p1, *p2;
+  XF xf1, xf2;
+  int edge1, flip;
+  if (sb > RELATIVE_TOL * sa + ABSOLUTE_TOL) {
+    p1 = B; p2 = A; xf1 = xfb; xf2 = xfa; edge1 = edge_b; m->type = B2G_M_FACE_B; flip = 1;
+  } else {
+    p1 = A; p2 = B; xf1 = xfa; xf2 = xfb; edge1 = edge_a; m->type = B2G_M_FACE_A; flip = 0;
+  }
+  ClipV inc[2];
+  find_incident_edge(inc, p1, xf1, edge1, p2, xf2);
+  int iv1 = edge1, iv2 = (edge1 + 1) % p1->n;
+  V2 v11 = p1->v[iv1], v12 = p1->v[iv2];
+  V2 ltan = normalized(sub(v12, v11));
+  V2 local_normal = cross_vs(ltan, 1.0);
+  V2 planepoint = scale(0.5, add(v11, v12));
+  V2 tangent = mulmv(xf1.R, ltan);
+  V2 normal = cross_vs(tangent, 1.0);
+  v11 = xmul(xf1, v11);
+  v12 = xmul(xf1, v12);
+  double front_offset = dot(normal, v11);
+  double side1 = -dot(tangent, v11) + total_radius;
+  double side2 = dot(tangent, v12) + total_radius;
+  ClipV cp1[3], cp2[3];
+  if (clip_segment_to_line(cp1, inc, neg(tangent), side1, iv1) < 2) return;
+  if (clip_segment_to_line(cp2, cp1, tangent, side2, iv2) < 2) return;
+  m->ln = local_normal;
+  m->lp = planepoint;
+  int pc = 0;
+  for (int i = 0; i < 2; ++i) {
+    double separation = dot(normal, cp2[i].v) - front_offset;
+    if (separation <= total_radius) {
+      MPoint* cp = &m->pts[pc];
+      cp->lp = xmult(xf2, cp2[i].v);
+      if (flip) set_id(cp->id, cp2[i].id[1], cp2[i].id[0], cp2[i].id[3], cp2[i].id[2]);
+      else set_id(cp->id, cp2[i].id[0], cp2[i].id[1], cp2[i].id[2], cp2[i].id[3]);
+      ++pc;
+    }
+  }
+  m->pc = pc;
+}
+
+static void collide_edge_circle(Manifold* m, const Shape* E, XF xfa, const Shape* C, XF xfb) {
+  /* collision.py:246-353 */
+  m->pc = 0;
+  V2 Q = xmult(xfa, xmul(xfb, C->p));
+  V2 A = E->e1, B = E->e2;
+  V2 e = sub(B, A);
+  double u = dot(e, sub(B, Q));
+  double v = dot(e, sub(Q, A));
+  double radius = E->radius + C->radius;
+  if (v <= 0.0) {
+    V2 P = A, d = sub(Q, P);
+    double dd = dot(d, d);
+    if (dd > pysq(radius)) return;
+    if (E->has0) {
+      V2 A1 = E->e0, B1 = A;
+      V2 e1 = sub(B1, A1);
+      double u1 = dot(e1, sub(B1, Q));
+      if (u1 > 0.0) return;
+    }
+    m->pc = 1; m->type = B2G_M_CIRCLES; m->ln = v2(0.0, 0.0); m->lp = P;
+    set_id(m->pts[0].id, 0, 0, 0, 0);
+    m->pts[0].lp = C->p;
+    return;
+  }
+  if (u <= 0.0) {
+    V2 P = B, d = sub(Q, P);
+    double dd = dot(d, d);
+    if (dd > pysq(radius)) return;
+    if (E->has3) {
+      V2 B2 = E->e3, A2 = B;
+      V2 e2 = sub(B2, A2);
+      double vv2 = dot(e2, sub(Q, A2));
+      if (vv2 > 0.0) return;
+    }
+    m->pc = 1; m->type = B2G_M_CIRCLES; m->ln = v2(0.0, 0.0); m->lp = P;
+    set_id(m->pts[0].id, 1, 0, 0, 0);
+    m->pts[0].lp = C->p;
+    return;
+  }
+  double den = dot(e, e);
+  V2 P = scale(1.0 / den, add(scale(u, A), scale(v, B)));
+  V2 d = sub(Q, P);
+  double dd = dot(d, d);
+  if (dd > pysq(radius)) return;
+  V2 n = v2(-e.y, e.x);
+  if (dot(n, sub(Q, A)) < 0.0) n = v2(-n.x, -n.y);
+  n = normalized(n);
+  m->pc = 1; m->type = B2G_M_FACE_A; m->ln = n; m->lp = A;
+  set_id(m->pts[0].id, 0, 0, 1, 0);
+  m->pts[0].lp = C->p;
+}
+
+/* EPCollider, collision.py:355-690 */
+enum { EP_UNKNOWN = 0, EP_EDGE_A = 1, EP_EDGE_B = 2 };
+typedef struct { int type, index; double sep; } EPAxis;
+typedef struct { int n; V2 v[B2G_MAX_POLY_VERTS], nrm[B2G_MAX_POLY_VERTS]; V2 centroid; } EPProxy;
+typedef struct {
+  XF xf;
+  V2 v0, v1, v2, v3, normal;
+  int has0, has3;
+  EPProxy pa, pb;
+  double radius;
+  V2 l11, l12, l21, l22;
+} EPC;
+
+static void ep_adjacency(EPC* c) { /* collision.py:541-604 */
+  V2 v0 = c->v0, v1 = c->v1, vv2 = c->v2, v3 = c->v3;
+  V2 cb = c->pb.centroid;
+  if (c->has0) {
+    V2 e0 = sub(v1, v0), e1 = sub(vv2, v1);
+    V2 n0 = normalized(v2(e0.y, -e0.x)), n1 = normalized(v2(e1.y, -e1.x));
+    int convex = cross(n0, n1) >= 0.0;
+    int front0 = dot(n0, sub(cb, v0)) >= 0.0;
+    int front1 = dot(n1, sub(cb, v1)) >= 0.0;
+    if (convex) {
+      if (front0 || front1) { c->l11 = n1; c->l12 = n0; }
+      else { c->l11 = neg(n1); c->l12 = neg(n0); }
+    } else {
+      if (front0 && front1) { c->l11 = n0; c->l12 = n1; }
+      else { c->l11 = neg(n0); c->l12 = neg(n1); }
+    }
+  } else {
+    c->l11 = v2(0.0, 0.0); c->l12 = v2(0.0, 0.0);
+  }
+  if (c->has3) {
+    V2 e1 = sub(vv2, v1), e2 = sub(v3, vv2);
+    V2 n1 = normalized(v2(e1.y, -e1.x)), n2 = normalized(v2(e2.y, -e2.x));
+    int convex = cross(n1, n2) >= 0.0;
+    int front1 = dot(n1, sub(cb, v1)) >= 0.0;
+    int front2 = dot(n2, sub(cb, vv2)) >= 0.0;
+    if (convex) {
+      if (front1 || front2) { c->l21 = n2; c->l22 = n1; }
+      else { c->l21 = neg(n2); c->l22 = neg(n1); }
+    } else {
+      if (front1 && front2) { c->l21 = n1; c->l22 = n2; }
+      else { c->l21 = neg(n1); c->l22 = neg(n2); }
+    }
+  } else {
+    c->l21 = v2(0.0, 0.0); c->l22 = v2(0.0, 0.0);
+  }
+}
+
+static inline int ep_valid(const EPC* c, V2 n) {
+  int valid1 = (cross(n, c->l11) >= -ANGULAR_SLOP) && (cross(c->l12, n) >= -ANGULAR_SLOP);
+  int valid2 = (cross(n, c->l21) >= -ANGULAR_SLOP) && (cross(c->l22, n) >= -ANGULAR_SLOP);
+  return valid1 && valid2;
+}
+
+static EPAxis ep_edge_separation(const EPC* c) { /* collision.py:606-635 */
+  EPAxis best; best.type = EP_UNKNOWN; best.index = -1; best.sep = -MAX_FLOAT;
+  V2 normals[2] = {c->normal, neg(c->normal)};
+  for (int i = 0; i < 2; ++i) {
+    V2 n = normals[i];
+    if (!ep_valid(c, n)) continue;
+    EPAxis axis; axis.type = EP_EDGE_A; axis.index = i;
+    double m = dot(n, sub(c->pb.v[0], c->v1));
+    for (int k = 1; k < c->pb.n; ++k) {
+      double d = dot(n, sub(c->pb.v[k], c->v1));
+      if (d < m) m = d;
+    }
+    axis.sep = m;
+    if (axis.sep > c->radius) return axis;
+    if (axis.sep > best.sep) best = axis;
+  }
+  return best;
+}
+
+static EPAxis ep_polygon_separation(const EPC* c) { /* collision.py:637-666, quirk Q10 */
+  EPAxis axis; axis.type = EP_UNKNOWN; axis.index = -1; axis.sep = -MAX_FLOAT;
+  for (int i = 0; i < c->pb.n; ++i) {
+    V2 n = neg(c->pb.nrm[i]);
+    if (!ep_valid(c, n)) continue;
+    double s1 = dot(n, sub(c->pb.v[i], c->v1));
+    double s2 = dot(n, sub(c->pb.v[i], c->v2));
+    double s = pmin(s1, s2);
+    if (s > c->radius) { axis.type = EP_EDGE_B; axis.index = i; axis.sep = s; }
+    if (s > axis.sep) { axis.type = EP_EDGE_B; axis.index = i; axis.sep = s; }
+  }
+  return axis;
+}
+
+static void ep_find_incident(ClipV out[2], const EPProxy* p1, int edge1, const EPProxy* p2) {
+  /* collision.py:668-690 */
+  V2 n1 = p1->nrm[edge1];
+  double dots[B2G_MAX_POLY_VERTS];
+  for (int i = 0; i < p2->n; ++i) dots[i] = dot(n1, p2->nrm[i]);
+  int i1 = argmin_first(dots, p2->n);
+  int i2 = (i1 + 1) % p2->n;
+  out[0].v = p2->v[i1]; set_id(out[0].id, edge1, i1, 1, 0);
+  out[1].v = p2->v[i2]; set_id(out[1].id, edge1, i2, 1, 0);
+}
+
+static void collide_edge_polygon(Manifold* m, const Shape* E, XF xfa, const Shape* P, XF xfb) {
+  EPC c;
+  c.xf = xmult_xf(xfa, xfb); /* collision.py:410 (Q4) */
+  c.has0 = E->has0; c.has3 = E->has3;
+  c.v0 = E->e0; c.v1 = E->e1; c.v2 = E->e2; c.v3 = E->e3;
+  V2 e = sub(c.v2, c.v1);
+  c.normal = normalized(v2(e.y, -e.x));
+  c.pa.n = 2;
+  c.pa.v[0] = c.v1; c.pa.v[1] = c.v2;
+  c.pa.nrm[0] = c.normal; c.pa.nrm[1] = neg(c.normal);
+  c.pa.centroid = scale(0.5, add(c.v1, c.v2));
+  c.pb.centroid = xmul(c.xf, P->centroid);
+  c.pb.n = P->n;
+  for (int i = 0; i < P->n; ++i) {
+    c.pb.v[i] = xmul(c.xf, P->v[i]);
+    c.pb.nrm[i] = mulmv(c.xf.R, P->nrm[i]);
+  }
+  c.radius = 2.0 * POLYGON_RADIUS;
+
+  /* EPCollider.collide, collision.py:442-535 */
+  m->pc = 0;
+  ep_adjacency(&c);
+  EPAxis edge_axis = ep_edge_separation(&c);
+  if (edge_axis.type == EP_UNKNOWN) return;
+  if (edge_axis.sep > c.radius) return;
+  EPAxis poly_axis = ep_polygon_separation(&c);
+  if (poly_axis.type != EP_UNKNOWN && poly_axis.sep > c.radius) return;
+  EPAxis primary;
+  if (poly_axis.type == EP_UNKNOWN) primary = edge_axis;
+  else if (poly_axis.sep > RELATIVE_TOL * edge_axis.sep + ABSOLUTE_TOL) primary = poly_axis;
+  else primary = edge_axis;
+  const EPProxy *p1, *p2;
+  if (primary.type == EP_EDGE_A) { p1 = &c.pa; p2 = &c.pb; m->type = B2G_M_FACE_A; }
+  else { p1 = &c.pb; p2 = &c.pa; m->type = B2G_M_FACE_B; }
+  int edge1 = primary.index;
+  ClipV inc[2];
+  ep_find_incident(inc, p1, edge1, p2);
+  int iv1 = edge1, iv2 = (edge1 + 1) % p1->n;
+  V2 v11 = p1->v[iv1], v12 = p1->v[iv2];
+  V2 tangent = normalized(sub(v12, v11));
+  V2 normal = cross_vs(tangent, 1.0);
+  V2 plane_point = scale(0.5, add(v11, v12));
+  double front_offset = dot(normal, v11);
+  double side1 = -dot(tangent, v11) + c.radius;
+  double side2 = dot(tangent, v12) + c.radius;
+  ClipV cp1[3], cp2[3];
+  if (clip_segment_to_line(cp1, inc, neg(tangent), side1, iv1) < 2) return;
+  int n2 = clip_segment_to_line(cp2, cp1, tangent, side2, iv2);
+  if (n2 < 2) return;
+  if (primary.type == EP_EDGE_A) { m->ln = normal; m->lp = plane_point; }
+  else { m->ln = mulmtv(c.xf.R, normal); m->lp = xmult(c.xf, plane_point); }
+  int pc = 0;
+  for (int i = 0; i < n2; ++i) {
+    double separation = dot(normal, cp2[i].v) - front_offset;
+    if (separation <= c.radius) {
+      MPoint* mp = &m->pts[pc];
+      if (primary.type == EP_EDGE_A) {
+        mp->lp = xmult(c.xf, cp2[i].v);
+        set_id(mp->id, cp2[i].id[0], cp2[i].id[1], cp2[i].id[2], cp2[i].id[3]);
+      } else {
+        mp->lp = cp2[i].v;
+        /* set_id(type_b, type_a, index_b, index_a), quirk Q9 */
+        set_id(mp->id, cp2[i].id[3], cp2[i].id[2], cp2[i].id[1], cp2[i].id[0]);
+      }
+      ++pc;
+    }
+  }
+  m->pc = pc;
+}
+
+/* Contact.evaluate dispatch (contact.py:343-373); fixtures are already
+   oriented as the contact registry orders them (contact.py:305-326). */
+static int evaluate(Manifold* m, const Shape* A, XF xfa, const Shape* B, XF xfb) {
+  int ta = A->type, tb = B->type;
+  if (ta == B2G_SHAPE_CIRCLE && tb == B2G_SHAPE_CIRCLE) collide_circles(m, A, xfa, B, xfb);
+  else if (ta == B2G_SHAPE_POLYGON && tb == B2G_SHAPE_POLYGON) collide_polygons(m, A, xfa, B, xfb);
+  else if (ta == B2G_SHAPE_POLYGON && tb == B2G_SHAPE_CIRCLE) collide_polygon_circle(m, A, xfa, B, xfb);
+  else if (ta == B2G_SHAPE_EDGE && tb == B2G_SHAPE_CIRCLE) collide_edge_circle(m, A, xfa, B, xfb);
+  else if (ta == B2G_SHAPE_EDGE && tb == B2G_SHAPE_POLYGON) collide_edge_polygon(m, A, xfa, B, xfb);
+  else return -1;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* body.py helpers                                                     */
+/* ------------------------------------------------------------------ */
+static void set_awake(Body* b, int awake) { /* body.py:800-812 */
+  if (awake) {
+    if (!b->awake) b->sleep_time = 0.0;
+  } else {
+    b->sleep_time = 0.0;
+    b->v = v2(0.0, 0.0);
+    b->w = 0.0;
+    b->force = v2(0.0, 0.0);
+    b->torque = 0.0;
+  }
+  b->awake = awake;
+}
+
+static inline int other_body(const World* w, const Contact* c, int body) { /* contact.py:92-99 */
+  int ba = w->f[c->fa].body;
+  return body == ba ? w->f[c->fb].body : ba;
+}
+
+/* ------------------------------------------------------------------ */
+/* broad-phase + ContactManager.find_new_contacts                      */
+/* ------------------------------------------------------------------ */
+typedef struct { int64_t* keys; int cap; } HSet;
+static int hs_init(HSet* h, int n) {
+  int cap = 16;
+  while (cap < 2 * n + 16) cap <<= 1;
+  h->keys = (int64_t*)malloc(sizeof(int64_t) * (size_t)cap);
+  if (!h->keys) return -1;
+  for (int i = 0; i < cap; ++i) h->keys[i] = -1;
+  h->cap = cap;
+  return 0;
+}
+static inline uint64_t hs_hash(int64_t k) {
+  uint64_t x = (uint64_t)k * 0x9E3779B97F4A7C15ull;
+  return x ^ (x >> 29);
+}
+static int hs_has(const HSet* h, int64_t k) {
+  uint64_t i = hs_hash(k) & (uint64_t)(h->cap - 1);
+  while (h->keys[i] != -1) {
+    if (h->keys[i] == k) return 1;
+    i = (i + 1) & (uint64_t)(h->cap - 1);
+  }
+  return 0;
+}
+static int hs_add(HSet* h, int64_t k) { /* returns 1 if inserted; caller sizes the set */
+  uint64_t i = hs_hash(k) & (uint64_t)(h->cap - 1);
+  while (h->keys[i] != -1) {
+    if (h->keys[i] == k) return 0;
+    i = (i + 1) & (uint64_t)(h->cap - 1);
+  }
+  h->keys[i] = k;
+  return 1;
+}
+static inline int64_t okey(int a, int b) { return ((int64_t)a << 32) | (int64_t)(uint32_t)b; }
+static inline int64_t ukey(int a, int b) { return a < b ? okey(a, b) : okey(b, a); }
+
+/* BroadPhase.update_pairs (broadphase.py:83-109) restated over all proxies:
+   the tree query returns exactly the proxies whose fat AABB overlaps the
+   query's (dynamictree.py:190-207; pinned by dynamictree_test.py:89-98). */
+static int update_pairs(const AABB* fat, int nf, const int* move, int nmove, int** out_pairs, int* out_n) {
+  int cap = 64, n = 0;
+  int* pairs = (int*)malloc(sizeof(int) * 2 * (size_t)cap);
+  HSet set;
+  if (!pairs || hs_init(&set, nmove * 8 + 64)) { free(pairs); return -1; }
+  for (int k = 0; k < nmove; ++k) {
+    int P = move[k];
+    for (int R = 0; R < nf; ++R) {
+      if (R == P) continue;
+      if (!aabb_overlap(&fat[R], &fat[P])) continue;
+      if (hs_has(&set, okey(R, P))) continue;
+      if (2 * (n + 1) > set.cap) { /* grow the set */
+        HSet bigger;
+        if (hs_init(&bigger, set.cap)) { free(set.keys); free(pairs); return -1; }
+        for (int i = 0; i < set.cap; ++i) if (set.keys[i] != -1) hs_add(&bigger, set.keys[i]);
+        free(set.keys);
+        set = bigger;
+      }
+      if (!hs_add(&set, okey(P, R))) continue;
+      if (n == cap) {
+        cap *= 2;
+        int* np = (int*)realloc(pairs, sizeof(int) * 2 * (size_t)cap);
+        if (!np) { free(set.keys); free(pairs); return -1; }
+        pairs = np;
+      }
+      pairs[2 * n] = P;
+      pairs[2 * n + 1] = R;
+      ++n;
+    }
+  }
+  free(set.keys);
+  *out_pairs = pairs;
+  *out_n = n;
+  return 0;
+}
+
+static int contact_reserve(World* w, int n) {
+  if (n <= w->ccap) return 0;
+  int cap = w->ccap ? w->ccap : 16;
+  while (cap < n) cap *= 2;
+  Contact* nc = (Contact*)realloc(w->c, sizeof(Contact) * (size_t)cap);
+  if (!nc) return -1;
+  w->c = nc;
+  w->ccap = cap;
+  return 0;
+}
+
+static int64_t* g_sort_rank_keys; /* used by the hint sort */
+static int g_sort_rank_n;
+static int hint_rank(int64_t key) {
+  for (int i = 0; i < g_sort_rank_n; ++i) if (g_sort_rank_keys[i] == key) return i;
+  return g_sort_rank_n;
+}
+
+static void find_new_contacts(World* w) { /* contact_manager.py:102-158 */
+  int* pairs = NULL;
+  int np = 0;
+  AABB* fat = (AABB*)malloc(sizeof(AABB) * (size_t)(w->nf > 0 ? w->nf : 1));
+  if (!fat) { w->err |= ORC_E_ALLOC; return; }
+  for (int i = 0; i < w->nf; ++i) fat[i] = w->f[i].fat;
+  if (update_pairs(fat, w->nf, w->move, w->nmove, &pairs, &np)) { free(fat); w->err |= ORC_E_ALLOC; return; }
+  free(fat);
+  w->nmove = 0;
+  HSet exist;
+  if (hs_init(&exist, w->nc)) { free(pairs); w->err |= ORC_E_ALLOC; return; }
+  for (int i = 0; i < w->nc; ++i) hs_add(&exist, ukey(w->c[i].fa, w->c[i].fb));
+  int first_new = w->nc;
+  for (int k = 0; k < np; ++k) {
+    int fa = pairs[2 * k], fb = pairs[2 * k + 1];
+    int ba = w->f[fa].body, bb = w->f[fb].body;
+    if (ba == bb) continue;
+    if (hs_has(&exist, ukey(fa, fb))) continue;
+    /* Body._should_collide (body.py:910-928), no joints */
+    if (w->b[bb].type != B2G_DYNAMIC && w->b[ba].type != B2G_DYNAMIC) continue;
+    /* World.default_contact_filter (world.py:958-970) */
+    const Fix *A = &w->f[fa], *B = &w->f[fb];
+    int ok;
+    if (A->group == B->group && A->group != 0) ok = A->group > 0;
+    else ok = ((A->mask & B->category) != 0) && ((A->category & B->mask) != 0);
+    if (!ok) continue;
+    /* Contact._create + registry (contact.py:294-326) */
+    int ta = A->sh.type, tb = B->sh.type, swap;
+    if (ta == B2G_SHAPE_LOOP || tb == B2G_SHAPE_LOOP) { w->err |= ORC_E_UNSUPPORTED; continue; }
+    if ((ta == 0 && tb == 0) || (ta == 1 && tb == 1) || (ta == 1 && tb == 0) || (ta == 3 && tb == 0) ||
+        (ta == 3 && tb == 1)) swap = 0;
+    else if ((ta == 0 && tb == 1) || (ta == 0 && tb == 3) || (ta == 1 && tb == 3)) swap = 1;
+    else continue; /* no collider registered (edge/edge) */
+    if (contact_reserve(w, w->nc + 1)) { w->err |= ORC_E_ALLOC; break; }
+    Contact* c = &w->c[w->nc++];
+    memset(c, 0, sizeof(*c));
+    c->fa = swap ? fb : fa;
+    c->fb = swap ? fa : fb;
+    /* mix_friction / mix_restitution (contact_util.py:36-48) */
+    c->friction = sqrt(w->f[c->fa].friction * w->f[c->fb].friction);
+    c->restitution = pmax(w->f[c->fa].restitution, w->f[c->fb].restitution);
+    c->m.type = B2G_M_CIRCLES;
+  }
+  free(exist.keys);
+  free(pairs);
+  if (w->hint_n > 0 && w->nc > first_new) {
+    /* replay: order this call's new contacts by their rank in the hint */
+    int nn = w->nc - first_new;
+    g_sort_rank_keys = w->hint;
+    g_sort_rank_n = w->hint_n;
+    int* rank = (int*)malloc(sizeof(int) * (size_t)nn);
+    Contact* tmp = (Contact*)malloc(sizeof(Contact) * (size_t)nn);
+    if (!rank || !tmp) { free(rank); free(tmp); w->err |= ORC_E_ALLOC; return; }
+    for (int i = 0; i < nn; ++i) {
+      rank[i] = hint_rank(ukey(w->c[first_new + i].fa, w->c[first_new + i].fb));
+      if (rank[i] == w->hint_n) w->err |= ORC_E_HINT;
+      tmp[i] = w->c[first_new + i];
+    }
+    for (int i = 1; i < nn; ++i) { /* stable insertion sort by rank */
+      Contact ct = tmp[i];
+      int r = rank[i], j = i - 1;
+      while (j >= 0 && rank[j] > r) { tmp[j + 1] = tmp[j]; rank[j + 1] = rank[j]; --j; }
+      tmp[j + 1] = ct; rank[j + 1] = r;
+    }
+    memcpy(&w->c[first_new], tmp, sizeof(Contact) * (size_t)nn);
+    free(rank);
+    free(tmp);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* ContactManager.collide + Contact.update                             */
+/* ------------------------------------------------------------------ */
+static void contact_update(World* w, Contact* c) { /* contact.py:210-268 */
+  Manifold old = c->m;
+  c->enabled = 1;
+  int was_touching = c->touching;
+  Fix *FA = &w->f[c->fa], *FB = &w->f[c->fb];
+  if (FA->sensor || FB->sensor) { w->err |= ORC_E_UNSUPPORTED; return; }
+  Body *A = &w->b[FA->body], *B = &w->b[FB->body];
+  if (evaluate(&c->m, &FA->sh, A->xf, &FB->sh, B->xf)) { w->err |= ORC_E_UNSUPPORTED; return; }
+  int touching = c->m.pc > 0;
+  for (int j = 0; j < c->m.pc; ++j) {
+    MPoint* mp2 = &c->m.pts[j];
+    int found = -1;
+    /* dict(...) keeps the last duplicate key */
+    for (int i = 0; i < old.pc; ++i)
+      if (memcmp(old.pts[i].id, mp2->id, sizeof(mp2->id)) == 0) found = i;
+    if (found >= 0) { mp2->ni = old.pts[found].ni; mp2->ti = old.pts[found].ti; }
+    else { mp2->ni = 0.0; mp2->ti = 0.0; }
+  }
+  if (touching != was_touching) { set_awake(A, 1); set_awake(B, 1); }
+  c->touching = touching;
+}
+
+static void collide(World* w) { /* contact_manager.py:174-223 */
+  char* remove = (char*)calloc((size_t)(w->nc > 0 ? w->nc : 1), 1);
+  if (!remove) { w->err |= ORC_E_ALLOC; return; }
+  int nremove = 0;
+  for (int i = 0; i < w->nc; ++i) {
+    Contact* c = &w->c[i];
+    Fix *FA = &w->f[c->fa], *FB = &w->f[c->fb];
+    Body *A = &w->b[FA->body], *B = &w->b[FB->body];
+    int active_a = A->awake && A->type != B2G_STATIC;
+    int active_b = B->awake && B->type != B2G_STATIC;
+    if (!active_a && !active_b) continue;
+    if (c->filter) {
+      int ok = (A->type == B2G_DYNAMIC || B->type == B2G_DYNAMIC);
+      if (ok) {
+        if (FA->group == FB->group && FA->group != 0) ok = FA->group > 0;
+        else ok = ((FA->mask & FB->category) != 0) && ((FA->category & FB->mask) != 0);
+      }
+      if (!ok) { remove[i] = 1; ++nremove; continue; }
+      c->filter = 0;
+    }
+    if (!aabb_overlap(&FA->fat, &FB->fat)) { remove[i] = 1; ++nremove; continue; }
+    contact_update(w, c);
+  }
+  if (nremove) { /* ContactManager.destroy keeps the order of the others */
+    int k = 0;
+    for (int i = 0; i < w->nc; ++i) if (!remove[i]) w->c[k++] = w->c[i];
+    w->nc = k;
+  }
+  free(remove);
+}
+
+/* ------------------------------------------------------------------ */
+/* ContactSolver (contact.py:375-864) and Island.solve (island.py)     */
+/* ------------------------------------------------------------------ */
+typedef struct { V2 ra, rb; double ni, ti, nm, tm, bias; } VCP;
+typedef struct {
+  int npts;
+  VCP p[2];
+  V2 normal;
+  M22 K, nmass;
+  int ia, ib;
+  double ma, mb, Ia, Ib, friction, restitution;
+  int contact;
+} VC;
+typedef struct {
+  int npts;
+  V2 lpts[2], ln, lp;
+  int type, ia, ib;
+  double ra, rb, ma, mb, Ia, Ib;
+  V2 lca, lcb;
+} PC;
+typedef struct { V2 c; double a; V2 v; double w; } PV;
+
+typedef struct { double dt, inv_dt, dt_ratio; int vel_iters, pos_iters, warm_starting; } Step;
+
+/* WorldManifold (contact_util.py:195-245) */
+static V2 world_manifold(const Manifold* m, XF xfa, double ra, XF xfb, double rb, V2 pts[2]) {
+  V2 normal = v2(0.0, 0.0);
+  if (m->pc == 0) return normal;
+  if (m->type == B2G_M_CIRCLES) {
+    normal = v2(1.0, 0.0);
+    V2 pa = xmul(xfa, m->lp);
+    V2 pb = xmul(xfb, m->pts[0].lp);
+    V2 d = sub(pa, pb);
+    if (dot(d, d) > EPSILON_SQR) normal = normalized(sub(pb, pa));
+    V2 ca = add(pa, scale(ra, normal));
+    V2 cb = sub(pb, scale(rb, normal));
+    pts[0] = scale(0.5, add(ca, cb));
+  } else if (m->type == B2G_M_FACE_A) {
+    normal = mulmv(xfa.R, m->ln);
+    V2 plane = xmul(xfa, m->lp);
+    for (int i = 0; i < m->pc; ++i) {
+      V2 clip = xmul(xfb, m->pts[i].lp);
+      V2 ca = add(clip, scale(ra - dot(sub(clip, plane), normal), normal));
+      V2 cb = sub(clip, scale(rb, normal));
+      pts[i] = scale(0.5, add(ca, cb));
+    }
+  } else {
+    normal = mulmv(xfb.R, m->ln);
+    V2 plane = xmul(xfb, m->lp);
+    for (int i = 0; i < m->pc; ++i) {
+      V2 clip = xmul(xfa, m->pts[i].lp);
+      V2 cb = add(clip, scale(rb - dot(sub(clip, plane), normal), normal));
+      V2 ca = sub(clip, scale(ra, normal));
+      pts[i] = scale(0.5, add(ca, cb));
+    }
+    normal = neg(normal);
+  }
+  return normal;
+}
+
+/* ContactPositionConstraint.solver_manifold (contact_util.py:323-360) */
+static void solver_manifold(const PC* pc, XF xfa, XF xfb, int j, V2* normal, V2* point, double* sep) {
+  if (pc->type == B2G_M_CIRCLES) {
+    V2 pa = xmul(xfa, pc->lp), pb = xmul(xfb, pc->lpts[0]);
+    *normal = normalized(sub(pb, pa));
+    *point = scale(0.5, add(pa, pb));
+    *sep = dot(sub(pb, pa), *normal) - pc->ra - pc->rb;
+  } else if (pc->type == B2G_M_FACE_A) {
+    *normal = mulmv(xfa.R, pc->ln);
+    V2 plane = xmul(xfa, pc->lp);
+    V2 clip = xmul(xfb, pc->lpts[j]);
+    *sep = dot(sub(clip, plane), *normal) - pc->ra - pc->rb;
+    *point = clip;
+  } else {
+    *normal = mulmv(xfb.R, pc->ln);
+    V2 plane = xmul(xfb, pc->lp);
+    V2 clip = xmul(xfa, pc->lpts[j]);
+    *sep = dot(sub(clip, plane), *normal) - pc->ra - pc->rb;
+    *point = clip;
+    *normal = neg(*normal);
+  }
+}
+
+static inline void apply_block(VC* vc, V2 x, V2 a, V2 normal, V2* va, double* wa, V2* vb, double* wb) {
+  V2 d = sub(x, a);
+  V2 p1 = scale(d.x, normal), p2 = scale(d.y, normal);
+  *va = sub(*va, scale(vc->ma, add(p1, p2)));
+  *wa -= vc->Ia * (cross(vc->p[0].ra, p1) + cross(vc->p[1].ra, p2));
+  *vb = add(*vb, scale(vc->mb, add(p1, p2)));
+  *wb += vc->Ib * (cross(vc->p[0].rb, p1) + cross(vc->p[1].rb, p2));
+  vc->p[0].ni = x.x;
+  vc->p[1].ni = x.y;
+}
+
+static void solve_velocity(VC* vcs, int nvc, PV* pv) { /* contact.py:527-799 */
+  for (int k = 0; k < nvc; ++k) {
+    VC* vc = &vcs[k];
+    V2 va = pv[vc->ia].v, vb = pv[vc->ib].v;
+    double wa = pv[vc->ia].w, wb = pv[vc->ib].w;
+    V2 normal = vc->normal;
+    V2 tangent = cross_vs(normal, 1.0);
+    double friction = vc->friction;
+    for (int j = 0; j < vc->npts; ++j) {
+      VCP* p = &vc->p[j];
+      V2 dv = sub(sub(add(vb, cross_sv(wb, p->rb)), va), cross_sv(wa, p->ra));
+      double vt = dot(dv, tangent);
+      double lambda = p->tm * (-vt);
+      double maxf = friction * p->ni;
+      double newi = pclamp(p->ti + lambda, -maxf, maxf);
+      lambda = newi - p->ti;
+      V2 P = scale(lambda, tangent);
+      va = sub(va, scale(vc->ma, P));
+      wa -= vc->Ia * cross(p->ra, P);
+      vb = add(vb, scale(vc->mb, P));
+      wb += vc->Ib * cross(p->rb, P);
+      p->ti = newi;
+    }
+    if (vc->npts == 1) {
+      VCP* p = &vc->p[0];
+      V2 dv = sub(sub(add(vb, cross_sv(wb, p->rb)), va), cross_sv(wa, p->ra));
+      double vn = dot(dv, normal);
+      double lambda = -p->nm * (vn - p->bias);
+      double newi = pmax(p->ni + lambda, 0.0);
+      lambda = newi - p->ni;
+      V2 P = scale(lambda, normal);
+      va = sub(va, scale(vc->ma, P));
+      wa -= vc->Ia * cross(p->ra, P);
+      vb = add(vb, scale(vc->mb, P));
+      wb += vc->Ib * cross(p->rb, P);
+      p->ni = newi;
+    } else {
+      VCP *cp1 = &vc->p[0], *cp2 = &vc->p[1];
+      V2 a = v2(cp1->ni, cp2->ni);
+      V2 dv1 = sub(sub(add(vb, cross_sv(wb, cp1->rb)), va), cross_sv(wa, cp1->ra));
+      V2 dv2 = sub(sub(add(vb, cross_sv(wb, cp2->rb)), va), cross_sv(wa, cp2->ra));
+      double vn1 = dot(dv1, normal);
+      double vn2 = dot(dv2, normal);
+      V2 b = v2(vn1 - cp1->bias, vn2 - cp2->bias);
+      b = sub(b, mulmv(vc->K, a));
+      for (;;) {
+        V2 x = neg(mulmv(vc->nmass, b)); /* case 1 */
+        if (x.x >= 0.0 && x.y >= 0.0) { apply_block(vc, x, a, normal, &va, &wa, &vb, &wb); break; }
+        x = v2(-cp1->nm * b.x, 0.0); /* case 2 */
+        vn2 = vc->K.c1y * x.x + b.y;
+        if (x.x >= 0.0 && vn2 >= 0.0) { apply_block(vc, x, a, normal, &va, &wa, &vb, &wb); break; }
+        x = v2(0.0, -cp2->nm * b.y); /* case 3 */
+        vn1 = vc->K.c2x * x.y + b.x;
+        if (x.y >= 0.0 && vn1 >= 0.0) { apply_block(vc, x, a, normal, &va, &wa, &vb, &wb); break; }
+        x = v2(0.0, 0.0); /* case 4 */
+        vn1 = b.x;
+        vn2 = b.y;
+        if (vn1 >= 0.0 && vn2 >= 0.0) { apply_block(vc, x, a, normal, &va, &wa, &vb, &wb); break; }
+        break;
+      }
+    }
+    pv[vc->ia].v = va; pv[vc->ia].w = wa;
+    pv[vc->ib].v = vb; pv[vc->ib].w = wb;
+  }
+}
+
+static int solve_position(PC* pcs, int npc, PV* pv) { /* contact.py:809-864 */
+  double min_sep = 0.0;
+  for (int k = 0; k < npc; ++k) {
+    PC* pc = &pcs[k];
+    V2 ca = pv[pc->ia].c, cb = pv[pc->ib].c;
+    double aa = pv[pc->ia].a, ab = pv[pc->ib].a;
+    for (int j = 0; j < pc->npts; ++j) {
+      XF xfa = xf_from(ca, aa, pc->lca);
+      XF xfb = xf_from(cb, ab, pc->lcb);
+      V2 normal, point;
+      double sep;
+      solver_manifold(pc, xfa, xfb, j, &normal, &point, &sep);
+      V2 ra = sub(point, ca), rb = sub(point, cb);
+      min_sep = pmin(min_sep, sep);
+      double C = pclamp(BAUMGARTE * (sep + LINEAR_SLOP), -MAX_LINEAR_CORRECTION, 0.0);
+      double rna = pysq(cross(ra, normal));
+      double rnb = pysq(cross(rb, normal));
+      double K = pc->ma + pc->mb + (pc->Ia * rna) + (pc->Ib * rnb);
+      double impulse = K > 0.0 ? -C / K : 0.0;
+      V2 P = scale(impulse, normal);
+      ca = sub(ca, scale(pc->ma, P));
+      aa -= pc->Ia * cross(ra, P);
+      cb = add(cb, scale(pc->mb, P));
+      ab += pc->Ib * cross(rb, P);
+    }
+    pv[pc->ia].c = ca; pv[pc->ia].a = aa;
+    pv[pc->ib].c = cb; pv[pc->ib].a = ab;
+  }
+  return min_sep >= -3.0 * LINEAR_SLOP;
+}
+
+typedef struct {
+  int* bodies; int nbodies;
+  int* contacts; int ncontacts;
+  PV* pv; VC* vc; PC* pc;
+} IslandBuf;
+
+static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py:65-179 */
+  double dt = st->dt;
+  PV* pv = is->pv;
+  for (int i = 0; i < is->nbodies; ++i) {
+    Body* b = &w->b[is->bodies[i]];
+    V2 c = b->c;
+    double a = b->a;
+    V2 v = b->v;
+    double wv = b->w;
+    b->c0 = c;
+    b->a0 = a;
+    if (b->type == B2G_DYNAMIC) {
+      v = add(v, scale(dt, add(scale(b->gscale, w->gravity), scale(b->inv_mass, b->force))));
+      wv += dt * b->invI * b->torque;
+      v = scale(pclamp(1.0 - dt * b->ldamp, 0.0, 1.0), v);
+      wv *= pclamp(1.0 - dt * b->adamp, 0.0, 1.0);
+    }
+    pv[i].c = c; pv[i].a = a; pv[i].v = v; pv[i].w = wv;
+  }
+  /* ContactSolver.__init__ (contact.py:376-412) */
+  int nvc = is->ncontacts;
+  for (int k = 0; k < nvc; ++k) {
+    Contact* c = &w->c[is->contacts[k]];
+    Fix *FA = &w->f[c->fa], *FB = &w->f[c->fb];
+    Body *A = &w->b[FA->body], *B = &w->b[FB->body];
+    VC* vc = &is->vc[k];
+    PC* pc = &is->pc[k];
+    memset(vc, 0, sizeof(*vc));
+    vc->contact = is->contacts[k];
+    vc->friction = c->friction;
+    vc->restitution = c->restitution;
+    vc->ia = A->island_index; vc->ib = B->island_index;
+    vc->ma = A->inv_mass; vc->mb = B->inv_mass;
+    vc->Ia = A->invI; vc->Ib = B->invI;
+    vc->K.c1x = 1.0; vc->K.c2y = 1.0;
+    vc->nmass.c1x = 1.0; vc->nmass.c2y = 1.0;
+    vc->npts = c->m.pc;
+    for (int j = 0; j < c->m.pc; ++j) { /* VelocityConstraintPoint (contact_util.py:253-277) */
+      if (st->warm_starting) {
+        vc->p[j].ni = st->dt_ratio * c->m.pts[j].ni;
+        vc->p[j].ti = st->dt_ratio * c->m.pts[j].ti;
+      } else {
+        vc->p[j].ni = c->m.pts[j].ni;
+        vc->p[j].ti = c->m.pts[j].ti;
+      }
+    }
+    pc->ia = vc->ia; pc->ib = vc->ib;
+    pc->ma = vc->ma; pc->mb = vc->mb;
+    pc->Ia = vc->Ia; pc->Ib = vc->Ib;
+    pc->lca = A->lc; pc->lcb = B->lc;
+    pc->ra = FA->sh.radius; pc->rb = FB->sh.radius;
+    pc->ln = c->m.ln; pc->lp = c->m.lp;
+    pc->type = c->m.type;
+    pc->npts = c->m.pc;
+    for (int j = 0; j < c->m.pc; ++j) pc->lpts[j] = c->m.pts[j].lp;
+  }
+  /* initialize_velocity_constraints (contact.py:414-496) */
+  for (int k = 0; k < nvc; ++k) {
+    VC* vc = &is->vc[k];
+    PC* pc = &is->pc[k];
+    Contact* c = &w->c[vc->contact];
+    double ma = vc->ma, mb = vc->mb, Ia = vc->Ia, Ib = vc->Ib;
+    V2 ca = pv[vc->ia].c, cb = pv[vc->ib].c;
+    double aa = pv[vc->ia].a, ab = pv[vc->ib].a;
+    V2 va = pv[vc->ia].v, vb = pv[vc->ib].v;
+    double wa = pv[vc->ia].w, wb = pv[vc->ib].w;
+    XF xfa = xf_from(ca, aa, pc->lca);
+    XF xfb = xf_from(cb, ab, pc->lcb);
+    V2 wpts[2];
+    vc->normal = world_manifold(&c->m, xfa, pc->ra, xfb, pc->rb, wpts);
+    for (int j = 0; j < vc->npts; ++j) {
+      VCP* p = &vc->p[j];
+      p->ra = sub(wpts[j], ca);
+      p->rb = sub(wpts[j], cb);
+      double rna = pysq(cross(p->ra, vc->normal));
+      double rnb = pysq(cross(p->rb, vc->normal));
+      double kn = ma + mb + Ia * rna + Ib * rnb;
+      p->nm = kn > 0.0 ? 1.0 / kn : 0.0;
+      V2 tangent = cross_vs(vc->normal, 1.0);
+      double rta = pysq(cross(p->ra, tangent));
+      double rtb = pysq(cross(p->rb, tangent));
+      double kt = ma + mb + Ia * rta + Ib * rtb;
+      p->tm = kt > 0.0 ? 1.0 / kt : 0.0;
+      p->bias = 0.0;
+      double vrel = dot(vc->normal, sub(sub(add(vb, cross_sv(wb, p->rb)), va), cross_sv(wa, p->ra)));
+      if (vrel < -VELOCITY_THRESHOLD) p->bias = -vc->restitution * vrel;
+    }
+    if (vc->npts == 2) {
+      V2 ra1 = vc->p[0].ra, rb1 = vc->p[0].rb, ra2 = vc->p[1].ra, rb2 = vc->p[1].rb;
+      double rn1a = cross(ra1, vc->normal), rn1b = cross(rb1, vc->normal);
+      double rn2a = cross(ra2, vc->normal), rn2b = cross(rb2, vc->normal);
+      double k11 = ma + mb + Ia * rn1a * rn1a + Ib * rn1b * rn1b;
+      double k22 = ma + mb + Ia * rn2a * rn2a + Ib * rn2b * rn2b;
+      double k12 = ma + mb + Ia * rn1a * rn2a + Ib * rn1b * rn2b;
+      if (pysq(k11) < (1000.0 * (k11 * k22 - pysq(k12)))) {
+        vc->K.c1x = k11; vc->K.c1y = k12; vc->K.c2x = k12; vc->K.c2y = k22;
+        /* PyMat22.inverse (common.py:502-515) */
+        double A_ = vc->K.c1x, B_ = vc->K.c2x, C_ = vc->K.c1y, D_ = vc->K.c2y;
+        double det = A_ * D_ - B_ * C_;
+        if (det != 0.0) det = 1.0 / det;
+        vc->nmass.c1x = det * D_; vc->nmass.c1y = -det * C_;
+        vc->nmass.c2x = -det * B_; vc->nmass.c2y = det * A_;
+      } else {
+        vc->npts = 1; /* quirk Q6 */
+      }
+    }
+  }
+  /* warm_start (contact.py:498-525) */
+  if (st->warm_starting) {
+    for (int k = 0; k < nvc; ++k) {
+      VC* vc = &is->vc[k];
+      V2 normal = vc->normal, tangent = cross_vs(normal, 1.0);
+      V2 va = pv[vc->ia].v, vb = pv[vc->ib].v;
+      double wa = pv[vc->ia].w, wb = pv[vc->ib].w;
+      for (int j = 0; j < vc->npts; ++j) {
+        VCP* p = &vc->p[j];
+        V2 P = add(scale(p->ni, normal), scale(p->ti, tangent));
+        wa -= vc->Ia * cross(p->ra, P);
+        va = sub(va, scale(vc->ma, P));
+        wb += vc->Ib * cross(p->rb, P);
+        vb = add(vb, scale(vc->mb, P));
+      }
+      pv[vc->ia].v = va; pv[vc->ia].w = wa;
+      pv[vc->ib].v = vb; pv[vc->ib].w = wb;
+    }
+  }
+  for (int it = 0; it < st->vel_iters; ++it) solve_velocity(is->vc, nvc, pv);
+  /* store_impulses (contact.py:801-806): zip(manifold.used_points, vc.points) */
+  for (int k = 0; k < nvc; ++k) {
+    VC* vc = &is->vc[k];
+    Contact* c = &w->c[vc->contact];
+    int n = c->m.pc < vc->npts ? c->m.pc : vc->npts;
+    for (int j = 0; j < n; ++j) { c->m.pts[j].ni = vc->p[j].ni; c->m.pts[j].ti = vc->p[j].ti; }
+  }
+  /* integrate positions (island.py:120-138) */
+  for (int i = 0; i < is->nbodies; ++i) {
+    V2 v = pv[i].v;
+    double wv = pv[i].w;
+    V2 tr = scale(dt, v);
+    if (dot(tr, tr) > MAX_TRANSLATION_SQR) {
+      double ratio = MAX_TRANSLATION / length_(tr);
+      v = scale(ratio, v);
+    }
+    double rotation = dt * wv;
+    if (pysq(rotation) > MAX_ROTATION_SQR) {
+      double ratio = MAX_ROTATION / fabs(rotation);
+      wv *= ratio;
+    }
+    pv[i].c = add(pv[i].c, scale(dt, v));
+    pv[i].a += dt * wv;
+    pv[i].v = v;
+    pv[i].w = wv;
+  }
+  int position_solved = 0;
+  for (int it = 0; it < st->pos_iters; ++it) {
+    if (solve_position(is->pc, nvc, pv)) { position_solved = 1; break; }
+  }
+  for (int i = 0; i < is->nbodies; ++i) {
+    Body* b = &w->b[is->bodies[i]];
+    b->c = pv[i].c; b->a = pv[i].a; b->v = pv[i].v; b->w = pv[i].w;
+    b->xf = xf_from(b->c, b->a, b->lc); /* _synchronize_transform */
+  }
+  if (w->allow_sleep) { /* island.py:161-179, quirk Q7 */
+    double min_sleep = MAX_FLOAT;
+    const Body* stale = &w->b[is->bodies[is->nbodies - 1]];
+    int last_ns = -1;
+    for (int i = 0; i < is->nbodies; ++i) {
+      Body* b = &w->b[is->bodies[i]];
+      if (b->type == B2G_STATIC) continue;
+      last_ns = is->bodies[i];
+      if (!stale->allow_sleep || pysq(b->w) > ANGULAR_SLEEP_TOLERANCE_SQR ||
+          length_sq(b->v) > LINEAR_SLEEP_TOLERANCE_SQR) {
+        b->sleep_time = 0.0;
+        min_sleep = 0.0;
+      } else {
+        b->sleep_time += dt;
+        min_sleep = pmin(min_sleep, b->sleep_time);
+      }
+    }
+    if (min_sleep >= TIME_TO_SLEEP && position_solved && last_ns >= 0) set_awake(&w->b[last_ns], 0);
+  }
+}
+
+static void solve(World* w, const Step* st) { /* world.py:475-592 */
+  int nb = w->nb, nc = w->nc;
+  for (int i = 0; i < nb; ++i) { w->b[i].island_flag = 0; w->b[i].island_id = -1; }
+  for (int i = 0; i < nc; ++i) w->c[i].island_flag = 0;
+  /* per-body contact lists in ContactManager order (= body._contacts order) */
+  int* off = (int*)calloc((size_t)nb + 1, sizeof(int));
+  int* adj = (int*)malloc(sizeof(int) * (size_t)(2 * nc + 1));
+  int* fill = (int*)calloc((size_t)nb + 1, sizeof(int));
+  IslandBuf is;
+  is.bodies = (int*)malloc(sizeof(int) * (size_t)(nb + 1));
+  is.contacts = (int*)malloc(sizeof(int) * (size_t)(nc + 1));
+  is.pv = (PV*)malloc(sizeof(PV) * (size_t)(nb + 1));
+  is.vc = (VC*)malloc(sizeof(VC) * (size_t)(nc + 1));
+  is.pc = (PC*)malloc(sizeof(PC) * (size_t)(nc + 1));
+  int* stack = (int*)malloc(sizeof(int) * (size_t)(nb + 1));
+  if (!off || !adj || !fill || !is.bodies || !is.contacts || !is.pv || !is.vc || !is.pc || !stack) {
+    w->err |= ORC_E_ALLOC;
+    goto done;
+  }
+  for (int i = 0; i < nc; ++i) {
+    off[w->f[w->c[i].fa].body + 1]++;
+    off[w->f[w->c[i].fb].body + 1]++;
+  }
+  for (int i = 0; i < nb; ++i) off[i + 1] += off[i];
+  for (int i = 0; i < nc; ++i) {
+    int ba = w->f[w->c[i].fa].body, bb = w->f[w->c[i].fb].body;
+    adj[off[ba] + fill[ba]++] = i;
+    adj[off[bb] + fill[bb]++] = i;
+  }
+  int island_id = 0;
+  for (int seed = 0; seed < nb; ++seed) {
+    Body* sb = &w->b[seed];
+    if (sb->island_flag) continue;
+    if (!sb->awake || !sb->active) continue;
+    if (sb->type == B2G_STATIC) continue;
+    is.nbodies = 0;
+    is.ncontacts = 0;
+    int sp = 0;
+    stack[sp++] = seed;
+    sb->island_flag = 1;
+    while (sp > 0) {
+      int bi = stack[--sp];
+      Body* b = &w->b[bi];
+      b->island_index = is.nbodies;
+      b->island_id = island_id;
+      is.bodies[is.nbodies++] = bi;
+      set_awake(b, 1);
+      if (b->type == B2G_STATIC) continue;
+      /* the filtered list is taken before the loop (world.py:528-533) */
+      int start = is.ncontacts;
+      for (int k = off[bi]; k < off[bi + 1]; ++k) {
+        Contact* c = &w->c[adj[k]];
+        if (!c->island_flag && c->enabled && c->touching && !w->f[c->fa].sensor && !w->f[c->fb].sensor)
+          is.contacts[is.ncontacts++] = adj[k];
+      }
+      int end = is.ncontacts;
+      for (int k = start; k < end; ++k) {
+        Contact* c = &w->c[is.contacts[k]];
+        c->island_flag = 1;
+        int other = other_body(w, c, bi);
+        if (w->b[other].island_flag) continue;
+        stack[sp++] = other;
+        w->b[other].island_flag = 1;
+      }
+    }
+    island_solve(w, &is, st);
+    for (int i = 0; i < is.nbodies; ++i)
+      if (w->b[is.bodies[i]].type == B2G_STATIC) w->b[is.bodies[i]].island_flag = 0;
+    ++island_id;
+  }
+  /* Body._synchronize_fixtures for island bodies (world.py:584-589, body.py:898-904) */
+  for (int bi = 0; bi < nb; ++bi) {
+    Body* b = &w->b[bi];
+    if (!b->island_flag || b->type == B2G_STATIC) continue;
+    XF xf1;
+    xf1.R = rot(b->a0);
+    xf1.p = sub(b->c0, mulmv(xf1.R, b->lc));
+    for (int k = w->bfix_off[bi]; k < w->bfix_off[bi + 1]; ++k) {
+      int fi = w->bfix[k];
+      Fix* f = &w->f[fi];
+      /* Fixture._synchronize (fixture.py:287-298) */
+      AABB a1 = compute_aabb(&f->sh, xf1);
+      AABB a2 = compute_aabb(&f->sh, b->xf);
+      AABB aabb = aabb_union(a1, a2);
+      V2 disp = sub(b->xf.p, xf1.p);
+      /* DynamicTree.move_proxy (dynamictree.py:152-188) */
+      if (aabb_contains(&f->fat, &aabb)) continue;
+      AABB nb_;
+      V2 r = v2(AABB_EXTENSION, AABB_EXTENSION);
+      nb_.lo = sub(aabb.lo, r);
+      nb_.hi = add(aabb.hi, r);
+      V2 d = scale(AABB_MULTIPLIER, disp);
+      if (d.x < 0.0) nb_.lo = add(nb_.lo, v2(d.x, 0.0));
+      else nb_.hi = add(nb_.hi, v2(d.x, 0.0));
+      if (d.y < 0.0) nb_.lo = add(nb_.lo, v2(0.0, d.y));
+      else nb_.hi = add(nb_.hi, v2(0.0, d.y));
+      f->fat = nb_;
+      if (w->nmove == w->mcap) {
+        int cap = w->mcap ? 2 * w->mcap : 16;
+        int* nm = (int*)realloc(w->move, sizeof(int) * (size_t)cap);
+        if (!nm) { w->err |= ORC_E_ALLOC; goto done; }
+        w->move = nm;
+        w->mcap = cap;
+      }
+      w->move[w->nmove++] = fi;
+    }
+  }
+  find_new_contacts(w);
+done:
+  free(off); free(adj); free(fill); free(stack);
+  free(is.bodies); free(is.contacts); free(is.pv); free(is.vc); free(is.pc);
+}
+
+static void world_step(World* w, double dt, int vel_iters, int pos_iters) { /* world.py:307-354 */
+  if (w->new_fixture) {
+    find_new_contacts(w);
+    w->new_fixture = 0;
+  }
+  Step st;
+  st.dt = dt;
+  st.inv_dt = dt > 0.0 ? 1.0 / dt : 0.0;
+  st.dt_ratio = w->inv_dt0 * dt;
+  st.vel_iters = vel_iters;
+  st.pos_iters = pos_iters;
+  st.warm_starting = w->warm_starting;
+  collide(w);
+  if (st.dt > 0.0) solve(w, &st);
+  if (st.dt > 0.0) w->inv_dt0 = st.inv_dt;
+  if (!w->keep_forces)
+    for (int i = 0; i < w->nb; ++i) { w->b[i].force = v2(0.0, 0.0); w->b[i].torque = 0.0; }
+}
+
+/* ------------------------------------------------------------------ */
+/* load / save                                                         */
+/* ------------------------------------------------------------------ */
+static void shape_load(Shape* s, const double* d, const int32_t* fi) {
+  memset(s, 0, sizeof(*s));
+  s->type = fi[B2G_FI_SHAPE];
+  s->radius = d[B2G_FD_RADIUS];
+  s->n = fi[B2G_FI_NVERTS];
+  for (int i = 0; i < B2G_MAX_POLY_VERTS; ++i) {
+    s->v[i] = v2(d[B2G_FD_VERTS + 2 * i], d[B2G_FD_VERTS + 2 * i + 1]);
+    s->nrm[i] = v2(d[B2G_FD_NORMALS + 2 * i], d[B2G_FD_NORMALS + 2 * i + 1]);
+  }
+  s->centroid = v2(d[B2G_FD_CENTROID_X], d[B2G_FD_CENTROID_Y]);
+  s->p = v2(d[B2G_FD_CIRCLE_PX], d[B2G_FD_CIRCLE_PY]);
+  s->e0 = v2(d[B2G_FD_EDGE_V0X], d[B2G_FD_EDGE_V0Y]);
+  s->e1 = v2(d[B2G_FD_EDGE_V1X], d[B2G_FD_EDGE_V1Y]);
+  s->e2 = v2(d[B2G_FD_EDGE_V2X], d[B2G_FD_EDGE_V2Y]);
+  s->e3 = v2(d[B2G_FD_EDGE_V3X], d[B2G_FD_EDGE_V3Y]);
+  s->has0 = (fi[B2G_FI_EDGE_FLAGS] & B2G_EF_HAS_V0) != 0;
+  s->has3 = (fi[B2G_FI_EDGE_FLAGS] & B2G_EF_HAS_V3) != 0;
+}
+
+static void manifold_load(Manifold* m, const double* d, const int32_t* ci) {
+  memset(m, 0, sizeof(*m));
+  m->type = ci[B2G_CI_MTYPE];
+  m->pc = ci[B2G_CI_PC];
+  m->ln = v2(d[B2G_CD_LN_X], d[B2G_CD_LN_Y]);
+  m->lp = v2(d[B2G_CD_LP_X], d[B2G_CD_LP_Y]);
+  m->pts[0].lp = v2(d[B2G_CD_P0_X], d[B2G_CD_P0_Y]);
+  m->pts[0].ni = d[B2G_CD_P0_NI];
+  m->pts[0].ti = d[B2G_CD_P0_TI];
+  m->pts[1].lp = v2(d[B2G_CD_P1_X], d[B2G_CD_P1_Y]);
+  m->pts[1].ni = d[B2G_CD_P1_NI];
+  m->pts[1].ti = d[B2G_CD_P1_TI];
+  mp_id_unpack(ci[B2G_CI_ID0], m->pts[0].id);
+  mp_id_unpack(ci[B2G_CI_ID1], m->pts[1].id);
+}
+
+static void manifold_save(const Manifold* m, double* d, int32_t* ci) {
+  ci[B2G_CI_MTYPE] = m->type;
+  ci[B2G_CI_PC] = m->pc;
+  d[B2G_CD_LN_X] = m->ln.x; d[B2G_CD_LN_Y] = m->ln.y;
+  d[B2G_CD_LP_X] = m->lp.x; d[B2G_CD_LP_Y] = m->lp.y;
+  d[B2G_CD_P0_X] = m->pts[0].lp.x; d[B2G_CD_P0_Y] = m->pts[0].lp.y;
+  d[B2G_CD_P0_NI] = m->pts[0].ni; d[B2G_CD_P0_TI] = m->pts[0].ti;
+  d[B2G_CD_P1_X] = m->pts[1].lp.x; d[B2G_CD_P1_Y] = m->pts[1].lp.y;
+  d[B2G_CD_P1_NI] = m->pts[1].ni; d[B2G_CD_P1_TI] = m->pts[1].ti;
+  ci[B2G_CI_ID0] = mp_id_pack(m->pts[0].id);
+  ci[B2G_CI_ID1] = mp_id_pack(m->pts[1].id);
+}
+
+static void world_free(World* w) {
+  free(w->b); free(w->f); free(w->c); free(w->move);
+  free(w->bfix_off); free(w->bfix); free(w->hint);
+  memset(w, 0, sizeof(*w));
+}
+
+static int world_load(World* w, const b2g_batch* d, int wi) {
+  memset(w, 0, sizeof(*w));
+  int b0 = d->body_off[wi], b1 = d->body_off[wi + 1];
+  int f0 = d->fix_off[wi], f1 = d->fix_off[wi + 1];
+  int c0 = d->con_off[wi], c1 = d->con_off[wi + 1];
+  int m0 = d->move_off[wi], m1 = d->move_off[wi + 1];
+  const double* wd = d->world_d + (size_t)wi * B2G_WORLD_ND;
+  int wf = d->world_i[(size_t)wi * B2G_WORLD_NI + B2G_WI_FLAGS];
+  w->gravity = v2(wd[B2G_WD_GX], wd[B2G_WD_GY]);
+  w->inv_dt0 = wd[B2G_WD_INV_DT0];
+  w->allow_sleep = (wf & B2G_WF_ALLOW_SLEEP) != 0;
+  w->warm_starting = (wf & B2G_WF_WARM_STARTING) != 0;
+  w->new_fixture = (wf & B2G_WF_NEW_FIXTURE) != 0;
+  w->keep_forces = (wf & 8) != 0;
+  w->nb = b1 - b0;
+  w->nf = f1 - f0;
+  w->nc = c1 - c0;
+  w->nmove = m1 - m0;
+  w->b = (Body*)calloc((size_t)w->nb + 1, sizeof(Body));
+  w->f = (Fix*)calloc((size_t)w->nf + 1, sizeof(Fix));
+  w->ccap = w->nc + 16;
+  w->c = (Contact*)calloc((size_t)w->ccap, sizeof(Contact));
+  w->mcap = w->nmove + w->nf + 16;
+  w->move = (int*)calloc((size_t)w->mcap, sizeof(int));
+  w->bfix_off = (int*)calloc((size_t)w->nb + 1, sizeof(int));
+  w->bfix = (int*)calloc((size_t)w->nf + 1, sizeof(int));
+  if (!w->b || !w->f || !w->c || !w->move || !w->bfix_off || !w->bfix) return -1;
+  for (int i = 0; i < w->nb; ++i) {
+    const double* bd = d->body_d + (size_t)(b0 + i) * B2G_BODY_ND;
+    const int32_t* bi = d->body_i + (size_t)(b0 + i) * B2G_BODY_NI;
+    Body* b = &w->b[i];
+    b->type = bi[B2G_BI_TYPE];
+    int fl = bi[B2G_BI_FLAGS];
+    b->awake = (fl & B2G_BF_AWAKE) != 0;
+    b->allow_sleep = (fl & B2G_BF_ALLOW_SLEEP) != 0;
+    b->fixed_rot = (fl & B2G_BF_FIXED_ROT) != 0;
+    b->active = (fl & B2G_BF_ACTIVE) != 0;
+    b->bullet = (fl & B2G_BF_BULLET) != 0;
+    b->island_id = -1;
+    b->xf.p = v2(bd[B2G_BD_XF_PX], bd[B2G_BD_XF_PY]);
+    b->xf.R.c1x = bd[B2G_BD_XF_C]; b->xf.R.c1y = bd[B2G_BD_XF_S];
+    b->xf.R.c2x = -bd[B2G_BD_XF_S]; b->xf.R.c2y = bd[B2G_BD_XF_C];
+    b->lc = v2(bd[B2G_BD_LC_X], bd[B2G_BD_LC_Y]);
+    b->c0 = v2(bd[B2G_BD_C0_X], bd[B2G_BD_C0_Y]);
+    b->c = v2(bd[B2G_BD_C_X], bd[B2G_BD_C_Y]);
+    b->a0 = bd[B2G_BD_A0]; b->a = bd[B2G_BD_A];
+    b->v = v2(bd[B2G_BD_V_X], bd[B2G_BD_V_Y]);
+    b->w = bd[B2G_BD_W];
+    b->force = v2(bd[B2G_BD_F_X], bd[B2G_BD_F_Y]);
+    b->torque = bd[B2G_BD_TORQUE];
+    b->mass = bd[B2G_BD_MASS]; b->inv_mass = bd[B2G_BD_INV_MASS];
+    b->I = bd[B2G_BD_I]; b->invI = bd[B2G_BD_INV_I];
+    b->gscale = bd[B2G_BD_GRAV_SCALE];
+    b->ldamp = bd[B2G_BD_LIN_DAMP]; b->adamp = bd[B2G_BD_ANG_DAMP];
+    b->sleep_time = bd[B2G_BD_SLEEP_TIME];
+  }
+  for (int i = 0; i < w->nf; ++i) {
+    const double* fd = d->fix_d + (size_t)(f0 + i) * B2G_FIX_ND;
+    const int32_t* fi = d->fix_i + (size_t)(f0 + i) * B2G_FIX_NI;
+    Fix* f = &w->f[i];
+    f->body = fi[B2G_FI_BODY];
+    if (f->body < 0 || f->body >= w->nb) return -2;
+    shape_load(&f->sh, fd, fi);
+    if (f->sh.type == B2G_SHAPE_LOOP || f->sh.n > B2G_MAX_POLY_VERTS) return -3;
+    f->friction = fd[B2G_FD_FRICTION];
+    f->restitution = fd[B2G_FD_RESTITUTION];
+    f->density = fd[B2G_FD_DENSITY];
+    f->category = fi[B2G_FI_CATEGORY];
+    f->mask = fi[B2G_FI_MASK];
+    f->group = fi[B2G_FI_GROUP];
+    f->sensor = fi[B2G_FI_SENSOR];
+    f->fat.lo = v2(fd[B2G_FD_FAT_LX], fd[B2G_FD_FAT_LY]);
+    f->fat.hi = v2(fd[B2G_FD_FAT_UX], fd[B2G_FD_FAT_UY]);
+    w->bfix_off[f->body + 1]++;
+  }
+  for (int i = 0; i < w->nb; ++i) w->bfix_off[i + 1] += w->bfix_off[i];
+  {
+    int* fill = (int*)calloc((size_t)w->nb + 1, sizeof(int));
+    if (!fill) return -1;
+    for (int i = 0; i < w->nf; ++i) {
+      int b = w->f[i].body;
+      w->bfix[w->bfix_off[b] + fill[b]++] = i;
+    }
+    free(fill);
+  }
+  for (int i = 0; i < w->nc; ++i) {
+    const double* cd = d->con_d + (size_t)(c0 + i) * B2G_CON_ND;
+    const int32_t* ci = d->con_i + (size_t)(c0 + i) * B2G_CON_NI;
+    Contact* c = &w->c[i];
+    c->fa = ci[B2G_CI_FA];
+    c->fb = ci[B2G_CI_FB];
+    if (c->fa < 0 || c->fa >= w->nf || c->fb < 0 || c->fb >= w->nf) return -2;
+    int fl = ci[B2G_CI_FLAGS];
+    c->touching = (fl & B2G_CF_TOUCHING) != 0;
+    c->enabled = (fl & B2G_CF_ENABLED) != 0;
+    c->filter = (fl & B2G_CF_FILTER) != 0;
+    c->friction = cd[B2G_CD_FRICTION];
+    c->restitution = cd[B2G_CD_RESTITUTION];
+    manifold_load(&c->m, cd, ci);
+  }
+  for (int i = 0; i < w->nmove; ++i) w->move[i] = d->move[m0 + i];
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* exported entry points (ctypes)                                      */
+/* ------------------------------------------------------------------ */
+ORC_EXPORT void orc_set_normalize_eps(double eps) { init_consts(); g_norm_eps = eps; }
+
+ORC_EXPORT void* orc_load(const b2g_batch* d) {
+  init_consts();
+  Batch* B = (Batch*)calloc(1, sizeof(Batch));
+  if (!B) return NULL;
+  B->nw = d->n_worlds;
+  B->w = (World*)calloc((size_t)(B->nw > 0 ? B->nw : 1), sizeof(World));
+  if (!B->w) { free(B); return NULL; }
+  for (int i = 0; i < B->nw; ++i) {
+    if (world_load(&B->w[i], d, i)) {
+      for (int j = 0; j <= i; ++j) world_free(&B->w[j]);
+      free(B->w);
+      free(B);
+      return NULL;
+    }
+  }
+  return B;
+}
+
+ORC_EXPORT void orc_free(void* h) {
+  Batch* B = (Batch*)h;
+  if (!B) return;
+  for (int i = 0; i < B->nw; ++i) world_free(&B->w[i]);
+  free(B->w);
+  free(B);
+}
+
+/* Steps every world once; returns the OR of the worlds' error words. */
+ORC_EXPORT int orc_step(void* h, double dt, int vel_iters, int pos_iters) {
+  Batch* B = (Batch*)h;
+  int err = 0;
+  for (int i = 0; i < B->nw; ++i) {
+    world_step(&B->w[i], dt, vel_iters, pos_iters);
+    err |= B->w[i].err;
+  }
+  return err;
+}
+
+ORC_EXPORT int orc_step_world(void* h, int wi, double dt, int vel_iters, int pos_iters) {
+  Batch* B = (Batch*)h;
+  if (wi < 0 || wi >= B->nw) return -1;
+  world_step(&B->w[wi], dt, vel_iters, pos_iters);
+  return B->w[wi].err;
+}
+
+/* counts[0..3] = total bodies, fixtures, contacts, move-buffer entries */
+ORC_EXPORT void orc_counts(void* h, int32_t* counts) {
+  Batch* B = (Batch*)h;
+  int32_t nb = 0, nf = 0, nc = 0, nm = 0;
+  for (int i = 0; i < B->nw; ++i) { nb += B->w[i].nb; nf += B->w[i].nf; nc += B->w[i].nc; nm += B->w[i].nmove; }
+  counts[0] = nb; counts[1] = nf; counts[2] = nc; counts[3] = nm;
+}
+
+ORC_EXPORT int orc_save(void* h, b2g_batch_out* o) {
+  Batch* B = (Batch*)h;
+  if (o->n_worlds != B->nw) return -1;
+  int bo = 0, fo = 0, co = 0, mo = 0;
+  o->con_off[0] = 0;
+  o->move_off[0] = 0;
+  for (int wi = 0; wi < B->nw; ++wi) {
+    World* w = &B->w[wi];
+    double* wd = o->world_d + (size_t)wi * B2G_WORLD_ND;
+    int32_t* wfl = o->world_i + (size_t)wi * B2G_WORLD_NI;
+    wd[B2G_WD_GX] = w->gravity.x; wd[B2G_WD_GY] = w->gravity.y; wd[B2G_WD_INV_DT0] = w->inv_dt0;
+    wfl[B2G_WI_FLAGS] = (w->allow_sleep ? B2G_WF_ALLOW_SLEEP : 0) | (w->warm_starting ? B2G_WF_WARM_STARTING : 0) |
+                        (w->new_fixture ? B2G_WF_NEW_FIXTURE : 0) | (w->keep_forces ? 8 : 0);
+    for (int i = 0; i < w->nb; ++i) {
+      double* bd = o->body_d + (size_t)(bo + i) * B2G_BODY_ND;
+      int32_t* bi = o->body_i + (size_t)(bo + i) * B2G_BODY_NI;
+      const Body* b = &w->b[i];
+      memset(bd, 0, sizeof(double) * B2G_BODY_ND);
+      bi[B2G_BI_TYPE] = b->type;
+      bi[B2G_BI_FLAGS] = (b->awake ? B2G_BF_AWAKE : 0) | (b->allow_sleep ? B2G_BF_ALLOW_SLEEP : 0) |
+                         (b->fixed_rot ? B2G_BF_FIXED_ROT : 0) | (b->active ? B2G_BF_ACTIVE : 0) |
+                         (b->bullet ? B2G_BF_BULLET : 0);
+      bi[B2G_BI_ISLAND] = b->island_id;
+      bi[B2G_BI_PAD] = 0;
+      bd[B2G_BD_XF_PX] = b->xf.p.x; bd[B2G_BD_XF_PY] = b->xf.p.y;
+      bd[B2G_BD_XF_C] = b->xf.R.c1x; bd[B2G_BD_XF_S] = b->xf.R.c1y;
+      bd[B2G_BD_LC_X] = b->lc.x; bd[B2G_BD_LC_Y] = b->lc.y;
+      bd[B2G_BD_C0_X] = b->c0.x; bd[B2G_BD_C0_Y] = b->c0.y;
+      bd[B2G_BD_C_X] = b->c.x; bd[B2G_BD_C_Y] = b->c.y;
+      bd[B2G_BD_A0] = b->a0; bd[B2G_BD_A] = b->a;
+      bd[B2G_BD_V_X] = b->v.x; bd[B2G_BD_V_Y] = b->v.y; bd[B2G_BD_W] = b->w;
+      bd[B2G_BD_F_X] = b->force.x; bd[B2G_BD_F_Y] = b->force.y; bd[B2G_BD_TORQUE] = b->torque;
+      bd[B2G_BD_MASS] = b->mass; bd[B2G_BD_INV_MASS] = b->inv_mass;
+      bd[B2G_BD_I] = b->I; bd[B2G_BD_INV_I] = b->invI;
+      bd[B2G_BD_GRAV_SCALE] = b->gscale; bd[B2G_BD_LIN_DAMP] = b->ldamp; bd[B2G_BD_ANG_DAMP] = b->adamp;
+      bd[B2G_BD_SLEEP_TIME] = b->sleep_time;
+    }
+    for (int i = 0; i < w->nf; ++i) {
+      double* fd = o->fix_d + (size_t)(fo + i) * B2G_FIX_ND;
+      const Fix* f = &w->f[i];
+      fd[B2G_FD_FAT_LX] = f->fat.lo.x; fd[B2G_FD_FAT_LY] = f->fat.lo.y;
+      fd[B2G_FD_FAT_UX] = f->fat.hi.x; fd[B2G_FD_FAT_UY] = f->fat.hi.y;
+    }
+    if (co + w->nc > o->con_cap || mo + w->nmove > o->move_cap) return -2;
+    for (int i = 0; i < w->nc; ++i) {
+      double* cd = o->con_d + (size_t)(co + i) * B2G_CON_ND;
+      int32_t* ci = o->con_i + (size_t)(co + i) * B2G_CON_NI;
+      const Contact* c = &w->c[i];
+      memset(cd, 0, sizeof(double) * B2G_CON_ND);
+      memset(ci, 0, sizeof(int32_t) * B2G_CON_NI);
+      ci[B2G_CI_FA] = c->fa; ci[B2G_CI_FB] = c->fb;
+      ci[B2G_CI_FLAGS] = (c->touching ? B2G_CF_TOUCHING : 0) | (c->enabled ? B2G_CF_ENABLED : 0) |
+                         (c->filter ? B2G_CF_FILTER : 0);
+      cd[B2G_CD_FRICTION] = c->friction; cd[B2G_CD_RESTITUTION] = c->restitution;
+      manifold_save(&c->m, cd, ci);
+    }
+    for (int i = 0; i < w->nmove; ++i) o->move[mo + i] = w->move[i];
+    bo += w->nb; fo += w->nf; co += w->nc; mo += w->nmove;
+    o->con_off[wi + 1] = co;
+    o->move_off[wi + 1] = mo;
+  }
+  return 0;
+}
+
+/* Replay hint for world wi: n unordered fixture pairs (2n ints) in the
+   reference's contact-creation order.  n = 0 removes the hint. */
+ORC_EXPORT int orc_set_hint(void* h, int wi, const int32_t* pairs, int n) {
+  Batch* B = (Batch*)h;
+  if (wi < 0 || wi >= B->nw) return -1;
+  World* w = &B->w[wi];
+  free(w->hint);
+  w->hint = NULL;
+  w->hint_n = 0;
+  if (n <= 0) return 0;
+  w->hint = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+  if (!w->hint) return -1;
+  for (int i = 0; i < n; ++i) w->hint[i] = ukey(pairs[2 * i], pairs[2 * i + 1]);
+  w->hint_n = n;
+  return 0;
+}
+
+ORC_EXPORT int orc_error(void* h) {
+  Batch* B = (Batch*)h;
+  int e = 0;
+  for (int i = 0; i < B->nw; ++i) e |= B->w[i].err;
+  return e;
+}
+
+/* Narrow-phase seam: evaluates one contact (fixtures A, B oriented as the
+   registry orders them) at body transforms xf = (px, py, c, s), starting from
+   a zeroed manifold.  Writes the manifold into one contact record. */
+ORC_EXPORT int orc_evaluate(const double* fa_d, const int32_t* fa_i, const double* fb_d, const int32_t* fb_i,
+                            const double* xfa, const double* xfb, double* con_d, int32_t* con_i) {
+  init_consts();
+  Shape A, B;
+  shape_load(&A, fa_d, fa_i);
+  shape_load(&B, fb_d, fb_i);
+  XF ta, tb;
+  ta.p = v2(xfa[0], xfa[1]); ta.R.c1x = xfa[2]; ta.R.c1y = xfa[3]; ta.R.c2x = -xfa[3]; ta.R.c2y = xfa[2];
+  tb.p = v2(xfb[0], xfb[1]); tb.R.c1x = xfb[2]; tb.R.c1y = xfb[3]; tb.R.c2x = -xfb[3]; tb.R.c2y = xfb[2];
+  Manifold m;
+  memset(&m, 0, sizeof(m));
+  if (evaluate(&m, &A, ta, &B, tb)) return -1;
+  manifold_save(&m, con_d, con_i);
+  return 0;
+}
+
+/* Broad-phase seam: pairs among n fat AABBs (n x 4: lx, ly, ux, uy) for the
+   move buffer `move`, in the oracle's canonical order (move-buffer order, then
+   ascending partner).  Returns the pair count, or -1 (allocation) / -2 (cap). */
+ORC_EXPORT int orc_pairs(int n, const double* fat, int nmove, const int32_t* move, int32_t* out, int cap) {
+  init_consts();
+  AABB* a = (AABB*)malloc(sizeof(AABB) * (size_t)(n > 0 ? n : 1));
+  int* mv = (int*)malloc(sizeof(int) * (size_t)(nmove > 0 ? nmove : 1));
+  if (!a || !mv) { free(a); free(mv); return -1; }
+  for (int i = 0; i < n; ++i) { a[i].lo = v2(fat[4 * i], fat[4 * i + 1]); a[i].hi = v2(fat[4 * i + 2], fat[4 * i + 3]); }
+  for (int i = 0; i < nmove; ++i) mv[i] = move[i];
+  int* pairs = NULL;
+  int np = 0;
+  int rc = update_pairs(a, n, mv, nmove, &pairs, &np);
+  free(a);
+  free(mv);
+  if (rc) return -1;
+  if (np > cap) { free(pairs); return -2; }
+  for (int i = 0; i < 2 * np; ++i) out[i] = pairs[i];
+  free(pairs);
+  return np;
+}
+
+/* AABB of one fixture's shape at a transform (px, py, c, s); out = lx, ly, ux, uy. */
+ORC_EXPORT void orc_compute_aabb(const double* fd, const int32_t* fi, const double* xf, double* out) {
+  init_consts();
+  Shape s;
+  shape_load(&s, fd, fi);
+  XF t;
+  t.p = v2(xf[0], xf[1]); t.R.c1x = xf[2]; t.R.c1y = xf[3]; t.R.c2x = -xf[3]; t.R.c2y = xf[2];
+  AABB a = compute_aabb(&s, t);
+  out[0] = a.lo.x; out[1] = a.lo.y; out[2] = a.hi.x; out[3] = a.hi.y;
+}

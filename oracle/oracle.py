@@ -1,0 +1,133 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+(as the checker) and by ``bench.py``'s cpu_baseline leg.  The product package
+``pypybox2d_amd`` never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from pypybox2d_amd import layout as L
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        l = ctypes.CDLL(_LIB_PATH)
+        vp, i32, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+        l.orc_load.restype = vp
+        l.orc_load.argtypes = [ctypes.POINTER(L.CBatch)]
+        l.orc_free.argtypes = [vp]
+        l.orc_step.argtypes = [vp, f64, ctypes.c_int, ctypes.c_int]
+        l.orc_step_world.argtypes = [vp, ctypes.c_int, f64, ctypes.c_int, ctypes.c_int]
+        l.orc_counts.argtypes = [vp, vp]
+        l.orc_save.argtypes = [vp, ctypes.POINTER(L.CBatchOut)]
+        l.orc_set_hint.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
+        l.orc_error.argtypes = [vp]
+        l.orc_evaluate.argtypes = [vp] * 8
+        l.orc_pairs.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int]
+        l.orc_compute_aabb.argtypes = [vp, vp, vp, vp]
+        l.orc_set_normalize_eps.argtypes = [f64]
+        _ = i32
+        _lib = l
+    return _lib
+
+
+ORC_E_UNSUPPORTED, ORC_E_HINT, ORC_E_ALLOC = 1, 2, 4
+
+
+class Oracle:
+    """A batch of worlds stepped by the oracle."""
+
+    def __init__(self, batch: L.BatchArrays):
+        self._batch = batch.normalized()
+        self._struct = self._batch.as_struct()
+        self.h = lib().orc_load(ctypes.byref(self._struct))
+        if not self.h:
+            raise ValueError("oracle rejected the batch")
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib().orc_free(h)
+            self.h = None
+
+    def step(self, dt=1.0 / 60, vel_iters=8, pos_iters=3) -> int:
+        return lib().orc_step(self.h, dt, vel_iters, pos_iters)
+
+    def step_world(self, w: int, dt=1.0 / 60, vel_iters=8, pos_iters=3) -> int:
+        return lib().orc_step_world(self.h, w, dt, vel_iters, pos_iters)
+
+    def set_hint(self, w: int, pairs) -> None:
+        p = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        rc = lib().orc_set_hint(self.h, w, p.ctypes.data if p.size else None, int(p.shape[0]))
+        if rc:
+            raise RuntimeError("orc_set_hint failed")
+
+    def error(self) -> int:
+        return lib().orc_error(self.h)
+
+    def counts(self):
+        c = np.zeros(4, np.int32)
+        lib().orc_counts(self.h, c.ctypes.data)
+        return c
+
+    def state(self) -> L.BatchArrays:
+        c = self.counts()
+        out = L.OutBuffers(self._batch, con_cap=int(c[2]) + 16, move_cap=int(c[3]) + 16)
+        rc = lib().orc_save(self.h, ctypes.byref(out.struct))
+        if rc:
+            raise RuntimeError("orc_save failed: %d" % rc)
+        return out.to_batch()
+
+
+def evaluate(fa_d, fa_i, fb_d, fb_i, xfa, xfb):
+    """Manifold of one contact (fixtures oriented A, B) -> (con_d row, con_i row)."""
+    cd = np.zeros(L.CON_ND)
+    ci = np.zeros(L.CON_NI, np.int32)
+    args = [np.ascontiguousarray(fa_d, np.float64), np.ascontiguousarray(fa_i, np.int32),
+            np.ascontiguousarray(fb_d, np.float64), np.ascontiguousarray(fb_i, np.int32),
+            np.ascontiguousarray(xfa, np.float64), np.ascontiguousarray(xfb, np.float64)]
+    rc = lib().orc_evaluate(*[a.ctypes.data for a in args], cd.ctypes.data, ci.ctypes.data)
+    if rc:
+        raise ValueError("no collider for this shape pair")
+    return cd, ci
+
+
+def pairs(fat, move):
+    """Canonical new-pair list of the broad-phase for fat AABBs (n x 4) and a move buffer."""
+    fat = np.ascontiguousarray(fat, np.float64).reshape(-1, 4)
+    move = np.ascontiguousarray(move, np.int32)
+    cap = max(16, fat.shape[0] * max(1, move.shape[0]))
+    out = np.zeros((cap, 2), np.int32)
+    n = lib().orc_pairs(fat.shape[0], fat.ctypes.data, move.shape[0],
+                        move.ctypes.data if move.size else None, out.ctypes.data, cap)
+    if n < 0:
+        raise RuntimeError("orc_pairs failed: %d" % n)
+    return out[:n].copy()
+
+
+def compute_aabb(fd, fi, xf):
+    out = np.zeros(4)
+    a = [np.ascontiguousarray(fd, np.float64), np.ascontiguousarray(fi, np.int32),
+         np.ascontiguousarray(xf, np.float64)]
+    lib().orc_compute_aabb(a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data, out.ctypes.data)
+    return out

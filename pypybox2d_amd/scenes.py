@@ -1,0 +1,151 @@
+"""Synthetic scenes of the benchmark and parity workloads, as plain data.
+
+A scene is a dict ``{"gravity", "do_sleep", "warm_starting", "bodies"}``;
+every body is a dict with ``type`` ("static" / "dynamic"), ``position``,
+``angle``, ``linear_velocity``, ``angular_velocity`` and a list of
+``fixtures`` (``shape``, ``density``, ``friction``, ``restitution``).  Shapes
+are tuples: ``("box", hx, hy)``, ``("circle", r)``, ``("polygon", [(x, y),
+...])`` (convex, counter-clockwise) or ``("edge", (x1, y1), (x2, y2))``.
+
+The same spec builds a pypybox2d world (through its public API) or a world of
+this package, so both start from the same inputs.
+
+Workloads (gravity (0, -10), dt = 1/60, 8 velocity / 3 position iterations):
+  pyramid      10-row box pyramid on an edge ground (55 boxes)
+  stacks       5 columns of 10 boxes on a static box ground, seeded x jitter
+  mixed_world  one world of boxes / circles / convex polygons above an edge
+  mixed_batch  many mixed worlds with body counts uniform in [32, 512]
+  pyramids     N pyramids side by side on their own ground edges
+  lognormal_batch  many mixed worlds with lognormal body counts
+"""
+from __future__ import annotations
+
+import math
+from typing import List
+
+import numpy as np
+
+
+def _body(type_, position, fixtures, angle=0.0, linear_velocity=(0.0, 0.0), angular_velocity=0.0):
+    return {"type": type_, "position": tuple(position), "angle": angle,
+            "linear_velocity": tuple(linear_velocity), "angular_velocity": angular_velocity,
+            "fixtures": fixtures}
+
+
+def _fix(shape, density=0.0, friction=0.2, restitution=0.0):
+    return {"shape": shape, "density": density, "friction": friction, "restitution": restitution}
+
+
+def pyramid(rows: int = 10, x0: float = 0.0, ground_half: float = 40.0, with_ground: bool = True):
+    """Box pyramid: row i has rows-i boxes at x = -7 + 0.5625 i + 1.125 j, y = 0.75 + i."""
+    bodies = []
+    if with_ground:
+        bodies.append(_body("static", (0.0, 0.0),
+                            [_fix(("edge", (x0 - ground_half, 0.0), (x0 + ground_half, 0.0)))]))
+    for i in range(rows):
+        for j in range(rows - i):
+            x = -7 + 0.5625 * i + 1.125 * j
+            if x0 != 0.0:
+                x = x + x0
+            bodies.append(_body("dynamic", (x, 0.75 + 1.0 * i),
+                                [_fix(("box", 0.5, 0.5), density=5.0, friction=0.6)]))
+    return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies}
+
+
+def stacks(seed: int = 0, columns=(-6.0, -3.0, 0.0, 3.0, 6.0), rows: int = 10):
+    """Vertical stacks of half-extent 0.5 boxes on a static 40x1 box ground at (0, -1)."""
+    rng = np.random.default_rng(seed)
+    bodies = [_body("static", (0.0, -1.0), [_fix(("box", 40.0, 1.0))])]
+    for cx in columns:
+        for k in range(rows):
+            x = float(cx) + float(rng.uniform(-0.01, 0.01))
+            bodies.append(_body("dynamic", (x, 0.75 + 1.1 * k),
+                                [_fix(("box", 0.5, 0.5), density=1.0, friction=0.3)]))
+    return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies}
+
+
+def _random_shape(rng):
+    kind = int(rng.choice(3, p=[0.4, 0.3, 0.3]))
+    if kind == 0:
+        hx, hy = (float(v) for v in rng.uniform(0.2, 1.0, 2))
+        return ("box", hx, hy), math.hypot(hx, hy)
+    if kind == 1:
+        r = float(rng.uniform(0.2, 1.0))
+        return ("circle", r), r
+    k = int(rng.integers(3, 9))
+    rad = float(rng.uniform(0.3, 1.0))
+    ang = 2.0 * np.pi * np.arange(k) / k + rng.uniform(-0.3, 0.3, k) * (np.pi / k)
+    ang = np.sort(ang)
+    verts = [(float(rad * math.cos(a)), float(rad * math.sin(a))) for a in ang]
+    return ("polygon", verts), rad
+
+
+def mixed_world(rng, n_bodies: int, half_width: float = 20.0, height: float = 40.0):
+    """Boxes / circles / convex polygons (40/30/30) placed without overlap above an edge."""
+    bodies = [_body("static", (0.0, 0.0), [_fix(("edge", (-40.0, 0.0), (40.0, 0.0)))])]
+    cell = 3.0
+    grid = {}
+    placed = []
+    for _ in range(n_bodies):
+        shape, R = _random_shape(rng)
+        density = float(rng.uniform(0.5, 2.0))
+        friction = float(rng.uniform(0.2, 0.8))
+        for _try in range(10000):
+            x = float(rng.uniform(-half_width, half_width))
+            y = float(rng.uniform(1.0, 1.0 + height))
+            if y - R < 0.05:
+                continue
+            gx, gy = int(math.floor(x / cell)), int(math.floor(y / cell))
+            ok = True
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for (px, py, pr) in grid.get((gx + dx, gy + dy), ()):
+                        if (px - x) ** 2 + (py - y) ** 2 < (pr + R + 0.05) ** 2:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                break
+        else:
+            raise RuntimeError("could not place body")
+        grid.setdefault((gx, gy), []).append((x, y, R))
+        placed.append((x, y, R))
+        v = rng.normal(0.0, 1.0, 2)
+        w = float(rng.normal(0.0, 0.5))
+        angle = float(rng.uniform(-np.pi, np.pi))
+        bodies.append(_body("dynamic", (x, y), [_fix(shape, density=density, friction=friction)],
+                            angle=angle, linear_velocity=(float(v[0]), float(v[1])), angular_velocity=w))
+    return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies}
+
+
+def mixed_batch(n_worlds: int = 4096, seed: int = 1, lo: int = 32, hi: int = 512) -> List[dict]:
+    """Independent mixed worlds with body counts uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(lo, hi + 1, n_worlds)
+    return [mixed_world(rng, int(n)) for n in counts]
+
+
+def lognormal_batch(n_worlds: int = 16384, seed: int = 2, median: float = 64.0, sigma: float = 0.75,
+                    lo: int = 16, hi: int = 1024) -> List[dict]:
+    """Mixed worlds with body counts clip(round(median * exp(sigma N(0,1))), lo, hi)."""
+    rng = np.random.default_rng(seed)
+    counts = np.clip(np.round(median * np.exp(sigma * rng.normal(size=n_worlds))), lo, hi).astype(int)
+    return [mixed_world(rng, int(n)) for n in counts]
+
+
+def pyramids(count: int = 200, spacing: float = 30.0):
+    """One world holding `count` 10-row pyramids, each on its own static ground edge."""
+    bodies = []
+    for k in range(count):
+        x0 = spacing * k
+        bodies.append(_body("static", (0.0, 0.0), [_fix(("edge", (-12.0 + x0, 0.0), (12.0 + x0, 0.0)))]))
+    for k in range(count):
+        bodies.extend(pyramid(x0=spacing * k, with_ground=False)["bodies"])
+    return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies}
+
+
+def dynamic_body_count(scene) -> int:
+    return sum(1 for b in scene["bodies"] if b["type"] == "dynamic")
