@@ -1,0 +1,117 @@
+/*
+ * b2g.h -- C-ABI of the MI355X contact-pipeline library (libb2g.so).
+ *
+ * The reference, pypybox2d, has no native boundary on this path: the per-step
+ * contact pipeline sits behind Python method seams.  Each entry point below
+ * names the reference interface it replaces.  A Python caller binds them with
+ * ctypes (pypybox2d_amd/_native.py); the data records are described in
+ * b2g_layout.h.
+ *
+ * All functions return B2G_OK (0) or a negative status.  A context owns one
+ * HIP stream on one GPU and is used from one host thread at a time.  No
+ * callback crosses the boundary; the caller owns every host array it passes
+ * and the library copies what it keeps.
+ */
+#ifndef B2G_H
+#define B2G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "b2g_layout.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define B2G_API __attribute__((visibility("default")))
+#define B2G_VERSION 1
+
+/* status codes */
+#define B2G_OK 0
+#define B2G_E_INVALID (-1)     /* bad argument or record (reference: ValueError) */
+#define B2G_E_CAPACITY (-2)    /* a device buffer is too small; re-create with larger capacity */
+#define B2G_E_HIP (-3)         /* a HIP runtime call failed */
+#define B2G_E_NAN (-4)         /* non-finite state detected */
+#define B2G_E_UNSUPPORTED (-5) /* feature outside this path (sensor, loop shape, ...) */
+#define B2G_E_NODEVICE (-6)    /* no usable GPU */
+
+/* bulk precision of a context */
+#define B2G_FP32 0 /* FP32 manifolds/solver; FP64 positions, angles, fat AABBs, sleep timers */
+#define B2G_FP64 1 /* everything FP64 */
+
+typedef struct b2g_desc {
+  int32_t precision;     /* B2G_FP32 or B2G_FP64 */
+  int32_t device;        /* HIP device ordinal */
+  double contact_factor; /* contact slots per fixture in each world (default 8) */
+  int32_t contact_extra; /* extra contact slots per world (default 64) */
+  int32_t reserved;
+} b2g_desc;
+
+typedef struct b2g_ctx b2g_ctx;
+
+B2G_API int b2g_version(void);
+
+/* Create / destroy a context (one GPU, one stream).  Replaces the construction
+   of World (world.py:64-91) and its ContactManager/BroadPhase
+   (contact_manager.py:34-44, broadphase.py:36-41) as far as the device state
+   goes. */
+B2G_API int b2g_create(const b2g_desc* desc, b2g_ctx** out);
+B2G_API int b2g_destroy(b2g_ctx* ctx);
+
+/* Copy a batch of worlds to the device (replaces the world-building calls
+   World.create_body / Body.attach_fixture / Fixture._create_proxies,
+   world.py:183-233, body.py:185-214, fixture.py:250-261, whose results the
+   caller encodes in the records).  Replaces any previous batch. */
+B2G_API int b2g_upload(b2g_ctx* ctx, const b2g_batch* batch);
+
+/* Advance every world by one World.step(dt, vel_iters, pos_iters)
+   (world.py:307-354) with continuous physics off: ContactManager.collide
+   (contact_manager.py:174-223), World._solve with Island.solve
+   (world.py:475-592, island.py:65-179) and find_new_contacts
+   (contact_manager.py:102-158).  Asynchronous; see b2g_sync. */
+B2G_API int b2g_step(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters);
+
+/* n steps; with use_graph != 0 the step is captured once in a HIP graph and
+   replayed (same semantics as n calls of b2g_step). */
+B2G_API int b2g_step_n(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, int use_graph);
+
+/* Wait for queued work and report device-side errors (capacity, unsupported). */
+B2G_API int b2g_sync(b2g_ctx* ctx);
+
+/* counts[0..3] = bodies, fixtures, contact slots in use, islands of the last step. */
+B2G_API int b2g_counts(b2g_ctx* ctx, int32_t* counts);
+
+/* Copy the full state back (bodies, fat AABBs, ordered contacts with their
+   manifolds and impulses, move buffer) into caller arrays.  Replaces reading
+   Body/Contact/Manifold attributes after World.step. */
+B2G_API int b2g_download(b2g_ctx* ctx, b2g_batch_out* out);
+
+B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
+
+/* Run n eager steps with HIP events between the step's kernels and write the
+   average milliseconds per kernel: ms_out[0..4] = collide, islands, solve,
+   sync, pairs. */
+B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, float* ms_out);
+
+/* Parity seam, narrow phase: evaluates n contacts (fixture records A and B
+   already oriented as the contact registry orders them, contact.py:305-326,
+   at body transforms xf = (px, py, cos, sin)) from zeroed manifolds, like
+   Contact.evaluate (contact.py:343-373, collision.py).  Writes one contact
+   record per contact (manifold fields only). */
+B2G_API int b2g_narrowphase(int precision, int n, const double* fa_d, const int32_t* fa_i, const double* fb_d,
+                            const int32_t* fb_i, const double* xfa, const double* xfb, double* con_d,
+                            int32_t* con_i);
+
+/* Parity seam, broad phase: the pair set BroadPhase.update_pairs
+   (broadphase.py:83-109) reports for fat AABBs fat[n][4] (lx, ly, ux, uy) and
+   a move buffer, as (moved proxy, partner) in canonical order (move-buffer
+   position, then partner index).  out holds cap pairs (2 * cap ints). */
+B2G_API int b2g_pairs(int n, const double* fat, int nmove, const int32_t* move, int32_t* out, int cap,
+                      int32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* B2G_H */

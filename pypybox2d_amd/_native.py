@@ -1,0 +1,171 @@
+"""ctypes binding of libb2g.so (the gfx950 HIP library, include/b2g.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+it is missing or the GPU is absent, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import layout as L
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libb2g.so")
+
+B2G_OK = 0
+B2G_E_INVALID, B2G_E_CAPACITY, B2G_E_HIP, B2G_E_NAN, B2G_E_UNSUPPORTED, B2G_E_NODEVICE = -1, -2, -3, -4, -5, -6
+B2G_FP32, B2G_FP64 = 0, 1
+
+EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step", "b2g_step_n", "b2g_sync",
+           "b2g_counts", "b2g_download", "b2g_last_error", "b2g_profile_step", "b2g_narrowphase", "b2g_pairs")
+
+_ERRNAMES = {B2G_E_INVALID: "invalid argument", B2G_E_CAPACITY: "capacity exceeded", B2G_E_HIP: "HIP error",
+             B2G_E_NAN: "non-finite state", B2G_E_UNSUPPORTED: "unsupported feature",
+             B2G_E_NODEVICE: "no GPU device"}
+
+
+class B2GError(RuntimeError):
+    def __init__(self, code: int, msg: str = ""):
+        self.code = code
+        super().__init__("b2g error %d (%s)%s" % (code, _ERRNAMES.get(code, "?"), (": " + msg) if msg else ""))
+
+
+class CDesc(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("device", ctypes.c_int32), ("contact_factor", ctypes.c_double),
+                ("contact_extra", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def load(path: Optional[str] = None):
+    """Load libb2g.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError("libb2g.so is not built; run `python -m pypybox2d_amd.build`")
+    lib = ctypes.CDLL(path)
+    vp, i32, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+    lib.b2g_version.restype = ctypes.c_int
+    lib.b2g_create.argtypes = [ctypes.POINTER(CDesc), ctypes.POINTER(vp)]
+    lib.b2g_destroy.argtypes = [vp]
+    lib.b2g_upload.argtypes = [vp, ctypes.POINTER(L.CBatch)]
+    lib.b2g_step.argtypes = [vp, f64, ctypes.c_int, ctypes.c_int]
+    lib.b2g_step_n.argtypes = [vp, ctypes.c_int, f64, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.b2g_sync.argtypes = [vp]
+    lib.b2g_counts.argtypes = [vp, vp]
+    lib.b2g_download.argtypes = [vp, ctypes.POINTER(L.CBatchOut)]
+    lib.b2g_last_error.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+    lib.b2g_profile_step.argtypes = [vp, ctypes.c_int, f64, ctypes.c_int, ctypes.c_int, vp]
+    lib.b2g_narrowphase.argtypes = [ctypes.c_int, ctypes.c_int] + [vp] * 8
+    lib.b2g_pairs.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
+    _ = i32
+    _lib = lib
+    return lib
+
+
+class Engine:
+    """A batch of worlds resident on one GPU (a b2g_ctx)."""
+
+    def __init__(self, precision: str = "fp32", device: int = 0, contact_factor: float = 8.0,
+                 contact_extra: int = 64):
+        self.lib = load()
+        self.precision = precision
+        code = {"fp32": B2G_FP32, "fp64": B2G_FP64}[precision]
+        desc = CDesc(code, device, float(contact_factor), int(contact_extra), 0)
+        h = ctypes.c_void_p()
+        rc = self.lib.b2g_create(ctypes.byref(desc), ctypes.byref(h))
+        if rc:
+            raise B2GError(rc, "b2g_create")
+        self.h = h
+        self._like: Optional[L.BatchArrays] = None
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc:
+            buf = ctypes.create_string_buffer(512)
+            self.lib.b2g_last_error(self.h, buf, 512)
+            raise B2GError(rc, "%s: %s" % (what, buf.value.decode(errors="replace")))
+
+    def close(self) -> None:
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            self.lib.b2g_destroy(h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, batch: L.BatchArrays) -> None:
+        b = batch.normalized()
+        s = b.as_struct()
+        self._check(self.lib.b2g_upload(self.h, ctypes.byref(s)), "upload")
+        self._like = b
+
+    def step(self, dt: float = 1.0 / 60, vel_iters: int = 8, pos_iters: int = 3) -> None:
+        self._check(self.lib.b2g_step(self.h, dt, vel_iters, pos_iters), "step")
+
+    def step_n(self, n: int, dt: float = 1.0 / 60, vel_iters: int = 8, pos_iters: int = 3,
+               graph: bool = True) -> None:
+        self._check(self.lib.b2g_step_n(self.h, n, dt, vel_iters, pos_iters, 1 if graph else 0), "step_n")
+
+    def sync(self) -> None:
+        self._check(self.lib.b2g_sync(self.h), "sync")
+
+    def counts(self) -> np.ndarray:
+        c = np.zeros(4, np.int32)
+        self._check(self.lib.b2g_counts(self.h, c.ctypes.data), "counts")
+        return c
+
+    def download(self) -> L.BatchArrays:
+        c = self.counts()
+        out = L.OutBuffers(self._like, con_cap=int(c[2]) + 16, move_cap=int(c[1]) + 16)
+        self._check(self.lib.b2g_download(self.h, ctypes.byref(out.struct)), "download")
+        return out.to_batch()
+
+    def profile(self, n: int, dt: float = 1.0 / 60, vel_iters: int = 8, pos_iters: int = 3) -> np.ndarray:
+        ms = np.zeros(5, np.float32)
+        self._check(self.lib.b2g_profile_step(self.h, n, dt, vel_iters, pos_iters, ms.ctypes.data), "profile")
+        return ms
+
+
+def narrowphase(precision: str, fa_d, fa_i, fb_d, fb_i, xfa, xfb):
+    """Parity seam: manifolds of n oriented fixture pairs (rows of the records)."""
+    lib = load()
+    arrs = [np.ascontiguousarray(fa_d, np.float64).reshape(-1, L.FIX_ND),
+            np.ascontiguousarray(fa_i, np.int32).reshape(-1, L.FIX_NI),
+            np.ascontiguousarray(fb_d, np.float64).reshape(-1, L.FIX_ND),
+            np.ascontiguousarray(fb_i, np.int32).reshape(-1, L.FIX_NI),
+            np.ascontiguousarray(xfa, np.float64).reshape(-1, 4),
+            np.ascontiguousarray(xfb, np.float64).reshape(-1, 4)]
+    n = arrs[0].shape[0]
+    cd = np.zeros((n, L.CON_ND))
+    ci = np.zeros((n, L.CON_NI), np.int32)
+    rc = lib.b2g_narrowphase({"fp32": B2G_FP32, "fp64": B2G_FP64}[precision], n,
+                             *[a.ctypes.data for a in arrs], cd.ctypes.data, ci.ctypes.data)
+    if rc:
+        raise B2GError(rc, "b2g_narrowphase")
+    return cd, ci
+
+
+def pairs(fat, move):
+    """Parity seam: canonical broad-phase pair list for fat AABBs and a move buffer."""
+    lib = load()
+    fat = np.ascontiguousarray(fat, np.float64).reshape(-1, 4)
+    move = np.ascontiguousarray(move, np.int32)
+    cap = max(16, fat.shape[0] * max(1, move.shape[0]))
+    out = np.zeros((cap, 2), np.int32)
+    cnt = np.zeros(1, np.int32)
+    rc = lib.b2g_pairs(fat.shape[0], fat.ctypes.data, move.shape[0], move.ctypes.data if move.size else None,
+                       out.ctypes.data, cap, cnt.ctypes.data)
+    if rc:
+        raise B2GError(rc, "b2g_pairs")
+    return out[:int(cnt[0])].copy()

@@ -1,0 +1,49 @@
+"""Scene specs (pypybox2d_amd.scenes) -> drop-in worlds and batch records."""
+from __future__ import annotations
+
+from typing import List
+
+from . import layout as L
+from .fixture import Fixture
+from .shapes import Circle, Edge, Polygon
+from .world import World
+
+
+def make_shape(spec):
+    kind = spec[0]
+    if kind == "box":
+        return Polygon(box=(spec[1], spec[2]))
+    if kind == "circle":
+        return Circle(radius=spec[1])
+    if kind == "polygon":
+        return Polygon(vertices=spec[1])
+    if kind == "edge":
+        return Edge(spec[1], spec[2])
+    raise ValueError("unknown shape %r" % (kind,))
+
+
+def build_world(scene, precision: str = "fp64", device: int = 0) -> World:
+    """Builds the scene through the public API, in the spec's body order."""
+    world = World(gravity=scene["gravity"], do_sleep=scene["do_sleep"], warm_starting=scene["warm_starting"],
+                  continuous_physics=False, precision=precision, device=device)
+    for bs in scene["bodies"]:
+        fixtures = [Fixture(make_shape(f["shape"]), density=f["density"], friction=f["friction"],
+                            restitution=f["restitution"]) for f in bs["fixtures"]]
+        kw = dict(position=bs["position"], angle=bs["angle"], linear_velocity=bs["linear_velocity"],
+                  angular_velocity=bs["angular_velocity"], fixtures=fixtures)
+        if bs["type"] == "static":
+            world.create_static_body(**kw)
+        elif bs["type"] == "kinematic":
+            world.create_kinematic_body(**kw)
+        else:
+            world.create_dynamic_body(**kw)
+    return world
+
+
+def scene_batch(scene) -> L.BatchArrays:
+    """Initial state of one scene in the batch layout (host only, no device)."""
+    return build_world(scene)._batch()
+
+
+def scenes_batch(scenes: List[dict]) -> L.BatchArrays:
+    return L.concat([scene_batch(s) for s in scenes])

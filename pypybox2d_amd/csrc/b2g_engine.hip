@@ -1,0 +1,2355 @@
+// b2g_engine.hip -- batched MI355X (gfx950) contact pipeline and its C-ABI.
+//
+// One call of b2g_step advances every world of a batch by one
+// World.step(dt, velocity_iterations, position_iterations) (world.py:307-354)
+// with continuous physics off.  Per step the device runs:
+//
+//   k_collide   one lane per contact slot: ContactManager.collide
+//               (contact_manager.py:174-223) for contacts with an awake
+//               non-static body at step start -- fat-AABB overlap test in FP64,
+//               manifold evaluation, warm-start id matching (contact.py:210-268).
+//   k_islands   one workgroup per world: the rest of collide's sequential
+//               awake gating (only worlds holding sleeping bodies), then the
+//               island search of World._solve (world.py:494-580) over an
+//               LDS-resident contact graph, in the reference's DFS order.
+//   k_solve     one lane per island, pulled from a device queue: Island.solve
+//               (island.py:65-179) with ContactSolver (contact.py:375-864).
+//   k_sync      one lane per fixture: Body._synchronize_fixtures + move_proxy
+//               (body.py:898-904, fixture.py:287-298, dynamictree.py:152-188)
+//               with FP64 swept AABBs.
+//   k_pairs     one workgroup per world: BroadPhase.update_pairs + ContactManager
+//               .find_new_contacts (broadphase.py:83-109, contact_manager.py:
+//               102-158): exact FP64 fat-AABB tests of every moved proxy against
+//               the world's proxies, filtering, and contact creation in a
+//               canonical order; then the step's bookkeeping.
+//
+// The bulk arithmetic (manifolds, effective masses, impulses, velocities) runs
+// in T = float (FP32-bulk) or T = double (FP64); positions, angles, transforms,
+// fat AABBs and sleep timers are FP64 in both.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/b2g.h"
+#include "b2g_collide.hpp"
+
+namespace b2g {
+
+// contact slot state bits
+enum : int {
+  CS_LIVE = 1,
+  CS_TOUCHING = 2,
+  CS_ENABLED = 4,
+  CS_FILTER = 8,
+  CS_MTYPE_SHIFT = 4,
+  CS_PC_SHIFT = 6,
+  CS_PROCESSED = 256,
+  CS_WOKE = 512,
+};
+// body flag bits (B2G_BF_*)
+enum : int { BF_AWAKE = 1, BF_ALLOW_SLEEP = 2, BF_FIXED_ROT = 4, BF_ACTIVE = 8, BF_BULLET = 16 };
+enum : int { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
+enum : int { WF_ALLOW_SLEEP = 1, WF_WARM = 2, WF_NEW_FIXTURE = 4, WF_KEEP_FORCES = 8 };
+enum : int { WERR_CAPACITY = 1, WERR_UNSUPPORTED = 2, WERR_NEWPAIRS = 4 };
+
+constexpr int kBlock = 256;
+
+template <typename T>
+struct V4 {
+  T x, y, z, w;
+};
+
+// Island-body state.  Positions, angles and velocities are FP64 in both
+// pipelines: they accumulate many small increments that cancel (weight vs.
+// contact impulses), which FP32 would resolve only to ~1e-4 relative.
+template <typename T>
+struct PV {
+  double cx, cy, a, vx, vy, w;
+};
+
+// Velocity-constraint point: FP32 (or FP64) geometry and effective masses,
+// FP64 accumulated impulses.
+template <typename T>
+struct VCP {
+  T rax, ray, rbx, rby, nm, tm, bias, pad;
+  double ni, ti;
+};
+template <typename T>
+struct VC {
+  T nx, ny, friction, restitution, ma, mb, Ia, Ib;
+  double k11, k12, k22, n11, n12, n21, n22;  // K and K^-1 (n11 = c1x, n21 = c1y, n12 = c2x, n22 = c2y)
+  int ia, ib, npts, pad;
+  VCP<T> p[2];
+};
+template <typename T>
+struct PC {
+  T lp0x, lp0y, lp1x, lp1y, lnx, lny, lpx, lpy;
+  T ra, rb, ma, mb, Ia, Ib, lcax, lcay, lcbx, lcby;
+  int type, npts, ia, ib;
+};
+
+template <typename T>
+struct Dev {
+  int nw, nb, nf, ncap;
+  double dt, inv_dt;
+  int vel_iters, pos_iters;
+  // worlds
+  const int* w_body_off;
+  const int* w_fix_off;
+  const int* w_con_base;
+  const int* w_con_cap;
+  int* w_con_used;
+  int* w_flags;
+  const double* w_grav;
+  double* w_inv_dt0;
+  int* w_err;
+  int* w_move_n;
+  int* w_asleep;
+  int* w_nisl;
+  // bodies
+  int* b_type;
+  int* b_flags;
+  int* b_island;
+  double2* b_c;
+  double* b_a;
+  double2* b_c0;
+  double* b_a0;
+  double4* b_xf;
+  double2* b_lc;
+  double* b_sleep;
+  V2<double>* b_v;
+  double* b_w;
+  T* b_im;
+  T* b_iI;
+  V2<T>* b_f;
+  T* b_t;
+  T* b_gs;
+  T* b_ld;
+  T* b_ad;
+  // fixtures
+  int4* f_info;  // x body (world-local), y shape | nverts << 8 | edge flags << 16, z category | mask << 16, w group
+  int2* f_wb;    // world, global body
+  T* f_geo;
+  double* f_geo64;
+  double* f_rad64;
+  V2<T>* f_mat;
+  double4* f_fat;
+  int* f_moved;
+  int* mv;  // move buffer, per-world segment at w_fix_off[w]
+  // contacts (per-world segments at w_con_base[w], capacity w_con_cap[w])
+  int2* c_pair;
+  int* c_state;
+  V2<T>* c_mat;
+  V4<T>* c_lnlp;
+  V4<T>* c_pts;
+  V4<T>* c_imp;
+  uint2* c_id;
+  // islands
+  int* i_bodies;     // per-world segment at w_body_off[w]
+  int* i_contacts;   // per-world segment at w_con_base[w]
+  int2* i_cidx;      // per-world segment at w_con_base[w]
+  int4* i_info;      // per-world segment at w_body_off[w]: (bstart, nb, cstart, nc | last_allow << 30)
+  int4* q_isl;       // global island queue: (world, bstart, nb, cstart)
+  int* q_nc;         // (nc | last_allow << 30)
+  int* q_count;
+  // solver scratch
+  PV<T>* pv;   // indexed like i_bodies
+  VC<T>* vc;   // indexed like i_contacts
+  PC<T>* pc;
+};
+
+// ---------------------------------------------------------------------------
+// constants (settings.py)
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.141592653589793;
+constexpr double AABB_EXTENSION = 0.1;
+constexpr double AABB_MULTIPLIER = 2.0;
+constexpr double LINEAR_SLOP = 0.005;
+constexpr double VELOCITY_THRESHOLD = 1.0;
+constexpr double MAX_LINEAR_CORRECTION = 0.2;
+constexpr double MAX_TRANSLATION = 2.0;
+constexpr double MAX_ROTATION = 0.5 * kPi;
+constexpr double BAUMGARTE = 0.2;
+constexpr double TIME_TO_SLEEP = 0.5;
+constexpr double LINEAR_SLEEP_TOL = 0.01;
+constexpr double ANGULAR_SLEEP_TOL = 2.0 / 180.0 * kPi;
+
+template <typename T>
+__device__ __forceinline__ Geo<T> geo_of(const Dev<T>& d, int fg) {
+  Geo<T> g;
+  int4 info = d.f_info[fg];
+  g.g = d.f_geo + (size_t)fg * GEO_STRIDE;
+  g.type = info.y & 255;
+  g.n = (info.y >> 8) & 255;
+  g.has0 = (info.y >> 16) & 1;
+  g.has3 = (info.y >> 17) & 1;
+  return g;
+}
+
+__device__ __forceinline__ bool is_static(int type) { return type == BT_STATIC; }
+
+// ---------------------------------------------------------------------------
+// narrow phase for one contact slot (Contact.update, contact.py:210-268)
+// returns the new state word; manifold written in place
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
+  const int f0 = d.w_fix_off[w], b0 = d.w_body_off[w];
+  int2 pr = d.c_pair[s];
+  const int fa = f0 + pr.x, fb = f0 + pr.y;
+  int4 ia = d.f_info[fa], ib = d.f_info[fb];
+  const int ba = b0 + ia.x, bb = b0 + ib.x;
+  if (st & CS_FILTER) {  // refilter (contact_manager.py:200-208)
+    bool ok = d.b_type[ba] == BT_DYNAMIC || d.b_type[bb] == BT_DYNAMIC;
+    if (ok) {
+      if (ia.w == ib.w && ia.w != 0) ok = ia.w > 0;
+      else ok = ((ia.z >> 16) & (ib.z & 0xFFFF)) != 0 && ((ia.z & 0xFFFF) & (ib.z >> 16)) != 0;
+    }
+    if (!ok) return 0;
+    st &= ~CS_FILTER;
+  }
+  double4 A = d.f_fat[fa], B = d.f_fat[fb];
+  if (!aabb_overlap(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w)) return 0;  // destroyed
+  Man<T> m;
+  m.type = (st >> CS_MTYPE_SHIFT) & 3;
+  m.pc = (st >> CS_PC_SHIFT) & 3;
+  V4<T> lnlp = d.c_lnlp[s], pts = d.c_pts[s], imp = d.c_imp[s];
+  uint2 ids = d.c_id[s];
+  m.ln = v2<T>(lnlp.x, lnlp.y);
+  m.lp = v2<T>(lnlp.z, lnlp.w);
+  m.p0 = v2<T>(pts.x, pts.y);
+  m.p1 = v2<T>(pts.z, pts.w);
+  m.id0 = ids.x;
+  m.id1 = ids.y;
+  const int old_pc = m.pc;
+  const uint32_t oid0 = ids.x, oid1 = ids.y;
+  double4 xa = d.b_xf[ba], xb = d.b_xf[bb];
+  XF<T> xfa, xfb;
+  xfa.p = v2<T>(0, 0);
+  xfa.q.c = (T)xa.z;
+  xfa.q.s = (T)xa.w;
+  xfb.p = v2<T>((T)(xb.x - xa.x), (T)(xb.y - xa.y));
+  xfb.q.c = (T)xb.z;
+  xfb.q.s = (T)xb.w;
+  Geo<T> ga = geo_of(d, fa), gb = geo_of(d, fb);
+  if (!evaluate(m, ga, xfa, gb, xfb)) {
+    atomicOr(&d.w_err[w], WERR_UNSUPPORTED);
+    return st;
+  }
+  // warm start: match ids (the dict keeps the last duplicate)
+  T n0 = 0, t0 = 0, n1 = 0, t1 = 0;
+  if (m.pc >= 1) {
+    if (old_pc >= 1 && oid0 == m.id0) { n0 = imp.x; t0 = imp.y; }
+    if (old_pc >= 2 && oid1 == m.id0) { n0 = imp.z; t0 = imp.w; }
+  }
+  if (m.pc >= 2) {
+    if (old_pc >= 1 && oid0 == m.id1) { n1 = imp.x; t1 = imp.y; }
+    if (old_pc >= 2 && oid1 == m.id1) { n1 = imp.z; t1 = imp.w; }
+  }
+  V4<T> nimp;
+  nimp.x = m.pc >= 1 ? n0 : imp.x;
+  nimp.y = m.pc >= 1 ? t0 : imp.y;
+  nimp.z = m.pc >= 2 ? n1 : imp.z;
+  nimp.w = m.pc >= 2 ? t1 : imp.w;
+  d.c_imp[s] = nimp;
+  lnlp.x = m.ln.x; lnlp.y = m.ln.y; lnlp.z = m.lp.x; lnlp.w = m.lp.y;
+  pts.x = m.p0.x; pts.y = m.p0.y; pts.z = m.p1.x; pts.w = m.p1.y;
+  d.c_lnlp[s] = lnlp;
+  d.c_pts[s] = pts;
+  ids.x = m.id0;
+  ids.y = m.id1;
+  d.c_id[s] = ids;
+  const bool touching = m.pc > 0;
+  const bool was = (st & CS_TOUCHING) != 0;
+  int nst = CS_LIVE | CS_ENABLED | (touching ? CS_TOUCHING : 0) | (m.type << CS_MTYPE_SHIFT) | (m.pc << CS_PC_SHIFT) |
+            CS_PROCESSED | (touching != was ? CS_WOKE : 0) | (st & CS_FILTER);
+  return nst;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_collide(Dev<T> d) {
+  const int w = blockIdx.y;
+  const int sl = blockIdx.x * kBlock + threadIdx.x;
+  if (sl >= d.w_con_used[w]) return;
+  const int s = d.w_con_base[w] + sl;
+  int st = d.c_state[s];
+  if (!(st & CS_LIVE)) return;
+  st &= ~(CS_PROCESSED | CS_WOKE);
+  const int f0 = d.w_fix_off[w], b0 = d.w_body_off[w];
+  int2 pr = d.c_pair[s];
+  const int ba = b0 + d.f_info[f0 + pr.x].x, bb = b0 + d.f_info[f0 + pr.y].x;
+  const bool act_a = (d.b_flags[ba] & BF_AWAKE) && !is_static(d.b_type[ba]);
+  const bool act_b = (d.b_flags[bb] & BF_AWAKE) && !is_static(d.b_type[bb]);
+  if (act_a || act_b) st = narrow_update(d, w, s, st);
+  d.c_state[s] = st;
+}
+
+// ---------------------------------------------------------------------------
+// per-world island search
+// ---------------------------------------------------------------------------
+struct IslandSmem {
+  int* off;        // nb + 1
+  int* cur;        // nb
+  uint16_t* adj_s;  // 2 * ccap
+  uint16_t* adj_o;  // 2 * ccap
+  uint8_t* bst;    // nb: bit0 static, bit1 awake, bit2 allow_sleep, bit3 active, bit4 flag, bit5 woken
+  uint16_t* bidx;  // nb
+  uint16_t* stack;  // nb
+  uint16_t* slist;  // nb
+  uint32_t* cflag;  // ccap / 32
+  int* scan;       // kBlock
+  int* misc;       // 8
+};
+
+__device__ __forceinline__ size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+__host__ __device__ inline size_t island_smem_bytes(int nb, int ccap) {
+  size_t s = 0;
+  s += ((size_t)(nb + 1) * 4 + 15) & ~(size_t)15;
+  s += ((size_t)nb * 4 + 15) & ~(size_t)15;
+  s += ((size_t)2 * ccap * 2 + 15) & ~(size_t)15;
+  s += ((size_t)2 * ccap * 2 + 15) & ~(size_t)15;
+  s += ((size_t)nb + 15) & ~(size_t)15;
+  s += ((size_t)nb * 2 + 15) & ~(size_t)15;
+  s += ((size_t)nb * 2 + 15) & ~(size_t)15;
+  s += ((size_t)nb * 2 + 15) & ~(size_t)15;
+  s += ((size_t)(ccap + 31) / 32 * 4 + 15) & ~(size_t)15;
+  s += (size_t)kBlock * 4;
+  s += 64;
+  return s;
+}
+
+// Exclusive scan of a[0..n) in place by one workgroup; returns the total.
+__device__ int block_exclusive_scan(int* a, int n, int* scratch) {
+  const int tid = threadIdx.x;
+  const int per = (n + kBlock - 1) / kBlock;
+  const int beg = tid * per, end = min(n, beg + per);
+  int sum = 0;
+  for (int i = beg; i < end; ++i) sum += a[i];
+  scratch[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    int v = tid >= off ? scratch[tid - off] : 0;
+    __syncthreads();
+    scratch[tid] += v;
+    __syncthreads();
+  }
+  int run = scratch[tid] - sum;
+  for (int i = beg; i < end; ++i) {
+    int v = a[i];
+    a[i] = run;
+    run += v;
+  }
+  __syncthreads();
+  int total = scratch[kBlock - 1];
+  __syncthreads();
+  return total;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
+  const int f0 = d.w_fix_off[w];
+  const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
+  const int used = d.w_con_used[w];
+  IslandSmem L;
+  {
+    unsigned char* p = smem;
+    L.off = (int*)p; p += ((size_t)(nb + 1) * 4 + 15) & ~(size_t)15;
+    L.cur = (int*)p; p += ((size_t)nb * 4 + 15) & ~(size_t)15;
+    L.adj_s = (uint16_t*)p; p += ((size_t)2 * ccap * 2 + 15) & ~(size_t)15;
+    L.adj_o = (uint16_t*)p; p += ((size_t)2 * ccap * 2 + 15) & ~(size_t)15;
+    L.bst = (uint8_t*)p; p += ((size_t)nb + 15) & ~(size_t)15;
+    L.bidx = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
+    L.stack = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
+    L.slist = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
+    L.cflag = (uint32_t*)p; p += ((size_t)(ccap + 31) / 32 * 4 + 15) & ~(size_t)15;
+    L.scan = (int*)p; p += (size_t)kBlock * 4;
+    L.misc = (int*)p;
+  }
+  for (int i = tid; i < nb; i += kBlock) {
+    int fl = d.b_flags[b0 + i];
+    int ty = d.b_type[b0 + i];
+    L.bst[i] = (uint8_t)((ty == BT_STATIC ? 1 : 0) | ((fl & BF_AWAKE) ? 2 : 0) | ((fl & BF_ALLOW_SLEEP) ? 4 : 0) |
+                         ((fl & BF_ACTIVE) ? 8 : 0));
+    L.off[i] = 0;
+  }
+  for (int i = tid; i < (ccap + 31) / 32; i += kBlock) L.cflag[i] = 0;
+  __syncthreads();
+
+  // --- rest of collide's awake gating (contact_manager.py:191-197), in order ---
+  if (d.w_asleep[w] > 0 && tid == 0) {
+    for (int sl = 0; sl < used; ++sl) {
+      const int s = cb + sl;
+      int st = d.c_state[s];
+      if (!(st & CS_LIVE)) continue;
+      int2 pr = d.c_pair[s];
+      const int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
+      if (!(st & CS_PROCESSED)) {
+        bool act = ((L.bst[la] & 2) && !(L.bst[la] & 1)) || ((L.bst[lb] & 2) && !(L.bst[lb] & 1));
+        if (!act) continue;
+        st = narrow_update(d, w, s, st);
+        d.c_state[s] = st;
+        if (!(st & CS_LIVE)) continue;
+      }
+      if (st & CS_WOKE) {
+        if (!(L.bst[la] & 2)) L.bst[la] |= 2 | 32;
+        if (!(L.bst[lb] & 2)) L.bst[lb] |= 2 | 32;
+      }
+    }
+  }
+  __syncthreads();
+
+  // --- touching-contact graph in ContactManager order (body._contacts order) ---
+  for (int sl = tid; sl < used; sl += kBlock) {
+    int st = d.c_state[cb + sl];
+    if ((st & (CS_LIVE | CS_TOUCHING | CS_ENABLED)) == (CS_LIVE | CS_TOUCHING | CS_ENABLED)) {
+      int2 pr = d.c_pair[cb + sl];
+      atomicAdd(&L.off[d.f_info[f0 + pr.x].x], 1);
+      atomicAdd(&L.off[d.f_info[f0 + pr.y].x], 1);
+    }
+  }
+  __syncthreads();
+  int total = block_exclusive_scan(L.off, nb, L.scan);
+  if (tid == 0) L.off[nb] = total;
+  for (int i = tid; i < nb; i += kBlock) L.cur[i] = L.off[i];
+  __syncthreads();
+  for (int sl = tid; sl < used; sl += kBlock) {
+    int st = d.c_state[cb + sl];
+    if ((st & (CS_LIVE | CS_TOUCHING | CS_ENABLED)) == (CS_LIVE | CS_TOUCHING | CS_ENABLED)) {
+      int2 pr = d.c_pair[cb + sl];
+      int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
+      int pa = atomicAdd(&L.cur[la], 1);
+      L.adj_s[pa] = (uint16_t)sl;
+      L.adj_o[pa] = (uint16_t)lb;
+      int pb = atomicAdd(&L.cur[lb], 1);
+      L.adj_s[pb] = (uint16_t)sl;
+      L.adj_o[pb] = (uint16_t)la;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nb; i += kBlock) {  // restore ascending slot order per body
+    int beg = L.off[i], end = L.off[i + 1];
+    for (int k = beg + 1; k < end; ++k) {
+      uint16_t s = L.adj_s[k], o = L.adj_o[k];
+      int j = k - 1;
+      while (j >= beg && L.adj_s[j] > s) {
+        L.adj_s[j + 1] = L.adj_s[j];
+        L.adj_o[j + 1] = L.adj_o[j];
+        --j;
+      }
+      L.adj_s[j + 1] = s;
+      L.adj_o[j + 1] = o;
+    }
+  }
+  __syncthreads();
+
+  // --- depth-first island search (world.py:494-580), one lane ---
+  int* isl_b = d.i_bodies + b0;
+  int* isl_c = d.i_contacts + cb;
+  int4* info = d.i_info + b0;
+  if (tid == 0) {
+    int nbi_tot = 0, nci_tot = 0, nisl = 0;
+    for (int seed = 0; seed < nb; ++seed) {
+      uint8_t sb = L.bst[seed];
+      if (sb & 16) continue;            // island flag
+      if (!(sb & 2) || !(sb & 8)) continue;  // not awake or not active
+      if (sb & 1) continue;             // static
+      int sp = 0, ns = 0;
+      L.stack[sp++] = (uint16_t)seed;
+      L.bst[seed] = sb | 16;
+      const int bstart = nbi_tot, cstart = nci_tot;
+      int last_allow = 0;
+      while (sp > 0) {
+        const int b = L.stack[--sp];
+        uint8_t bs = L.bst[b];
+        last_allow = (bs >> 2) & 1;
+        if (!(bs & 2)) bs |= 2 | 32;  // body.awake = True
+        L.bst[b] = bs;
+        if (bs & 1) {  // static bodies join but do not propagate
+          L.slist[ns++] = (uint16_t)b;
+          continue;
+        }
+        L.bidx[b] = (uint16_t)nbi_tot;
+        isl_b[nbi_tot++] = b;
+        for (int k = L.off[b]; k < L.off[b + 1]; ++k) {
+          const int s = L.adj_s[k];
+          const uint32_t bit = 1u << (s & 31);
+          if (L.cflag[s >> 5] & bit) continue;
+          L.cflag[s >> 5] |= bit;
+          isl_c[nci_tot++] = s;
+          const int o = L.adj_o[k];
+          if (L.bst[o] & 16) continue;
+          L.stack[sp++] = (uint16_t)o;
+          L.bst[o] |= 16;
+        }
+      }
+      for (int k = 0; k < ns; ++k) L.bst[L.slist[k]] &= (uint8_t)~16;  // statics may join other islands
+      info[nisl++] = make_int4(bstart, nbi_tot - bstart, cstart, (nci_tot - cstart) | (last_allow << 30));
+    }
+    L.misc[0] = nisl;
+    L.misc[1] = nbi_tot;
+    L.misc[2] = nci_tot;
+    L.misc[3] = nisl > 0 ? atomicAdd(d.q_count, nisl) : 0;
+  }
+  __syncthreads();
+  const int nisl = L.misc[0], nci_tot = L.misc[2], qbase = L.misc[3];
+  for (int k = tid; k < nisl; k += kBlock) {
+    int4 in = info[k];
+    d.q_isl[qbase + k] = make_int4(w, b0 + in.x, in.y, cb + in.z);
+    d.q_nc[qbase + k] = in.w;
+  }
+  for (int k = tid; k < nci_tot; k += kBlock) {
+    const int s = isl_c[k];
+    int2 pr = d.c_pair[cb + s];
+    const int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
+    int ia = (L.bst[la] & 1) ? -1 - la : (int)L.bidx[la];
+    int ib = (L.bst[lb] & 1) ? -1 - lb : (int)L.bidx[lb];
+    d.i_cidx[cb + k] = make_int2(ia, ib);
+  }
+  for (int i = tid; i < nb; i += kBlock) {
+    uint8_t bs = L.bst[i];
+    int fl = d.b_flags[b0 + i];
+    if (bs & 32) {  // woken: Body.awake setter (body.py:800-812)
+      fl |= BF_AWAKE;
+      d.b_sleep[b0 + i] = 0.0;
+      d.b_flags[b0 + i] = fl;
+    }
+    d.b_island[b0 + i] = ((bs & 16) && !(bs & 1)) ? 1 : -1;
+  }
+  if (tid == 0) d.w_nisl[w] = nisl;
+}
+
+// ---------------------------------------------------------------------------
+// Island.solve, one lane per island (island.py:65-179, contact.py:375-864)
+//
+// In the FP32 pipeline the constraint geometry (world manifold, lever arms,
+// normals, effective masses) and the position solver's geometry are FP32;
+// positions, angles, velocities, accumulated impulses and the 2x2 block LCP
+// are FP64.
+// ---------------------------------------------------------------------------
+using V2d = V2<double>;
+
+template <typename T>
+__device__ __forceinline__ V2d dv2(T x, T y) {
+  return v2<double>((double)x, (double)y);
+}
+
+template <typename T>
+__device__ __forceinline__ void load_pos(const Dev<T>& d, int pvb, int b0, int i, double& cx, double& cy, double& a) {
+  if (i >= 0) {
+    const PV<T>& p = d.pv[pvb + i];
+    cx = p.cx;
+    cy = p.cy;
+    a = p.a;
+  } else {
+    double2 c = d.b_c[b0 + (-1 - i)];
+    cx = c.x;
+    cy = c.y;
+    a = d.b_a[b0 + (-1 - i)];
+  }
+}
+template <typename T>
+__device__ __forceinline__ void load_vel(const Dev<T>& d, int pvb, int b0, int i, V2d& v, double& w) {
+  if (i >= 0) {
+    const PV<T>& p = d.pv[pvb + i];
+    v = v2<double>(p.vx, p.vy);
+    w = p.w;
+  } else {  // static bodies never change
+    v = d.b_v[b0 + (-1 - i)];
+    w = d.b_w[b0 + (-1 - i)];
+  }
+}
+template <typename T>
+__device__ __forceinline__ void store_vel(const Dev<T>& d, int pvb, int i, V2d v, double w) {
+  if (i >= 0) {
+    PV<T>& p = d.pv[pvb + i];
+    p.vx = v.x;
+    p.vy = v.y;
+    p.w = w;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void store_pos(const Dev<T>& d, int pvb, int i, double cx, double cy, double a) {
+  if (i >= 0) {
+    PV<T>& p = d.pv[pvb + i];
+    p.cx = cx;
+    p.cy = cy;
+    p.a = a;
+  }
+}
+
+// World manifold (contact_util.py:195-245) in a frame centred on body A's centre.
+template <typename T>
+__device__ V2<T> world_manifold(int type, int pc, V2<T> ln, V2<T> lp, V2<T> p0, V2<T> p1, XF<T> xfa, T ra,
+                                XF<T> xfb, T rb, V2<T>& w0, V2<T>& w1) {
+  V2<T> normal = v2<T>(0, 0);
+  if (pc == 0) return normal;
+  if (type == M_CIRCLES) {
+    normal = v2<T>(1, 0);
+    V2<T> pa = xmul(xfa, lp), pb = xmul(xfb, p0);
+    V2<T> dd = pa - pb;
+    if (dot(dd, dd) > NormEps<T>::value * NormEps<T>::value) normal = normalized(pb - pa);
+    V2<T> ca = pa + scale(ra, normal);
+    V2<T> cbp = pb - scale(rb, normal);
+    w0 = scale((T)0.5, ca + cbp);
+  } else if (type == M_FACE_A) {
+    normal = rmul(xfa.q, ln);
+    V2<T> plane = xmul(xfa, lp);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      if (i >= pc) break;
+      V2<T> clip = xmul(xfb, i == 0 ? p0 : p1);
+      V2<T> ca = clip + scale(ra - dot(clip - plane, normal), normal);
+      V2<T> cbp = clip - scale(rb, normal);
+      V2<T> r = scale((T)0.5, ca + cbp);
+      if (i == 0) w0 = r; else w1 = r;
+    }
+  } else {
+    normal = rmul(xfb.q, ln);
+    V2<T> plane = xmul(xfb, lp);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      if (i >= pc) break;
+      V2<T> clip = xmul(xfa, i == 0 ? p0 : p1);
+      V2<T> cbp = clip + scale(rb - dot(clip - plane, normal), normal);
+      V2<T> ca = clip - scale(ra, normal);
+      V2<T> r = scale((T)0.5, ca + cbp);
+      if (i == 0) w0 = r; else w1 = r;
+    }
+    normal = -normal;
+  }
+  return normal;
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_block(VC<T>& vc, V2d x, V2d a, V2d normal, V2d& va, double& wa, V2d& vb,
+                                            double& wb) {
+  V2d dd = x - a;
+  V2d p1 = scale(dd.x, normal), p2 = scale(dd.y, normal);
+  V2d ra1 = dv2(vc.p[0].rax, vc.p[0].ray), ra2 = dv2(vc.p[1].rax, vc.p[1].ray);
+  V2d rb1 = dv2(vc.p[0].rbx, vc.p[0].rby), rb2 = dv2(vc.p[1].rbx, vc.p[1].rby);
+  va = va - scale((double)vc.ma, p1 + p2);
+  wa -= (double)vc.Ia * (cross(ra1, p1) + cross(ra2, p2));
+  vb = vb + scale((double)vc.mb, p1 + p2);
+  wb += (double)vc.Ib * (cross(rb1, p1) + cross(rb2, p2));
+  vc.p[0].ni = x.x;
+  vc.p[1].ni = x.y;
+}
+
+// One contact of solve_velocity_constraints (contact.py:527-799).
+template <typename T>
+__device__ __forceinline__ void solve_contact_velocity(VC<T>& vc, V2d& va, double& wa, V2d& vb, double& wb) {
+  const V2d normal = dv2(vc.nx, vc.ny), tangent = cross_vs(normal, 1.0);
+  const double ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
+  for (int j = 0; j < vc.npts; ++j) {  // tangent constraints first
+    VCP<T>& p = vc.p[j];
+    V2d ra = dv2(p.rax, p.ray), rb = dv2(p.rbx, p.rby);
+    V2d dv = vb + cross_sv(wb, rb) - va - cross_sv(wa, ra);
+    double vt = dot(dv, tangent);
+    double lambda = (double)p.tm * (-vt);
+    double maxf = (double)vc.friction * p.ni;
+    double newi = pclamp(p.ti + lambda, -maxf, maxf);
+    lambda = newi - p.ti;
+    V2d P = scale(lambda, tangent);
+    va = va - scale(ma, P);
+    wa -= Ia * cross(ra, P);
+    vb = vb + scale(mb, P);
+    wb += Ib * cross(rb, P);
+    p.ti = newi;
+  }
+  if (vc.npts == 1) {
+    VCP<T>& p = vc.p[0];
+    V2d ra = dv2(p.rax, p.ray), rb = dv2(p.rbx, p.rby);
+    V2d dv = vb + cross_sv(wb, rb) - va - cross_sv(wa, ra);
+    double vn = dot(dv, normal);
+    double lambda = -(double)p.nm * (vn - (double)p.bias);
+    double newi = pmax(p.ni + lambda, 0.0);
+    lambda = newi - p.ni;
+    V2d P = scale(lambda, normal);
+    va = va - scale(ma, P);
+    wa -= Ia * cross(ra, P);
+    vb = vb + scale(mb, P);
+    wb += Ib * cross(rb, P);
+    p.ni = newi;
+  } else if (vc.npts == 2) {  // block solver, total enumeration (contact.py:600-797)
+    VCP<T>&cp1 = vc.p[0], &cp2 = vc.p[1];
+    V2d a = v2<double>(cp1.ni, cp2.ni);
+    V2d ra1 = dv2(cp1.rax, cp1.ray), rb1 = dv2(cp1.rbx, cp1.rby);
+    V2d ra2 = dv2(cp2.rax, cp2.ray), rb2 = dv2(cp2.rbx, cp2.rby);
+    V2d dv1 = vb + cross_sv(wb, rb1) - va - cross_sv(wa, ra1);
+    V2d dv2_ = vb + cross_sv(wb, rb2) - va - cross_sv(wa, ra2);
+    double vn1 = dot(dv1, normal), vn2 = dot(dv2_, normal);
+    V2d b = v2<double>(vn1 - (double)cp1.bias, vn2 - (double)cp2.bias);
+    M22<double> K;
+    K.c1x = vc.k11; K.c1y = vc.k12; K.c2x = vc.k12; K.c2y = vc.k22;
+    M22<double> N;
+    N.c1x = vc.n11; N.c1y = vc.n21; N.c2x = vc.n12; N.c2y = vc.n22;
+    b = b - mmul(K, a);
+    V2d x = -mmul(N, b);  // case 1
+    if (x.x >= 0.0 && x.y >= 0.0) {
+      apply_block(vc, x, a, normal, va, wa, vb, wb);
+    } else {
+      x = v2<double>(-(double)cp1.nm * b.x, 0.0);  // case 2
+      vn2 = K.c1y * x.x + b.y;
+      if (x.x >= 0.0 && vn2 >= 0.0) {
+        apply_block(vc, x, a, normal, va, wa, vb, wb);
+      } else {
+        x = v2<double>(0.0, -(double)cp2.nm * b.y);  // case 3
+        vn1 = K.c2x * x.y + b.x;
+        if (x.y >= 0.0 && vn1 >= 0.0) {
+          apply_block(vc, x, a, normal, va, wa, vb, wb);
+        } else {
+          x = v2<double>(0.0, 0.0);  // case 4
+          if (b.x >= 0.0 && b.y >= 0.0) apply_block(vc, x, a, normal, va, wa, vb, wb);
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci, int last_allow) {
+  const double dt = d.dt;
+  const int b0 = d.w_body_off[w];
+  const int f0 = d.w_fix_off[w];
+  const int cbase = d.w_con_base[w];
+  const int wflags = d.w_flags[w];
+  const V2d grav = v2<double>(d.w_grav[2 * w], d.w_grav[2 * w + 1]);
+  const int pvb = bstart - b0;  // island body i is world-segment entry pvb + i
+  // 1. integrate velocities (island.py:71-95)
+  for (int i = 0; i < nbi; ++i) {
+    const int gb = b0 + d.i_bodies[bstart + i];
+    double2 c = d.b_c[gb];
+    double a = d.b_a[gb];
+    d.b_c0[gb] = c;
+    d.b_a0[gb] = a;
+    V2d v = d.b_v[gb];
+    double wv = d.b_w[gb];
+    if (d.b_type[gb] == BT_DYNAMIC) {
+      V2<T> f = d.b_f[gb];
+      V2d acc = scale((double)d.b_gs[gb], grav) + scale((double)d.b_im[gb], dv2(f.x, f.y));
+      v = v + scale(dt, acc);
+      wv += dt * (double)d.b_iI[gb] * (double)d.b_t[gb];
+      v = scale(pclamp(1.0 - dt * (double)d.b_ld[gb], 0.0, 1.0), v);
+      wv *= pclamp(1.0 - dt * (double)d.b_ad[gb], 0.0, 1.0);
+    }
+    PV<T> p;
+    p.cx = c.x;
+    p.cy = c.y;
+    p.a = a;
+    p.vx = v.x;
+    p.vy = v.y;
+    p.w = wv;
+    d.pv[bstart + i] = p;
+  }
+  const double dt_ratio = d.w_inv_dt0[w] * dt;
+  const bool warm = (wflags & WF_WARM) != 0;
+  // 2. ContactSolver.__init__ + initialize_velocity_constraints (contact.py:376-496)
+  for (int k = 0; k < nci; ++k) {
+    const int gs_ = cbase + d.i_contacts[cstart + k];
+    int2 cix = d.i_cidx[cstart + k];
+    const int ia = cix.x >= 0 ? cix.x - pvb : cix.x, ib = cix.y >= 0 ? cix.y - pvb : cix.y;
+    int2 pr = d.c_pair[gs_];
+    const int fa = f0 + pr.x, fb = f0 + pr.y;
+    const int ba = b0 + d.f_info[fa].x, bb = b0 + d.f_info[fb].x;
+    const int st = d.c_state[gs_];
+    const int mtype = (st >> CS_MTYPE_SHIFT) & 3, pc = (st >> CS_PC_SHIFT) & 3;
+    V4<T> lnlp = d.c_lnlp[gs_], pts = d.c_pts[gs_], imp = d.c_imp[gs_];
+    V2<T> mat = d.c_mat[gs_];
+    VC<T> vc;
+    PC<T> pcn;
+    vc.friction = mat.x;
+    vc.restitution = mat.y;
+    vc.ia = ia;
+    vc.ib = ib;
+    vc.ma = d.b_im[ba];
+    vc.mb = d.b_im[bb];
+    vc.Ia = d.b_iI[ba];
+    vc.Ib = d.b_iI[bb];
+    vc.npts = pc;
+    vc.pad = 0;
+    vc.p[0].ni = warm ? dt_ratio * (double)imp.x : (double)imp.x;
+    vc.p[0].ti = warm ? dt_ratio * (double)imp.y : (double)imp.y;
+    vc.p[1].ni = warm ? dt_ratio * (double)imp.z : (double)imp.z;
+    vc.p[1].ti = warm ? dt_ratio * (double)imp.w : (double)imp.w;
+    double2 lca = d.b_lc[ba], lcb = d.b_lc[bb];
+    pcn.lp0x = pts.x; pcn.lp0y = pts.y; pcn.lp1x = pts.z; pcn.lp1y = pts.w;
+    pcn.lnx = lnlp.x; pcn.lny = lnlp.y; pcn.lpx = lnlp.z; pcn.lpy = lnlp.w;
+    pcn.ra = (T)d.f_rad64[fa];
+    pcn.rb = (T)d.f_rad64[fb];
+    pcn.ma = vc.ma; pcn.mb = vc.mb; pcn.Ia = vc.Ia; pcn.Ib = vc.Ib;
+    pcn.lcax = (T)lca.x; pcn.lcay = (T)lca.y; pcn.lcbx = (T)lcb.x; pcn.lcby = (T)lcb.y;
+    pcn.type = mtype;
+    pcn.npts = pc;
+    pcn.ia = ia;
+    pcn.ib = ib;
+    double cax, cay, aa, cbx, cby, ab;
+    load_pos(d, bstart, b0, ia, cax, cay, aa);
+    load_pos(d, bstart, b0, ib, cbx, cby, ab);
+    V2d va, vb;
+    double wa, wb;
+    load_vel(d, bstart, b0, ia, va, wa);
+    load_vel(d, bstart, b0, ib, vb, wb);
+    Rot<T> qa = rot_of<T>(aa), qb = rot_of<T>(ab);
+    V2<T> dcb = v2<T>((T)(cbx - cax), (T)(cby - cay));
+    XF<T> xfa, xfb;
+    xfa.q = qa;
+    xfa.p = -rmul(qa, v2<T>(pcn.lcax, pcn.lcay));
+    xfb.q = qb;
+    xfb.p = dcb - rmul(qb, v2<T>(pcn.lcbx, pcn.lcby));
+    V2<T> w0 = v2<T>(0, 0), w1 = v2<T>(0, 0);
+    V2<T> normal = world_manifold<T>(mtype, pc, v2<T>(lnlp.x, lnlp.y), v2<T>(lnlp.z, lnlp.w), v2<T>(pts.x, pts.y),
+                                     v2<T>(pts.z, pts.w), xfa, pcn.ra, xfb, pcn.rb, w0, w1);
+    vc.nx = normal.x;
+    vc.ny = normal.y;
+    const T ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
+    const V2d nd = dv2(normal.x, normal.y);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (j >= pc) break;
+      V2<T> wp = j == 0 ? w0 : w1;
+      V2<T> ra = wp, rb = wp - dcb;
+      T rna = sq(cross(ra, normal)), rnb = sq(cross(rb, normal));
+      T kn = ma + mb + Ia * rna + Ib * rnb;
+      V2<T> tangent = cross_vs(normal, (T)1);
+      T rta = sq(cross(ra, tangent)), rtb = sq(cross(rb, tangent));
+      T kt = ma + mb + Ia * rta + Ib * rtb;
+      double vrel = dot(nd, vb + cross_sv(wb, dv2(rb.x, rb.y)) - va - cross_sv(wa, dv2(ra.x, ra.y)));
+      VCP<T>& p = vc.p[j];
+      p.rax = ra.x; p.ray = ra.y; p.rbx = rb.x; p.rby = rb.y;
+      p.nm = kn > (T)0 ? (T)1 / kn : (T)0;
+      p.tm = kt > (T)0 ? (T)1 / kt : (T)0;
+      p.bias = vrel < -VELOCITY_THRESHOLD ? (T)(-(double)vc.restitution * vrel) : (T)0;
+      p.pad = 0;
+    }
+    vc.k11 = 1; vc.k12 = 0; vc.k22 = 1; vc.n11 = 1; vc.n12 = 0; vc.n21 = 0; vc.n22 = 1;
+    if (pc == 2) {
+      V2d ra1 = dv2(vc.p[0].rax, vc.p[0].ray), rb1 = dv2(vc.p[0].rbx, vc.p[0].rby);
+      V2d ra2 = dv2(vc.p[1].rax, vc.p[1].ray), rb2 = dv2(vc.p[1].rbx, vc.p[1].rby);
+      const double mad = ma, mbd = mb, Iad = Ia, Ibd = Ib;
+      double rn1a = cross(ra1, nd), rn1b = cross(rb1, nd);
+      double rn2a = cross(ra2, nd), rn2b = cross(rb2, nd);
+      double k11 = mad + mbd + Iad * rn1a * rn1a + Ibd * rn1b * rn1b;
+      double k22 = mad + mbd + Iad * rn2a * rn2a + Ibd * rn2b * rn2b;
+      double k12 = mad + mbd + Iad * rn1a * rn2a + Ibd * rn1b * rn2b;
+      if (sq(k11) < 1000.0 * (k11 * k22 - sq(k12))) {
+        vc.k11 = k11; vc.k12 = k12; vc.k22 = k22;
+        double det = k11 * k22 - k12 * k12;
+        if (det != 0.0) det = 1.0 / det;
+        vc.n11 = det * k22; vc.n21 = -det * k12; vc.n12 = -det * k12; vc.n22 = det * k11;
+      } else {
+        vc.npts = 1;  // quirk Q6: keep only the first point
+      }
+    }
+    d.vc[cstart + k] = vc;
+    d.pc[cstart + k] = pcn;
+  }
+  // 3. warm_start (contact.py:498-525)
+  if (warm) {
+    for (int k = 0; k < nci; ++k) {
+      VC<T> vc = d.vc[cstart + k];
+      V2d normal = dv2(vc.nx, vc.ny), tangent = cross_vs(normal, 1.0);
+      V2d va, vb;
+      double wa, wb;
+      load_vel(d, bstart, b0, vc.ia, va, wa);
+      load_vel(d, bstart, b0, vc.ib, vb, wb);
+      for (int j = 0; j < vc.npts; ++j) {
+        const VCP<T>& p = vc.p[j];
+        V2d P = scale(p.ni, normal) + scale(p.ti, tangent);
+        wa -= (double)vc.Ia * cross(dv2(p.rax, p.ray), P);
+        va = va - scale((double)vc.ma, P);
+        wb += (double)vc.Ib * cross(dv2(p.rbx, p.rby), P);
+        vb = vb + scale((double)vc.mb, P);
+      }
+      store_vel(d, bstart, vc.ia, va, wa);
+      store_vel(d, bstart, vc.ib, vb, wb);
+    }
+  }
+  // 4. velocity iterations
+  for (int it = 0; it < d.vel_iters; ++it) {
+    for (int k = 0; k < nci; ++k) {
+      VC<T> vc = d.vc[cstart + k];
+      V2d va, vb;
+      double wa, wb;
+      load_vel(d, bstart, b0, vc.ia, va, wa);
+      load_vel(d, bstart, b0, vc.ib, vb, wb);
+      solve_contact_velocity(vc, va, wa, vb, wb);
+      store_vel(d, bstart, vc.ia, va, wa);
+      store_vel(d, bstart, vc.ib, vb, wb);
+      VCP<T>* dst = d.vc[cstart + k].p;
+      dst[0].ni = vc.p[0].ni;
+      dst[0].ti = vc.p[0].ti;
+      dst[1].ni = vc.p[1].ni;
+      dst[1].ti = vc.p[1].ti;
+    }
+  }
+  // 5. store_impulses (contact.py:801-806): zip(manifold points, constraint points)
+  for (int k = 0; k < nci; ++k) {
+    const VC<T>& vc = d.vc[cstart + k];
+    const int gs_ = cbase + d.i_contacts[cstart + k];
+    const int pc = (d.c_state[gs_] >> CS_PC_SHIFT) & 3;
+    const int n = pc < vc.npts ? pc : vc.npts;
+    V4<T> imp = d.c_imp[gs_];
+    if (n >= 1) { imp.x = (T)vc.p[0].ni; imp.y = (T)vc.p[0].ti; }
+    if (n >= 2) { imp.z = (T)vc.p[1].ni; imp.w = (T)vc.p[1].ti; }
+    d.c_imp[gs_] = imp;
+  }
+  // 6. integrate positions (island.py:120-138)
+  for (int i = 0; i < nbi; ++i) {
+    PV<T> p = d.pv[bstart + i];
+    V2d v = v2<double>(p.vx, p.vy);
+    double wv = p.w;
+    V2d tr = scale(dt, v);
+    if (dot(tr, tr) > MAX_TRANSLATION * MAX_TRANSLATION) {
+      double ratio = MAX_TRANSLATION / sqrt(sq(tr.x) + sq(tr.y));
+      v = scale(ratio, v);
+    }
+    double rotation = dt * wv;
+    if (sq(rotation) > MAX_ROTATION * MAX_ROTATION) {
+      double ratio = MAX_ROTATION / fabs(rotation);
+      wv *= ratio;
+    }
+    V2d dc = scale(dt, v);
+    p.cx = p.cx + dc.x;
+    p.cy = p.cy + dc.y;
+    p.a += dt * wv;
+    p.vx = v.x;
+    p.vy = v.y;
+    p.w = wv;
+    d.pv[bstart + i] = p;
+  }
+  // 7. position iterations with early exit (contact.py:809-864)
+  bool position_solved = false;
+  for (int it = 0; it < d.pos_iters; ++it) {
+    T min_sep = 0;
+    for (int k = 0; k < nci; ++k) {
+      const PC<T> pc = d.pc[cstart + k];
+      double cax, cay, aa, cbx, cby, ab;
+      load_pos(d, bstart, b0, pc.ia, cax, cay, aa);
+      load_pos(d, bstart, b0, pc.ib, cbx, cby, ab);
+      for (int j = 0; j < pc.npts; ++j) {
+        Rot<T> qa = rot_of<T>(aa), qb = rot_of<T>(ab);
+        V2<T> dcb = v2<T>((T)(cbx - cax), (T)(cby - cay));
+        XF<T> xfa, xfb;
+        xfa.q = qa;
+        xfa.p = -rmul(qa, v2<T>(pc.lcax, pc.lcay));
+        xfb.q = qb;
+        xfb.p = dcb - rmul(qb, v2<T>(pc.lcbx, pc.lcby));
+        V2<T> normal, point;
+        T sep;
+        V2<T> lpj = j == 0 ? v2<T>(pc.lp0x, pc.lp0y) : v2<T>(pc.lp1x, pc.lp1y);
+        V2<T> ln = v2<T>(pc.lnx, pc.lny), lpp = v2<T>(pc.lpx, pc.lpy);
+        if (pc.type == M_CIRCLES) {
+          V2<T> pa = xmul(xfa, lpp), pb = xmul(xfb, v2<T>(pc.lp0x, pc.lp0y));
+          normal = normalized(pb - pa);
+          point = scale((T)0.5, pa + pb);
+          sep = dot(pb - pa, normal) - pc.ra - pc.rb;
+        } else if (pc.type == M_FACE_A) {
+          normal = rmul(qa, ln);
+          V2<T> plane = xmul(xfa, lpp);
+          V2<T> clip = xmul(xfb, lpj);
+          sep = dot(clip - plane, normal) - pc.ra - pc.rb;
+          point = clip;
+        } else {
+          normal = rmul(qb, ln);
+          V2<T> plane = xmul(xfb, lpp);
+          V2<T> clip = xmul(xfa, lpj);
+          sep = dot(clip - plane, normal) - pc.ra - pc.rb;
+          point = clip;
+          normal = -normal;
+        }
+        V2<T> ra = point, rb = point - dcb;
+        min_sep = pmin(min_sep, sep);
+        T C = pclamp((T)BAUMGARTE * (sep + (T)LINEAR_SLOP), -(T)MAX_LINEAR_CORRECTION, (T)0);
+        T rna = sq(cross(ra, normal)), rnb = sq(cross(rb, normal));
+        T K = pc.ma + pc.mb + (pc.Ia * rna) + (pc.Ib * rnb);
+        T impulse = K > (T)0 ? -C / K : (T)0;
+        V2<T> P = scale(impulse, normal);
+        cax -= (double)(pc.ma * P.x);
+        cay -= (double)(pc.ma * P.y);
+        aa -= (double)(pc.Ia * cross(ra, P));
+        cbx += (double)(pc.mb * P.x);
+        cby += (double)(pc.mb * P.y);
+        ab += (double)(pc.Ib * cross(rb, P));
+      }
+      store_pos(d, bstart, pc.ia, cax, cay, aa);
+      store_pos(d, bstart, pc.ib, cbx, cby, ab);
+    }
+    if (min_sep >= -(T)(3.0 * LINEAR_SLOP)) {
+      position_solved = true;
+      break;
+    }
+  }
+  // 8. copy back and Body._synchronize_transform (island.py:150-159, body.py:906-908)
+  for (int i = 0; i < nbi; ++i) {
+    const int gb = b0 + d.i_bodies[bstart + i];
+    PV<T> p = d.pv[bstart + i];
+    d.b_c[gb] = make_double2(p.cx, p.cy);
+    d.b_a[gb] = p.a;
+    d.b_v[gb] = v2<double>(p.vx, p.vy);
+    d.b_w[gb] = p.w;
+    double sn, cs;
+    sincos(p.a, &sn, &cs);
+    double2 lc = d.b_lc[gb];
+    d.b_xf[gb] = make_double4(p.cx - (cs * lc.x + (-sn) * lc.y), p.cy - (sn * lc.x + cs * lc.y), cs, sn);
+  }
+  // 9. sleep (island.py:161-179, quirk Q7: the stale loop variables)
+  if (wflags & WF_ALLOW_SLEEP) {
+    double min_sleep = 1.0e300;
+    for (int i = 0; i < nbi; ++i) {
+      const int gb = b0 + d.i_bodies[bstart + i];
+      V2d v = d.b_v[gb];
+      double wv = d.b_w[gb];
+      if (!last_allow || sq(wv) > ANGULAR_SLEEP_TOL * ANGULAR_SLEEP_TOL ||
+          sq(v.x) + sq(v.y) > LINEAR_SLEEP_TOL * LINEAR_SLEEP_TOL) {
+        d.b_sleep[gb] = 0.0;
+        min_sleep = 0.0;
+      } else {
+        double t = d.b_sleep[gb] + dt;
+        d.b_sleep[gb] = t;
+        min_sleep = pmin(min_sleep, t);
+      }
+    }
+    if (min_sleep >= TIME_TO_SLEEP && position_solved && nbi > 0) {
+      const int gb = b0 + d.i_bodies[bstart + nbi - 1];
+      d.b_sleep[gb] = 0.0;
+      d.b_v[gb] = v2<double>(0, 0);
+      d.b_w[gb] = 0.0;
+      d.b_f[gb] = v2<T>(0, 0);
+      d.b_t[gb] = (T)0;
+      d.b_flags[gb] &= ~BF_AWAKE;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_solve(Dev<T> d) {
+  const int n = *d.q_count;
+  for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
+    int4 in = d.q_isl[q];
+    int nc = d.q_nc[q];
+    solve_island(d, in.x, in.y, in.z, in.w, nc & 0x3FFFFFFF, (nc >> 30) & 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fixture synchronisation + move_proxy (FP64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void compute_aabb64(int type, int n, const double* g, double r, double px, double py,
+                                               double c, double s, double& lx, double& ly, double& ux, double& uy) {
+  if (type == SHAPE_CIRCLE) {  // shapes.py:178-189: p = xf.p + R * position
+    double qx = c * g[0] + (-s) * g[1], qy = s * g[0] + c * g[1];
+    double x = px + qx, y = py + qy;
+    lx = x - r; ly = y - r; ux = x + r; uy = y + r;
+  } else if (type == SHAPE_POLYGON) {  // shapes.py:316-334
+    double x0 = px + c * g[0] + (-s) * g[1], y0 = py + s * g[0] + c * g[1];
+    double lox = x0, loy = y0, hix = x0, hiy = y0;
+    for (int i = 0; i < n; ++i) {
+      double x = px + c * g[2 * i] + (-s) * g[2 * i + 1];
+      double y = py + s * g[2 * i] + c * g[2 * i + 1];
+      lox = pmin(lox, x); loy = pmin(loy, y);
+      hix = pmax(hix, x); hiy = pmax(hiy, y);
+    }
+    lx = lox - r; ly = loy - r; ux = hix + r; uy = hiy + r;
+  } else {  // edge, shapes.py:702-718
+    double ax = px + c * g[0] + (-s) * g[1], ay = py + s * g[0] + c * g[1];
+    double bx = px + c * g[2] + (-s) * g[3], by = py + s * g[2] + c * g[3];
+    lx = pmin(ax, bx) - r; ly = pmin(ay, by) - r; ux = pmax(ax, bx) + r; uy = pmax(ay, by) + r;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sync(Dev<T> d) {
+  const int f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= d.nf) return;
+  int2 wb = d.f_wb[f];
+  const int gb = wb.y;
+  d.f_moved[f] = 0;
+  if (d.b_island[gb] < 0 || d.b_type[gb] == BT_STATIC) return;
+  int4 info = d.f_info[f];
+  const int type = info.y & 255, n = (info.y >> 8) & 255;
+  const double* g = d.f_geo64 + (size_t)f * 16;
+  const double r = d.f_rad64[f];
+  double2 c0 = d.b_c0[gb], lc = d.b_lc[gb];
+  double s0, k0;
+  sincos(d.b_a0[gb], &s0, &k0);
+  double x1 = c0.x - (k0 * lc.x + (-s0) * lc.y), y1 = c0.y - (s0 * lc.x + k0 * lc.y);
+  double4 xf = d.b_xf[gb];
+  double alx, aly, aux, auy, blx, bly, bux, buy;
+  compute_aabb64(type, n, g, r, x1, y1, k0, s0, alx, aly, aux, auy);
+  compute_aabb64(type, n, g, r, xf.x, xf.y, xf.z, xf.w, blx, bly, bux, buy);
+  double lx = pmin(alx, blx), ly = pmin(aly, bly), ux = pmax(aux, bux), uy = pmax(auy, buy);
+  double dx = xf.x - x1, dy = xf.y - y1;
+  double4 fat = d.f_fat[f];
+  if (fat.x <= lx && fat.y <= ly && ux <= fat.z && uy <= fat.w) return;  // contains
+  double nlx = lx - AABB_EXTENSION, nly = ly - AABB_EXTENSION, nux = ux + AABB_EXTENSION, nuy = uy + AABB_EXTENSION;
+  double mdx = AABB_MULTIPLIER * dx, mdy = AABB_MULTIPLIER * dy;
+  if (mdx < 0.0) { nlx = nlx + mdx; nly = nly + 0.0; }
+  else { nux = nux + mdx; nuy = nuy + 0.0; }
+  if (mdy < 0.0) { nlx = nlx + 0.0; nly = nly + mdy; }
+  else { nux = nux + 0.0; nuy = nuy + mdy; }
+  d.f_fat[f] = make_double4(nlx, nly, nux, nuy);
+  d.f_moved[f] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// broad-phase pairs + contact creation, one workgroup per world
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int next_pow2(int x) {
+  int p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+__host__ __device__ inline size_t pairs_smem_bytes(int nf, int ccap) {
+  size_t s = 0;
+  s += ((size_t)nf * 4 + 15) & ~(size_t)15;                 // mpos
+  s += ((size_t)nf * 2 + 15) & ~(size_t)15;                 // mlist
+  s += (size_t)next_pow2(ccap + 1) * 4;                     // hash
+  s += (size_t)next_pow2(8 * nf + 64) * 4;                  // new pairs
+  s += (size_t)kBlock * 4 + 64;
+  return s;
+}
+
+__device__ __forceinline__ uint32_t hkey(int a, int b) {
+  return a < b ? ((uint32_t)a << 16) | (uint32_t)b : ((uint32_t)b << 16) | (uint32_t)a;
+}
+__device__ __forceinline__ uint32_t hhash(uint32_t k) {
+  k ^= k >> 16;
+  k *= 0x7feb352du;
+  k ^= k >> 15;
+  k *= 0x846ca68bu;
+  k ^= k >> 16;
+  return k;
+}
+
+template <typename T>
+__device__ void copy_contact(const Dev<T>& d, int dst, int src) {
+  d.c_pair[dst] = d.c_pair[src];
+  d.c_state[dst] = d.c_state[src];
+  d.c_mat[dst] = d.c_mat[src];
+  d.c_lnlp[dst] = d.c_lnlp[src];
+  d.c_pts[dst] = d.c_pts[src];
+  d.c_imp[dst] = d.c_imp[src];
+  d.c_id[dst] = d.c_id[src];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int from_upload, int finish, int do_pairs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
+  const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
+  const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
+  unsigned char* p = smem;
+  int* mpos = (int*)p; p += ((size_t)nf * 4 + 15) & ~(size_t)15;
+  uint16_t* mlist = (uint16_t*)p; p += ((size_t)nf * 2 + 15) & ~(size_t)15;
+  const int hcap = next_pow2(ccap + 1);
+  uint32_t* hash = (uint32_t*)p; p += (size_t)hcap * 4;
+  const int ncap = next_pow2(8 * nf + 64);
+  uint32_t* nlist = (uint32_t*)p; p += (size_t)ncap * 4;
+  int* scan = (int*)p; p += (size_t)kBlock * 4;
+  int* misc = (int*)p;
+
+  // ---- the move buffer (broadphase.py:61-68) ----
+  int nmove = 0;
+  if (!do_pairs) {
+    // dt == 0: World._solve (and with it find_new_contacts) does not run
+  } else if (from_upload) {
+    nmove = d.w_move_n[w];
+    for (int i = tid; i < nf; i += kBlock) mpos[i] = -1;
+    __syncthreads();
+    if (tid == 0)
+      for (int k = 0; k < nmove; ++k) {
+        int f = d.mv[f0 + k];
+        mlist[k] = (uint16_t)f;
+        if (mpos[f] < 0) mpos[f] = k;
+      }
+    __syncthreads();
+  } else {
+    // moved fixtures in synchronisation order (bodies in order, fixtures body-major)
+    for (int i = tid; i < nf; i += kBlock) mpos[i] = d.f_moved[f0 + i] ? 1 : 0;
+    __syncthreads();
+    nmove = block_exclusive_scan(mpos, nf, scan);
+    for (int i = tid; i < nf; i += kBlock) {
+      if (d.f_moved[f0 + i]) mlist[mpos[i]] = (uint16_t)i;
+      else mpos[i] = -1;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) { misc[0] = 0; misc[1] = 0; }
+  __syncthreads();
+  if (nmove > 0) {
+    // existing contacts (either orientation), for the "does a contact exist" test
+    for (int i = tid; i < hcap; i += kBlock) hash[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    const int used = d.w_con_used[w];
+    for (int sl = tid; sl < used; sl += kBlock) {
+      if (!(d.c_state[cb + sl] & CS_LIVE)) continue;
+      int2 pr = d.c_pair[cb + sl];
+      uint32_t k = hkey(pr.x, pr.y);
+      uint32_t h = hhash(k) & (uint32_t)(hcap - 1);
+      for (;;) {
+        uint32_t prev = atomicCAS(&hash[h], 0xFFFFFFFFu, k);
+        if (prev == 0xFFFFFFFFu || prev == k) break;
+        h = (h + 1) & (uint32_t)(hcap - 1);
+      }
+    }
+    __syncthreads();
+    // every moved proxy against every proxy of the world; exact FP64 tests
+    const long long work = (long long)nmove * nf;
+    for (long long t = tid; t < work; t += kBlock) {
+      const int i = (int)(t / nf), R = (int)(t % nf);
+      const int P = mlist[i];
+      if (R == P) continue;
+      const int mr = mpos[R];
+      if (mr >= 0 && mr < i) continue;  // (R, P) is already in the pair set
+      double4 fp = d.f_fat[f0 + P], fr = d.f_fat[f0 + R];
+      if (!aabb_overlap(fr.x, fr.y, fr.z, fr.w, fp.x, fp.y, fp.z, fp.w)) continue;
+      int4 ip = d.f_info[f0 + P], ir = d.f_info[f0 + R];
+      if (ip.x == ir.x) continue;  // same body
+      const int tb_p = d.b_type[b0 + ip.x], tb_r = d.b_type[b0 + ir.x];
+      if (!(d.b_flags[b0 + ip.x] & BF_ACTIVE) || !(d.b_flags[b0 + ir.x] & BF_ACTIVE)) continue;
+      {  // exists?
+        uint32_t k = hkey(P, R);
+        uint32_t h = hhash(k) & (uint32_t)(hcap - 1);
+        bool found = false;
+        for (;;) {
+          uint32_t v = hash[h];
+          if (v == 0xFFFFFFFFu) break;
+          if (v == k) { found = true; break; }
+          h = (h + 1) & (uint32_t)(hcap - 1);
+        }
+        if (found) continue;
+      }
+      if (tb_r != BT_DYNAMIC && tb_p != BT_DYNAMIC) continue;  // Body._should_collide
+      bool ok;  // World.default_contact_filter(fixture P, fixture R)
+      if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
+      else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
+      if (!ok) continue;
+      const int ta = ip.y & 255, tb = ir.y & 255;
+      const bool reg = (ta == SHAPE_CIRCLE || ta == SHAPE_POLYGON || ta == SHAPE_EDGE) &&
+                       (tb == SHAPE_CIRCLE || tb == SHAPE_POLYGON || tb == SHAPE_EDGE) &&
+                       !(ta == SHAPE_EDGE && tb == SHAPE_EDGE);
+      if (!reg) continue;
+      int slot = atomicAdd(&misc[0], 1);
+      if (slot < ncap) nlist[slot] = ((uint32_t)i << 16) | (uint32_t)R;
+      else atomicOr(&misc[1], 1);
+    }
+    __syncthreads();
+  }
+  int nnew = misc[0];
+  if (misc[1]) {
+    if (tid == 0) atomicOr(&d.w_err[w], WERR_NEWPAIRS);
+    nnew = 0;
+  }
+  if (nnew > 0) {
+    // canonical order: move-buffer position, then partner index (bitonic sort)
+    const int np = next_pow2(nnew);
+    for (int i = nnew + tid; i < np; i += kBlock) nlist[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    for (int k = 2; k <= np; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = tid; i < np; i += kBlock) {
+          int ixj = i ^ j;
+          if (ixj > i) {
+            uint32_t a = nlist[i], b = nlist[ixj];
+            bool up = (i & k) == 0;
+            if ((a > b) == up) { nlist[i] = b; nlist[ixj] = a; }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // make room: squeeze out destroyed slots when the segment would overflow
+    int used = d.w_con_used[w];
+    if (used + nnew > ccap) {
+      int base = 0;
+      for (int c0 = 0; c0 < used; c0 += kBlock) {
+        const int sl = c0 + tid;
+        const bool live = sl < used && (d.c_state[cb + sl] & CS_LIVE);
+        int2 pr = make_int2(0, 0);
+        int st = 0;
+        V2<T> mat = v2<T>(0, 0);
+        V4<T> lnlp{}, pts{}, imp{};
+        uint2 id = make_uint2(0, 0);
+        if (live) {
+          pr = d.c_pair[cb + sl]; st = d.c_state[cb + sl]; mat = d.c_mat[cb + sl];
+          lnlp = d.c_lnlp[cb + sl]; pts = d.c_pts[cb + sl]; imp = d.c_imp[cb + sl]; id = d.c_id[cb + sl];
+        }
+        scan[tid] = live ? 1 : 0;
+        __syncthreads();
+        for (int off = 1; off < kBlock; off <<= 1) {
+          int v = tid >= off ? scan[tid - off] : 0;
+          __syncthreads();
+          scan[tid] += v;
+          __syncthreads();
+        }
+        const int dst = base + scan[tid] - (live ? 1 : 0);
+        const int cnt = scan[kBlock - 1];
+        __syncthreads();
+        if (live) {
+          const int s = cb + dst;
+          d.c_pair[s] = pr; d.c_state[s] = st; d.c_mat[s] = mat; d.c_lnlp[s] = lnlp; d.c_pts[s] = pts;
+          d.c_imp[s] = imp; d.c_id[s] = id;
+        }
+        base += cnt;
+        __syncthreads();
+      }
+      used = base;
+    }
+    if (used + nnew > ccap) {
+      if (tid == 0) atomicOr(&d.w_err[w], WERR_CAPACITY);
+      nnew = 0;
+    }
+    // Contact._create (contact.py:313-326) + Contact.__init__ (contact.py:55-90)
+    for (int k = tid; k < nnew; k += kBlock) {
+      uint32_t key = nlist[k];
+      const int P = mlist[key >> 16], R = (int)(key & 0xFFFFu);
+      const int ta = d.f_info[f0 + P].y & 255, tb = d.f_info[f0 + R].y & 255;
+      const bool swap = (ta == SHAPE_CIRCLE && tb == SHAPE_POLYGON) || (ta == SHAPE_CIRCLE && tb == SHAPE_EDGE) ||
+                        (ta == SHAPE_POLYGON && tb == SHAPE_EDGE);
+      const int fa = swap ? R : P, fb = swap ? P : R;
+      const int s = cb + used + k;
+      d.c_pair[s] = make_int2(fa, fb);
+      d.c_state[s] = CS_LIVE | (M_CIRCLES << CS_MTYPE_SHIFT);
+      V2<T> ma = d.f_mat[f0 + fa], mb = d.f_mat[f0 + fb];
+      d.c_mat[s] = v2<T>(sqrt(ma.x * mb.x), pmax(ma.y, mb.y));
+      d.c_lnlp[s] = V4<T>{0, 0, 0, 0};
+      d.c_pts[s] = V4<T>{0, 0, 0, 0};
+      d.c_imp[s] = V4<T>{0, 0, 0, 0};
+      d.c_id[s] = make_uint2(0, 0);
+    }
+    __syncthreads();
+    if (tid == 0) d.w_con_used[w] = used + nnew;
+  }
+  if (tid == 0) d.w_move_n[w] = 0;
+  if (!finish) return;
+  // ---- end-of-step bookkeeping (world.py:348-352) ----
+  if (tid == 0) {
+    if (d.dt > 0.0) d.w_inv_dt0[w] = d.inv_dt;
+    misc[2] = 0;
+  }
+  __syncthreads();
+  const bool clear = !(d.w_flags[w] & WF_KEEP_FORCES);
+  int asleep = 0;
+  for (int i = tid; i < nb; i += kBlock) {
+    const int gb = b0 + i;
+    if (clear) {
+      d.b_f[gb] = v2<T>(0, 0);
+      d.b_t[gb] = (T)0;
+    }
+    if (d.b_type[gb] != BT_STATIC && !(d.b_flags[gb] & BF_AWAKE)) ++asleep;
+  }
+  if (asleep) atomicAdd(&misc[2], asleep);
+  __syncthreads();
+  if (tid == 0) {
+    d.w_asleep[w] = misc[2];
+    if (d.w_flags[w] & WF_NEW_FIXTURE) d.w_flags[w] &= ~WF_NEW_FIXTURE;
+  }
+}
+
+template <typename T>
+__global__ void k_reset_queue(Dev<T> d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *d.q_count = 0;
+}
+
+// ---------------------------------------------------------------------------
+// parity seams
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_narrow_seam(int n, const T* geo_a, const int* info_a, const T* geo_b, const int* info_b,
+                              const double* xfa, const double* xfb, int* out_i, T* out_t) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  Geo<T> A, B;
+  A.g = geo_a + (size_t)k * GEO_STRIDE;
+  A.type = info_a[k] & 255;
+  A.n = (info_a[k] >> 8) & 255;
+  A.has0 = (info_a[k] >> 16) & 1;
+  A.has3 = (info_a[k] >> 17) & 1;
+  B.g = geo_b + (size_t)k * GEO_STRIDE;
+  B.type = info_b[k] & 255;
+  B.n = (info_b[k] >> 8) & 255;
+  B.has0 = (info_b[k] >> 16) & 1;
+  B.has3 = (info_b[k] >> 17) & 1;
+  XF<T> ta, tb;
+  ta.p = v2<T>(0, 0);
+  ta.q.c = (T)xfa[4 * k + 2];
+  ta.q.s = (T)xfa[4 * k + 3];
+  tb.p = v2<T>((T)(xfb[4 * k] - xfa[4 * k]), (T)(xfb[4 * k + 1] - xfa[4 * k + 1]));
+  tb.q.c = (T)xfb[4 * k + 2];
+  tb.q.s = (T)xfb[4 * k + 3];
+  Man<T> m;
+  m.type = 0; m.pc = 0;
+  m.ln = m.lp = m.p0 = m.p1 = v2<T>(0, 0);
+  m.id0 = m.id1 = 0;
+  bool ok = evaluate(m, A, ta, B, tb);
+  out_i[4 * k] = ok ? m.type : -1;
+  out_i[4 * k + 1] = m.pc;
+  out_i[4 * k + 2] = (int)m.id0;
+  out_i[4 * k + 3] = (int)m.id1;
+  T* o = out_t + (size_t)k * 8;
+  o[0] = m.ln.x; o[1] = m.ln.y; o[2] = m.lp.x; o[3] = m.lp.y;
+  o[4] = m.p0.x; o[5] = m.p0.y; o[6] = m.p1.x; o[7] = m.p1.y;
+}
+
+// pair seam: exact FP64 fat-AABB tests of a move buffer against n proxies;
+// one lane per (move entry, proxy) with the pair-set dedup rule.
+__global__ void k_pairs_seam(int n, const double4* fat, int nmove, const int* move, const int* mpos, uint32_t* out,
+                             int cap, int* count) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nmove * n) return;
+  const int i = (int)(t / n), R = (int)(t % n);
+  const int P = move[i];
+  if (R == P) return;
+  const int mr = mpos[R];
+  if (mr >= 0 && mr < i) return;
+  double4 fp = fat[P], fr = fat[R];
+  if (!aabb_overlap(fr.x, fr.y, fr.z, fr.w, fp.x, fp.y, fp.z, fp.w)) return;
+  int slot = atomicAdd(count, 1);
+  if (slot < cap) {
+    out[2 * slot] = (uint32_t)i;
+    out[2 * slot + 1] = (uint32_t)R;
+  }
+}
+
+}  // namespace b2g
+
+// ===========================================================================
+// host side
+// ===========================================================================
+using namespace b2g;
+
+#define HIPCHK(expr)                                                                                  \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess) {                                                                           \
+      ctx_error(ctx, std::string(#expr) + ": " + hipGetErrorString(e_));                              \
+      return B2G_E_HIP;                                                                               \
+    }                                                                                                 \
+  } while (0)
+
+struct b2g_ctx {
+  int precision = B2G_FP32;
+  int device = 0;
+  double contact_factor = 8.0;
+  int contact_extra = 64;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // sizes
+  int nw = 0, nb = 0, nf = 0, ncap = 0;
+  int max_nb = 0, max_nf = 0, max_ccap = 0;
+  size_t smem_islands = 0, smem_pairs = 0;
+  bool pending_new_fixture = false;
+  // host mirrors of constant per-body / per-world data
+  std::vector<int> body_off, fix_off, con_base, con_cap;
+  std::vector<double> mass, inertia;
+  std::vector<double> world_d;
+  std::vector<int> world_i;
+  std::vector<int> fix_body_global;
+  // device buffers
+  std::vector<void*> allocs;
+  void* dev = nullptr;  // Dev<float> or Dev<double>, host copy of the pointer table
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = 0;
+  int graph_vi = -1, graph_pi = -1;
+  // timing
+  hipEvent_t ev[8] = {};
+  bool have_events = false;
+};
+
+static void ctx_error(b2g_ctx* ctx, const std::string& s) {
+  if (ctx) ctx->err = s;
+}
+
+static int set_error(b2g_ctx* ctx, int code, const std::string& s) {
+  ctx_error(ctx, s);
+  return code;
+}
+
+template <typename X>
+static int dalloc(b2g_ctx* ctx, X** p, size_t n) {
+  *p = nullptr;
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(X);
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    ctx_error(ctx, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return B2G_E_HIP;
+  }
+  ctx->allocs.push_back(q);
+  *p = (X*)q;
+  return 0;
+}
+
+static void free_device(b2g_ctx* ctx) {
+  if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph) hipGraphDestroy(ctx->graph);
+  ctx->graph_exec = nullptr;
+  ctx->graph = nullptr;
+  ctx->graph_vi = -1;
+  for (void* p : ctx->allocs) hipFree(p);
+  ctx->allocs.clear();
+  if (ctx->dev) {
+    if (ctx->precision == B2G_FP64) delete (Dev<double>*)ctx->dev;
+    else delete (Dev<float>*)ctx->dev;
+  }
+  ctx->dev = nullptr;
+}
+
+template <typename X>
+static int upload(b2g_ctx* ctx, X* dst, const X* src, size_t n) {
+  if (n == 0) return 0;
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(X), hipMemcpyHostToDevice, ctx->stream));
+  return 0;
+}
+template <typename X>
+static int download(b2g_ctx* ctx, X* dst, const X* src, size_t n) {
+  if (n == 0) return 0;
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(X), hipMemcpyDeviceToHost, ctx->stream));
+  return 0;
+}
+
+#define TRY(x)           \
+  do {                   \
+    int rc_ = (x);       \
+    if (rc_) return rc_; \
+  } while (0)
+
+template <typename T>
+static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
+  const int nw = b->n_worlds;
+  const int nb = b->body_off[nw], nf = b->fix_off[nw], nc = b->con_off[nw];
+  ctx->nw = nw;
+  ctx->nb = nb;
+  ctx->nf = nf;
+  ctx->body_off.assign(b->body_off, b->body_off + nw + 1);
+  ctx->fix_off.assign(b->fix_off, b->fix_off + nw + 1);
+  ctx->con_base.resize(nw + 1);
+  ctx->con_cap.resize(nw);
+  ctx->max_nb = ctx->max_nf = ctx->max_ccap = 0;
+  int ncap = 0;
+  for (int w = 0; w < nw; ++w) {
+    const int wnb = b->body_off[w + 1] - b->body_off[w];
+    const int wnf = b->fix_off[w + 1] - b->fix_off[w];
+    const int wnc = b->con_off[w + 1] - b->con_off[w];
+    if (wnb < 0 || wnf < 0 || wnc < 0) return set_error(ctx, B2G_E_INVALID, "negative world size");
+    int cap = (int)std::ceil(ctx->contact_factor * wnf) + ctx->contact_extra;
+    cap = std::max(cap, 2 * wnc + 16);
+    ctx->con_base[w] = ncap;
+    ctx->con_cap[w] = cap;
+    ncap += cap;
+    ctx->max_nb = std::max(ctx->max_nb, wnb);
+    ctx->max_nf = std::max(ctx->max_nf, wnf);
+    ctx->max_ccap = std::max(ctx->max_ccap, cap);
+    if (wnb >= 65535 || wnf >= 65535 || cap >= 65535)
+      return set_error(ctx, B2G_E_UNSUPPORTED, "world too large for the LDS-resident path (>= 65535 entries)");
+  }
+  ctx->con_base[nw] = ncap;
+  ctx->ncap = ncap;
+  ctx->smem_islands = island_smem_bytes(ctx->max_nb, ctx->max_ccap);
+  ctx->smem_pairs = pairs_smem_bytes(ctx->max_nf, ctx->max_ccap);
+  if (ctx->smem_islands > 163840 || ctx->smem_pairs > 163840)
+    return set_error(ctx, B2G_E_UNSUPPORTED, "world too large for the LDS-resident path (LDS > 160 KB)");
+
+  HIPCHK(hipFuncSetAttribute((const void*)k_islands<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)ctx->smem_islands));
+  HIPCHK(hipFuncSetAttribute((const void*)k_pairs<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)ctx->smem_pairs));
+  Dev<T>* dp = new Dev<T>();
+  ctx->dev = dp;
+  Dev<T>& d = *dp;
+  d.nw = nw;
+  d.nb = nb;
+  d.nf = nf;
+  d.ncap = ncap;
+  // ---- host staging ----
+  std::vector<int> w_flags(nw), w_move_n(nw), w_asleep(nw, 0), w_con_used(nw), zeros_w(nw, 0);
+  std::vector<double> w_grav(2 * (size_t)nw), w_inv_dt0(nw);
+  ctx->world_d.assign(b->world_d, b->world_d + (size_t)nw * B2G_WORLD_ND);
+  ctx->world_i.assign(b->world_i, b->world_i + (size_t)nw * B2G_WORLD_NI);
+  ctx->pending_new_fixture = false;
+  for (int w = 0; w < nw; ++w) {
+    w_flags[w] = b->world_i[(size_t)w * B2G_WORLD_NI + B2G_WI_FLAGS];
+    w_grav[2 * w] = b->world_d[(size_t)w * B2G_WORLD_ND + B2G_WD_GX];
+    w_grav[2 * w + 1] = b->world_d[(size_t)w * B2G_WORLD_ND + B2G_WD_GY];
+    w_inv_dt0[w] = b->world_d[(size_t)w * B2G_WORLD_ND + B2G_WD_INV_DT0];
+    w_move_n[w] = b->move_off[w + 1] - b->move_off[w];
+    w_con_used[w] = b->con_off[w + 1] - b->con_off[w];
+    if (w_flags[w] & WF_NEW_FIXTURE) ctx->pending_new_fixture = true;
+    if (w_move_n[w] > b->fix_off[w + 1] - b->fix_off[w]) return set_error(ctx, B2G_E_INVALID, "move buffer too long");
+  }
+  std::vector<int> b_type(nb), b_flags(nb), b_island(nb, -1);
+  std::vector<double2> b_c(nb), b_c0(nb), b_lc(nb);
+  std::vector<double> b_a(nb), b_a0(nb), b_sleep(nb);
+  std::vector<double4> b_xf(nb);
+  std::vector<V2<double>> b_v(nb);
+  std::vector<V2<T>> b_f(nb);
+  std::vector<double> b_w(nb);
+  std::vector<T> b_im(nb), b_iI(nb), b_t(nb), b_gs(nb), b_ld(nb), b_ad(nb);
+  ctx->mass.resize(nb);
+  ctx->inertia.resize(nb);
+  for (int w = 0; w < nw; ++w) {
+    for (int i = b->body_off[w]; i < b->body_off[w + 1]; ++i) {
+      const double* r = b->body_d + (size_t)i * B2G_BODY_ND;
+      const int32_t* ri = b->body_i + (size_t)i * B2G_BODY_NI;
+      b_type[i] = ri[B2G_BI_TYPE];
+      b_flags[i] = ri[B2G_BI_FLAGS];
+      if (b_type[i] < 0 || b_type[i] > 2) return set_error(ctx, B2G_E_INVALID, "bad body type");
+      if (b_type[i] == BT_STATIC && (r[B2G_BD_V_X] != 0.0 || r[B2G_BD_V_Y] != 0.0 || r[B2G_BD_W] != 0.0))
+        return set_error(ctx, B2G_E_UNSUPPORTED, "static body with non-zero velocity");
+      if (b_type[i] != BT_STATIC && !(b_flags[i] & BF_AWAKE) && (b_flags[i] & BF_ACTIVE)) w_asleep[w]++;
+      b_c[i] = make_double2(r[B2G_BD_C_X], r[B2G_BD_C_Y]);
+      b_c0[i] = make_double2(r[B2G_BD_C0_X], r[B2G_BD_C0_Y]);
+      b_lc[i] = make_double2(r[B2G_BD_LC_X], r[B2G_BD_LC_Y]);
+      b_a[i] = r[B2G_BD_A];
+      b_a0[i] = r[B2G_BD_A0];
+      b_sleep[i] = r[B2G_BD_SLEEP_TIME];
+      b_xf[i] = make_double4(r[B2G_BD_XF_PX], r[B2G_BD_XF_PY], r[B2G_BD_XF_C], r[B2G_BD_XF_S]);
+      b_v[i] = v2<double>(r[B2G_BD_V_X], r[B2G_BD_V_Y]);
+      b_w[i] = r[B2G_BD_W];
+      b_f[i] = v2<T>((T)r[B2G_BD_F_X], (T)r[B2G_BD_F_Y]);
+      b_t[i] = (T)r[B2G_BD_TORQUE];
+      b_im[i] = (T)r[B2G_BD_INV_MASS];
+      b_iI[i] = (T)r[B2G_BD_INV_I];
+      b_gs[i] = (T)r[B2G_BD_GRAV_SCALE];
+      b_ld[i] = (T)r[B2G_BD_LIN_DAMP];
+      b_ad[i] = (T)r[B2G_BD_ANG_DAMP];
+      ctx->mass[i] = r[B2G_BD_MASS];
+      ctx->inertia[i] = r[B2G_BD_I];
+    }
+  }
+  std::vector<int4> f_info(nf);
+  std::vector<int2> f_wb(nf);
+  std::vector<T> f_geo((size_t)nf * GEO_STRIDE, (T)0);
+  std::vector<double> f_geo64((size_t)nf * 16, 0.0), f_rad64(nf);
+  std::vector<V2<T>> f_mat(nf);
+  std::vector<double4> f_fat(nf);
+  ctx->fix_body_global.resize(nf);
+  for (int w = 0; w < nw; ++w) {
+    const int wnb = b->body_off[w + 1] - b->body_off[w];
+    int last_body = -1;
+    for (int i = b->fix_off[w]; i < b->fix_off[w + 1]; ++i) {
+      const double* r = b->fix_d + (size_t)i * B2G_FIX_ND;
+      const int32_t* ri = b->fix_i + (size_t)i * B2G_FIX_NI;
+      const int body = ri[B2G_FI_BODY], type = ri[B2G_FI_SHAPE], nv = ri[B2G_FI_NVERTS];
+      if (body < 0 || body >= wnb) return set_error(ctx, B2G_E_INVALID, "fixture body out of range");
+      if (body < last_body) return set_error(ctx, B2G_E_INVALID, "fixtures must be grouped by body in body order");
+      last_body = body;
+      if (type == B2G_SHAPE_LOOP) return set_error(ctx, B2G_E_UNSUPPORTED, "loop shapes are not supported");
+      if (type != B2G_SHAPE_CIRCLE && type != B2G_SHAPE_POLYGON && type != B2G_SHAPE_EDGE)
+        return set_error(ctx, B2G_E_INVALID, "bad shape type");
+      if (type == B2G_SHAPE_POLYGON && (nv < 3 || nv > B2G_MAX_POLY_VERTS))
+        return set_error(ctx, B2G_E_UNSUPPORTED, "polygons need 3..8 vertices");
+      if (ri[B2G_FI_SENSOR]) return set_error(ctx, B2G_E_UNSUPPORTED, "sensors are not supported");
+      const int cat = ri[B2G_FI_CATEGORY], mask = ri[B2G_FI_MASK];
+      if (cat < 0 || cat > 0xFFFF || mask < 0 || mask > 0xFFFF)
+        return set_error(ctx, B2G_E_UNSUPPORTED, "collision bits must fit in 16 bits");
+      f_info[i] = make_int4(body, type | (nv << 8) | (ri[B2G_FI_EDGE_FLAGS] << 16), cat | (mask << 16),
+                            ri[B2G_FI_GROUP]);
+      f_wb[i] = make_int2(w, b->body_off[w] + body);
+      ctx->fix_body_global[i] = b->body_off[w] + body;
+      T* g = &f_geo[(size_t)i * GEO_STRIDE];
+      for (int k = 0; k < 16; ++k) g[GEO_VERTS + k] = (T)r[B2G_FD_VERTS + k];
+      for (int k = 0; k < 16; ++k) g[GEO_NORMALS + k] = (T)r[B2G_FD_NORMALS + k];
+      g[GEO_CENTROID] = (T)r[B2G_FD_CENTROID_X];
+      g[GEO_CENTROID + 1] = (T)r[B2G_FD_CENTROID_Y];
+      g[GEO_CIRCLE_P] = (T)r[B2G_FD_CIRCLE_PX];
+      g[GEO_CIRCLE_P + 1] = (T)r[B2G_FD_CIRCLE_PY];
+      for (int k = 0; k < 8; ++k) g[GEO_EDGE + k] = (T)r[B2G_FD_EDGE_V0X + k];
+      g[GEO_RADIUS] = (T)r[B2G_FD_RADIUS];
+      double* g64 = &f_geo64[(size_t)i * 16];
+      if (type == B2G_SHAPE_POLYGON) {
+        for (int k = 0; k < 16; ++k) g64[k] = r[B2G_FD_VERTS + k];
+      } else if (type == B2G_SHAPE_CIRCLE) {
+        g64[0] = r[B2G_FD_CIRCLE_PX];
+        g64[1] = r[B2G_FD_CIRCLE_PY];
+      } else {
+        g64[0] = r[B2G_FD_EDGE_V1X];
+        g64[1] = r[B2G_FD_EDGE_V1Y];
+        g64[2] = r[B2G_FD_EDGE_V2X];
+        g64[3] = r[B2G_FD_EDGE_V2Y];
+      }
+      f_rad64[i] = r[B2G_FD_RADIUS];
+      f_mat[i] = v2<T>((T)r[B2G_FD_FRICTION], (T)r[B2G_FD_RESTITUTION]);
+      f_fat[i] = make_double4(r[B2G_FD_FAT_LX], r[B2G_FD_FAT_LY], r[B2G_FD_FAT_UX], r[B2G_FD_FAT_UY]);
+    }
+  }
+  std::vector<int2> c_pair(ncap, make_int2(0, 0));
+  std::vector<int> c_state(ncap, 0);
+  std::vector<V2<T>> c_mat(ncap);
+  std::vector<V4<T>> c_lnlp(ncap), c_pts(ncap), c_imp(ncap);
+  std::vector<uint2> c_id(ncap, make_uint2(0, 0));
+  for (int w = 0; w < nw; ++w) {
+    const int wnf = b->fix_off[w + 1] - b->fix_off[w];
+    for (int k = 0; k < b->con_off[w + 1] - b->con_off[w]; ++k) {
+      const int i = b->con_off[w] + k, s = ctx->con_base[w] + k;
+      const double* r = b->con_d + (size_t)i * B2G_CON_ND;
+      const int32_t* ri = b->con_i + (size_t)i * B2G_CON_NI;
+      if (ri[B2G_CI_FA] < 0 || ri[B2G_CI_FA] >= wnf || ri[B2G_CI_FB] < 0 || ri[B2G_CI_FB] >= wnf)
+        return set_error(ctx, B2G_E_INVALID, "contact fixture out of range");
+      const int fl = ri[B2G_CI_FLAGS];
+      const int pc = ri[B2G_CI_PC], mt = ri[B2G_CI_MTYPE];
+      if (pc < 0 || pc > 2 || mt < 0 || mt > 2) return set_error(ctx, B2G_E_INVALID, "bad manifold");
+      c_pair[s] = make_int2(ri[B2G_CI_FA], ri[B2G_CI_FB]);
+      c_state[s] = CS_LIVE | ((fl & B2G_CF_TOUCHING) ? CS_TOUCHING : 0) | ((fl & B2G_CF_ENABLED) ? CS_ENABLED : 0) |
+                   ((fl & B2G_CF_FILTER) ? CS_FILTER : 0) | (mt << CS_MTYPE_SHIFT) | (pc << CS_PC_SHIFT);
+      c_mat[s] = v2<T>((T)r[B2G_CD_FRICTION], (T)r[B2G_CD_RESTITUTION]);
+      c_lnlp[s] = V4<T>{(T)r[B2G_CD_LN_X], (T)r[B2G_CD_LN_Y], (T)r[B2G_CD_LP_X], (T)r[B2G_CD_LP_Y]};
+      c_pts[s] = V4<T>{(T)r[B2G_CD_P0_X], (T)r[B2G_CD_P0_Y], (T)r[B2G_CD_P1_X], (T)r[B2G_CD_P1_Y]};
+      c_imp[s] = V4<T>{(T)r[B2G_CD_P0_NI], (T)r[B2G_CD_P0_TI], (T)r[B2G_CD_P1_NI], (T)r[B2G_CD_P1_TI]};
+      c_id[s] = make_uint2((uint32_t)ri[B2G_CI_ID0], (uint32_t)ri[B2G_CI_ID1]);
+    }
+  }
+  std::vector<int> mv(nf, 0);
+  for (int w = 0; w < nw; ++w) {
+    const int wnf = b->fix_off[w + 1] - b->fix_off[w];
+    std::vector<char> seen(wnf, 0);
+    for (int k = 0; k < b->move_off[w + 1] - b->move_off[w]; ++k) {
+      const int f = b->move[b->move_off[w] + k];
+      if (f < 0 || f >= wnf) return set_error(ctx, B2G_E_INVALID, "move buffer entry out of range");
+      if (seen[f]) return set_error(ctx, B2G_E_INVALID, "duplicate move buffer entry");
+      seen[f] = 1;
+      mv[b->fix_off[w] + k] = f;
+    }
+  }
+
+  // ---- device allocation ----
+  int *w_body_off, *w_fix_off, *w_con_base, *w_con_cap;
+  TRY(dalloc(ctx, &w_body_off, nw + 1));
+  TRY(dalloc(ctx, &w_fix_off, nw + 1));
+  TRY(dalloc(ctx, &w_con_base, nw + 1));
+  TRY(dalloc(ctx, &w_con_cap, nw));
+  d.w_body_off = w_body_off;
+  d.w_fix_off = w_fix_off;
+  d.w_con_base = w_con_base;
+  d.w_con_cap = w_con_cap;
+  TRY(dalloc(ctx, &d.w_con_used, nw));
+  TRY(dalloc(ctx, &d.w_flags, nw));
+  double* w_grav_d;
+  TRY(dalloc(ctx, &w_grav_d, 2 * (size_t)nw));
+  d.w_grav = w_grav_d;
+  TRY(dalloc(ctx, &d.w_inv_dt0, nw));
+  TRY(dalloc(ctx, &d.w_err, nw));
+  TRY(dalloc(ctx, &d.w_move_n, nw));
+  TRY(dalloc(ctx, &d.w_asleep, nw));
+  TRY(dalloc(ctx, &d.w_nisl, nw));
+  TRY(dalloc(ctx, &d.b_type, nb));
+  TRY(dalloc(ctx, &d.b_flags, nb));
+  TRY(dalloc(ctx, &d.b_island, nb));
+  TRY(dalloc(ctx, &d.b_c, nb));
+  TRY(dalloc(ctx, &d.b_a, nb));
+  TRY(dalloc(ctx, &d.b_c0, nb));
+  TRY(dalloc(ctx, &d.b_a0, nb));
+  TRY(dalloc(ctx, &d.b_xf, nb));
+  TRY(dalloc(ctx, &d.b_lc, nb));
+  TRY(dalloc(ctx, &d.b_sleep, nb));
+  TRY(dalloc(ctx, &d.b_v, nb));
+  TRY(dalloc(ctx, &d.b_w, nb));
+  TRY(dalloc(ctx, &d.b_im, nb));
+  TRY(dalloc(ctx, &d.b_iI, nb));
+  TRY(dalloc(ctx, &d.b_f, nb));
+  TRY(dalloc(ctx, &d.b_t, nb));
+  TRY(dalloc(ctx, &d.b_gs, nb));
+  TRY(dalloc(ctx, &d.b_ld, nb));
+  TRY(dalloc(ctx, &d.b_ad, nb));
+  TRY(dalloc(ctx, &d.f_info, nf));
+  TRY(dalloc(ctx, &d.f_wb, nf));
+  TRY(dalloc(ctx, &d.f_geo, (size_t)nf * GEO_STRIDE));
+  TRY(dalloc(ctx, &d.f_geo64, (size_t)nf * 16));
+  TRY(dalloc(ctx, &d.f_rad64, nf));
+  TRY(dalloc(ctx, &d.f_mat, nf));
+  TRY(dalloc(ctx, &d.f_fat, nf));
+  TRY(dalloc(ctx, &d.f_moved, nf));
+  TRY(dalloc(ctx, &d.mv, nf));
+  TRY(dalloc(ctx, &d.c_pair, ncap));
+  TRY(dalloc(ctx, &d.c_state, ncap));
+  TRY(dalloc(ctx, &d.c_mat, ncap));
+  TRY(dalloc(ctx, &d.c_lnlp, ncap));
+  TRY(dalloc(ctx, &d.c_pts, ncap));
+  TRY(dalloc(ctx, &d.c_imp, ncap));
+  TRY(dalloc(ctx, &d.c_id, ncap));
+  TRY(dalloc(ctx, &d.i_bodies, nb));
+  TRY(dalloc(ctx, &d.i_contacts, ncap));
+  TRY(dalloc(ctx, &d.i_cidx, ncap));
+  TRY(dalloc(ctx, &d.i_info, nb));
+  TRY(dalloc(ctx, &d.q_isl, nb));
+  TRY(dalloc(ctx, &d.q_nc, nb));
+  TRY(dalloc(ctx, &d.q_count, 1));
+  TRY(dalloc(ctx, &d.pv, nb));
+  TRY(dalloc(ctx, &d.vc, ncap));
+  TRY(dalloc(ctx, &d.pc, ncap));
+
+  TRY(upload(ctx, w_body_off, b->body_off, nw + 1));
+  TRY(upload(ctx, w_fix_off, b->fix_off, nw + 1));
+  TRY(upload(ctx, w_con_base, ctx->con_base.data(), nw + 1));
+  TRY(upload(ctx, w_con_cap, ctx->con_cap.data(), nw));
+  TRY(upload(ctx, d.w_con_used, w_con_used.data(), nw));
+  TRY(upload(ctx, d.w_flags, w_flags.data(), nw));
+  TRY(upload(ctx, w_grav_d, w_grav.data(), 2 * (size_t)nw));
+  TRY(upload(ctx, d.w_inv_dt0, w_inv_dt0.data(), nw));
+  TRY(upload(ctx, d.w_err, zeros_w.data(), nw));
+  TRY(upload(ctx, d.w_move_n, w_move_n.data(), nw));
+  TRY(upload(ctx, d.w_asleep, w_asleep.data(), nw));
+  TRY(upload(ctx, d.w_nisl, zeros_w.data(), nw));
+  TRY(upload(ctx, d.b_type, b_type.data(), nb));
+  TRY(upload(ctx, d.b_flags, b_flags.data(), nb));
+  TRY(upload(ctx, d.b_island, b_island.data(), nb));
+  TRY(upload(ctx, d.b_c, b_c.data(), nb));
+  TRY(upload(ctx, d.b_a, b_a.data(), nb));
+  TRY(upload(ctx, d.b_c0, b_c0.data(), nb));
+  TRY(upload(ctx, d.b_a0, b_a0.data(), nb));
+  TRY(upload(ctx, d.b_xf, b_xf.data(), nb));
+  TRY(upload(ctx, d.b_lc, b_lc.data(), nb));
+  TRY(upload(ctx, d.b_sleep, b_sleep.data(), nb));
+  TRY(upload(ctx, d.b_v, b_v.data(), nb));
+  TRY(upload(ctx, d.b_w, b_w.data(), nb));
+  TRY(upload(ctx, d.b_im, b_im.data(), nb));
+  TRY(upload(ctx, d.b_iI, b_iI.data(), nb));
+  TRY(upload(ctx, d.b_f, b_f.data(), nb));
+  TRY(upload(ctx, d.b_t, b_t.data(), nb));
+  TRY(upload(ctx, d.b_gs, b_gs.data(), nb));
+  TRY(upload(ctx, d.b_ld, b_ld.data(), nb));
+  TRY(upload(ctx, d.b_ad, b_ad.data(), nb));
+  TRY(upload(ctx, d.f_info, f_info.data(), nf));
+  TRY(upload(ctx, d.f_wb, f_wb.data(), nf));
+  TRY(upload(ctx, d.f_geo, f_geo.data(), (size_t)nf * GEO_STRIDE));
+  TRY(upload(ctx, d.f_geo64, f_geo64.data(), (size_t)nf * 16));
+  TRY(upload(ctx, d.f_rad64, f_rad64.data(), nf));
+  TRY(upload(ctx, d.f_mat, f_mat.data(), nf));
+  TRY(upload(ctx, d.f_fat, f_fat.data(), nf));
+  TRY(upload(ctx, d.mv, mv.data(), nf));
+  TRY(upload(ctx, d.c_pair, c_pair.data(), ncap));
+  TRY(upload(ctx, d.c_state, c_state.data(), ncap));
+  TRY(upload(ctx, d.c_mat, c_mat.data(), ncap));
+  TRY(upload(ctx, d.c_lnlp, c_lnlp.data(), ncap));
+  TRY(upload(ctx, d.c_pts, c_pts.data(), ncap));
+  TRY(upload(ctx, d.c_imp, c_imp.data(), ncap));
+  TRY(upload(ctx, d.c_id, c_id.data(), ncap));
+  std::vector<int> zeros_f(nf, 0);
+  TRY(upload(ctx, d.f_moved, zeros_f.data(), nf));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+template <typename T>
+static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, bool timed) {
+  Dev<T> d = *(Dev<T>*)ctx->dev;
+  d.dt = dt;
+  d.inv_dt = dt > 0.0 ? 1.0 / dt : 0.0;
+  d.vel_iters = vi;
+  d.pos_iters = pi;
+  const int nw = ctx->nw;
+  hipStream_t st = ctx->stream;
+  if (first_pairs) {
+    hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, 1, 0, 1);
+  }
+  if (timed) HIPCHK(hipEventRecord(ctx->ev[0], st));
+  hipLaunchKernelGGL(k_reset_queue<T>, dim3(1), dim3(64), 0, st, d);
+  const int chunks = (ctx->max_ccap + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_collide<T>, dim3(std::max(chunks, 1), nw), dim3(kBlock), 0, st, d);
+  if (timed) HIPCHK(hipEventRecord(ctx->ev[1], st));
+  if (dt > 0.0) {
+    hipLaunchKernelGGL(k_islands<T>, dim3(nw), dim3(kBlock), ctx->smem_islands, st, d);
+    if (timed) HIPCHK(hipEventRecord(ctx->ev[2], st));
+    int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024 * 8));
+    hipLaunchKernelGGL(k_solve<T>, dim3(nblk), dim3(kBlock), 0, st, d);
+    if (timed) HIPCHK(hipEventRecord(ctx->ev[3], st));
+    hipLaunchKernelGGL(k_sync<T>, dim3(std::max(1, (ctx->nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d);
+    if (timed) HIPCHK(hipEventRecord(ctx->ev[4], st));
+    hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, 0, 1, 1);
+  } else {
+    if (timed) {
+      HIPCHK(hipEventRecord(ctx->ev[2], st));
+      HIPCHK(hipEventRecord(ctx->ev[3], st));
+      HIPCHK(hipEventRecord(ctx->ev[4], st));
+    }
+    hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, 0, 1, 0);
+  }
+  if (timed) HIPCHK(hipEventRecord(ctx->ev[5], st));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int dispatch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, bool timed) {
+  if (ctx->precision == B2G_FP64) return launch_step<double>(ctx, dt, vi, pi, first_pairs, timed);
+  return launch_step<float>(ctx, dt, vi, pi, first_pairs, timed);
+}
+
+template <typename T>
+static int download_batch(b2g_ctx* ctx, b2g_batch_out* o) {
+  Dev<T>& d = *(Dev<T>*)ctx->dev;
+  const int nw = ctx->nw, nb = ctx->nb, nf = ctx->nf, ncap = ctx->ncap;
+  if (o->n_worlds != nw) return set_error(ctx, B2G_E_INVALID, "n_worlds mismatch");
+  std::vector<int> w_used(nw), w_flags(nw), w_move_n(nw);
+  std::vector<double> w_inv_dt0(nw);
+  TRY(download(ctx, w_used.data(), d.w_con_used, nw));
+  TRY(download(ctx, w_flags.data(), d.w_flags, nw));
+  TRY(download(ctx, w_inv_dt0.data(), d.w_inv_dt0, nw));
+  TRY(download(ctx, w_move_n.data(), d.w_move_n, nw));
+  std::vector<int> b_type(nb), b_flags(nb), b_island(nb);
+  std::vector<double2> b_c(nb), b_c0(nb), b_lc(nb);
+  std::vector<double> b_a(nb), b_a0(nb), b_sleep(nb);
+  std::vector<double4> b_xf(nb);
+  std::vector<V2<double>> b_v(nb);
+  std::vector<V2<T>> b_f(nb);
+  std::vector<double> b_w(nb);
+  std::vector<T> b_im(nb), b_iI(nb), b_t(nb), b_gs(nb), b_ld(nb), b_ad(nb);
+  TRY(download(ctx, b_type.data(), d.b_type, nb));
+  TRY(download(ctx, b_flags.data(), d.b_flags, nb));
+  TRY(download(ctx, b_island.data(), d.b_island, nb));
+  TRY(download(ctx, b_c.data(), d.b_c, nb));
+  TRY(download(ctx, b_c0.data(), d.b_c0, nb));
+  TRY(download(ctx, b_lc.data(), d.b_lc, nb));
+  TRY(download(ctx, b_a.data(), d.b_a, nb));
+  TRY(download(ctx, b_a0.data(), d.b_a0, nb));
+  TRY(download(ctx, b_sleep.data(), d.b_sleep, nb));
+  TRY(download(ctx, b_xf.data(), d.b_xf, nb));
+  TRY(download(ctx, b_v.data(), d.b_v, nb));
+  TRY(download(ctx, b_w.data(), d.b_w, nb));
+  TRY(download(ctx, b_f.data(), d.b_f, nb));
+  TRY(download(ctx, b_t.data(), d.b_t, nb));
+  TRY(download(ctx, b_im.data(), d.b_im, nb));
+  TRY(download(ctx, b_iI.data(), d.b_iI, nb));
+  TRY(download(ctx, b_gs.data(), d.b_gs, nb));
+  TRY(download(ctx, b_ld.data(), d.b_ld, nb));
+  TRY(download(ctx, b_ad.data(), d.b_ad, nb));
+  std::vector<double4> f_fat(nf);
+  TRY(download(ctx, f_fat.data(), d.f_fat, nf));
+  std::vector<int2> c_pair(ncap);
+  std::vector<int> c_state(ncap);
+  std::vector<V2<T>> c_mat(ncap);
+  std::vector<V4<T>> c_lnlp(ncap), c_pts(ncap), c_imp(ncap);
+  std::vector<uint2> c_id(ncap);
+  TRY(download(ctx, c_pair.data(), d.c_pair, ncap));
+  TRY(download(ctx, c_state.data(), d.c_state, ncap));
+  TRY(download(ctx, c_mat.data(), d.c_mat, ncap));
+  TRY(download(ctx, c_lnlp.data(), d.c_lnlp, ncap));
+  TRY(download(ctx, c_pts.data(), d.c_pts, ncap));
+  TRY(download(ctx, c_imp.data(), d.c_imp, ncap));
+  TRY(download(ctx, c_id.data(), d.c_id, ncap));
+  std::vector<int> mv(nf);
+  TRY(download(ctx, mv.data(), d.mv, nf));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+
+  for (int w = 0; w < nw; ++w) {
+    double* wd = o->world_d + (size_t)w * B2G_WORLD_ND;
+    int32_t* wi = o->world_i + (size_t)w * B2G_WORLD_NI;
+    for (int k = 0; k < B2G_WORLD_ND; ++k) wd[k] = ctx->world_d[(size_t)w * B2G_WORLD_ND + k];
+    for (int k = 0; k < B2G_WORLD_NI; ++k) wi[k] = ctx->world_i[(size_t)w * B2G_WORLD_NI + k];
+    wd[B2G_WD_INV_DT0] = w_inv_dt0[w];
+    wi[B2G_WI_FLAGS] = w_flags[w];
+  }
+  for (int i = 0; i < nb; ++i) {
+    double* r = o->body_d + (size_t)i * B2G_BODY_ND;
+    int32_t* ri = o->body_i + (size_t)i * B2G_BODY_NI;
+    std::memset(r, 0, sizeof(double) * B2G_BODY_ND);
+    ri[B2G_BI_TYPE] = b_type[i];
+    ri[B2G_BI_FLAGS] = b_flags[i];
+    ri[B2G_BI_ISLAND] = b_island[i];
+    ri[B2G_BI_PAD] = 0;
+    r[B2G_BD_XF_PX] = b_xf[i].x; r[B2G_BD_XF_PY] = b_xf[i].y; r[B2G_BD_XF_C] = b_xf[i].z; r[B2G_BD_XF_S] = b_xf[i].w;
+    r[B2G_BD_LC_X] = b_lc[i].x; r[B2G_BD_LC_Y] = b_lc[i].y;
+    r[B2G_BD_C0_X] = b_c0[i].x; r[B2G_BD_C0_Y] = b_c0[i].y;
+    r[B2G_BD_C_X] = b_c[i].x; r[B2G_BD_C_Y] = b_c[i].y;
+    r[B2G_BD_A0] = b_a0[i]; r[B2G_BD_A] = b_a[i];
+    r[B2G_BD_V_X] = (double)b_v[i].x; r[B2G_BD_V_Y] = (double)b_v[i].y; r[B2G_BD_W] = (double)b_w[i];
+    r[B2G_BD_F_X] = (double)b_f[i].x; r[B2G_BD_F_Y] = (double)b_f[i].y; r[B2G_BD_TORQUE] = (double)b_t[i];
+    r[B2G_BD_MASS] = ctx->mass[i]; r[B2G_BD_INV_MASS] = (double)b_im[i];
+    r[B2G_BD_I] = ctx->inertia[i]; r[B2G_BD_INV_I] = (double)b_iI[i];
+    r[B2G_BD_GRAV_SCALE] = (double)b_gs[i]; r[B2G_BD_LIN_DAMP] = (double)b_ld[i]; r[B2G_BD_ANG_DAMP] = (double)b_ad[i];
+    r[B2G_BD_SLEEP_TIME] = b_sleep[i];
+  }
+  for (int i = 0; i < nf; ++i) {
+    double* r = o->fix_d + (size_t)i * B2G_FIX_ND;
+    r[B2G_FD_FAT_LX] = f_fat[i].x; r[B2G_FD_FAT_LY] = f_fat[i].y;
+    r[B2G_FD_FAT_UX] = f_fat[i].z; r[B2G_FD_FAT_UY] = f_fat[i].w;
+  }
+  int co = 0, mo = 0;
+  o->con_off[0] = 0;
+  o->move_off[0] = 0;
+  for (int w = 0; w < nw; ++w) {
+    const int cb = ctx->con_base[w];
+    for (int sl = 0; sl < w_used[w]; ++sl) {
+      const int s = cb + sl;
+      const int st = c_state[s];
+      if (!(st & CS_LIVE)) continue;
+      if (co >= o->con_cap) return set_error(ctx, B2G_E_CAPACITY, "output contact capacity too small");
+      double* r = o->con_d + (size_t)co * B2G_CON_ND;
+      int32_t* ri = o->con_i + (size_t)co * B2G_CON_NI;
+      std::memset(r, 0, sizeof(double) * B2G_CON_ND);
+      std::memset(ri, 0, sizeof(int32_t) * B2G_CON_NI);
+      ri[B2G_CI_FA] = c_pair[s].x;
+      ri[B2G_CI_FB] = c_pair[s].y;
+      ri[B2G_CI_FLAGS] = ((st & CS_TOUCHING) ? B2G_CF_TOUCHING : 0) | ((st & CS_ENABLED) ? B2G_CF_ENABLED : 0) |
+                         ((st & CS_FILTER) ? B2G_CF_FILTER : 0);
+      ri[B2G_CI_MTYPE] = (st >> CS_MTYPE_SHIFT) & 3;
+      ri[B2G_CI_PC] = (st >> CS_PC_SHIFT) & 3;
+      ri[B2G_CI_ID0] = (int32_t)c_id[s].x;
+      ri[B2G_CI_ID1] = (int32_t)c_id[s].y;
+      r[B2G_CD_FRICTION] = (double)c_mat[s].x; r[B2G_CD_RESTITUTION] = (double)c_mat[s].y;
+      r[B2G_CD_LN_X] = (double)c_lnlp[s].x; r[B2G_CD_LN_Y] = (double)c_lnlp[s].y;
+      r[B2G_CD_LP_X] = (double)c_lnlp[s].z; r[B2G_CD_LP_Y] = (double)c_lnlp[s].w;
+      r[B2G_CD_P0_X] = (double)c_pts[s].x; r[B2G_CD_P0_Y] = (double)c_pts[s].y;
+      r[B2G_CD_P1_X] = (double)c_pts[s].z; r[B2G_CD_P1_Y] = (double)c_pts[s].w;
+      r[B2G_CD_P0_NI] = (double)c_imp[s].x; r[B2G_CD_P0_TI] = (double)c_imp[s].y;
+      r[B2G_CD_P1_NI] = (double)c_imp[s].z; r[B2G_CD_P1_TI] = (double)c_imp[s].w;
+      ++co;
+    }
+    for (int k = 0; k < w_move_n[w]; ++k) {
+      if (mo >= o->move_cap) return set_error(ctx, B2G_E_CAPACITY, "output move capacity too small");
+      o->move[mo++] = mv[ctx->fix_off[w] + k];
+    }
+    o->con_off[w + 1] = co;
+    o->move_off[w + 1] = mo;
+  }
+  return 0;
+}
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+B2G_API int b2g_version(void) { return B2G_VERSION; }
+
+B2G_API int b2g_create(const b2g_desc* desc, b2g_ctx** out) {
+  if (!out) return B2G_E_INVALID;
+  *out = nullptr;
+  b2g_ctx* ctx = new b2g_ctx();
+  if (desc) {
+    if (desc->precision != B2G_FP32 && desc->precision != B2G_FP64) {
+      delete ctx;
+      return B2G_E_INVALID;
+    }
+    ctx->precision = desc->precision;
+    ctx->device = desc->device;
+    if (desc->contact_factor > 0) ctx->contact_factor = desc->contact_factor;
+    if (desc->contact_extra > 0) ctx->contact_extra = desc->contact_extra;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    delete ctx;
+    return B2G_E_NODEVICE;
+  }
+  if (ctx->device < 0 || ctx->device >= ndev || hipSetDevice(ctx->device) != hipSuccess) {
+    delete ctx;
+    return B2G_E_NODEVICE;
+  }
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return B2G_E_HIP;
+  }
+  for (int i = 0; i < 8; ++i) hipEventCreate(&ctx->ev[i]);
+  ctx->have_events = true;
+  *out = ctx;
+  return B2G_OK;
+}
+
+B2G_API int b2g_destroy(b2g_ctx* ctx) {
+  if (!ctx) return B2G_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_device(ctx);
+  if (ctx->have_events)
+    for (int i = 0; i < 8; ++i) hipEventDestroy(ctx->ev[i]);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return B2G_OK;
+}
+
+B2G_API int b2g_upload(b2g_ctx* ctx, const b2g_batch* batch) {
+  if (!ctx || !batch || batch->n_worlds <= 0) return set_error(ctx, B2G_E_INVALID, "bad arguments");
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  free_device(ctx);
+  int rc = ctx->precision == B2G_FP64 ? upload_batch<double>(ctx, batch) : upload_batch<float>(ctx, batch);
+  if (rc) free_device(ctx);
+  return rc;
+}
+
+B2G_API int b2g_step(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (vel_iters < 0 || pos_iters < 0) return set_error(ctx, B2G_E_INVALID, "negative iteration count");
+  hipSetDevice(ctx->device);
+  bool first = ctx->pending_new_fixture;
+  ctx->pending_new_fixture = false;
+  return dispatch_step(ctx, dt, vel_iters, pos_iters, first, false);
+}
+
+B2G_API int b2g_step_n(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, int use_graph) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (n < 0 || vel_iters < 0 || pos_iters < 0) return set_error(ctx, B2G_E_INVALID, "negative count");
+  hipSetDevice(ctx->device);
+  int i = 0;
+  if (ctx->pending_new_fixture && n > 0) {
+    ctx->pending_new_fixture = false;
+    TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, true, false));
+    ++i;
+  }
+  if (!use_graph) {
+    for (; i < n; ++i) TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, false, false));
+    return B2G_OK;
+  }
+  if (i < n) {
+    if (!ctx->graph_exec || ctx->graph_dt != dt || ctx->graph_vi != vel_iters || ctx->graph_pi != pos_iters) {
+      if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+      if (ctx->graph) hipGraphDestroy(ctx->graph);
+      ctx->graph_exec = nullptr;
+      ctx->graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int rc = dispatch_step(ctx, dt, vel_iters, pos_iters, false, false);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return set_error(ctx, B2G_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+      ctx->graph = g;
+      HIPCHK(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+      ctx->graph_dt = dt;
+      ctx->graph_vi = vel_iters;
+      ctx->graph_pi = pos_iters;
+    }
+    for (; i < n; ++i) HIPCHK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+  }
+  return B2G_OK;
+}
+
+B2G_API int b2g_sync(b2g_ctx* ctx) {
+  if (!ctx) return B2G_E_INVALID;
+  hipSetDevice(ctx->device);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (!ctx->dev) return B2G_OK;
+  std::vector<int> err(ctx->nw);
+  int* w_err = ctx->precision == B2G_FP64 ? ((Dev<double>*)ctx->dev)->w_err : ((Dev<float>*)ctx->dev)->w_err;
+  HIPCHK(hipMemcpy(err.data(), w_err, sizeof(int) * ctx->nw, hipMemcpyDeviceToHost));
+  int e = 0;
+  for (int v : err) e |= v;
+  if (e & WERR_CAPACITY) return set_error(ctx, B2G_E_CAPACITY, "contact segment capacity exceeded");
+  if (e & WERR_NEWPAIRS) return set_error(ctx, B2G_E_CAPACITY, "too many new pairs in one step");
+  if (e & WERR_UNSUPPORTED) return set_error(ctx, B2G_E_UNSUPPORTED, "unsupported shape pair");
+  return B2G_OK;
+}
+
+B2G_API int b2g_counts(b2g_ctx* ctx, int32_t* counts) {
+  if (!ctx || !ctx->dev || !counts) return B2G_E_INVALID;
+  hipSetDevice(ctx->device);
+  TRY(b2g_sync(ctx));
+  const int nw = ctx->nw;
+  std::vector<int> used(nw), isl(nw);
+  int* d_used = ctx->precision == B2G_FP64 ? ((Dev<double>*)ctx->dev)->w_con_used : ((Dev<float>*)ctx->dev)->w_con_used;
+  int* d_isl = ctx->precision == B2G_FP64 ? ((Dev<double>*)ctx->dev)->w_nisl : ((Dev<float>*)ctx->dev)->w_nisl;
+  HIPCHK(hipMemcpy(used.data(), d_used, sizeof(int) * nw, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(isl.data(), d_isl, sizeof(int) * nw, hipMemcpyDeviceToHost));
+  long long su = 0, si = 0;
+  for (int w = 0; w < nw; ++w) {
+    su += used[w];
+    si += isl[w];
+  }
+  counts[0] = ctx->nb;
+  counts[1] = ctx->nf;
+  counts[2] = (int32_t)std::min<long long>(su, 0x7FFFFFFF);  // slots in use (live contacts are <= this)
+  counts[3] = (int32_t)std::min<long long>(si, 0x7FFFFFFF);  // islands of the last step
+  return B2G_OK;
+}
+
+B2G_API int b2g_download(b2g_ctx* ctx, b2g_batch_out* out) {
+  if (!ctx || !ctx->dev || !out) return set_error(ctx, B2G_E_INVALID, "bad arguments");
+  hipSetDevice(ctx->device);
+  TRY(b2g_sync(ctx));
+  return ctx->precision == B2G_FP64 ? download_batch<double>(ctx, out) : download_batch<float>(ctx, out);
+}
+
+B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n) {
+  if (!ctx || !buf || n == 0) return B2G_E_INVALID;
+  std::snprintf(buf, n, "%s", ctx->err.c_str());
+  return B2G_OK;
+}
+
+B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, float* ms_out) {
+  if (!ctx || !ctx->dev || !ms_out || n <= 0) return set_error(ctx, B2G_E_INVALID, "bad arguments");
+  hipSetDevice(ctx->device);
+  if (ctx->pending_new_fixture) {
+    ctx->pending_new_fixture = false;
+    TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, true, false));
+  }
+  for (int k = 0; k < 5; ++k) ms_out[k] = 0.f;
+  for (int i = 0; i < n; ++i) {
+    TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, false, true));
+    HIPCHK(hipEventSynchronize(ctx->ev[5]));
+    for (int k = 0; k < 5; ++k) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+      ms_out[k] += ms;
+    }
+  }
+  for (int k = 0; k < 5; ++k) ms_out[k] /= (float)n;
+  return B2G_OK;
+}
+
+B2G_API int b2g_narrowphase(int precision, int n, const double* fa_d, const int32_t* fa_i, const double* fb_d,
+                            const int32_t* fb_i, const double* xfa, const double* xfb, double* con_d,
+                            int32_t* con_i) {
+  if (n < 0 || (precision != B2G_FP32 && precision != B2G_FP64)) return B2G_E_INVALID;
+  if (n == 0) return B2G_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return B2G_E_NODEVICE;
+  const bool f64 = precision == B2G_FP64;
+  const size_t ts = f64 ? sizeof(double) : sizeof(float);
+  std::vector<unsigned char> ga((size_t)n * GEO_STRIDE * ts), gb((size_t)n * GEO_STRIDE * ts);
+  std::vector<int> ia(n), ib(n);
+  auto fill = [&](const double* fd, const int32_t* fi, std::vector<unsigned char>& g, std::vector<int>& info) {
+    for (int k = 0; k < n; ++k) {
+      const double* r = fd + (size_t)k * B2G_FIX_ND;
+      const int32_t* ri = fi + (size_t)k * B2G_FIX_NI;
+      double tmp[GEO_STRIDE] = {0};
+      for (int j = 0; j < 16; ++j) tmp[GEO_VERTS + j] = r[B2G_FD_VERTS + j];
+      for (int j = 0; j < 16; ++j) tmp[GEO_NORMALS + j] = r[B2G_FD_NORMALS + j];
+      tmp[GEO_CENTROID] = r[B2G_FD_CENTROID_X];
+      tmp[GEO_CENTROID + 1] = r[B2G_FD_CENTROID_Y];
+      tmp[GEO_CIRCLE_P] = r[B2G_FD_CIRCLE_PX];
+      tmp[GEO_CIRCLE_P + 1] = r[B2G_FD_CIRCLE_PY];
+      for (int j = 0; j < 8; ++j) tmp[GEO_EDGE + j] = r[B2G_FD_EDGE_V0X + j];
+      tmp[GEO_RADIUS] = r[B2G_FD_RADIUS];
+      for (int j = 0; j < GEO_STRIDE; ++j) {
+        if (f64) ((double*)g.data())[(size_t)k * GEO_STRIDE + j] = tmp[j];
+        else ((float*)g.data())[(size_t)k * GEO_STRIDE + j] = (float)tmp[j];
+      }
+      info[k] = ri[B2G_FI_SHAPE] | (ri[B2G_FI_NVERTS] << 8) | (ri[B2G_FI_EDGE_FLAGS] << 16);
+    }
+  };
+  fill(fa_d, fa_i, ga, ia);
+  fill(fb_d, fb_i, gb, ib);
+  void *dga = nullptr, *dgb = nullptr, *dia = nullptr, *dib = nullptr, *dxa = nullptr, *dxb = nullptr,
+       *doi = nullptr, *dot_ = nullptr;
+  int rc = B2G_OK;
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return B2G_E_HIP;
+  if (hipMalloc(&dga, ga.size()) != hipSuccess || hipMalloc(&dgb, gb.size()) != hipSuccess ||
+      hipMalloc(&dia, n * 4) != hipSuccess || hipMalloc(&dib, n * 4) != hipSuccess ||
+      hipMalloc(&dxa, (size_t)n * 32) != hipSuccess || hipMalloc(&dxb, (size_t)n * 32) != hipSuccess ||
+      hipMalloc(&doi, (size_t)n * 16) != hipSuccess || hipMalloc(&dot_, (size_t)n * 8 * ts) != hipSuccess) {
+    rc = B2G_E_HIP;
+  }
+  std::vector<int> oi((size_t)n * 4);
+  std::vector<unsigned char> ot((size_t)n * 8 * ts);
+  if (!rc) {
+    hipMemcpyAsync(dga, ga.data(), ga.size(), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dgb, gb.data(), gb.size(), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dia, ia.data(), n * 4, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dib, ib.data(), n * 4, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dxa, xfa, (size_t)n * 32, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dxb, xfb, (size_t)n * 32, hipMemcpyHostToDevice, st);
+    const int blocks = (n + 127) / 128;
+    if (f64)
+      hipLaunchKernelGGL(k_narrow_seam<double>, dim3(blocks), dim3(128), 0, st, n, (const double*)dga,
+                         (const int*)dia, (const double*)dgb, (const int*)dib, (const double*)dxa,
+                         (const double*)dxb, (int*)doi, (double*)dot_);
+    else
+      hipLaunchKernelGGL(k_narrow_seam<float>, dim3(blocks), dim3(128), 0, st, n, (const float*)dga,
+                         (const int*)dia, (const float*)dgb, (const int*)dib, (const double*)dxa,
+                         (const double*)dxb, (int*)doi, (float*)dot_);
+    hipMemcpyAsync(oi.data(), doi, (size_t)n * 16, hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(ot.data(), dot_, (size_t)n * 8 * ts, hipMemcpyDeviceToHost, st);
+    if (hipStreamSynchronize(st) != hipSuccess) rc = B2G_E_HIP;
+  }
+  hipFree(dga); hipFree(dgb); hipFree(dia); hipFree(dib); hipFree(dxa); hipFree(dxb); hipFree(doi); hipFree(dot_);
+  hipStreamDestroy(st);
+  if (rc) return rc;
+  for (int k = 0; k < n; ++k) {
+    double* r = con_d + (size_t)k * B2G_CON_ND;
+    int32_t* ri = con_i + (size_t)k * B2G_CON_NI;
+    std::memset(r, 0, sizeof(double) * B2G_CON_ND);
+    std::memset(ri, 0, sizeof(int32_t) * B2G_CON_NI);
+    if (oi[4 * k] < 0) {
+      rc = B2G_E_UNSUPPORTED;
+      continue;
+    }
+    ri[B2G_CI_MTYPE] = oi[4 * k];
+    ri[B2G_CI_PC] = oi[4 * k + 1];
+    ri[B2G_CI_ID0] = oi[4 * k + 2];
+    ri[B2G_CI_ID1] = oi[4 * k + 3];
+    double v[8];
+    for (int j = 0; j < 8; ++j)
+      v[j] = f64 ? ((double*)ot.data())[(size_t)k * 8 + j] : (double)((float*)ot.data())[(size_t)k * 8 + j];
+    r[B2G_CD_LN_X] = v[0]; r[B2G_CD_LN_Y] = v[1]; r[B2G_CD_LP_X] = v[2]; r[B2G_CD_LP_Y] = v[3];
+    r[B2G_CD_P0_X] = v[4]; r[B2G_CD_P0_Y] = v[5]; r[B2G_CD_P1_X] = v[6]; r[B2G_CD_P1_Y] = v[7];
+  }
+  return rc;
+}
+
+B2G_API int b2g_pairs(int n, const double* fat, int nmove, const int32_t* move, int32_t* out, int cap,
+                      int32_t* count) {
+  if (n < 0 || nmove < 0 || !count || (n > 0 && !fat) || (nmove > 0 && !move)) return B2G_E_INVALID;
+  *count = 0;
+  if (n == 0 || nmove == 0) return B2G_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return B2G_E_NODEVICE;
+  std::vector<int> mpos(n, -1);
+  for (int k = 0; k < nmove; ++k) {
+    if (move[k] < 0 || move[k] >= n) return B2G_E_INVALID;
+    if (mpos[move[k]] >= 0) return B2G_E_INVALID;  // duplicate entries are not part of this seam
+    mpos[move[k]] = k;
+  }
+  const long long work = (long long)nmove * n;
+  const int dcap = (int)std::min<long long>(work, 1 << 26);
+  void *dfat = nullptr, *dmove = nullptr, *dmpos = nullptr, *dout = nullptr, *dcnt = nullptr;
+  int rc = B2G_OK;
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return B2G_E_HIP;
+  if (hipMalloc(&dfat, (size_t)n * 32) != hipSuccess || hipMalloc(&dmove, (size_t)nmove * 4) != hipSuccess ||
+      hipMalloc(&dmpos, (size_t)n * 4) != hipSuccess || hipMalloc(&dout, (size_t)dcap * 8) != hipSuccess ||
+      hipMalloc(&dcnt, 4) != hipSuccess)
+    rc = B2G_E_HIP;
+  int hcount = 0;
+  std::vector<uint32_t> hout;
+  if (!rc) {
+    hipMemcpyAsync(dfat, fat, (size_t)n * 32, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dmove, move, (size_t)nmove * 4, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dmpos, mpos.data(), (size_t)n * 4, hipMemcpyHostToDevice, st);
+    int zero = 0;
+    hipMemcpyAsync(dcnt, &zero, 4, hipMemcpyHostToDevice, st);
+    const long long blocks = (work + 255) / 256;
+    hipLaunchKernelGGL(k_pairs_seam, dim3((unsigned)blocks), dim3(256), 0, st, n, (const double4*)dfat, nmove,
+                       (const int*)dmove, (const int*)dmpos, (uint32_t*)dout, dcap, (int*)dcnt);
+    hipMemcpyAsync(&hcount, dcnt, 4, hipMemcpyDeviceToHost, st);
+    if (hipStreamSynchronize(st) != hipSuccess) rc = B2G_E_HIP;
+    if (!rc) {
+      if (hcount > dcap) rc = B2G_E_CAPACITY;
+      else {
+        hout.resize((size_t)hcount * 2);
+        hipMemcpy(hout.data(), dout, (size_t)hcount * 8, hipMemcpyDeviceToHost);
+      }
+    }
+  }
+  hipFree(dfat); hipFree(dmove); hipFree(dmpos); hipFree(dout); hipFree(dcnt);
+  hipStreamDestroy(st);
+  if (rc) return rc;
+  // canonical order: move-buffer position, then partner index
+  std::vector<uint64_t> keys(hcount);
+  for (int k = 0; k < hcount; ++k) keys[k] = ((uint64_t)hout[2 * k] << 32) | hout[2 * k + 1];
+  std::sort(keys.begin(), keys.end());
+  if (hcount > cap) {
+    *count = hcount;
+    return B2G_E_CAPACITY;
+  }
+  for (int k = 0; k < hcount; ++k) {
+    out[2 * k] = move[(int)(keys[k] >> 32)];
+    out[2 * k + 1] = (int32_t)(keys[k] & 0xFFFFFFFFu);
+  }
+  *count = hcount;
+  return B2G_OK;
+}
+
+}  // extern "C"

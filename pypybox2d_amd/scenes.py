@@ -80,8 +80,16 @@ def _random_shape(rng):
     return ("polygon", verts), rad
 
 
-def mixed_world(rng, n_bodies: int, half_width: float = 20.0, height: float = 40.0):
-    """Boxes / circles / convex polygons (40/30/30) placed without overlap above an edge."""
+def mixed_world(rng, n_bodies: int, half_width: float = 20.0, height: float = None):
+    """Boxes / circles / convex polygons (40/30/30) placed without overlap above an edge.
+
+    Centres are rejection-sampled in [-half_width, half_width] x [1, 1 + height]
+    with non-overlapping bounding circles.  The default region is 40 x 40 and
+    grows in height for large worlds so that the bounding circles cover at
+    most ~40% of it (sequential rejection sampling jams near 55%).
+    """
+    if height is None:
+        height = max(40.0, 4.7 * n_bodies / (2.0 * half_width))
     bodies = [_body("static", (0.0, 0.0), [_fix(("edge", (-40.0, 0.0), (40.0, 0.0)))])]
     cell = 3.0
     grid = {}

@@ -1,0 +1,189 @@
+"""GPU parity: the HIP pipeline through the C-ABI against the FP64 oracle.
+
+Bars:
+  * broad phase: the pair list equals the oracle's exactly (set and order);
+  * narrow phase, FP64: integer fields exact, floats within 1e-9;
+    FP32: integer fields exact on >= 98% of the cases, floats within 2e-4;
+  * one step from an identical state: relative error (floors: 1 m, 1 rad,
+    |g| dt) <= 1e-10 in FP64 and <= 1e-5 in FP32 for positions, angles and
+    velocities; contact lists identical;
+  * graph replay equals eager stepping bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import parity_util as PU
+from conftest import GOLDEN
+from oracle import oracle as O
+from pypybox2d_amd import _native as N
+from pypybox2d_amd import builder, layout as L, scenes
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 60
+
+
+def load_init(name):
+    z = np.load(os.path.join(GOLDEN, name))
+    return L.BatchArrays(*[z["init_" + n] for n in L._FIELDS]), z
+
+
+def golden_state(z, init, s):
+    """Full batch at checkpoint s of a recorded reference run."""
+    b = init.copy()
+    b.body_d, b.body_i = z["s%d_body_d" % s].copy(), z["s%d_body_i" % s].copy()
+    b.fix_d = z["s%d_fix_d" % s].copy()
+    b.con_d, b.con_i = z["s%d_con_d" % s].copy(), z["s%d_con_i" % s].copy()
+    b.move = z["s%d_move" % s].copy()
+    b.con_off = np.array([0, b.con_d.shape[0]], np.int32)
+    b.move_off = np.array([0, b.move.shape[0]], np.int32)
+    b.world_d[0, L.WD_INV_DT0] = 1.0 / DT
+    b.world_i[0, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    return b
+
+
+def gpu_step(batch, prec, n=1, graph=False):
+    e = N.Engine(prec)
+    e.upload(batch)
+    if n == 1 and not graph:
+        e.step(DT, 8, 3)
+    else:
+        e.step_n(n, DT, 8, 3, graph)
+    st = e.download()
+    e.close()
+    return st
+
+
+def oracle_step(batch, n=1):
+    o = O.Oracle(batch)
+    for _ in range(n):
+        assert o.step(DT, 8, 3) == 0
+    return o.state()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_narrowphase_seam_matches_oracle(prec):
+    z = np.load(os.path.join(GOLDEN, "narrowphase.npz"))
+    cd, ci = N.narrowphase(prec, z["fa_d"], z["fa_i"], z["fb_d"], z["fb_i"], z["xfa"], z["xfb"])
+    n = cd.shape[0]
+    print("narrowphase case count", n, prec, flush=True)
+    ints_ok = np.all(ci[:, [L.CI_MTYPE, L.CI_PC, L.CI_ID0, L.CI_ID1]] ==
+                     z["con_i"][:, [L.CI_MTYPE, L.CI_PC, L.CI_ID0, L.CI_ID1]], axis=1)
+    pc_ok = ci[:, L.CI_PC] == z["con_i"][:, L.CI_PC]
+    tol = 1e-9 if prec == "fp64" else 2e-4
+    cols = [L.CD_LN_X, L.CD_LN_Y, L.CD_LP_X, L.CD_LP_Y]
+    worst = 0.0
+    for k in np.nonzero(ints_ok & (ci[:, L.CI_PC] > 0))[0]:
+        pc = ci[k, L.CI_PC]
+        c = cols + [L.CD_P0_X, L.CD_P0_Y] + ([L.CD_P1_X, L.CD_P1_Y] if pc == 2 else [])
+        worst = max(worst, float(np.max(np.abs(cd[k, c] - z["con_d"][k, c]))))
+    print(prec, "int mismatches", int((~ints_ok).sum()), "pc mismatches", int((~pc_ok).sum()),
+          "worst float diff", worst, flush=True)
+    if prec == "fp64":
+        assert ints_ok.all(), np.nonzero(~ints_ok)[0][:10]
+    else:
+        assert ints_ok.mean() >= 0.98
+    assert worst <= tol
+
+
+def test_pair_seam_matches_oracle_exactly():
+    rng = np.random.default_rng(5)
+    for trial in range(6):
+        n = int(rng.integers(50, 400))
+        lo = rng.uniform(-20, 20, (n, 2))
+        if trial % 2:  # snap to a grid so boxes touch exactly (inclusive overlap test)
+            lo = np.round(lo * 2) / 2
+        size = rng.uniform(0.2, 2.0, (n, 2)) if trial % 2 == 0 else np.full((n, 2), 0.5)
+        fat = np.concatenate([lo, lo + size], axis=1)
+        move = rng.permutation(n)[: int(rng.integers(1, n))].astype(np.int32)
+        want = O.pairs(fat, move)
+        got = N.pairs(fat, move)
+        print("pairs trial", trial, "n", n, "moves", len(move), "pairs", len(want), flush=True)
+        assert got.shape == want.shape
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
+@pytest.mark.parametrize("run", ["pyramid_600.npz", "stacks_600.npz", "mixed24_300.npz"])
+def test_one_step_from_reference_states(run, prec, tol):
+    init, z = load_init(run)
+    for s in [int(x) for x in z["checkpoints"]][::3]:
+        b = golden_state(z, init, s)
+        print("one step", run, "from checkpoint", s, prec, flush=True)
+        ref = oracle_step(b)
+        got = gpu_step(b, prec)
+        assert PU.same_contact_lists(got, ref), (run, s)
+        be = PU.body_errors(got, ref)
+        ce = PU.contact_errors(got, ref)
+        print("  body", be, "contacts", ce, flush=True)
+        for q in ("position", "angle", "velocity", "angular_velocity"):
+            assert be[q] <= tol, (run, s, q, be[q])
+        assert np.array_equal(got.body_i[:, :2], ref.body_i[:, :2]), (run, s)
+
+
+def test_first_step_from_construction_fp64():
+    """Start from the freshly built world (new-fixture pair search on the device)."""
+    for run in ["pyramid_600.npz", "stacks_600.npz", "mixed24_300.npz"]:
+        init, z = load_init(run)
+        ref = oracle_step(init, 3)
+        got = gpu_step(init, "fp64", 3)
+        assert PU.same_contact_lists(got, ref), run
+        be = PU.body_errors(got, ref)
+        print(run, "3 steps from construction", be, flush=True)
+        assert max(be.values()) <= 1e-10
+
+
+def test_pyramid_trajectory_fp64_tracks_oracle():
+    init, z = load_init("pyramid_600.npz")
+    o = O.Oracle(init)
+    e = N.Engine("fp64")
+    e.upload(init)
+    for s in range(1, 601):
+        assert o.step(DT, 8, 3) == 0
+        e.step(DT, 8, 3)
+        if s in (10, 100, 300, 600):
+            ref, got = o.state(), e.download()
+            be = PU.body_errors(got, ref)
+            print("pyramid fp64 step", s, be, flush=True)
+            assert be["position"] <= 1e-6, (s, be)
+    e.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batched_mixed_worlds_one_step(prec):
+    rng = np.random.default_rng(3)
+    worlds = [scenes.mixed_world(rng, int(n), half_width=10.0, height=15.0) for n in (20, 45, 33, 60, 12, 80)]
+    batch = builder.scenes_batch(worlds)
+    settled = oracle_step(batch, 40)  # falling and colliding by now
+    settled.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    ref = oracle_step(settled)
+    got = gpu_step(settled, prec)
+    assert PU.same_contact_lists(got, ref)
+    be = PU.body_errors(got, ref)
+    ce = PU.contact_errors(got, ref)
+    print(prec, "mixed batch one step", be, ce, flush=True)
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    for q in ("position", "angle", "velocity", "angular_velocity"):
+        assert be[q] <= tol, (q, be[q])
+
+
+def test_graph_replay_matches_eager():
+    rng = np.random.default_rng(4)
+    batch = builder.scenes_batch([scenes.mixed_world(rng, n, half_width=10.0, height=15.0) for n in (30, 50, 40)])
+    a, b = N.Engine("fp32"), N.Engine("fp32")
+    a.upload(batch)
+    b.upload(batch)
+    a.step(DT, 8, 3)
+    b.step(DT, 8, 3)  # the new-fixture step runs eagerly in both
+    for r in range(5):
+        for _ in range(4):
+            a.step(DT, 8, 3)
+        b.step_n(4, DT, 8, 3, graph=True)
+        sa, sb = a.download(), b.download()
+        print("graph replay round", r, flush=True)
+        assert np.array_equal(sa.body_d, sb.body_d)
+        assert np.array_equal(sa.con_d, sb.con_d) and np.array_equal(sa.con_i, sb.con_i)
+    a.close()
+    b.close()
