@@ -1,0 +1,215 @@
+#!/usr/bin/env python
+"""Benchmark of the per-step contact pipeline on one MI355X.
+
+Workload (default): 4096 independent mixed worlds (body counts uniform in
+[32, 512], seed 1; boxes / circles / convex polygons 40/30/30 above a static
+edge), dt = 1/60, 8 velocity and 3 position iterations, HIP-graph-captured
+steps, continuous physics off.  A step is one World.step of every world.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 10]
+
+Prints one JSON line: value = dynamic bodies x timed steps / wall seconds of
+the timed loop (FP32-bulk pipeline, state resident in HBM), plus the FP64
+pipeline's rate, the dominant kernel's roofline, a one-step error against the
+FP64 oracle on a sample of worlds, and the oracle's CPU rate on a bounded
+sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+from pypybox2d_amd import builder, layout as L, scenes  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+KERNELS = ("k_collide", "k_islands", "k_solve", "k_sync", "k_pairs")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_counts(state: L.BatchArrays):
+    """Per-step unit counts of a state: bodies, fixtures, live and touching contacts."""
+    bi = state.body_i
+    ci = state.con_i
+    dyn = int(np.count_nonzero(bi[:, L.BI_TYPE] == L.DYNAMIC))
+    touching = int(np.count_nonzero(ci[:, L.CI_FLAGS] & L.CF_TOUCHING))
+    points = int(ci[(ci[:, L.CI_FLAGS] & L.CF_TOUCHING) != 0, L.CI_PC].sum())
+    island_bodies = int(np.count_nonzero(bi[:, L.BI_ISLAND] >= 0))
+    nverts = state.fix_i[:, L.FI_NVERTS]
+    shape = state.fix_i[:, L.FI_SHAPE]
+    geo = np.where(shape == L.SHAPE_POLYGON, 16 * nverts + 4, np.where(shape == L.SHAPE_CIRCLE, 12, 20))
+    return {"bodies": int(bi.shape[0]), "dynamic_bodies": dyn, "fixtures": int(state.fix_i.shape[0]),
+            "contacts": int(ci.shape[0]), "touching": touching, "points": points,
+            "island_bodies": island_bodies, "geometry_bytes": int(geo.sum())}
+
+
+def algorithmic_bytes(kernel: str, c: dict, fp64: bool) -> float:
+    """Minimum HBM bytes one launch of `kernel` must move (per-unit figures in DESIGN.md)."""
+    t = 8 if fp64 else 4
+    man = 4 + 4 * 2 * t + 4 * t + 8      # state word + ln,lp,p0,p1 + 4 impulses + 2 ids
+    if kernel == "k_collide":
+        # per contact: pair + manifold read + write; per body: xf and fat AABB and geometry once
+        return c["contacts"] * (8 + 2 * man) + c["dynamic_bodies"] * (32 + 32) + c["geometry_bytes"]
+    if kernel == "k_solve":
+        # per island body: c, a, v, w read + write, c0/a0 write, constants; per touching contact:
+        # pair + manifold read, impulses write
+        return c["island_bodies"] * (48 + 48 + 24 + 4 * t + 16) + c["touching"] * (8 + man + 4 * t)
+    if kernel == "k_islands":
+        return c["contacts"] * (8 + 4) + c["bodies"] * 8 + c["touching"] * 8
+    if kernel == "k_sync":
+        return c["fixtures"] * (32 + 32) + c["island_bodies"] * (56 + 32) + c["geometry_bytes"]
+    if kernel == "k_pairs":
+        return c["contacts"] * 12 + c["fixtures"] * 4
+    return 0.0
+
+
+def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, profile_steps: int):
+    from pypybox2d_amd._native import Engine
+    eng = Engine(precision)
+    eng.upload(batch)
+    log("[%s] uploaded; warmup %d steps" % (precision, warmup))
+    eng.step_n(warmup, graph=True)
+    eng.sync()
+    log("[%s] timing %d graph-captured steps" % (precision, steps))
+    t0 = time.perf_counter()
+    done = 0
+    chunk = max(1, min(steps, 200))
+    while done < steps:
+        n = min(chunk, steps - done)
+        eng.step_n(n, graph=True)
+        eng.sync()
+        done += n
+        log("[%s] %d/%d steps, %.2f s" % (precision, done, steps, time.perf_counter() - t0))
+    elapsed = time.perf_counter() - t0
+    ms = eng.profile(profile_steps) if profile_steps > 0 else np.zeros(5)
+    state = eng.download()
+    eng.close()
+    return elapsed, ms, state
+
+
+def one_step_error(state: L.BatchArrays, precision: str, worlds):
+    """Max relative one-step error of the GPU pipeline vs the FP64 oracle from an identical state."""
+    from oracle import oracle as O
+    from pypybox2d_amd._native import Engine
+    sample = L.concat([state.world(w) for w in worlds])
+    sample.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    o = O.Oracle(sample)
+    o.step(1.0 / 60, 8, 3)
+    ref = o.state()
+    e = Engine(precision)
+    e.upload(sample)
+    e.step(1.0 / 60, 8, 3)
+    got = e.download()
+    e.close()
+    out = {}
+    g, r = got.body_d, ref.body_d
+    for name, cols, floor in (("position", [L.BD_C_X, L.BD_C_Y], 1.0), ("angle", [L.BD_A], 1.0),
+                              ("velocity", [L.BD_V_X, L.BD_V_Y], 10.0 / 60),
+                              ("angular_velocity", [L.BD_W], 10.0 / 60)):
+        out[name] = float(np.max(np.abs(g[:, cols] - r[:, cols]) / np.maximum(np.abs(r[:, cols]), floor)))
+    same = all(np.array_equal(got.con_i[got.con_off[w]:got.con_off[w + 1], :2],
+                              ref.con_i[ref.con_off[w]:ref.con_off[w + 1], :2]) for w in range(sample.n_worlds))
+    out["contact_lists_equal"] = bool(same)
+    out["worlds"] = len(worlds)
+    return out
+
+
+def cpu_baseline(batch: L.BatchArrays, worlds, seconds: float):
+    """The FP64 oracle stepping a sample of worlds on one host core."""
+    from oracle import oracle as O
+    sample = L.concat([batch.world(w) for w in worlds])
+    dyn = int(np.count_nonzero(sample.body_i[:, L.BI_TYPE] == L.DYNAMIC))
+    o = O.Oracle(sample)
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        o.step(1.0 / 60, 8, 3)
+        steps += 1
+    el = time.perf_counter() - t0
+    return {"value": dyn * steps / el, "unit": "body-steps/s", "cores": 1, "kind": "port",
+            "sample": "FP64 oracle (oracle/b2oracle.c), %d worlds (%d dynamic bodies) of the same batch, "
+                      "%d steps from the initial state, one host core" % (len(worlds), dyn, steps)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=None, help="timed steps of the FP64 run (default: --steps)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    world_size = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world_size != 1:
+        print("bench.py runs on one GPU only (the batch is one GPU's share; worlds would shard across GPUs "
+              "with no communication); got --gpus %d, WORLD_SIZE=%d" % (args.gpus, world_size))
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    counts = scenes.mixed_batch_counts(args.worlds, args.seed)
+    log("building %d worlds (%d dynamic bodies) on the host" % (args.worlds, int(counts.sum())))
+    batch = builder.batch_from_counts(counts, args.seed)
+    log("built in %.1f s" % (time.perf_counter() - t0))
+    dyn = int(np.count_nonzero(batch.body_i[:, L.BI_TYPE] == L.DYNAMIC))
+
+    el32, ms32, st32 = run_gpu(batch, "fp32", args.steps, args.warmup, args.profile_steps)
+    value = dyn * args.steps / el32
+    log("fp32: %.3e body-steps/s (%.3f ms/step)" % (value, 1e3 * el32 / args.steps))
+    c = workload_counts(st32)
+    k = int(np.argmax(ms32))
+    dom = KERNELS[k]
+    dom_ms = float(ms32[k])
+    ab = algorithmic_bytes(dom, c, fp64=False)
+    achieved = ab / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    line = {
+        "metric": "body-steps/sec, 4096 mixed worlds, FP32 vs FP64 on GPU; one-step err vs FP64",
+        "value": value, "unit": "body-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * el32 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 bulk (f64 positions/angles/velocities/impulses/AABBs)",
+        "data": "synthetic",
+        "config": {"workload": "mixed_batch: %d worlds, body counts U[32,512] (seed %d), boxes/circles/polygons "
+                               "40/30/30, dt 1/60, 8/3 iterations, graph-captured step" % (args.worlds, args.seed),
+                   "worlds": args.worlds, "dynamic_bodies": dyn, "contacts_at_end": c["contacts"],
+                   "touching_at_end": c["touching"]},
+        "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms32)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": ab},
+    }
+    if not args.no_fp64:
+        n64 = args.fp64_steps or args.steps
+        el64, ms64, st64 = run_gpu(batch, "fp64", n64, args.warmup, args.profile_steps)
+        v64 = dyn * n64 / el64
+        line["fp64"] = {"value": v64, "ms_per_step": 1e3 * el64 / n64, "steps": n64,
+                        "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms64)}}
+        line["fp32_over_fp64"] = value / v64
+        log("fp64: %.3e body-steps/s; fp32/fp64 = %.2f" % (v64, value / v64))
+    try:
+        rng = np.random.default_rng(0)
+        worlds = sorted(rng.choice(args.worlds, size=min(16, args.worlds), replace=False).tolist())
+        line["one_step_error_fp32_vs_fp64_oracle"] = one_step_error(st32, "fp32", worlds)
+        log("one-step error:", line["one_step_error_fp32_vs_fp64_oracle"])
+    except Exception as exc:  # the error check never hides the measurement
+        line["one_step_error_fp32_vs_fp64_oracle"] = {"error": str(exc)}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(batch, list(range(min(8, args.worlds))), args.cpu_seconds)
+        log("cpu baseline: %.3e body-steps/s" % line["cpu_baseline"]["value"])
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

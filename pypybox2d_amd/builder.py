@@ -47,3 +47,26 @@ def scene_batch(scene) -> L.BatchArrays:
 
 def scenes_batch(scenes: List[dict]) -> L.BatchArrays:
     return L.concat([scene_batch(s) for s in scenes])
+
+
+def _build_range(args):
+    seed, counts, w0 = args
+    from . import scenes
+    return scenes_batch([scenes.batch_world(seed, w0 + k, n) for k, n in enumerate(counts)])
+
+
+def batch_from_counts(counts, seed: int, procs: int = 0, chunk: int = 64) -> L.BatchArrays:
+    """Builds the worlds of a seeded batch (scenes.batch_world) in worker processes.
+
+    Call it before the process touches the GPU: the workers are forked."""
+    import multiprocessing as mp
+    import os
+    counts = [int(c) for c in counts]
+    jobs = [(seed, counts[i:i + chunk], i) for i in range(0, len(counts), chunk)]
+    procs = procs or min(16, os.cpu_count() or 1)
+    if procs <= 1 or len(jobs) == 1:
+        parts = [_build_range(j) for j in jobs]
+    else:
+        with mp.get_context("fork").Pool(procs) as pool:
+            parts = pool.map(_build_range, jobs)
+    return L.concat(parts)

@@ -84,21 +84,25 @@ def mixed_world(rng, n_bodies: int, half_width: float = 20.0, height: float = No
     """Boxes / circles / convex polygons (40/30/30) placed without overlap above an edge.
 
     Centres are rejection-sampled in [-half_width, half_width] x [1, 1 + height]
-    with non-overlapping bounding circles.  The default region is 40 x 40 and
-    grows in height for large worlds so that the bounding circles cover at
-    most ~40% of it (sequential rejection sampling jams near 55%).
+    with non-overlapping bounding circles (0.05 m apart), largest bodies
+    first.  The default region is 40 x 40 and grows in height for large worlds
+    so that the bounding circles cover at most ~40% of it.  Each body then gets
+    a uniform random angle, v ~ N(0, 1) per axis and w ~ N(0, 0.5).
     """
     if height is None:
         height = max(40.0, 4.7 * n_bodies / (2.0 * half_width))
-    bodies = [_body("static", (0.0, 0.0), [_fix(("edge", (-40.0, 0.0), (40.0, 0.0)))])]
-    cell = 3.0
-    grid = {}
-    placed = []
+    shapes = []
     for _ in range(n_bodies):
         shape, R = _random_shape(rng)
         density = float(rng.uniform(0.5, 2.0))
         friction = float(rng.uniform(0.2, 0.8))
-        for _try in range(10000):
+        shapes.append((shape, R, density, friction))
+    cell = 3.0
+    grid = {}
+    pos = [None] * n_bodies
+    for i in sorted(range(n_bodies), key=lambda k: -shapes[k][1]):
+        R = shapes[i][1]
+        for _try in range(100000):
             x = float(rng.uniform(-half_width, half_width))
             y = float(rng.uniform(1.0, 1.0 + height))
             if y - R < 0.05:
@@ -120,28 +124,45 @@ def mixed_world(rng, n_bodies: int, half_width: float = 20.0, height: float = No
         else:
             raise RuntimeError("could not place body")
         grid.setdefault((gx, gy), []).append((x, y, R))
-        placed.append((x, y, R))
+        pos[i] = (x, y)
+    bodies = [_body("static", (0.0, 0.0), [_fix(("edge", (-40.0, 0.0), (40.0, 0.0)))])]
+    for i, (shape, R, density, friction) in enumerate(shapes):
         v = rng.normal(0.0, 1.0, 2)
         w = float(rng.normal(0.0, 0.5))
         angle = float(rng.uniform(-np.pi, np.pi))
-        bodies.append(_body("dynamic", (x, y), [_fix(shape, density=density, friction=friction)],
+        bodies.append(_body("dynamic", pos[i], [_fix(shape, density=density, friction=friction)],
                             angle=angle, linear_velocity=(float(v[0]), float(v[1])), angular_velocity=w))
     return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies}
 
 
+def mixed_batch_counts(n_worlds: int = 4096, seed: int = 1, lo: int = 32, hi: int = 512) -> np.ndarray:
+    """Body counts of the uniform batch: uniform integers in [lo, hi]."""
+    return np.random.default_rng(seed).integers(lo, hi + 1, n_worlds)
+
+
+def lognormal_batch_counts(n_worlds: int = 16384, seed: int = 2, median: float = 64.0, sigma: float = 0.75,
+                           lo: int = 16, hi: int = 1024) -> np.ndarray:
+    """Body counts clip(round(median * exp(sigma N(0,1))), lo, hi)."""
+    rng = np.random.default_rng(seed)
+    return np.clip(np.round(median * np.exp(sigma * rng.normal(size=n_worlds))), lo, hi).astype(int)
+
+
+def batch_world(seed: int, w: int, n_bodies: int) -> dict:
+    """World w of a batch: its own random stream, seeded by (seed, w)."""
+    return mixed_world(np.random.default_rng([seed, w]), int(n_bodies))
+
+
 def mixed_batch(n_worlds: int = 4096, seed: int = 1, lo: int = 32, hi: int = 512) -> List[dict]:
     """Independent mixed worlds with body counts uniform in [lo, hi]."""
-    rng = np.random.default_rng(seed)
-    counts = rng.integers(lo, hi + 1, n_worlds)
-    return [mixed_world(rng, int(n)) for n in counts]
+    counts = mixed_batch_counts(n_worlds, seed, lo, hi)
+    return [batch_world(seed, w, n) for w, n in enumerate(counts)]
 
 
 def lognormal_batch(n_worlds: int = 16384, seed: int = 2, median: float = 64.0, sigma: float = 0.75,
                     lo: int = 16, hi: int = 1024) -> List[dict]:
-    """Mixed worlds with body counts clip(round(median * exp(sigma N(0,1))), lo, hi)."""
-    rng = np.random.default_rng(seed)
-    counts = np.clip(np.round(median * np.exp(sigma * rng.normal(size=n_worlds))), lo, hi).astype(int)
-    return [mixed_world(rng, int(n)) for n in counts]
+    """Mixed worlds with lognormal body counts (see lognormal_batch_counts)."""
+    counts = lognormal_batch_counts(n_worlds, seed, median, sigma, lo, hi)
+    return [batch_world(seed, w, n) for w, n in enumerate(counts)]
 
 
 def pyramids(count: int = 200, spacing: float = 30.0):
