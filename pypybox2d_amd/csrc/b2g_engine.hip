@@ -59,6 +59,13 @@ enum : int { WF_ALLOW_SLEEP = 1, WF_WARM = 2, WF_NEW_FIXTURE = 4, WF_KEEP_FORCES
 enum : int { WERR_CAPACITY = 1, WERR_UNSUPPORTED = 2, WERR_NEWPAIRS = 4 };
 
 constexpr int kBlock = 256;
+constexpr int kSizeClasses = 16;
+
+__host__ __device__ inline int size_class(int nc) {  // floor(log2(nc + 1)), capped
+  int c = 0;
+  while ((2 << c) <= nc + 1 && c < kSizeClasses - 1) ++c;
+  return c;
+}
 
 template <typename T>
 struct V4 {
@@ -92,6 +99,7 @@ struct PC {
   T lp0x, lp0y, lp1x, lp1y, lnx, lny, lpx, lpy;
   T ra, rb, ma, mb, Ia, Ib, lcax, lcay, lcbx, lcby;
   int type, npts, ia, ib;
+  double scx, scy, sa;  // position of the contact's static body (static bodies never move)
 };
 
 template <typename T>
@@ -158,6 +166,9 @@ struct Dev {
   int4* q_isl;       // global island queue: (world, bstart, nb, cstart)
   int* q_nc;         // (nc | last_allow << 30)
   int* q_count;
+  int4* qs_isl;      // the queue ordered by island size, largest first
+  int* qs_nc;
+  int* q_cls;        // [2 * kSizeClasses]: per-class counts, then cursors
   // solver scratch
   PV<T>* pv;   // indexed like i_bodies
   VC<T>* vc;   // indexed like i_contacts
@@ -506,6 +517,7 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
     int4 in = info[k];
     d.q_isl[qbase + k] = make_int4(w, b0 + in.x, in.y, cb + in.z);
     d.q_nc[qbase + k] = in.w;
+    atomicAdd(&d.q_cls[size_class(in.w & 0x3FFFFFFF)], 1);
   }
   for (int k = tid; k < nci_tot; k += kBlock) {
     const int s = isl_c[k];
@@ -647,11 +659,10 @@ __device__ __forceinline__ void apply_block(VC<T>& vc, V2d x, V2d a, V2d normal,
 
 // One contact of solve_velocity_constraints (contact.py:527-799).
 template <typename T>
-__device__ __forceinline__ void solve_contact_velocity(VC<T>& vc, V2d& va, double& wa, V2d& vb, double& wb) {
-  const V2d normal = dv2(vc.nx, vc.ny), tangent = cross_vs(normal, 1.0);
+__device__ __forceinline__ void tangent_point(const VC<T>& vc, VCP<T>& p, V2d tangent, V2d& va, double& wa, V2d& vb,
+                                              double& wb) {
   const double ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
-  for (int j = 0; j < vc.npts; ++j) {  // tangent constraints first
-    VCP<T>& p = vc.p[j];
+  {
     V2d ra = dv2(p.rax, p.ray), rb = dv2(p.rbx, p.rby);
     V2d dv = vb + cross_sv(wb, rb) - va - cross_sv(wa, ra);
     double vt = dot(dv, tangent);
@@ -666,6 +677,15 @@ __device__ __forceinline__ void solve_contact_velocity(VC<T>& vc, V2d& va, doubl
     wb += Ib * cross(rb, P);
     p.ti = newi;
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void solve_contact_velocity(VC<T>& vc, V2d& va, double& wa, V2d& vb, double& wb) {
+  const V2d normal = dv2(vc.nx, vc.ny), tangent = cross_vs(normal, 1.0);
+  const double ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
+  // tangent constraints first (points in order), then normal
+  tangent_point(vc, vc.p[0], tangent, va, wa, vb, wb);
+  if (vc.npts == 2) tangent_point(vc, vc.p[1], tangent, va, wa, vb, wb);
   if (vc.npts == 1) {
     VCP<T>& p = vc.p[0];
     V2d ra = dv2(p.rax, p.ray), rb = dv2(p.rbx, p.rby);
@@ -714,6 +734,206 @@ __device__ __forceinline__ void solve_contact_velocity(VC<T>& vc, V2d& va, doubl
       }
     }
   }
+}
+
+
+// ---- software-pipelined Gauss-Seidel sweeps ----
+// Each lane walks its island's contacts in order.  While contact k is being
+// solved, the record of contact k+1 and the velocities (positions) of its
+// bodies that contact k does not touch are already in flight; bodies shared
+// with contact k are forwarded from registers.  This keeps the exact
+// sequential order and hides most of the memory latency of the chain.
+template <typename T>
+__device__ __forceinline__ void vel_of(const Dev<T>& d, int bstart, int i, V2d& v, double& w) {
+  if (i >= 0) {
+    const PV<T>* p = &d.pv[bstart + i];
+    v = v2<double>(p->vx, p->vy);
+    w = p->w;
+  } else {  // static bodies have zero velocity (checked at upload)
+    v = v2<double>(0.0, 0.0);
+    w = 0.0;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void warm_point(const VC<T>& vc, const VCP<T>& p, V2d normal, V2d tangent, V2d& va,
+                                           double& wa, V2d& vb, double& wb) {
+  V2d P = scale(p.ni, normal) + scale(p.ti, tangent);
+  wa -= (double)vc.Ia * cross(dv2(p.rax, p.ray), P);
+  va = va - scale((double)vc.ma, P);
+  wb += (double)vc.Ib * cross(dv2(p.rbx, p.rby), P);
+  vb = vb + scale((double)vc.mb, P);
+}
+
+template <typename T>
+__device__ __forceinline__ void warm_contact(const VC<T>& vc, V2d& va, double& wa, V2d& vb, double& wb) {
+  V2d normal = dv2(vc.nx, vc.ny), tangent = cross_vs(normal, 1.0);
+  if (vc.npts >= 1) warm_point(vc, vc.p[0], normal, tangent, va, wa, vb, wb);
+  if (vc.npts == 2) warm_point(vc, vc.p[1], normal, tangent, va, wa, vb, wb);
+}
+
+template <typename T, bool kWarm>
+__device__ void velocity_sweep(const Dev<T>& d, int bstart, int cstart, int nci) {
+  if (nci <= 0) return;
+  VC<T> A = d.vc[cstart];
+  V2d va, vb;
+  double wa, wb;
+  vel_of(d, bstart, A.ia, va, wa);
+  vel_of(d, bstart, A.ib, vb, wb);
+  int2 nidx = nci > 1 ? d.i_cidx[cstart + 1] : make_int2(0, 0);
+  for (int k = 0; k < nci; ++k) {
+    const bool more = k + 1 < nci;
+    VC<T> B;
+    int2 idx2 = make_int2(0, 0);
+    V2d pva = v2<double>(0, 0), pvb = v2<double>(0, 0);
+    double pwa = 0.0, pwb = 0.0;
+    if (more) {
+      B = d.vc[cstart + k + 1];
+      if (k + 2 < nci) idx2 = d.i_cidx[cstart + k + 2];
+      if (nidx.x != A.ia && nidx.x != A.ib) vel_of(d, bstart, nidx.x, pva, pwa);
+      if (nidx.y != A.ia && nidx.y != A.ib) vel_of(d, bstart, nidx.y, pvb, pwb);
+    }
+    if (kWarm) warm_contact(A, va, wa, vb, wb);
+    else solve_contact_velocity(A, va, wa, vb, wb);
+    store_vel(d, bstart, A.ia, va, wa);
+    store_vel(d, bstart, A.ib, vb, wb);
+    if (!kWarm) {
+      VCP<T>* dst = d.vc[cstart + k].p;
+      dst[0].ni = A.p[0].ni;
+      dst[0].ti = A.p[0].ti;
+      if (A.npts == 2) {
+        dst[1].ni = A.p[1].ni;
+        dst[1].ti = A.p[1].ti;
+      }
+    }
+    if (more) {
+      V2d nva, nvb;
+      double nwa, nwb;
+      if (nidx.x == A.ia) { nva = va; nwa = wa; }
+      else if (nidx.x == A.ib) { nva = vb; nwa = wb; }
+      else { nva = pva; nwa = pwa; }
+      if (nidx.y == A.ia) { nvb = va; nwb = wa; }
+      else if (nidx.y == A.ib) { nvb = vb; nwb = wb; }
+      else { nvb = pvb; nwb = pwb; }
+      A = B;
+      va = nva; wa = nwa; vb = nvb; wb = nwb;
+      nidx = idx2;
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void pos_of(const Dev<T>& d, int bstart, int i, const PC<T>& pc, double& x, double& y,
+                                       double& a) {
+  if (i >= 0) {
+    const PV<T>* p = &d.pv[bstart + i];
+    x = p->cx;
+    y = p->cy;
+    a = p->a;
+  } else {
+    x = pc.scx;
+    y = pc.scy;
+    a = pc.sa;
+  }
+}
+
+// One contact of solve_position_constraints (contact.py:809-864).
+template <typename T>
+__device__ __forceinline__ void solve_contact_position(const PC<T>& pc, double& cax, double& cay, double& aa,
+                                                       double& cbx, double& cby, double& ab, T& min_sep) {
+  for (int j = 0; j < pc.npts; ++j) {
+    Rot<T> qa = rot_of<T>(aa), qb = rot_of<T>(ab);
+    V2<T> dcb = v2<T>((T)(cbx - cax), (T)(cby - cay));
+    XF<T> xfa, xfb;
+    xfa.q = qa;
+    xfa.p = -rmul(qa, v2<T>(pc.lcax, pc.lcay));
+    xfb.q = qb;
+    xfb.p = dcb - rmul(qb, v2<T>(pc.lcbx, pc.lcby));
+    V2<T> normal, point;
+    T sep;
+    V2<T> lpj = j == 0 ? v2<T>(pc.lp0x, pc.lp0y) : v2<T>(pc.lp1x, pc.lp1y);
+    V2<T> ln = v2<T>(pc.lnx, pc.lny), lpp = v2<T>(pc.lpx, pc.lpy);
+    if (pc.type == M_CIRCLES) {
+      V2<T> pa = xmul(xfa, lpp), pb = xmul(xfb, v2<T>(pc.lp0x, pc.lp0y));
+      normal = normalized(pb - pa);
+      point = scale((T)0.5, pa + pb);
+      sep = dot(pb - pa, normal) - pc.ra - pc.rb;
+    } else if (pc.type == M_FACE_A) {
+      normal = rmul(qa, ln);
+      V2<T> plane = xmul(xfa, lpp);
+      V2<T> clip = xmul(xfb, lpj);
+      sep = dot(clip - plane, normal) - pc.ra - pc.rb;
+      point = clip;
+    } else {
+      normal = rmul(qb, ln);
+      V2<T> plane = xmul(xfb, lpp);
+      V2<T> clip = xmul(xfa, lpj);
+      sep = dot(clip - plane, normal) - pc.ra - pc.rb;
+      point = clip;
+      normal = -normal;
+    }
+    V2<T> ra = point, rb = point - dcb;
+    min_sep = pmin(min_sep, sep);
+    T C = pclamp((T)BAUMGARTE * (sep + (T)LINEAR_SLOP), -(T)MAX_LINEAR_CORRECTION, (T)0);
+    T rna = sq(cross(ra, normal)), rnb = sq(cross(rb, normal));
+    T K = pc.ma + pc.mb + (pc.Ia * rna) + (pc.Ib * rnb);
+    T impulse = K > (T)0 ? -C / K : (T)0;
+    V2<T> P = scale(impulse, normal);
+    cax -= (double)(pc.ma * P.x);
+    cay -= (double)(pc.ma * P.y);
+    aa -= (double)(pc.Ia * cross(ra, P));
+    cbx += (double)(pc.mb * P.x);
+    cby += (double)(pc.mb * P.y);
+    ab += (double)(pc.Ib * cross(rb, P));
+  }
+}
+
+template <typename T>
+__device__ bool position_sweep(const Dev<T>& d, int bstart, int cstart, int nci) {
+  T min_sep = 0;
+  if (nci > 0) {
+    PC<T> A = d.pc[cstart];
+    double ax, ay, aa, bx, by, ab;
+    pos_of(d, bstart, A.ia, A, ax, ay, aa);
+    pos_of(d, bstart, A.ib, A, bx, by, ab);
+    int2 nidx = nci > 1 ? d.i_cidx[cstart + 1] : make_int2(0, 0);
+    for (int k = 0; k < nci; ++k) {
+      const bool more = k + 1 < nci;
+      PC<T> B;
+      int2 idx2 = make_int2(0, 0);
+      double p0x = 0, p0y = 0, p0a = 0, p1x = 0, p1y = 0, p1a = 0;
+      if (more) {
+        B = d.pc[cstart + k + 1];
+        if (k + 2 < nci) idx2 = d.i_cidx[cstart + k + 2];
+        if (nidx.x >= 0 && nidx.x != A.ia && nidx.x != A.ib) {
+          const PV<T>* p = &d.pv[bstart + nidx.x];
+          p0x = p->cx; p0y = p->cy; p0a = p->a;
+        }
+        if (nidx.y >= 0 && nidx.y != A.ia && nidx.y != A.ib) {
+          const PV<T>* p = &d.pv[bstart + nidx.y];
+          p1x = p->cx; p1y = p->cy; p1a = p->a;
+        }
+      }
+      solve_contact_position(A, ax, ay, aa, bx, by, ab, min_sep);
+      store_pos(d, bstart, A.ia, ax, ay, aa);
+      store_pos(d, bstart, A.ib, bx, by, ab);
+      if (more) {
+        double nax, nay, naa, nbx, nby, nab;
+        if (nidx.x < 0) { nax = B.scx; nay = B.scy; naa = B.sa; }
+        else if (nidx.x == A.ia) { nax = ax; nay = ay; naa = aa; }
+        else if (nidx.x == A.ib) { nax = bx; nay = by; naa = ab; }
+        else { nax = p0x; nay = p0y; naa = p0a; }
+        if (nidx.y < 0) { nbx = B.scx; nby = B.scy; nab = B.sa; }
+        else if (nidx.y == A.ia) { nbx = ax; nby = ay; nab = aa; }
+        else if (nidx.y == A.ib) { nbx = bx; nby = by; nab = ab; }
+        else { nbx = p1x; nby = p1y; nab = p1a; }
+        A = B;
+        ax = nax; ay = nay; aa = naa; bx = nbx; by = nby; ab = nab;
+        nidx = idx2;
+      }
+    }
+  }
+  return min_sep >= -(T)(3.0 * LINEAR_SLOP);
 }
 
 template <typename T>
@@ -795,6 +1015,10 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
     double cax, cay, aa, cbx, cby, ab;
     load_pos(d, bstart, b0, ia, cax, cay, aa);
     load_pos(d, bstart, b0, ib, cbx, cby, ab);
+    pcn.scx = ia < 0 ? cax : cbx;
+    pcn.scy = ia < 0 ? cay : cby;
+    pcn.sa = ia < 0 ? aa : ab;
+    d.i_cidx[cstart + k] = make_int2(ia, ib);  // island-local from here on
     V2d va, vb;
     double wa, wb;
     load_vel(d, bstart, b0, ia, va, wa);
@@ -813,10 +1037,7 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
     vc.ny = normal.y;
     const T ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
     const V2d nd = dv2(normal.x, normal.y);
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      if (j >= pc) break;
-      V2<T> wp = j == 0 ? w0 : w1;
+    auto init_point = [&](VCP<T>& p, V2<T> wp) {
       V2<T> ra = wp, rb = wp - dcb;
       T rna = sq(cross(ra, normal)), rnb = sq(cross(rb, normal));
       T kn = ma + mb + Ia * rna + Ib * rnb;
@@ -824,13 +1045,16 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
       T rta = sq(cross(ra, tangent)), rtb = sq(cross(rb, tangent));
       T kt = ma + mb + Ia * rta + Ib * rtb;
       double vrel = dot(nd, vb + cross_sv(wb, dv2(rb.x, rb.y)) - va - cross_sv(wa, dv2(ra.x, ra.y)));
-      VCP<T>& p = vc.p[j];
       p.rax = ra.x; p.ray = ra.y; p.rbx = rb.x; p.rby = rb.y;
       p.nm = kn > (T)0 ? (T)1 / kn : (T)0;
       p.tm = kt > (T)0 ? (T)1 / kt : (T)0;
       p.bias = vrel < -VELOCITY_THRESHOLD ? (T)(-(double)vc.restitution * vrel) : (T)0;
       p.pad = 0;
-    }
+    };
+    vc.p[1].rax = vc.p[1].ray = vc.p[1].rbx = vc.p[1].rby = (T)0;
+    vc.p[1].nm = vc.p[1].tm = vc.p[1].bias = vc.p[1].pad = (T)0;
+    if (pc >= 1) init_point(vc.p[0], w0);
+    if (pc >= 2) init_point(vc.p[1], w1);
     vc.k11 = 1; vc.k12 = 0; vc.k22 = 1; vc.n11 = 1; vc.n12 = 0; vc.n21 = 0; vc.n22 = 1;
     if (pc == 2) {
       V2d ra1 = dv2(vc.p[0].rax, vc.p[0].ray), rb1 = dv2(vc.p[0].rbx, vc.p[0].rby);
@@ -853,45 +1077,9 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
     d.vc[cstart + k] = vc;
     d.pc[cstart + k] = pcn;
   }
-  // 3. warm_start (contact.py:498-525)
-  if (warm) {
-    for (int k = 0; k < nci; ++k) {
-      VC<T> vc = d.vc[cstart + k];
-      V2d normal = dv2(vc.nx, vc.ny), tangent = cross_vs(normal, 1.0);
-      V2d va, vb;
-      double wa, wb;
-      load_vel(d, bstart, b0, vc.ia, va, wa);
-      load_vel(d, bstart, b0, vc.ib, vb, wb);
-      for (int j = 0; j < vc.npts; ++j) {
-        const VCP<T>& p = vc.p[j];
-        V2d P = scale(p.ni, normal) + scale(p.ti, tangent);
-        wa -= (double)vc.Ia * cross(dv2(p.rax, p.ray), P);
-        va = va - scale((double)vc.ma, P);
-        wb += (double)vc.Ib * cross(dv2(p.rbx, p.rby), P);
-        vb = vb + scale((double)vc.mb, P);
-      }
-      store_vel(d, bstart, vc.ia, va, wa);
-      store_vel(d, bstart, vc.ib, vb, wb);
-    }
-  }
-  // 4. velocity iterations
-  for (int it = 0; it < d.vel_iters; ++it) {
-    for (int k = 0; k < nci; ++k) {
-      VC<T> vc = d.vc[cstart + k];
-      V2d va, vb;
-      double wa, wb;
-      load_vel(d, bstart, b0, vc.ia, va, wa);
-      load_vel(d, bstart, b0, vc.ib, vb, wb);
-      solve_contact_velocity(vc, va, wa, vb, wb);
-      store_vel(d, bstart, vc.ia, va, wa);
-      store_vel(d, bstart, vc.ib, vb, wb);
-      VCP<T>* dst = d.vc[cstart + k].p;
-      dst[0].ni = vc.p[0].ni;
-      dst[0].ti = vc.p[0].ti;
-      dst[1].ni = vc.p[1].ni;
-      dst[1].ti = vc.p[1].ti;
-    }
-  }
+  // 3. warm_start (contact.py:498-525) and 4. velocity iterations (contact.py:527-799)
+  if (warm) velocity_sweep<T, true>(d, bstart, cstart, nci);
+  for (int it = 0; it < d.vel_iters; ++it) velocity_sweep<T, false>(d, bstart, cstart, nci);
   // 5. store_impulses (contact.py:801-806): zip(manifold points, constraint points)
   for (int k = 0; k < nci; ++k) {
     const VC<T>& vc = d.vc[cstart + k];
@@ -930,61 +1118,7 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
   // 7. position iterations with early exit (contact.py:809-864)
   bool position_solved = false;
   for (int it = 0; it < d.pos_iters; ++it) {
-    T min_sep = 0;
-    for (int k = 0; k < nci; ++k) {
-      const PC<T> pc = d.pc[cstart + k];
-      double cax, cay, aa, cbx, cby, ab;
-      load_pos(d, bstart, b0, pc.ia, cax, cay, aa);
-      load_pos(d, bstart, b0, pc.ib, cbx, cby, ab);
-      for (int j = 0; j < pc.npts; ++j) {
-        Rot<T> qa = rot_of<T>(aa), qb = rot_of<T>(ab);
-        V2<T> dcb = v2<T>((T)(cbx - cax), (T)(cby - cay));
-        XF<T> xfa, xfb;
-        xfa.q = qa;
-        xfa.p = -rmul(qa, v2<T>(pc.lcax, pc.lcay));
-        xfb.q = qb;
-        xfb.p = dcb - rmul(qb, v2<T>(pc.lcbx, pc.lcby));
-        V2<T> normal, point;
-        T sep;
-        V2<T> lpj = j == 0 ? v2<T>(pc.lp0x, pc.lp0y) : v2<T>(pc.lp1x, pc.lp1y);
-        V2<T> ln = v2<T>(pc.lnx, pc.lny), lpp = v2<T>(pc.lpx, pc.lpy);
-        if (pc.type == M_CIRCLES) {
-          V2<T> pa = xmul(xfa, lpp), pb = xmul(xfb, v2<T>(pc.lp0x, pc.lp0y));
-          normal = normalized(pb - pa);
-          point = scale((T)0.5, pa + pb);
-          sep = dot(pb - pa, normal) - pc.ra - pc.rb;
-        } else if (pc.type == M_FACE_A) {
-          normal = rmul(qa, ln);
-          V2<T> plane = xmul(xfa, lpp);
-          V2<T> clip = xmul(xfb, lpj);
-          sep = dot(clip - plane, normal) - pc.ra - pc.rb;
-          point = clip;
-        } else {
-          normal = rmul(qb, ln);
-          V2<T> plane = xmul(xfb, lpp);
-          V2<T> clip = xmul(xfa, lpj);
-          sep = dot(clip - plane, normal) - pc.ra - pc.rb;
-          point = clip;
-          normal = -normal;
-        }
-        V2<T> ra = point, rb = point - dcb;
-        min_sep = pmin(min_sep, sep);
-        T C = pclamp((T)BAUMGARTE * (sep + (T)LINEAR_SLOP), -(T)MAX_LINEAR_CORRECTION, (T)0);
-        T rna = sq(cross(ra, normal)), rnb = sq(cross(rb, normal));
-        T K = pc.ma + pc.mb + (pc.Ia * rna) + (pc.Ib * rnb);
-        T impulse = K > (T)0 ? -C / K : (T)0;
-        V2<T> P = scale(impulse, normal);
-        cax -= (double)(pc.ma * P.x);
-        cay -= (double)(pc.ma * P.y);
-        aa -= (double)(pc.Ia * cross(ra, P));
-        cbx += (double)(pc.mb * P.x);
-        cby += (double)(pc.mb * P.y);
-        ab += (double)(pc.Ib * cross(rb, P));
-      }
-      store_pos(d, bstart, pc.ia, cax, cay, aa);
-      store_pos(d, bstart, pc.ib, cbx, cby, ab);
-    }
-    if (min_sep >= -(T)(3.0 * LINEAR_SLOP)) {
+    if (position_sweep(d, bstart, cstart, nci)) {
       position_solved = true;
       break;
     }
@@ -1035,8 +1169,8 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_solve(Dev<T> d) {
   const int n = *d.q_count;
   for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
-    int4 in = d.q_isl[q];
-    int nc = d.q_nc[q];
+    int4 in = d.qs_isl[q];
+    int nc = d.qs_nc[q];
     solve_island(d, in.x, in.y, in.z, in.w, nc & 0x3FFFFFFF, (nc >> 30) & 1);
   }
 }
@@ -1360,7 +1494,33 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int from_upload, int
 
 template <typename T>
 __global__ void k_reset_queue(Dev<T> d) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *d.q_count = 0;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) *d.q_count = 0;
+    if (threadIdx.x < 2 * kSizeClasses) d.q_cls[threadIdx.x] = 0;
+  }
+}
+
+// Orders the island queue by size class, largest first, so that the lanes of
+// a wave of k_solve carry islands of similar length.
+template <typename T>
+__global__ void k_order_islands(Dev<T> d) {
+  __shared__ int base[kSizeClasses];
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int c = kSizeClasses - 1; c >= 0; --c) {
+      base[c] = run;
+      run += d.q_cls[c];
+    }
+  }
+  __syncthreads();
+  const int n = *d.q_count;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int nc = d.q_nc[q];
+    const int c = size_class(nc & 0x3FFFFFFF);
+    const int pos = base[c] + atomicAdd(&d.q_cls[kSizeClasses + c], 1);
+    d.qs_isl[pos] = d.q_isl[q];
+    d.qs_nc[pos] = nc;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1788,6 +1948,9 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.q_isl, nb));
   TRY(dalloc(ctx, &d.q_nc, nb));
   TRY(dalloc(ctx, &d.q_count, 1));
+  TRY(dalloc(ctx, &d.qs_isl, nb));
+  TRY(dalloc(ctx, &d.qs_nc, nb));
+  TRY(dalloc(ctx, &d.q_cls, 2 * kSizeClasses));
   TRY(dalloc(ctx, &d.pv, nb));
   TRY(dalloc(ctx, &d.vc, ncap));
   TRY(dalloc(ctx, &d.pc, ncap));
@@ -1865,6 +2028,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     hipLaunchKernelGGL(k_islands<T>, dim3(nw), dim3(kBlock), ctx->smem_islands, st, d);
     if (timed) HIPCHK(hipEventRecord(ctx->ev[2], st));
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024 * 8));
+    hipLaunchKernelGGL(k_order_islands<T>, dim3(std::min(nblk, 1024)), dim3(kBlock), 0, st, d);
     hipLaunchKernelGGL(k_solve<T>, dim3(nblk), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ctx->ev[3], st));
     hipLaunchKernelGGL(k_sync<T>, dim3(std::max(1, (ctx->nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d);
