@@ -89,6 +89,30 @@ B2G_API int b2g_download(b2g_ctx* ctx, b2g_batch_out* out);
 
 B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
 
+/* Launch shape of the island solver.  The reference solves islands one after
+   another on one thread (world.py:475-580, island.py:65-179), so these have no
+   counterpart there; they change only how islands are dealt to wavefronts,
+   never the order of work inside an island, and so never the results.
+     B2G_PARAM_HEAVY_CLASS      an island is heavy when its size class
+                                floor(log2(contacts + 1)) is at least this,
+                                0..16 (16: none); a heavy island is solved by
+                                a team of lanes, the others by one lane each
+     B2G_PARAM_TEAM_FULL_CLASS  heavy islands of this class and above get a
+                                whole wavefront (64 lanes), one class below 32
+                                lanes, then 16, and 8 for the rest; 0..16
+     B2G_PARAM_TEAM_LDS         LDS bytes per wavefront, shared by its teams,
+                                a multiple of 256 up to 163840; an island whose
+                                state needs more than its team's share (24 B
+                                per body, 4 per body rounded to even bodies, 12
+                                per contact) is solved by one lane instead
+     B2G_PARAM_SOLVE_GRID       most wavefronts launched for the solver, >= 1 */
+#define B2G_PARAM_TEAM_LDS 1
+#define B2G_PARAM_HEAVY_CLASS 2
+#define B2G_PARAM_SOLVE_GRID 3
+#define B2G_PARAM_TEAM_FULL_CLASS 4
+B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value);
+B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
+
 /* Run n eager steps with HIP events between the step's kernels and write the
    average milliseconds per kernel: ms_out[0..4] = collide, islands, solve,
    sync, pairs. */

@@ -21,7 +21,11 @@ B2G_E_INVALID, B2G_E_CAPACITY, B2G_E_HIP, B2G_E_NAN, B2G_E_UNSUPPORTED, B2G_E_NO
 B2G_FP32, B2G_FP64 = 0, 1
 
 EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step", "b2g_step_n", "b2g_sync",
-           "b2g_counts", "b2g_download", "b2g_last_error", "b2g_profile_step", "b2g_narrowphase", "b2g_pairs")
+           "b2g_counts", "b2g_download", "b2g_last_error", "b2g_set_param", "b2g_get_param", "b2g_profile_step",
+           "b2g_narrowphase", "b2g_pairs")
+
+# solver launch-shape keys (b2g_set_param)
+PARAMS = {"team_lds": 1, "heavy_class": 2, "solve_grid": 3, "team_full_class": 4}
 
 _ERRNAMES = {B2G_E_INVALID: "invalid argument", B2G_E_CAPACITY: "capacity exceeded", B2G_E_HIP: "HIP error",
              B2G_E_NAN: "non-finite state", B2G_E_UNSUPPORTED: "unsupported feature",
@@ -65,6 +69,8 @@ def load(path: Optional[str] = None):
     lib.b2g_profile_step.argtypes = [vp, ctypes.c_int, f64, ctypes.c_int, ctypes.c_int, vp]
     lib.b2g_narrowphase.argtypes = [ctypes.c_int, ctypes.c_int] + [vp] * 8
     lib.b2g_pairs.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
+    lib.b2g_set_param.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+    lib.b2g_get_param.argtypes = [vp, ctypes.c_int32, vp]
     _ = i32
     _lib = lib
     return lib
@@ -130,6 +136,15 @@ class Engine:
         out = L.OutBuffers(self._like, con_cap=int(c[2]) + 16, move_cap=int(c[1]) + 16)
         self._check(self.lib.b2g_download(self.h, ctypes.byref(out.struct)), "download")
         return out.to_batch()
+
+    def set_param(self, name: str, value: int) -> None:
+        """Solver launch shape: team_lds, heavy_class, team_full_class or solve_grid (results do not depend on them)."""
+        self._check(self.lib.b2g_set_param(self.h, PARAMS[name], int(value)), "set_param")
+
+    def get_param(self, name: str) -> int:
+        v = np.zeros(1, np.int32)
+        self._check(self.lib.b2g_get_param(self.h, PARAMS[name], v.ctypes.data), "get_param")
+        return int(v[0])
 
     def profile(self, n: int, dt: float = 1.0 / 60, vel_iters: int = 8, pos_iters: int = 3) -> np.ndarray:
         ms = np.zeros(5, np.float32)

@@ -56,10 +56,16 @@ enum : int {
 enum : int { BF_AWAKE = 1, BF_ALLOW_SLEEP = 2, BF_FIXED_ROT = 4, BF_ACTIVE = 8, BF_BULLET = 16 };
 enum : int { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
 enum : int { WF_ALLOW_SLEEP = 1, WF_WARM = 2, WF_NEW_FIXTURE = 4, WF_KEEP_FORCES = 8 };
-enum : int { WERR_CAPACITY = 1, WERR_UNSUPPORTED = 2, WERR_NEWPAIRS = 4 };
+enum : int { WERR_CAPACITY = 1, WERR_UNSUPPORTED = 2, WERR_NEWPAIRS = 4, WERR_SCHEDULE = 8 };
 
 constexpr int kBlock = 256;
+constexpr int kMaxLds = 163840;  // LDS per workgroup on gfx950
 constexpr int kSizeClasses = 16;
+// q_cls layout: [0, 16) island count per class, [16, 32) cursors,
+// [32] islands in team classes, [33] wavefronts for them,
+// [34, 50) first wavefront per class, [50, 66) first queue entry per class
+constexpr int kQHeavy = 2 * kSizeClasses, kQHeavyWaves = kQHeavy + 1, kQWaveBase = kQHeavy + 2,
+              kQBase = kQWaveBase + kSizeClasses, kQcls = kQBase + kSizeClasses;
 
 __host__ __device__ inline int size_class(int nc) {  // floor(log2(nc + 1)), capped
   int c = 0;
@@ -107,6 +113,9 @@ struct Dev {
   int nw, nb, nf, ncap;
   double dt, inv_dt;
   int vel_iters, pos_iters;
+  int heavy_class;      // islands of this size class and above are solved by a team of lanes
+  int team_full_class;  // ... by a whole wavefront from this class up (half one class below, ...)
+  int team_lds;         // LDS bytes per wavefront of teams
   // worlds
   const int* w_body_off;
   const int* w_fix_off;
@@ -162,13 +171,15 @@ struct Dev {
   int* i_bodies;     // per-world segment at w_body_off[w]
   int* i_contacts;   // per-world segment at w_con_base[w]
   int2* i_cidx;      // per-world segment at w_con_base[w]
+  uint2* i_rank;     // per-world segment at w_con_base[w]: the contact's rank in body A's and body B's
+                     // contact sequence of the island (x = rank A | rank B << 16)
   int4* i_info;      // per-world segment at w_body_off[w]: (bstart, nb, cstart, nc | last_allow << 30)
   int4* q_isl;       // global island queue: (world, bstart, nb, cstart)
   int* q_nc;         // (nc | last_allow << 30)
   int* q_count;
   int4* qs_isl;      // the queue ordered by island size, largest first
   int* qs_nc;
-  int* q_cls;        // [2 * kSizeClasses]: per-class counts, then cursors
+  int* q_cls;        // [kQcls]: per-class counts, cursors, then the wave plan of k_solve
   // solver scratch
   PV<T>* pv;   // indexed like i_bodies
   VC<T>* vc;   // indexed like i_contacts
@@ -448,6 +459,7 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
   }
   __syncthreads();
   for (int i = tid; i < nb; i += kBlock) {  // restore ascending slot order per body
+    L.cur[i] = 0;  // from here on: contacts of the body seen by the island search
     int beg = L.off[i], end = L.off[i + 1];
     for (int k = beg + 1; k < end; ++k) {
       uint16_t s = L.adj_s[k], o = L.adj_o[k];
@@ -496,8 +508,11 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
           const uint32_t bit = 1u << (s & 31);
           if (L.cflag[s >> 5] & bit) continue;
           L.cflag[s >> 5] |= bit;
-          isl_c[nci_tot++] = s;
           const int o = L.adj_o[k];
+          const uint32_t rank_b = (uint32_t)L.cur[b]++;
+          const uint32_t rank_o = (L.bst[o] & 1) ? 0u : (uint32_t)L.cur[o]++;
+          d.i_rank[cb + nci_tot] = make_uint2((uint32_t)b, rank_b | rank_o << 16);
+          isl_c[nci_tot++] = s;
           if (L.bst[o] & 16) continue;
           L.stack[sp++] = (uint16_t)o;
           L.bst[o] |= 16;
@@ -526,6 +541,9 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
     int ia = (L.bst[la] & 1) ? -1 - la : (int)L.bidx[la];
     int ib = (L.bst[lb] & 1) ? -1 - lb : (int)L.bidx[lb];
     d.i_cidx[cb + k] = make_int2(ia, ib);
+    const uint2 r = d.i_rank[cb + k];  // (expanding body, its rank | the other body's rank << 16)
+    const uint32_t lo = r.y & 0xFFFFu, hi = r.y >> 16;
+    d.i_rank[cb + k] = make_uint2((int)r.x == la ? (lo | hi << 16) : (hi | lo << 16), 0u);
   }
   for (int i = tid; i < nb; i += kBlock) {
     uint8_t bs = L.bst[i];
@@ -936,183 +954,267 @@ __device__ bool position_sweep(const Dev<T>& d, int bstart, int cstart, int nci)
   return min_sep >= -(T)(3.0 * LINEAR_SLOP);
 }
 
+// Per-island constants.
+struct IslandK {
+  int b0, f0, cbase, wflags, pvb;
+  double dt, dt_ratio;
+  V2d grav;
+  bool warm;
+};
+
+template <typename T>
+__device__ __forceinline__ IslandK island_k(const Dev<T>& d, int w, int bstart) {
+  IslandK c;
+  c.b0 = d.w_body_off[w];
+  c.f0 = d.w_fix_off[w];
+  c.cbase = d.w_con_base[w];
+  c.wflags = d.w_flags[w];
+  c.pvb = bstart - c.b0;  // island body i is world-segment entry pvb + i
+  c.dt = d.dt;
+  c.dt_ratio = d.w_inv_dt0[w] * d.dt;
+  c.grav = v2<double>(d.w_grav[2 * w], d.w_grav[2 * w + 1]);
+  c.warm = (c.wflags & WF_WARM) != 0;
+  return c;
+}
+
+// 1. integrate velocities of island body i (island.py:71-95)
+template <typename T>
+__device__ __forceinline__ PV<T> integrate_velocity(const Dev<T>& d, const IslandK& c, int bstart, int i) {
+  const int gb = c.b0 + d.i_bodies[bstart + i];
+  double2 cc = d.b_c[gb];
+  double a = d.b_a[gb];
+  d.b_c0[gb] = cc;
+  d.b_a0[gb] = a;
+  V2d v = d.b_v[gb];
+  double wv = d.b_w[gb];
+  if (d.b_type[gb] == BT_DYNAMIC) {
+    const double dt = c.dt;
+    V2<T> f = d.b_f[gb];
+    V2d acc = scale((double)d.b_gs[gb], c.grav) + scale((double)d.b_im[gb], dv2(f.x, f.y));
+    v = v + scale(dt, acc);
+    wv += dt * (double)d.b_iI[gb] * (double)d.b_t[gb];
+    v = scale(pclamp(1.0 - dt * (double)d.b_ld[gb], 0.0, 1.0), v);
+    wv *= pclamp(1.0 - dt * (double)d.b_ad[gb], 0.0, 1.0);
+  }
+  PV<T> p;
+  p.cx = cc.x;
+  p.cy = cc.y;
+  p.a = a;
+  p.vx = v.x;
+  p.vy = v.y;
+  p.w = wv;
+  return p;
+}
+
+// 2. ContactSolver.__init__ + initialize_velocity_constraints for island
+// contact k (contact.py:376-496).  Reads the island's integrated state from
+// d.pv; writes the constraint records and turns i_cidx island-local.
+template <typename T>
+__device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int cstart, int k, int& ia_out,
+                             int& ib_out) {
+  const int b0 = c.b0, f0 = c.f0, pvb = c.pvb;
+  const int gs_ = c.cbase + d.i_contacts[cstart + k];
+  int2 cix = d.i_cidx[cstart + k];
+  const int ia = cix.x >= 0 ? cix.x - pvb : cix.x, ib = cix.y >= 0 ? cix.y - pvb : cix.y;
+  ia_out = ia;
+  ib_out = ib;
+  int2 pr = d.c_pair[gs_];
+  const int fa = f0 + pr.x, fb = f0 + pr.y;
+  const int ba = b0 + d.f_info[fa].x, bb = b0 + d.f_info[fb].x;
+  const int st = d.c_state[gs_];
+  const int mtype = (st >> CS_MTYPE_SHIFT) & 3, pc = (st >> CS_PC_SHIFT) & 3;
+  V4<T> lnlp = d.c_lnlp[gs_], pts = d.c_pts[gs_], imp = d.c_imp[gs_];
+  V2<T> mat = d.c_mat[gs_];
+  VC<T> vc;
+  PC<T> pcn;
+  vc.friction = mat.x;
+  vc.restitution = mat.y;
+  vc.ia = ia;
+  vc.ib = ib;
+  vc.ma = d.b_im[ba];
+  vc.mb = d.b_im[bb];
+  vc.Ia = d.b_iI[ba];
+  vc.Ib = d.b_iI[bb];
+  vc.npts = pc;
+  vc.pad = 0;
+  const bool warm = c.warm;
+  const double dt_ratio = c.dt_ratio;
+  vc.p[0].ni = warm ? dt_ratio * (double)imp.x : (double)imp.x;
+  vc.p[0].ti = warm ? dt_ratio * (double)imp.y : (double)imp.y;
+  vc.p[1].ni = warm ? dt_ratio * (double)imp.z : (double)imp.z;
+  vc.p[1].ti = warm ? dt_ratio * (double)imp.w : (double)imp.w;
+  double2 lca = d.b_lc[ba], lcb = d.b_lc[bb];
+  pcn.lp0x = pts.x; pcn.lp0y = pts.y; pcn.lp1x = pts.z; pcn.lp1y = pts.w;
+  pcn.lnx = lnlp.x; pcn.lny = lnlp.y; pcn.lpx = lnlp.z; pcn.lpy = lnlp.w;
+  pcn.ra = (T)d.f_rad64[fa];
+  pcn.rb = (T)d.f_rad64[fb];
+  pcn.ma = vc.ma; pcn.mb = vc.mb; pcn.Ia = vc.Ia; pcn.Ib = vc.Ib;
+  pcn.lcax = (T)lca.x; pcn.lcay = (T)lca.y; pcn.lcbx = (T)lcb.x; pcn.lcby = (T)lcb.y;
+  pcn.type = mtype;
+  pcn.npts = pc;
+  pcn.ia = ia;
+  pcn.ib = ib;
+  double cax, cay, aa, cbx, cby, ab;
+  load_pos(d, bstart, b0, ia, cax, cay, aa);
+  load_pos(d, bstart, b0, ib, cbx, cby, ab);
+  pcn.scx = ia < 0 ? cax : cbx;
+  pcn.scy = ia < 0 ? cay : cby;
+  pcn.sa = ia < 0 ? aa : ab;
+  d.i_cidx[cstart + k] = make_int2(ia, ib);  // island-local from here on
+  V2d va, vb;
+  double wa, wb;
+  load_vel(d, bstart, b0, ia, va, wa);
+  load_vel(d, bstart, b0, ib, vb, wb);
+  Rot<T> qa = rot_of<T>(aa), qb = rot_of<T>(ab);
+  V2<T> dcb = v2<T>((T)(cbx - cax), (T)(cby - cay));
+  XF<T> xfa, xfb;
+  xfa.q = qa;
+  xfa.p = -rmul(qa, v2<T>(pcn.lcax, pcn.lcay));
+  xfb.q = qb;
+  xfb.p = dcb - rmul(qb, v2<T>(pcn.lcbx, pcn.lcby));
+  V2<T> w0 = v2<T>(0, 0), w1 = v2<T>(0, 0);
+  V2<T> normal = world_manifold<T>(mtype, pc, v2<T>(lnlp.x, lnlp.y), v2<T>(lnlp.z, lnlp.w), v2<T>(pts.x, pts.y),
+                                   v2<T>(pts.z, pts.w), xfa, pcn.ra, xfb, pcn.rb, w0, w1);
+  vc.nx = normal.x;
+  vc.ny = normal.y;
+  const T ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
+  const V2d nd = dv2(normal.x, normal.y);
+  auto init_point = [&](VCP<T>& p, V2<T> wp) {
+    V2<T> ra = wp, rb = wp - dcb;
+    T rna = sq(cross(ra, normal)), rnb = sq(cross(rb, normal));
+    T kn = ma + mb + Ia * rna + Ib * rnb;
+    V2<T> tangent = cross_vs(normal, (T)1);
+    T rta = sq(cross(ra, tangent)), rtb = sq(cross(rb, tangent));
+    T kt = ma + mb + Ia * rta + Ib * rtb;
+    double vrel = dot(nd, vb + cross_sv(wb, dv2(rb.x, rb.y)) - va - cross_sv(wa, dv2(ra.x, ra.y)));
+    p.rax = ra.x; p.ray = ra.y; p.rbx = rb.x; p.rby = rb.y;
+    p.nm = kn > (T)0 ? (T)1 / kn : (T)0;
+    p.tm = kt > (T)0 ? (T)1 / kt : (T)0;
+    p.bias = vrel < -VELOCITY_THRESHOLD ? (T)(-(double)vc.restitution * vrel) : (T)0;
+    p.pad = 0;
+  };
+  vc.p[1].rax = vc.p[1].ray = vc.p[1].rbx = vc.p[1].rby = (T)0;
+  vc.p[1].nm = vc.p[1].tm = vc.p[1].bias = vc.p[1].pad = (T)0;
+  if (pc >= 1) init_point(vc.p[0], w0);
+  if (pc >= 2) init_point(vc.p[1], w1);
+  vc.k11 = 1; vc.k12 = 0; vc.k22 = 1; vc.n11 = 1; vc.n12 = 0; vc.n21 = 0; vc.n22 = 1;
+  if (pc == 2) {
+    V2d ra1 = dv2(vc.p[0].rax, vc.p[0].ray), rb1 = dv2(vc.p[0].rbx, vc.p[0].rby);
+    V2d ra2 = dv2(vc.p[1].rax, vc.p[1].ray), rb2 = dv2(vc.p[1].rbx, vc.p[1].rby);
+    const double mad = ma, mbd = mb, Iad = Ia, Ibd = Ib;
+    double rn1a = cross(ra1, nd), rn1b = cross(rb1, nd);
+    double rn2a = cross(ra2, nd), rn2b = cross(rb2, nd);
+    double k11 = mad + mbd + Iad * rn1a * rn1a + Ibd * rn1b * rn1b;
+    double k22 = mad + mbd + Iad * rn2a * rn2a + Ibd * rn2b * rn2b;
+    double k12 = mad + mbd + Iad * rn1a * rn2a + Ibd * rn1b * rn2b;
+    if (sq(k11) < 1000.0 * (k11 * k22 - sq(k12))) {
+      vc.k11 = k11; vc.k12 = k12; vc.k22 = k22;
+      double det = k11 * k22 - k12 * k12;
+      if (det != 0.0) det = 1.0 / det;
+      vc.n11 = det * k22; vc.n21 = -det * k12; vc.n12 = -det * k12; vc.n22 = det * k11;
+    } else {
+      vc.npts = 1;  // quirk Q6: keep only the first point
+    }
+  }
+  d.vc[cstart + k] = vc;
+  d.pc[cstart + k] = pcn;
+}
+
+// 5. store_impulses (contact.py:801-806): zip(manifold points, constraint points)
+template <typename T>
+__device__ __forceinline__ void store_impulse(const Dev<T>& d, const IslandK& c, int cstart, int k) {
+  const VC<T>& vc = d.vc[cstart + k];
+  const int gs_ = c.cbase + d.i_contacts[cstart + k];
+  const int pc = (d.c_state[gs_] >> CS_PC_SHIFT) & 3;
+  const int n = pc < vc.npts ? pc : vc.npts;
+  V4<T> imp = d.c_imp[gs_];
+  if (n >= 1) { imp.x = (T)vc.p[0].ni; imp.y = (T)vc.p[0].ti; }
+  if (n >= 2) { imp.z = (T)vc.p[1].ni; imp.w = (T)vc.p[1].ti; }
+  d.c_imp[gs_] = imp;
+}
+
+// 6. integrate positions (island.py:120-138)
+template <typename T>
+__device__ __forceinline__ void integrate_position(PV<T>& p, double dt) {
+  V2d v = v2<double>(p.vx, p.vy);
+  double wv = p.w;
+  V2d tr = scale(dt, v);
+  if (dot(tr, tr) > MAX_TRANSLATION * MAX_TRANSLATION) {
+    double ratio = MAX_TRANSLATION / sqrt(sq(tr.x) + sq(tr.y));
+    v = scale(ratio, v);
+  }
+  double rotation = dt * wv;
+  if (sq(rotation) > MAX_ROTATION * MAX_ROTATION) {
+    double ratio = MAX_ROTATION / fabs(rotation);
+    wv *= ratio;
+  }
+  V2d dc = scale(dt, v);
+  p.cx = p.cx + dc.x;
+  p.cy = p.cy + dc.y;
+  p.a += dt * wv;
+  p.vx = v.x;
+  p.vy = v.y;
+  p.w = wv;
+}
+
+// 8. copy back and Body._synchronize_transform (island.py:150-159, body.py:906-908)
+template <typename T>
+__device__ __forceinline__ void write_back(const Dev<T>& d, const IslandK& c, int bstart, int i, const PV<T>& p) {
+  const int gb = c.b0 + d.i_bodies[bstart + i];
+  d.b_c[gb] = make_double2(p.cx, p.cy);
+  d.b_a[gb] = p.a;
+  d.b_v[gb] = v2<double>(p.vx, p.vy);
+  d.b_w[gb] = p.w;
+  double sn, cs;
+  sincos(p.a, &sn, &cs);
+  double2 lc = d.b_lc[gb];
+  d.b_xf[gb] = make_double4(p.cx - (cs * lc.x + (-sn) * lc.y), p.cy - (sn * lc.x + cs * lc.y), cs, sn);
+}
+
+// 9. sleep timer of island body i (island.py:161-179, quirk Q7: the stale
+// loop variables); returns the body's contribution to min_sleep_time.
+template <typename T>
+__device__ __forceinline__ double sleep_body(const Dev<T>& d, const IslandK& c, int bstart, int i, int last_allow) {
+  const int gb = c.b0 + d.i_bodies[bstart + i];
+  V2d v = d.b_v[gb];
+  double wv = d.b_w[gb];
+  if (!last_allow || sq(wv) > ANGULAR_SLEEP_TOL * ANGULAR_SLEEP_TOL ||
+      sq(v.x) + sq(v.y) > LINEAR_SLEEP_TOL * LINEAR_SLEEP_TOL) {
+    d.b_sleep[gb] = 0.0;
+    return 0.0;
+  }
+  double t = d.b_sleep[gb] + c.dt;
+  d.b_sleep[gb] = t;
+  return t;
+}
+
+template <typename T>
+__device__ __forceinline__ void sleep_last(const Dev<T>& d, const IslandK& c, int bstart, int nbi) {
+  const int gb = c.b0 + d.i_bodies[bstart + nbi - 1];
+  d.b_sleep[gb] = 0.0;
+  d.b_v[gb] = v2<double>(0, 0);
+  d.b_w[gb] = 0.0;
+  d.b_f[gb] = v2<T>(0, 0);
+  d.b_t[gb] = (T)0;
+  d.b_flags[gb] &= ~BF_AWAKE;
+}
+
+// Island.solve on one lane (island.py:65-179).
 template <typename T>
 __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci, int last_allow) {
-  const double dt = d.dt;
-  const int b0 = d.w_body_off[w];
-  const int f0 = d.w_fix_off[w];
-  const int cbase = d.w_con_base[w];
-  const int wflags = d.w_flags[w];
-  const V2d grav = v2<double>(d.w_grav[2 * w], d.w_grav[2 * w + 1]);
-  const int pvb = bstart - b0;  // island body i is world-segment entry pvb + i
-  // 1. integrate velocities (island.py:71-95)
-  for (int i = 0; i < nbi; ++i) {
-    const int gb = b0 + d.i_bodies[bstart + i];
-    double2 c = d.b_c[gb];
-    double a = d.b_a[gb];
-    d.b_c0[gb] = c;
-    d.b_a0[gb] = a;
-    V2d v = d.b_v[gb];
-    double wv = d.b_w[gb];
-    if (d.b_type[gb] == BT_DYNAMIC) {
-      V2<T> f = d.b_f[gb];
-      V2d acc = scale((double)d.b_gs[gb], grav) + scale((double)d.b_im[gb], dv2(f.x, f.y));
-      v = v + scale(dt, acc);
-      wv += dt * (double)d.b_iI[gb] * (double)d.b_t[gb];
-      v = scale(pclamp(1.0 - dt * (double)d.b_ld[gb], 0.0, 1.0), v);
-      wv *= pclamp(1.0 - dt * (double)d.b_ad[gb], 0.0, 1.0);
-    }
-    PV<T> p;
-    p.cx = c.x;
-    p.cy = c.y;
-    p.a = a;
-    p.vx = v.x;
-    p.vy = v.y;
-    p.w = wv;
-    d.pv[bstart + i] = p;
-  }
-  const double dt_ratio = d.w_inv_dt0[w] * dt;
-  const bool warm = (wflags & WF_WARM) != 0;
-  // 2. ContactSolver.__init__ + initialize_velocity_constraints (contact.py:376-496)
+  const IslandK c = island_k(d, w, bstart);
+  for (int i = 0; i < nbi; ++i) d.pv[bstart + i] = integrate_velocity(d, c, bstart, i);
   for (int k = 0; k < nci; ++k) {
-    const int gs_ = cbase + d.i_contacts[cstart + k];
-    int2 cix = d.i_cidx[cstart + k];
-    const int ia = cix.x >= 0 ? cix.x - pvb : cix.x, ib = cix.y >= 0 ? cix.y - pvb : cix.y;
-    int2 pr = d.c_pair[gs_];
-    const int fa = f0 + pr.x, fb = f0 + pr.y;
-    const int ba = b0 + d.f_info[fa].x, bb = b0 + d.f_info[fb].x;
-    const int st = d.c_state[gs_];
-    const int mtype = (st >> CS_MTYPE_SHIFT) & 3, pc = (st >> CS_PC_SHIFT) & 3;
-    V4<T> lnlp = d.c_lnlp[gs_], pts = d.c_pts[gs_], imp = d.c_imp[gs_];
-    V2<T> mat = d.c_mat[gs_];
-    VC<T> vc;
-    PC<T> pcn;
-    vc.friction = mat.x;
-    vc.restitution = mat.y;
-    vc.ia = ia;
-    vc.ib = ib;
-    vc.ma = d.b_im[ba];
-    vc.mb = d.b_im[bb];
-    vc.Ia = d.b_iI[ba];
-    vc.Ib = d.b_iI[bb];
-    vc.npts = pc;
-    vc.pad = 0;
-    vc.p[0].ni = warm ? dt_ratio * (double)imp.x : (double)imp.x;
-    vc.p[0].ti = warm ? dt_ratio * (double)imp.y : (double)imp.y;
-    vc.p[1].ni = warm ? dt_ratio * (double)imp.z : (double)imp.z;
-    vc.p[1].ti = warm ? dt_ratio * (double)imp.w : (double)imp.w;
-    double2 lca = d.b_lc[ba], lcb = d.b_lc[bb];
-    pcn.lp0x = pts.x; pcn.lp0y = pts.y; pcn.lp1x = pts.z; pcn.lp1y = pts.w;
-    pcn.lnx = lnlp.x; pcn.lny = lnlp.y; pcn.lpx = lnlp.z; pcn.lpy = lnlp.w;
-    pcn.ra = (T)d.f_rad64[fa];
-    pcn.rb = (T)d.f_rad64[fb];
-    pcn.ma = vc.ma; pcn.mb = vc.mb; pcn.Ia = vc.Ia; pcn.Ib = vc.Ib;
-    pcn.lcax = (T)lca.x; pcn.lcay = (T)lca.y; pcn.lcbx = (T)lcb.x; pcn.lcby = (T)lcb.y;
-    pcn.type = mtype;
-    pcn.npts = pc;
-    pcn.ia = ia;
-    pcn.ib = ib;
-    double cax, cay, aa, cbx, cby, ab;
-    load_pos(d, bstart, b0, ia, cax, cay, aa);
-    load_pos(d, bstart, b0, ib, cbx, cby, ab);
-    pcn.scx = ia < 0 ? cax : cbx;
-    pcn.scy = ia < 0 ? cay : cby;
-    pcn.sa = ia < 0 ? aa : ab;
-    d.i_cidx[cstart + k] = make_int2(ia, ib);  // island-local from here on
-    V2d va, vb;
-    double wa, wb;
-    load_vel(d, bstart, b0, ia, va, wa);
-    load_vel(d, bstart, b0, ib, vb, wb);
-    Rot<T> qa = rot_of<T>(aa), qb = rot_of<T>(ab);
-    V2<T> dcb = v2<T>((T)(cbx - cax), (T)(cby - cay));
-    XF<T> xfa, xfb;
-    xfa.q = qa;
-    xfa.p = -rmul(qa, v2<T>(pcn.lcax, pcn.lcay));
-    xfb.q = qb;
-    xfb.p = dcb - rmul(qb, v2<T>(pcn.lcbx, pcn.lcby));
-    V2<T> w0 = v2<T>(0, 0), w1 = v2<T>(0, 0);
-    V2<T> normal = world_manifold<T>(mtype, pc, v2<T>(lnlp.x, lnlp.y), v2<T>(lnlp.z, lnlp.w), v2<T>(pts.x, pts.y),
-                                     v2<T>(pts.z, pts.w), xfa, pcn.ra, xfb, pcn.rb, w0, w1);
-    vc.nx = normal.x;
-    vc.ny = normal.y;
-    const T ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
-    const V2d nd = dv2(normal.x, normal.y);
-    auto init_point = [&](VCP<T>& p, V2<T> wp) {
-      V2<T> ra = wp, rb = wp - dcb;
-      T rna = sq(cross(ra, normal)), rnb = sq(cross(rb, normal));
-      T kn = ma + mb + Ia * rna + Ib * rnb;
-      V2<T> tangent = cross_vs(normal, (T)1);
-      T rta = sq(cross(ra, tangent)), rtb = sq(cross(rb, tangent));
-      T kt = ma + mb + Ia * rta + Ib * rtb;
-      double vrel = dot(nd, vb + cross_sv(wb, dv2(rb.x, rb.y)) - va - cross_sv(wa, dv2(ra.x, ra.y)));
-      p.rax = ra.x; p.ray = ra.y; p.rbx = rb.x; p.rby = rb.y;
-      p.nm = kn > (T)0 ? (T)1 / kn : (T)0;
-      p.tm = kt > (T)0 ? (T)1 / kt : (T)0;
-      p.bias = vrel < -VELOCITY_THRESHOLD ? (T)(-(double)vc.restitution * vrel) : (T)0;
-      p.pad = 0;
-    };
-    vc.p[1].rax = vc.p[1].ray = vc.p[1].rbx = vc.p[1].rby = (T)0;
-    vc.p[1].nm = vc.p[1].tm = vc.p[1].bias = vc.p[1].pad = (T)0;
-    if (pc >= 1) init_point(vc.p[0], w0);
-    if (pc >= 2) init_point(vc.p[1], w1);
-    vc.k11 = 1; vc.k12 = 0; vc.k22 = 1; vc.n11 = 1; vc.n12 = 0; vc.n21 = 0; vc.n22 = 1;
-    if (pc == 2) {
-      V2d ra1 = dv2(vc.p[0].rax, vc.p[0].ray), rb1 = dv2(vc.p[0].rbx, vc.p[0].rby);
-      V2d ra2 = dv2(vc.p[1].rax, vc.p[1].ray), rb2 = dv2(vc.p[1].rbx, vc.p[1].rby);
-      const double mad = ma, mbd = mb, Iad = Ia, Ibd = Ib;
-      double rn1a = cross(ra1, nd), rn1b = cross(rb1, nd);
-      double rn2a = cross(ra2, nd), rn2b = cross(rb2, nd);
-      double k11 = mad + mbd + Iad * rn1a * rn1a + Ibd * rn1b * rn1b;
-      double k22 = mad + mbd + Iad * rn2a * rn2a + Ibd * rn2b * rn2b;
-      double k12 = mad + mbd + Iad * rn1a * rn2a + Ibd * rn1b * rn2b;
-      if (sq(k11) < 1000.0 * (k11 * k22 - sq(k12))) {
-        vc.k11 = k11; vc.k12 = k12; vc.k22 = k22;
-        double det = k11 * k22 - k12 * k12;
-        if (det != 0.0) det = 1.0 / det;
-        vc.n11 = det * k22; vc.n21 = -det * k12; vc.n12 = -det * k12; vc.n22 = det * k11;
-      } else {
-        vc.npts = 1;  // quirk Q6: keep only the first point
-      }
-    }
-    d.vc[cstart + k] = vc;
-    d.pc[cstart + k] = pcn;
+    int ia, ib;
+    init_contact(d, c, bstart, cstart, k, ia, ib);
   }
   // 3. warm_start (contact.py:498-525) and 4. velocity iterations (contact.py:527-799)
-  if (warm) velocity_sweep<T, true>(d, bstart, cstart, nci);
+  if (c.warm) velocity_sweep<T, true>(d, bstart, cstart, nci);
   for (int it = 0; it < d.vel_iters; ++it) velocity_sweep<T, false>(d, bstart, cstart, nci);
-  // 5. store_impulses (contact.py:801-806): zip(manifold points, constraint points)
-  for (int k = 0; k < nci; ++k) {
-    const VC<T>& vc = d.vc[cstart + k];
-    const int gs_ = cbase + d.i_contacts[cstart + k];
-    const int pc = (d.c_state[gs_] >> CS_PC_SHIFT) & 3;
-    const int n = pc < vc.npts ? pc : vc.npts;
-    V4<T> imp = d.c_imp[gs_];
-    if (n >= 1) { imp.x = (T)vc.p[0].ni; imp.y = (T)vc.p[0].ti; }
-    if (n >= 2) { imp.z = (T)vc.p[1].ni; imp.w = (T)vc.p[1].ti; }
-    d.c_imp[gs_] = imp;
-  }
-  // 6. integrate positions (island.py:120-138)
+  for (int k = 0; k < nci; ++k) store_impulse(d, c, cstart, k);
   for (int i = 0; i < nbi; ++i) {
     PV<T> p = d.pv[bstart + i];
-    V2d v = v2<double>(p.vx, p.vy);
-    double wv = p.w;
-    V2d tr = scale(dt, v);
-    if (dot(tr, tr) > MAX_TRANSLATION * MAX_TRANSLATION) {
-      double ratio = MAX_TRANSLATION / sqrt(sq(tr.x) + sq(tr.y));
-      v = scale(ratio, v);
-    }
-    double rotation = dt * wv;
-    if (sq(rotation) > MAX_ROTATION * MAX_ROTATION) {
-      double ratio = MAX_ROTATION / fabs(rotation);
-      wv *= ratio;
-    }
-    V2d dc = scale(dt, v);
-    p.cx = p.cx + dc.x;
-    p.cy = p.cy + dc.y;
-    p.a += dt * wv;
-    p.vx = v.x;
-    p.vy = v.y;
-    p.w = wv;
+    integrate_position(p, c.dt);
     d.pv[bstart + i] = p;
   }
   // 7. position iterations with early exit (contact.py:809-864)
@@ -1123,55 +1225,293 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
       break;
     }
   }
-  // 8. copy back and Body._synchronize_transform (island.py:150-159, body.py:906-908)
-  for (int i = 0; i < nbi; ++i) {
-    const int gb = b0 + d.i_bodies[bstart + i];
-    PV<T> p = d.pv[bstart + i];
-    d.b_c[gb] = make_double2(p.cx, p.cy);
-    d.b_a[gb] = p.a;
-    d.b_v[gb] = v2<double>(p.vx, p.vy);
-    d.b_w[gb] = p.w;
-    double sn, cs;
-    sincos(p.a, &sn, &cs);
-    double2 lc = d.b_lc[gb];
-    d.b_xf[gb] = make_double4(p.cx - (cs * lc.x + (-sn) * lc.y), p.cy - (sn * lc.x + cs * lc.y), cs, sn);
-  }
-  // 9. sleep (island.py:161-179, quirk Q7: the stale loop variables)
-  if (wflags & WF_ALLOW_SLEEP) {
+  for (int i = 0; i < nbi; ++i) write_back(d, c, bstart, i, d.pv[bstart + i]);
+  if (c.wflags & WF_ALLOW_SLEEP) {
     double min_sleep = 1.0e300;
-    for (int i = 0; i < nbi; ++i) {
-      const int gb = b0 + d.i_bodies[bstart + i];
-      V2d v = d.b_v[gb];
-      double wv = d.b_w[gb];
-      if (!last_allow || sq(wv) > ANGULAR_SLEEP_TOL * ANGULAR_SLEEP_TOL ||
-          sq(v.x) + sq(v.y) > LINEAR_SLEEP_TOL * LINEAR_SLEEP_TOL) {
-        d.b_sleep[gb] = 0.0;
-        min_sleep = 0.0;
-      } else {
-        double t = d.b_sleep[gb] + dt;
-        d.b_sleep[gb] = t;
-        min_sleep = pmin(min_sleep, t);
-      }
-    }
-    if (min_sleep >= TIME_TO_SLEEP && position_solved && nbi > 0) {
-      const int gb = b0 + d.i_bodies[bstart + nbi - 1];
-      d.b_sleep[gb] = 0.0;
-      d.b_v[gb] = v2<double>(0, 0);
-      d.b_w[gb] = 0.0;
-      d.b_f[gb] = v2<T>(0, 0);
-      d.b_t[gb] = (T)0;
-      d.b_flags[gb] &= ~BF_AWAKE;
-    }
+    for (int i = 0; i < nbi; ++i) min_sleep = pmin(min_sleep, sleep_body(d, c, bstart, i, last_allow));
+    if (min_sleep >= TIME_TO_SLEEP && position_solved && nbi > 0) sleep_last(d, c, bstart, nbi);
   }
 }
 
+// ---------------------------------------------------------------------------
+// Island.solve by a team of lanes (the path for large islands)
+//
+// A team is 8, 16, 32 or 64 lanes of one wavefront (by the island's size
+// class); a wavefront carries 64 / T islands of one class.  Lane t of a team
+// owns the island's contacts k = t, t + T, t + 2T, ... and walks them sweep
+// by sweep.  A contact's solve at sweep s must see every earlier solve (in the
+// reference's Gauss-Seidel order) that touches one of its dynamic bodies, and
+// nothing later.  Each dynamic body X keeps a counter of completed solves that
+// touched it; the contact is ready when, for each of its dynamic bodies, the
+// counter equals its rank in that body's sequence (s * deg(X) + the number of
+// the body's contacts before it in the island list, from the island search).
+// Ready lanes run in the same pass, which is safe because at most one pending
+// solve per body can be ready.  The earliest unfinished solve of a team is
+// always ready, so every pass makes progress; passes are still capped.  Body
+// velocities, then positions, live in LDS; the constraint records stay in HBM
+// and are prefetched one solve ahead.  The arithmetic per contact is the same
+// code as the one-lane path, so results are identical.
+// ---------------------------------------------------------------------------
+constexpr int kSolveBlock = 64;
+
+__host__ __device__ inline int team_size(int cls, int full_cls) {
+  int sh = full_cls - cls;
+  sh = sh < 0 ? 0 : (sh > 3 ? 3 : sh);
+  return kSolveBlock >> sh;
+}
+
+__host__ __device__ inline int team_lds_bytes(int nbi, int nci) {
+  // 3 doubles and an int counter per body (bodies padded to even), ushort4
+  // ranks and degrees and short2 bodies per contact
+  return 24 * nbi + 4 * ((nbi + 1) & ~1) + 12 * nci;
+}
+
+// The lanes of one wavefront exchange body state through LDS between passes.
+// LDS instructions of a wavefront execute in issue order, so wavefront-scope
+// ordering (which emits no waits) is all a pass boundary needs.
+__device__ __forceinline__ int lds_ld(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+__device__ __forceinline__ void lds_st(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+__device__ __forceinline__ void pass_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+// Phase boundary inside a one-wavefront block: makes the team's LDS and global
+// writes visible to the other lanes (no s_barrier, so divergent teams are fine).
+__device__ __forceinline__ void team_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ bool wave_any(bool p) {
+  return __ballot(p) != 0;
+}
+template <typename X>
+__device__ __forceinline__ X team_min(X v, int T) {
+  for (int off = T >> 1; off > 0; off >>= 1) v = pmin(v, (X)__shfl_xor(v, off, T));
+  return v;
+}
+
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_solve(Dev<T> d) {
+__device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci, int last_allow,
+                                  char* lds, int t, int TS) {
+  const IslandK c = island_k(d, w, bstart);
+  double* sx = (double*)lds;                         // [3 * nbi]
+  int* cnt = (int*)(sx + 3 * nbi);                   // [nbi]
+  ushort4* rk = (ushort4*)(cnt + ((nbi + 1) & ~1));  // [nci]: rank and degree for body A, for body B
+  short2* ab = (short2*)(rk + nci);                  // [nci]: island-local bodies
+  // 1. integrate velocities
+  for (int i = t; i < nbi; i += TS) {
+    PV<T> p = integrate_velocity(d, c, bstart, i);
+    d.pv[bstart + i] = p;
+    sx[3 * i] = p.vx;
+    sx[3 * i + 1] = p.vy;
+    sx[3 * i + 2] = p.w;
+    cnt[i] = 0;
+  }
+  team_sync();
+  // 2. constraint records, and each body's number of contacts
+  for (int k = t; k < nci; k += TS) {
+    int ia, ib;
+    init_contact(d, c, bstart, cstart, k, ia, ib);
+    ab[k] = make_short2((short)(ia < 0 ? -1 : ia), (short)(ib < 0 ? -1 : ib));
+    if (ia >= 0) atomicAdd(&cnt[ia], 1);
+    if (ib >= 0) atomicAdd(&cnt[ib], 1);
+  }
+  team_sync();
+  for (int k = t; k < nci; k += TS) {
+    const short2 x = ab[k];
+    const uint32_t r = d.i_rank[cstart + k].x;
+    rk[k] = make_ushort4((unsigned short)(r & 0xFFFFu), x.x >= 0 ? (unsigned short)cnt[x.x] : 0,
+                         (unsigned short)(r >> 16), x.y >= 0 ? (unsigned short)cnt[x.y] : 0);
+  }
+  team_sync();
+  for (int i = t; i < nbi; i += TS) cnt[i] = 0;
+  team_sync();
+  // 3 + 4. warm start and velocity sweeps as one dataflow sequence
+  const int s0 = c.warm ? 0 : 1;
+  const int nsw = d.vel_iters + 1 - s0;
+  const int mine = t < nci ? (nci - 1 - t) / TS + 1 : 0;  // contacts owned by this lane
+  {
+    bool active = nsw > 0 && mine > 0;
+    const int cap = 2 * nsw * nci + 64;
+    int k = t, s = 0;
+    VC<T> cur, nxt;
+    if (active) cur = d.vc[cstart + k];
+    int nk = k + TS, ns = s;
+    if (nk >= nci) { nk = t; ns = s + 1; }
+    if (active && mine > 1 && ns < nsw) nxt = d.vc[cstart + nk];
+    int pass = 0;
+    while (wave_any(active && pass < cap)) {
+      ++pass;
+      if (active) {
+        const ushort4 r = rk[k];
+        const int ia = cur.ia, ib = cur.ib;
+        const int ra = s * (int)r.y + (int)r.x, rb = s * (int)r.w + (int)r.z;
+        const bool ready = (ia < 0 || lds_ld(&cnt[ia]) == ra) && (ib < 0 || lds_ld(&cnt[ib]) == rb);
+        if (ready) {
+          V2d va = v2<double>(0, 0), vb = v2<double>(0, 0);
+          double wa = 0.0, wb = 0.0;  // static bodies have zero velocity (checked at upload)
+          if (ia >= 0) { va = v2<double>(sx[3 * ia], sx[3 * ia + 1]); wa = sx[3 * ia + 2]; }
+          if (ib >= 0) { vb = v2<double>(sx[3 * ib], sx[3 * ib + 1]); wb = sx[3 * ib + 2]; }
+          if (s + s0 == 0) {
+            warm_contact(cur, va, wa, vb, wb);
+          } else {
+            solve_contact_velocity(cur, va, wa, vb, wb);
+            VCP<T>* dst = d.vc[cstart + k].p;
+            dst[0].ni = cur.p[0].ni;
+            dst[0].ti = cur.p[0].ti;
+            if (cur.npts == 2) {
+              dst[1].ni = cur.p[1].ni;
+              dst[1].ti = cur.p[1].ti;
+            }
+          }
+          if (ia >= 0) { sx[3 * ia] = va.x; sx[3 * ia + 1] = va.y; sx[3 * ia + 2] = wa; lds_st(&cnt[ia], ra + 1); }
+          if (ib >= 0) { sx[3 * ib] = vb.x; sx[3 * ib + 1] = vb.y; sx[3 * ib + 2] = wb; lds_st(&cnt[ib], rb + 1); }
+          k = nk;
+          s = ns;
+          if (s >= nsw) {
+            active = false;
+          } else {
+            nk = k + TS;
+            ns = s;
+            if (nk >= nci) { nk = t; ns = s + 1; }
+            if (mine > 1) {  // with one contact the record in registers is the current one
+              cur = nxt;
+              if (ns < nsw) nxt = d.vc[cstart + nk];
+            }
+          }
+        }
+      }
+      pass_fence();
+    }
+    if (active) atomicOr(&d.w_err[w], WERR_SCHEDULE);  // pass limit reached
+  }
+  team_sync();
+  // 5. store impulses, 6. integrate positions (positions to LDS)
+  for (int k = t; k < nci; k += TS) store_impulse(d, c, cstart, k);
+  for (int i = t; i < nbi; i += TS) {
+    PV<T> p = d.pv[bstart + i];
+    p.vx = sx[3 * i];
+    p.vy = sx[3 * i + 1];
+    p.w = sx[3 * i + 2];
+    integrate_position(p, c.dt);
+    d.pv[bstart + i] = p;
+    sx[3 * i] = p.cx;
+    sx[3 * i + 1] = p.cy;
+    sx[3 * i + 2] = p.a;
+  }
+  // 7. position iterations with early exit, one dataflow sweep each
+  bool position_solved = false;
+  for (int it = 0; it < d.pos_iters; ++it) {
+    if (!position_solved) {  // team-uniform
+      for (int i = t; i < nbi; i += TS) cnt[i] = 0;
+      team_sync();
+      T min_sep = 0;
+      bool active = mine > 0;
+      int k = t;
+      PC<T> cur, nxt;
+      if (active) cur = d.pc[cstart + k];
+      if (active && k + TS < nci) nxt = d.pc[cstart + k + TS];
+      int pass = 0;
+      const int pcap = 2 * nci + 64;
+      while (wave_any(active && pass < pcap)) {
+        ++pass;
+        if (active) {
+          const ushort4 r = rk[k];
+          const int ia = cur.ia, ib = cur.ib;
+          const bool ready = (ia < 0 || lds_ld(&cnt[ia]) == (int)r.x) && (ib < 0 || lds_ld(&cnt[ib]) == (int)r.z);
+          if (ready) {
+            double ax, ay, aa, bx, by, bb;
+            if (ia >= 0) { ax = sx[3 * ia]; ay = sx[3 * ia + 1]; aa = sx[3 * ia + 2]; }
+            else { ax = cur.scx; ay = cur.scy; aa = cur.sa; }
+            if (ib >= 0) { bx = sx[3 * ib]; by = sx[3 * ib + 1]; bb = sx[3 * ib + 2]; }
+            else { bx = cur.scx; by = cur.scy; bb = cur.sa; }
+            solve_contact_position(cur, ax, ay, aa, bx, by, bb, min_sep);
+            if (ia >= 0) { sx[3 * ia] = ax; sx[3 * ia + 1] = ay; sx[3 * ia + 2] = aa; lds_st(&cnt[ia], (int)r.x + 1); }
+            if (ib >= 0) { sx[3 * ib] = bx; sx[3 * ib + 1] = by; sx[3 * ib + 2] = bb; lds_st(&cnt[ib], (int)r.z + 1); }
+            k += TS;
+            if (k >= nci) {
+              active = false;
+            } else {
+              cur = nxt;
+              if (k + TS < nci) nxt = d.pc[cstart + k + TS];
+            }
+          }
+        }
+        pass_fence();
+      }
+      if (active) atomicOr(&d.w_err[w], WERR_SCHEDULE);
+      min_sep = team_min(min_sep, TS);  // the sequential loop's min_separation
+      team_sync();
+      if (min_sep >= -(T)(3.0 * LINEAR_SLOP)) position_solved = true;
+    }
+  }
+  // 8. copy back
+  for (int i = t; i < nbi; i += TS) {
+    PV<T> p = d.pv[bstart + i];
+    p.cx = sx[3 * i];
+    p.cy = sx[3 * i + 1];
+    p.a = sx[3 * i + 2];
+    write_back(d, c, bstart, i, p);
+  }
+  // 9. sleep (each lane reads back only the bodies it wrote)
+  if (c.wflags & WF_ALLOW_SLEEP) {
+    double min_sleep = 1.0e300;
+    for (int i = t; i < nbi; i += TS) min_sleep = pmin(min_sleep, sleep_body(d, c, bstart, i, last_allow));
+    min_sleep = team_min(min_sleep, TS);
+    if (min_sleep >= TIME_TO_SLEEP && position_solved && nbi > 0 && t == (nbi - 1) % TS)
+      sleep_last(d, c, bstart, nbi);
+  }
+  team_sync();  // the next islands reuse the LDS
+}
+
+// One wavefront per block.  The queue is ordered largest class first.  Its
+// entries of classes >= heavy_class are dealt to wavefronts 64 / T islands at
+// a time (T = team_size of the class), each team with its share of the LDS
+// budget; an island whose state does not fit that share is solved by the
+// team's first lane alone.  The remaining islands are dealt 64 per wavefront,
+// one lane each.  Wavefronts are taken in queue order, so the longest islands
+// start first and are spread over the whole chip.
+template <typename T>
+__global__ void __launch_bounds__(kSolveBlock) k_solve(Dev<T> d) {
+  extern __shared__ __align__(16) char team_lds[];
   const int n = *d.q_count;
-  for (int q = blockIdx.x * kBlock + threadIdx.x; q < n; q += gridDim.x * kBlock) {
-    int4 in = d.qs_isl[q];
-    int nc = d.qs_nc[q];
-    solve_island(d, in.x, in.y, in.z, in.w, nc & 0x3FFFFFFF, (nc >> 30) & 1);
+  const int nh = d.q_cls[kQHeavy];
+  const int hw = d.q_cls[kQHeavyWaves];
+  const int nwaves = hw + (n - nh + kSolveBlock - 1) / kSolveBlock;
+  const int l = threadIdx.x;
+  for (int g = blockIdx.x; g < nwaves; g += gridDim.x) {
+    if (g < hw) {
+      // classes own consecutive wave ranges, highest class first
+      int cls = kSizeClasses - 1;
+      while (cls > d.heavy_class && g >= d.q_cls[kQWaveBase + cls - 1]) --cls;
+      const int TS = team_size(cls, d.team_full_class);
+      const int per = kSolveBlock / TS;
+      const int team = l / TS, t = l % TS;
+      const int q = d.q_cls[kQBase + cls] + (g - d.q_cls[kQWaveBase + cls]) * per + team;
+      const int qend = d.q_cls[kQBase + cls] + d.q_cls[cls];
+      if (q < qend) {
+        const int4 in = d.qs_isl[q];
+        const int nc = d.qs_nc[q];
+        const int nci = nc & 0x3FFFFFFF, last_allow = (nc >> 30) & 1;
+        const int share = d.team_lds / per;
+        if (team_lds_bytes(in.z, nci) <= share) {
+          solve_island_team(d, in.x, in.y, in.z, in.w, nci, last_allow, team_lds + team * share, t, TS);
+        } else if (t == 0) {
+          solve_island(d, in.x, in.y, in.z, in.w, nci, last_allow);
+        }
+      }
+      team_sync();
+    } else {
+      const int q = nh + (g - hw) * kSolveBlock + l;
+      if (q < n) {
+        const int4 in = d.qs_isl[q];
+        const int nc = d.qs_nc[q];
+        solve_island(d, in.x, in.y, in.z, in.w, nc & 0x3FFFFFFF, (nc >> 30) & 1);
+      }
+    }
   }
 }
 
@@ -1496,7 +1836,7 @@ template <typename T>
 __global__ void k_reset_queue(Dev<T> d) {
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) *d.q_count = 0;
-    if (threadIdx.x < 2 * kSizeClasses) d.q_cls[threadIdx.x] = 0;
+    if (threadIdx.x < kQWaveBase) d.q_cls[threadIdx.x] = 0;
   }
 }
 
@@ -1506,10 +1846,20 @@ template <typename T>
 __global__ void k_order_islands(Dev<T> d) {
   __shared__ int base[kSizeClasses];
   if (threadIdx.x == 0) {
-    int run = 0;
+    int run = 0, waves = 0;
     for (int c = kSizeClasses - 1; c >= 0; --c) {
       base[c] = run;
-      run += d.q_cls[c];
+      const int nc = d.q_cls[c];
+      if (blockIdx.x == 0) {
+        d.q_cls[kQBase + c] = run;
+        d.q_cls[kQWaveBase + c] = waves;
+        if (c >= d.heavy_class) waves += (nc * team_size(c, d.team_full_class) + kSolveBlock - 1) / kSolveBlock;
+      }
+      run += nc;
+      if (c == d.heavy_class && blockIdx.x == 0) {
+        d.q_cls[kQHeavy] = run;
+        d.q_cls[kQHeavyWaves] = waves;
+      }
     }
   }
   __syncthreads();
@@ -1604,6 +1954,11 @@ struct b2g_ctx {
   int device = 0;
   double contact_factor = 8.0;
   int contact_extra = 64;
+  // solver launch shape (b2g_set_param)
+  int heavy_class = 5;
+  int team_full_class = 9;
+  int team_lds = 32768;
+  int solve_grid = 16384;
   hipStream_t stream = nullptr;
   std::string err;
   // sizes
@@ -1654,12 +2009,16 @@ static int dalloc(b2g_ctx* ctx, X** p, size_t n) {
   return 0;
 }
 
-static void free_device(b2g_ctx* ctx) {
+static void drop_graph(b2g_ctx* ctx) {
   if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->graph) hipGraphDestroy(ctx->graph);
   ctx->graph_exec = nullptr;
   ctx->graph = nullptr;
   ctx->graph_vi = -1;
+}
+
+static void free_device(b2g_ctx* ctx) {
+  drop_graph(ctx);
   for (void* p : ctx->allocs) hipFree(p);
   ctx->allocs.clear();
   if (ctx->dev) {
@@ -1721,8 +2080,9 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   ctx->ncap = ncap;
   ctx->smem_islands = island_smem_bytes(ctx->max_nb, ctx->max_ccap);
   ctx->smem_pairs = pairs_smem_bytes(ctx->max_nf, ctx->max_ccap);
-  if (ctx->smem_islands > 163840 || ctx->smem_pairs > 163840)
+  if (ctx->smem_islands > (size_t)kMaxLds || ctx->smem_pairs > (size_t)kMaxLds)
     return set_error(ctx, B2G_E_UNSUPPORTED, "world too large for the LDS-resident path (LDS > 160 KB)");
+  HIPCHK(hipFuncSetAttribute((const void*)k_solve<T>, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
 
   HIPCHK(hipFuncSetAttribute((const void*)k_islands<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)ctx->smem_islands));
@@ -1944,13 +2304,14 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.i_bodies, nb));
   TRY(dalloc(ctx, &d.i_contacts, ncap));
   TRY(dalloc(ctx, &d.i_cidx, ncap));
+  TRY(dalloc(ctx, &d.i_rank, ncap));
   TRY(dalloc(ctx, &d.i_info, nb));
   TRY(dalloc(ctx, &d.q_isl, nb));
   TRY(dalloc(ctx, &d.q_nc, nb));
   TRY(dalloc(ctx, &d.q_count, 1));
   TRY(dalloc(ctx, &d.qs_isl, nb));
   TRY(dalloc(ctx, &d.qs_nc, nb));
-  TRY(dalloc(ctx, &d.q_cls, 2 * kSizeClasses));
+  TRY(dalloc(ctx, &d.q_cls, kQcls));
   TRY(dalloc(ctx, &d.pv, nb));
   TRY(dalloc(ctx, &d.vc, ncap));
   TRY(dalloc(ctx, &d.pc, ncap));
@@ -2014,6 +2375,9 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   d.inv_dt = dt > 0.0 ? 1.0 / dt : 0.0;
   d.vel_iters = vi;
   d.pos_iters = pi;
+  d.heavy_class = ctx->heavy_class;
+  d.team_full_class = ctx->team_full_class;
+  d.team_lds = ctx->team_lds;
   const int nw = ctx->nw;
   hipStream_t st = ctx->stream;
   if (first_pairs) {
@@ -2027,9 +2391,10 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   if (dt > 0.0) {
     hipLaunchKernelGGL(k_islands<T>, dim3(nw), dim3(kBlock), ctx->smem_islands, st, d);
     if (timed) HIPCHK(hipEventRecord(ctx->ev[2], st));
-    int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024 * 8));
-    hipLaunchKernelGGL(k_order_islands<T>, dim3(std::min(nblk, 1024)), dim3(kBlock), 0, st, d);
-    hipLaunchKernelGGL(k_solve<T>, dim3(nblk), dim3(kBlock), 0, st, d);
+    int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
+    hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
+    const int swaves = std::max(1, std::min((ctx->nb + kSolveBlock - 1) / kSolveBlock, ctx->solve_grid));
+    hipLaunchKernelGGL(k_solve<T>, dim3(swaves), dim3(kSolveBlock), ctx->team_lds, st, d);
     if (timed) HIPCHK(hipEventRecord(ctx->ev[3], st));
     hipLaunchKernelGGL(k_sync<T>, dim3(std::max(1, (ctx->nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ctx->ev[4], st));
@@ -2303,6 +2668,7 @@ B2G_API int b2g_sync(b2g_ctx* ctx) {
   if (e & WERR_CAPACITY) return set_error(ctx, B2G_E_CAPACITY, "contact segment capacity exceeded");
   if (e & WERR_NEWPAIRS) return set_error(ctx, B2G_E_CAPACITY, "too many new pairs in one step");
   if (e & WERR_UNSUPPORTED) return set_error(ctx, B2G_E_UNSUPPORTED, "unsupported shape pair");
+  if (e & WERR_SCHEDULE) return set_error(ctx, B2G_E_HIP, "island solver pass limit reached (internal error)");
   return B2G_OK;
 }
 
@@ -2339,6 +2705,44 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n) {
   if (!ctx || !buf || n == 0) return B2G_E_INVALID;
   std::snprintf(buf, n, "%s", ctx->err.c_str());
   return B2G_OK;
+}
+
+B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value) {
+  if (!ctx) return B2G_E_INVALID;
+  int* slot = nullptr;
+  int lo = 0, hi = 0;
+  switch (key) {
+    case B2G_PARAM_HEAVY_CLASS: slot = &ctx->heavy_class; lo = 0; hi = kSizeClasses; break;
+    case B2G_PARAM_TEAM_FULL_CLASS: slot = &ctx->team_full_class; lo = 0; hi = kSizeClasses; break;
+    case B2G_PARAM_TEAM_LDS: slot = &ctx->team_lds; lo = 0; hi = kMaxLds; break;
+    case B2G_PARAM_SOLVE_GRID: slot = &ctx->solve_grid; lo = 1; hi = 1 << 30; break;
+    default: return set_error(ctx, B2G_E_INVALID, "b2g_set_param: unknown key " + std::to_string(key));
+  }
+  if (key == B2G_PARAM_TEAM_LDS && value % 256 != 0)
+    return set_error(ctx, B2G_E_INVALID, "b2g_set_param: the team LDS budget must be a multiple of 256");
+  if (value < lo || value > hi)
+    return set_error(ctx, B2G_E_INVALID, "b2g_set_param: value " + std::to_string(value) + " outside [" +
+                                             std::to_string(lo) + ", " + std::to_string(hi) + "]");
+  if (*slot != value) {
+    if (ctx->stream) {
+      hipSetDevice(ctx->device);
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    *slot = value;
+    drop_graph(ctx);  // kernel arguments are baked into a captured step
+  }
+  return B2G_OK;
+}
+
+B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value) {
+  if (!ctx || !value) return B2G_E_INVALID;
+  switch (key) {
+    case B2G_PARAM_HEAVY_CLASS: *value = ctx->heavy_class; return B2G_OK;
+    case B2G_PARAM_TEAM_FULL_CLASS: *value = ctx->team_full_class; return B2G_OK;
+    case B2G_PARAM_TEAM_LDS: *value = ctx->team_lds; return B2G_OK;
+    case B2G_PARAM_SOLVE_GRID: *value = ctx->solve_grid; return B2G_OK;
+    default: return set_error(ctx, B2G_E_INVALID, "b2g_get_param: unknown key " + std::to_string(key));
+  }
 }
 
 B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, float* ms_out) {

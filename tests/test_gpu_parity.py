@@ -187,3 +187,46 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(sa.con_d, sb.con_d) and np.array_equal(sa.con_i, sb.con_i)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_island_launch_shapes_agree_bit_exactly(prec):
+    """The wavefront-per-island path, the lane-per-island path and the LDS
+    fallback run the same solves in the same order: identical results."""
+    counts = [420, 300, 512, 180, 96, 260]
+    batch = builder.scenes_batch([scenes.batch_world(9, w, n) for w, n in enumerate(counts)])
+    e = N.Engine(prec)
+    e.upload(batch)
+    e.step_n(150, DT, 8, 3, graph=True)  # piles have formed
+    state = e.download()
+    e.close()
+    state.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    results = {}
+    for name, params in (("lanes", {"heavy_class": 16}), ("teams64", {"heavy_class": 0, "team_full_class": 0}),
+                         ("teams8", {"heavy_class": 0, "team_full_class": 16}), ("default", {}),
+                         ("fallback", {"heavy_class": 3, "team_lds": 4096})):
+        e = N.Engine(prec)
+        for k, v in params.items():
+            e.set_param(k, v)
+        e.upload(state)
+        for _ in range(3):
+            e.step(DT, 8, 3)
+        e.step_n(3, DT, 8, 3, graph=True)
+        results[name] = e.download()
+        print(prec, name, "done", flush=True)
+        e.close()
+    ref = results["lanes"]
+    for name, st in results.items():
+        assert np.array_equal(st.body_d, ref.body_d), name
+        assert np.array_equal(st.body_i, ref.body_i), name
+        assert np.array_equal(st.con_d, ref.con_d) and np.array_equal(st.con_i, ref.con_i), name
+
+
+def test_set_param_rejects_bad_values():
+    e = N.Engine("fp32")
+    assert e.get_param("heavy_class") >= 0
+    for k, v in (("heavy_class", 17), ("team_lds", 200000), ("team_lds", 1000), ("solve_grid", 0),
+                 ("team_full_class", -1)):
+        with pytest.raises(N.B2GError):
+            e.set_param(k, v)
+    e.close()
