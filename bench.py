@@ -73,32 +73,51 @@ def algorithmic_bytes(kernel: str, c: dict, fp64: bool) -> float:
     return 0.0
 
 
-def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, profile_steps: int):
+def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, profile_steps: int, tune: bool = True):
+    """Times `steps` steps; the launch-shape tuner (real steps, timed with the rest) runs every 200 steps."""
     from pypybox2d_amd._native import Engine
+    from pypybox2d_amd.autotune import LaunchTuner
     eng = Engine(precision)
     eng.upload(batch)
+    tuner = LaunchTuner(eng) if tune else None
     log("[%s] uploaded; warmup %d steps" % (precision, warmup))
     eng.step_n(warmup, graph=True)
     eng.sync()
-    log("[%s] timing %d graph-captured steps" % (precision, steps))
+    log("[%s] timing %d steps (graph-captured; launch shape re-tuned every 200 steps)" % (precision, steps))
     t0 = time.perf_counter()
     done = 0
     chunk = max(1, min(steps, 200))
+    choices = []
     while done < steps:
         n = min(chunk, steps - done)
-        eng.step_n(n, graph=True)
+        if tuner is not None:
+            tuner.step_n(n)
+        else:
+            eng.step_n(n, graph=True)
         eng.sync()
         done += n
-        log("[%s] %d/%d steps, %.2f s" % (precision, done, steps, time.perf_counter() - t0))
+        if tuner is not None and tuner.best is not None:
+            choices.append(dict(tuner.best))
+        log("[%s] %d/%d steps, %.2f s%s" % (precision, done, steps, time.perf_counter() - t0,
+                                           (", launch shape %s" % tuner.best) if tuner else ""))
     elapsed = time.perf_counter() - t0
     ms = eng.profile(profile_steps) if profile_steps > 0 else np.zeros(5)
     state = eng.download()
     eng.close()
-    return elapsed, ms, state
+    return elapsed, ms, state, choices
+
+
+G_DT = 10.0 / 60
 
 
 def one_step_error(state: L.BatchArrays, precision: str, worlds):
-    """Max relative one-step error of the GPU pipeline vs the FP64 oracle from an identical state."""
+    """Relative one-step error of a GPU pipeline vs the FP64 oracle from an identical state.
+
+    Per body and quantity |gpu - ref| / max(|ref|, floor) with floors 1 m, 1 rad,
+    |g| dt and |g| dt / 1 m; accumulated impulses of contacts present in both
+    with floor m_min |g| dt.  Reports max, 99.9th and 99th percentiles and the
+    fraction of bodies above 1e-5.
+    """
     from oracle import oracle as O
     from pypybox2d_amd._native import Engine
     sample = L.concat([state.world(w) for w in worlds])
@@ -113,21 +132,58 @@ def one_step_error(state: L.BatchArrays, precision: str, worlds):
     e.close()
     out = {}
     g, r = got.body_d, ref.body_d
+    dyn = sample.body_i[:, L.BI_TYPE] == L.DYNAMIC
     for name, cols, floor in (("position", [L.BD_C_X, L.BD_C_Y], 1.0), ("angle", [L.BD_A], 1.0),
-                              ("velocity", [L.BD_V_X, L.BD_V_Y], 10.0 / 60),
-                              ("angular_velocity", [L.BD_W], 10.0 / 60)):
-        out[name] = float(np.max(np.abs(g[:, cols] - r[:, cols]) / np.maximum(np.abs(r[:, cols]), floor)))
+                              ("velocity", [L.BD_V_X, L.BD_V_Y], G_DT),
+                              ("angular_velocity", [L.BD_W], G_DT)):
+        e_ = np.max(np.abs(g[:, cols] - r[:, cols]) / np.maximum(np.abs(r[:, cols]), floor), axis=1)[dyn]
+        out[name] = {"max": float(e_.max()), "p99.9": float(np.percentile(e_, 99.9)),
+                     "p99": float(np.percentile(e_, 99)), "frac_over_1e-5": float(np.mean(e_ > 1e-5))}
+    # accumulated impulses of contacts present in both with the same discrete state
+    imp = []
+    flips = 0
+    bm = ref.body_d[:, L.BD_MASS]
+    for w in range(sample.n_worlds):
+        rc = {tuple(ref.con_i[k, :2]): k for k in range(ref.con_off[w], ref.con_off[w + 1])}
+        b0, f0 = ref.body_off[w], ref.fix_off[w]
+        for k in range(got.con_off[w], got.con_off[w + 1]):
+            j = rc.get(tuple(got.con_i[k, :2]))
+            if j is None or got.con_i[k, L.CI_PC] != ref.con_i[j, L.CI_PC] or \
+                    (got.con_i[k, L.CI_FLAGS] & 3) != (ref.con_i[j, L.CI_FLAGS] & 3):
+                flips += 1
+                continue
+            ms = [m for m in (bm[b0 + ref.fix_i[f0 + ref.con_i[j, L.CI_FA], L.FI_BODY]],
+                              bm[b0 + ref.fix_i[f0 + ref.con_i[j, L.CI_FB], L.FI_BODY]]) if m > 0]
+            floor = (min(ms) if ms else 1.0) * G_DT
+            for col in ((L.CD_P0_NI, L.CD_P0_TI, L.CD_P1_NI, L.CD_P1_TI)[:2 * int(ref.con_i[j, L.CI_PC])]):
+                imp.append(abs(got.con_d[k, col] - ref.con_d[j, col]) / max(abs(ref.con_d[j, col]), floor))
+    imp = np.array(imp) if imp else np.zeros(1)
+    out["impulse"] = {"max": float(imp.max()), "p99.9": float(np.percentile(imp, 99.9)),
+                      "p99": float(np.percentile(imp, 99)), "frac_over_1e-5": float(np.mean(imp > 1e-5))}
+    out["contact_discrete_flips"] = flips
     same = all(np.array_equal(got.con_i[got.con_off[w]:got.con_off[w + 1], :2],
                               ref.con_i[ref.con_off[w]:ref.con_off[w + 1], :2]) for w in range(sample.n_worlds))
-    out["contact_lists_equal"] = bool(same)
+    out["pair_lists_equal"] = bool(same)
     out["worlds"] = len(worlds)
+    out["dynamic_bodies"] = int(dyn.sum())
     return out
 
 
-def cpu_baseline(batch: L.BatchArrays, worlds, seconds: float):
-    """The FP64 oracle stepping a sample of worlds on one host core."""
+def long_horizon(state: L.BatchArrays) -> dict:
+    """Kinetic energy and mean height of the dynamic bodies (stacking is chaotic, so trajectories
+    of different precisions are compared through such statistics)."""
+    bd = state.body_d
+    dyn = state.body_i[:, L.BI_TYPE] == L.DYNAMIC
+    m, I = bd[dyn, L.BD_MASS], bd[dyn, L.BD_I]
+    ke = 0.5 * (m * (bd[dyn, L.BD_V_X] ** 2 + bd[dyn, L.BD_V_Y] ** 2) + I * bd[dyn, L.BD_W] ** 2)
+    return {"kinetic_energy_per_body": float(ke.sum() / max(1, dyn.sum())),
+            "mean_height": float(bd[dyn, L.BD_C_Y].mean()),
+            "awake_fraction": float(np.mean((state.body_i[dyn, L.BI_FLAGS] & L.BF_AWAKE) != 0))}
+
+
+def _cpu_worker(args):
+    sample, seconds = args
     from oracle import oracle as O
-    sample = L.concat([batch.world(w) for w in worlds])
     dyn = int(np.count_nonzero(sample.body_i[:, L.BI_TYPE] == L.DYNAMIC))
     o = O.Oracle(sample)
     steps = 0
@@ -135,10 +191,45 @@ def cpu_baseline(batch: L.BatchArrays, worlds, seconds: float):
     while time.perf_counter() - t0 < seconds:
         o.step(1.0 / 60, 8, 3)
         steps += 1
-    el = time.perf_counter() - t0
-    return {"value": dyn * steps / el, "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": "FP64 oracle (oracle/b2oracle.c), %d worlds (%d dynamic bodies) of the same batch, "
-                      "%d steps from the initial state, one host core" % (len(worlds), dyn, steps)}
+    return dyn * steps, time.perf_counter() - t0, steps
+
+
+def cpu_baseline(batch: L.BatchArrays, seconds: float, procs: int):
+    """The FP64 oracle (oracle/b2oracle.c) on the host cores: one world per process."""
+    import multiprocessing as mp
+    worlds = list(range(min(procs, batch.n_worlds)))
+    jobs = [(batch.world(w), seconds) for w in worlds]
+    if procs > 1:
+        with mp.get_context("fork").Pool(procs) as pool:
+            res = pool.map(_cpu_worker, jobs)
+    else:
+        res = [_cpu_worker(j) for j in jobs]
+    rate = sum(bs / el for bs, el, _ in res)
+    dyn = sum(int(np.count_nonzero(batch.world(w).body_i[:, L.BI_TYPE] == L.DYNAMIC)) for w in worlds)
+    return {"value": rate, "unit": "body-steps/s", "cores": len(worlds), "kind": "port",
+            "sample": "FP64 oracle (oracle/b2oracle.c, a C restatement of the reference's step), worlds 0-%d of "
+                      "the same batch (%d dynamic bodies), one world per process on %d host cores, %.0f s from the "
+                      "initial state (%d-%d steps per world)" % (len(worlds) - 1, dyn, len(worlds), seconds,
+                                                                   min(r[2] for r in res), max(r[2] for r in res))}
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary."""
+    path = os.path.join(HERE, "profiles", "%s_mixed4096_traffic.txt" % kernel)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def roofline(kernel: str, ms: float, counts: dict, fp64: bool) -> dict:
+    ab = algorithmic_bytes(kernel, counts, fp64)
+    achieved = ab / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel), "algorithmic_bytes": ab,
+            "avg_ms": ms}
 
 
 def main():
@@ -151,8 +242,11 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None, help="timed steps of the FP64 run (default: --steps)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default solver launch shape")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--error-worlds", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
@@ -167,15 +261,12 @@ def main():
     log("built in %.1f s" % (time.perf_counter() - t0))
     dyn = int(np.count_nonzero(batch.body_i[:, L.BI_TYPE] == L.DYNAMIC))
 
-    el32, ms32, st32 = run_gpu(batch, "fp32", args.steps, args.warmup, args.profile_steps)
+    el32, ms32, st32, tuned32 = run_gpu(batch, "fp32", args.steps, args.warmup, args.profile_steps,
+                                        not args.no_tune)
     value = dyn * args.steps / el32
     log("fp32: %.3e body-steps/s (%.3f ms/step)" % (value, 1e3 * el32 / args.steps))
     c = workload_counts(st32)
     k = int(np.argmax(ms32))
-    dom = KERNELS[k]
-    dom_ms = float(ms32[k])
-    ab = algorithmic_bytes(dom, c, fp64=False)
-    achieved = ab / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     line = {
         "metric": "body-steps/sec, 4096 mixed worlds, FP32 vs FP64 on GPU; one-step err vs FP64",
         "value": value, "unit": "body-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -187,26 +278,35 @@ def main():
                    "worlds": args.worlds, "dynamic_bodies": dyn, "contacts_at_end": c["contacts"],
                    "touching_at_end": c["touching"]},
         "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms32)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": ab},
+        "roofline": roofline(KERNELS[k], float(ms32[k]), c, False),
+        "roofline_other": {n: roofline(n, float(ms32[i]), c, False) for i, n in enumerate(KERNELS)
+                           if n in ("k_collide", "k_pairs", "k_sync")},
+        "launch_shape": tuned32[-1] if tuned32 else None,
     }
     if not args.no_fp64:
         n64 = args.fp64_steps or args.steps
-        el64, ms64, st64 = run_gpu(batch, "fp64", n64, args.warmup, args.profile_steps)
+        el64, ms64, st64, tuned64 = run_gpu(batch, "fp64", n64, args.warmup, args.profile_steps, not args.no_tune)
         v64 = dyn * n64 / el64
         line["fp64"] = {"value": v64, "ms_per_step": 1e3 * el64 / n64, "steps": n64,
-                        "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms64)}}
+                        "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms64)},
+                        "launch_shape": tuned64[-1] if tuned64 else None}
         line["fp32_over_fp64"] = value / v64
         log("fp64: %.3e body-steps/s; fp32/fp64 = %.2f" % (v64, value / v64))
-    try:
-        rng = np.random.default_rng(0)
-        worlds = sorted(rng.choice(args.worlds, size=min(16, args.worlds), replace=False).tolist())
-        line["one_step_error_fp32_vs_fp64_oracle"] = one_step_error(st32, "fp32", worlds)
-        log("one-step error:", line["one_step_error_fp32_vs_fp64_oracle"])
-    except Exception as exc:  # the error check never hides the measurement
-        line["one_step_error_fp32_vs_fp64_oracle"] = {"error": str(exc)}
+        if n64 == args.steps:
+            line["long_horizon"] = {"fp32": long_horizon(st32), "fp64": long_horizon(st64),
+                                    "steps": args.warmup + args.steps}
+    rng = np.random.default_rng(0)
+    worlds = sorted(rng.choice(args.worlds, size=min(args.error_worlds, args.worlds), replace=False).tolist())
+    for prec, st in (("fp32", st32),) + ((("fp64", st64),) if not args.no_fp64 else ()):
+        key = "one_step_error_%s_vs_fp64_oracle" % prec
+        try:
+            line[key] = one_step_error(st, prec, worlds)
+            line[key]["from_step"] = args.warmup + ((args.fp64_steps or args.steps) if prec == "fp64" else args.steps)
+            log("%s one-step error:" % prec, line[key])
+        except Exception as exc:  # the error check never hides the measurement
+            line[key] = {"error": str(exc)}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(batch, list(range(min(8, args.worlds))), args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(batch, args.cpu_seconds, args.cpu_procs)
         log("cpu baseline: %.3e body-steps/s" % line["cpu_baseline"]["value"])
     print(json.dumps(line), flush=True)
 

@@ -103,8 +103,8 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
      B2G_PARAM_TEAM_LDS         LDS bytes per wavefront, shared by its teams,
                                 a multiple of 256 up to 163840; an island whose
                                 state needs more than its team's share (24 B
-                                per body, 4 per body rounded to even bodies, 12
-                                per contact) is solved by one lane instead
+                                per body and 4 per body rounded to even
+                                bodies) is solved by one lane instead
      B2G_PARAM_SOLVE_GRID       most wavefronts launched for the solver, >= 1 */
 #define B2G_PARAM_TEAM_LDS 1
 #define B2G_PARAM_HEAVY_CLASS 2

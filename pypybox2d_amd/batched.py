@@ -12,16 +12,19 @@ import numpy as np
 
 from . import layout as L
 from ._native import Engine
+from .autotune import LaunchTuner
 from .builder import scenes_batch
 
 
 class BatchedWorld(object):
     def __init__(self, batch: L.BatchArrays, precision: str = "fp32", device: int = 0,
-                 contact_factor: float = 8.0):
+                 contact_factor: float = 8.0, autotune: bool = True):
         self.batch = batch.normalized()
         self.engine = Engine(precision, device, contact_factor=contact_factor)
         self.engine.upload(self.batch)
         self.precision = precision
+        # the solver's launch shape changes its speed, never its results
+        self.tuner = LaunchTuner(self.engine) if autotune else None
 
     @classmethod
     def from_scenes(cls, scenes: List[dict], precision: str = "fp32", device: int = 0) -> "BatchedWorld":
@@ -39,6 +42,8 @@ class BatchedWorld(object):
              graph: bool = True) -> None:
         if n == 1 and not graph:
             self.engine.step(dt, vel_iters, pos_iters)
+        elif self.tuner is not None and graph:
+            self.tuner.step_n(n, dt, vel_iters, pos_iters)
         else:
             self.engine.step_n(n, dt, vel_iters, pos_iters, graph)
 

@@ -97,7 +97,9 @@ template <typename T>
 struct VC {
   T nx, ny, friction, restitution, ma, mb, Ia, Ib;
   double k11, k12, k22, n11, n12, n21, n22;  // K and K^-1 (n11 = c1x, n21 = c1y, n12 = c2x, n22 = c2y)
-  int ia, ib, npts, pad;
+  int ia, ib;
+  int rka, rkb;  // team path: rank | degree << 16 of the contact in body A's (B's) island sequence
+  int npts, pad;
   VCP<T> p[2];
 };
 template <typename T>
@@ -105,6 +107,7 @@ struct PC {
   T lp0x, lp0y, lp1x, lp1y, lnx, lny, lpx, lpy;
   T ra, rb, ma, mb, Ia, Ib, lcax, lcay, lcbx, lcby;
   int type, npts, ia, ib;
+  int rka, rkb;         // as in VC
   double scx, scy, sa;  // position of the contact's static body (static bodies never move)
 };
 
@@ -1037,6 +1040,7 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
   vc.Ib = d.b_iI[bb];
   vc.npts = pc;
   vc.pad = 0;
+  vc.rka = vc.rkb = 0;
   const bool warm = c.warm;
   const double dt_ratio = c.dt_ratio;
   vc.p[0].ni = warm ? dt_ratio * (double)imp.x : (double)imp.x;
@@ -1054,6 +1058,7 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
   pcn.npts = pc;
   pcn.ia = ia;
   pcn.ib = ib;
+  pcn.rka = pcn.rkb = 0;
   double cax, cay, aa, cbx, cby, ab;
   load_pos(d, bstart, b0, ia, cax, cay, aa);
   load_pos(d, bstart, b0, ib, cbx, cby, ab);
@@ -1260,10 +1265,8 @@ __host__ __device__ inline int team_size(int cls, int full_cls) {
   return kSolveBlock >> sh;
 }
 
-__host__ __device__ inline int team_lds_bytes(int nbi, int nci) {
-  // 3 doubles and an int counter per body (bodies padded to even), ushort4
-  // ranks and degrees and short2 bodies per contact
-  return 24 * nbi + 4 * ((nbi + 1) & ~1) + 12 * nci;
+__host__ __device__ inline int team_lds_bytes(int nbi) {
+  return 24 * nbi + 4 * ((nbi + 1) & ~1);  // 3 doubles and an int counter per body
 }
 
 // The lanes of one wavefront exchange body state through LDS between passes.
@@ -1297,10 +1300,8 @@ template <typename T>
 __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci, int last_allow,
                                   char* lds, int t, int TS) {
   const IslandK c = island_k(d, w, bstart);
-  double* sx = (double*)lds;                         // [3 * nbi]
-  int* cnt = (int*)(sx + 3 * nbi);                   // [nbi]
-  ushort4* rk = (ushort4*)(cnt + ((nbi + 1) & ~1));  // [nci]: rank and degree for body A, for body B
-  short2* ab = (short2*)(rk + nci);                  // [nci]: island-local bodies
+  double* sx = (double*)lds;        // [3 * nbi]
+  int* cnt = (int*)(sx + 3 * nbi);  // [nbi]
   // 1. integrate velocities
   for (int i = t; i < nbi; i += TS) {
     PV<T> p = integrate_velocity(d, c, bstart, i);
@@ -1315,16 +1316,20 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
   for (int k = t; k < nci; k += TS) {
     int ia, ib;
     init_contact(d, c, bstart, cstart, k, ia, ib);
-    ab[k] = make_short2((short)(ia < 0 ? -1 : ia), (short)(ib < 0 ? -1 : ib));
     if (ia >= 0) atomicAdd(&cnt[ia], 1);
     if (ib >= 0) atomicAdd(&cnt[ib], 1);
   }
   team_sync();
+  // ranks and degrees into the records (the lane that wrote them reads them)
   for (int k = t; k < nci; k += TS) {
-    const short2 x = ab[k];
+    const int2 x = d.i_cidx[cstart + k];
     const uint32_t r = d.i_rank[cstart + k].x;
-    rk[k] = make_ushort4((unsigned short)(r & 0xFFFFu), x.x >= 0 ? (unsigned short)cnt[x.x] : 0,
-                         (unsigned short)(r >> 16), x.y >= 0 ? (unsigned short)cnt[x.y] : 0);
+    const int rka = (int)(r & 0xFFFFu) | (x.x >= 0 ? cnt[x.x] << 16 : 0);
+    const int rkb = (int)(r >> 16) | (x.y >= 0 ? cnt[x.y] << 16 : 0);
+    d.vc[cstart + k].rka = rka;
+    d.vc[cstart + k].rkb = rkb;
+    d.pc[cstart + k].rka = rka;
+    d.pc[cstart + k].rkb = rkb;
   }
   team_sync();
   for (int i = t; i < nbi; i += TS) cnt[i] = 0;
@@ -1346,9 +1351,9 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
     while (wave_any(active && pass < cap)) {
       ++pass;
       if (active) {
-        const ushort4 r = rk[k];
         const int ia = cur.ia, ib = cur.ib;
-        const int ra = s * (int)r.y + (int)r.x, rb = s * (int)r.w + (int)r.z;
+        const int ra = s * (int)((unsigned)cur.rka >> 16) + (cur.rka & 0xFFFF);
+        const int rb = s * (int)((unsigned)cur.rkb >> 16) + (cur.rkb & 0xFFFF);
         const bool ready = (ia < 0 || lds_ld(&cnt[ia]) == ra) && (ib < 0 || lds_ld(&cnt[ib]) == rb);
         if (ready) {
           V2d va = v2<double>(0, 0), vb = v2<double>(0, 0);
@@ -1419,9 +1424,9 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
       while (wave_any(active && pass < pcap)) {
         ++pass;
         if (active) {
-          const ushort4 r = rk[k];
           const int ia = cur.ia, ib = cur.ib;
-          const bool ready = (ia < 0 || lds_ld(&cnt[ia]) == (int)r.x) && (ib < 0 || lds_ld(&cnt[ib]) == (int)r.z);
+          const int ra = cur.rka & 0xFFFF, rb = cur.rkb & 0xFFFF;
+          const bool ready = (ia < 0 || lds_ld(&cnt[ia]) == ra) && (ib < 0 || lds_ld(&cnt[ib]) == rb);
           if (ready) {
             double ax, ay, aa, bx, by, bb;
             if (ia >= 0) { ax = sx[3 * ia]; ay = sx[3 * ia + 1]; aa = sx[3 * ia + 2]; }
@@ -1429,8 +1434,8 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
             if (ib >= 0) { bx = sx[3 * ib]; by = sx[3 * ib + 1]; bb = sx[3 * ib + 2]; }
             else { bx = cur.scx; by = cur.scy; bb = cur.sa; }
             solve_contact_position(cur, ax, ay, aa, bx, by, bb, min_sep);
-            if (ia >= 0) { sx[3 * ia] = ax; sx[3 * ia + 1] = ay; sx[3 * ia + 2] = aa; lds_st(&cnt[ia], (int)r.x + 1); }
-            if (ib >= 0) { sx[3 * ib] = bx; sx[3 * ib + 1] = by; sx[3 * ib + 2] = bb; lds_st(&cnt[ib], (int)r.z + 1); }
+            if (ia >= 0) { sx[3 * ia] = ax; sx[3 * ia + 1] = ay; sx[3 * ia + 2] = aa; lds_st(&cnt[ia], ra + 1); }
+            if (ib >= 0) { sx[3 * ib] = bx; sx[3 * ib + 1] = by; sx[3 * ib + 2] = bb; lds_st(&cnt[ib], rb + 1); }
             k += TS;
             if (k >= nci) {
               active = false;
@@ -1474,8 +1479,10 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
 // team's first lane alone.  The remaining islands are dealt 64 per wavefront,
 // one lane each.  Wavefronts are taken in queue order, so the longest islands
 // start first and are spread over the whole chip.
+// Two wavefronts per SIMD at least: the FP64 records would otherwise take
+// more than 256 VGPRs and halve the resident wavefronts.
 template <typename T>
-__global__ void __launch_bounds__(kSolveBlock) k_solve(Dev<T> d) {
+__global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_eu(2))) k_solve(Dev<T> d) {
   extern __shared__ __align__(16) char team_lds[];
   const int n = *d.q_count;
   const int nh = d.q_cls[kQHeavy];
@@ -1497,7 +1504,7 @@ __global__ void __launch_bounds__(kSolveBlock) k_solve(Dev<T> d) {
         const int nc = d.qs_nc[q];
         const int nci = nc & 0x3FFFFFFF, last_allow = (nc >> 30) & 1;
         const int share = d.team_lds / per;
-        if (team_lds_bytes(in.z, nci) <= share) {
+        if (team_lds_bytes(in.z) <= share) {
           solve_island_team(d, in.x, in.y, in.z, in.w, nci, last_allow, team_lds + team * share, t, TS);
         } else if (t == 0) {
           solve_island(d, in.x, in.y, in.z, in.w, nci, last_allow);
