@@ -3,16 +3,19 @@
 
 Workload (default): 4096 independent mixed worlds (body counts uniform in
 [32, 512], seed 1; boxes / circles / convex polygons 40/30/30 above a static
-edge), dt = 1/60, 8 velocity and 3 position iterations, HIP-graph-captured
-steps, continuous physics off.  A step is one World.step of every world.
+edge), dt = 1/60, 8 velocity and 3 position iterations, continuous physics
+off.  A step is one World.step of every world.
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 10]
 
 Prints one JSON line: value = dynamic bodies x timed steps / wall seconds of
 the timed loop (FP32-bulk pipeline, state resident in HBM), plus the FP64
-pipeline's rate, the dominant kernel's roofline, a one-step error against the
-FP64 oracle on a sample of worlds, and the oracle's CPU rate on a bounded
-sample of the same workload.
+pipeline's rate, the dominant kernel's roofline, one-step errors against the
+FP64 oracle on a sample of worlds, long-horizon statistics, and the oracle's
+rate on the host cores for the same worlds and step count.  Timed steps are
+launched with HIP events between their kernels (for the per-kernel averages);
+the rate of HIP-graph-captured steps, measured right after, is reported
+beside it.
 """
 from __future__ import annotations
 
@@ -30,6 +33,7 @@ sys.path.insert(0, HERE)
 from pypybox2d_amd import builder, layout as L, scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+GRAPH_CHECK_STEPS = 100
 KERNELS = ("k_collide", "k_islands", "k_solve", "k_sync", "k_pairs")
 
 
@@ -73,38 +77,49 @@ def algorithmic_bytes(kernel: str, c: dict, fp64: bool) -> float:
     return 0.0
 
 
-def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, profile_steps: int, tune: bool = True):
-    """Times `steps` steps; the launch-shape tuner (real steps, timed with the rest) runs every 200 steps."""
+def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, tune: bool = True):
+    """Times `steps` steps and the average duration of each kernel over them.
+
+    Every timed step is launched with HIP events between its kernels on the
+    library's stream (b2g_profile_step), so the per-kernel averages cover exactly
+    the timed launches.  Every 200 steps the launch-shape tuner takes the next
+    steps, one per candidate shape (real steps, timed with the rest).
+    """
     from pypybox2d_amd._native import Engine
     from pypybox2d_amd.autotune import LaunchTuner
     eng = Engine(precision)
     eng.upload(batch)
     tuner = LaunchTuner(eng) if tune else None
     log("[%s] uploaded; warmup %d steps" % (precision, warmup))
-    eng.step_n(warmup, graph=True)
+    if warmup > 0:
+        eng.step_n(warmup, graph=True)
     eng.sync()
-    log("[%s] timing %d steps (graph-captured; launch shape re-tuned every 200 steps)" % (precision, steps))
+    log("[%s] timing %d steps%s" % (precision, steps, " (launch shape re-tuned every 200 steps)" if tune else ""))
+    sums = np.zeros(5)
     t0 = time.perf_counter()
     done = 0
     chunk = max(1, min(steps, 200))
-    choices = []
     while done < steps:
         n = min(chunk, steps - done)
-        if tuner is not None:
-            tuner.step_n(n)
-        else:
-            eng.step_n(n, graph=True)
-        eng.sync()
+        if tuner is not None and n > len(tuner.candidates):
+            tuner.tune(1.0 / 60, 8, 3)
+            sums += tuner.last_ms.sum(axis=0)
+            done += len(tuner.candidates)
+            n -= len(tuner.candidates)
+        sums += eng.profile(n) * n
         done += n
-        if tuner is not None and tuner.best is not None:
-            choices.append(dict(tuner.best))
         log("[%s] %d/%d steps, %.2f s%s" % (precision, done, steps, time.perf_counter() - t0,
                                            (", launch shape %s" % tuner.best) if tuner else ""))
+    eng.sync()
     elapsed = time.perf_counter() - t0
-    ms = eng.profile(profile_steps) if profile_steps > 0 else np.zeros(5)
     state = eng.download()
+    # the same launch sequence captured in a HIP graph, continuing from here
+    g0 = time.perf_counter()
+    eng.step_n(GRAPH_CHECK_STEPS, graph=True)
+    eng.sync()
+    graph_ms = 1e3 * (time.perf_counter() - g0) / GRAPH_CHECK_STEPS
     eng.close()
-    return elapsed, ms, state, choices
+    return elapsed, sums / steps, state, (dict(tuner.best) if tuner and tuner.best else None), graph_ms
 
 
 G_DT = 10.0 / 60
@@ -181,36 +196,46 @@ def long_horizon(state: L.BatchArrays) -> dict:
             "awake_fraction": float(np.mean((state.body_i[dyn, L.BI_FLAGS] & L.BF_AWAKE) != 0))}
 
 
+_CPU_BATCH = None
+
+
 def _cpu_worker(args):
-    sample, seconds = args
+    """Whole trajectories of worlds p, p + P, ... (the GPU's step count each) until `seconds` pass."""
+    p, nproc, steps, seconds = args
     from oracle import oracle as O
-    dyn = int(np.count_nonzero(sample.body_i[:, L.BI_TYPE] == L.DYNAMIC))
-    o = O.Oracle(sample)
-    steps = 0
     t0 = time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        o.step(1.0 / 60, 8, 3)
-        steps += 1
-    return dyn * steps, time.perf_counter() - t0, steps
+    body_steps, worlds = 0, 0
+    for w in range(p, _CPU_BATCH.n_worlds, nproc):
+        sample = _CPU_BATCH.world(w)
+        o = O.Oracle(sample)
+        for _ in range(steps):
+            o.step(1.0 / 60, 8, 3)
+        body_steps += int(np.count_nonzero(sample.body_i[:, L.BI_TYPE] == L.DYNAMIC)) * steps
+        worlds += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    return body_steps, time.perf_counter() - t0, worlds
 
 
-def cpu_baseline(batch: L.BatchArrays, seconds: float, procs: int):
-    """The FP64 oracle (oracle/b2oracle.c) on the host cores: one world per process."""
+def cpu_baseline(batch: L.BatchArrays, steps: int, seconds: float, procs: int):
+    """The FP64 oracle (oracle/b2oracle.c) on the host cores: one world per process at a time, each
+    world stepped from the same initial state for the same number of steps as the GPU run."""
     import multiprocessing as mp
-    worlds = list(range(min(procs, batch.n_worlds)))
-    jobs = [(batch.world(w), seconds) for w in worlds]
+    global _CPU_BATCH
+    _CPU_BATCH = batch
+    jobs = [(p, procs, steps, seconds) for p in range(procs)]
     if procs > 1:
         with mp.get_context("fork").Pool(procs) as pool:
             res = pool.map(_cpu_worker, jobs)
     else:
         res = [_cpu_worker(j) for j in jobs]
     rate = sum(bs / el for bs, el, _ in res)
-    dyn = sum(int(np.count_nonzero(batch.world(w).body_i[:, L.BI_TYPE] == L.DYNAMIC)) for w in worlds)
-    return {"value": rate, "unit": "body-steps/s", "cores": len(worlds), "kind": "port",
-            "sample": "FP64 oracle (oracle/b2oracle.c, a C restatement of the reference's step), worlds 0-%d of "
-                      "the same batch (%d dynamic bodies), one world per process on %d host cores, %.0f s from the "
-                      "initial state (%d-%d steps per world)" % (len(worlds) - 1, dyn, len(worlds), seconds,
-                                                                   min(r[2] for r in res), max(r[2] for r in res))}
+    nw = sum(r[2] for r in res)
+    return {"value": rate, "unit": "body-steps/s", "cores": procs, "kind": "port",
+            "sample": "FP64 oracle (oracle/b2oracle.c, a C restatement of the reference's step) on %d host cores, "
+                      "one process per core, each stepping whole worlds of the same batch (worlds p, p+%d, ...) "
+                      "for %d steps from the initial state until %.0f s had passed: %d worlds in total"
+                      % (procs, procs, steps, seconds, nw)}
 
 
 def measured_traffic(kernel: str):
@@ -243,7 +268,6 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default solver launch shape")
-    ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--error-worlds", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1))
@@ -261,8 +285,7 @@ def main():
     log("built in %.1f s" % (time.perf_counter() - t0))
     dyn = int(np.count_nonzero(batch.body_i[:, L.BI_TYPE] == L.DYNAMIC))
 
-    el32, ms32, st32, tuned32 = run_gpu(batch, "fp32", args.steps, args.warmup, args.profile_steps,
-                                        not args.no_tune)
+    el32, ms32, st32, tuned32, gms32 = run_gpu(batch, "fp32", args.steps, args.warmup, not args.no_tune)
     value = dyn * args.steps / el32
     log("fp32: %.3e body-steps/s (%.3f ms/step)" % (value, 1e3 * el32 / args.steps))
     c = workload_counts(st32)
@@ -281,15 +304,17 @@ def main():
         "roofline": roofline(KERNELS[k], float(ms32[k]), c, False),
         "roofline_other": {n: roofline(n, float(ms32[i]), c, False) for i, n in enumerate(KERNELS)
                            if n in ("k_collide", "k_pairs", "k_sync")},
-        "launch_shape": tuned32[-1] if tuned32 else None,
+        "launch_shape": tuned32,
+        "graph_ms_per_step_next_%d" % GRAPH_CHECK_STEPS: gms32,
+        "timing": "wall clock of the timed steps; per-kernel averages from HIP events around every timed launch",
     }
     if not args.no_fp64:
         n64 = args.fp64_steps or args.steps
-        el64, ms64, st64, tuned64 = run_gpu(batch, "fp64", n64, args.warmup, args.profile_steps, not args.no_tune)
+        el64, ms64, st64, tuned64, gms64 = run_gpu(batch, "fp64", n64, args.warmup, not args.no_tune)
         v64 = dyn * n64 / el64
         line["fp64"] = {"value": v64, "ms_per_step": 1e3 * el64 / n64, "steps": n64,
                         "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms64)},
-                        "launch_shape": tuned64[-1] if tuned64 else None}
+                        "launch_shape": tuned64, "graph_ms_per_step_next_%d" % GRAPH_CHECK_STEPS: gms64}
         line["fp32_over_fp64"] = value / v64
         log("fp64: %.3e body-steps/s; fp32/fp64 = %.2f" % (v64, value / v64))
         if n64 == args.steps:
@@ -297,7 +322,7 @@ def main():
                                     "steps": args.warmup + args.steps}
     rng = np.random.default_rng(0)
     worlds = sorted(rng.choice(args.worlds, size=min(args.error_worlds, args.worlds), replace=False).tolist())
-    for prec, st in (("fp32", st32),) + ((("fp64", st64),) if not args.no_fp64 else ()):
+    for prec, st in ((("fp32", st32),) + ((("fp64", st64),) if not args.no_fp64 else ())) if worlds else ():
         key = "one_step_error_%s_vs_fp64_oracle" % prec
         try:
             line[key] = one_step_error(st, prec, worlds)
@@ -306,7 +331,7 @@ def main():
         except Exception as exc:  # the error check never hides the measurement
             line[key] = {"error": str(exc)}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(batch, args.cpu_seconds, args.cpu_procs)
+        line["cpu_baseline"] = cpu_baseline(batch, args.warmup + args.steps, args.cpu_seconds, args.cpu_procs)
         log("cpu baseline: %.3e body-steps/s" % line["cpu_baseline"]["value"])
     print(json.dumps(line), flush=True)
 

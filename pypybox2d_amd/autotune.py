@@ -35,6 +35,7 @@ class LaunchTuner(object):
         self.period = max(period, len(self.candidates) + 1)
         self.best: Optional[Dict[str, int]] = None
         self.last_times: Optional[np.ndarray] = None
+        self.last_ms: Optional[np.ndarray] = None  # per candidate: collide, islands, solve, sync, pairs
         self._since = None
 
     def apply(self, params: Dict[str, int]) -> None:
@@ -43,10 +44,12 @@ class LaunchTuner(object):
 
     def tune(self, dt: float, vel_iters: int, pos_iters: int) -> Dict[str, int]:
         """Steps once per candidate (eagerly, timed) and applies the fastest."""
-        times = np.zeros(len(self.candidates))
+        ms = np.zeros((len(self.candidates), 5))
         for i, c in enumerate(self.candidates):
             self.apply(c)
-            times[i] = self.engine.profile(1, dt, vel_iters, pos_iters)[2]
+            ms[i] = self.engine.profile(1, dt, vel_iters, pos_iters)
+        times = ms[:, 2]  # the solve kernel
+        self.last_ms = ms
         self.last_times = times
         self.best = self.candidates[int(np.argmin(times))]
         self.apply(self.best)

@@ -60,6 +60,7 @@ enum : int { WERR_CAPACITY = 1, WERR_UNSUPPORTED = 2, WERR_NEWPAIRS = 4, WERR_SC
 
 constexpr int kBlock = 256;
 constexpr int kMaxLds = 163840;  // LDS per workgroup on gfx950
+constexpr int kProfileBatch = 128;  // steps per event read-back in b2g_profile_step
 constexpr int kSizeClasses = 16;
 // q_cls layout: [0, 16) island count per class, [16, 32) cursors,
 // [32] islands in team classes, [33] wavefronts for them,
@@ -346,7 +347,7 @@ __host__ __device__ inline size_t island_smem_bytes(int nb, int ccap) {
   s += ((size_t)nb * 2 + 15) & ~(size_t)15;
   s += ((size_t)(ccap + 31) / 32 * 4 + 15) & ~(size_t)15;
   s += (size_t)kBlock * 4;
-  s += 64;
+  s += 64 * 4;  // misc: 16 scalars, 16 class counts, spare
   return s;
 }
 
@@ -408,26 +409,64 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
     L.off[i] = 0;
   }
   for (int i = tid; i < (ccap + 31) / 32; i += kBlock) L.cflag[i] = 0;
+  if (tid == 0) L.misc[4] = 0;
   __syncthreads();
 
   // --- rest of collide's awake gating (contact_manager.py:191-197), in order ---
-  if (d.w_asleep[w] > 0 && tid == 0) {
-    for (int sl = 0; sl < used; ++sl) {
-      const int s = cb + sl;
-      int st = d.c_state[s];
-      if (!(st & CS_LIVE)) continue;
-      int2 pr = d.c_pair[s];
-      const int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
-      if (!(st & CS_PROCESSED)) {
-        bool act = ((L.bst[la] & 2) && !(L.bst[la] & 1)) || ((L.bst[lb] & 2) && !(L.bst[lb] & 1));
-        if (!act) continue;
-        st = narrow_update(d, w, s, st);
-        d.c_state[s] = st;
-        if (!(st & CS_LIVE)) continue;
+  // Only live contacts that k_collide skipped (both bodies asleep) or whose
+  // touching flag changed (which wakes bodies) can matter.  They are first
+  // compacted in slot order, with their bodies, into LDS (the adjacency arrays
+  // are free until the graph is built); one lane then walks them in order.
+  if (d.w_asleep[w] > 0) {
+    uint16_t* cs = L.adj_s;
+    uint16_t* cla = L.adj_s + ccap;
+    uint16_t* clb = L.adj_o;
+    uint16_t* cfl = L.adj_o + ccap;  // bit 0 evaluated this step, bit 1 touching changed
+    int* wcount = L.scan;
+    const int lane = tid & 63, wv = tid >> 6;
+    for (int base = 0; base < used; base += kBlock) {
+      const int sl = base + tid;
+      int st = 0;
+      bool cand = false;
+      if (sl < used) {
+        st = d.c_state[cb + sl];
+        cand = (st & CS_LIVE) && (!(st & CS_PROCESSED) || (st & CS_WOKE));
       }
-      if (st & CS_WOKE) {
-        if (!(L.bst[la] & 2)) L.bst[la] |= 2 | 32;
-        if (!(L.bst[lb] & 2)) L.bst[lb] |= 2 | 32;
+      const unsigned long long m = __ballot(cand);
+      if (lane == 0) wcount[wv] = __popcll(m);
+      __syncthreads();
+      int off = L.misc[4];
+      for (int j = 0; j < wv; ++j) off += wcount[j];
+      if (cand) {
+        const int idx = off + __popcll(m & ((1ull << lane) - 1));
+        const int2 pr = d.c_pair[cb + sl];
+        cs[idx] = (uint16_t)sl;
+        cla[idx] = (uint16_t)d.f_info[f0 + pr.x].x;
+        clb[idx] = (uint16_t)d.f_info[f0 + pr.y].x;
+        cfl[idx] = (uint16_t)(((st & CS_PROCESSED) ? 1 : 0) | ((st & CS_WOKE) ? 2 : 0));
+      }
+      __syncthreads();
+      if (tid == 0) L.misc[4] += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const int ncand = L.misc[4];
+      for (int i = 0; i < ncand; ++i) {
+        const int la = cla[i], lb = clb[i], fl = cfl[i];
+        bool woke = (fl & 2) != 0;
+        if (!(fl & 1)) {
+          const bool act = ((L.bst[la] & 2) && !(L.bst[la] & 1)) || ((L.bst[lb] & 2) && !(L.bst[lb] & 1));
+          if (!act) continue;
+          const int s = cb + cs[i];
+          const int st = narrow_update(d, w, s, d.c_state[s]);
+          d.c_state[s] = st;
+          if (!(st & CS_LIVE)) continue;
+          woke = (st & CS_WOKE) != 0;
+        }
+        if (woke) {
+          if (!(L.bst[la] & 2)) L.bst[la] |= 2 | 32;
+          if (!(L.bst[lb] & 2)) L.bst[lb] |= 2 | 32;
+        }
       }
     }
   }
@@ -531,12 +570,17 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
   }
   __syncthreads();
   const int nisl = L.misc[0], nci_tot = L.misc[2], qbase = L.misc[3];
+  int* ccount = L.misc + 16;  // islands per size class, then one global add per class
+  if (tid < kSizeClasses) ccount[tid] = 0;
+  __syncthreads();
   for (int k = tid; k < nisl; k += kBlock) {
     int4 in = info[k];
     d.q_isl[qbase + k] = make_int4(w, b0 + in.x, in.y, cb + in.z);
     d.q_nc[qbase + k] = in.w;
-    atomicAdd(&d.q_cls[size_class(in.w & 0x3FFFFFFF)], 1);
+    atomicAdd(&ccount[size_class(in.w & 0x3FFFFFFF)], 1);
   }
+  __syncthreads();
+  if (tid < kSizeClasses && ccount[tid] > 0) atomicAdd(&d.q_cls[tid], ccount[tid]);
   for (int k = tid; k < nci_tot; k += kBlock) {
     const int s = isl_c[k];
     int2 pr = d.c_pair[cb + s];
@@ -1848,10 +1892,13 @@ __global__ void k_reset_queue(Dev<T> d) {
 }
 
 // Orders the island queue by size class, largest first, so that the lanes of
-// a wave of k_solve carry islands of similar length.
+// a wave of k_solve carry islands of similar length.  Each block takes a
+// contiguous chunk of the queue, counts its classes in LDS and reserves its
+// ranges with one global atomic per class.
 template <typename T>
 __global__ void k_order_islands(Dev<T> d) {
-  __shared__ int base[kSizeClasses];
+  __shared__ int base[kSizeClasses], lcnt[kSizeClasses], lbase[kSizeClasses];
+  if (threadIdx.x < kSizeClasses) lcnt[threadIdx.x] = 0;
   if (threadIdx.x == 0) {
     int run = 0, waves = 0;
     for (int c = kSizeClasses - 1; c >= 0; --c) {
@@ -1871,10 +1918,20 @@ __global__ void k_order_islands(Dev<T> d) {
   }
   __syncthreads();
   const int n = *d.q_count;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+  const int per = (n + gridDim.x - 1) / gridDim.x;
+  const int q0 = blockIdx.x * per, q1 = min(n, q0 + per);
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) atomicAdd(&lcnt[size_class(d.q_nc[q] & 0x3FFFFFFF)], 1);
+  __syncthreads();
+  if (threadIdx.x < kSizeClasses) {
+    const int c = threadIdx.x;
+    lbase[c] = lcnt[c] > 0 ? base[c] + atomicAdd(&d.q_cls[kSizeClasses + c], lcnt[c]) : 0;
+    lcnt[c] = 0;
+  }
+  __syncthreads();
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int nc = d.q_nc[q];
     const int c = size_class(nc & 0x3FFFFFFF);
-    const int pos = base[c] + atomicAdd(&d.q_cls[kSizeClasses + c], 1);
+    const int pos = lbase[c] + atomicAdd(&lcnt[c], 1);
     d.qs_isl[pos] = d.q_isl[q];
     d.qs_nc[pos] = nc;
   }
@@ -1988,8 +2045,7 @@ struct b2g_ctx {
   double graph_dt = 0;
   int graph_vi = -1, graph_pi = -1;
   // timing
-  hipEvent_t ev[8] = {};
-  bool have_events = false;
+  std::vector<hipEvent_t> pev;  // b2g_profile_step
 };
 
 static void ctx_error(b2g_ctx* ctx, const std::string& s) {
@@ -2376,7 +2432,8 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
 }
 
 template <typename T>
-static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, bool timed) {
+static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, hipEvent_t* ev) {
+  const bool timed = ev != nullptr;
   Dev<T> d = *(Dev<T>*)ctx->dev;
   d.dt = dt;
   d.inv_dt = dt > 0.0 ? 1.0 / dt : 0.0;
@@ -2390,38 +2447,38 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   if (first_pairs) {
     hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, 1, 0, 1);
   }
-  if (timed) HIPCHK(hipEventRecord(ctx->ev[0], st));
+  if (timed) HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(k_reset_queue<T>, dim3(1), dim3(64), 0, st, d);
   const int chunks = (ctx->max_ccap + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_collide<T>, dim3(std::max(chunks, 1), nw), dim3(kBlock), 0, st, d);
-  if (timed) HIPCHK(hipEventRecord(ctx->ev[1], st));
+  if (timed) HIPCHK(hipEventRecord(ev[1], st));
   if (dt > 0.0) {
     hipLaunchKernelGGL(k_islands<T>, dim3(nw), dim3(kBlock), ctx->smem_islands, st, d);
-    if (timed) HIPCHK(hipEventRecord(ctx->ev[2], st));
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
+    if (timed) HIPCHK(hipEventRecord(ev[2], st));
     const int swaves = std::max(1, std::min((ctx->nb + kSolveBlock - 1) / kSolveBlock, ctx->solve_grid));
     hipLaunchKernelGGL(k_solve<T>, dim3(swaves), dim3(kSolveBlock), ctx->team_lds, st, d);
-    if (timed) HIPCHK(hipEventRecord(ctx->ev[3], st));
+    if (timed) HIPCHK(hipEventRecord(ev[3], st));
     hipLaunchKernelGGL(k_sync<T>, dim3(std::max(1, (ctx->nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d);
-    if (timed) HIPCHK(hipEventRecord(ctx->ev[4], st));
+    if (timed) HIPCHK(hipEventRecord(ev[4], st));
     hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, 0, 1, 1);
   } else {
     if (timed) {
-      HIPCHK(hipEventRecord(ctx->ev[2], st));
-      HIPCHK(hipEventRecord(ctx->ev[3], st));
-      HIPCHK(hipEventRecord(ctx->ev[4], st));
+      HIPCHK(hipEventRecord(ev[2], st));
+      HIPCHK(hipEventRecord(ev[3], st));
+      HIPCHK(hipEventRecord(ev[4], st));
     }
     hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, 0, 1, 0);
   }
-  if (timed) HIPCHK(hipEventRecord(ctx->ev[5], st));
+  if (timed) HIPCHK(hipEventRecord(ev[5], st));
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-static int dispatch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, bool timed) {
-  if (ctx->precision == B2G_FP64) return launch_step<double>(ctx, dt, vi, pi, first_pairs, timed);
-  return launch_step<float>(ctx, dt, vi, pi, first_pairs, timed);
+static int dispatch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, hipEvent_t* ev = nullptr) {
+  if (ctx->precision == B2G_FP64) return launch_step<double>(ctx, dt, vi, pi, first_pairs, ev);
+  return launch_step<float>(ctx, dt, vi, pi, first_pairs, ev);
 }
 
 template <typename T>
@@ -2588,8 +2645,7 @@ B2G_API int b2g_create(const b2g_desc* desc, b2g_ctx** out) {
     delete ctx;
     return B2G_E_HIP;
   }
-  for (int i = 0; i < 8; ++i) hipEventCreate(&ctx->ev[i]);
-  ctx->have_events = true;
+
   *out = ctx;
   return B2G_OK;
 }
@@ -2599,8 +2655,8 @@ B2G_API int b2g_destroy(b2g_ctx* ctx) {
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   free_device(ctx);
-  if (ctx->have_events)
-    for (int i = 0; i < 8; ++i) hipEventDestroy(ctx->ev[i]);
+  for (auto& e : ctx->pev) hipEventDestroy(e);
+  ctx->pev.clear();
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return B2G_OK;
@@ -2622,7 +2678,7 @@ B2G_API int b2g_step(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters) {
   hipSetDevice(ctx->device);
   bool first = ctx->pending_new_fixture;
   ctx->pending_new_fixture = false;
-  return dispatch_step(ctx, dt, vel_iters, pos_iters, first, false);
+  return dispatch_step(ctx, dt, vel_iters, pos_iters, first);
 }
 
 B2G_API int b2g_step_n(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, int use_graph) {
@@ -2632,11 +2688,11 @@ B2G_API int b2g_step_n(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_it
   int i = 0;
   if (ctx->pending_new_fixture && n > 0) {
     ctx->pending_new_fixture = false;
-    TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, true, false));
+    TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, true));
     ++i;
   }
   if (!use_graph) {
-    for (; i < n; ++i) TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, false, false));
+    for (; i < n; ++i) TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, false));
     return B2G_OK;
   }
   if (i < n) {
@@ -2646,7 +2702,7 @@ B2G_API int b2g_step_n(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_it
       ctx->graph_exec = nullptr;
       ctx->graph = nullptr;
       HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      int rc = dispatch_step(ctx, dt, vel_iters, pos_iters, false, false);
+      int rc = dispatch_step(ctx, dt, vel_iters, pos_iters, false);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(ctx->stream, &g);
       if (rc) return rc;
@@ -2757,19 +2813,26 @@ B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int 
   hipSetDevice(ctx->device);
   if (ctx->pending_new_fixture) {
     ctx->pending_new_fixture = false;
-    TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, true, false));
+    TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, true));
   }
-  for (int k = 0; k < 5; ++k) ms_out[k] = 0.f;
-  for (int i = 0; i < n; ++i) {
-    TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, false, true));
-    HIPCHK(hipEventSynchronize(ctx->ev[5]));
-    for (int k = 0; k < 5; ++k) {
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
-      ms_out[k] += ms;
-    }
+  if (ctx->pev.empty()) {  // a ring of 6 events per step, read once per batch of steps
+    ctx->pev.resize(6 * kProfileBatch);
+    for (auto& e : ctx->pev) HIPCHK(hipEventCreate(&e));
   }
-  for (int k = 0; k < 5; ++k) ms_out[k] /= (float)n;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int done = 0; done < n;) {
+    const int m = std::min(kProfileBatch, n - done);
+    for (int i = 0; i < m; ++i) TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, false, &ctx->pev[6 * i]));
+    HIPCHK(hipEventSynchronize(ctx->pev[6 * (m - 1) + 5]));
+    for (int i = 0; i < m; ++i)
+      for (int k = 0; k < 5; ++k) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->pev[6 * i + k], ctx->pev[6 * i + k + 1]));
+        acc[k] += ms;
+      }
+    done += m;
+  }
+  for (int k = 0; k < 5; ++k) ms_out[k] = (float)(acc[k] / n);
   return B2G_OK;
 }
 
