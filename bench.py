@@ -262,8 +262,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--worlds", type=int, default=4096)
-    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--workload", choices=("mixed", "lognormal"), default="mixed",
+                    help="mixed: body counts U[32,512]; lognormal: median 64, sigma 0.75, clipped to [16,1024]")
+    ap.add_argument("--worlds", type=int, default=None, help="default 4096 (mixed) or 16384 (lognormal)")
+    ap.add_argument("--seed", type=int, default=None, help="default 1 (mixed) or 2 (lognormal)")
     ap.add_argument("--fp64-steps", type=int, default=None, help="timed steps of the FP64 run (default: --steps)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -278,8 +280,19 @@ def main():
               "with no communication); got --gpus %d, WORLD_SIZE=%d" % (args.gpus, world_size))
         sys.exit(2)
 
+    if args.worlds is None:
+        args.worlds = 4096 if args.workload == "mixed" else 16384
+    if args.seed is None:
+        args.seed = 1 if args.workload == "mixed" else 2
     t0 = time.perf_counter()
-    counts = scenes.mixed_batch_counts(args.worlds, args.seed)
+    if args.workload == "mixed":
+        counts = scenes.mixed_batch_counts(args.worlds, args.seed)
+        desc = ("mixed_batch: %d worlds, body counts U[32,512] (seed %d), boxes/circles/polygons 40/30/30, "
+                "dt 1/60, 8/3 iterations" % (args.worlds, args.seed))
+    else:
+        counts = scenes.lognormal_batch_counts(args.worlds, args.seed)
+        desc = ("lognormal_batch: %d worlds, body counts clip(round(64 exp(0.75 N(0,1))), 16, 1024) (seed %d), "
+                "boxes/circles/polygons 40/30/30, dt 1/60, 8/3 iterations" % (args.worlds, args.seed))
     log("building %d worlds (%d dynamic bodies) on the host" % (args.worlds, int(counts.sum())))
     batch = builder.batch_from_counts(counts, args.seed)
     log("built in %.1f s" % (time.perf_counter() - t0))
@@ -296,8 +309,7 @@ def main():
         "ms_per_step": 1e3 * el32 / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 bulk (f64 positions/angles/velocities/impulses/AABBs)",
         "data": "synthetic",
-        "config": {"workload": "mixed_batch: %d worlds, body counts U[32,512] (seed %d), boxes/circles/polygons "
-                               "40/30/30, dt 1/60, 8/3 iterations, graph-captured step" % (args.worlds, args.seed),
+        "config": {"workload": desc,
                    "worlds": args.worlds, "dynamic_bodies": dyn, "contacts_at_end": c["contacts"],
                    "touching_at_end": c["touching"]},
         "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms32)},

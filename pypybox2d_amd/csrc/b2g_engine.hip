@@ -335,12 +335,26 @@ struct IslandSmem {
 
 __device__ __forceinline__ size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-__host__ __device__ inline size_t island_smem_bytes(int nb, int ccap) {
+// Touching contacts of a world are a planar graph over its fixtures (at most
+// 3 per fixture on average); the island search keeps room for 4 per fixture
+// and reports a capacity error beyond that.  The adjacency arrays also hold
+// the awake-gating candidates, kGateChunk slots at a time.
+constexpr int kGateChunk = 1024;
+__host__ __device__ inline int touch_cap(int nf, int ccap) {
+  const int t = 4 * nf + 64;
+  return t < ccap ? t : ccap;
+}
+__host__ __device__ inline int adj_entries(int nf, int ccap) {
+  const int e = 2 * touch_cap(nf, ccap);
+  return e > 2 * kGateChunk ? e : 2 * kGateChunk;
+}
+
+__host__ __device__ inline size_t island_smem_bytes(int nb, int nf, int ccap) {
   size_t s = 0;
   s += ((size_t)(nb + 1) * 4 + 15) & ~(size_t)15;
   s += ((size_t)nb * 4 + 15) & ~(size_t)15;
-  s += ((size_t)2 * ccap * 2 + 15) & ~(size_t)15;
-  s += ((size_t)2 * ccap * 2 + 15) & ~(size_t)15;
+  s += ((size_t)adj_entries(nf, ccap) * 2 + 15) & ~(size_t)15;
+  s += ((size_t)adj_entries(nf, ccap) * 2 + 15) & ~(size_t)15;
   s += ((size_t)nb + 15) & ~(size_t)15;
   s += ((size_t)nb * 2 + 15) & ~(size_t)15;
   s += ((size_t)nb * 2 + 15) & ~(size_t)15;
@@ -383,16 +397,17 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = blockIdx.x, tid = threadIdx.x;
   const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
-  const int f0 = d.w_fix_off[w];
+  const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
   const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
   const int used = d.w_con_used[w];
+  const int nadj = adj_entries(nf, ccap);
   IslandSmem L;
   {
     unsigned char* p = smem;
     L.off = (int*)p; p += ((size_t)(nb + 1) * 4 + 15) & ~(size_t)15;
     L.cur = (int*)p; p += ((size_t)nb * 4 + 15) & ~(size_t)15;
-    L.adj_s = (uint16_t*)p; p += ((size_t)2 * ccap * 2 + 15) & ~(size_t)15;
-    L.adj_o = (uint16_t*)p; p += ((size_t)2 * ccap * 2 + 15) & ~(size_t)15;
+    L.adj_s = (uint16_t*)p; p += ((size_t)nadj * 2 + 15) & ~(size_t)15;
+    L.adj_o = (uint16_t*)p; p += ((size_t)nadj * 2 + 15) & ~(size_t)15;
     L.bst = (uint8_t*)p; p += ((size_t)nb + 15) & ~(size_t)15;
     L.bidx = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
     L.stack = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
@@ -419,55 +434,60 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
   // are free until the graph is built); one lane then walks them in order.
   if (d.w_asleep[w] > 0) {
     uint16_t* cs = L.adj_s;
-    uint16_t* cla = L.adj_s + ccap;
+    uint16_t* cla = L.adj_s + kGateChunk;
     uint16_t* clb = L.adj_o;
-    uint16_t* cfl = L.adj_o + ccap;  // bit 0 evaluated this step, bit 1 touching changed
+    uint16_t* cfl = L.adj_o + kGateChunk;  // bit 0 evaluated this step, bit 1 touching changed
     int* wcount = L.scan;
     const int lane = tid & 63, wv = tid >> 6;
-    for (int base = 0; base < used; base += kBlock) {
-      const int sl = base + tid;
-      int st = 0;
-      bool cand = false;
-      if (sl < used) {
-        st = d.c_state[cb + sl];
-        cand = (st & CS_LIVE) && (!(st & CS_PROCESSED) || (st & CS_WOKE));
-      }
-      const unsigned long long m = __ballot(cand);
-      if (lane == 0) wcount[wv] = __popcll(m);
-      __syncthreads();
-      int off = L.misc[4];
-      for (int j = 0; j < wv; ++j) off += wcount[j];
-      if (cand) {
-        const int idx = off + __popcll(m & ((1ull << lane) - 1));
-        const int2 pr = d.c_pair[cb + sl];
-        cs[idx] = (uint16_t)sl;
-        cla[idx] = (uint16_t)d.f_info[f0 + pr.x].x;
-        clb[idx] = (uint16_t)d.f_info[f0 + pr.y].x;
-        cfl[idx] = (uint16_t)(((st & CS_PROCESSED) ? 1 : 0) | ((st & CS_WOKE) ? 2 : 0));
-      }
-      __syncthreads();
-      if (tid == 0) L.misc[4] += wcount[0] + wcount[1] + wcount[2] + wcount[3];
-      __syncthreads();
-    }
-    if (tid == 0) {
-      const int ncand = L.misc[4];
-      for (int i = 0; i < ncand; ++i) {
-        const int la = cla[i], lb = clb[i], fl = cfl[i];
-        bool woke = (fl & 2) != 0;
-        if (!(fl & 1)) {
-          const bool act = ((L.bst[la] & 2) && !(L.bst[la] & 1)) || ((L.bst[lb] & 2) && !(L.bst[lb] & 1));
-          if (!act) continue;
-          const int s = cb + cs[i];
-          const int st = narrow_update(d, w, s, d.c_state[s]);
-          d.c_state[s] = st;
-          if (!(st & CS_LIVE)) continue;
-          woke = (st & CS_WOKE) != 0;
+    for (int chunk = 0; chunk < used; chunk += kGateChunk) {
+      const int cend = min(used, chunk + kGateChunk);
+      for (int base = chunk; base < cend; base += kBlock) {
+        const int sl = base + tid;
+        int st = 0;
+        bool cand = false;
+        if (sl < cend) {
+          st = d.c_state[cb + sl];
+          cand = (st & CS_LIVE) && (!(st & CS_PROCESSED) || (st & CS_WOKE));
         }
-        if (woke) {
-          if (!(L.bst[la] & 2)) L.bst[la] |= 2 | 32;
-          if (!(L.bst[lb] & 2)) L.bst[lb] |= 2 | 32;
+        const unsigned long long m = __ballot(cand);
+        if (lane == 0) wcount[wv] = __popcll(m);
+        __syncthreads();
+        int off = L.misc[4];
+        for (int j = 0; j < wv; ++j) off += wcount[j];
+        if (cand) {
+          const int idx = off + __popcll(m & ((1ull << lane) - 1));
+          const int2 pr = d.c_pair[cb + sl];
+          cs[idx] = (uint16_t)sl;
+          cla[idx] = (uint16_t)d.f_info[f0 + pr.x].x;
+          clb[idx] = (uint16_t)d.f_info[f0 + pr.y].x;
+          cfl[idx] = (uint16_t)(((st & CS_PROCESSED) ? 1 : 0) | ((st & CS_WOKE) ? 2 : 0));
         }
+        __syncthreads();
+        if (tid == 0) L.misc[4] += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        __syncthreads();
       }
+      if (tid == 0) {
+        const int ncand = L.misc[4];
+        for (int i = 0; i < ncand; ++i) {
+          const int la = cla[i], lb = clb[i], fl = cfl[i];
+          bool woke = (fl & 2) != 0;
+          if (!(fl & 1)) {
+            const bool act = ((L.bst[la] & 2) && !(L.bst[la] & 1)) || ((L.bst[lb] & 2) && !(L.bst[lb] & 1));
+            if (!act) continue;
+            const int s = cb + cs[i];
+            const int st = narrow_update(d, w, s, d.c_state[s]);
+            d.c_state[s] = st;
+            if (!(st & CS_LIVE)) continue;
+            woke = (st & CS_WOKE) != 0;
+          }
+          if (woke) {
+            if (!(L.bst[la] & 2)) L.bst[la] |= 2 | 32;
+            if (!(L.bst[lb] & 2)) L.bst[lb] |= 2 | 32;
+          }
+        }
+        L.misc[4] = 0;
+      }
+      __syncthreads();
     }
   }
   __syncthreads();
@@ -483,6 +503,13 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
   }
   __syncthreads();
   int total = block_exclusive_scan(L.off, nb, L.scan);
+  if (total > nadj) {  // more touching contacts than the island search has room for
+    if (tid == 0) {
+      atomicOr(&d.w_err[w], WERR_CAPACITY);
+      d.w_nisl[w] = 0;
+    }
+    return;
+  }
   if (tid == 0) L.off[nb] = total;
   for (int i = tid; i < nb; i += kBlock) L.cur[i] = L.off[i];
   __syncthreads();
@@ -2141,7 +2168,11 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   }
   ctx->con_base[nw] = ncap;
   ctx->ncap = ncap;
-  ctx->smem_islands = island_smem_bytes(ctx->max_nb, ctx->max_ccap);
+  ctx->smem_islands = 0;
+  for (int w = 0; w < nw; ++w)
+    ctx->smem_islands = std::max(ctx->smem_islands,
+                                 island_smem_bytes(ctx->body_off[w + 1] - ctx->body_off[w],
+                                                   ctx->fix_off[w + 1] - ctx->fix_off[w], ctx->con_cap[w]));
   ctx->smem_pairs = pairs_smem_bytes(ctx->max_nf, ctx->max_ccap);
   if (ctx->smem_islands > (size_t)kMaxLds || ctx->smem_pairs > (size_t)kMaxLds)
     return set_error(ctx, B2G_E_UNSUPPORTED, "world too large for the LDS-resident path (LDS > 160 KB)");

@@ -88,6 +88,15 @@ class Fixture(object):
     def sensor(self):
         return self._sensor
 
+    @sensor.setter
+    def sensor(self, sensor):
+        # fixture.py:169-174: wakes the body; the next step rejects sensors
+        if sensor != self._sensor:
+            if self._body is not None:
+                self._body.awake = True
+            self._sensor = sensor
+            self._changed()
+
     @property
     def category_bits(self):
         return self._category_bits

@@ -230,3 +230,89 @@ def test_set_param_rejects_bad_values():
         with pytest.raises(N.B2GError):
             e.set_param(k, v)
     e.close()
+
+
+def test_restitution_trajectory_fp64_and_one_step_fp32():
+    """Bouncing bodies (restitution 0.6) exercise the restitution bias of the velocity constraints."""
+    rng = np.random.default_rng(11)
+    sc = scenes.mixed_world(rng, 40, half_width=8.0, height=12.0)
+    for b in sc["bodies"]:
+        for f in b["fixtures"]:
+            f["restitution"] = 0.6
+    batch = builder.scene_batch(sc)
+    o = O.Oracle(batch)
+    e = N.Engine("fp64")
+    e.upload(batch)
+    for s in range(60):
+        o.step(DT, 8, 3)
+        e.step(DT, 8, 3)
+    got, ref = e.download(), o.state()
+    e.close()
+    assert PU.same_contact_lists(got, ref)
+    be = PU.body_errors(got, ref)
+    print("restitution fp64 60 steps", be, flush=True)
+    assert max(be[q] for q in ("position", "angle", "velocity", "angular_velocity")) <= 1e-9
+    ref.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    one = oracle_step(ref)
+    g32 = gpu_step(ref, "fp32")
+    be = PU.body_errors(g32, one)
+    print("restitution fp32 one step", be, flush=True)
+    assert PU.same_contact_lists(g32, one)
+    for q in ("position", "angle", "velocity", "angular_velocity"):
+        assert be[q] <= 1e-5, (q, be[q])
+
+
+def test_drop_in_world_api_tracks_oracle():
+    """The pypybox2d-style World/Body API, stepped on the GPU, against the oracle on the same scene."""
+    import pypybox2d_amd as b2
+    world = b2.World(gravity=(0, -10), do_sleep=True, continuous_physics=False)
+    ground = world.create_static_body(position=(0, 0))
+    ground.create_edge_fixture((-40, 0), (40, 0))
+    boxes = []
+    for i in range(10):
+        for j in range(10 - i):
+            b = world.create_dynamic_body(position=(-7 + 0.5625 * i + 1.125 * j, 0.75 + 1.0 * i))
+            b.create_polygon_fixture(box=(0.5, 0.5), density=5, friction=0.6)
+            boxes.append(b)
+    o = O.Oracle(builder.scene_batch(scenes.pyramid()))
+    for s in range(120):
+        world.step(DT, 8, 3)
+        o.step(DT, 8, 3)
+        if s % 40 == 39:
+            ref = o.state().body_d
+            worst = 0.0
+            for k, b in enumerate(boxes):  # body 0 of the scene is the ground
+                p, r = b.position, ref[k + 1]
+                worst = max(worst, abs(p.x - r[L.BD_XF_PX]), abs(p.y - r[L.BD_XF_PY]),
+                            abs(b.angle - r[L.BD_A]), abs(b.linear_velocity.y - r[L.BD_V_Y]))
+            print("drop-in world step", s + 1, "max abs diff", worst, flush=True)
+            assert worst <= 1e-9
+    assert world.contact_count == o.state().con_i.shape[0]
+
+
+def test_stacks_2000_steps_settle_height():
+    """Vertical stacks, 2000 steps: the FP64 pipeline tracks the oracle; the FP32 pipeline's
+    settle height (mean top-box height over the last 100 steps) stays within 1 mm of it."""
+    batch = builder.scene_batch(scenes.stacks(seed=0))
+    o = O.Oracle(batch)
+    e64, e32 = N.Engine("fp64"), N.Engine("fp32")
+    e64.upload(batch)
+    e32.upload(batch)
+    tops = [k for k in range(1, 51) if k % 10 == 0]  # the top box of each column
+    h = {"ref": [], "fp64": [], "fp32": []}
+    for s in range(2000):
+        o.step(DT, 8, 3)
+        e64.step(DT, 8, 3)
+        e32.step(DT, 8, 3)
+        if s >= 1900:
+            for name, st in (("ref", o.state()), ("fp64", e64.download()), ("fp32", e32.download())):
+                h[name].append(st.body_d[tops, L.BD_C_Y].mean())
+    ref, g64, g32 = o.state(), e64.download(), e32.download()
+    e64.close()
+    e32.close()
+    settle = {k: float(np.mean(v)) for k, v in h.items()}
+    be = PU.body_errors(g64, ref)
+    print("stacks 2000: settle heights", settle, "fp64 errors", be, flush=True)
+    assert max(be[q] for q in ("position", "angle")) <= 1e-9
+    assert abs(settle["fp64"] - settle["ref"]) <= 1e-9
+    assert abs(settle["fp32"] - settle["ref"]) <= 1e-3
