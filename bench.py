@@ -238,22 +238,24 @@ def cpu_baseline(batch: L.BatchArrays, steps: int, seconds: float, procs: int):
                       % (procs, procs, steps, seconds, nw)}
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary."""
-    path = os.path.join(HERE, "profiles", "%s_mixed4096_traffic.txt" % kernel)
-    if not os.path.exists(path):
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    of the default workload (FP32 pipeline), or None."""
+    path = os.path.join(HERE, "profiles", "traffic_mixed4096_fp32.md")
+    if workload != "mixed" or not os.path.exists(path):
         return None
+    key = "traffic_bytes_per_launch %s:" % kernel
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key):
             return float(line.split(":", 1)[1])
     return None
 
 
-def roofline(kernel: str, ms: float, counts: dict, fp64: bool) -> dict:
+def roofline(kernel: str, ms: float, counts: dict, fp64: bool, workload: str = "mixed") -> dict:
     ab = algorithmic_bytes(kernel, counts, fp64)
     achieved = ab / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel), "algorithmic_bytes": ab,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, workload), "algorithmic_bytes": ab,
             "avg_ms": ms}
 
 
@@ -313,8 +315,8 @@ def main():
                    "worlds": args.worlds, "dynamic_bodies": dyn, "contacts_at_end": c["contacts"],
                    "touching_at_end": c["touching"]},
         "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms32)},
-        "roofline": roofline(KERNELS[k], float(ms32[k]), c, False),
-        "roofline_other": {n: roofline(n, float(ms32[i]), c, False) for i, n in enumerate(KERNELS)
+        "roofline": roofline(KERNELS[k], float(ms32[k]), c, False, args.workload),
+        "roofline_other": {n: roofline(n, float(ms32[i]), c, False, args.workload) for i, n in enumerate(KERNELS)
                            if n in ("k_collide", "k_pairs", "k_sync")},
         "launch_shape": tuned32,
         "graph_ms_per_step_next_%d" % GRAPH_CHECK_STEPS: gms32,

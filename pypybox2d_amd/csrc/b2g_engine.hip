@@ -177,6 +177,8 @@ struct Dev {
   int2* i_cidx;      // per-world segment at w_con_base[w]
   uint2* i_rank;     // per-world segment at w_con_base[w]: the contact's rank in body A's and body B's
                      // contact sequence of the island (x = rank A | rank B << 16)
+  uint16_t* i_order; // per-world segment at w_con_base[w]: each island's contacts (island-local
+                     // indices) ordered by dependency level within one sweep
   int4* i_info;      // per-world segment at w_body_off[w]: (bstart, nb, cstart, nc | last_allow << 30)
   int4* q_isl;       // global island queue: (world, bstart, nb, cstart)
   int* q_nc;         // (nc | last_allow << 30)
@@ -328,6 +330,7 @@ struct IslandSmem {
   uint16_t* bidx;  // nb
   uint16_t* stack;  // nb
   uint16_t* slist;  // nb
+  uint16_t* lev;    // nb: dependency level of the body's last contact in the island list
   uint32_t* cflag;  // ccap / 32
   int* scan;       // kBlock
   int* misc;       // 8
@@ -359,6 +362,7 @@ __host__ __device__ inline size_t island_smem_bytes(int nb, int nf, int ccap) {
   s += ((size_t)nb * 2 + 15) & ~(size_t)15;
   s += ((size_t)nb * 2 + 15) & ~(size_t)15;
   s += ((size_t)nb * 2 + 15) & ~(size_t)15;
+  s += ((size_t)nb * 2 + 15) & ~(size_t)15;  // lev
   s += ((size_t)(ccap + 31) / 32 * 4 + 15) & ~(size_t)15;
   s += (size_t)kBlock * 4;
   s += 64 * 4;  // misc: 16 scalars, 16 class counts, spare
@@ -412,6 +416,7 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
     L.bidx = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
     L.stack = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
     L.slist = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
+    L.lev = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
     L.cflag = (uint32_t*)p; p += ((size_t)(ccap + 31) / 32 * 4 + 15) & ~(size_t)15;
     L.scan = (int*)p; p += (size_t)kBlock * 4;
     L.misc = (int*)p;
@@ -529,6 +534,7 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
   __syncthreads();
   for (int i = tid; i < nb; i += kBlock) {  // restore ascending slot order per body
     L.cur[i] = 0;  // from here on: contacts of the body seen by the island search
+    L.lev[i] = 0;
     int beg = L.off[i], end = L.off[i + 1];
     for (int k = beg + 1; k < end; ++k) {
       uint16_t s = L.adj_s[k], o = L.adj_o[k];
@@ -578,9 +584,17 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
           if (L.cflag[s >> 5] & bit) continue;
           L.cflag[s >> 5] |= bit;
           const int o = L.adj_o[k];
+          const bool o_dyn = !(L.bst[o] & 1);
           const uint32_t rank_b = (uint32_t)L.cur[b]++;
-          const uint32_t rank_o = (L.bst[o] & 1) ? 0u : (uint32_t)L.cur[o]++;
-          d.i_rank[cb + nci_tot] = make_uint2((uint32_t)b, rank_b | rank_o << 16);
+          const uint32_t rank_o = o_dyn ? (uint32_t)L.cur[o]++ : 0u;
+          // level: 1 + the latest level among the contact's dynamic bodies (one sweep)
+          const int lo = o_dyn ? L.lev[o] : 0;
+          const int lvl = 1 + (L.lev[b] > lo ? L.lev[b] : lo);
+          L.lev[b] = (uint16_t)lvl;
+          if (o_dyn) L.lev[o] = (uint16_t)lvl;
+          const uint32_t key = (uint32_t)(cstart + lvl - 1);  // < cstart + island contacts
+          d.i_rank[cb + nci_tot] = make_uint2((uint32_t)b | key << 16, rank_b | rank_o << 16);
+          d.i_order[cb + nci_tot] = (uint16_t)(nci_tot - cstart);
           isl_c[nci_tot++] = s;
           if (L.bst[o] & 16) continue;
           L.stack[sp++] = (uint16_t)o;
@@ -615,9 +629,25 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
     int ia = (L.bst[la] & 1) ? -1 - la : (int)L.bidx[la];
     int ib = (L.bst[lb] & 1) ? -1 - lb : (int)L.bidx[lb];
     d.i_cidx[cb + k] = make_int2(ia, ib);
-    const uint2 r = d.i_rank[cb + k];  // (expanding body, its rank | the other body's rank << 16)
+    // (expanding body | sort key << 16, its rank | the other body's rank << 16)
+    const uint2 r = d.i_rank[cb + k];
     const uint32_t lo = r.y & 0xFFFFu, hi = r.y >> 16;
-    d.i_rank[cb + k] = make_uint2((int)r.x == la ? (lo | hi << 16) : (hi | lo << 16), 0u);
+    const uint32_t key = r.x >> 16;
+    d.i_rank[cb + k] = make_uint2((int)(r.x & 0xFFFFu) == la ? (lo | hi << 16) : (hi | lo << 16),
+                                  key | (uint32_t)d.i_order[cb + k] << 16);
+  }
+  // each island's contacts by level: a counting sort on key = island start + level - 1
+  // (keys of one island stay inside its contact range); the adjacency arrays are free now
+  int* hist = (int*)L.adj_s;
+  for (int k = tid; k < nci_tot; k += kBlock) hist[k] = 0;
+  __syncthreads();
+  for (int k = tid; k < nci_tot; k += kBlock) atomicAdd(&hist[d.i_rank[cb + k].y & 0xFFFFu], 1);
+  __syncthreads();
+  block_exclusive_scan(hist, nci_tot, L.scan);
+  for (int k = tid; k < nci_tot; k += kBlock) {
+    const uint32_t y = d.i_rank[cb + k].y;
+    const int pos = atomicAdd(&hist[y & 0xFFFFu], 1);
+    d.i_order[cb + pos] = (uint16_t)(y >> 16);
   }
   for (int i = tid; i < nb; i += kBlock) {
     uint8_t bs = L.bst[i];
@@ -1313,17 +1343,20 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
 // Island.solve by a team of lanes (the path for large islands)
 //
 // A team is 8, 16, 32 or 64 lanes of one wavefront (by the island's size
-// class); a wavefront carries 64 / T islands of one class.  Lane t of a team
-// owns the island's contacts k = t, t + T, t + 2T, ... and walks them sweep
-// by sweep.  A contact's solve at sweep s must see every earlier solve (in the
+// class); a wavefront carries 64 / T islands of one class.  The island search
+// orders each island's contacts by dependency level within one sweep; lane t
+// of a team owns entries t, t + T, t + 2T, ... of that order and walks them
+// sweep by sweep, so its contacts come in non-decreasing level.  A contact's solve at sweep s must see every earlier solve (in the
 // reference's Gauss-Seidel order) that touches one of its dynamic bodies, and
 // nothing later.  Each dynamic body X keeps a counter of completed solves that
 // touched it; the contact is ready when, for each of its dynamic bodies, the
 // counter equals its rank in that body's sequence (s * deg(X) + the number of
 // the body's contacts before it in the island list, from the island search).
 // Ready lanes run in the same pass, which is safe because at most one pending
-// solve per body can be ready.  The earliest unfinished solve of a team is
-// always ready, so every pass makes progress; passes are still capped.  Body
+// solve per body can be ready.  An unfinished solve with the smallest (sweep,
+// level) has only finished predecessors, and so has the head of its lane (the
+// lane's sequence is sorted by sweep and level), so every pass makes progress;
+// passes are still capped.  Body
 // velocities, then positions, live in LDS; the constraint records stay in HBM
 // and are prefetched one solve ahead.  The arithmetic per contact is the same
 // code as the one-lane path, so results are identical.
@@ -1409,15 +1442,22 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
   const int s0 = c.warm ? 0 : 1;
   const int nsw = d.vel_iters + 1 - s0;
   const int mine = t < nci ? (nci - 1 - t) / TS + 1 : 0;  // contacts owned by this lane
+  // Lane t walks entries t, t + TS, ... of the island's level order (i_order): its contacts
+  // come in non-decreasing dependency level, so each is ready soon after the previous one.
+  const uint16_t* order = d.i_order + cstart;
   {
     bool active = nsw > 0 && mine > 0;
     const int cap = 2 * nsw * nci + 64;
-    int k = t, s = 0;
+    int j = t, s = 0, k = 0;
     VC<T> cur, nxt;
-    if (active) cur = d.vc[cstart + k];
-    int nk = k + TS, ns = s;
-    if (nk >= nci) { nk = t; ns = s + 1; }
-    if (active && mine > 1 && ns < nsw) nxt = d.vc[cstart + nk];
+    int nj = j + TS, ns = s, nk = 0;
+    if (nj >= nci) { nj = t; ns = s + 1; }
+    if (active) {
+      k = order[j];
+      cur = d.vc[cstart + k];
+      nk = order[nj];
+      if (mine > 1 && ns < nsw) nxt = d.vc[cstart + nk];
+    }
     int pass = 0;
     while (wave_any(active && pass < cap)) {
       ++pass;
@@ -1445,16 +1485,18 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
           }
           if (ia >= 0) { sx[3 * ia] = va.x; sx[3 * ia + 1] = va.y; sx[3 * ia + 2] = wa; lds_st(&cnt[ia], ra + 1); }
           if (ib >= 0) { sx[3 * ib] = vb.x; sx[3 * ib + 1] = vb.y; sx[3 * ib + 2] = wb; lds_st(&cnt[ib], rb + 1); }
-          k = nk;
+          j = nj;
           s = ns;
+          k = nk;
           if (s >= nsw) {
             active = false;
           } else {
-            nk = k + TS;
+            nj = j + TS;
             ns = s;
-            if (nk >= nci) { nk = t; ns = s + 1; }
+            if (nj >= nci) { nj = t; ns = s + 1; }
             if (mine > 1) {  // with one contact the record in registers is the current one
               cur = nxt;
+              nk = order[nj];
               if (ns < nsw) nxt = d.vc[cstart + nk];
             }
           }
@@ -1486,10 +1528,10 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
       team_sync();
       T min_sep = 0;
       bool active = mine > 0;
-      int k = t;
+      int j = t;
       PC<T> cur, nxt;
-      if (active) cur = d.pc[cstart + k];
-      if (active && k + TS < nci) nxt = d.pc[cstart + k + TS];
+      if (active) cur = d.pc[cstart + order[j]];
+      if (active && j + TS < nci) nxt = d.pc[cstart + order[j + TS]];
       int pass = 0;
       const int pcap = 2 * nci + 64;
       while (wave_any(active && pass < pcap)) {
@@ -1507,12 +1549,12 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
             solve_contact_position(cur, ax, ay, aa, bx, by, bb, min_sep);
             if (ia >= 0) { sx[3 * ia] = ax; sx[3 * ia + 1] = ay; sx[3 * ia + 2] = aa; lds_st(&cnt[ia], ra + 1); }
             if (ib >= 0) { sx[3 * ib] = bx; sx[3 * ib + 1] = by; sx[3 * ib + 2] = bb; lds_st(&cnt[ib], rb + 1); }
-            k += TS;
-            if (k >= nci) {
+            j += TS;
+            if (j >= nci) {
               active = false;
             } else {
               cur = nxt;
-              if (k + TS < nci) nxt = d.pc[cstart + k + TS];
+              if (j + TS < nci) nxt = d.pc[cstart + order[j + TS]];
             }
           }
         }
@@ -2399,6 +2441,7 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.i_contacts, ncap));
   TRY(dalloc(ctx, &d.i_cidx, ncap));
   TRY(dalloc(ctx, &d.i_rank, ncap));
+  TRY(dalloc(ctx, &d.i_order, ncap));
   TRY(dalloc(ctx, &d.i_info, nb));
   TRY(dalloc(ctx, &d.q_isl, nb));
   TRY(dalloc(ctx, &d.q_nc, nb));
