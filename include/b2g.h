@@ -105,11 +105,16 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
                                 state needs more than its team's share (24 B
                                 per body and 4 per body rounded to even
                                 bodies) is solved by one lane instead
-     B2G_PARAM_SOLVE_GRID       most wavefronts launched for the solver, >= 1 */
+     B2G_PARAM_SOLVE_GRID       most wavefronts launched for the solver, >= 1
+     B2G_PARAM_LARGE_WORLDS     1: every world takes the global-memory island
+                                and pair search (otherwise only worlds whose
+                                search does not fit 160 KB of LDS, or with
+                                >= 65535 contact slots); read at b2g_upload */
 #define B2G_PARAM_TEAM_LDS 1
 #define B2G_PARAM_HEAVY_CLASS 2
 #define B2G_PARAM_SOLVE_GRID 3
 #define B2G_PARAM_TEAM_FULL_CLASS 4
+#define B2G_PARAM_LARGE_WORLDS 5
 B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value);
 B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
 

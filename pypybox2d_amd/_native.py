@@ -25,7 +25,7 @@ EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step",
            "b2g_narrowphase", "b2g_pairs")
 
 # solver launch-shape keys (b2g_set_param)
-PARAMS = {"team_lds": 1, "heavy_class": 2, "solve_grid": 3, "team_full_class": 4}
+PARAMS = {"team_lds": 1, "heavy_class": 2, "solve_grid": 3, "team_full_class": 4, "large_worlds": 5}
 
 _ERRNAMES = {B2G_E_INVALID: "invalid argument", B2G_E_CAPACITY: "capacity exceeded", B2G_E_HIP: "HIP error",
              B2G_E_NAN: "non-finite state", B2G_E_UNSUPPORTED: "unsupported feature",

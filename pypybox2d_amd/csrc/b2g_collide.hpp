@@ -33,16 +33,18 @@ B2G_HD uint32_t pack_id(int a, int b, int c, int d) {
 }
 B2G_HD int id_part(uint32_t id, int k) { return (int)((id >> (8 * k)) & 255u); }
 
-template <typename T>
+// Shape geometry stored as S and evaluated in T (the FP32 pipeline stores
+// FP32 geometry and evaluates it in FP64).
+template <typename T, typename S = T>
 struct Geo {
-  const T* g;
+  const S* g;
   int n, type, has0, has3;
-  B2G_HD V2<T> v(int i) const { return v2<T>(g[GEO_VERTS + 2 * i], g[GEO_VERTS + 2 * i + 1]); }
-  B2G_HD V2<T> nrm(int i) const { return v2<T>(g[GEO_NORMALS + 2 * i], g[GEO_NORMALS + 2 * i + 1]); }
-  B2G_HD V2<T> centroid() const { return v2<T>(g[GEO_CENTROID], g[GEO_CENTROID + 1]); }
-  B2G_HD V2<T> cp() const { return v2<T>(g[GEO_CIRCLE_P], g[GEO_CIRCLE_P + 1]); }
-  B2G_HD V2<T> e(int k) const { return v2<T>(g[GEO_EDGE + 2 * k], g[GEO_EDGE + 2 * k + 1]); }
-  B2G_HD T radius() const { return g[GEO_RADIUS]; }
+  B2G_HD V2<T> v(int i) const { return v2<T>((T)g[GEO_VERTS + 2 * i], (T)g[GEO_VERTS + 2 * i + 1]); }
+  B2G_HD V2<T> nrm(int i) const { return v2<T>((T)g[GEO_NORMALS + 2 * i], (T)g[GEO_NORMALS + 2 * i + 1]); }
+  B2G_HD V2<T> centroid() const { return v2<T>((T)g[GEO_CENTROID], (T)g[GEO_CENTROID + 1]); }
+  B2G_HD V2<T> cp() const { return v2<T>((T)g[GEO_CIRCLE_P], (T)g[GEO_CIRCLE_P + 1]); }
+  B2G_HD V2<T> e(int k) const { return v2<T>((T)g[GEO_EDGE + 2 * k], (T)g[GEO_EDGE + 2 * k + 1]); }
+  B2G_HD T radius() const { return (T)g[GEO_RADIUS]; }
 };
 
 template <typename T>
@@ -69,8 +71,8 @@ struct Consts {
   static constexpr T ABSOLUTE_TOL = (T)0.001;
 };
 
-template <typename T>
-B2G_HD void collide_circles(Man<T>& m, const Geo<T>& A, XF<T> xfa, const Geo<T>& B, XF<T> xfb) {
+template <typename T, typename S>
+B2G_HD void collide_circles(Man<T>& m, const Geo<T, S>& A, XF<T> xfa, const Geo<T, S>& B, XF<T> xfb) {
   m.pc = 0;
   V2<T> pa = xmul(xfa, A.cp()), pb = xmul(xfb, B.cp());
   V2<T> d = pb - pa;
@@ -85,8 +87,8 @@ B2G_HD void collide_circles(Man<T>& m, const Geo<T>& A, XF<T> xfa, const Geo<T>&
   m.id0 = 0;
 }
 
-template <typename T>
-B2G_HD void collide_polygon_circle(Man<T>& m, const Geo<T>& P, XF<T> xfa, const Geo<T>& C, XF<T> xfb) {
+template <typename T, typename S>
+B2G_HD void collide_polygon_circle(Man<T>& m, const Geo<T, S>& P, XF<T> xfa, const Geo<T, S>& C, XF<T> xfb) {
   m.pc = 0;
   V2<T> c = xmul(xfb, C.cp());
   V2<T> cl = xmul_t(xfa, c);
@@ -139,8 +141,8 @@ B2G_HD void collide_polygon_circle(Man<T>& m, const Geo<T>& P, XF<T> xfa, const 
 }
 
 // collision_util.py:31-52
-template <typename T>
-B2G_HD T edge_separation(const Geo<T>& p1, XF<T> xf1, int edge1, const Geo<T>& p2, XF<T> xf2) {
+template <typename T, typename S>
+B2G_HD T edge_separation(const Geo<T, S>& p1, XF<T> xf1, int edge1, const Geo<T, S>& p2, XF<T> xf2) {
   V2<T> n1w = rmul(xf1.q, p1.nrm(edge1));
   V2<T> n1 = rmul_t(xf2.q, n1w);
   int index = 0;
@@ -158,8 +160,8 @@ B2G_HD T edge_separation(const Geo<T>& p1, XF<T> xf1, int edge1, const Geo<T>& p
 }
 
 // collision_util.py:54-109
-template <typename T>
-B2G_HD T max_separation(const Geo<T>& p1, XF<T> xf1, const Geo<T>& p2, XF<T> xf2, int& edge_out) {
+template <typename T, typename S>
+B2G_HD T max_separation(const Geo<T, S>& p1, XF<T> xf1, const Geo<T, S>& p2, XF<T> xf2, int& edge_out) {
   const int count1 = p1.n;
   V2<T> d = xmul(xf2, p2.centroid()) - xmul(xf1, p1.centroid());
   V2<T> dl = rmul_t(xf1.q, d);
@@ -206,8 +208,8 @@ B2G_HD T max_separation(const Geo<T>& p1, XF<T> xf1, const Geo<T>& p2, XF<T> xf2
 }
 
 // collision_util.py:111-133
-template <typename T>
-B2G_HD void find_incident_edge(ClipV<T>& c0, ClipV<T>& c1, const Geo<T>& p1, XF<T> xf1, int edge1, const Geo<T>& p2,
+template <typename T, typename S>
+B2G_HD void find_incident_edge(ClipV<T>& c0, ClipV<T>& c1, const Geo<T, S>& p1, XF<T> xf1, int edge1, const Geo<T, S>& p2,
                                XF<T> xf2) {
   V2<T> n1 = rmul_t(xf2.q, rmul(xf1.q, p1.nrm(edge1)));
   int i1 = 0;
@@ -254,8 +256,8 @@ B2G_HD int clip_segment_to_line(ClipV<T>& o0, ClipV<T>& o1, ClipV<T> i0, ClipV<T
   return n;
 }
 
-template <typename T>
-B2G_HD void collide_polygons(Man<T>& m, const Geo<T>& A, XF<T> xfa, const Geo<T>& B, XF<T> xfb) {
+template <typename T, typename S>
+B2G_HD void collide_polygons(Man<T>& m, const Geo<T, S>& A, XF<T> xfa, const Geo<T, S>& B, XF<T> xfb) {
   m.pc = 0;
   T total_radius = A.radius() + B.radius();
   int edge_a, edge_b;
@@ -264,8 +266,8 @@ B2G_HD void collide_polygons(Man<T>& m, const Geo<T>& A, XF<T> xfa, const Geo<T>
   T sb = max_separation(B, xfb, A, xfa, edge_b);
   if (sb > total_radius) return;
   const bool flip = sb > Consts<T>::RELATIVE_TOL * sa + Consts<T>::ABSOLUTE_TOL;
-  const Geo<T>& p1 = flip ? B : A;
-  const Geo<T>& p2 = flip ? A : B;
+  const Geo<T, S>& p1 = flip ? B : A;
+  const Geo<T, S>& p2 = flip ? A : B;
   XF<T> xf1 = flip ? xfb : xfa;
   XF<T> xf2 = flip ? xfa : xfb;
   int edge1 = flip ? edge_b : edge_a;
@@ -311,8 +313,8 @@ B2G_HD void collide_polygons(Man<T>& m, const Geo<T>& A, XF<T> xfa, const Geo<T>
   m.pc = pc;
 }
 
-template <typename T>
-B2G_HD void collide_edge_circle(Man<T>& m, const Geo<T>& E, XF<T> xfa, const Geo<T>& C, XF<T> xfb) {
+template <typename T, typename S>
+B2G_HD void collide_edge_circle(Man<T>& m, const Geo<T, S>& E, XF<T> xfa, const Geo<T, S>& C, XF<T> xfb) {
   m.pc = 0;
   V2<T> Q = xmul_t(xfa, xmul(xfb, C.cp()));
   V2<T> A = E.e(1), B = E.e(2);
@@ -368,10 +370,10 @@ B2G_HD void collide_edge_circle(Man<T>& m, const Geo<T>& E, XF<T> xfa, const Geo
 }
 
 // ---- EPCollider (collision.py:355-690) ----
-template <typename T>
+template <typename T, typename S>
 struct EPC {
   XFM<T> xf;
-  Geo<T> P;
+  Geo<T, S> P;
   V2<T> v0, v1, v2, v3, normal;
   T radius;
   V2<T> l11, l12, l21, l22;
@@ -389,8 +391,8 @@ struct EPC {
   }
 };
 
-template <typename T>
-B2G_HD void ep_adjacency(EPC<T>& c, bool has0, bool has3) {  // collision.py:541-604
+template <typename T, typename S>
+B2G_HD void ep_adjacency(EPC<T, S>& c, bool has0, bool has3) {  // collision.py:541-604
   V2<T> cb = xmul(c.xf, c.P.centroid());
   if (has0) {
     V2<T> e0 = c.v1 - c.v0, e1 = c.v2 - c.v1;
@@ -430,9 +432,9 @@ B2G_HD void ep_adjacency(EPC<T>& c, bool has0, bool has3) {  // collision.py:541
 
 enum { EP_UNKNOWN = 0, EP_EDGE_A = 1, EP_EDGE_B = 2 };
 
-template <typename T>
-B2G_HD void collide_edge_polygon(Man<T>& m, const Geo<T>& E, XF<T> xfa, const Geo<T>& P, XF<T> xfb) {
-  EPC<T> c;
+template <typename T, typename S>
+B2G_HD void collide_edge_polygon(Man<T>& m, const Geo<T, S>& E, XF<T> xfa, const Geo<T, S>& P, XF<T> xfb) {
+  EPC<T, S> c;
   c.xf = xmul_t_xf(xfa, xfb);
   c.P = P;
   c.v0 = E.e(0);
@@ -565,8 +567,8 @@ B2G_HD void collide_edge_polygon(Man<T>& m, const Geo<T>& E, XF<T> xfa, const Ge
 }
 
 // Contact.evaluate dispatch for registry-oriented fixtures (contact.py:343-373).
-template <typename T>
-B2G_HD bool evaluate(Man<T>& m, const Geo<T>& A, XF<T> xfa, const Geo<T>& B, XF<T> xfb) {
+template <typename T, typename S>
+B2G_HD bool evaluate(Man<T>& m, const Geo<T, S>& A, XF<T> xfa, const Geo<T, S>& B, XF<T> xfb) {
   const int ta = A.type, tb = B.type;
   if (ta == SHAPE_CIRCLE && tb == SHAPE_CIRCLE) collide_circles(m, A, xfa, B, xfb);
   else if (ta == SHAPE_POLYGON && tb == SHAPE_POLYGON) collide_polygons(m, A, xfa, B, xfb);

@@ -190,6 +190,31 @@ struct Dev {
   PV<T>* pv;   // indexed like i_bodies
   VC<T>* vc;   // indexed like i_contacts
   PC<T>* pc;
+  // worlds too large for the LDS-resident kernels (k_islands_big, k_pairs_big_*);
+  // the scratch below is allocated only when the batch has such worlds
+  const int* w_big;    // per world: 1 = large path
+  const int* big_list; // the large worlds
+  int* g_bst;          // per body: state bits as in k_islands
+  int* g_wake;         // per body: slot of the contact that woke it (awake gating)
+  int* g_off;          // per world nb + 1 at w_body_off[w] + w: CSR offsets
+  int* g_cur;          // per body
+  int* g_lev;          // per body
+  int* g_parent;       // per body: union-find
+  int* g_seed;         // per body (by component root)
+  int* g_cntb;         // per body (by component root)
+  int* g_cntc;         // per body (by component root)
+  int* g_bidx;         // per body: island-local index
+  int2* g_adj;         // 2 per contact slot at 2 * w_con_base[w]: (slot, other body)
+  int* g_cflag;        // per contact slot
+  int* g_stack;        // per world nb + 2 * ccap at w_body_off[w] + 2 * w_con_base[w]
+  int4* g_isl;         // per body: island records (root, bstart, cstart, seed)
+  int* g_wmisc;        // per world: 16 ints
+  int* g_mpos;         // per fixture
+  int* g_mlist;        // per fixture
+  uint32_t* g_hash;    // per world next_pow2(ccap + 1) at g_hbase[w]
+  uint32_t* g_nlist;   // per world next_pow2(ccap) at g_nbase[w]
+  const long long* g_hbase;
+  const long long* g_nbase;
 };
 
 // ---------------------------------------------------------------------------
@@ -209,8 +234,8 @@ constexpr double LINEAR_SLEEP_TOL = 0.01;
 constexpr double ANGULAR_SLEEP_TOL = 2.0 / 180.0 * kPi;
 
 template <typename T>
-__device__ __forceinline__ Geo<T> geo_of(const Dev<T>& d, int fg) {
-  Geo<T> g;
+__device__ __forceinline__ Geo<double, T> geo_of(const Dev<T>& d, int fg) {
+  Geo<double, T> g;
   int4 info = d.f_info[fg];
   g.g = d.f_geo + (size_t)fg * GEO_STRIDE;
   g.type = info.y & 255;
@@ -244,28 +269,31 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
   }
   double4 A = d.f_fat[fa], B = d.f_fat[fb];
   if (!aabb_overlap(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w)) return 0;  // destroyed
-  Man<T> m;
+  // The manifold is evaluated in FP64 from the stored (T) geometry and stored
+  // back as T: FP32 arithmetic in the clipping and frame changes would set the
+  // FP32 pipeline's one-step error (5e-5 on stacked boxes), FP32 storage does not.
+  Man<double> m;
   m.type = (st >> CS_MTYPE_SHIFT) & 3;
   m.pc = (st >> CS_PC_SHIFT) & 3;
   V4<T> lnlp = d.c_lnlp[s], pts = d.c_pts[s], imp = d.c_imp[s];
   uint2 ids = d.c_id[s];
-  m.ln = v2<T>(lnlp.x, lnlp.y);
-  m.lp = v2<T>(lnlp.z, lnlp.w);
-  m.p0 = v2<T>(pts.x, pts.y);
-  m.p1 = v2<T>(pts.z, pts.w);
+  m.ln = v2<double>(lnlp.x, lnlp.y);
+  m.lp = v2<double>(lnlp.z, lnlp.w);
+  m.p0 = v2<double>(pts.x, pts.y);
+  m.p1 = v2<double>(pts.z, pts.w);
   m.id0 = ids.x;
   m.id1 = ids.y;
   const int old_pc = m.pc;
   const uint32_t oid0 = ids.x, oid1 = ids.y;
   double4 xa = d.b_xf[ba], xb = d.b_xf[bb];
-  XF<T> xfa, xfb;
-  xfa.p = v2<T>(0, 0);
-  xfa.q.c = (T)xa.z;
-  xfa.q.s = (T)xa.w;
-  xfb.p = v2<T>((T)(xb.x - xa.x), (T)(xb.y - xa.y));
-  xfb.q.c = (T)xb.z;
-  xfb.q.s = (T)xb.w;
-  Geo<T> ga = geo_of(d, fa), gb = geo_of(d, fb);
+  XF<double> xfa, xfb;  // relative to body A's position
+  xfa.p = v2<double>(0, 0);
+  xfa.q.c = xa.z;
+  xfa.q.s = xa.w;
+  xfb.p = v2<double>(xb.x - xa.x, xb.y - xa.y);
+  xfb.q.c = xb.z;
+  xfb.q.s = xb.w;
+  const Geo<double, T> ga = geo_of(d, fa), gb = geo_of(d, fb);
   if (!evaluate(m, ga, xfa, gb, xfb)) {
     atomicOr(&d.w_err[w], WERR_UNSUPPORTED);
     return st;
@@ -286,8 +314,8 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
   nimp.z = m.pc >= 2 ? n1 : imp.z;
   nimp.w = m.pc >= 2 ? t1 : imp.w;
   d.c_imp[s] = nimp;
-  lnlp.x = m.ln.x; lnlp.y = m.ln.y; lnlp.z = m.lp.x; lnlp.w = m.lp.y;
-  pts.x = m.p0.x; pts.y = m.p0.y; pts.z = m.p1.x; pts.w = m.p1.y;
+  lnlp.x = (T)m.ln.x; lnlp.y = (T)m.ln.y; lnlp.z = (T)m.lp.x; lnlp.w = (T)m.lp.y;
+  pts.x = (T)m.p0.x; pts.y = (T)m.p0.y; pts.z = (T)m.p1.x; pts.w = (T)m.p1.y;
   d.c_lnlp[s] = lnlp;
   d.c_pts[s] = pts;
   ids.x = m.id0;
@@ -370,7 +398,9 @@ __host__ __device__ inline size_t island_smem_bytes(int nb, int nf, int ccap) {
 }
 
 // Exclusive scan of a[0..n) in place by one workgroup; returns the total.
+template <int B = kBlock>
 __device__ int block_exclusive_scan(int* a, int n, int* scratch) {
+  constexpr int kBlock = B;
   const int tid = threadIdx.x;
   const int per = (n + kBlock - 1) / kBlock;
   const int beg = tid * per, end = min(n, beg + per);
@@ -400,6 +430,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = blockIdx.x, tid = threadIdx.x;
+  if (d.w_big && d.w_big[w]) return;  // k_islands_big
   const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
   const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
   const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
@@ -1171,31 +1202,36 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
   double wa, wb;
   load_vel(d, bstart, b0, ia, va, wa);
   load_vel(d, bstart, b0, ib, vb, wb);
-  Rot<T> qa = rot_of<T>(aa), qb = rot_of<T>(ab);
-  V2<T> dcb = v2<T>((T)(cbx - cax), (T)(cby - cay));
-  XF<T> xfa, xfb;
+  // The constraint geometry is computed in FP64 in both pipelines and stored as
+  // T: it runs once per contact and step, and FP32 arithmetic here (rotations,
+  // world manifold, effective masses) is what would otherwise set the FP32
+  // pipeline's one-step error; rounding the results to FP32 is harmless.
+  const Rot<double> qa = rot_of<double>(aa), qb = rot_of<double>(ab);
+  const V2d dcb = v2<double>(cbx - cax, cby - cay);
+  XF<double> xfa, xfb;
   xfa.q = qa;
-  xfa.p = -rmul(qa, v2<T>(pcn.lcax, pcn.lcay));
+  xfa.p = -rmul(qa, v2<double>(lca.x, lca.y));
   xfb.q = qb;
-  xfb.p = dcb - rmul(qb, v2<T>(pcn.lcbx, pcn.lcby));
-  V2<T> w0 = v2<T>(0, 0), w1 = v2<T>(0, 0);
-  V2<T> normal = world_manifold<T>(mtype, pc, v2<T>(lnlp.x, lnlp.y), v2<T>(lnlp.z, lnlp.w), v2<T>(pts.x, pts.y),
-                                   v2<T>(pts.z, pts.w), xfa, pcn.ra, xfb, pcn.rb, w0, w1);
-  vc.nx = normal.x;
-  vc.ny = normal.y;
+  xfb.p = dcb - rmul(qb, v2<double>(lcb.x, lcb.y));
+  V2d w0 = v2<double>(0, 0), w1 = v2<double>(0, 0);
+  const V2d normal = world_manifold<double>(mtype, pc, dv2(lnlp.x, lnlp.y), dv2(lnlp.z, lnlp.w), dv2(pts.x, pts.y),
+                                            dv2(pts.z, pts.w), xfa, d.f_rad64[fa], xfb, d.f_rad64[fb], w0, w1);
+  vc.nx = (T)normal.x;
+  vc.ny = (T)normal.y;
   const T ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
-  const V2d nd = dv2(normal.x, normal.y);
-  auto init_point = [&](VCP<T>& p, V2<T> wp) {
-    V2<T> ra = wp, rb = wp - dcb;
-    T rna = sq(cross(ra, normal)), rnb = sq(cross(rb, normal));
-    T kn = ma + mb + Ia * rna + Ib * rnb;
-    V2<T> tangent = cross_vs(normal, (T)1);
-    T rta = sq(cross(ra, tangent)), rtb = sq(cross(rb, tangent));
-    T kt = ma + mb + Ia * rta + Ib * rtb;
-    double vrel = dot(nd, vb + cross_sv(wb, dv2(rb.x, rb.y)) - va - cross_sv(wa, dv2(ra.x, ra.y)));
-    p.rax = ra.x; p.ray = ra.y; p.rbx = rb.x; p.rby = rb.y;
-    p.nm = kn > (T)0 ? (T)1 / kn : (T)0;
-    p.tm = kt > (T)0 ? (T)1 / kt : (T)0;
+  const V2d nd = dv2(vc.nx, vc.ny);
+  auto init_point = [&](VCP<T>& p, V2d wp) {
+    const double mad = ma, mbd = mb, Iad = Ia, Ibd = Ib;
+    V2d ra = wp, rb = wp - dcb;
+    double rna = sq(cross(ra, normal)), rnb = sq(cross(rb, normal));
+    double kn = mad + mbd + Iad * rna + Ibd * rnb;
+    V2d tangent = cross_vs(normal, 1.0);
+    double rta = sq(cross(ra, tangent)), rtb = sq(cross(rb, tangent));
+    double kt = mad + mbd + Iad * rta + Ibd * rtb;
+    p.rax = (T)ra.x; p.ray = (T)ra.y; p.rbx = (T)rb.x; p.rby = (T)rb.y;
+    double vrel = dot(nd, vb + cross_sv(wb, dv2(p.rbx, p.rby)) - va - cross_sv(wa, dv2(p.rax, p.ray)));
+    p.nm = kn > 0.0 ? (T)(1.0 / kn) : (T)0;
+    p.tm = kt > 0.0 ? (T)(1.0 / kt) : (T)0;
     p.bias = vrel < -VELOCITY_THRESHOLD ? (T)(-(double)vc.restitution * vrel) : (T)0;
     p.pad = 0;
   };
@@ -1737,10 +1773,29 @@ __device__ void copy_contact(const Dev<T>& d, int dst, int src) {
   d.c_id[dst] = d.c_id[src];
 }
 
+// Contact._create (contact.py:313-326) + Contact.__init__ (contact.py:55-90):
+// the registry orientation, mixed friction and restitution, an empty manifold.
+template <typename T>
+__device__ __forceinline__ void create_contact(const Dev<T>& d, int f0, int s, int P, int R) {
+  const int ta = d.f_info[f0 + P].y & 255, tb = d.f_info[f0 + R].y & 255;
+  const bool swap = (ta == SHAPE_CIRCLE && tb == SHAPE_POLYGON) || (ta == SHAPE_CIRCLE && tb == SHAPE_EDGE) ||
+                    (ta == SHAPE_POLYGON && tb == SHAPE_EDGE);
+  const int fa = swap ? R : P, fb = swap ? P : R;
+  d.c_pair[s] = make_int2(fa, fb);
+  d.c_state[s] = CS_LIVE | (M_CIRCLES << CS_MTYPE_SHIFT);
+  V2<T> ma = d.f_mat[f0 + fa], mb = d.f_mat[f0 + fb];
+  d.c_mat[s] = v2<T>(sqrt(ma.x * mb.x), pmax(ma.y, mb.y));
+  d.c_lnlp[s] = V4<T>{0, 0, 0, 0};
+  d.c_pts[s] = V4<T>{0, 0, 0, 0};
+  d.c_imp[s] = V4<T>{0, 0, 0, 0};
+  d.c_id[s] = make_uint2(0, 0);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int from_upload, int finish, int do_pairs) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = blockIdx.x, tid = threadIdx.x;
+  if (d.w_big && d.w_big[w]) return;  // k_pairs_big_*
   const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
   const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
   const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
@@ -1905,23 +1960,9 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int from_upload, int
       if (tid == 0) atomicOr(&d.w_err[w], WERR_CAPACITY);
       nnew = 0;
     }
-    // Contact._create (contact.py:313-326) + Contact.__init__ (contact.py:55-90)
     for (int k = tid; k < nnew; k += kBlock) {
-      uint32_t key = nlist[k];
-      const int P = mlist[key >> 16], R = (int)(key & 0xFFFFu);
-      const int ta = d.f_info[f0 + P].y & 255, tb = d.f_info[f0 + R].y & 255;
-      const bool swap = (ta == SHAPE_CIRCLE && tb == SHAPE_POLYGON) || (ta == SHAPE_CIRCLE && tb == SHAPE_EDGE) ||
-                        (ta == SHAPE_POLYGON && tb == SHAPE_EDGE);
-      const int fa = swap ? R : P, fb = swap ? P : R;
-      const int s = cb + used + k;
-      d.c_pair[s] = make_int2(fa, fb);
-      d.c_state[s] = CS_LIVE | (M_CIRCLES << CS_MTYPE_SHIFT);
-      V2<T> ma = d.f_mat[f0 + fa], mb = d.f_mat[f0 + fb];
-      d.c_mat[s] = v2<T>(sqrt(ma.x * mb.x), pmax(ma.y, mb.y));
-      d.c_lnlp[s] = V4<T>{0, 0, 0, 0};
-      d.c_pts[s] = V4<T>{0, 0, 0, 0};
-      d.c_imp[s] = V4<T>{0, 0, 0, 0};
-      d.c_id[s] = make_uint2(0, 0);
+      const uint32_t key = nlist[k];
+      create_contact(d, f0, cb + used + k, (int)mlist[key >> 16], (int)(key & 0xFFFFu));
     }
     __syncthreads();
     if (tid == 0) d.w_con_used[w] = used + nnew;
@@ -1948,6 +1989,525 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int from_upload, int
   __syncthreads();
   if (tid == 0) {
     d.w_asleep[w] = misc[2];
+    if (d.w_flags[w] & WF_NEW_FIXTURE) d.w_flags[w] &= ~WF_NEW_FIXTURE;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Worlds too large for one workgroup's LDS (e.g. one world of 11 200 proxies)
+//
+// The island search and the pair search of such worlds keep their state in
+// global memory.  The awake gating becomes a fixed point over "the slot of the
+// contact that woke the body" (a contact is evaluated when one of its bodies
+// was awake before its slot, as in the in-order loop); the islands are the
+// union-find components, over touching contacts between non-static bodies,
+// that hold an awake body; each island's depth-first search then runs on its
+// own lane from the island's first awake body, which visits bodies and
+// contacts in exactly the order the world-wide search would.  New pairs come
+// from a many-workgroup brute-force FP64 overlap test, are sorted into the
+// canonical order and created by one workgroup per world.
+// ---------------------------------------------------------------------------
+constexpr int kBigBlock = 1024;
+constexpr int kIntMax = 0x7fffffff;
+
+__device__ __forceinline__ int g_ld(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void g_st(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ int uf_find(int* parent, int x) {
+  for (;;) {
+    const int p = g_ld(&parent[x]);
+    if (p == x) return x;
+    const int gp = g_ld(&parent[p]);
+    if (gp != p) g_st(&parent[x], gp);  // path halving; x is not a root, so no hook targets it
+    x = gp;
+  }
+}
+__device__ void uf_unite(int* parent, int a, int b) {
+  for (;;) {
+    a = uf_find(parent, a);
+    b = uf_find(parent, b);
+    if (a == b) return;
+    if (a < b) { const int t = a; a = b; b = t; }
+    if (atomicCAS(&parent[a], a, b) == a) return;  // the root with the larger index joins the smaller
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
+  __shared__ int scan[kBigBlock];
+  __shared__ int sh[8];
+  const int w = d.big_list[blockIdx.x], tid = threadIdx.x;
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
+  const int f0 = d.w_fix_off[w];
+  const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
+  const int used = d.w_con_used[w];
+  int* bst = d.g_bst + b0;
+  int* wake = d.g_wake + b0;
+  int* off = d.g_off + b0 + w;
+  int* cur = d.g_cur + b0;
+  int* lev = d.g_lev + b0;
+  int* parent = d.g_parent + b0;
+  int* seed = d.g_seed + b0;
+  int* cntb = d.g_cntb + b0;
+  int* cntc = d.g_cntc + b0;
+  int* bidx = d.g_bidx + b0;
+  int2* adj = d.g_adj + 2 * (size_t)cb;
+  int* cflag = d.g_cflag + cb;
+  int4* isl = d.g_isl + b0;
+  int* isl_b = d.i_bodies + b0;
+  int* isl_c = d.i_contacts + cb;
+  for (int i = tid; i < nb; i += kBigBlock) {
+    const int fl = d.b_flags[b0 + i], ty = d.b_type[b0 + i];
+    bst[i] = (ty == BT_STATIC ? 1 : 0) | ((fl & BF_AWAKE) ? 2 : 0) | ((fl & BF_ALLOW_SLEEP) ? 4 : 0) |
+             ((fl & BF_ACTIVE) ? 8 : 0);
+    off[i] = 0;
+    lev[i] = 0;
+    parent[i] = i;
+    seed[i] = kIntMax;
+    cntb[i] = 0;
+    cntc[i] = 0;
+  }
+  for (int sl = tid; sl < used; sl += kBigBlock) cflag[sl] = 0;
+  if (tid < 8) sh[tid] = 0;
+  __syncthreads();
+
+  // --- awake gating (contact_manager.py:191-197) as a fixed point over wake slots ---
+  if (d.w_asleep[w] > 0) {
+    for (int i = tid; i < nb; i += kBigBlock) wake[i] = (bst[i] & 2) ? -1 : kIntMax;
+    __syncthreads();
+    for (;;) {
+      if (tid == 0) sh[0] = 0;
+      __syncthreads();
+      for (int sl = tid; sl < used; sl += kBigBlock) {
+        const int s = cb + sl;
+        int st = d.c_state[s];
+        if (!(st & CS_LIVE)) continue;
+        const int2 pr = d.c_pair[s];
+        const int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
+        if (!(st & CS_PROCESSED)) {
+          const bool act = (!(bst[la] & 1) && g_ld(&wake[la]) < sl) || (!(bst[lb] & 1) && g_ld(&wake[lb]) < sl);
+          if (!act) continue;
+          st = narrow_update(d, w, s, st);
+          d.c_state[s] = st;
+          if (!(st & CS_LIVE)) continue;
+        }
+        if (st & CS_WOKE) {
+          if (atomicMin(&wake[la], sl) > sl) sh[0] = 1;
+          if (atomicMin(&wake[lb], sl) > sl) sh[0] = 1;
+        }
+      }
+      __syncthreads();
+      const int changed = sh[0];
+      __syncthreads();
+      if (!changed) break;
+    }
+    for (int i = tid; i < nb; i += kBigBlock) {
+      const int wk = wake[i];
+      if (wk >= 0 && wk < kIntMax && !(bst[i] & 2)) bst[i] |= 2 | 32;
+    }
+    __syncthreads();
+  }
+
+  // --- touching-contact graph in ContactManager order ---
+  for (int sl = tid; sl < used; sl += kBigBlock) {
+    const int st = d.c_state[cb + sl];
+    if ((st & (CS_LIVE | CS_TOUCHING | CS_ENABLED)) == (CS_LIVE | CS_TOUCHING | CS_ENABLED)) {
+      const int2 pr = d.c_pair[cb + sl];
+      const int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
+      atomicAdd(&off[la], 1);
+      atomicAdd(&off[lb], 1);
+      if (!(bst[la] & 1) && !(bst[lb] & 1)) uf_unite(parent, la, lb);
+    }
+  }
+  __syncthreads();
+  const int total = block_exclusive_scan<kBigBlock>(off, nb, scan);
+  if (tid == 0) off[nb] = total;
+  for (int i = tid; i < nb; i += kBigBlock) cur[i] = off[i];
+  __syncthreads();
+  for (int sl = tid; sl < used; sl += kBigBlock) {
+    const int st = d.c_state[cb + sl];
+    if ((st & (CS_LIVE | CS_TOUCHING | CS_ENABLED)) == (CS_LIVE | CS_TOUCHING | CS_ENABLED)) {
+      const int2 pr = d.c_pair[cb + sl];
+      const int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
+      adj[atomicAdd(&cur[la], 1)] = make_int2(sl, lb);
+      adj[atomicAdd(&cur[lb], 1)] = make_int2(sl, la);
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < nb; i += kBigBlock) {  // ascending slot order per body; component roots
+    cur[i] = 0;  // from here on: contacts of the body seen by the island search
+    const int beg = off[i], end = off[i + 1];
+    for (int k = beg + 1; k < end; ++k) {
+      const int2 e = adj[k];
+      int j = k - 1;
+      while (j >= beg && adj[j].x > e.x) {
+        adj[j + 1] = adj[j];
+        --j;
+      }
+      adj[j + 1] = e;
+    }
+    if (!(bst[i] & 1)) parent[i] = uf_find(parent, i);
+  }
+  __syncthreads();
+  // --- islands: components with an awake, active, non-static body; seed = the first such body ---
+  for (int i = tid; i < nb; i += kBigBlock) {
+    const int bs = bst[i];
+    if ((bs & 1) || !(bs & 2) || !(bs & 8)) continue;
+    atomicMin(&seed[parent[i]], i);
+  }
+  __syncthreads();
+  for (int i = tid; i < nb; i += kBigBlock)
+    if (!(bst[i] & 1) && seed[parent[i]] != kIntMax) atomicAdd(&cntb[parent[i]], 1);
+  for (int sl = tid; sl < used; sl += kBigBlock) {
+    const int st = d.c_state[cb + sl];
+    if ((st & (CS_LIVE | CS_TOUCHING | CS_ENABLED)) == (CS_LIVE | CS_TOUCHING | CS_ENABLED)) {
+      const int2 pr = d.c_pair[cb + sl];
+      int x = d.f_info[f0 + pr.x].x;
+      if (bst[x] & 1) x = d.f_info[f0 + pr.y].x;
+      if (seed[parent[x]] != kIntMax) atomicAdd(&cntc[parent[x]], 1);
+    }
+  }
+  __syncthreads();
+  unsigned long long* alloc = (unsigned long long*)(d.g_wmisc + 16 * w);
+  if (tid == 0) *alloc = 0ull;
+  __syncthreads();
+  for (int i = tid; i < nb; i += kBigBlock) {
+    if ((bst[i] & 1) || parent[i] != i || seed[i] == kIntMax) continue;
+    const unsigned long long r = atomicAdd(alloc, ((unsigned long long)cntb[i] << 32) | (unsigned)cntc[i]);
+    const int k = atomicAdd(&sh[1], 1);
+    isl[k] = make_int4(i, (int)(r >> 32), (int)(r & 0xffffffffu), seed[i]);
+  }
+  __syncthreads();
+  const int nisl = sh[1];
+  const int nbi_tot = (int)(*alloc >> 32), nci_tot = (int)(*alloc & 0xffffffffu);
+  // --- depth-first search of each island (world.py:494-580), one lane per island ---
+  for (int k = tid; k < nisl; k += kBigBlock) {
+    const int4 r = isl[k];
+    const int bstart = r.y, cstart = r.z, s0 = r.w;
+    const int cap_b = cntb[r.x], cap_c = cntc[r.x];
+    int* stack = d.g_stack + b0 + 2 * (size_t)cb + bstart + 2 * cstart;  // cap_b + 2 cap_c entries
+    int* statics = stack + cap_b + 2 * cap_c;                              // grows downwards
+    int nstat = 0, sp = 0, nbi = 0, nci = 0, last_allow = 0;
+    stack[sp++] = s0;
+    bst[s0] |= 16;
+    bool bad = false;
+    while (sp > 0 && !bad) {
+      const int b = stack[--sp];
+      int bs = bst[b];
+      last_allow = (bs >> 2) & 1;
+      if (!(bs & 2)) bs |= 2 | 32;  // body.awake = True
+      bst[b] = bs;
+      if (bs & 1) continue;  // static bodies join but do not propagate
+      if (nbi >= cap_b) { bad = true; break; }
+      bidx[b] = bstart + nbi;  // position in the world's island-body list, like the LDS search
+      isl_b[bstart + nbi++] = b;
+      for (int e = off[b]; e < off[b + 1]; ++e) {
+        const int2 a = adj[e];
+        if (cflag[a.x]) continue;
+        if (nci >= cap_c || sp + nstat + 2 > cap_b + 2 * cap_c) { bad = true; break; }
+        cflag[a.x] = 1;
+        const int o = a.y;
+        const bool o_dyn = !(bst[o] & 1);
+        const uint32_t rank_b = (uint32_t)cur[b]++;
+        const uint32_t rank_o = o_dyn ? (uint32_t)cur[o]++ : 0u;
+        const int lo = o_dyn ? lev[o] : 0;
+        const int lvl = 1 + (lev[b] > lo ? lev[b] : lo);
+        lev[b] = lvl;
+        if (o_dyn) lev[o] = lvl;
+        d.i_rank[cb + cstart + nci] = make_uint2((uint32_t)b, rank_b | rank_o << 16);
+        d.i_order[cb + cstart + nci] = (uint16_t)(lvl - 1);  // level, until sorted below
+        isl_c[cstart + nci++] = a.x;
+        if (o_dyn) {
+          if (bst[o] & 16) continue;
+          bst[o] |= 16;
+        } else {
+          bool seen = false;
+          for (int q = 1; q <= nstat; ++q) seen = seen || statics[-q] == o;
+          if (seen) continue;
+          statics[-(++nstat)] = o;
+        }
+        stack[sp++] = o;
+      }
+    }
+    if (bad || nbi != cap_b || nci != cap_c) {  // the component sizes and the search disagree
+      atomicOr(&d.w_err[w], WERR_SCHEDULE);
+      for (int q = nci; q < cap_c; ++q) isl_c[cstart + q] = 0;  // keep later passes in range
+      nbi = nci = 0;
+    }
+    d.i_info[b0 + k] = make_int4(bstart, nbi, cstart, nci | (last_allow << 30));
+    // the island's contacts by level (counting sort in the stack area)
+    int* hist = stack;
+    for (int q = 0; q < nci; ++q) hist[q] = 0;
+    for (int q = 0; q < nci; ++q) hist[d.i_order[cb + cstart + q]] += 1;
+    int run = 0;
+    for (int q = 0; q < nci; ++q) {
+      const int v = hist[q];
+      hist[q] = run;
+      run += v;
+    }
+    for (int q = 0; q < nci; ++q) {
+      const int lv = d.i_order[cb + cstart + q];
+      stack[nci + hist[lv]] = q;
+      hist[lv] += 1;
+    }
+    for (int q = 0; q < nci; ++q) d.i_order[cb + cstart + q] = (uint16_t)stack[nci + q];
+  }
+  __syncthreads();
+  // --- queue, island-local contact ends, ranks oriented as (body A, body B) ---
+  if (tid == 0) sh[2] = nisl > 0 ? atomicAdd(d.q_count, nisl) : 0;
+  if (tid < kSizeClasses) scan[tid] = 0;
+  __syncthreads();
+  const int qbase = sh[2];
+  for (int k = tid; k < nisl; k += kBigBlock) {
+    const int4 in = d.i_info[b0 + k];
+    d.q_isl[qbase + k] = make_int4(w, b0 + in.x, in.y, cb + in.z);
+    d.q_nc[qbase + k] = in.w;
+    atomicAdd(&scan[size_class(in.w & 0x3FFFFFFF)], 1);
+  }
+  for (int k = tid; k < nci_tot; k += kBigBlock) {
+    const int2 pr = d.c_pair[cb + isl_c[k]];
+    const int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
+    const int ia = (bst[la] & 1) ? -1 - la : bidx[la];
+    const int ib = (bst[lb] & 1) ? -1 - lb : bidx[lb];
+    d.i_cidx[cb + k] = make_int2(ia, ib);
+    const uint2 r = d.i_rank[cb + k];
+    const uint32_t lo = r.y & 0xFFFFu, hi = r.y >> 16;
+    d.i_rank[cb + k] = make_uint2((int)r.x == la ? (lo | hi << 16) : (hi | lo << 16), 0u);
+  }
+  __syncthreads();
+  if (tid < kSizeClasses && scan[tid] > 0) atomicAdd(&d.q_cls[tid], scan[tid]);
+  for (int i = tid; i < nb; i += kBigBlock) {
+    const int bs = bst[i];
+    int fl = d.b_flags[b0 + i];
+    if (bs & 32) {  // woken: Body.awake setter (body.py:800-812)
+      fl |= BF_AWAKE;
+      d.b_sleep[b0 + i] = 0.0;
+      d.b_flags[b0 + i] = fl;
+    }
+    d.b_island[b0 + i] = ((bs & 16) && !(bs & 1)) ? 1 : -1;
+  }
+  if (tid == 0) d.w_nisl[w] = nisl;
+  (void)nbi_tot;
+  (void)ccap;
+}
+
+// Move buffer and the hash of existing contacts of each large world.
+template <typename T>
+__global__ void __launch_bounds__(kBigBlock) k_pairs_big_prep(Dev<T> d, int from_upload, int do_pairs) {
+  __shared__ int scan[kBigBlock];
+  const int w = d.big_list[blockIdx.x], tid = threadIdx.x;
+  const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
+  const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
+  int* mpos = d.g_mpos + f0;
+  int* mlist = d.g_mlist + f0;
+  int* misc = d.g_wmisc + 16 * w;  // [2] moves, [3] new pairs, [4] overflow
+  int nmove = 0;
+  if (!do_pairs) {
+  } else if (from_upload) {
+    nmove = d.w_move_n[w];
+    for (int i = tid; i < nf; i += kBigBlock) mpos[i] = -1;
+    __syncthreads();
+    if (tid == 0)
+      for (int k = 0; k < nmove; ++k) {
+        const int f = d.mv[f0 + k];
+        mlist[k] = f;
+        if (mpos[f] < 0) mpos[f] = k;
+      }
+  } else {
+    for (int i = tid; i < nf; i += kBigBlock) mpos[i] = d.f_moved[f0 + i] ? 1 : 0;
+    __syncthreads();
+    nmove = block_exclusive_scan<kBigBlock>(mpos, nf, scan);
+    for (int i = tid; i < nf; i += kBigBlock) {
+      if (d.f_moved[f0 + i]) mlist[mpos[i]] = i;
+      else mpos[i] = -1;
+    }
+  }
+  if (tid == 0) {
+    misc[2] = nmove;
+    misc[3] = 0;
+    misc[4] = 0;
+  }
+  if (nmove > 0) {
+    const int hcap = next_pow2(ccap + 1);
+    uint32_t* hash = d.g_hash + d.g_hbase[w];
+    for (int i = tid; i < hcap; i += kBigBlock) hash[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    const int used = d.w_con_used[w];
+    for (int sl = tid; sl < used; sl += kBigBlock) {
+      if (!(d.c_state[cb + sl] & CS_LIVE)) continue;
+      const int2 pr = d.c_pair[cb + sl];
+      const uint32_t k = hkey(pr.x, pr.y);
+      uint32_t h = hhash(k) & (uint32_t)(hcap - 1);
+      for (;;) {
+        const uint32_t prev = atomicCAS(&hash[h], 0xFFFFFFFFu, k);
+        if (prev == 0xFFFFFFFFu || prev == k) break;
+        h = (h + 1) & (uint32_t)(hcap - 1);
+      }
+    }
+  }
+}
+
+// Every moved proxy of a large world against every proxy of the world (exact
+// FP64 tests), with the filters of find_new_contacts; grid (chunks, worlds).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pairs_big_find(Dev<T> d) {
+  const int w = d.big_list[blockIdx.y];
+  const int b0 = d.w_body_off[w];
+  const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
+  const int ccap = d.w_con_cap[w];
+  int* misc = d.g_wmisc + 16 * w;
+  const int nmove = misc[2];
+  const int* mpos = d.g_mpos + f0;
+  const int* mlist = d.g_mlist + f0;
+  const int hcap = next_pow2(ccap + 1);
+  const uint32_t* hash = d.g_hash + d.g_hbase[w];
+  const int ncap = next_pow2(ccap);
+  uint32_t* nlist = d.g_nlist + d.g_nbase[w];
+  const long long work = (long long)nmove * nf;
+  for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < work; t += (long long)gridDim.x * kBlock) {
+    const int i = (int)(t / nf), R = (int)(t % nf);
+    const int P = mlist[i];
+    if (R == P) continue;
+    const int mr = mpos[R];
+    if (mr >= 0 && mr < i) continue;  // (R, P) is already in the pair set
+    const double4 fp = d.f_fat[f0 + P], fr = d.f_fat[f0 + R];
+    if (!aabb_overlap(fr.x, fr.y, fr.z, fr.w, fp.x, fp.y, fp.z, fp.w)) continue;
+    const int4 ip = d.f_info[f0 + P], ir = d.f_info[f0 + R];
+    if (ip.x == ir.x) continue;  // same body
+    const int tb_p = d.b_type[b0 + ip.x], tb_r = d.b_type[b0 + ir.x];
+    if (!(d.b_flags[b0 + ip.x] & BF_ACTIVE) || !(d.b_flags[b0 + ir.x] & BF_ACTIVE)) continue;
+    {
+      const uint32_t k = hkey(P, R);
+      uint32_t h = hhash(k) & (uint32_t)(hcap - 1);
+      bool found = false;
+      for (;;) {
+        const uint32_t v = hash[h];
+        if (v == 0xFFFFFFFFu) break;
+        if (v == k) { found = true; break; }
+        h = (h + 1) & (uint32_t)(hcap - 1);
+      }
+      if (found) continue;
+    }
+    if (tb_r != BT_DYNAMIC && tb_p != BT_DYNAMIC) continue;  // Body._should_collide
+    bool ok;  // World.default_contact_filter(fixture P, fixture R)
+    if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
+    else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
+    if (!ok) continue;
+    const int ta = ip.y & 255, tb = ir.y & 255;
+    const bool reg = (ta == SHAPE_CIRCLE || ta == SHAPE_POLYGON || ta == SHAPE_EDGE) &&
+                     (tb == SHAPE_CIRCLE || tb == SHAPE_POLYGON || tb == SHAPE_EDGE) &&
+                     !(ta == SHAPE_EDGE && tb == SHAPE_EDGE);
+    if (!reg) continue;
+    const int slot = atomicAdd(&misc[3], 1);
+    if (slot < ncap) nlist[slot] = ((uint32_t)i << 16) | (uint32_t)R;
+    else atomicOr(&misc[4], 1);
+  }
+}
+
+// Canonical order, contact creation and end-of-step bookkeeping of each large world.
+template <typename T>
+__global__ void __launch_bounds__(kBigBlock) k_pairs_big_finish(Dev<T> d, int finish) {
+  __shared__ int scan[kBigBlock];
+  __shared__ int sh[4];
+  const int w = d.big_list[blockIdx.x], tid = threadIdx.x;
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
+  const int f0 = d.w_fix_off[w];
+  const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
+  int* misc = d.g_wmisc + 16 * w;
+  const int* mlist = d.g_mlist + f0;
+  uint32_t* nlist = d.g_nlist + d.g_nbase[w];
+  int nnew = misc[3];
+  if (misc[4]) {
+    if (tid == 0) atomicOr(&d.w_err[w], WERR_NEWPAIRS);
+    nnew = 0;
+  }
+  if (nnew > 0) {
+    // canonical order: move-buffer position, then partner index (bitonic sort)
+    const int np = next_pow2(nnew);
+    for (int i = nnew + tid; i < np; i += kBigBlock) nlist[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    for (int k = 2; k <= np; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = tid; i < np; i += kBigBlock) {
+          const int ixj = i ^ j;
+          if (ixj > i) {
+            const uint32_t a = nlist[i], b = nlist[ixj];
+            const bool up = (i & k) == 0;
+            if ((a > b) == up) { nlist[i] = b; nlist[ixj] = a; }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    int used = d.w_con_used[w];
+    if (used + nnew > ccap) {  // squeeze out destroyed slots
+      int base = 0;
+      for (int c0 = 0; c0 < used; c0 += kBigBlock) {
+        const int sl = c0 + tid;
+        const bool live = sl < used && (d.c_state[cb + sl] & CS_LIVE);
+        int2 pr = make_int2(0, 0);
+        int st = 0;
+        V2<T> mat = v2<T>(0, 0);
+        V4<T> lnlp{}, pts{}, imp{};
+        uint2 id = make_uint2(0, 0);
+        if (live) {
+          pr = d.c_pair[cb + sl]; st = d.c_state[cb + sl]; mat = d.c_mat[cb + sl];
+          lnlp = d.c_lnlp[cb + sl]; pts = d.c_pts[cb + sl]; imp = d.c_imp[cb + sl]; id = d.c_id[cb + sl];
+        }
+        scan[tid] = live ? 1 : 0;
+        __syncthreads();
+        for (int o = 1; o < kBigBlock; o <<= 1) {
+          const int v = tid >= o ? scan[tid - o] : 0;
+          __syncthreads();
+          scan[tid] += v;
+          __syncthreads();
+        }
+        const int dst = base + scan[tid] - (live ? 1 : 0);
+        const int cnt = scan[kBigBlock - 1];
+        __syncthreads();
+        if (live) {
+          const int s = cb + dst;
+          d.c_pair[s] = pr; d.c_state[s] = st; d.c_mat[s] = mat; d.c_lnlp[s] = lnlp; d.c_pts[s] = pts;
+          d.c_imp[s] = imp; d.c_id[s] = id;
+        }
+        base += cnt;
+        __syncthreads();
+      }
+      used = base;
+    }
+    if (used + nnew > ccap) {
+      if (tid == 0) atomicOr(&d.w_err[w], WERR_CAPACITY);
+      nnew = 0;
+    }
+    for (int k = tid; k < nnew; k += kBigBlock) create_contact(d, f0, cb + used + k, mlist[nlist[k] >> 16],
+                                                               (int)(nlist[k] & 0xFFFFu));
+    __syncthreads();
+    if (tid == 0) d.w_con_used[w] = used + nnew;
+  }
+  if (tid == 0) d.w_move_n[w] = 0;
+  if (!finish) return;
+  if (tid == 0) {
+    if (d.dt > 0.0) d.w_inv_dt0[w] = d.inv_dt;
+    sh[0] = 0;
+  }
+  __syncthreads();
+  const bool clear = !(d.w_flags[w] & WF_KEEP_FORCES);
+  int asleep = 0;
+  for (int i = tid; i < nb; i += kBigBlock) {
+    const int gb = b0 + i;
+    if (clear) {
+      d.b_f[gb] = v2<T>(0, 0);
+      d.b_t[gb] = (T)0;
+    }
+    if (d.b_type[gb] != BT_STATIC && !(d.b_flags[gb] & BF_AWAKE)) ++asleep;
+  }
+  if (asleep) atomicAdd(&sh[0], asleep);
+  __syncthreads();
+  if (tid == 0) {
+    d.w_asleep[w] = sh[0];
     if (d.w_flags[w] & WF_NEW_FIXTURE) d.w_flags[w] &= ~WF_NEW_FIXTURE;
   }
 }
@@ -2014,7 +2574,7 @@ __global__ void k_narrow_seam(int n, const T* geo_a, const int* info_a, const T*
                               const double* xfa, const double* xfb, int* out_i, T* out_t) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  Geo<T> A, B;
+  Geo<double, T> A, B;  // evaluated in FP64 from T geometry, as the step does
   A.g = geo_a + (size_t)k * GEO_STRIDE;
   A.type = info_a[k] & 255;
   A.n = (info_a[k] >> 8) & 255;
@@ -2025,16 +2585,16 @@ __global__ void k_narrow_seam(int n, const T* geo_a, const int* info_a, const T*
   B.n = (info_b[k] >> 8) & 255;
   B.has0 = (info_b[k] >> 16) & 1;
   B.has3 = (info_b[k] >> 17) & 1;
-  XF<T> ta, tb;
-  ta.p = v2<T>(0, 0);
-  ta.q.c = (T)xfa[4 * k + 2];
-  ta.q.s = (T)xfa[4 * k + 3];
-  tb.p = v2<T>((T)(xfb[4 * k] - xfa[4 * k]), (T)(xfb[4 * k + 1] - xfa[4 * k + 1]));
-  tb.q.c = (T)xfb[4 * k + 2];
-  tb.q.s = (T)xfb[4 * k + 3];
-  Man<T> m;
+  XF<double> ta, tb;
+  ta.p = v2<double>(0, 0);
+  ta.q.c = xfa[4 * k + 2];
+  ta.q.s = xfa[4 * k + 3];
+  tb.p = v2<double>(xfb[4 * k] - xfa[4 * k], xfb[4 * k + 1] - xfa[4 * k + 1]);
+  tb.q.c = xfb[4 * k + 2];
+  tb.q.s = xfb[4 * k + 3];
+  Man<double> m;
   m.type = 0; m.pc = 0;
-  m.ln = m.lp = m.p0 = m.p1 = v2<T>(0, 0);
+  m.ln = m.lp = m.p0 = m.p1 = v2<double>(0, 0);
   m.id0 = m.id1 = 0;
   bool ok = evaluate(m, A, ta, B, tb);
   out_i[4 * k] = ok ? m.type : -1;
@@ -2042,8 +2602,8 @@ __global__ void k_narrow_seam(int n, const T* geo_a, const int* info_a, const T*
   out_i[4 * k + 2] = (int)m.id0;
   out_i[4 * k + 3] = (int)m.id1;
   T* o = out_t + (size_t)k * 8;
-  o[0] = m.ln.x; o[1] = m.ln.y; o[2] = m.lp.x; o[3] = m.lp.y;
-  o[4] = m.p0.x; o[5] = m.p0.y; o[6] = m.p1.x; o[7] = m.p1.y;
+  o[0] = (T)m.ln.x; o[1] = (T)m.ln.y; o[2] = (T)m.lp.x; o[3] = (T)m.lp.y;
+  o[4] = (T)m.p0.x; o[5] = (T)m.p0.y; o[6] = (T)m.p1.x; o[7] = (T)m.p1.y;
 }
 
 // pair seam: exact FP64 fat-AABB tests of a move buffer against n proxies;
@@ -2088,6 +2648,8 @@ struct b2g_ctx {
   double contact_factor = 8.0;
   int contact_extra = 64;
   // solver launch shape (b2g_set_param)
+  int large_worlds = 0;  // 1: every world takes the global-memory island and pair search
+  std::vector<int> big;  // worlds on that path
   int heavy_class = 5;
   int team_full_class = 9;
   int team_lds = 32768;
@@ -2205,19 +2767,28 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
     ctx->max_nb = std::max(ctx->max_nb, wnb);
     ctx->max_nf = std::max(ctx->max_nf, wnf);
     ctx->max_ccap = std::max(ctx->max_ccap, cap);
-    if (wnb >= 65535 || wnf >= 65535 || cap >= 65535)
-      return set_error(ctx, B2G_E_UNSUPPORTED, "world too large for the LDS-resident path (>= 65535 entries)");
+    if (wnb >= 65535 || wnf >= 65535)
+      return set_error(ctx, B2G_E_UNSUPPORTED, "world too large (>= 65535 bodies or fixtures)");
   }
   ctx->con_base[nw] = ncap;
   ctx->ncap = ncap;
+  // worlds whose island or pair search does not fit one workgroup's LDS (or whose
+  // contact slots exceed 16-bit indices) take the global-memory path
+  ctx->big.clear();
+  std::vector<int> w_big(nw, 0);
   ctx->smem_islands = 0;
-  for (int w = 0; w < nw; ++w)
-    ctx->smem_islands = std::max(ctx->smem_islands,
-                                 island_smem_bytes(ctx->body_off[w + 1] - ctx->body_off[w],
-                                                   ctx->fix_off[w + 1] - ctx->fix_off[w], ctx->con_cap[w]));
-  ctx->smem_pairs = pairs_smem_bytes(ctx->max_nf, ctx->max_ccap);
-  if (ctx->smem_islands > (size_t)kMaxLds || ctx->smem_pairs > (size_t)kMaxLds)
-    return set_error(ctx, B2G_E_UNSUPPORTED, "world too large for the LDS-resident path (LDS > 160 KB)");
+  ctx->smem_pairs = 0;
+  for (int w = 0; w < nw; ++w) {
+    const int wnb = ctx->body_off[w + 1] - ctx->body_off[w], wnf = ctx->fix_off[w + 1] - ctx->fix_off[w];
+    const size_t si = island_smem_bytes(wnb, wnf, ctx->con_cap[w]), sp = pairs_smem_bytes(wnf, ctx->con_cap[w]);
+    if (ctx->large_worlds || ctx->con_cap[w] >= 65535 || si > (size_t)kMaxLds || sp > (size_t)kMaxLds) {
+      w_big[w] = 1;
+      ctx->big.push_back(w);
+    } else {
+      ctx->smem_islands = std::max(ctx->smem_islands, si);
+      ctx->smem_pairs = std::max(ctx->smem_pairs, sp);
+    }
+  }
   HIPCHK(hipFuncSetAttribute((const void*)k_solve<T>, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
 
   HIPCHK(hipFuncSetAttribute((const void*)k_islands<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2452,6 +3023,51 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.pv, nb));
   TRY(dalloc(ctx, &d.vc, ncap));
   TRY(dalloc(ctx, &d.pc, ncap));
+  std::vector<long long> hb(nw, 0), nbse(nw, 0);  // kept alive until the copies below complete
+  std::vector<int> zero16(16 * (size_t)nw, 0);
+  if (!ctx->big.empty()) {  // scratch of the large-world path
+    int *wb = nullptr, *bl = nullptr;
+    TRY(dalloc(ctx, &wb, nw));
+    TRY(dalloc(ctx, &bl, ctx->big.size()));
+    TRY(upload(ctx, wb, w_big.data(), nw));
+    TRY(upload(ctx, bl, ctx->big.data(), ctx->big.size()));
+    d.w_big = wb;
+    d.big_list = bl;
+    TRY(dalloc(ctx, &d.g_bst, nb));
+    TRY(dalloc(ctx, &d.g_wake, nb));
+    TRY(dalloc(ctx, &d.g_off, (size_t)nb + nw));
+    TRY(dalloc(ctx, &d.g_cur, nb));
+    TRY(dalloc(ctx, &d.g_lev, nb));
+    TRY(dalloc(ctx, &d.g_parent, nb));
+    TRY(dalloc(ctx, &d.g_seed, nb));
+    TRY(dalloc(ctx, &d.g_cntb, nb));
+    TRY(dalloc(ctx, &d.g_cntc, nb));
+    TRY(dalloc(ctx, &d.g_bidx, nb));
+    TRY(dalloc(ctx, &d.g_adj, 2 * (size_t)ncap));
+    TRY(dalloc(ctx, &d.g_cflag, ncap));
+    TRY(dalloc(ctx, &d.g_stack, (size_t)nb + 2 * (size_t)ncap));
+    TRY(dalloc(ctx, &d.g_isl, nb));
+    TRY(dalloc(ctx, &d.g_wmisc, 16 * (size_t)nw));
+    TRY(dalloc(ctx, &d.g_mpos, nf));
+    TRY(dalloc(ctx, &d.g_mlist, nf));
+    long long ht = 0, nt = 0;
+    for (int w : ctx->big) {
+      hb[w] = ht;
+      nbse[w] = nt;
+      ht += next_pow2(ctx->con_cap[w] + 1);
+      nt += next_pow2(ctx->con_cap[w]);
+    }
+    long long *hbd = nullptr, *nbd = nullptr;
+    TRY(dalloc(ctx, &hbd, nw));
+    TRY(dalloc(ctx, &nbd, nw));
+    TRY(upload(ctx, hbd, hb.data(), nw));
+    TRY(upload(ctx, nbd, nbse.data(), nw));
+    d.g_hbase = hbd;
+    d.g_nbase = nbd;
+    TRY(dalloc(ctx, &d.g_hash, (size_t)ht));
+    TRY(dalloc(ctx, &d.g_nlist, (size_t)nt));
+    TRY(upload(ctx, d.g_wmisc, zero16.data(), 16 * (size_t)nw));
+  }
 
   TRY(upload(ctx, w_body_off, b->body_off, nw + 1));
   TRY(upload(ctx, w_fix_off, b->fix_off, nw + 1));
@@ -2518,9 +3134,16 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   d.team_lds = ctx->team_lds;
   const int nw = ctx->nw;
   hipStream_t st = ctx->stream;
-  if (first_pairs) {
-    hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, 1, 0, 1);
-  }
+  const int nbig = (int)ctx->big.size();
+  auto pairs = [&](int from_upload, int finish, int do_pairs) {
+    hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, from_upload, finish, do_pairs);
+    if (nbig > 0) {
+      hipLaunchKernelGGL(k_pairs_big_prep<T>, dim3(nbig), dim3(kBigBlock), 0, st, d, from_upload, do_pairs);
+      hipLaunchKernelGGL(k_pairs_big_find<T>, dim3(256, nbig), dim3(kBlock), 0, st, d);
+      hipLaunchKernelGGL(k_pairs_big_finish<T>, dim3(nbig), dim3(kBigBlock), 0, st, d, finish);
+    }
+  };
+  if (first_pairs) pairs(1, 0, 1);
   if (timed) HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(k_reset_queue<T>, dim3(1), dim3(64), 0, st, d);
   const int chunks = (ctx->max_ccap + kBlock - 1) / kBlock;
@@ -2528,6 +3151,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   if (timed) HIPCHK(hipEventRecord(ev[1], st));
   if (dt > 0.0) {
     hipLaunchKernelGGL(k_islands<T>, dim3(nw), dim3(kBlock), ctx->smem_islands, st, d);
+    if (nbig > 0) hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), 0, st, d);
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[2], st));
@@ -2536,14 +3160,14 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     if (timed) HIPCHK(hipEventRecord(ev[3], st));
     hipLaunchKernelGGL(k_sync<T>, dim3(std::max(1, (ctx->nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[4], st));
-    hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, 0, 1, 1);
+    pairs(0, 1, 1);
   } else {
     if (timed) {
       HIPCHK(hipEventRecord(ev[2], st));
       HIPCHK(hipEventRecord(ev[3], st));
       HIPCHK(hipEventRecord(ev[4], st));
     }
-    hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, 0, 1, 0);
+    pairs(0, 1, 0);
   }
   if (timed) HIPCHK(hipEventRecord(ev[5], st));
   HIPCHK(hipGetLastError());
@@ -2851,6 +3475,7 @@ B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value) {
   switch (key) {
     case B2G_PARAM_HEAVY_CLASS: slot = &ctx->heavy_class; lo = 0; hi = kSizeClasses; break;
     case B2G_PARAM_TEAM_FULL_CLASS: slot = &ctx->team_full_class; lo = 0; hi = kSizeClasses; break;
+    case B2G_PARAM_LARGE_WORLDS: slot = &ctx->large_worlds; lo = 0; hi = 1; break;
     case B2G_PARAM_TEAM_LDS: slot = &ctx->team_lds; lo = 0; hi = kMaxLds; break;
     case B2G_PARAM_SOLVE_GRID: slot = &ctx->solve_grid; lo = 1; hi = 1 << 30; break;
     default: return set_error(ctx, B2G_E_INVALID, "b2g_set_param: unknown key " + std::to_string(key));
@@ -2876,6 +3501,7 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value) {
   switch (key) {
     case B2G_PARAM_HEAVY_CLASS: *value = ctx->heavy_class; return B2G_OK;
     case B2G_PARAM_TEAM_FULL_CLASS: *value = ctx->team_full_class; return B2G_OK;
+    case B2G_PARAM_LARGE_WORLDS: *value = ctx->large_worlds; return B2G_OK;
     case B2G_PARAM_TEAM_LDS: *value = ctx->team_lds; return B2G_OK;
     case B2G_PARAM_SOLVE_GRID: *value = ctx->solve_grid; return B2G_OK;
     default: return set_error(ctx, B2G_E_INVALID, "b2g_get_param: unknown key " + std::to_string(key));

@@ -3,7 +3,8 @@
 Bars:
   * broad phase: the pair list equals the oracle's exactly (set and order);
   * narrow phase, FP64: integer fields exact, floats within 1e-9;
-    FP32: integer fields exact on >= 98% of the cases, floats within 2e-4;
+    FP32 (FP32 geometry and manifold storage, FP64 evaluation): integer fields
+    exact on >= 99.5% of the cases, floats within 2e-6;
   * one step from an identical state: relative error (floors: 1 m, 1 rad,
     |g| dt) <= 1e-10 in FP64 and <= 1e-5 in FP32 for positions, angles and
     velocities; contact lists identical;
@@ -72,7 +73,7 @@ def test_narrowphase_seam_matches_oracle(prec):
     ints_ok = np.all(ci[:, [L.CI_MTYPE, L.CI_PC, L.CI_ID0, L.CI_ID1]] ==
                      z["con_i"][:, [L.CI_MTYPE, L.CI_PC, L.CI_ID0, L.CI_ID1]], axis=1)
     pc_ok = ci[:, L.CI_PC] == z["con_i"][:, L.CI_PC]
-    tol = 1e-9 if prec == "fp64" else 2e-4
+    tol = 1e-9 if prec == "fp64" else 2e-6
     cols = [L.CD_LN_X, L.CD_LN_Y, L.CD_LP_X, L.CD_LP_Y]
     worst = 0.0
     for k in np.nonzero(ints_ok & (ci[:, L.CI_PC] > 0))[0]:
@@ -84,7 +85,7 @@ def test_narrowphase_seam_matches_oracle(prec):
     if prec == "fp64":
         assert ints_ok.all(), np.nonzero(~ints_ok)[0][:10]
     else:
-        assert ints_ok.mean() >= 0.98
+        assert ints_ok.mean() >= 0.995
     assert worst <= tol
 
 
@@ -204,7 +205,7 @@ def test_island_launch_shapes_agree_bit_exactly(prec):
     results = {}
     for name, params in (("lanes", {"heavy_class": 16}), ("teams64", {"heavy_class": 0, "team_full_class": 0}),
                          ("teams8", {"heavy_class": 0, "team_full_class": 16}), ("default", {}),
-                         ("fallback", {"heavy_class": 3, "team_lds": 4096})):
+                         ("fallback", {"heavy_class": 3, "team_lds": 4096}), ("large_worlds", {"large_worlds": 1})):
         e = N.Engine(prec)
         for k, v in params.items():
             e.set_param(k, v)
@@ -316,3 +317,55 @@ def test_stacks_2000_steps_settle_height():
     assert max(be[q] for q in ("position", "angle")) <= 1e-9
     assert abs(settle["fp64"] - settle["ref"]) <= 1e-9
     assert abs(settle["fp32"] - settle["ref"]) <= 1e-3
+
+
+def test_large_world_path_tracks_oracle():
+    """The global-memory island and pair search (forced for every world) over the pyramid's
+    600 steps, through contact creation, sleep and the awake gating, against the oracle."""
+    batch = builder.scene_batch(scenes.pyramid())
+    o = O.Oracle(batch)
+    e = N.Engine("fp64")
+    e.set_param("large_worlds", 1)
+    e.upload(batch)
+    asleep_steps = 0
+    for s in range(600):
+        o.step(DT, 8, 3)
+        e.step(DT, 8, 3)
+        ref = o.state()
+        asleep_steps += int(not np.all(ref.body_i[1:, L.BI_FLAGS] & L.BF_AWAKE))  # next step runs the gating
+        if s % 100 == 99:
+            got = e.download()
+            assert PU.same_contact_lists(got, ref), s
+            be = PU.body_errors(got, ref)
+            print("large-world path step", s + 1, be, flush=True)
+            assert be["position"] <= 1e-6, (s, be)
+    e.close()
+    print("steps that ended with a sleeping body:", asleep_steps, flush=True)
+    assert asleep_steps > 0
+
+
+def test_single_large_world_200_pyramids():
+    """One world of 200 pyramids (11 000 boxes, 200 ground edges; 11 200 proxies): too large for
+    the LDS kernels, so it runs on the global-memory path. FP64 tracks the oracle for 60 steps;
+    FP32 one step from there is within 1e-5."""
+    batch = builder.scene_batch(scenes.pyramids(200))
+    o = O.Oracle(batch)
+    e = N.Engine("fp64")
+    e.upload(batch)
+    for s in range(60):
+        o.step(DT, 8, 3)
+        e.step(DT, 8, 3)
+    got, ref = e.download(), o.state()
+    e.close()
+    assert PU.same_contact_lists(got, ref)
+    be = PU.body_errors(got, ref)
+    print("200 pyramids fp64, 60 steps", be, flush=True)
+    assert max(be[q] for q in ("position", "angle", "velocity", "angular_velocity")) <= 1e-9
+    ref.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    one = oracle_step(ref)
+    g32 = gpu_step(ref, "fp32")
+    assert PU.same_contact_lists(g32, one)
+    be = PU.body_errors(g32, one)
+    print("200 pyramids fp32 one step", be, flush=True)
+    for q in ("position", "angle", "velocity", "angular_velocity"):
+        assert be[q] <= 1e-5, (q, be[q])
