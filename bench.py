@@ -184,6 +184,25 @@ def one_step_error(state: L.BatchArrays, precision: str, worlds):
     return out
 
 
+def sampled_errors(batch: L.BatchArrays, steps: int, every: int) -> dict:
+    """One-step errors of the FP32 pipeline against the FP64 oracle from the FP64 pipeline's state
+    every `every` steps (a separate, untimed run)."""
+    from pypybox2d_amd._native import Engine
+    e = Engine("fp64")
+    e.upload(batch)
+    samples = []
+    for s in range(every, steps + 1, every):
+        e.step_n(every, graph=True)
+        st = e.download()
+        err = one_step_error(st, "fp32", list(range(st.n_worlds)))
+        samples.append({"step": s, **{k: (v["max"] if isinstance(v, dict) else v) for k, v in err.items()}})
+    e.close()
+    worst = {}
+    for q in ("position", "angle", "velocity", "angular_velocity", "impulse"):
+        worst[q] = max(smp[q] for smp in samples)
+    return {"samples": samples, "max": worst}
+
+
 def long_horizon(state: L.BatchArrays) -> dict:
     """Kinetic energy and mean height of the dynamic bodies (stacking is chaotic, so trajectories
     of different precisions are compared through such statistics)."""
@@ -223,6 +242,7 @@ def cpu_baseline(batch: L.BatchArrays, steps: int, seconds: float, procs: int):
     import multiprocessing as mp
     global _CPU_BATCH
     _CPU_BATCH = batch
+    procs = max(1, min(procs, batch.n_worlds))  # one world per process
     jobs = [(p, procs, steps, seconds) for p in range(procs)]
     if procs > 1:
         with mp.get_context("fork").Pool(procs) as pool:
@@ -262,10 +282,11 @@ def roofline(kernel: str, ms: float, counts: dict, fp64: bool, workload: str = "
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None, help="default 1000 (600 for pyramids)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=("mixed", "lognormal"), default="mixed",
-                    help="mixed: body counts U[32,512]; lognormal: median 64, sigma 0.75, clipped to [16,1024]")
+    ap.add_argument("--workload", choices=("mixed", "lognormal", "pyramids"), default="mixed",
+                    help="mixed: 4096 worlds, body counts U[32,512]; lognormal: 16384 worlds, median 64, sigma 0.75, "
+                         "clipped to [16,1024]; pyramids: one world of 200 ten-row box pyramids")
     ap.add_argument("--worlds", type=int, default=None, help="default 4096 (mixed) or 16384 (lognormal)")
     ap.add_argument("--seed", type=int, default=None, help="default 1 (mixed) or 2 (lognormal)")
     ap.add_argument("--fp64-steps", type=int, default=None, help="timed steps of the FP64 run (default: --steps)")
@@ -283,20 +304,28 @@ def main():
         sys.exit(2)
 
     if args.worlds is None:
-        args.worlds = 4096 if args.workload == "mixed" else 16384
+        args.worlds = {"mixed": 4096, "lognormal": 16384, "pyramids": 1}[args.workload]
     if args.seed is None:
         args.seed = 1 if args.workload == "mixed" else 2
+    if args.steps is None:
+        args.steps = 600 if args.workload == "pyramids" else 1000
     t0 = time.perf_counter()
-    if args.workload == "mixed":
-        counts = scenes.mixed_batch_counts(args.worlds, args.seed)
-        desc = ("mixed_batch: %d worlds, body counts U[32,512] (seed %d), boxes/circles/polygons 40/30/30, "
-                "dt 1/60, 8/3 iterations" % (args.worlds, args.seed))
+    if args.workload == "pyramids":
+        args.worlds = 1
+        batch = builder.scene_batch(scenes.pyramids(200))
+        desc = ("pyramids: one world of 200 ten-row box pyramids (11 000 boxes, half-extent 0.5) on 200 static "
+                "ground edges 30 m apart, dt 1/60, 8/3 iterations")
     else:
-        counts = scenes.lognormal_batch_counts(args.worlds, args.seed)
-        desc = ("lognormal_batch: %d worlds, body counts clip(round(64 exp(0.75 N(0,1))), 16, 1024) (seed %d), "
-                "boxes/circles/polygons 40/30/30, dt 1/60, 8/3 iterations" % (args.worlds, args.seed))
-    log("building %d worlds (%d dynamic bodies) on the host" % (args.worlds, int(counts.sum())))
-    batch = builder.batch_from_counts(counts, args.seed)
+        if args.workload == "mixed":
+            counts = scenes.mixed_batch_counts(args.worlds, args.seed)
+            desc = ("mixed_batch: %d worlds, body counts U[32,512] (seed %d), boxes/circles/polygons 40/30/30, "
+                    "dt 1/60, 8/3 iterations" % (args.worlds, args.seed))
+        else:
+            counts = scenes.lognormal_batch_counts(args.worlds, args.seed)
+            desc = ("lognormal_batch: %d worlds, body counts clip(round(64 exp(0.75 N(0,1))), 16, 1024) (seed %d), "
+                    "boxes/circles/polygons 40/30/30, dt 1/60, 8/3 iterations" % (args.worlds, args.seed))
+        log("building %d worlds (%d dynamic bodies) on the host" % (args.worlds, int(counts.sum())))
+        batch = builder.batch_from_counts(counts, args.seed)
     log("built in %.1f s" % (time.perf_counter() - t0))
     dyn = int(np.count_nonzero(batch.body_i[:, L.BI_TYPE] == L.DYNAMIC))
 
@@ -334,6 +363,9 @@ def main():
         if n64 == args.steps:
             line["long_horizon"] = {"fp32": long_horizon(st32), "fp64": long_horizon(st64),
                                     "steps": args.warmup + args.steps}
+    if args.workload == "pyramids" and args.error_worlds > 0:
+        line["one_step_error_fp32_vs_fp64_oracle_every_50"] = sampled_errors(batch, args.warmup + args.steps, 50)
+        log("sampled one-step errors:", line["one_step_error_fp32_vs_fp64_oracle_every_50"]["max"])
     rng = np.random.default_rng(0)
     worlds = sorted(rng.choice(args.worlds, size=min(args.error_worlds, args.worlds), replace=False).tolist())
     for prec, st in ((("fp32", st32),) + ((("fp64", st64),) if not args.no_fp64 else ())) if worlds else ():
