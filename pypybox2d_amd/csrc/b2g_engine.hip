@@ -329,9 +329,14 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_collide(Dev<T> d) {
-  const int w = blockIdx.y;
-  const int sl = blockIdx.x * kBlock + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_collide(Dev<T> d, int chunks) {
+  // Workgroups are dealt to the 8 XCDs round-robin; all chunks of world w get
+  // workgroup ids = w (mod 8), so a world's fixtures and bodies are fetched
+  // into one XCD's L2 only.
+  const int b = blockIdx.x, xcd = b & 7, k = b >> 3;
+  const int w = (k / chunks) * 8 + xcd, chunk = k % chunks;
+  if (w >= d.nw) return;
+  const int sl = chunk * kBlock + threadIdx.x;
   if (sl >= d.w_con_used[w]) return;
   const int s = d.w_con_base[w] + sl;
   int st = d.c_state[s];
@@ -609,28 +614,32 @@ __global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
         }
         L.bidx[b] = (uint16_t)nbi_tot;
         isl_b[nbi_tot++] = b;
+        int cur_b = L.cur[b], lev_b = L.lev[b];  // the expanding body's counters stay in registers
         for (int k = L.off[b]; k < L.off[b + 1]; ++k) {
           const int s = L.adj_s[k];
           const uint32_t bit = 1u << (s & 31);
           if (L.cflag[s >> 5] & bit) continue;
           L.cflag[s >> 5] |= bit;
           const int o = L.adj_o[k];
-          const bool o_dyn = !(L.bst[o] & 1);
-          const uint32_t rank_b = (uint32_t)L.cur[b]++;
+          const uint8_t so = L.bst[o];
+          const bool o_dyn = !(so & 1);
+          const uint32_t rank_b = (uint32_t)cur_b++;
           const uint32_t rank_o = o_dyn ? (uint32_t)L.cur[o]++ : 0u;
           // level: 1 + the latest level among the contact's dynamic bodies (one sweep)
           const int lo = o_dyn ? L.lev[o] : 0;
-          const int lvl = 1 + (L.lev[b] > lo ? L.lev[b] : lo);
-          L.lev[b] = (uint16_t)lvl;
+          const int lvl = 1 + (lev_b > lo ? lev_b : lo);
+          lev_b = lvl;
           if (o_dyn) L.lev[o] = (uint16_t)lvl;
           const uint32_t key = (uint32_t)(cstart + lvl - 1);  // < cstart + island contacts
           d.i_rank[cb + nci_tot] = make_uint2((uint32_t)b | key << 16, rank_b | rank_o << 16);
           d.i_order[cb + nci_tot] = (uint16_t)(nci_tot - cstart);
           isl_c[nci_tot++] = s;
-          if (L.bst[o] & 16) continue;
+          if (so & 16) continue;
           L.stack[sp++] = (uint16_t)o;
-          L.bst[o] |= 16;
+          L.bst[o] = so | 16;
         }
+        L.cur[b] = cur_b;
+        L.lev[b] = (uint16_t)lev_b;
       }
       for (int k = 0; k < ns; ++k) L.bst[L.slist[k]] &= (uint8_t)~16;  // statics may join other islands
       info[nisl++] = make_int4(bstart, nbi_tot - bstart, cstart, (nci_tot - cstart) | (last_allow << 30));
@@ -3147,7 +3156,8 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   if (timed) HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(k_reset_queue<T>, dim3(1), dim3(64), 0, st, d);
   const int chunks = (ctx->max_ccap + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_collide<T>, dim3(std::max(chunks, 1), nw), dim3(kBlock), 0, st, d);
+  const int nch = std::max(chunks, 1);
+  hipLaunchKernelGGL(k_collide<T>, dim3((unsigned)(((nw + 7) / 8) * 8 * nch)), dim3(kBlock), 0, st, d, nch);
   if (timed) HIPCHK(hipEventRecord(ev[1], st));
   if (dt > 0.0) {
     hipLaunchKernelGGL(k_islands<T>, dim3(nw), dim3(kBlock), ctx->smem_islands, st, d);
