@@ -156,7 +156,9 @@ struct Dev {
   // fixtures
   int4* f_info;  // x body (world-local), y shape | nverts << 8 | edge flags << 16, z category | mask << 16, w group
   int2* f_wb;    // world, global body
-  T* f_geo;
+  double* f_geo;    // shape geometry in FP64 in both precisions: rounding vertices, normals or radii
+                    // to FP32 moves near-ties of reference-face choice and clipping (a different
+                    // manifold for the whole island), so the narrowphase reads exact inputs
   double* f_geo64;
   double* f_rad64;
   V2<T>* f_mat;
@@ -234,8 +236,8 @@ constexpr double LINEAR_SLEEP_TOL = 0.01;
 constexpr double ANGULAR_SLEEP_TOL = 2.0 / 180.0 * kPi;
 
 template <typename T>
-__device__ __forceinline__ Geo<double, T> geo_of(const Dev<T>& d, int fg) {
-  Geo<double, T> g;
+__device__ __forceinline__ Geo<double, double> geo_of(const Dev<T>& d, int fg) {
+  Geo<double, double> g;
   int4 info = d.f_info[fg];
   g.g = d.f_geo + (size_t)fg * GEO_STRIDE;
   g.type = info.y & 255;
@@ -293,7 +295,7 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
   xfb.p = v2<double>(xb.x - xa.x, xb.y - xa.y);
   xfb.q.c = xb.z;
   xfb.q.s = xb.w;
-  const Geo<double, T> ga = geo_of(d, fa), gb = geo_of(d, fb);
+  const Geo<double, double> ga = geo_of(d, fa), gb = geo_of(d, fb);
   if (!evaluate(m, ga, xfa, gb, xfb)) {
     atomicOr(&d.w_err[w], WERR_UNSUPPORTED);
     return st;
@@ -2579,11 +2581,11 @@ __global__ void k_order_islands(Dev<T> d) {
 // parity seams
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_narrow_seam(int n, const T* geo_a, const int* info_a, const T* geo_b, const int* info_b,
+__global__ void k_narrow_seam(int n, const double* geo_a, const int* info_a, const double* geo_b, const int* info_b,
                               const double* xfa, const double* xfb, int* out_i, T* out_t) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  Geo<double, T> A, B;  // evaluated in FP64 from T geometry, as the step does
+  Geo<double, double> A, B;  // FP64 geometry and evaluation, as the step does; the manifold is stored as T
   A.g = geo_a + (size_t)k * GEO_STRIDE;
   A.type = info_a[k] & 255;
   A.n = (info_a[k] >> 8) & 255;
@@ -2869,7 +2871,7 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   }
   std::vector<int4> f_info(nf);
   std::vector<int2> f_wb(nf);
-  std::vector<T> f_geo((size_t)nf * GEO_STRIDE, (T)0);
+  std::vector<double> f_geo((size_t)nf * GEO_STRIDE, 0.0);
   std::vector<double> f_geo64((size_t)nf * 16, 0.0), f_rad64(nf);
   std::vector<V2<T>> f_mat(nf);
   std::vector<double4> f_fat(nf);
@@ -2897,15 +2899,15 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
                             ri[B2G_FI_GROUP]);
       f_wb[i] = make_int2(w, b->body_off[w] + body);
       ctx->fix_body_global[i] = b->body_off[w] + body;
-      T* g = &f_geo[(size_t)i * GEO_STRIDE];
-      for (int k = 0; k < 16; ++k) g[GEO_VERTS + k] = (T)r[B2G_FD_VERTS + k];
-      for (int k = 0; k < 16; ++k) g[GEO_NORMALS + k] = (T)r[B2G_FD_NORMALS + k];
-      g[GEO_CENTROID] = (T)r[B2G_FD_CENTROID_X];
-      g[GEO_CENTROID + 1] = (T)r[B2G_FD_CENTROID_Y];
-      g[GEO_CIRCLE_P] = (T)r[B2G_FD_CIRCLE_PX];
-      g[GEO_CIRCLE_P + 1] = (T)r[B2G_FD_CIRCLE_PY];
-      for (int k = 0; k < 8; ++k) g[GEO_EDGE + k] = (T)r[B2G_FD_EDGE_V0X + k];
-      g[GEO_RADIUS] = (T)r[B2G_FD_RADIUS];
+      double* g = &f_geo[(size_t)i * GEO_STRIDE];
+      for (int k = 0; k < 16; ++k) g[GEO_VERTS + k] = r[B2G_FD_VERTS + k];
+      for (int k = 0; k < 16; ++k) g[GEO_NORMALS + k] = r[B2G_FD_NORMALS + k];
+      g[GEO_CENTROID] = r[B2G_FD_CENTROID_X];
+      g[GEO_CENTROID + 1] = r[B2G_FD_CENTROID_Y];
+      g[GEO_CIRCLE_P] = r[B2G_FD_CIRCLE_PX];
+      g[GEO_CIRCLE_P + 1] = r[B2G_FD_CIRCLE_PY];
+      for (int k = 0; k < 8; ++k) g[GEO_EDGE + k] = r[B2G_FD_EDGE_V0X + k];
+      g[GEO_RADIUS] = r[B2G_FD_RADIUS];
       double* g64 = &f_geo64[(size_t)i * 16];
       if (type == B2G_SHAPE_POLYGON) {
         for (int k = 0; k < 16; ++k) g64[k] = r[B2G_FD_VERTS + k];
@@ -3555,9 +3557,9 @@ B2G_API int b2g_narrowphase(int precision, int n, const double* fa_d, const int3
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return B2G_E_NODEVICE;
   const bool f64 = precision == B2G_FP64;
   const size_t ts = f64 ? sizeof(double) : sizeof(float);
-  std::vector<unsigned char> ga((size_t)n * GEO_STRIDE * ts), gb((size_t)n * GEO_STRIDE * ts);
+  std::vector<double> ga((size_t)n * GEO_STRIDE), gb((size_t)n * GEO_STRIDE);
   std::vector<int> ia(n), ib(n);
-  auto fill = [&](const double* fd, const int32_t* fi, std::vector<unsigned char>& g, std::vector<int>& info) {
+  auto fill = [&](const double* fd, const int32_t* fi, std::vector<double>& g, std::vector<int>& info) {
     for (int k = 0; k < n; ++k) {
       const double* r = fd + (size_t)k * B2G_FIX_ND;
       const int32_t* ri = fi + (size_t)k * B2G_FIX_NI;
@@ -3570,10 +3572,7 @@ B2G_API int b2g_narrowphase(int precision, int n, const double* fa_d, const int3
       tmp[GEO_CIRCLE_P + 1] = r[B2G_FD_CIRCLE_PY];
       for (int j = 0; j < 8; ++j) tmp[GEO_EDGE + j] = r[B2G_FD_EDGE_V0X + j];
       tmp[GEO_RADIUS] = r[B2G_FD_RADIUS];
-      for (int j = 0; j < GEO_STRIDE; ++j) {
-        if (f64) ((double*)g.data())[(size_t)k * GEO_STRIDE + j] = tmp[j];
-        else ((float*)g.data())[(size_t)k * GEO_STRIDE + j] = (float)tmp[j];
-      }
+      for (int j = 0; j < GEO_STRIDE; ++j) g[(size_t)k * GEO_STRIDE + j] = tmp[j];
       info[k] = ri[B2G_FI_SHAPE] | (ri[B2G_FI_NVERTS] << 8) | (ri[B2G_FI_EDGE_FLAGS] << 16);
     }
   };
@@ -3584,7 +3583,7 @@ B2G_API int b2g_narrowphase(int precision, int n, const double* fa_d, const int3
   int rc = B2G_OK;
   hipStream_t st = nullptr;
   if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return B2G_E_HIP;
-  if (hipMalloc(&dga, ga.size()) != hipSuccess || hipMalloc(&dgb, gb.size()) != hipSuccess ||
+  if (hipMalloc(&dga, ga.size() * 8) != hipSuccess || hipMalloc(&dgb, gb.size() * 8) != hipSuccess ||
       hipMalloc(&dia, n * 4) != hipSuccess || hipMalloc(&dib, n * 4) != hipSuccess ||
       hipMalloc(&dxa, (size_t)n * 32) != hipSuccess || hipMalloc(&dxb, (size_t)n * 32) != hipSuccess ||
       hipMalloc(&doi, (size_t)n * 16) != hipSuccess || hipMalloc(&dot_, (size_t)n * 8 * ts) != hipSuccess) {
@@ -3593,8 +3592,8 @@ B2G_API int b2g_narrowphase(int precision, int n, const double* fa_d, const int3
   std::vector<int> oi((size_t)n * 4);
   std::vector<unsigned char> ot((size_t)n * 8 * ts);
   if (!rc) {
-    hipMemcpyAsync(dga, ga.data(), ga.size(), hipMemcpyHostToDevice, st);
-    hipMemcpyAsync(dgb, gb.data(), gb.size(), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dga, ga.data(), ga.size() * 8, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dgb, gb.data(), gb.size() * 8, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dia, ia.data(), n * 4, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dib, ib.data(), n * 4, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dxa, xfa, (size_t)n * 32, hipMemcpyHostToDevice, st);
@@ -3605,8 +3604,8 @@ B2G_API int b2g_narrowphase(int precision, int n, const double* fa_d, const int3
                          (const int*)dia, (const double*)dgb, (const int*)dib, (const double*)dxa,
                          (const double*)dxb, (int*)doi, (double*)dot_);
     else
-      hipLaunchKernelGGL(k_narrow_seam<float>, dim3(blocks), dim3(128), 0, st, n, (const float*)dga,
-                         (const int*)dia, (const float*)dgb, (const int*)dib, (const double*)dxa,
+      hipLaunchKernelGGL(k_narrow_seam<float>, dim3(blocks), dim3(128), 0, st, n, (const double*)dga,
+                         (const int*)dia, (const double*)dgb, (const int*)dib, (const double*)dxa,
                          (const double*)dxb, (int*)doi, (float*)dot_);
     hipMemcpyAsync(oi.data(), doi, (size_t)n * 16, hipMemcpyDeviceToHost, st);
     hipMemcpyAsync(ot.data(), dot_, (size_t)n * 8 * ts, hipMemcpyDeviceToHost, st);

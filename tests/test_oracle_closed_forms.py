@@ -128,3 +128,37 @@ def test_pair_set_equals_brute_force(seed):
     keys = [(pos[a], b) for a, b in got.tolist()]
     assert keys == sorted(keys)
     assert len(set(map(tuple, got.tolist()))) == len(got)
+
+
+def _near_tie_state():
+    import os
+    from conftest import GOLDEN
+    z = np.load(os.path.join(GOLDEN, "near_tie_mixed3975.npz"))
+    st = L.BatchArrays(*[z[k] for k in L._FIELDS])
+    st.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    return st
+
+
+def _contact(st, fa, fb):
+    k = np.nonzero((st.con_i[:, L.CI_FA] == fa) & (st.con_i[:, L.CI_FB] == fb))[0]
+    assert len(k) == 1
+    return st.con_i[k[0]]
+
+
+def test_near_tie_fixture_flips_with_fp32_rounded_shapes():
+    """The recorded state sits on the touching threshold of contact (229, 290): exact shapes
+    give one touching point, the same shapes rounded to FP32 give none.  This is why the
+    device keeps shape geometry in FP64 in both pipelines."""
+    st = _near_tie_state()
+    o = O.Oracle(st)
+    o.step(DT, 8, 3)
+    exact = _contact(o.state(), 229, 290)
+    geo = [L.FD_RADIUS, L.FD_CENTROID_X, L.FD_CENTROID_Y, L.FD_CIRCLE_PX, L.FD_CIRCLE_PY] + \
+        list(range(L.FD_EDGE_V0X, L.FD_EDGE_V3Y + 1)) + list(range(L.FD_VERTS, L.FD_NORMALS + 16))
+    rounded = st.copy()
+    rounded.fix_d[:, geo] = rounded.fix_d[:, geo].astype(np.float32).astype(np.float64)
+    o2 = O.Oracle(rounded)
+    o2.step(DT, 8, 3)
+    flipped = _contact(o2.state(), 229, 290)
+    assert exact[L.CI_PC] == 1 and exact[L.CI_FLAGS] & L.CF_TOUCHING
+    assert flipped[L.CI_PC] == 0 and not flipped[L.CI_FLAGS] & L.CF_TOUCHING
