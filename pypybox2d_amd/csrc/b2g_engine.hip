@@ -2036,6 +2036,12 @@ __device__ int uf_find(int* parent, int x) {
     x = gp;
   }
 }
+// Root without path halving: once the unions are done, a halving write could replace a root
+// that another thread has just stored with a non-root ancestor.
+__device__ int uf_root(const int* parent, int x) {
+  for (int p = parent[x]; p != x; p = parent[x]) x = p;
+  return x;
+}
 __device__ void uf_unite(int* parent, int a, int b) {
   for (;;) {
     a = uf_find(parent, a);
@@ -2160,8 +2166,11 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
       }
       adj[j + 1] = e;
     }
-    if (!(bst[i] & 1)) parent[i] = uf_find(parent, i);
+    if (!(bst[i] & 1)) wake[i] = uf_root(parent, i);  // the wake slots are free again
   }
+  __syncthreads();
+  for (int i = tid; i < nb; i += kBigBlock)
+    if (!(bst[i] & 1)) parent[i] = wake[i];
   __syncthreads();
   // --- islands: components with an awake, active, non-static body; seed = the first such body ---
   for (int i = tid; i < nb; i += kBigBlock) {
