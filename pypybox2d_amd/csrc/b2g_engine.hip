@@ -196,6 +196,7 @@ struct Dev {
   // the scratch below is allocated only when the batch has such worlds
   const int* w_big;    // per world: 1 = large path
   const int* big_list; // the large worlds
+  const int* w_order;  // worlds by descending body count: the island search takes the longest first
   int* g_bst;          // per body: state bits as in k_islands
   int* g_wake;         // per body: slot of the contact that woke it (awake gating)
   int* g_off;          // per world nb + 1 at w_body_off[w] + w: CSR offsets
@@ -434,9 +435,9 @@ __device__ int block_exclusive_scan(int* a, int n, int* scratch) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_islands(Dev<T> d) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_islands(Dev<T> d) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int w = blockIdx.x, tid = threadIdx.x;
+  const int w = d.w_order[blockIdx.x], tid = threadIdx.x;
   if (d.w_big && d.w_big[w]) return;  // k_islands_big
   const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
   const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
@@ -3045,6 +3046,17 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.pc, ncap));
   std::vector<long long> hb(nw, 0), nbse(nw, 0);  // kept alive until the copies below complete
   std::vector<int> zero16(16 * (size_t)nw, 0);
+  std::vector<int> w_order(nw);
+  for (int w = 0; w < nw; ++w) w_order[w] = w;
+  std::stable_sort(w_order.begin(), w_order.end(), [&](int x, int y) {
+    return ctx->body_off[x + 1] - ctx->body_off[x] > ctx->body_off[y + 1] - ctx->body_off[y];
+  });
+  {
+    int* wo = nullptr;
+    TRY(dalloc(ctx, &wo, nw));
+    TRY(upload(ctx, wo, w_order.data(), nw));
+    d.w_order = wo;
+  }
   if (!ctx->big.empty()) {  // scratch of the large-world path
     int *wb = nullptr, *bl = nullptr;
     TRY(dalloc(ctx, &wb, nw));
