@@ -1004,9 +1004,9 @@ __device__ __forceinline__ void pos_of(const Dev<T>& d, int bstart, int i, const
 
 // One contact of solve_position_constraints (contact.py:809-864).
 template <typename T>
-__device__ __forceinline__ void solve_contact_position(const PC<T>& pc, double& cax, double& cay, double& aa,
-                                                       double& cbx, double& cby, double& ab, T& min_sep) {
-  for (int j = 0; j < pc.npts; ++j) {
+__device__ __forceinline__ void solve_point_position(const PC<T>& pc, int j, double& cax, double& cay, double& aa,
+                                                     double& cbx, double& cby, double& ab, T& min_sep) {
+  {
     Rot<T> qa = rot_of<T>(aa), qb = rot_of<T>(ab);
     V2<T> dcb = v2<T>((T)(cbx - cax), (T)(cby - cay));
     XF<T> xfa, xfb;
@@ -1051,6 +1051,15 @@ __device__ __forceinline__ void solve_contact_position(const PC<T>& pc, double& 
     cby += (double)(pc.mb * P.y);
     ab += (double)(pc.Ib * cross(rb, P));
   }
+}
+// The points are solved in order, each from the positions the previous one left
+// (written out rather than as a loop: a loop here makes the compiler copy the
+// team path's prefetched records at every branch of its pass).
+template <typename T>
+__device__ __forceinline__ void solve_contact_position(const PC<T>& pc, double& cax, double& cay, double& aa,
+                                                       double& cbx, double& cby, double& ab, T& min_sep) {
+  if (pc.npts > 0) solve_point_position(pc, 0, cax, cay, aa, cbx, cby, ab, min_sep);
+  if (pc.npts > 1) solve_point_position(pc, 1, cax, cay, aa, cbx, cby, ab, min_sep);
 }
 
 template <typename T>
@@ -1506,19 +1515,26 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
       nk = order[nj];
       if (mine > 1 && ns < nsw) nxt = d.vc[cstart + nk];
     }
+    // The pass test sits at the bottom and readiness is computed without branches: the
+    // compiler then keeps the prefetched records where they are instead of copying them at
+    // every join (inactive lanes and static sides read entry 0 and ignore it).
     int pass = 0;
-    while (wave_any(active && pass < cap)) {
+    for (bool more = wave_any(active); more; more = wave_any(active && pass < cap)) {
       ++pass;
-      if (active) {
+      {
         const int ia = cur.ia, ib = cur.ib;
         const int ra = s * (int)((unsigned)cur.rka >> 16) + (cur.rka & 0xFFFF);
         const int rb = s * (int)((unsigned)cur.rkb >> 16) + (cur.rkb & 0xFFFF);
-        const bool ready = (ia < 0 || lds_ld(&cnt[ia]) == ra) && (ib < 0 || lds_ld(&cnt[ib]) == rb);
+        const int xa = active && ia >= 0 ? ia : 0, xb = active && ib >= 0 ? ib : 0;
+        const int na = lds_ld(&cnt[xa]), nb_ = lds_ld(&cnt[xb]);
+        const bool ready = active & ((ia < 0) | (na == ra)) & ((ib < 0) | (nb_ == rb));
         if (ready) {
-          V2d va = v2<double>(0, 0), vb = v2<double>(0, 0);
-          double wa = 0.0, wb = 0.0;  // static bodies have zero velocity (checked at upload)
-          if (ia >= 0) { va = v2<double>(sx[3 * ia], sx[3 * ia + 1]); wa = sx[3 * ia + 2]; }
-          if (ib >= 0) { vb = v2<double>(sx[3 * ib], sx[3 * ib + 1]); wb = sx[3 * ib + 2]; }
+          const double pax = sx[3 * xa], pay = sx[3 * xa + 1], paw = sx[3 * xa + 2];
+          const double pbx = sx[3 * xb], pby = sx[3 * xb + 1], pbw = sx[3 * xb + 2];
+          // static bodies have zero velocity (checked at upload)
+          V2d va = v2<double>(ia >= 0 ? pax : 0.0, ia >= 0 ? pay : 0.0);
+          V2d vb = v2<double>(ib >= 0 ? pbx : 0.0, ib >= 0 ? pby : 0.0);
+          double wa = ia >= 0 ? paw : 0.0, wb = ib >= 0 ? pbw : 0.0;
           if (s + s0 == 0) {
             warm_contact(cur, va, wa, vb, wb);
           } else {
@@ -1582,29 +1598,29 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
       if (active && j + TS < nci) nxt = d.pc[cstart + order[j + TS]];
       int pass = 0;
       const int pcap = 2 * nci + 64;
-      while (wave_any(active && pass < pcap)) {
+      for (bool more = wave_any(active); more; more = wave_any(active && pass < pcap)) {
         ++pass;
-        if (active) {
-          const int ia = cur.ia, ib = cur.ib;
-          const int ra = cur.rka & 0xFFFF, rb = cur.rkb & 0xFFFF;
-          const bool ready = (ia < 0 || lds_ld(&cnt[ia]) == ra) && (ib < 0 || lds_ld(&cnt[ib]) == rb);
-          if (ready) {
-            double ax, ay, aa, bx, by, bb;
-            if (ia >= 0) { ax = sx[3 * ia]; ay = sx[3 * ia + 1]; aa = sx[3 * ia + 2]; }
-            else { ax = cur.scx; ay = cur.scy; aa = cur.sa; }
-            if (ib >= 0) { bx = sx[3 * ib]; by = sx[3 * ib + 1]; bb = sx[3 * ib + 2]; }
-            else { bx = cur.scx; by = cur.scy; bb = cur.sa; }
-            solve_contact_position(cur, ax, ay, aa, bx, by, bb, min_sep);
-            if (ia >= 0) { sx[3 * ia] = ax; sx[3 * ia + 1] = ay; sx[3 * ia + 2] = aa; lds_st(&cnt[ia], ra + 1); }
-            if (ib >= 0) { sx[3 * ib] = bx; sx[3 * ib + 1] = by; sx[3 * ib + 2] = bb; lds_st(&cnt[ib], rb + 1); }
-            j += TS;
-            if (j >= nci) {
-              active = false;
-            } else {
-              cur = nxt;
-              if (j + TS < nci) nxt = d.pc[cstart + order[j + TS]];
-            }
-          }
+        // readiness without branches (a branchy test makes the compiler copy the prefetched
+        // records at every join); inactive lanes and static sides read entry 0 and ignore it
+        const int ia = cur.ia, ib = cur.ib;
+        const int ra = cur.rka & 0xFFFF, rb = cur.rkb & 0xFFFF;
+        const int xa = active && ia >= 0 ? ia : 0, xb = active && ib >= 0 ? ib : 0;
+        const int na = lds_ld(&cnt[xa]), nb_ = lds_ld(&cnt[xb]);
+        const bool adv = active & ((ia < 0) | (na == ra)) & ((ib < 0) | (nb_ == rb));
+        if (adv) {
+          const double pax = sx[3 * xa], pay = sx[3 * xa + 1], paa = sx[3 * xa + 2];
+          const double pbx = sx[3 * xb], pby = sx[3 * xb + 1], pbb = sx[3 * xb + 2];
+          double ax = ia >= 0 ? pax : cur.scx, ay = ia >= 0 ? pay : cur.scy, aa = ia >= 0 ? paa : cur.sa;
+          double bx = ib >= 0 ? pbx : cur.scx, by = ib >= 0 ? pby : cur.scy, bb = ib >= 0 ? pbb : cur.sa;
+          solve_contact_position(cur, ax, ay, aa, bx, by, bb, min_sep);
+          if (ia >= 0) { sx[3 * ia] = ax; sx[3 * ia + 1] = ay; sx[3 * ia + 2] = aa; lds_st(&cnt[ia], ra + 1); }
+          if (ib >= 0) { sx[3 * ib] = bx; sx[3 * ib + 1] = by; sx[3 * ib + 2] = bb; lds_st(&cnt[ib], rb + 1); }
+        }
+        if (adv) {
+          j += TS;
+          active = j < nci;
+          cur = nxt;
+          if (j + TS < nci) nxt = d.pc[cstart + order[j + TS]];
         }
         pass_fence();
       }
