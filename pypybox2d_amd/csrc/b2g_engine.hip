@@ -336,22 +336,54 @@ __global__ void __launch_bounds__(kBlock) k_collide(Dev<T> d, int chunks) {
   // Workgroups are dealt to the 8 XCDs round-robin; all chunks of world w get
   // workgroup ids = w (mod 8), so a world's fixtures and bodies are fetched
   // into one XCD's L2 only.
+  // The chunk's contacts that need a manifold are then regrouped by shape pair (a counting
+  // sort in LDS), so that a wavefront runs one collider instead of all of them in turn.
+  __shared__ int count[16], base[16], total_s;
+  __shared__ uint16_t sorted[kBlock];
   const int b = blockIdx.x, xcd = b & 7, k = b >> 3;
   const int w = (k / chunks) * 8 + xcd, chunk = k % chunks;
-  if (w >= d.nw) return;
-  const int sl = chunk * kBlock + threadIdx.x;
-  if (sl >= d.w_con_used[w]) return;
-  const int s = d.w_con_base[w] + sl;
-  int st = d.c_state[s];
-  if (!(st & CS_LIVE)) return;
-  st &= ~(CS_PROCESSED | CS_WOKE);
-  const int f0 = d.w_fix_off[w], b0 = d.w_body_off[w];
-  int2 pr = d.c_pair[s];
-  const int ba = b0 + d.f_info[f0 + pr.x].x, bb = b0 + d.f_info[f0 + pr.y].x;
-  const bool act_a = (d.b_flags[ba] & BF_AWAKE) && !is_static(d.b_type[ba]);
-  const bool act_b = (d.b_flags[bb] & BF_AWAKE) && !is_static(d.b_type[bb]);
-  if (act_a || act_b) st = narrow_update(d, w, s, st);
-  d.c_state[s] = st;
+  if (w >= d.nw) return;  // the whole workgroup
+  const int tid = threadIdx.x;
+  if (tid < 16) count[tid] = 0;
+  __syncthreads();
+  const int sl = chunk * kBlock + tid;
+  const int f0 = d.w_fix_off[w], b0 = d.w_body_off[w], cbase = d.w_con_base[w];
+  int key = -1, rank = 0;
+  if (sl < d.w_con_used[w]) {
+    const int s = cbase + sl;
+    int st = d.c_state[s];
+    if (st & CS_LIVE) {
+      const int2 pr = d.c_pair[s];
+      const int4 ia = d.f_info[f0 + pr.x], ib = d.f_info[f0 + pr.y];
+      const int ba = b0 + ia.x, bb = b0 + ib.x;
+      const bool act_a = (d.b_flags[ba] & BF_AWAKE) && !is_static(d.b_type[ba]);
+      const bool act_b = (d.b_flags[bb] & BF_AWAKE) && !is_static(d.b_type[bb]);
+      if (act_a || act_b) {
+        key = ((ia.y & 3) << 2) | (ib.y & 3);  // shape types of A and B
+        rank = atomicAdd(&count[key], 1);
+      } else {
+        d.c_state[s] = st & ~(CS_PROCESSED | CS_WOKE);
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int q = 0; q < 16; ++q) {
+      base[q] = run;
+      run += count[q];
+    }
+    total_s = run;
+  }
+  __syncthreads();
+  const int total = total_s;
+  if (key >= 0) sorted[base[key] + rank] = (uint16_t)tid;
+  __syncthreads();
+  if (tid < total) {
+    const int s = cbase + chunk * kBlock + sorted[tid];
+    const int st = d.c_state[s] & ~(CS_PROCESSED | CS_WOKE);
+    d.c_state[s] = narrow_update(d, w, s, st);
+  }
 }
 
 // ---------------------------------------------------------------------------
