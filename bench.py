@@ -94,20 +94,22 @@ def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, tune:
     if warmup > 0:
         eng.step_n(warmup, graph=True)
     eng.sync()
-    log("[%s] timing %d steps%s" % (precision, steps, " (launch shape re-tuned every 200 steps)" if tune else ""))
+    log("[%s] timing %d steps%s" % (precision, steps, " (launch shape tuned at the start, after 50, 50 and 100 steps, then every 200)" if tune else ""))
     sums = np.zeros(5)
     t0 = time.perf_counter()
     done = 0
-    chunk = max(1, min(steps, 200))
     while done < steps:
-        n = min(chunk, steps - done)
-        if tuner is not None and n > len(tuner.candidates):
+        left = steps - done
+        if tuner is not None and left > len(tuner.candidates):
             tuner.tune(1.0 / 60, 8, 3)
             sums += tuner.last_ms.sum(axis=0)
             done += len(tuner.candidates)
-            n -= len(tuner.candidates)
-        sums += eng.profile(n) * n
-        done += n
+            n = min(tuner.interval() - len(tuner.candidates), steps - done)
+        else:
+            n = min(200, left)
+        if n > 0:
+            sums += eng.profile(n) * n
+            done += n
         log("[%s] %d/%d steps, %.2f s%s" % (precision, done, steps, time.perf_counter() - t0,
                                            (", launch shape %s" % tuner.best) if tuner else ""))
     eng.sync()

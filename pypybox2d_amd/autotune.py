@@ -27,16 +27,25 @@ CANDIDATES: Sequence[Dict[str, int]] = (
 
 
 class LaunchTuner(object):
-    """Re-tunes the solver launch shape every `period` steps of `step_n`."""
+    """Re-tunes the solver launch shape during `step_n`: after `early` steps at first (while a
+    scene is still changing fast), then every `period` steps."""
 
-    def __init__(self, engine, candidates: Sequence[Dict[str, int]] = CANDIDATES, period: int = 200):
+    def __init__(self, engine, candidates: Sequence[Dict[str, int]] = CANDIDATES, period: int = 200,
+                 early: Sequence[int] = (50, 50, 100)):
         self.engine = engine
         self.candidates: List[Dict[str, int]] = [dict(c) for c in candidates]
         self.period = max(period, len(self.candidates) + 1)
+        self.early = [max(e, len(self.candidates) + 1) for e in early]
+        self.tunes = 0
         self.best: Optional[Dict[str, int]] = None
         self.last_times: Optional[np.ndarray] = None
         self.last_ms: Optional[np.ndarray] = None  # per candidate: collide, islands, solve, sync, pairs
         self._since = None
+
+    def interval(self) -> int:
+        """Steps from the last tuning (its own steps included) to the next one."""
+        k = self.tunes - 1
+        return self.early[k] if 0 <= k < len(self.early) else self.period
 
     def apply(self, params: Dict[str, int]) -> None:
         for k, v in params.items():
@@ -54,13 +63,14 @@ class LaunchTuner(object):
         self.best = self.candidates[int(np.argmin(times))]
         self.apply(self.best)
         self._since = 0
+        self.tunes += 1
         return self.best
 
     def step_n(self, n: int, dt: float = 1.0 / 60, vel_iters: int = 8, pos_iters: int = 3) -> None:
         """n steps: tuning steps at the start of every period, graph-captured steps otherwise."""
         done = 0
         while done < n:
-            if self._since is None or self._since >= self.period:
+            if self._since is None or self._since >= self.interval():
                 if n - done < len(self.candidates):
                     self.engine.step_n(n - done, dt, vel_iters, pos_iters, graph=True)
                     self._since = (self._since or 0) + n - done
@@ -69,7 +79,7 @@ class LaunchTuner(object):
                 done += len(self.candidates)
                 self._since = len(self.candidates)
                 continue
-            k = min(n - done, self.period - self._since)
+            k = min(n - done, self.interval() - self._since)
             self.engine.step_n(k, dt, vel_iters, pos_iters, graph=True)
             done += k
             self._since += k
