@@ -115,13 +115,38 @@ def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, tune:
     eng.sync()
     elapsed = time.perf_counter() - t0
     state = eng.download()
+    # one more (untimed) step with per-island timing, for the solver's latency report
+    eng.set_param("island_times", 1)
+    eng.profile(1)
+    solver = solver_report(eng.island_times(), eng.get_param("solve_waves_per_cu"))
+    eng.set_param("island_times", 0)
     # the same launch sequence captured in a HIP graph, continuing from here
     g0 = time.perf_counter()
     eng.step_n(GRAPH_CHECK_STEPS, graph=True)
     eng.sync()
     graph_ms = 1e3 * (time.perf_counter() - g0) / GRAPH_CHECK_STEPS
     eng.close()
-    return elapsed, sums / steps, state, (dict(tuner.best) if tuner and tuner.best else None), graph_ms
+    return elapsed, sums / steps, state, (dict(tuner.best) if tuner and tuner.best else None), graph_ms, solver
+
+
+def solver_report(rows: np.ndarray, waves_per_cu: int) -> dict:
+    """The island solver as occupancy plus per-island latency (it is bound by the dependency
+    chains inside islands, not by bytes): one step's islands grouped by the lanes that solved
+    them (a team's size; 1: one lane per island; 0: a team island solved by one lane)."""
+    start, end = rows[:, 0], rows[:, 1]
+    lat = (end - start) / 1e3
+    span = (end.max() - start.min()) / 1e3
+    groups = {}
+    for lanes in sorted(set(rows[:, 4].tolist()), reverse=True):
+        m = rows[:, 4] == lanes
+        groups[str(int(lanes))] = {"islands": int(m.sum()), "contacts": [int(rows[m, 2].min()), int(rows[m, 2].max())],
+                                   "latency_us": {"mean": float(lat[m].mean()), "p50": float(np.median(lat[m])),
+                                                  "max": float(lat[m].max())}}
+    c = int(np.argmax(end))
+    return {"islands": int(rows.shape[0]), "span_us": float(span), "mean_islands_in_flight": float(lat.sum() / span),
+            "resident_wavefronts_per_cu": int(waves_per_cu), "by_lanes": groups,
+            "last_island": {"contacts": int(rows[c, 2]), "bodies": int(rows[c, 3]), "lanes": int(rows[c, 4]),
+                            "start_us": float((start[c] - start.min()) / 1e3), "latency_us": float(lat[c])}}
 
 
 G_DT = 10.0 / 60
@@ -331,7 +356,7 @@ def main():
     log("built in %.1f s" % (time.perf_counter() - t0))
     dyn = int(np.count_nonzero(batch.body_i[:, L.BI_TYPE] == L.DYNAMIC))
 
-    el32, ms32, st32, tuned32, gms32 = run_gpu(batch, "fp32", args.steps, args.warmup, not args.no_tune)
+    el32, ms32, st32, tuned32, gms32, solver32 = run_gpu(batch, "fp32", args.steps, args.warmup, not args.no_tune)
     value = dyn * args.steps / el32
     log("fp32: %.3e body-steps/s (%.3f ms/step)" % (value, 1e3 * el32 / args.steps))
     c = workload_counts(st32)
@@ -349,17 +374,19 @@ def main():
         "roofline": roofline(KERNELS[k], float(ms32[k]), c, False, args.workload),
         "roofline_other": {n: roofline(n, float(ms32[i]), c, False, args.workload) for i, n in enumerate(KERNELS)
                            if n in ("k_collide", "k_pairs", "k_sync")},
+        "solver": solver32,
         "launch_shape": tuned32,
         "graph_ms_per_step_next_%d" % GRAPH_CHECK_STEPS: gms32,
         "timing": "wall clock of the timed steps; per-kernel averages from HIP events around every timed launch",
     }
     if not args.no_fp64:
         n64 = args.fp64_steps or args.steps
-        el64, ms64, st64, tuned64, gms64 = run_gpu(batch, "fp64", n64, args.warmup, not args.no_tune)
+        el64, ms64, st64, tuned64, gms64, solver64 = run_gpu(batch, "fp64", n64, args.warmup, not args.no_tune)
         v64 = dyn * n64 / el64
         line["fp64"] = {"value": v64, "ms_per_step": 1e3 * el64 / n64, "steps": n64,
                         "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms64)},
-                        "launch_shape": tuned64, "graph_ms_per_step_next_%d" % GRAPH_CHECK_STEPS: gms64}
+                        "launch_shape": tuned64, "graph_ms_per_step_next_%d" % GRAPH_CHECK_STEPS: gms64,
+                        "solver": solver64}
         line["fp32_over_fp64"] = value / v64
         log("fp64: %.3e body-steps/s; fp32/fp64 = %.2f" % (v64, value / v64))
         if n64 == args.steps:

@@ -109,12 +109,19 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
      B2G_PARAM_LARGE_WORLDS     1: every world takes the global-memory island
                                 and pair search (otherwise only worlds whose
                                 search does not fit 160 KB of LDS, or with
-                                >= 65535 contact slots); read at b2g_upload */
+                                >= 65535 contact slots); read at b2g_upload
+     B2G_PARAM_ISLAND_TIMES     1: the solver records each island's start and
+                                end time (b2g_island_times); 0 (default): off
+     B2G_PARAM_SOLVE_WAVES_PER_CU  read only: solver wavefronts resident per
+                                compute unit under the current launch shape
+                                (the lesser of the VGPR and LDS limits) */
 #define B2G_PARAM_TEAM_LDS 1
 #define B2G_PARAM_HEAVY_CLASS 2
 #define B2G_PARAM_SOLVE_GRID 3
 #define B2G_PARAM_TEAM_FULL_CLASS 4
 #define B2G_PARAM_LARGE_WORLDS 5
+#define B2G_PARAM_ISLAND_TIMES 6
+#define B2G_PARAM_SOLVE_WAVES_PER_CU 7
 B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value);
 B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
 
@@ -123,6 +130,15 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
    reset), islands (island search and the size-ordered queue), solve, sync,
    pairs.  The events are read back once per 128 steps. */
 B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, float* ms_out);
+
+/* With B2G_PARAM_ISLAND_TIMES on: the islands the last step solved, in the
+   solver's queue order, as 4 int64 each: start and end (ns, device wall
+   clock), contacts | bodies << 32, and the lanes that solved it (a team's
+   size, 1 for one lane per island, 0 for a team island whose state did not
+   fit the team's LDS share).  *count receives the number of islands; at most
+   cap rows are written.  No counterpart in the reference (per-island latency
+   of the parallel solver). */
+B2G_API int b2g_island_times(b2g_ctx* ctx, int64_t* out, int32_t cap, int32_t* count);
 
 /* Parity seam, narrow phase: evaluates n contacts (fixture records A and B
    already oriented as the contact registry orders them, contact.py:305-326,

@@ -22,10 +22,12 @@ B2G_FP32, B2G_FP64 = 0, 1
 
 EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step", "b2g_step_n", "b2g_sync",
            "b2g_counts", "b2g_download", "b2g_last_error", "b2g_set_param", "b2g_get_param", "b2g_profile_step",
-           "b2g_narrowphase", "b2g_pairs")
+           "b2g_island_times", "b2g_narrowphase", "b2g_pairs")
 
-# solver launch-shape keys (b2g_set_param)
-PARAMS = {"team_lds": 1, "heavy_class": 2, "solve_grid": 3, "team_full_class": 4, "large_worlds": 5}
+# keys of b2g_set_param / b2g_get_param: the solver's launch shape, island timing, and the
+# resident solver wavefronts per compute unit (read only)
+PARAMS = {"team_lds": 1, "heavy_class": 2, "solve_grid": 3, "team_full_class": 4, "large_worlds": 5,
+          "island_times": 6, "solve_waves_per_cu": 7}
 
 _ERRNAMES = {B2G_E_INVALID: "invalid argument", B2G_E_CAPACITY: "capacity exceeded", B2G_E_HIP: "HIP error",
              B2G_E_NAN: "non-finite state", B2G_E_UNSUPPORTED: "unsupported feature",
@@ -71,6 +73,7 @@ def load(path: Optional[str] = None):
     lib.b2g_pairs.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int, vp]
     lib.b2g_set_param.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
     lib.b2g_get_param.argtypes = [vp, ctypes.c_int32, vp]
+    lib.b2g_island_times.argtypes = [vp, vp, ctypes.c_int32, vp]
     _ = i32
     _lib = lib
     return lib
@@ -150,6 +153,20 @@ class Engine:
         ms = np.zeros(5, np.float32)
         self._check(self.lib.b2g_profile_step(self.h, n, dt, vel_iters, pos_iters, ms.ctypes.data), "profile")
         return ms
+
+    def island_times(self) -> np.ndarray:
+        """Islands of the last step (island timing on): rows of start_ns, end_ns, contacts,
+        bodies, lanes."""
+        n = np.zeros(1, np.int32)
+        self._check(self.lib.b2g_island_times(self.h, None, 0, n.ctypes.data), "island_times")
+        raw = np.zeros((max(int(n[0]), 1), 4), np.int64)
+        self._check(self.lib.b2g_island_times(self.h, raw.ctypes.data, int(n[0]), n.ctypes.data), "island_times")
+        raw = raw[:int(n[0])]
+        out = np.zeros((raw.shape[0], 5), np.int64)
+        out[:, 0], out[:, 1] = raw[:, 0], raw[:, 1]
+        out[:, 2], out[:, 3] = raw[:, 2] & 0xFFFFFFFF, raw[:, 2] >> 32
+        out[:, 4] = raw[:, 3]
+        return out
 
 
 def narrowphase(precision: str, fa_d, fa_i, fb_d, fb_i, xfa, xfb):

@@ -197,6 +197,7 @@ struct Dev {
   const int* w_big;    // per world: 1 = large path
   const int* big_list; // the large worlds
   const int* w_order;  // worlds by descending body count: the island search takes the longest first
+  long long* i_time;   // null, or per queued island: start and end ticks, contacts | bodies << 32, lanes
   int* g_bst;          // per body: state bits as in k_islands
   int* g_wake;         // per body: slot of the contact that woke it (awake gating)
   int* g_off;          // per world nb + 1 at w_body_off[w] + w: CSR offsets
@@ -1681,6 +1682,18 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
   team_sync();  // the next islands reuse the LDS
 }
 
+// Island timing (b2g_island_times): wall-clock ticks around one island's solve, its size and
+// the lanes that solved it (a team's size, 1 for the lane path, 0 for a team island solved by
+// one lane because its state did not fit the team's LDS share).
+template <typename T>
+__device__ __forceinline__ void record_island_time(const Dev<T>& d, int q, long long t0, int nci, int nbi, int lanes) {
+  long long* o = d.i_time + 4 * (size_t)q;
+  o[0] = t0;
+  o[1] = (long long)wall_clock64();
+  o[2] = (long long)nci | (long long)nbi << 32;
+  o[3] = lanes;
+}
+
 // One wavefront per block.  The queue is ordered largest class first.  Its
 // entries of classes >= heavy_class are dealt to wavefronts 64 / T islands at
 // a time (T = team_size of the class), each team with its share of the LDS
@@ -1713,19 +1726,24 @@ __global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_e
         const int nc = d.qs_nc[q];
         const int nci = nc & 0x3FFFFFFF, last_allow = (nc >> 30) & 1;
         const int share = d.team_lds / per;
-        if (team_lds_bytes(in.z) <= share) {
+        const long long t0 = d.i_time ? (long long)wall_clock64() : 0;
+        const bool fits = team_lds_bytes(in.z) <= share;
+        if (fits) {
           solve_island_team(d, in.x, in.y, in.z, in.w, nci, last_allow, team_lds + team * share, t, TS);
         } else if (t == 0) {
           solve_island(d, in.x, in.y, in.z, in.w, nci, last_allow);
         }
+        if (d.i_time && t == 0) record_island_time(d, q, t0, nci, in.z, fits ? TS : 0);
       }
       team_sync();
     } else {
       const int q = nh + (g - hw) * kSolveBlock + l;
       if (q < n) {
+        const long long t0 = d.i_time ? (long long)wall_clock64() : 0;
         const int4 in = d.qs_isl[q];
         const int nc = d.qs_nc[q];
         solve_island(d, in.x, in.y, in.z, in.w, nc & 0x3FFFFFFF, (nc >> 30) & 1);
+        if (d.i_time) record_island_time(d, q, t0, nc & 0x3FFFFFFF, in.z, 1);
       }
     }
   }
@@ -2746,6 +2764,8 @@ struct b2g_ctx {
   int graph_vi = -1, graph_pi = -1;
   // timing
   std::vector<hipEvent_t> pev;  // b2g_profile_step
+  int island_times = 0;         // B2G_PARAM_ISLAND_TIMES
+  long long* itime = nullptr;   // 4 values per island slot (nb slots)
 };
 
 static void ctx_error(b2g_ctx* ctx, const std::string& s) {
@@ -2784,6 +2804,8 @@ static void free_device(b2g_ctx* ctx) {
   drop_graph(ctx);
   for (void* p : ctx->allocs) hipFree(p);
   ctx->allocs.clear();
+  if (ctx->itime) hipFree(ctx->itime);
+  ctx->itime = nullptr;
   if (ctx->dev) {
     if (ctx->precision == B2G_FP64) delete (Dev<double>*)ctx->dev;
     else delete (Dev<float>*)ctx->dev;
@@ -3212,6 +3234,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   d.heavy_class = ctx->heavy_class;
   d.team_full_class = ctx->team_full_class;
   d.team_lds = ctx->team_lds;
+  d.i_time = ctx->island_times ? ctx->itime : nullptr;
   const int nw = ctx->nw;
   hipStream_t st = ctx->stream;
   const int nbig = (int)ctx->big.size();
@@ -3252,6 +3275,17 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   }
   if (timed) HIPCHK(hipEventRecord(ev[5], st));
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// The island-timing buffer is allocated when timing is on and a step is about to run (the
+// step entry points call this before they launch or capture anything).
+static int ensure_island_times(b2g_ctx* ctx) {
+  if (!ctx->island_times || ctx->itime || !ctx->dev) return 0;
+  void* q = nullptr;
+  const hipError_t e = hipMalloc(&q, (size_t)std::max(ctx->nb, 1) * 4 * sizeof(long long));
+  if (e != hipSuccess) return set_error(ctx, B2G_E_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  ctx->itime = (long long*)q;
   return 0;
 }
 
@@ -3455,6 +3489,7 @@ B2G_API int b2g_step(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters) {
   if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
   if (vel_iters < 0 || pos_iters < 0) return set_error(ctx, B2G_E_INVALID, "negative iteration count");
   hipSetDevice(ctx->device);
+  TRY(ensure_island_times(ctx));
   bool first = ctx->pending_new_fixture;
   ctx->pending_new_fixture = false;
   return dispatch_step(ctx, dt, vel_iters, pos_iters, first);
@@ -3464,6 +3499,7 @@ B2G_API int b2g_step_n(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_it
   if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
   if (n < 0 || vel_iters < 0 || pos_iters < 0) return set_error(ctx, B2G_E_INVALID, "negative count");
   hipSetDevice(ctx->device);
+  TRY(ensure_island_times(ctx));  // before any capture
   int i = 0;
   if (ctx->pending_new_fixture && n > 0) {
     ctx->pending_new_fixture = false;
@@ -3559,6 +3595,9 @@ B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value) {
     case B2G_PARAM_LARGE_WORLDS: slot = &ctx->large_worlds; lo = 0; hi = 1; break;
     case B2G_PARAM_TEAM_LDS: slot = &ctx->team_lds; lo = 0; hi = kMaxLds; break;
     case B2G_PARAM_SOLVE_GRID: slot = &ctx->solve_grid; lo = 1; hi = 1 << 30; break;
+    case B2G_PARAM_ISLAND_TIMES: slot = &ctx->island_times; lo = 0; hi = 1; break;
+    case B2G_PARAM_SOLVE_WAVES_PER_CU:
+      return set_error(ctx, B2G_E_INVALID, "b2g_set_param: B2G_PARAM_SOLVE_WAVES_PER_CU is read-only");
     default: return set_error(ctx, B2G_E_INVALID, "b2g_set_param: unknown key " + std::to_string(key));
   }
   if (key == B2G_PARAM_TEAM_LDS && value % 256 != 0)
@@ -3585,6 +3624,20 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value) {
     case B2G_PARAM_LARGE_WORLDS: *value = ctx->large_worlds; return B2G_OK;
     case B2G_PARAM_TEAM_LDS: *value = ctx->team_lds; return B2G_OK;
     case B2G_PARAM_SOLVE_GRID: *value = ctx->solve_grid; return B2G_OK;
+    case B2G_PARAM_ISLAND_TIMES: *value = ctx->island_times; return B2G_OK;
+    case B2G_PARAM_SOLVE_WAVES_PER_CU: {
+      // resident solver wavefronts per compute unit: 4 SIMDs x (512 VGPRs / VGPRs per lane), and
+      // the LDS the launch reserves per one-wavefront workgroup
+      hipSetDevice(ctx->device);
+      hipFuncAttributes fa;
+      const void* fn = ctx->precision == B2G_FP64 ? (const void*)k_solve<double> : (const void*)k_solve<float>;
+      HIPCHK(hipFuncGetAttributes(&fa, fn));
+      const int vgpr = fa.numRegs > 0 ? ((fa.numRegs + 7) / 8) * 8 : 512;
+      const int by_vgpr = 4 * std::min(8, 512 / vgpr);
+      const int by_lds = ctx->team_lds > 0 ? kMaxLds / ctx->team_lds : 32;
+      *value = std::min(by_vgpr, by_lds);
+      return B2G_OK;
+    }
     default: return set_error(ctx, B2G_E_INVALID, "b2g_get_param: unknown key " + std::to_string(key));
   }
 }
@@ -3592,6 +3645,7 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value) {
 B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, float* ms_out) {
   if (!ctx || !ctx->dev || !ms_out || n <= 0) return set_error(ctx, B2G_E_INVALID, "bad arguments");
   hipSetDevice(ctx->device);
+  TRY(ensure_island_times(ctx));
   if (ctx->pending_new_fixture) {
     ctx->pending_new_fixture = false;
     TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, true));
@@ -3614,6 +3668,32 @@ B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int 
     done += m;
   }
   for (int k = 0; k < 5; ++k) ms_out[k] = (float)(acc[k] / n);
+  return B2G_OK;
+}
+
+B2G_API int b2g_island_times(b2g_ctx* ctx, int64_t* out, int32_t cap, int32_t* count) {
+  if (!ctx || !ctx->dev || !count || cap < 0 || (cap > 0 && !out)) return set_error(ctx, B2G_E_INVALID, "bad arguments");
+  if (!ctx->island_times || !ctx->itime)
+    return set_error(ctx, B2G_E_INVALID, "island timing is off (set B2G_PARAM_ISLAND_TIMES, then step)");
+  hipSetDevice(ctx->device);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const int* qc = ctx->precision == B2G_FP64 ? ((Dev<double>*)ctx->dev)->q_count : ((Dev<float>*)ctx->dev)->q_count;
+  int n = 0;
+  HIPCHK(hipMemcpy(&n, qc, sizeof(int), hipMemcpyDeviceToHost));
+  *count = n;
+  const int m = std::min(n, (int)cap);
+  if (m <= 0) return B2G_OK;
+  int rate_khz = 0;
+  HIPCHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+  if (rate_khz <= 0) return set_error(ctx, B2G_E_HIP, "no wall clock rate");
+  std::vector<long long> raw((size_t)4 * m);
+  HIPCHK(hipMemcpy(raw.data(), ctx->itime, raw.size() * sizeof(long long), hipMemcpyDeviceToHost));
+  for (int i = 0; i < m; ++i) {
+    out[4 * i] = (int64_t)((double)raw[4 * i] * 1.0e6 / rate_khz);
+    out[4 * i + 1] = (int64_t)((double)raw[4 * i + 1] * 1.0e6 / rate_khz);
+    out[4 * i + 2] = raw[4 * i + 2];
+    out[4 * i + 3] = raw[4 * i + 3];
+  }
   return B2G_OK;
 }
 

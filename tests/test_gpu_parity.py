@@ -388,3 +388,29 @@ def test_single_large_world_200_pyramids():
     print("200 pyramids fp32 one step", be, flush=True)
     for q in ("position", "angle", "velocity", "angular_velocity"):
         assert be[q] <= 1e-5, (q, be[q])
+
+
+def test_island_times_report_every_island_and_change_nothing():
+    counts = [420, 300, 512, 180, 96, 260]
+    batch = builder.scenes_batch([scenes.batch_world(9, w, n) for w, n in enumerate(counts)])
+    out = {}
+    for timed in (0, 1):
+        e = N.Engine("fp32")
+        e.set_param("island_times", timed)
+        e.upload(batch)
+        e.step_n(60, DT, 8, 3, graph=True)
+        out[timed] = e.download()
+        if timed:
+            rows = e.island_times()
+            waves = e.get_param("solve_waves_per_cu")
+            with pytest.raises(N.B2GError):
+                e.set_param("solve_waves_per_cu", 4)
+        e.close()
+    assert np.array_equal(out[0].body_d, out[1].body_d)
+    assert np.array_equal(out[0].con_d, out[1].con_d)
+    print("islands", rows.shape[0], "waves per CU", waves, "lanes", sorted(set(rows[:, 4].tolist())), flush=True)
+    assert rows.shape[0] > 0 and 1 <= waves <= 32
+    assert np.all(rows[:, 1] >= rows[:, 0])
+    assert set(rows[:, 4].tolist()) <= {0, 1, 8, 16, 32, 64}
+    dyn = int(np.count_nonzero(batch.body_i[:, L.BI_TYPE] == L.DYNAMIC))
+    assert rows[:, 3].sum() <= dyn and np.all(rows[:, 3] >= 1)
