@@ -1869,6 +1869,46 @@ __device__ __forceinline__ void create_contact(const Dev<T>& d, int f0, int s, i
   d.c_id[s] = make_uint2(0, 0);
 }
 
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// A pair of overlapping fat AABBs (moved proxy P at move position i, proxy R): the rest of
+// find_new_contacts' filter (contact_manager.py:102-158); a new contact is listed as i << 16 | R.
+template <typename T>
+__device__ void pair_candidate(const Dev<T>& d, int w, int f0, int b0, int P, int R, int i, const uint32_t* hash,
+                               int hcap, uint32_t* nlist, int ncap, int* misc) {
+  const int4 ip = d.f_info[f0 + P], ir = d.f_info[f0 + R];
+  if (ip.x == ir.x) return;  // same body
+  const int tb_p = d.b_type[b0 + ip.x], tb_r = d.b_type[b0 + ir.x];
+  if (!(d.b_flags[b0 + ip.x] & BF_ACTIVE) || !(d.b_flags[b0 + ir.x] & BF_ACTIVE)) return;
+  {  // exists?
+    const uint32_t k = hkey(P, R);
+    uint32_t h = hhash(k) & (uint32_t)(hcap - 1);
+    for (;;) {
+      const uint32_t v = hash[h];
+      if (v == 0xFFFFFFFFu) break;
+      if (v == k) return;
+      h = (h + 1) & (uint32_t)(hcap - 1);
+    }
+  }
+  if (tb_r != BT_DYNAMIC && tb_p != BT_DYNAMIC) return;  // Body._should_collide
+  bool ok;  // World.default_contact_filter(fixture P, fixture R)
+  if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
+  else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
+  if (!ok) return;
+  const int ta = ip.y & 255, tb = ir.y & 255;
+  const bool reg = (ta == SHAPE_CIRCLE || ta == SHAPE_POLYGON || ta == SHAPE_EDGE) &&
+                   (tb == SHAPE_CIRCLE || tb == SHAPE_POLYGON || tb == SHAPE_EDGE) &&
+                   !(ta == SHAPE_EDGE && tb == SHAPE_EDGE);
+  if (!reg) return;
+  const int slot = atomicAdd(&misc[0], 1);
+  if (slot < ncap) nlist[slot] = ((uint32_t)i << 16) | (uint32_t)R;
+  else atomicOr(&misc[1], 1);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int from_upload, int finish, int do_pairs) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1932,45 +1972,32 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int from_upload, int
       }
     }
     __syncthreads();
-    // every moved proxy against every proxy of the world; exact FP64 tests
-    const long long work = (long long)nmove * nf;
-    for (long long t = tid; t < work; t += kBlock) {
-      const int i = (int)(t / nf), R = (int)(t % nf);
-      const int P = mlist[i];
-      if (R == P) continue;
-      const int mr = mpos[R];
-      if (mr >= 0 && mr < i) continue;  // (R, P) is already in the pair set
-      double4 fp = d.f_fat[f0 + P], fr = d.f_fat[f0 + R];
-      if (!aabb_overlap(fr.x, fr.y, fr.z, fr.w, fp.x, fp.y, fp.z, fp.w)) continue;
-      int4 ip = d.f_info[f0 + P], ir = d.f_info[f0 + R];
-      if (ip.x == ir.x) continue;  // same body
-      const int tb_p = d.b_type[b0 + ip.x], tb_r = d.b_type[b0 + ir.x];
-      if (!(d.b_flags[b0 + ip.x] & BF_ACTIVE) || !(d.b_flags[b0 + ir.x] & BF_ACTIVE)) continue;
-      {  // exists?
-        uint32_t k = hkey(P, R);
-        uint32_t h = hhash(k) & (uint32_t)(hcap - 1);
-        bool found = false;
-        for (;;) {
-          uint32_t v = hash[h];
-          if (v == 0xFFFFFFFFu) break;
-          if (v == k) { found = true; break; }
-          h = (h + 1) & (uint32_t)(hcap - 1);
-        }
-        if (found) continue;
+    // every moved proxy against every proxy of the world; exact FP64 tests.  A lane keeps
+    // proxy R's fat AABB in registers; the wavefront loads 64 moved proxies at a time, one
+    // per lane, and every lane walks them with (uniform) lane reads.
+    const int lane = tid & 63;
+    for (int base = 0; base < nmove; base += 64) {
+      const int nm = nmove - base < 64 ? nmove - base : 64;
+      int Pl = 0;
+      double4 fl = make_double4(0, 0, 0, 0);
+      if (lane < nm) {
+        Pl = mlist[base + lane];
+        fl = d.f_fat[f0 + Pl];
       }
-      if (tb_r != BT_DYNAMIC && tb_p != BT_DYNAMIC) continue;  // Body._should_collide
-      bool ok;  // World.default_contact_filter(fixture P, fixture R)
-      if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
-      else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
-      if (!ok) continue;
-      const int ta = ip.y & 255, tb = ir.y & 255;
-      const bool reg = (ta == SHAPE_CIRCLE || ta == SHAPE_POLYGON || ta == SHAPE_EDGE) &&
-                       (tb == SHAPE_CIRCLE || tb == SHAPE_POLYGON || tb == SHAPE_EDGE) &&
-                       !(ta == SHAPE_EDGE && tb == SHAPE_EDGE);
-      if (!reg) continue;
-      int slot = atomicAdd(&misc[0], 1);
-      if (slot < ncap) nlist[slot] = ((uint32_t)i << 16) | (uint32_t)R;
-      else atomicOr(&misc[1], 1);
+      for (int R = tid; R < nf; R += kBlock) {
+        const double4 fr = d.f_fat[f0 + R];
+        const int mr = mpos[R];
+        for (int jj = 0; jj < nm; ++jj) {
+          const int i = base + jj;
+          const int P = __builtin_amdgcn_readlane(Pl, jj);
+          if (R == P) continue;
+          if (mr >= 0 && mr < i) continue;  // (R, P) is already in the pair set
+          const double4 fp = make_double4(readlane_f64(fl.x, jj), readlane_f64(fl.y, jj), readlane_f64(fl.z, jj),
+                                          readlane_f64(fl.w, jj));
+          if (!aabb_overlap(fr.x, fr.y, fr.z, fr.w, fp.x, fp.y, fp.z, fp.w)) continue;
+          pair_candidate(d, w, f0, b0, P, R, i, hash, hcap, nlist, ncap, misc);
+        }
+      }
     }
     __syncthreads();
   }
