@@ -393,8 +393,7 @@ __global__ void __launch_bounds__(kBlock) k_collide(Dev<T> d, int chunks) {
 struct IslandSmem {
   int* off;        // nb + 1
   int* cur;        // nb
-  uint16_t* adj_s;  // 2 * ccap
-  uint16_t* adj_o;  // 2 * ccap
+  uint32_t* adj;    // adjacency entries: contact slot | other body << 16
   uint8_t* bst;    // nb: bit0 static, bit1 awake, bit2 allow_sleep, bit3 active, bit4 flag, bit5 woken
   uint16_t* bidx;  // nb
   uint16_t* stack;  // nb
@@ -482,8 +481,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     unsigned char* p = smem;
     L.off = (int*)p; p += ((size_t)(nb + 1) * 4 + 15) & ~(size_t)15;
     L.cur = (int*)p; p += ((size_t)nb * 4 + 15) & ~(size_t)15;
-    L.adj_s = (uint16_t*)p; p += ((size_t)nadj * 2 + 15) & ~(size_t)15;
-    L.adj_o = (uint16_t*)p; p += ((size_t)nadj * 2 + 15) & ~(size_t)15;
+    L.adj = (uint32_t*)p; p += ((size_t)nadj * 4 + 15) & ~(size_t)15;
     L.bst = (uint8_t*)p; p += ((size_t)nb + 15) & ~(size_t)15;
     L.bidx = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
     L.stack = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
@@ -510,10 +508,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
   // compacted in slot order, with their bodies, into LDS (the adjacency arrays
   // are free until the graph is built); one lane then walks them in order.
   if (d.w_asleep[w] > 0) {
-    uint16_t* cs = L.adj_s;
-    uint16_t* cla = L.adj_s + kGateChunk;
-    uint16_t* clb = L.adj_o;
-    uint16_t* cfl = L.adj_o + kGateChunk;  // bit 0 evaluated this step, bit 1 touching changed
+    uint16_t* cs = (uint16_t*)L.adj;  // the adjacency area holds 4 * kGateChunk halves at least
+    uint16_t* cla = cs + kGateChunk;
+    uint16_t* clb = cla + kGateChunk;
+    uint16_t* cfl = clb + kGateChunk;  // bit 0 evaluated this step, bit 1 touching changed
     int* wcount = L.scan;
     const int lane = tid & 63, wv = tid >> 6;
     for (int chunk = 0; chunk < used; chunk += kGateChunk) {
@@ -596,11 +594,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
       int2 pr = d.c_pair[cb + sl];
       int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
       int pa = atomicAdd(&L.cur[la], 1);
-      L.adj_s[pa] = (uint16_t)sl;
-      L.adj_o[pa] = (uint16_t)lb;
+      L.adj[pa] = (uint32_t)sl | (uint32_t)lb << 16;
       int pb = atomicAdd(&L.cur[lb], 1);
-      L.adj_s[pb] = (uint16_t)sl;
-      L.adj_o[pb] = (uint16_t)la;
+      L.adj[pb] = (uint32_t)sl | (uint32_t)la << 16;
     }
   }
   __syncthreads();
@@ -609,15 +605,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     L.lev[i] = 0;
     int beg = L.off[i], end = L.off[i + 1];
     for (int k = beg + 1; k < end; ++k) {
-      uint16_t s = L.adj_s[k], o = L.adj_o[k];
+      const uint32_t e = L.adj[k];
       int j = k - 1;
-      while (j >= beg && L.adj_s[j] > s) {
-        L.adj_s[j + 1] = L.adj_s[j];
-        L.adj_o[j + 1] = L.adj_o[j];
+      while (j >= beg && (L.adj[j] & 0xFFFFu) > (e & 0xFFFFu)) {
+        L.adj[j + 1] = L.adj[j];
         --j;
       }
-      L.adj_s[j + 1] = s;
-      L.adj_o[j + 1] = o;
+      L.adj[j + 1] = e;
     }
   }
   __syncthreads();
@@ -651,18 +645,27 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
         L.bidx[b] = (uint16_t)nbi_tot;
         isl_b[nbi_tot++] = b;
         int cur_b = L.cur[b], lev_b = L.lev[b];  // the expanding body's counters stay in registers
-        for (int k = L.off[b]; k < L.off[b + 1]; ++k) {
-          const int s = L.adj_s[k];
+        // The next entry is fetched one step ahead, and an entry's contact flag and its other
+        // body's state are read together (also for entries that turn out to be taken), so an
+        // entry costs about one LDS round trip.
+        const int kend = L.off[b + 1];
+        uint32_t e = L.off[b] < kend ? L.adj[L.off[b]] : 0u;
+        for (int k = L.off[b]; k < kend; ++k) {
+          const uint32_t en = k + 1 < kend ? L.adj[k + 1] : 0u;
+          const int s = (int)(e & 0xFFFFu), o = (int)(e >> 16);
+          e = en;
           const uint32_t bit = 1u << (s & 31);
-          if (L.cflag[s >> 5] & bit) continue;
-          L.cflag[s >> 5] |= bit;
-          const int o = L.adj_o[k];
+          const uint32_t fw = L.cflag[s >> 5];
           const uint8_t so = L.bst[o];
+          const int co = L.cur[o], lo_read = L.lev[o];
+          if (fw & bit) continue;
+          L.cflag[s >> 5] = fw | bit;
           const bool o_dyn = !(so & 1);
           const uint32_t rank_b = (uint32_t)cur_b++;
-          const uint32_t rank_o = o_dyn ? (uint32_t)L.cur[o]++ : 0u;
+          const uint32_t rank_o = o_dyn ? (uint32_t)co : 0u;
+          if (o_dyn) L.cur[o] = co + 1;
           // level: 1 + the latest level among the contact's dynamic bodies (one sweep)
-          const int lo = o_dyn ? L.lev[o] : 0;
+          const int lo = o_dyn ? lo_read : 0;
           const int lvl = 1 + (lev_b > lo ? lev_b : lo);
           lev_b = lvl;
           if (o_dyn) L.lev[o] = (uint16_t)lvl;
@@ -714,7 +717,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
   }
   // each island's contacts by level: a counting sort on key = island start + level - 1
   // (keys of one island stay inside its contact range); the adjacency arrays are free now
-  int* hist = (int*)L.adj_s;
+  int* hist = (int*)L.adj;
   for (int k = tid; k < nci_tot; k += kBlock) hist[k] = 0;
   __syncthreads();
   for (int k = tid; k < nci_tot; k += kBlock) atomicAdd(&hist[d.i_rank[cb + k].y & 0xFFFFu], 1);
