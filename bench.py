@@ -77,7 +77,8 @@ def algorithmic_bytes(kernel: str, c: dict, fp64: bool) -> float:
     return 0.0
 
 
-def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, tune: bool = True):
+def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, tune: bool = True,
+            solver_times: bool = True, graph_check: bool = True):
     """Times `steps` steps and the average duration of each kernel over them.
 
     Every timed step is launched with HIP events between its kernels on the
@@ -115,16 +116,20 @@ def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, tune:
     eng.sync()
     elapsed = time.perf_counter() - t0
     state = eng.download()
-    # one more (untimed) step with per-island timing, for the solver's latency report
-    eng.set_param("island_times", 1)
-    eng.profile(1)
-    solver = solver_report(eng.island_times(), eng.get_param("solve_waves_per_cu"))
-    eng.set_param("island_times", 0)
     # the same launch sequence captured in a HIP graph, continuing from here
-    g0 = time.perf_counter()
-    eng.step_n(GRAPH_CHECK_STEPS, graph=True)
-    eng.sync()
-    graph_ms = 1e3 * (time.perf_counter() - g0) / GRAPH_CHECK_STEPS
+    graph_ms = None
+    if graph_check:
+        g0 = time.perf_counter()
+        eng.step_n(GRAPH_CHECK_STEPS, graph=True)
+        eng.sync()
+        graph_ms = 1e3 * (time.perf_counter() - g0) / GRAPH_CHECK_STEPS
+    # one more (untimed) step with per-island timing, for the solver's latency report (last,
+    # since switching the timing on drops the captured graph)
+    solver = None
+    if solver_times:
+        eng.set_param("island_times", 1)
+        eng.profile(1)
+        solver = solver_report(eng.island_times(), eng.get_param("solve_waves_per_cu"))
     eng.close()
     return elapsed, sums / steps, state, (dict(tuner.best) if tuner and tuner.best else None), graph_ms, solver
 
@@ -321,6 +326,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default solver launch shape")
     ap.add_argument("--error-worlds", type=int, default=64)
+    ap.add_argument("--no-solver-report", action="store_true",
+                    help="skip the extra step with per-island solver timing (used for profiler runs)")
+    ap.add_argument("--no-graph-check", action="store_true",
+                    help="skip the graph-replayed steps after the timed region (used for profiler runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
@@ -356,7 +365,8 @@ def main():
     log("built in %.1f s" % (time.perf_counter() - t0))
     dyn = int(np.count_nonzero(batch.body_i[:, L.BI_TYPE] == L.DYNAMIC))
 
-    el32, ms32, st32, tuned32, gms32, solver32 = run_gpu(batch, "fp32", args.steps, args.warmup, not args.no_tune)
+    el32, ms32, st32, tuned32, gms32, solver32 = run_gpu(batch, "fp32", args.steps, args.warmup, not args.no_tune,
+                                                         not args.no_solver_report, not args.no_graph_check)
     value = dyn * args.steps / el32
     log("fp32: %.3e body-steps/s (%.3f ms/step)" % (value, 1e3 * el32 / args.steps))
     c = workload_counts(st32)
@@ -381,7 +391,8 @@ def main():
     }
     if not args.no_fp64:
         n64 = args.fp64_steps or args.steps
-        el64, ms64, st64, tuned64, gms64, solver64 = run_gpu(batch, "fp64", n64, args.warmup, not args.no_tune)
+        el64, ms64, st64, tuned64, gms64, solver64 = run_gpu(batch, "fp64", n64, args.warmup, not args.no_tune,
+                                                             not args.no_solver_report, not args.no_graph_check)
         v64 = dyn * n64 / el64
         line["fp64"] = {"value": v64, "ms_per_step": 1e3 * el64 / n64, "steps": n64,
                         "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms64)},
