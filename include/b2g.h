@@ -100,11 +100,14 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
      B2G_PARAM_TEAM_FULL_CLASS  heavy islands of this class and above get a
                                 whole wavefront (64 lanes), one class below 32
                                 lanes, then 16, and 8 for the rest; 0..16
+     B2G_PARAM_TEAM_MAX         the largest team, 8, 16, 32 or 64 lanes: a
+                                wavefront carries 64 / team islands (default 32)
      B2G_PARAM_TEAM_LDS         LDS bytes per wavefront, shared by its teams,
                                 a multiple of 256 up to 163840; an island whose
                                 state needs more than its team's share (24 B
                                 per body and 4 per body rounded to even
-                                bodies) is solved by one lane instead
+                                bodies) gets a whole wavefront and its budget,
+                                and if it needs more than that, one lane
      B2G_PARAM_SOLVE_GRID       most wavefronts launched for the solver, >= 1
      B2G_PARAM_LARGE_WORLDS     1: every world takes the global-memory island
                                 and pair search (otherwise only worlds whose
@@ -122,6 +125,7 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
 #define B2G_PARAM_LARGE_WORLDS 5
 #define B2G_PARAM_ISLAND_TIMES 6
 #define B2G_PARAM_SOLVE_WAVES_PER_CU 7
+#define B2G_PARAM_TEAM_MAX 8
 B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value);
 B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
 

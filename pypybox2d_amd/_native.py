@@ -27,7 +27,7 @@ EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step",
 # keys of b2g_set_param / b2g_get_param: the solver's launch shape, island timing, and the
 # resident solver wavefronts per compute unit (read only)
 PARAMS = {"team_lds": 1, "heavy_class": 2, "solve_grid": 3, "team_full_class": 4, "large_worlds": 5,
-          "island_times": 6, "solve_waves_per_cu": 7}
+          "island_times": 6, "solve_waves_per_cu": 7, "team_max": 8}
 
 _ERRNAMES = {B2G_E_INVALID: "invalid argument", B2G_E_CAPACITY: "capacity exceeded", B2G_E_HIP: "HIP error",
              B2G_E_NAN: "non-finite state", B2G_E_UNSUPPORTED: "unsupported feature",
@@ -141,7 +141,8 @@ class Engine:
         return out.to_batch()
 
     def set_param(self, name: str, value: int) -> None:
-        """Solver launch shape: team_lds, heavy_class, team_full_class or solve_grid (results do not depend on them)."""
+        """Solver launch shape: team_lds, heavy_class, team_full_class, team_max or solve_grid (results do not
+        depend on them); island_times; large_worlds."""
         self._check(self.lib.b2g_set_param(self.h, PARAMS[name], int(value)), "set_param")
 
     def get_param(self, name: str) -> int:

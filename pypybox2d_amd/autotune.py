@@ -17,12 +17,12 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 CANDIDATES: Sequence[Dict[str, int]] = (
-    {"heavy_class": 5, "team_full_class": 9, "team_lds": 20480},
-    {"heavy_class": 5, "team_full_class": 9, "team_lds": 24576},
-    {"heavy_class": 5, "team_full_class": 9, "team_lds": 32768},
-    {"heavy_class": 5, "team_full_class": 9, "team_lds": 40960},
-    {"heavy_class": 5, "team_full_class": 8, "team_lds": 20480},
-    {"heavy_class": 5, "team_full_class": 10, "team_lds": 49152},
+    {"heavy_class": 5, "team_full_class": 9, "team_max": 32, "team_lds": 24576},
+    {"heavy_class": 5, "team_full_class": 9, "team_max": 32, "team_lds": 20480},
+    {"heavy_class": 5, "team_full_class": 9, "team_max": 32, "team_lds": 32768},
+    {"heavy_class": 5, "team_full_class": 9, "team_max": 64, "team_lds": 20480},
+    {"heavy_class": 5, "team_full_class": 8, "team_max": 64, "team_lds": 20480},
+    {"heavy_class": 5, "team_full_class": 10, "team_max": 32, "team_lds": 24576},
 )
 
 

@@ -74,6 +74,50 @@ __host__ __device__ inline int size_class(int nc) {  // floor(log2(nc + 1)), cap
   return c;
 }
 
+constexpr int kSolveBlock = 64;  // k_solve: one wavefront per workgroup
+
+// Team size of a heavy size class: 64 lanes from full_cls up, halved per class
+// below down to 8, and at most tmax lanes, except in the top class, which holds
+// the islands whose state does not fit a smaller team's LDS share (queue_class).
+__host__ __device__ inline int team_size(int cls, int full_cls, int tmax) {
+  int sh = full_cls - cls;
+  sh = sh < 0 ? 0 : (sh > 3 ? 3 : sh);
+  const int ts = kSolveBlock >> sh;
+  return cls == kSizeClasses - 1 ? kSolveBlock : (ts < tmax ? ts : tmax);
+}
+
+__host__ __device__ inline int team_lds_bytes(int nbi) {
+  return 24 * nbi + 4 * ((nbi + 1) & ~1);  // 3 doubles and an int counter per body
+}
+
+// The queue class of an island of nc contacts and nbi bodies: its size class,
+// except that a heavy island whose body state does not fit the LDS share of
+// its class's team moves to the top class, where a whole wavefront and its
+// LDS budget take it alone.  Scheduling only: results do not depend on it.
+template <typename D>
+__device__ __forceinline__ int queue_class(const D& d, int nc, int nbi) {
+  int c = size_class(nc);
+  if (c >= d.heavy_class && c < kSizeClasses - 1) {
+    const int ts = team_size(c, d.team_full_class, d.team_max);
+    if (ts < kSolveBlock && team_lds_bytes(nbi) > d.team_lds / (kSolveBlock / ts)) c = kSizeClasses - 1;
+  }
+  return c;
+}
+
+// The sweep period of an island whose contacts have one-sweep dependency levels 1..depth and
+// whose dynamic bodies see their first and last contact of a sweep at most span levels apart.
+// A team lane takes its solves (sweep s, contact of level l) in order of s * period + l: with
+// period > span every dependency (inside a sweep, and from a body's last contact of sweep s - 1
+// to its first of sweep s) goes to a larger key, so the lanes' orders are consistent and the
+// dataflow cannot deadlock; with 2 * period >= depth no more than two sweeps interleave in a
+// lane.  Sweeps thereby overlap in the pipeline: the passes approach the dependency depth of
+// all sweeps together instead of a sweep-by-sweep walk.
+__host__ __device__ inline int sweep_period(int span, int depth) {
+  const int p = span + 1, h = (depth + 1) / 2;
+  const int r = p > h ? p : h;
+  return r < 0xFFFF ? r : 0xFFFF;
+}
+
 template <typename T>
 struct V4 {
   T x, y, z, w;
@@ -120,6 +164,7 @@ struct Dev {
   int heavy_class;      // islands of this size class and above are solved by a team of lanes
   int team_full_class;  // ... by a whole wavefront from this class up (half one class below, ...)
   int team_lds;         // LDS bytes per wavefront of teams
+  int team_max;         // largest team below the top class (8, 16, 32 or 64)
   // worlds
   const int* w_body_off;
   const int* w_fix_off;
@@ -179,10 +224,13 @@ struct Dev {
   int2* i_cidx;      // per-world segment at w_con_base[w]
   uint2* i_rank;     // per-world segment at w_con_base[w]: the contact's rank in body A's and body B's
                      // contact sequence of the island (x = rank A | rank B << 16)
-  uint16_t* i_order; // per-world segment at w_con_base[w]: each island's contacts (island-local
-                     // indices) ordered by dependency level within one sweep
-  int4* i_info;      // per-world segment at w_body_off[w]: (bstart, nb, cstart, nc | last_allow << 30)
-  int4* q_isl;       // global island queue: (world, bstart, nb, cstart)
+  uint32_t* i_order; // per-world segment at w_con_base[w]: each island's contacts ordered by
+                     // dependency level within one sweep, as island-local index | level key << 16
+                     // (the key is the level plus a per-island offset)
+  int4* i_info;      // per-world segment at w_body_off[w]:
+                     // (bstart, nb | period << 16, cstart, nc | last_allow << 30)
+  int4* q_isl;       // global island queue: (world, bstart, nb | period << 16, cstart); the period
+                     // orders a solver lane's sweeps (sweep_period)
   int* q_nc;         // (nc | last_allow << 30)
   int* q_count;
   int4* qs_isl;      // the queue ordered by island size, largest first
@@ -631,7 +679,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
       L.stack[sp++] = (uint16_t)seed;
       L.bst[seed] = sb | 16;
       const int bstart = nbi_tot, cstart = nci_tot;
-      int last_allow = 0;
+      int last_allow = 0, span = 0, depth = 0;
       while (sp > 0) {
         const int b = L.stack[--sp];
         uint8_t bs = L.bst[b];
@@ -644,7 +692,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
         }
         L.bidx[b] = (uint16_t)nbi_tot;
         isl_b[nbi_tot++] = b;
-        int cur_b = L.cur[b], lev_b = L.lev[b];  // the expanding body's counters stay in registers
+        // the expanding body's counters stay in registers; cur = rank | first level << 16
+        int cur_b = L.cur[b], lev_b = L.lev[b];
         // The next entry is fetched one step ahead, and an entry's contact flag and its other
         // body's state are read together (also for entries that turn out to be taken), so an
         // entry costs about one LDS round trip.
@@ -661,17 +710,25 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
           if (fw & bit) continue;
           L.cflag[s >> 5] = fw | bit;
           const bool o_dyn = !(so & 1);
-          const uint32_t rank_b = (uint32_t)cur_b++;
-          const uint32_t rank_o = o_dyn ? (uint32_t)co : 0u;
-          if (o_dyn) L.cur[o] = co + 1;
           // level: 1 + the latest level among the contact's dynamic bodies (one sweep)
           const int lo = o_dyn ? lo_read : 0;
           const int lvl = 1 + (lev_b > lo ? lev_b : lo);
+          const uint32_t rank_b = (uint32_t)(cur_b & 0xFFFF);
+          const uint32_t rank_o = o_dyn ? (uint32_t)(co & 0xFFFF) : 0u;
+          if (rank_b == 0) cur_b |= lvl << 16;
+          span = max(span, lvl - (int)((uint32_t)cur_b >> 16));
+          cur_b += 1;
+          if (o_dyn) {
+            const int fo = rank_o == 0 ? lvl : (int)((uint32_t)co >> 16);
+            span = max(span, lvl - fo);
+            L.cur[o] = (int)(rank_o + 1) | fo << 16;
+          }
+          depth = max(depth, lvl);
           lev_b = lvl;
           if (o_dyn) L.lev[o] = (uint16_t)lvl;
           const uint32_t key = (uint32_t)(cstart + lvl - 1);  // < cstart + island contacts
           d.i_rank[cb + nci_tot] = make_uint2((uint32_t)b | key << 16, rank_b | rank_o << 16);
-          d.i_order[cb + nci_tot] = (uint16_t)(nci_tot - cstart);
+          d.i_order[cb + nci_tot] = (uint32_t)(nci_tot - cstart);
           isl_c[nci_tot++] = s;
           if (so & 16) continue;
           L.stack[sp++] = (uint16_t)o;
@@ -681,7 +738,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
         L.lev[b] = (uint16_t)lev_b;
       }
       for (int k = 0; k < ns; ++k) L.bst[L.slist[k]] &= (uint8_t)~16;  // statics may join other islands
-      info[nisl++] = make_int4(bstart, nbi_tot - bstart, cstart, (nci_tot - cstart) | (last_allow << 30));
+      info[nisl++] = make_int4(bstart, (nbi_tot - bstart) | sweep_period(span, depth) << 16, cstart,
+                               (nci_tot - cstart) | (last_allow << 30));
     }
     L.misc[0] = nisl;
     L.misc[1] = nbi_tot;
@@ -697,7 +755,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     int4 in = info[k];
     d.q_isl[qbase + k] = make_int4(w, b0 + in.x, in.y, cb + in.z);
     d.q_nc[qbase + k] = in.w;
-    atomicAdd(&ccount[size_class(in.w & 0x3FFFFFFF)], 1);
+    atomicAdd(&ccount[queue_class(d, in.w & 0x3FFFFFFF, in.y & 0xFFFF)], 1);
   }
   __syncthreads();
   if (tid < kSizeClasses && ccount[tid] > 0) atomicAdd(&d.q_cls[tid], ccount[tid]);
@@ -713,7 +771,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     const uint32_t lo = r.y & 0xFFFFu, hi = r.y >> 16;
     const uint32_t key = r.x >> 16;
     d.i_rank[cb + k] = make_uint2((int)(r.x & 0xFFFFu) == la ? (lo | hi << 16) : (hi | lo << 16),
-                                  key | (uint32_t)d.i_order[cb + k] << 16);
+                                  key | d.i_order[cb + k] << 16);
   }
   // each island's contacts by level: a counting sort on key = island start + level - 1
   // (keys of one island stay inside its contact range); the adjacency arrays are free now
@@ -726,7 +784,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
   for (int k = tid; k < nci_tot; k += kBlock) {
     const uint32_t y = d.i_rank[cb + k].y;
     const int pos = atomicAdd(&hist[y & 0xFFFFu], 1);
-    d.i_order[cb + pos] = (uint16_t)(y >> 16);
+    d.i_order[cb + pos] = (y >> 16) | (y << 16);  // island-local index | level key << 16
   }
   for (int i = tid; i < nb; i += kBlock) {
     uint8_t bs = L.bst[i];
@@ -915,25 +973,19 @@ __device__ __forceinline__ void solve_contact_velocity(VC<T>& vc, V2d& va, doubl
     M22<double> N;
     N.c1x = vc.n11; N.c1y = vc.n21; N.c2x = vc.n12; N.c2y = vc.n22;
     b = b - mmul(K, a);
-    V2d x = -mmul(N, b);  // case 1
-    if (x.x >= 0.0 && x.y >= 0.0) {
-      apply_block(vc, x, a, normal, va, wa, vb, wb);
-    } else {
-      x = v2<double>(-(double)cp1.nm * b.x, 0.0);  // case 2
-      vn2 = K.c1y * x.x + b.y;
-      if (x.x >= 0.0 && vn2 >= 0.0) {
-        apply_block(vc, x, a, normal, va, wa, vb, wb);
-      } else {
-        x = v2<double>(0.0, -(double)cp2.nm * b.y);  // case 3
-        vn1 = K.c2x * x.y + b.x;
-        if (x.y >= 0.0 && vn1 >= 0.0) {
-          apply_block(vc, x, a, normal, va, wa, vb, wb);
-        } else {
-          x = v2<double>(0.0, 0.0);  // case 4
-          if (b.x >= 0.0 && b.y >= 0.0) apply_block(vc, x, a, normal, va, wa, vb, wb);
-        }
-      }
-    }
+    // The four cases are evaluated side by side and the first valid one is applied once:
+    // the same values as trying them in turn, without a divergent path per case.
+    const V2d x1 = -mmul(N, b);                                    // case 1
+    const V2d x2 = v2<double>(-(double)cp1.nm * b.x, 0.0);         // case 2
+    const double vn2_2 = K.c1y * x2.x + b.y;
+    const V2d x3 = v2<double>(0.0, -(double)cp2.nm * b.y);         // case 3
+    const double vn1_3 = K.c2x * x3.y + b.x;
+    const bool c1 = x1.x >= 0.0 && x1.y >= 0.0;
+    const bool c2 = x2.x >= 0.0 && vn2_2 >= 0.0;
+    const bool c3 = x3.y >= 0.0 && vn1_3 >= 0.0;
+    const bool c4 = b.x >= 0.0 && b.y >= 0.0;                       // case 4: x = 0
+    const V2d x = c1 ? x1 : (c2 ? x2 : (c3 ? x3 : v2<double>(0.0, 0.0)));
+    if (c1 || c2 || c3 || c4) apply_block(vc, x, a, normal, va, wa, vb, wb);
   }
 }
 
@@ -1454,17 +1506,6 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
 // and are prefetched one solve ahead.  The arithmetic per contact is the same
 // code as the one-lane path, so results are identical.
 // ---------------------------------------------------------------------------
-constexpr int kSolveBlock = 64;
-
-__host__ __device__ inline int team_size(int cls, int full_cls) {
-  int sh = full_cls - cls;
-  sh = sh < 0 ? 0 : (sh > 3 ? 3 : sh);
-  return kSolveBlock >> sh;
-}
-
-__host__ __device__ inline int team_lds_bytes(int nbi) {
-  return 24 * nbi + 4 * ((nbi + 1) & ~1);  // 3 doubles and an int counter per body
-}
 
 // The lanes of one wavefront exchange body state through LDS between passes.
 // LDS instructions of a wavefront execute in issue order, so wavefront-scope
@@ -1495,7 +1536,7 @@ __device__ __forceinline__ X team_min(X v, int T) {
 
 template <typename T>
 __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci, int last_allow,
-                                  char* lds, int t, int TS) {
+                                  int period, char* lds, int t, int TS) {
   const IslandK c = island_k(d, w, bstart);
   double* sx = (double*)lds;        // [3 * nbi]
   int* cnt = (int*)(sx + 3 * nbi);  // [nbi]
@@ -1535,22 +1576,48 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
   const int s0 = c.warm ? 0 : 1;
   const int nsw = d.vel_iters + 1 - s0;
   const int mine = t < nci ? (nci - 1 - t) / TS + 1 : 0;  // contacts owned by this lane
-  // Lane t walks entries t, t + TS, ... of the island's level order (i_order): its contacts
-  // come in non-decreasing dependency level, so each is ready soon after the previous one.
-  const uint16_t* order = d.i_order + cstart;
+  // Lane t owns entries t, t + TS, ... of the island's level order (i_order) and takes its
+  // solves (sweep s, owned entry j) in order of s * period + level (sweep_period): a merge of
+  // two cursors, A on sweep sa and B on sweep sa + 1, whose entries are read one step ahead.
+  const uint32_t* order = d.i_order + cstart;
   {
-    bool active = nsw > 0 && mine > 0;
-    const int cap = 2 * nsw * nci + 64;
-    int j = t, s = 0, k = 0;
+    const uint32_t e0 = mine > 0 ? order[t] : 0u;
+    bool a_on = nsw > 0 && mine > 0, b_on = nsw > 1 && mine > 0;
+    int sa = 0, ja = 0, jb = 0;
+    uint32_t ea = e0, eb = e0;
+    // the lane's next solve: B when its key (sa + 1) * period + level is below A's
+    auto next_solve = [&](int& s_out, int& k_out) -> bool {
+      if (!a_on) return false;
+      const bool take_b = b_on && jb < mine && (int)(eb >> 16) + period < (int)(ea >> 16);
+      if (take_b) {
+        s_out = sa + 1;
+        k_out = (int)(eb & 0xFFFFu);
+        if (++jb < mine) eb = order[t + jb * TS];
+      } else {
+        s_out = sa;
+        k_out = (int)(ea & 0xFFFFu);
+        if (++ja < mine) {
+          ea = order[t + ja * TS];
+        } else if (b_on) {  // sweep sa done (B is never done first): B becomes A, B restarts
+          sa += 1;
+          ja = jb;
+          ea = eb;
+          jb = 0;
+          eb = e0;
+          b_on = sa + 1 < nsw;
+        } else {
+          a_on = false;
+        }
+      }
+      return true;
+    };
+    int s = 0, k = 0, ns = 0, nk = 0;
     VC<T> cur, nxt;
-    int nj = j + TS, ns = s, nk = 0;
-    if (nj >= nci) { nj = t; ns = s + 1; }
-    if (active) {
-      k = order[j];
-      cur = d.vc[cstart + k];
-      nk = order[nj];
-      if (mine > 1 && ns < nsw) nxt = d.vc[cstart + nk];
-    }
+    bool active = next_solve(s, k);
+    if (active) cur = d.vc[cstart + k];
+    bool have_nxt = next_solve(ns, nk);
+    if (have_nxt) nxt = d.vc[cstart + nk];
+    const int cap = 2 * nsw * nci + 64;
     // The pass test sits at the bottom and readiness is computed without branches: the
     // compiler then keeps the prefetched records where they are instead of copying them at
     // every join (inactive lanes and static sides read entry 0 and ignore it).
@@ -1562,11 +1629,13 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
         const int ra = s * (int)((unsigned)cur.rka >> 16) + (cur.rka & 0xFFFF);
         const int rb = s * (int)((unsigned)cur.rkb >> 16) + (cur.rkb & 0xFFFF);
         const int xa = active && ia >= 0 ? ia : 0, xb = active && ib >= 0 ? ib : 0;
+        // The bodies' velocities are read with their counters: when a counter shows the
+        // contact's turn, the velocity it pairs with was written before the last pass boundary.
         const int na = lds_ld(&cnt[xa]), nb_ = lds_ld(&cnt[xb]);
+        const double pax = sx[3 * xa], pay = sx[3 * xa + 1], paw = sx[3 * xa + 2];
+        const double pbx = sx[3 * xb], pby = sx[3 * xb + 1], pbw = sx[3 * xb + 2];
         const bool ready = active & ((ia < 0) | (na == ra)) & ((ib < 0) | (nb_ == rb));
         if (ready) {
-          const double pax = sx[3 * xa], pay = sx[3 * xa + 1], paw = sx[3 * xa + 2];
-          const double pbx = sx[3 * xb], pby = sx[3 * xb + 1], pbw = sx[3 * xb + 2];
           // static bodies have zero velocity (checked at upload)
           V2d va = v2<double>(ia >= 0 ? pax : 0.0, ia >= 0 ? pay : 0.0);
           V2d vb = v2<double>(ib >= 0 ? pbx : 0.0, ib >= 0 ? pby : 0.0);
@@ -1585,20 +1654,21 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
           }
           if (ia >= 0) { sx[3 * ia] = va.x; sx[3 * ia + 1] = va.y; sx[3 * ia + 2] = wa; lds_st(&cnt[ia], ra + 1); }
           if (ib >= 0) { sx[3 * ib] = vb.x; sx[3 * ib + 1] = vb.y; sx[3 * ib + 2] = wb; lds_st(&cnt[ib], rb + 1); }
-          j = nj;
-          s = ns;
-          k = nk;
-          if (s >= nsw) {
-            active = false;
+          if (have_nxt) {
+            // the next record was read before this solve stored its impulses: when it is the
+            // same contact, the impulses come from the registers
+            const bool same = nk == k;
+            nxt.p[0].ni = same ? cur.p[0].ni : nxt.p[0].ni;
+            nxt.p[0].ti = same ? cur.p[0].ti : nxt.p[0].ti;
+            nxt.p[1].ni = same ? cur.p[1].ni : nxt.p[1].ni;
+            nxt.p[1].ti = same ? cur.p[1].ti : nxt.p[1].ti;
+            cur = nxt;
+            s = ns;
+            k = nk;
+            have_nxt = next_solve(ns, nk);
+            if (have_nxt) nxt = d.vc[cstart + nk];
           } else {
-            nj = j + TS;
-            ns = s;
-            if (nj >= nci) { nj = t; ns = s + 1; }
-            if (mine > 1) {  // with one contact the record in registers is the current one
-              cur = nxt;
-              nk = order[nj];
-              if (ns < nsw) nxt = d.vc[cstart + nk];
-            }
+            active = false;
           }
         }
       }
@@ -1630,8 +1700,8 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
       bool active = mine > 0;
       int j = t;
       PC<T> cur, nxt;
-      if (active) cur = d.pc[cstart + order[j]];
-      if (active && j + TS < nci) nxt = d.pc[cstart + order[j + TS]];
+      if (active) cur = d.pc[cstart + (order[j] & 0xFFFFu)];
+      if (active && j + TS < nci) nxt = d.pc[cstart + (order[j + TS] & 0xFFFFu)];
       int pass = 0;
       const int pcap = 2 * nci + 64;
       for (bool more = wave_any(active); more; more = wave_any(active && pass < pcap)) {
@@ -1642,10 +1712,10 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
         const int ra = cur.rka & 0xFFFF, rb = cur.rkb & 0xFFFF;
         const int xa = active && ia >= 0 ? ia : 0, xb = active && ib >= 0 ? ib : 0;
         const int na = lds_ld(&cnt[xa]), nb_ = lds_ld(&cnt[xb]);
+        const double pax = sx[3 * xa], pay = sx[3 * xa + 1], paa = sx[3 * xa + 2];  // read with the counters
+        const double pbx = sx[3 * xb], pby = sx[3 * xb + 1], pbb = sx[3 * xb + 2];
         const bool adv = active & ((ia < 0) | (na == ra)) & ((ib < 0) | (nb_ == rb));
         if (adv) {
-          const double pax = sx[3 * xa], pay = sx[3 * xa + 1], paa = sx[3 * xa + 2];
-          const double pbx = sx[3 * xb], pby = sx[3 * xb + 1], pbb = sx[3 * xb + 2];
           double ax = ia >= 0 ? pax : cur.scx, ay = ia >= 0 ? pay : cur.scy, aa = ia >= 0 ? paa : cur.sa;
           double bx = ib >= 0 ? pbx : cur.scx, by = ib >= 0 ? pby : cur.scy, bb = ib >= 0 ? pbb : cur.sa;
           solve_contact_position(cur, ax, ay, aa, bx, by, bb, min_sep);
@@ -1656,7 +1726,7 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
           j += TS;
           active = j < nci;
           cur = nxt;
-          if (j + TS < nci) nxt = d.pc[cstart + order[j + TS]];
+          if (j + TS < nci) nxt = d.pc[cstart + (order[j + TS] & 0xFFFFu)];
         }
         pass_fence();
       }
@@ -1719,7 +1789,7 @@ __global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_e
       // classes own consecutive wave ranges, highest class first
       int cls = kSizeClasses - 1;
       while (cls > d.heavy_class && g >= d.q_cls[kQWaveBase + cls - 1]) --cls;
-      const int TS = team_size(cls, d.team_full_class);
+      const int TS = team_size(cls, d.team_full_class, d.team_max);
       const int per = kSolveBlock / TS;
       const int team = l / TS, t = l % TS;
       const int q = d.q_cls[kQBase + cls] + (g - d.q_cls[kQWaveBase + cls]) * per + team;
@@ -1730,13 +1800,14 @@ __global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_e
         const int nci = nc & 0x3FFFFFFF, last_allow = (nc >> 30) & 1;
         const int share = d.team_lds / per;
         const long long t0 = d.i_time ? (long long)wall_clock64() : 0;
-        const bool fits = team_lds_bytes(in.z) <= share;
+        const int nbi = in.z & 0xFFFF, period = (int)((uint32_t)in.z >> 16);
+        const bool fits = team_lds_bytes(nbi) <= share;
         if (fits) {
-          solve_island_team(d, in.x, in.y, in.z, in.w, nci, last_allow, team_lds + team * share, t, TS);
+          solve_island_team(d, in.x, in.y, nbi, in.w, nci, last_allow, period, team_lds + team * share, t, TS);
         } else if (t == 0) {
-          solve_island(d, in.x, in.y, in.z, in.w, nci, last_allow);
+          solve_island(d, in.x, in.y, nbi, in.w, nci, last_allow);
         }
-        if (d.i_time && t == 0) record_island_time(d, q, t0, nci, in.z, fits ? TS : 0);
+        if (d.i_time && t == 0) record_island_time(d, q, t0, nci, nbi, fits ? TS : 0);
       }
       team_sync();
     } else {
@@ -1745,8 +1816,8 @@ __global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_e
         const long long t0 = d.i_time ? (long long)wall_clock64() : 0;
         const int4 in = d.qs_isl[q];
         const int nc = d.qs_nc[q];
-        solve_island(d, in.x, in.y, in.z, in.w, nc & 0x3FFFFFFF, (nc >> 30) & 1);
-        if (d.i_time) record_island_time(d, q, t0, nc & 0x3FFFFFFF, in.z, 1);
+        solve_island(d, in.x, in.y, in.z & 0xFFFF, in.w, nc & 0x3FFFFFFF, (nc >> 30) & 1);
+        if (d.i_time) record_island_time(d, q, t0, nc & 0x3FFFFFFF, in.z & 0xFFFF, 1);
       }
     }
   }
@@ -2307,7 +2378,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
     const int cap_b = cntb[r.x], cap_c = cntc[r.x];
     int* stack = d.g_stack + b0 + 2 * (size_t)cb + bstart + 2 * cstart;  // cap_b + 2 cap_c entries
     int* statics = stack + cap_b + 2 * cap_c;                              // grows downwards
-    int nstat = 0, sp = 0, nbi = 0, nci = 0, last_allow = 0;
+    int nstat = 0, sp = 0, nbi = 0, nci = 0, last_allow = 0, span = 0, depth = 0;
     stack[sp++] = s0;
     bst[s0] |= 16;
     bool bad = false;
@@ -2328,14 +2399,27 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
         cflag[a.x] = 1;
         const int o = a.y;
         const bool o_dyn = !(bst[o] & 1);
-        const uint32_t rank_b = (uint32_t)cur[b]++;
-        const uint32_t rank_o = o_dyn ? (uint32_t)cur[o]++ : 0u;
+        // cur = rank | first level << 16
         const int lo = o_dyn ? lev[o] : 0;
         const int lvl = 1 + (lev[b] > lo ? lev[b] : lo);
+        const int cb_ = cur[b];
+        const uint32_t rank_b = (uint32_t)(cb_ & 0xFFFF);
+        const int fb = rank_b == 0 ? lvl : (int)((uint32_t)cb_ >> 16);
+        cur[b] = (int)(rank_b + 1) | fb << 16;
+        span = max(span, lvl - fb);
+        uint32_t rank_o = 0u;
+        if (o_dyn) {
+          const int co = cur[o];
+          rank_o = (uint32_t)(co & 0xFFFF);
+          const int fo = rank_o == 0 ? lvl : (int)((uint32_t)co >> 16);
+          cur[o] = (int)(rank_o + 1) | fo << 16;
+          span = max(span, lvl - fo);
+        }
+        depth = max(depth, lvl);
         lev[b] = lvl;
         if (o_dyn) lev[o] = lvl;
         d.i_rank[cb + cstart + nci] = make_uint2((uint32_t)b, rank_b | rank_o << 16);
-        d.i_order[cb + cstart + nci] = (uint16_t)(lvl - 1);  // level, until sorted below
+        d.i_order[cb + cstart + nci] = (uint32_t)(lvl - 1);  // level, until sorted below
         isl_c[cstart + nci++] = a.x;
         if (o_dyn) {
           if (bst[o] & 16) continue;
@@ -2354,7 +2438,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
       for (int q = nci; q < cap_c; ++q) isl_c[cstart + q] = 0;  // keep later passes in range
       nbi = nci = 0;
     }
-    d.i_info[b0 + k] = make_int4(bstart, nbi, cstart, nci | (last_allow << 30));
+    d.i_info[b0 + k] = make_int4(bstart, nbi | sweep_period(span, depth) << 16, cstart, nci | (last_allow << 30));
     // the island's contacts by level (counting sort in the stack area)
     int* hist = stack;
     for (int q = 0; q < nci; ++q) hist[q] = 0;
@@ -2366,11 +2450,11 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
       run += v;
     }
     for (int q = 0; q < nci; ++q) {
-      const int lv = d.i_order[cb + cstart + q];
-      stack[nci + hist[lv]] = q;
+      const int lv = (int)d.i_order[cb + cstart + q];
+      stack[nci + hist[lv]] = q | lv << 16;  // island-local index | level key << 16
       hist[lv] += 1;
     }
-    for (int q = 0; q < nci; ++q) d.i_order[cb + cstart + q] = (uint16_t)stack[nci + q];
+    for (int q = 0; q < nci; ++q) d.i_order[cb + cstart + q] = (uint32_t)stack[nci + q];
   }
   __syncthreads();
   // --- queue, island-local contact ends, ranks oriented as (body A, body B) ---
@@ -2382,7 +2466,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
     const int4 in = d.i_info[b0 + k];
     d.q_isl[qbase + k] = make_int4(w, b0 + in.x, in.y, cb + in.z);
     d.q_nc[qbase + k] = in.w;
-    atomicAdd(&scan[size_class(in.w & 0x3FFFFFFF)], 1);
+    atomicAdd(&scan[queue_class(d, in.w & 0x3FFFFFFF, in.y & 0xFFFF)], 1);
   }
   for (int k = tid; k < nci_tot; k += kBigBlock) {
     const int2 pr = d.c_pair[cb + isl_c[k]];
@@ -2653,7 +2737,8 @@ __global__ void k_order_islands(Dev<T> d) {
       if (blockIdx.x == 0) {
         d.q_cls[kQBase + c] = run;
         d.q_cls[kQWaveBase + c] = waves;
-        if (c >= d.heavy_class) waves += (nc * team_size(c, d.team_full_class) + kSolveBlock - 1) / kSolveBlock;
+        if (c >= d.heavy_class)
+          waves += (nc * team_size(c, d.team_full_class, d.team_max) + kSolveBlock - 1) / kSolveBlock;
       }
       run += nc;
       if (c == d.heavy_class && blockIdx.x == 0) {
@@ -2666,7 +2751,8 @@ __global__ void k_order_islands(Dev<T> d) {
   const int n = *d.q_count;
   const int per = (n + gridDim.x - 1) / gridDim.x;
   const int q0 = blockIdx.x * per, q1 = min(n, q0 + per);
-  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) atomicAdd(&lcnt[size_class(d.q_nc[q] & 0x3FFFFFFF)], 1);
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x)
+    atomicAdd(&lcnt[queue_class(d, d.q_nc[q] & 0x3FFFFFFF, d.q_isl[q].z & 0xFFFF)], 1);
   __syncthreads();
   if (threadIdx.x < kSizeClasses) {
     const int c = threadIdx.x;
@@ -2676,9 +2762,10 @@ __global__ void k_order_islands(Dev<T> d) {
   __syncthreads();
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     const int nc = d.q_nc[q];
-    const int c = size_class(nc & 0x3FFFFFFF);
+    const int4 isl = d.q_isl[q];
+    const int c = queue_class(d, nc & 0x3FFFFFFF, isl.z & 0xFFFF);
     const int pos = lbase[c] + atomicAdd(&lcnt[c], 1);
-    d.qs_isl[pos] = d.q_isl[q];
+    d.qs_isl[pos] = isl;
     d.qs_nc[pos] = nc;
   }
 }
@@ -2769,7 +2856,8 @@ struct b2g_ctx {
   std::vector<int> big;  // worlds on that path
   int heavy_class = 5;
   int team_full_class = 9;
-  int team_lds = 32768;
+  int team_lds = 24576;
+  int team_max = 32;
   int solve_grid = 16384;
   hipStream_t stream = nullptr;
   std::string err;
@@ -3264,6 +3352,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   d.heavy_class = ctx->heavy_class;
   d.team_full_class = ctx->team_full_class;
   d.team_lds = ctx->team_lds;
+  d.team_max = ctx->team_max;
   d.i_time = ctx->island_times ? ctx->itime : nullptr;
   const int nw = ctx->nw;
   hipStream_t st = ctx->stream;
@@ -3626,10 +3715,13 @@ B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value) {
     case B2G_PARAM_TEAM_LDS: slot = &ctx->team_lds; lo = 0; hi = kMaxLds; break;
     case B2G_PARAM_SOLVE_GRID: slot = &ctx->solve_grid; lo = 1; hi = 1 << 30; break;
     case B2G_PARAM_ISLAND_TIMES: slot = &ctx->island_times; lo = 0; hi = 1; break;
+    case B2G_PARAM_TEAM_MAX: slot = &ctx->team_max; lo = 8; hi = kSolveBlock; break;
     case B2G_PARAM_SOLVE_WAVES_PER_CU:
       return set_error(ctx, B2G_E_INVALID, "b2g_set_param: B2G_PARAM_SOLVE_WAVES_PER_CU is read-only");
     default: return set_error(ctx, B2G_E_INVALID, "b2g_set_param: unknown key " + std::to_string(key));
   }
+  if (key == B2G_PARAM_TEAM_MAX && (value & (value - 1)) != 0)
+    return set_error(ctx, B2G_E_INVALID, "b2g_set_param: the largest team must be 8, 16, 32 or 64 lanes");
   if (key == B2G_PARAM_TEAM_LDS && value % 256 != 0)
     return set_error(ctx, B2G_E_INVALID, "b2g_set_param: the team LDS budget must be a multiple of 256");
   if (value < lo || value > hi)
@@ -3655,6 +3747,7 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value) {
     case B2G_PARAM_TEAM_LDS: *value = ctx->team_lds; return B2G_OK;
     case B2G_PARAM_SOLVE_GRID: *value = ctx->solve_grid; return B2G_OK;
     case B2G_PARAM_ISLAND_TIMES: *value = ctx->island_times; return B2G_OK;
+    case B2G_PARAM_TEAM_MAX: *value = ctx->team_max; return B2G_OK;
     case B2G_PARAM_SOLVE_WAVES_PER_CU: {
       // resident solver wavefronts per compute unit: 4 SIMDs x (512 VGPRs / VGPRs per lane), and
       // the LDS the launch reserves per one-wavefront workgroup

@@ -224,7 +224,9 @@ def test_island_launch_shapes_agree_bit_exactly(prec):
     results = {}
     for name, params in (("lanes", {"heavy_class": 16}), ("teams64", {"heavy_class": 0, "team_full_class": 0}),
                          ("teams8", {"heavy_class": 0, "team_full_class": 16}), ("default", {}),
-                         ("fallback", {"heavy_class": 3, "team_lds": 4096}), ("large_worlds", {"large_worlds": 1})):
+                         ("fallback", {"heavy_class": 3, "team_lds": 4096}), ("large_worlds", {"large_worlds": 1}),
+                         ("pairs", {"heavy_class": 5, "team_full_class": 9, "team_max": 32, "team_lds": 24576}),
+                         ("whole_wave", {"heavy_class": 3, "team_full_class": 9, "team_max": 16, "team_lds": 8192})):
         e = N.Engine(prec)
         for k, v in params.items():
             e.set_param(k, v)
@@ -246,7 +248,7 @@ def test_set_param_rejects_bad_values():
     e = N.Engine("fp32")
     assert e.get_param("heavy_class") >= 0
     for k, v in (("heavy_class", 17), ("team_lds", 200000), ("team_lds", 1000), ("solve_grid", 0),
-                 ("team_full_class", -1)):
+                 ("team_full_class", -1), ("team_max", 24), ("team_max", 4), ("team_max", 128)):
         with pytest.raises(N.B2GError):
             e.set_param(k, v)
     e.close()
