@@ -1611,16 +1611,17 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
       }
       return true;
     };
-    int s = 0, k = 0, ns = 0, nk = 0;
-    VC<T> cur, nxt;
+    // A lane holds one record in registers, read when it advances: a second, prefetched
+    // record would have to be copied into place at every advance, which costs more than the
+    // read latency it hides (a lane rarely takes two solves in consecutive passes).
+    int s = 0, k = 0;
+    VC<T> cur;
     bool active = next_solve(s, k);
     if (active) cur = d.vc[cstart + k];
-    bool have_nxt = next_solve(ns, nk);
-    if (have_nxt) nxt = d.vc[cstart + nk];
     const int cap = 2 * nsw * nci + 64;
-    // The pass test sits at the bottom and readiness is computed without branches: the
-    // compiler then keeps the prefetched records where they are instead of copying them at
-    // every join (inactive lanes and static sides read entry 0 and ignore it).
+    // The pass test sits at the bottom and readiness is computed without branches, so the
+    // compiler keeps the record where it is (inactive lanes and static sides read entry 0 and
+    // ignore it).
     int pass = 0;
     for (bool more = wave_any(active); more; more = wave_any(active && pass < cap)) {
       ++pass;
@@ -1654,19 +1655,10 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
           }
           if (ia >= 0) { sx[3 * ia] = va.x; sx[3 * ia + 1] = va.y; sx[3 * ia + 2] = wa; lds_st(&cnt[ia], ra + 1); }
           if (ib >= 0) { sx[3 * ib] = vb.x; sx[3 * ib + 1] = vb.y; sx[3 * ib + 2] = wb; lds_st(&cnt[ib], rb + 1); }
-          if (have_nxt) {
-            // the next record was read before this solve stored its impulses: when it is the
-            // same contact, the impulses come from the registers
-            const bool same = nk == k;
-            nxt.p[0].ni = same ? cur.p[0].ni : nxt.p[0].ni;
-            nxt.p[0].ti = same ? cur.p[0].ti : nxt.p[0].ti;
-            nxt.p[1].ni = same ? cur.p[1].ni : nxt.p[1].ni;
-            nxt.p[1].ti = same ? cur.p[1].ti : nxt.p[1].ti;
-            cur = nxt;
-            s = ns;
-            k = nk;
-            have_nxt = next_solve(ns, nk);
-            if (have_nxt) nxt = d.vc[cstart + nk];
+          // the read follows this lane's own impulse stores, so a later sweep of the same
+          // contact sees them
+          if (next_solve(s, k)) {
+            cur = d.vc[cstart + k];
           } else {
             active = false;
           }
@@ -1699,15 +1691,14 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
       T min_sep = 0;
       bool active = mine > 0;
       int j = t;
-      PC<T> cur, nxt;
+      PC<T> cur;  // one record per lane, read when it advances (as in the velocity sweeps)
       if (active) cur = d.pc[cstart + (order[j] & 0xFFFFu)];
-      if (active && j + TS < nci) nxt = d.pc[cstart + (order[j + TS] & 0xFFFFu)];
       int pass = 0;
       const int pcap = 2 * nci + 64;
       for (bool more = wave_any(active); more; more = wave_any(active && pass < pcap)) {
         ++pass;
-        // readiness without branches (a branchy test makes the compiler copy the prefetched
-        // records at every join); inactive lanes and static sides read entry 0 and ignore it
+        // readiness without branches (a branchy test makes the compiler copy the record at
+        // every join); inactive lanes and static sides read entry 0 and ignore it
         const int ia = cur.ia, ib = cur.ib;
         const int ra = cur.rka & 0xFFFF, rb = cur.rkb & 0xFFFF;
         const int xa = active && ia >= 0 ? ia : 0, xb = active && ib >= 0 ? ib : 0;
@@ -1725,8 +1716,7 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
         if (adv) {
           j += TS;
           active = j < nci;
-          cur = nxt;
-          if (j + TS < nci) nxt = d.pc[cstart + (order[j + TS] & 0xFFFFu)];
+          if (active) cur = d.pc[cstart + (order[j] & 0xFFFFu)];
         }
         pass_fence();
       }
