@@ -222,8 +222,11 @@ struct Dev {
   int* i_bodies;     // per-world segment at w_body_off[w]
   int* i_contacts;   // per-world segment at w_con_base[w]
   int2* i_cidx;      // per-world segment at w_con_base[w]
+  uint2* i_fb;       // per-world segment at w_con_base[w]: the contact's fixtures (x = A | B << 16) and
+                     // bodies (y = A | B << 16), world-local, so the solver reads them in one step
   uint2* i_rank;     // per-world segment at w_con_base[w]: the contact's rank in body A's and body B's
-                     // contact sequence of the island (x = rank A | rank B << 16)
+                     // contact sequence of the island (x = rank A | rank B << 16) and those bodies'
+                     // numbers of island contacts (y = degree A | degree B << 16; 0 for a static side)
   uint32_t* i_order; // per-world segment at w_con_base[w]: each island's contacts ordered by
                      // dependency level within one sweep, as island-local index | level key << 16
                      // (the key is the level plus a per-island offset)
@@ -727,8 +730,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
           lev_b = lvl;
           if (o_dyn) L.lev[o] = (uint16_t)lvl;
           const uint32_t key = (uint32_t)(cstart + lvl - 1);  // < cstart + island contacts
-          d.i_rank[cb + nci_tot] = make_uint2((uint32_t)b | key << 16, rank_b | rank_o << 16);
-          d.i_order[cb + nci_tot] = (uint32_t)(nci_tot - cstart);
+          d.i_rank[cb + nci_tot] = make_uint2((uint32_t)b, rank_b | rank_o << 16);
+          d.i_order[cb + nci_tot] = (uint32_t)(nci_tot - cstart) | key << 16;  // sorted below
           isl_c[nci_tot++] = s;
           if (so & 16) continue;
           L.stack[sp++] = (uint16_t)o;
@@ -766,25 +769,31 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     int ia = (L.bst[la] & 1) ? -1 - la : (int)L.bidx[la];
     int ib = (L.bst[lb] & 1) ? -1 - lb : (int)L.bidx[lb];
     d.i_cidx[cb + k] = make_int2(ia, ib);
-    // (expanding body | sort key << 16, its rank | the other body's rank << 16)
+    d.i_fb[cb + k] = make_uint2((uint32_t)pr.x | (uint32_t)pr.y << 16, (uint32_t)la | (uint32_t)lb << 16);
+    // (expanding body, its rank | the other body's rank << 16) -> ranks and degrees of (A, B);
+    // a dynamic body's rank counter ends at its number of island contacts
     const uint2 r = d.i_rank[cb + k];
     const uint32_t lo = r.y & 0xFFFFu, hi = r.y >> 16;
-    const uint32_t key = r.x >> 16;
-    d.i_rank[cb + k] = make_uint2((int)(r.x & 0xFFFFu) == la ? (lo | hi << 16) : (hi | lo << 16),
-                                  key | d.i_order[cb + k] << 16);
+    const uint32_t dga = ia >= 0 ? (uint32_t)L.cur[la] & 0xFFFFu : 0u;
+    const uint32_t dgb = ib >= 0 ? (uint32_t)L.cur[lb] & 0xFFFFu : 0u;
+    d.i_rank[cb + k] = make_uint2((int)r.x == la ? (lo | hi << 16) : (hi | lo << 16), dga | dgb << 16);
   }
-  // each island's contacts by level: a counting sort on key = island start + level - 1
-  // (keys of one island stay inside its contact range); the adjacency arrays are free now
+  // each island's contacts by level: a counting sort of the entries (island-local index |
+  // key << 16, key = island start + level - 1, so keys of one island stay inside its contact
+  // range) through LDS; the adjacency arrays are free now (they hold 2 entries per contact)
   int* hist = (int*)L.adj;
-  for (int k = tid; k < nci_tot; k += kBlock) hist[k] = 0;
+  uint32_t* ent = L.adj + nci_tot;
+  for (int k = tid; k < nci_tot; k += kBlock) {
+    hist[k] = 0;
+    ent[k] = d.i_order[cb + k];
+  }
   __syncthreads();
-  for (int k = tid; k < nci_tot; k += kBlock) atomicAdd(&hist[d.i_rank[cb + k].y & 0xFFFFu], 1);
+  for (int k = tid; k < nci_tot; k += kBlock) atomicAdd(&hist[ent[k] >> 16], 1);
   __syncthreads();
   block_exclusive_scan(hist, nci_tot, L.scan);
   for (int k = tid; k < nci_tot; k += kBlock) {
-    const uint32_t y = d.i_rank[cb + k].y;
-    const int pos = atomicAdd(&hist[y & 0xFFFFu], 1);
-    d.i_order[cb + pos] = (y >> 16) | (y << 16);  // island-local index | level key << 16
+    const uint32_t e = ent[k];
+    d.i_order[cb + atomicAdd(&hist[e >> 16], 1)] = e;
   }
   for (int i = tid; i < nb; i += kBlock) {
     uint8_t bs = L.bst[i];
@@ -1255,16 +1264,16 @@ __device__ __forceinline__ PV<T> integrate_velocity(const Dev<T>& d, const Islan
 // d.pv; writes the constraint records and turns i_cidx island-local.
 template <typename T>
 __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int cstart, int k, int& ia_out,
-                             int& ib_out) {
+                             int& ib_out, int rka = 0, int rkb = 0) {
   const int b0 = c.b0, f0 = c.f0, pvb = c.pvb;
   const int gs_ = c.cbase + d.i_contacts[cstart + k];
   int2 cix = d.i_cidx[cstart + k];
+  const uint2 fbx = d.i_fb[cstart + k];
   const int ia = cix.x >= 0 ? cix.x - pvb : cix.x, ib = cix.y >= 0 ? cix.y - pvb : cix.y;
   ia_out = ia;
   ib_out = ib;
-  int2 pr = d.c_pair[gs_];
-  const int fa = f0 + pr.x, fb = f0 + pr.y;
-  const int ba = b0 + d.f_info[fa].x, bb = b0 + d.f_info[fb].x;
+  const int fa = f0 + (int)(fbx.x & 0xFFFFu), fb = f0 + (int)(fbx.x >> 16);
+  const int ba = b0 + (int)(fbx.y & 0xFFFFu), bb = b0 + (int)(fbx.y >> 16);
   const int st = d.c_state[gs_];
   const int mtype = (st >> CS_MTYPE_SHIFT) & 3, pc = (st >> CS_PC_SHIFT) & 3;
   V4<T> lnlp = d.c_lnlp[gs_], pts = d.c_pts[gs_], imp = d.c_imp[gs_];
@@ -1281,7 +1290,8 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
   vc.Ib = d.b_iI[bb];
   vc.npts = pc;
   vc.pad = 0;
-  vc.rka = vc.rkb = 0;
+  vc.rka = rka;
+  vc.rkb = rkb;
   const bool warm = c.warm;
   const double dt_ratio = c.dt_ratio;
   vc.p[0].ni = warm ? dt_ratio * (double)imp.x : (double)imp.x;
@@ -1299,7 +1309,8 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
   pcn.npts = pc;
   pcn.ia = ia;
   pcn.ib = ib;
-  pcn.rka = pcn.rkb = 0;
+  pcn.rka = rka;
+  pcn.rkb = rkb;
   double cax, cay, aa, cbx, cby, ab;
   load_pos(d, bstart, b0, ia, cax, cay, aa);
   load_pos(d, bstart, b0, ib, cbx, cby, ab);
@@ -1550,27 +1561,15 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
     cnt[i] = 0;
   }
   team_sync();
-  // 2. constraint records, and each body's number of contacts
+  // 2. constraint records, with each contact's rank in its bodies' sequences and their
+  // degrees (from the island search): rank | degree << 16
   for (int k = t; k < nci; k += TS) {
+    const uint2 r = d.i_rank[cstart + k];
+    const int rka = (int)((r.x & 0xFFFFu) | (r.y & 0xFFFFu) << 16);
+    const int rkb = (int)((r.x >> 16) | (r.y >> 16) << 16);
     int ia, ib;
-    init_contact(d, c, bstart, cstart, k, ia, ib);
-    if (ia >= 0) atomicAdd(&cnt[ia], 1);
-    if (ib >= 0) atomicAdd(&cnt[ib], 1);
+    init_contact(d, c, bstart, cstart, k, ia, ib, rka, rkb);
   }
-  team_sync();
-  // ranks and degrees into the records (the lane that wrote them reads them)
-  for (int k = t; k < nci; k += TS) {
-    const int2 x = d.i_cidx[cstart + k];
-    const uint32_t r = d.i_rank[cstart + k].x;
-    const int rka = (int)(r & 0xFFFFu) | (x.x >= 0 ? cnt[x.x] << 16 : 0);
-    const int rkb = (int)(r >> 16) | (x.y >= 0 ? cnt[x.y] << 16 : 0);
-    d.vc[cstart + k].rka = rka;
-    d.vc[cstart + k].rkb = rkb;
-    d.pc[cstart + k].rka = rka;
-    d.pc[cstart + k].rkb = rkb;
-  }
-  team_sync();
-  for (int i = t; i < nbi; i += TS) cnt[i] = 0;
   team_sync();
   // 3 + 4. warm start and velocity sweeps as one dataflow sequence
   const int s0 = c.warm ? 0 : 1;
@@ -2464,9 +2463,12 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
     const int ia = (bst[la] & 1) ? -1 - la : bidx[la];
     const int ib = (bst[lb] & 1) ? -1 - lb : bidx[lb];
     d.i_cidx[cb + k] = make_int2(ia, ib);
+    d.i_fb[cb + k] = make_uint2((uint32_t)pr.x | (uint32_t)pr.y << 16, (uint32_t)la | (uint32_t)lb << 16);
     const uint2 r = d.i_rank[cb + k];
     const uint32_t lo = r.y & 0xFFFFu, hi = r.y >> 16;
-    d.i_rank[cb + k] = make_uint2((int)r.x == la ? (lo | hi << 16) : (hi | lo << 16), 0u);
+    const uint32_t dga = ia >= 0 ? (uint32_t)cur[la] & 0xFFFFu : 0u;  // rank counters end at the degrees
+    const uint32_t dgb = ib >= 0 ? (uint32_t)cur[lb] & 0xFFFFu : 0u;
+    d.i_rank[cb + k] = make_uint2((int)r.x == la ? (lo | hi << 16) : (hi | lo << 16), dga | dgb << 16);
   }
   __syncthreads();
   if (tid < kSizeClasses && scan[tid] > 0) atomicAdd(&d.q_cls[tid], scan[tid]);
@@ -3210,6 +3212,7 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.i_bodies, nb));
   TRY(dalloc(ctx, &d.i_contacts, ncap));
   TRY(dalloc(ctx, &d.i_cidx, ncap));
+  TRY(dalloc(ctx, &d.i_fb, ncap));
   TRY(dalloc(ctx, &d.i_rank, ncap));
   TRY(dalloc(ctx, &d.i_order, ncap));
   TRY(dalloc(ctx, &d.i_info, nb));
