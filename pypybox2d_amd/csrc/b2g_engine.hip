@@ -718,15 +718,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
           const int lvl = 1 + (lev_b > lo ? lev_b : lo);
           const uint32_t rank_b = (uint32_t)(cur_b & 0xFFFF);
           const uint32_t rank_o = o_dyn ? (uint32_t)(co & 0xFFFF) : 0u;
-          if (rank_b == 0) cur_b |= lvl << 16;
-          span = max(span, lvl - (int)((uint32_t)cur_b >> 16));
-          cur_b += 1;
-          if (o_dyn) {
-            const int fo = rank_o == 0 ? lvl : (int)((uint32_t)co >> 16);
-            span = max(span, lvl - fo);
-            L.cur[o] = (int)(rank_o + 1) | fo << 16;
-          }
-          depth = max(depth, lvl);
+          cur_b += 1 + (rank_b == 0 ? lvl << 16 : 0);  // a first contact records its level
+          if (o_dyn) L.cur[o] = co + 1 + (rank_o == 0 ? lvl << 16 : 0);
           lev_b = lvl;
           if (o_dyn) L.lev[o] = (uint16_t)lvl;
           const uint32_t key = (uint32_t)(cstart + lvl - 1);  // < cstart + island contacts
@@ -739,6 +732,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
         }
         L.cur[b] = cur_b;
         L.lev[b] = (uint16_t)lev_b;
+        // b has no contacts left to take: its levels run from its first contact's to lev_b
+        if (cur_b & 0xFFFF) {
+          span = max(span, lev_b - (int)((uint32_t)cur_b >> 16));
+          depth = max(depth, lev_b);
+        }
       }
       for (int k = 0; k < ns; ++k) L.bst[L.slist[k]] &= (uint8_t)~16;  // statics may join other islands
       info[nisl++] = make_int4(bstart, (nbi_tot - bstart) | sweep_period(span, depth) << 16, cstart,
