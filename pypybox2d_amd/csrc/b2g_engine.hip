@@ -707,9 +707,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
           const int s = (int)(e & 0xFFFFu), o = (int)(e >> 16);
           e = en;
           const uint32_t bit = 1u << (s & 31);
-          const uint32_t fw = L.cflag[s >> 5];
-          const uint8_t so = L.bst[o];
-          const int co = L.cur[o], lo_read = L.lev[o];
+          // relaxed atomic loads are not moved below the test, so the four reads share one wait
+          const uint32_t fw = __hip_atomic_load(&L.cflag[s >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const uint8_t so = __hip_atomic_load(&L.bst[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const int co = __hip_atomic_load(&L.cur[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const int lo_read = __hip_atomic_load(&L.lev[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           if (fw & bit) continue;
           L.cflag[s >> 5] = fw | bit;
           const bool o_dyn = !(so & 1);
