@@ -247,7 +247,7 @@ struct Dev {
   // the scratch below is allocated only when the batch has such worlds
   const int* w_big;    // per world: 1 = large path
   const int* big_list; // the large worlds
-  const int* w_order;  // worlds by descending body count: the island search takes the longest first
+  const int* w_order;  // worlds by descending body count: the island and pair searches take the longest first
   long long* i_time;   // null, or per queued island: start and end ticks, contacts | bodies << 32, lanes
   int* g_bst;          // per body: state bits as in k_islands
   int* g_wake;         // per body: slot of the contact that woke it (awake gating)
@@ -1975,7 +1975,7 @@ __device__ void pair_candidate(const Dev<T>& d, int w, int f0, int b0, int P, in
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int from_upload, int finish, int do_pairs) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int w = blockIdx.x, tid = threadIdx.x;
+  const int w = d.w_order[blockIdx.x], tid = threadIdx.x;  // largest worlds first
   if (d.w_big && d.w_big[w]) return;  // k_pairs_big_*
   const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
   const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
