@@ -108,11 +108,16 @@ def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, tune:
             n = min(tuner.interval() - len(tuner.candidates), steps - done)
         else:
             n = min(200, left)
+        seg = None
         if n > 0:
-            sums += eng.profile(n) * n
+            seg = eng.profile(n)
+            sums += seg * n
             done += n
-        log("[%s] %d/%d steps, %.2f s%s" % (precision, done, steps, time.perf_counter() - t0,
-                                           (", launch shape %s" % tuner.best) if tuner else ""))
+        log("[%s] %d/%d steps, %.2f s%s%s" % (
+            precision, done, steps, time.perf_counter() - t0,
+            (", launch shape %s (solve ms per candidate %s)" % (tuner.best, np.round(tuner.last_times, 2).tolist()))
+            if tuner and tuner.last_times is not None else "",
+            (", solve %.2f ms over the next %d steps" % (seg[2], n)) if seg is not None else ""))
     eng.sync()
     elapsed = time.perf_counter() - t0
     state = eng.download()
