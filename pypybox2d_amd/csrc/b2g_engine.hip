@@ -1151,7 +1151,7 @@ __device__ __forceinline__ void solve_point_position(const PC<T>& pc, int j, dou
 }
 // The points are solved in order, each from the positions the previous one left
 // (written out rather than as a loop: a loop here makes the compiler copy the
-// team path's prefetched records at every branch of its pass).
+// team path's record at every branch of its pass).
 template <typename T>
 __device__ __forceinline__ void solve_contact_position(const PC<T>& pc, double& cax, double& cay, double& aa,
                                                        double& cbx, double& cby, double& ab, T& min_sep) {
@@ -1501,21 +1501,21 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
 // A team is 8, 16, 32 or 64 lanes of one wavefront (by the island's size
 // class); a wavefront carries 64 / T islands of one class.  The island search
 // orders each island's contacts by dependency level within one sweep; lane t
-// of a team owns entries t, t + T, t + 2T, ... of that order and walks them
-// sweep by sweep, so its contacts come in non-decreasing level.  A contact's solve at sweep s must see every earlier solve (in the
-// reference's Gauss-Seidel order) that touches one of its dynamic bodies, and
-// nothing later.  Each dynamic body X keeps a counter of completed solves that
-// touched it; the contact is ready when, for each of its dynamic bodies, the
-// counter equals its rank in that body's sequence (s * deg(X) + the number of
-// the body's contacts before it in the island list, from the island search).
+// of a team owns entries t, t + T, t + 2T, ... of that order.  A contact's
+// solve at sweep s must see every earlier solve (in the reference's
+// Gauss-Seidel order) that touches one of its dynamic bodies, and nothing
+// later.  Each dynamic body X keeps a counter of completed solves that touched
+// it; the contact is ready when, for each of its dynamic bodies, the counter
+// equals its rank in that body's sequence (s * deg(X) + the number of the
+// body's contacts before it in the island list, from the island search).
 // Ready lanes run in the same pass, which is safe because at most one pending
-// solve per body can be ready.  An unfinished solve with the smallest (sweep,
-// level) has only finished predecessors, and so has the head of its lane (the
-// lane's sequence is sorted by sweep and level), so every pass makes progress;
-// passes are still capped.  Body
-// velocities, then positions, live in LDS; the constraint records stay in HBM
-// and are prefetched one solve ahead.  The arithmetic per contact is the same
-// code as the one-lane path, so results are identical.
+// solve per body can be ready.  A lane takes its solves in order of
+// s * period + level (sweep_period): every dependency goes to a larger key, so
+// the unfinished solve with the smallest key has only finished predecessors
+// and is at the head of its lane, and every pass makes progress; passes are
+// still capped.  Body velocities, then positions, live in LDS; the constraint
+// records stay in HBM, one per lane in registers.  The arithmetic per contact
+// is the same code as the one-lane path, so results are identical.
 // ---------------------------------------------------------------------------
 
 // The lanes of one wavefront exchange body state through LDS between passes.
