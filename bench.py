@@ -311,9 +311,21 @@ def measured_traffic(kernel: str, workload: str):
 def roofline(kernel: str, ms: float, counts: dict, fp64: bool, workload: str = "mixed") -> dict:
     ab = algorithmic_bytes(kernel, counts, fp64)
     achieved = ab / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, workload), "algorithmic_bytes": ab,
-            "avg_ms": ms}
+    out = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, workload), "algorithmic_bytes": ab,
+           "avg_ms": ms}
+    if kernel in LATENCY_LIMITS:
+        out["limit"] = LATENCY_LIMITS[kernel]
+    return out
+
+
+# Kernels whose time is set by serial dependency chains rather than by bytes or flops; their HBM
+# fraction is reported for completeness and the "solver" report gives the solve's own measure.
+LATENCY_LIMITS = {
+    "k_solve": "dependency chains inside the largest islands: passes (about the Gauss-Seidel dependency depth "
+               "of all sweeps) x pass latency; see the solver report (occupancy and per-island latency)",
+    "k_islands": "one lane walks each world's (or island's) depth-first search",
+}
 
 
 def main():
