@@ -2184,28 +2184,75 @@ __device__ __forceinline__ int g_ld(const int* p) {
 __device__ __forceinline__ void g_st(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ int uf_find(int* parent, int x) {
+// Union-find over a large world's bodies (lock-free hooking of the larger root under the
+// smaller, path halving), on 16-bit parents packed two to a word in LDS (a world has fewer
+// than 65535 bodies, so any world's parents fit 128 KB): a half is replaced by a
+// compare-and-swap of its word, so a concurrent change to the other half only retries.  Once
+// the unions are done, roots are found without halving: a halving write could replace a root
+// that another thread has just stored with a non-root ancestor.
+__device__ __forceinline__ int ufl_get(const uint32_t* P, int x) {
+  const uint32_t w = __hip_atomic_load(&P[x >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return (int)((x & 1) ? (w >> 16) : (w & 0xFFFFu));
+}
+// parent[x] := v if it is still `expect`; false if it was not
+__device__ __forceinline__ bool ufl_cas(uint32_t* P, int x, int expect, int v) {
+  const int sh = (x & 1) * 16;
+  uint32_t w = __hip_atomic_load(&P[x >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   for (;;) {
-    const int p = g_ld(&parent[x]);
+    if ((int)((w >> sh) & 0xFFFFu) != expect) return false;
+    const uint32_t nw = (w & ~(0xFFFFu << sh)) | ((uint32_t)v << sh);
+    const uint32_t old = atomicCAS(&P[x >> 1], w, nw);
+    if (old == w) return true;
+    w = old;
+  }
+}
+__device__ int ufl_find(uint32_t* P, int x) {
+  for (;;) {
+    const int p = ufl_get(P, x);
     if (p == x) return x;
-    const int gp = g_ld(&parent[p]);
-    if (gp != p) g_st(&parent[x], gp);  // path halving; x is not a root, so no hook targets it
+    const int gp = ufl_get(P, p);
+    if (gp != p) ufl_cas(P, x, p, gp);  // path halving (skipped if x moved meanwhile)
     x = gp;
   }
 }
-// Root without path halving: once the unions are done, a halving write could replace a root
-// that another thread has just stored with a non-root ancestor.
-__device__ int uf_root(const int* parent, int x) {
-  for (int p = parent[x]; p != x; p = parent[x]) x = p;
+__device__ int ufl_root(const uint32_t* P, int x) {
+  for (int p = ufl_get(P, x); p != x; p = ufl_get(P, x)) x = p;
   return x;
 }
-__device__ void uf_unite(int* parent, int a, int b) {
+__device__ void ufl_unite(uint32_t* P, int a, int b) {
   for (;;) {
-    a = uf_find(parent, a);
-    b = uf_find(parent, b);
+    a = ufl_find(P, a);
+    b = ufl_find(P, b);
     if (a == b) return;
     if (a < b) { const int t = a; a = b; b = t; }
-    if (atomicCAS(&parent[a], a, b) == a) return;  // the root with the larger index joins the smaller
+    if (ufl_cas(P, a, a, b)) return;  // the root with the larger index joins the smaller
+  }
+}
+
+// The touching contacts among kEndsBatch slots base, base + kBigBlock, ... of a large world, with
+// their bodies (world-local).  The three dependent global reads (state, fixture pair, fixture
+// bodies) are issued for all slots of the batch before any is waited for: one lane of the
+// single workgroup otherwise waits out each chain in turn.
+constexpr int kEndsBatch = 4;
+template <typename T>
+__device__ __forceinline__ void touching_ends(const Dev<T>& d, int cb, int f0, int used, int base, bool* ok,
+                                              int* la, int* lb) {
+  int st[kEndsBatch];
+#pragma unroll
+  for (int u = 0; u < kEndsBatch; ++u) {
+    const int sl = base + u * kBigBlock;
+    st[u] = sl < used ? d.c_state[cb + sl] : 0;
+  }
+  int2 pr[kEndsBatch];
+#pragma unroll
+  for (int u = 0; u < kEndsBatch; ++u) {
+    ok[u] = (st[u] & (CS_LIVE | CS_TOUCHING | CS_ENABLED)) == (CS_LIVE | CS_TOUCHING | CS_ENABLED);
+    pr[u] = ok[u] ? d.c_pair[cb + base + u * kBigBlock] : make_int2(0, 0);
+  }
+#pragma unroll
+  for (int u = 0; u < kEndsBatch; ++u) {
+    la[u] = ok[u] ? d.f_info[f0 + pr[u].x].x : 0;
+    lb[u] = ok[u] ? d.f_info[f0 + pr[u].y].x : 0;
   }
 }
 
@@ -2213,6 +2260,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
   __shared__ int scan[kBigBlock];
   __shared__ int sh[8];
+  extern __shared__ uint32_t uf_lds[];  // the union-find parents, 16 bits each
   const int w = d.big_list[blockIdx.x], tid = threadIdx.x;
   const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
   const int f0 = d.w_fix_off[w];
@@ -2245,6 +2293,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
     cntc[i] = 0;
   }
   for (int sl = tid; sl < used; sl += kBigBlock) cflag[sl] = 0;
+  for (int i = tid; i < (nb + 1) / 2; i += kBigBlock) uf_lds[i] = (uint32_t)(2 * i) | (uint32_t)(2 * i + 1) << 16;
   if (tid < 8) sh[tid] = 0;
   __syncthreads();
 
@@ -2286,14 +2335,22 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
   }
 
   // --- touching-contact graph in ContactManager order ---
-  for (int sl = tid; sl < used; sl += kBigBlock) {
-    const int st = d.c_state[cb + sl];
-    if ((st & (CS_LIVE | CS_TOUCHING | CS_ENABLED)) == (CS_LIVE | CS_TOUCHING | CS_ENABLED)) {
-      const int2 pr = d.c_pair[cb + sl];
-      const int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
-      atomicAdd(&off[la], 1);
-      atomicAdd(&off[lb], 1);
-      if (!(bst[la] & 1) && !(bst[lb] & 1)) uf_unite(parent, la, lb);
+  for (int base = tid; base < used; base += kEndsBatch * kBigBlock) {
+    bool ok[kEndsBatch];
+    int la[kEndsBatch], lb[kEndsBatch];
+    touching_ends(d, cb, f0, used, base, ok, la, lb);
+    int sa[kEndsBatch], sb[kEndsBatch];
+#pragma unroll
+    for (int u = 0; u < kEndsBatch; ++u) {
+      sa[u] = ok[u] ? bst[la[u]] : 1;
+      sb[u] = ok[u] ? bst[lb[u]] : 1;
+    }
+#pragma unroll
+    for (int u = 0; u < kEndsBatch; ++u) {
+      if (!ok[u]) continue;
+      atomicAdd(&off[la[u]], 1);
+      atomicAdd(&off[lb[u]], 1);
+      if (!(sa[u] & 1) && !(sb[u] & 1)) ufl_unite(uf_lds, la[u], lb[u]);
     }
   }
   __syncthreads();
@@ -2301,13 +2358,22 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
   if (tid == 0) off[nb] = total;
   for (int i = tid; i < nb; i += kBigBlock) cur[i] = off[i];
   __syncthreads();
-  for (int sl = tid; sl < used; sl += kBigBlock) {
-    const int st = d.c_state[cb + sl];
-    if ((st & (CS_LIVE | CS_TOUCHING | CS_ENABLED)) == (CS_LIVE | CS_TOUCHING | CS_ENABLED)) {
-      const int2 pr = d.c_pair[cb + sl];
-      const int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
-      adj[atomicAdd(&cur[la], 1)] = make_int2(sl, lb);
-      adj[atomicAdd(&cur[lb], 1)] = make_int2(sl, la);
+  for (int base = tid; base < used; base += kEndsBatch * kBigBlock) {
+    bool ok[kEndsBatch];
+    int la[kEndsBatch], lb[kEndsBatch];
+    touching_ends(d, cb, f0, used, base, ok, la, lb);
+    int pa[kEndsBatch], pb[kEndsBatch];
+#pragma unroll
+    for (int u = 0; u < kEndsBatch; ++u) {
+      pa[u] = ok[u] ? atomicAdd(&cur[la[u]], 1) : 0;
+      pb[u] = ok[u] ? atomicAdd(&cur[lb[u]], 1) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kEndsBatch; ++u) {
+      if (!ok[u]) continue;
+      const int sl = base + u * kBigBlock;
+      adj[pa[u]] = make_int2(sl, lb[u]);
+      adj[pb[u]] = make_int2(sl, la[u]);
     }
   }
   __syncthreads();
@@ -2323,7 +2389,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
       }
       adj[j + 1] = e;
     }
-    if (!(bst[i] & 1)) wake[i] = uf_root(parent, i);  // the wake slots are free again
+    if (!(bst[i] & 1)) wake[i] = ufl_root(uf_lds, i);  // the wake slots are free again
   }
   __syncthreads();
   for (int i = tid; i < nb; i += kBigBlock)
@@ -2338,14 +2404,22 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
   __syncthreads();
   for (int i = tid; i < nb; i += kBigBlock)
     if (!(bst[i] & 1) && seed[parent[i]] != kIntMax) atomicAdd(&cntb[parent[i]], 1);
-  for (int sl = tid; sl < used; sl += kBigBlock) {
-    const int st = d.c_state[cb + sl];
-    if ((st & (CS_LIVE | CS_TOUCHING | CS_ENABLED)) == (CS_LIVE | CS_TOUCHING | CS_ENABLED)) {
-      const int2 pr = d.c_pair[cb + sl];
-      int x = d.f_info[f0 + pr.x].x;
-      if (bst[x] & 1) x = d.f_info[f0 + pr.y].x;
-      if (seed[parent[x]] != kIntMax) atomicAdd(&cntc[parent[x]], 1);
-    }
+  for (int base = tid; base < used; base += kEndsBatch * kBigBlock) {
+    bool ok[kEndsBatch];
+    int la[kEndsBatch], lb[kEndsBatch];
+    touching_ends(d, cb, f0, used, base, ok, la, lb);
+    int x[kEndsBatch];
+#pragma unroll
+    for (int u = 0; u < kEndsBatch; ++u) x[u] = ok[u] && (bst[la[u]] & 1) ? lb[u] : la[u];
+    int px[kEndsBatch];
+#pragma unroll
+    for (int u = 0; u < kEndsBatch; ++u) px[u] = ok[u] ? parent[x[u]] : 0;
+    int sd[kEndsBatch];
+#pragma unroll
+    for (int u = 0; u < kEndsBatch; ++u) sd[u] = ok[u] ? seed[px[u]] : kIntMax;
+#pragma unroll
+    for (int u = 0; u < kEndsBatch; ++u)
+      if (ok[u] && sd[u] != kIntMax) atomicAdd(&cntc[px[u]], 1);
   }
   __syncthreads();
   unsigned long long* alloc = (unsigned long long*)(d.g_wmisc + 16 * w);
@@ -2381,38 +2455,47 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
       if (nbi >= cap_b) { bad = true; break; }
       bidx[b] = bstart + nbi;  // position in the world's island-body list, like the LDS search
       isl_b[bstart + nbi++] = b;
-      for (int e = off[b]; e < off[b + 1]; ++e) {
-        const int2 a = adj[e];
-        if (cflag[a.x]) continue;
+      // The expanding body's counters stay in registers (no other lane touches this island);
+      // cur = rank | first level << 16.  The next entry is read one step ahead, and an entry's
+      // contact flag and other body's state are read together by relaxed atomic loads, which
+      // the compiler keeps ahead of the test.
+      int cur_b = cur[b], lev_b = lev[b];
+      const int eend = off[b + 1];
+      int2 a = off[b] < eend ? adj[off[b]] : make_int2(0, 0);
+      for (int e = off[b]; e < eend; ++e) {
+        const int2 an = e + 1 < eend ? adj[e + 1] : make_int2(0, 0);
+        const int sl_ = a.x, o = a.y;
+        a = an;
+        const int fw = g_ld(&cflag[sl_]);
+        const int so = g_ld(&bst[o]);
+        const int co = g_ld(&cur[o]);
+        const int lo_read = g_ld(&lev[o]);
+        if (fw) continue;
         if (nci >= cap_c || sp + nstat + 2 > cap_b + 2 * cap_c) { bad = true; break; }
-        cflag[a.x] = 1;
-        const int o = a.y;
-        const bool o_dyn = !(bst[o] & 1);
-        // cur = rank | first level << 16
-        const int lo = o_dyn ? lev[o] : 0;
-        const int lvl = 1 + (lev[b] > lo ? lev[b] : lo);
-        const int cb_ = cur[b];
-        const uint32_t rank_b = (uint32_t)(cb_ & 0xFFFF);
-        const int fb = rank_b == 0 ? lvl : (int)((uint32_t)cb_ >> 16);
-        cur[b] = (int)(rank_b + 1) | fb << 16;
+        cflag[sl_] = 1;
+        const bool o_dyn = !(so & 1);
+        const int lo = o_dyn ? lo_read : 0;
+        const int lvl = 1 + (lev_b > lo ? lev_b : lo);
+        const uint32_t rank_b = (uint32_t)(cur_b & 0xFFFF);
+        const int fb = rank_b == 0 ? lvl : (int)((uint32_t)cur_b >> 16);
+        cur_b = (int)(rank_b + 1) | fb << 16;
         span = max(span, lvl - fb);
         uint32_t rank_o = 0u;
         if (o_dyn) {
-          const int co = cur[o];
           rank_o = (uint32_t)(co & 0xFFFF);
           const int fo = rank_o == 0 ? lvl : (int)((uint32_t)co >> 16);
           cur[o] = (int)(rank_o + 1) | fo << 16;
           span = max(span, lvl - fo);
+          lev[o] = lvl;
         }
         depth = max(depth, lvl);
-        lev[b] = lvl;
-        if (o_dyn) lev[o] = lvl;
+        lev_b = lvl;
         d.i_rank[cb + cstart + nci] = make_uint2((uint32_t)b, rank_b | rank_o << 16);
         d.i_order[cb + cstart + nci] = (uint32_t)(lvl - 1);  // level, until sorted below
-        isl_c[cstart + nci++] = a.x;
+        isl_c[cstart + nci++] = sl_;
         if (o_dyn) {
-          if (bst[o] & 16) continue;
-          bst[o] |= 16;
+          if (so & 16) continue;
+          bst[o] = so | 16;
         } else {
           bool seen = false;
           for (int q = 1; q <= nstat; ++q) seen = seen || statics[-q] == o;
@@ -2421,6 +2504,8 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
         }
         stack[sp++] = o;
       }
+      cur[b] = cur_b;
+      lev[b] = lev_b;
     }
     if (bad || nbi != cap_b || nci != cap_c) {  // the component sizes and the search disagree
       atomicOr(&d.w_err[w], WERR_SCHEDULE);
@@ -2856,7 +2941,7 @@ struct b2g_ctx {
   // sizes
   int nw = 0, nb = 0, nf = 0, ncap = 0;
   int max_nb = 0, max_nf = 0, max_ccap = 0;
-  size_t smem_islands = 0, smem_pairs = 0;
+  size_t smem_islands = 0, smem_pairs = 0, smem_big = 0;
   bool pending_new_fixture = false;
   // host mirrors of constant per-body / per-world data
   std::vector<int> body_off, fix_off, con_base, con_cap;
@@ -2979,12 +3064,14 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   std::vector<int> w_big(nw, 0);
   ctx->smem_islands = 0;
   ctx->smem_pairs = 0;
+  ctx->smem_big = 0;
   for (int w = 0; w < nw; ++w) {
     const int wnb = ctx->body_off[w + 1] - ctx->body_off[w], wnf = ctx->fix_off[w + 1] - ctx->fix_off[w];
     const size_t si = island_smem_bytes(wnb, wnf, ctx->con_cap[w]), sp = pairs_smem_bytes(wnf, ctx->con_cap[w]);
     if (ctx->large_worlds || ctx->con_cap[w] >= 65535 || si > (size_t)kMaxLds || sp > (size_t)kMaxLds) {
       w_big[w] = 1;
       ctx->big.push_back(w);
+      ctx->smem_big = std::max(ctx->smem_big, (size_t)((wnb + 1) / 2) * 4);  // 16-bit union-find parents
     } else {
       ctx->smem_islands = std::max(ctx->smem_islands, si);
       ctx->smem_pairs = std::max(ctx->smem_pairs, sp);
@@ -2996,6 +3083,8 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
                               (int)ctx->smem_islands));
   HIPCHK(hipFuncSetAttribute((const void*)k_pairs<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)ctx->smem_pairs));
+  HIPCHK(hipFuncSetAttribute((const void*)k_islands_big<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)ctx->smem_big));
   Dev<T>* dp = new Dev<T>();
   ctx->dev = dp;
   Dev<T>& d = *dp;
@@ -3367,7 +3456,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   if (timed) HIPCHK(hipEventRecord(ev[1], st));
   if (dt > 0.0) {
     hipLaunchKernelGGL(k_islands<T>, dim3(nw), dim3(kBlock), ctx->smem_islands, st, d);
-    if (nbig > 0) hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), 0, st, d);
+    if (nbig > 0) hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), ctx->smem_big, st, d);
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[2], st));
