@@ -2229,25 +2229,25 @@ __device__ void ufl_unite(uint32_t* P, int a, int b) {
   }
 }
 
-// The touching contacts among kEndsBatch slots base, base + kBigBlock, ... of a large world, with
+// The touching contacts among kEndsBatch slots base, base + stride, ... of a large world, with
 // their bodies (world-local).  The three dependent global reads (state, fixture pair, fixture
 // bodies) are issued for all slots of the batch before any is waited for: one lane of the
 // single workgroup otherwise waits out each chain in turn.
 constexpr int kEndsBatch = 4;
 template <typename T>
 __device__ __forceinline__ void touching_ends(const Dev<T>& d, int cb, int f0, int used, int base, bool* ok,
-                                              int* la, int* lb) {
+                                              int* la, int* lb, int stride = kBigBlock) {
   int st[kEndsBatch];
 #pragma unroll
   for (int u = 0; u < kEndsBatch; ++u) {
-    const int sl = base + u * kBigBlock;
+    const int sl = base + u * stride;
     st[u] = sl < used ? d.c_state[cb + sl] : 0;
   }
   int2 pr[kEndsBatch];
 #pragma unroll
   for (int u = 0; u < kEndsBatch; ++u) {
     ok[u] = (st[u] & (CS_LIVE | CS_TOUCHING | CS_ENABLED)) == (CS_LIVE | CS_TOUCHING | CS_ENABLED);
-    pr[u] = ok[u] ? d.c_pair[cb + base + u * kBigBlock] : make_int2(0, 0);
+    pr[u] = ok[u] ? d.c_pair[cb + base + u * stride] : make_int2(0, 0);
   }
 #pragma unroll
   for (int u = 0; u < kEndsBatch; ++u) {
@@ -2396,30 +2396,68 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
     if (!(bst[i] & 1)) parent[i] = wake[i];
   __syncthreads();
   // --- islands: components with an awake, active, non-static body; seed = the first such body ---
-  for (int i = tid; i < nb; i += kBigBlock) {
-    const int bs = bst[i];
-    if ((bs & 1) || !(bs & 2) || !(bs & 8)) continue;
-    atomicMin(&seed[parent[i]], i);
+  // Each thread takes a contiguous run of bodies (then of slots) and adds to a component's
+  // seed and counts once per run of equal roots: the bodies and contacts of one component
+  // mostly sit together, and one atomic per body on a few hot roots serialises in L2.
+  const int bper = (nb + kBigBlock - 1) / kBigBlock, bbeg = tid * bper, bend = min(nb, bbeg + bper);
+  {
+    int root = -1, first = kIntMax;
+    for (int i = bbeg; i < bend; ++i) {
+      const int bs = bst[i];
+      if ((bs & 1) || !(bs & 2) || !(bs & 8)) continue;
+      const int r = parent[i];
+      if (r != root) {
+        if (root >= 0) atomicMin(&seed[root], first);
+        root = r;
+        first = i;
+      }
+    }
+    if (root >= 0) atomicMin(&seed[root], first);
   }
   __syncthreads();
-  for (int i = tid; i < nb; i += kBigBlock)
-    if (!(bst[i] & 1) && seed[parent[i]] != kIntMax) atomicAdd(&cntb[parent[i]], 1);
-  for (int base = tid; base < used; base += kEndsBatch * kBigBlock) {
-    bool ok[kEndsBatch];
-    int la[kEndsBatch], lb[kEndsBatch];
-    touching_ends(d, cb, f0, used, base, ok, la, lb);
-    int x[kEndsBatch];
+  {
+    int root = -1, n = 0;
+    for (int i = bbeg; i < bend; ++i) {
+      if (bst[i] & 1) continue;
+      const int r = parent[i];
+      if (r != root) {
+        if (root >= 0 && n > 0) atomicAdd(&cntb[root], n);
+        root = r;
+        n = seed[r] != kIntMax ? 1 : 0;
+      } else if (n > 0) {
+        ++n;
+      }
+    }
+    if (root >= 0 && n > 0) atomicAdd(&cntb[root], n);
+  }
+  {
+    const int sper = (used + kBigBlock - 1) / kBigBlock, sbeg = tid * sper, send = min(used, sbeg + sper);
+    int root = -1, n = 0;
+    for (int base = sbeg; base < send; base += kEndsBatch) {
+      bool ok[kEndsBatch];
+      int la[kEndsBatch], lb[kEndsBatch];
+      touching_ends(d, cb, f0, send, base, ok, la, lb, 1);
+      int x[kEndsBatch];
 #pragma unroll
-    for (int u = 0; u < kEndsBatch; ++u) x[u] = ok[u] && (bst[la[u]] & 1) ? lb[u] : la[u];
-    int px[kEndsBatch];
+      for (int u = 0; u < kEndsBatch; ++u) x[u] = ok[u] && (bst[la[u]] & 1) ? lb[u] : la[u];
+      int px[kEndsBatch];
 #pragma unroll
-    for (int u = 0; u < kEndsBatch; ++u) px[u] = ok[u] ? parent[x[u]] : 0;
-    int sd[kEndsBatch];
+      for (int u = 0; u < kEndsBatch; ++u) px[u] = ok[u] ? parent[x[u]] : 0;
+      int sd[kEndsBatch];
 #pragma unroll
-    for (int u = 0; u < kEndsBatch; ++u) sd[u] = ok[u] ? seed[px[u]] : kIntMax;
+      for (int u = 0; u < kEndsBatch; ++u) sd[u] = ok[u] ? seed[px[u]] : kIntMax;
 #pragma unroll
-    for (int u = 0; u < kEndsBatch; ++u)
-      if (ok[u] && sd[u] != kIntMax) atomicAdd(&cntc[px[u]], 1);
+      for (int u = 0; u < kEndsBatch; ++u) {
+        if (!ok[u] || sd[u] == kIntMax) continue;
+        if (px[u] != root) {
+          if (root >= 0) atomicAdd(&cntc[root], n);
+          root = px[u];
+          n = 0;
+        }
+        ++n;
+      }
+    }
+    if (root >= 0 && n > 0) atomicAdd(&cntc[root], n);
   }
   __syncthreads();
   unsigned long long* alloc = (unsigned long long*)(d.g_wmisc + 16 * w);
