@@ -392,6 +392,26 @@ def test_single_large_world_200_pyramids():
         assert be[q] <= 1e-5, (q, be[q])
 
 
+def test_single_world_with_body_indices_above_32767():
+    """One world of 640 pyramids (35 840 bodies, about 64 000 contacts): the large-world path's
+    16-bit union-find parents and packed fixture/body indices run above 32767. FP64 tracks the
+    oracle for 20 steps."""
+    batch = builder.scene_batch(scenes.pyramids(640))
+    assert batch.body_off[1] > 32768
+    o = O.Oracle(batch)
+    e = N.Engine("fp64")
+    e.upload(batch)
+    for s in range(20):
+        o.step(DT, 8, 3)
+        e.step(DT, 8, 3)
+    got, ref = e.download(), o.state()
+    e.close()
+    assert PU.same_contact_lists(got, ref)
+    be = PU.body_errors(got, ref)
+    print("640 pyramids fp64, 20 steps", be, flush=True)
+    assert max(be[q] for q in ("position", "angle", "velocity", "angular_velocity")) <= 1e-9
+
+
 def test_island_times_report_every_island_and_change_nothing():
     counts = [420, 300, 512, 180, 96, 260]
     batch = builder.scenes_batch([scenes.batch_world(9, w, n) for w, n in enumerate(counts)])
