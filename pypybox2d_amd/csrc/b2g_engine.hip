@@ -178,6 +178,7 @@ struct Dev {
   int* w_move_n;
   int* w_asleep;
   int* w_nisl;
+  int2* w_itot;        // per world: bodies and contacts of its islands (the island lists' lengths)
   // bodies
   int* b_type;
   int* b_flags;
@@ -222,6 +223,8 @@ struct Dev {
   int* i_bodies;     // per-world segment at w_body_off[w]
   int* i_contacts;   // per-world segment at w_con_base[w]
   int2* i_cidx;      // per-world segment at w_con_base[w]
+  int* i_isb;        // per-world segment at w_con_base[w]: the world-local start of the contact's
+                     // island in the island-body list
   uint2* i_fb;       // per-world segment at w_con_base[w]: the contact's fixtures (x = A | B << 16) and
                      // bodies (y = A | B << 16), world-local, so the solver reads them in one step
   uint2* i_rank;     // per-world segment at w_con_base[w]: the contact's rank in body A's and body B's
@@ -450,6 +453,7 @@ struct IslandSmem {
   uint16_t* stack;  // nb
   uint16_t* slist;  // nb
   uint16_t* lev;    // nb: dependency level of the body's last contact in the island list
+  uint16_t* ibs;    // nb: start of the body's island in the island-body list
   uint32_t* cflag;  // ccap / 32
   int* scan;       // kBlock
   int* misc;       // 8
@@ -482,6 +486,7 @@ __host__ __device__ inline size_t island_smem_bytes(int nb, int nf, int ccap) {
   s += ((size_t)nb * 2 + 15) & ~(size_t)15;
   s += ((size_t)nb * 2 + 15) & ~(size_t)15;
   s += ((size_t)nb * 2 + 15) & ~(size_t)15;  // lev
+  s += ((size_t)nb * 2 + 15) & ~(size_t)15;  // ibs
   s += ((size_t)(ccap + 31) / 32 * 4 + 15) & ~(size_t)15;
   s += (size_t)kBlock * 4;
   s += 64 * 4;  // misc: 16 scalars, 16 class counts, spare
@@ -538,6 +543,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     L.stack = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
     L.slist = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
     L.lev = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
+    L.ibs = (uint16_t*)p; p += ((size_t)nb * 2 + 15) & ~(size_t)15;
     L.cflag = (uint32_t*)p; p += ((size_t)(ccap + 31) / 32 * 4 + 15) & ~(size_t)15;
     L.scan = (int*)p; p += (size_t)kBlock * 4;
     L.misc = (int*)p;
@@ -633,6 +639,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     if (tid == 0) {
       atomicOr(&d.w_err[w], WERR_CAPACITY);
       d.w_nisl[w] = 0;
+      d.w_itot[w] = make_int2(0, 0);
     }
     return;
   }
@@ -694,6 +701,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
           continue;
         }
         L.bidx[b] = (uint16_t)nbi_tot;
+        L.ibs[b] = (uint16_t)bstart;
         isl_b[nbi_tot++] = b;
         // the expanding body's counters stay in registers; cur = rank | first level << 16
         int cur_b = L.cur[b], lev_b = L.lev[b];
@@ -770,6 +778,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     int ib = (L.bst[lb] & 1) ? -1 - lb : (int)L.bidx[lb];
     d.i_cidx[cb + k] = make_int2(ia, ib);
     d.i_fb[cb + k] = make_uint2((uint32_t)pr.x | (uint32_t)pr.y << 16, (uint32_t)la | (uint32_t)lb << 16);
+    d.i_isb[cb + k] = (int)L.ibs[ia >= 0 ? la : lb];
     // (expanding body, its rank | the other body's rank << 16) -> ranks and degrees of (A, B);
     // a dynamic body's rank counter ends at its number of island contacts
     const uint2 r = d.i_rank[cb + k];
@@ -805,7 +814,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     }
     d.b_island[b0 + i] = ((bs & 16) && !(bs & 1)) ? 1 : -1;
   }
-  if (tid == 0) d.w_nisl[w] = nisl;
+  if (tid == 0) {
+    d.w_nisl[w] = nisl;
+    d.w_itot[w] = make_int2(L.misc[1], nci_tot);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1465,11 +1477,7 @@ __device__ __forceinline__ void sleep_last(const Dev<T>& d, const IslandK& c, in
 template <typename T>
 __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci, int last_allow) {
   const IslandK c = island_k(d, w, bstart);
-  for (int i = 0; i < nbi; ++i) d.pv[bstart + i] = integrate_velocity(d, c, bstart, i);
-  for (int k = 0; k < nci; ++k) {
-    int ia, ib;
-    init_contact(d, c, bstart, cstart, k, ia, ib);
-  }
+  // 1 + 2. velocities integrated and constraint records built by k_prep_bodies / k_prep_contacts
   // 3. warm_start (contact.py:498-525) and 4. velocity iterations (contact.py:527-799)
   if (c.warm) velocity_sweep<T, true>(d, bstart, cstart, nci);
   for (int it = 0; it < d.vel_iters; ++it) velocity_sweep<T, false>(d, bstart, cstart, nci);
@@ -1551,24 +1559,14 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
   const IslandK c = island_k(d, w, bstart);
   double* sx = (double*)lds;        // [3 * nbi]
   int* cnt = (int*)(sx + 3 * nbi);  // [nbi]
-  // 1. integrate velocities
+  // 1 + 2. the integrated velocities and the constraint records come from k_prep_bodies and
+  // k_prep_contacts; the velocities go to LDS
   for (int i = t; i < nbi; i += TS) {
-    PV<T> p = integrate_velocity(d, c, bstart, i);
-    d.pv[bstart + i] = p;
+    const PV<T> p = d.pv[bstart + i];
     sx[3 * i] = p.vx;
     sx[3 * i + 1] = p.vy;
     sx[3 * i + 2] = p.w;
     cnt[i] = 0;
-  }
-  team_sync();
-  // 2. constraint records, with each contact's rank in its bodies' sequences and their
-  // degrees (from the island search): rank | degree << 16
-  for (int k = t; k < nci; k += TS) {
-    const uint2 r = d.i_rank[cstart + k];
-    const int rka = (int)((r.x & 0xFFFFu) | (r.y & 0xFFFFu) << 16);
-    const int rkb = (int)((r.x >> 16) | (r.y >> 16) << 16);
-    int ia, ib;
-    init_contact(d, c, bstart, cstart, k, ia, ib, rka, rkb);
   }
   team_sync();
   // 3 + 4. warm start and velocity sweeps as one dataflow sequence
@@ -1742,6 +1740,34 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
       sleep_last(d, c, bstart, nbi);
   }
   team_sync();  // the next islands reuse the LDS
+}
+
+// 1. integrate velocities (island.py:71-95) of every island body, and 2. the constraint records
+// of every island contact (contact.py:376-496), as flat launches ahead of k_solve: the work
+// has no dependencies inside an island, so it need not sit at the head of each island's
+// solve.  Grid: (chunks of 256) x worlds; w_itot bounds each world's island lists.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_prep_bodies(Dev<T> d, int chunks) {
+  const int w = blockIdx.x / chunks, i = (blockIdx.x % chunks) * kBlock + threadIdx.x;
+  if (i >= d.w_itot[w].x) return;
+  const int b0 = d.w_body_off[w];
+  const IslandK c = island_k(d, w, b0);
+  d.pv[b0 + i] = integrate_velocity(d, c, b0, i);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_prep_contacts(Dev<T> d, int chunks) {
+  const int w = blockIdx.x / chunks, k = (blockIdx.x % chunks) * kBlock + threadIdx.x;
+  if (k >= d.w_itot[w].y) return;
+  const int b0 = d.w_body_off[w], cb = d.w_con_base[w];
+  const int bstart = b0 + d.i_isb[cb + k];  // the contact's island
+  const IslandK c = island_k(d, w, bstart);
+  // each contact's rank in its bodies' sequences and their degrees: rank | degree << 16
+  const uint2 r = d.i_rank[cb + k];
+  const int rka = (int)((r.x & 0xFFFFu) | (r.y & 0xFFFFu) << 16);
+  const int rkb = (int)((r.x >> 16) | (r.y >> 16) << 16);
+  int ia, ib;
+  init_contact(d, c, bstart, cb, k, ia, ib, rka, rkb);
 }
 
 // Island timing (b2g_island_times): wall-clock ticks around one island's solve, its size and
@@ -2492,6 +2518,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
       if (bs & 1) continue;  // static bodies join but do not propagate
       if (nbi >= cap_b) { bad = true; break; }
       bidx[b] = bstart + nbi;  // position in the world's island-body list, like the LDS search
+      wake[b] = bstart;        // the start of its island (the wake slots are free again)
       isl_b[bstart + nbi++] = b;
       // The expanding body's counters stay in registers (no other lane touches this island);
       // cur = rank | first level << 16.  The next entry is read one step ahead, and an entry's
@@ -2548,6 +2575,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
     if (bad || nbi != cap_b || nci != cap_c) {  // the component sizes and the search disagree
       atomicOr(&d.w_err[w], WERR_SCHEDULE);
       for (int q = nci; q < cap_c; ++q) isl_c[cstart + q] = 0;  // keep later passes in range
+      for (int q = nbi; q < cap_b; ++q) isl_b[bstart + q] = s0;
       nbi = nci = 0;
     }
     d.i_info[b0 + k] = make_int4(bstart, nbi | sweep_period(span, depth) << 16, cstart, nci | (last_allow << 30));
@@ -2587,6 +2615,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
     const int ib = (bst[lb] & 1) ? -1 - lb : bidx[lb];
     d.i_cidx[cb + k] = make_int2(ia, ib);
     d.i_fb[cb + k] = make_uint2((uint32_t)pr.x | (uint32_t)pr.y << 16, (uint32_t)la | (uint32_t)lb << 16);
+    d.i_isb[cb + k] = wake[ia >= 0 ? la : lb];
     const uint2 r = d.i_rank[cb + k];
     const uint32_t lo = r.y & 0xFFFFu, hi = r.y >> 16;
     const uint32_t dga = ia >= 0 ? (uint32_t)cur[la] & 0xFFFFu : 0u;  // rank counters end at the degrees
@@ -2605,8 +2634,10 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
     }
     d.b_island[b0 + i] = ((bs & 16) && !(bs & 1)) ? 1 : -1;
   }
-  if (tid == 0) d.w_nisl[w] = nisl;
-  (void)nbi_tot;
+  if (tid == 0) {
+    d.w_nisl[w] = nisl;
+    d.w_itot[w] = make_int2(nbi_tot, nci_tot);
+  }
   (void)ccap;
 }
 
@@ -3301,6 +3332,7 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.w_move_n, nw));
   TRY(dalloc(ctx, &d.w_asleep, nw));
   TRY(dalloc(ctx, &d.w_nisl, nw));
+  TRY(dalloc(ctx, &d.w_itot, nw));
   TRY(dalloc(ctx, &d.b_type, nb));
   TRY(dalloc(ctx, &d.b_flags, nb));
   TRY(dalloc(ctx, &d.b_island, nb));
@@ -3340,6 +3372,7 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.i_contacts, ncap));
   TRY(dalloc(ctx, &d.i_cidx, ncap));
   TRY(dalloc(ctx, &d.i_fb, ncap));
+  TRY(dalloc(ctx, &d.i_isb, ncap));
   TRY(dalloc(ctx, &d.i_rank, ncap));
   TRY(dalloc(ctx, &d.i_order, ncap));
   TRY(dalloc(ctx, &d.i_info, nb));
@@ -3498,6 +3531,11 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[2], st));
+    // the island solve: per-body and per-contact setup over all islands, then the islands
+    const int bch = std::max(1, (ctx->max_nb + kBlock - 1) / kBlock);
+    const int cch = std::max(1, (ctx->max_ccap + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_prep_bodies<T>, dim3((unsigned)(bch * nw)), dim3(kBlock), 0, st, d, bch);
+    hipLaunchKernelGGL(k_prep_contacts<T>, dim3((unsigned)(cch * nw)), dim3(kBlock), 0, st, d, cch);
     const int swaves = std::max(1, std::min((ctx->nb + kSolveBlock - 1) / kSolveBlock, ctx->solve_grid));
     hipLaunchKernelGGL(k_solve<T>, dim3(swaves), dim3(kSolveBlock), ctx->team_lds, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[3], st));
