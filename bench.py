@@ -295,23 +295,34 @@ def cpu_baseline(batch: L.BatchArrays, steps: int, seconds: float, procs: int):
                       % (procs, procs, steps, seconds, nw)}
 
 
+# the kernels of each timed stage (the stage names are those of the stage's main kernel)
+STAGE_KERNELS = {"k_collide": ("k_reset_queue", "k_collide"), "k_islands": ("k_islands", "k_order_islands"),
+                 "k_solve": ("k_prep_bodies", "k_prep_contacts", "k_solve"), "k_sync": ("k_sync",),
+                 "k_pairs": ("k_pairs",)}
+
+
 def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    of the default workload (FP32 pipeline), or None."""
+    """HBM bytes per launch of a stage (the sum over its kernels) from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of the default workload (FP32 pipeline), or None."""
     path = os.path.join(HERE, "profiles", "traffic_mixed4096_fp32.md")
     if workload != "mixed" or not os.path.exists(path):
         return None
-    key = "traffic_bytes_per_launch %s:" % kernel
+    per = {}
     for line in open(path):
-        if line.startswith(key):
-            return float(line.split(":", 1)[1])
-    return None
+        if line.startswith("traffic_bytes_per_launch "):
+            name, val = line[len("traffic_bytes_per_launch "):].split(":", 1)
+            per[name.strip()] = float(val)
+    parts = STAGE_KERNELS.get(kernel, (kernel,))
+    if kernel not in per:
+        return None
+    return sum(per.get(k, 0.0) for k in parts)
 
 
 def roofline(kernel: str, ms: float, counts: dict, fp64: bool, workload: str = "mixed") -> dict:
     ab = algorithmic_bytes(kernel, counts, fp64)
     achieved = ab / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    out = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    out = {"bound": "hbm", "kernel": kernel, "kernels": list(STAGE_KERNELS.get(kernel, (kernel,))),
+           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, workload), "algorithmic_bytes": ab,
            "avg_ms": ms}
     if kernel in LATENCY_LIMITS:
