@@ -131,8 +131,9 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
 
 /* Run n eager steps with HIP events between the step's kernels and write the
    average milliseconds per stage: ms_out[0..4] = collide (with the queue
-   reset), islands (island search and the size-ordered queue), solve, sync,
-   pairs.  The events are read back once per 128 steps. */
+   reset), islands (island search and the size-ordered queue), solve (the
+   per-body and per-contact setup kernels and the island solve), sync, pairs.
+   The events are read back once per 128 steps. */
 B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, float* ms_out);
 
 /* With B2G_PARAM_ISLAND_TIMES on: the islands the last step solved, in the

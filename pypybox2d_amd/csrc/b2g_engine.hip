@@ -2207,9 +2207,6 @@ constexpr int kIntMax = 0x7fffffff;
 __device__ __forceinline__ int g_ld(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void g_st(int* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 // Union-find over a large world's bodies (lock-free hooking of the larger root under the
 // smaller, path halving), on 16-bit parents packed two to a word in LDS (a world has fewer
 // than 65535 bodies, so any world's parents fit 128 KB): a half is replaced by a
