@@ -922,6 +922,38 @@ __device__ V2<T> world_manifold(int type, int pc, V2<T> ln, V2<T> lp, V2<T> p0, 
   return normal;
 }
 
+// The FP32 pipeline evaluates the velocity constraints with fused multiply-adds (one rounding
+// per product-and-sum instead of two, and about 40 % fewer FP64 instructions per solve); its
+// results are held to the FP64 oracle within the FP32 tolerance.  The FP64 pipeline keeps the
+// reference's separate products and sums, operation by operation, and so stays exact.
+template <typename T>
+struct FusedSolve {
+  static constexpr bool value = false;
+};
+template <>
+struct FusedSolve<float> {
+  static constexpr bool value = true;
+};
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// relative velocity of the contact point (dv = vb + wb x rb - va - wa x ra), fused
+__device__ __forceinline__ void rel_vel_fused(V2d va, double wa, V2d vb, double wb, double rax, double ray,
+                                              double rbx, double rby, double& dvx, double& dvy) {
+  dvx = fmad(-wb, rby, vb.x) - fmad(-wa, ray, va.x);
+  dvy = fmad(wb, rbx, vb.y) - fmad(wa, rax, va.y);
+}
+// applies impulse P at lever arms ra, rb (va -= ma P, wa -= Ia ra x P, vb += mb P, wb += Ib rb x P), fused
+__device__ __forceinline__ void apply_fused(double ma, double mb, double Ia, double Ib, double rax, double ray,
+                                            double rbx, double rby, double Px, double Py, V2d& va, double& wa,
+                                            V2d& vb, double& wb) {
+  va.x = fmad(-ma, Px, va.x);
+  va.y = fmad(-ma, Py, va.y);
+  wa = fmad(-Ia, fmad(rax, Py, -ray * Px), wa);
+  vb.x = fmad(mb, Px, vb.x);
+  vb.y = fmad(mb, Py, vb.y);
+  wb = fmad(Ib, fmad(rbx, Py, -rby * Px), wb);
+}
+
 template <typename T>
 __device__ __forceinline__ void apply_block(VC<T>& vc, V2d x, V2d a, V2d normal, V2d& va, double& wa, V2d& vb,
                                             double& wb) {
@@ -942,7 +974,18 @@ template <typename T>
 __device__ __forceinline__ void tangent_point(const VC<T>& vc, VCP<T>& p, V2d tangent, V2d& va, double& wa, V2d& vb,
                                               double& wb) {
   const double ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
-  {
+  if constexpr (FusedSolve<T>::value) {
+    const double rax = p.rax, ray = p.ray, rbx = p.rbx, rby = p.rby;
+    double dvx, dvy;
+    rel_vel_fused(va, wa, vb, wb, rax, ray, rbx, rby, dvx, dvy);
+    const double vt = fmad(dvx, tangent.x, dvy * tangent.y);
+    double lambda = (double)p.tm * (-vt);
+    const double maxf = (double)vc.friction * p.ni;
+    const double newi = pclamp(p.ti + lambda, -maxf, maxf);
+    lambda = newi - p.ti;
+    apply_fused(ma, mb, Ia, Ib, rax, ray, rbx, rby, lambda * tangent.x, lambda * tangent.y, va, wa, vb, wb);
+    p.ti = newi;
+  } else {
     V2d ra = dv2(p.rax, p.ray), rb = dv2(p.rbx, p.rby);
     V2d dv = vb + cross_sv(wb, rb) - va - cross_sv(wa, ra);
     double vt = dot(dv, tangent);
@@ -966,7 +1009,53 @@ __device__ __forceinline__ void solve_contact_velocity(VC<T>& vc, V2d& va, doubl
   // tangent constraints first (points in order), then normal
   tangent_point(vc, vc.p[0], tangent, va, wa, vb, wb);
   if (vc.npts == 2) tangent_point(vc, vc.p[1], tangent, va, wa, vb, wb);
-  if (vc.npts == 1) {
+  if constexpr (FusedSolve<T>::value) {
+    if (vc.npts == 1) {
+      VCP<T>& p = vc.p[0];
+      const double rax = p.rax, ray = p.ray, rbx = p.rbx, rby = p.rby;
+      double dvx, dvy;
+      rel_vel_fused(va, wa, vb, wb, rax, ray, rbx, rby, dvx, dvy);
+      const double vn = fmad(dvx, normal.x, dvy * normal.y);
+      double lambda = -(double)p.nm * (vn - (double)p.bias);
+      const double newi = pmax(p.ni + lambda, 0.0);
+      lambda = newi - p.ni;
+      apply_fused(ma, mb, Ia, Ib, rax, ray, rbx, rby, lambda * normal.x, lambda * normal.y, va, wa, vb, wb);
+      p.ni = newi;
+    } else if (vc.npts == 2) {  // block solver, total enumeration (contact.py:600-797)
+      VCP<T>&cp1 = vc.p[0], &cp2 = vc.p[1];
+      const double ax = cp1.ni, ay = cp2.ni;
+      const double ra1x = cp1.rax, ra1y = cp1.ray, rb1x = cp1.rbx, rb1y = cp1.rby;
+      const double ra2x = cp2.rax, ra2y = cp2.ray, rb2x = cp2.rbx, rb2y = cp2.rby;
+      double d1x, d1y, d2x, d2y;
+      rel_vel_fused(va, wa, vb, wb, ra1x, ra1y, rb1x, rb1y, d1x, d1y);
+      rel_vel_fused(va, wa, vb, wb, ra2x, ra2y, rb2x, rb2y, d2x, d2y);
+      const double vn1 = fmad(d1x, normal.x, d1y * normal.y), vn2 = fmad(d2x, normal.x, d2y * normal.y);
+      // b = vn - bias - K a, K = [[k11, k12], [k12, k22]]
+      const double bx = (vn1 - (double)cp1.bias) - fmad(vc.k11, ax, vc.k12 * ay);
+      const double by = (vn2 - (double)cp2.bias) - fmad(vc.k12, ax, vc.k22 * ay);
+      const double x1x = -fmad(vc.n11, bx, vc.n12 * by), x1y = -fmad(vc.n21, bx, vc.n22 * by);  // case 1
+      const double x2x = -(double)cp1.nm * bx, vn2_2 = fmad(vc.k12, x2x, by);                    // case 2
+      const double x3y = -(double)cp2.nm * by, vn1_3 = fmad(vc.k12, x3y, bx);                    // case 3
+      const bool c1 = x1x >= 0.0 && x1y >= 0.0;
+      const bool c2 = x2x >= 0.0 && vn2_2 >= 0.0;
+      const bool c3 = x3y >= 0.0 && vn1_3 >= 0.0;
+      const bool c4 = bx >= 0.0 && by >= 0.0;  // case 4: x = 0
+      const double xx = c1 ? x1x : (c2 ? x2x : 0.0), xy = c1 ? x1y : (c3 && !c2 ? x3y : 0.0);
+      if (c1 || c2 || c3 || c4) {
+        const double dx = xx - ax, dy = xy - ay;
+        const double p1x = dx * normal.x, p1y = dx * normal.y, p2x = dy * normal.x, p2y = dy * normal.y;
+        const double px = p1x + p2x, py = p1y + p2y;
+        va.x = fmad(-ma, px, va.x);
+        va.y = fmad(-ma, py, va.y);
+        wa = fmad(-Ia, fmad(ra1x, p1y, -ra1y * p1x) + fmad(ra2x, p2y, -ra2y * p2x), wa);
+        vb.x = fmad(mb, px, vb.x);
+        vb.y = fmad(mb, py, vb.y);
+        wb = fmad(Ib, fmad(rb1x, p1y, -rb1y * p1x) + fmad(rb2x, p2y, -rb2y * p2x), wb);
+        cp1.ni = xx;
+        cp2.ni = xy;
+      }
+    }
+  } else if (vc.npts == 1) {
     VCP<T>& p = vc.p[0];
     V2d ra = dv2(p.rax, p.ray), rb = dv2(p.rbx, p.rby);
     V2d dv = vb + cross_sv(wb, rb) - va - cross_sv(wa, ra);
@@ -1032,6 +1121,11 @@ __device__ __forceinline__ void vel_of(const Dev<T>& d, int bstart, int i, V2d& 
 template <typename T>
 __device__ __forceinline__ void warm_point(const VC<T>& vc, const VCP<T>& p, V2d normal, V2d tangent, V2d& va,
                                            double& wa, V2d& vb, double& wb) {
+  if constexpr (FusedSolve<T>::value) {
+    const double Px = fmad(p.ni, normal.x, p.ti * tangent.x), Py = fmad(p.ni, normal.y, p.ti * tangent.y);
+    apply_fused(vc.ma, vc.mb, vc.Ia, vc.Ib, p.rax, p.ray, p.rbx, p.rby, Px, Py, va, wa, vb, wb);
+    return;
+  }
   V2d P = scale(p.ni, normal) + scale(p.ti, tangent);
   wa -= (double)vc.Ia * cross(dv2(p.rax, p.ray), P);
   va = va - scale((double)vc.ma, P);
