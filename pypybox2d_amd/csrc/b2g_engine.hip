@@ -523,9 +523,9 @@ __device__ int block_exclusive_scan(int* a, int n, int* scratch) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_islands(Dev<T> d) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_islands(Dev<T> d, int w_first) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int w = d.w_order[blockIdx.x], tid = threadIdx.x;
+  const int w = d.w_order[w_first + blockIdx.x], tid = threadIdx.x;
   if (d.w_big && d.w_big[w]) return;  // k_islands_big
   const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
   const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
@@ -2093,9 +2093,9 @@ __device__ void pair_candidate(const Dev<T>& d, int w, int f0, int b0, int P, in
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int from_upload, int finish, int do_pairs) {
+__global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int w_first, int from_upload, int finish, int do_pairs) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int w = d.w_order[blockIdx.x], tid = threadIdx.x;  // largest worlds first
+  const int w = d.w_order[w_first + blockIdx.x], tid = threadIdx.x;  // largest worlds first
   if (d.w_big && d.w_big[w]) return;  // k_pairs_big_*
   const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
   const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
@@ -3083,6 +3083,41 @@ using namespace b2g;
     }                                                                                                 \
   } while (0)
 
+// A run of consecutive worlds in launch order (w_order) that one launch of the island or
+// pair search covers, with the dynamic LDS of its largest world.  Splitting the worlds into
+// runs whose workgroups fit the same number of times into a CU's LDS keeps the many small
+// worlds from being held to the footprint of the few largest; the runs of one search go to
+// separate streams and run side by side.
+struct WorldRun {
+  int first, count;
+  size_t smem;
+};
+constexpr int kMaxRuns = 4;
+constexpr int kIslandsPerCu = 5;  // k_islands workgroups per CU allowed by its registers
+constexpr int kPairsPerCu = 7;    // k_pairs workgroups per CU allowed by its registers
+
+// Runs of `order` (at most kMaxRuns): a run ends where the next world's workgroups would
+// fit more often per CU (`cap` = the kernel's limit from registers) than the run's largest;
+// surplus runs are merged from the front, where the few largest worlds sit.  need[w] == 0
+// marks a world the kernel skips.
+static std::vector<WorldRun> plan_runs(const std::vector<int>& order, const std::vector<size_t>& need, int cap) {
+  auto fits = [&](size_t s) { return s == 0 ? cap : std::min(cap, (int)((size_t)kMaxLds / s)); };
+  std::vector<WorldRun> runs;
+  for (int p = 0; p < (int)order.size(); ++p) {
+    const size_t s = need[order[p]];
+    if (runs.empty() || fits(s) > fits(runs.back().smem)) runs.push_back({p, 0, 0});
+    runs.back().count += 1;
+    runs.back().smem = std::max(runs.back().smem, s);
+  }
+  while ((int)runs.size() > kMaxRuns) {
+    runs[1].first = runs[0].first;
+    runs[1].count += runs[0].count;
+    runs[1].smem = std::max(runs[1].smem, runs[0].smem);
+    runs.erase(runs.begin());
+  }
+  return runs;
+}
+
 struct b2g_ctx {
   int precision = B2G_FP32;
   int device = 0;
@@ -3102,6 +3137,9 @@ struct b2g_ctx {
   int nw = 0, nb = 0, nf = 0, ncap = 0;
   int max_nb = 0, max_nf = 0, max_ccap = 0;
   size_t smem_islands = 0, smem_pairs = 0, smem_big = 0;
+  std::vector<WorldRun> isl_runs, pair_runs;
+  hipStream_t aux[kMaxRuns - 1] = {};  // streams of runs 1.. (run 0 stays on `stream`)
+  hipEvent_t fork_ev = nullptr, join_ev[kMaxRuns - 1] = {};
   bool pending_new_fixture = false;
   // host mirrors of constant per-body / per-world data
   std::vector<int> body_off, fix_off, con_base, con_cap;
@@ -3489,6 +3527,17 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
     TRY(upload(ctx, wo, w_order.data(), nw));
     d.w_order = wo;
   }
+  {
+    std::vector<size_t> ni(nw, 0), np(nw, 0);
+    for (int w = 0; w < nw; ++w) {
+      if (w_big[w]) continue;
+      const int wnb = ctx->body_off[w + 1] - ctx->body_off[w], wnf = ctx->fix_off[w + 1] - ctx->fix_off[w];
+      ni[w] = island_smem_bytes(wnb, wnf, ctx->con_cap[w]);
+      np[w] = pairs_smem_bytes(wnf, ctx->con_cap[w]);
+    }
+    ctx->isl_runs = plan_runs(w_order, ni, kIslandsPerCu);
+    ctx->pair_runs = plan_runs(w_order, np, kPairsPerCu);
+  }
   if (!ctx->big.empty()) {  // scratch of the large-world path
     int *wb = nullptr, *bl = nullptr;
     TRY(dalloc(ctx, &wb, nw));
@@ -3601,15 +3650,32 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   const int nw = ctx->nw;
   hipStream_t st = ctx->stream;
   const int nbig = (int)ctx->big.size();
-  auto pairs = [&](int from_upload, int finish, int do_pairs) {
-    hipLaunchKernelGGL(k_pairs<T>, dim3(nw), dim3(kBlock), ctx->smem_pairs, st, d, from_upload, finish, do_pairs);
+  // the runs of one search side by side: run 0 on the step's stream, the others on the
+  // auxiliary streams, forked from and joined back into the step's stream
+  auto fan = [&](const std::vector<WorldRun>& runs, auto&& launch) -> int {
+    const int n = (int)runs.size();
+    if (n > 1) HIPCHK(hipEventRecord(ctx->fork_ev, st));
+    for (int r = 1; r < n; ++r) HIPCHK(hipStreamWaitEvent(ctx->aux[r - 1], ctx->fork_ev, 0));
+    for (int r = 0; r < n; ++r) launch(runs[r], r == 0 ? st : ctx->aux[r - 1]);
+    for (int r = 1; r < n; ++r) {
+      HIPCHK(hipEventRecord(ctx->join_ev[r - 1], ctx->aux[r - 1]));
+      HIPCHK(hipStreamWaitEvent(st, ctx->join_ev[r - 1], 0));
+    }
+    return 0;
+  };
+  auto pairs = [&](int from_upload, int finish, int do_pairs) -> int {
+    TRY(fan(ctx->pair_runs, [&](const WorldRun& r, hipStream_t s) {
+      hipLaunchKernelGGL(k_pairs<T>, dim3(r.count), dim3(kBlock), r.smem, s, d, r.first, from_upload, finish,
+                         do_pairs);
+    }));
     if (nbig > 0) {
       hipLaunchKernelGGL(k_pairs_big_prep<T>, dim3(nbig), dim3(kBigBlock), 0, st, d, from_upload, do_pairs);
       hipLaunchKernelGGL(k_pairs_big_find<T>, dim3(256, nbig), dim3(kBlock), 0, st, d);
       hipLaunchKernelGGL(k_pairs_big_finish<T>, dim3(nbig), dim3(kBigBlock), 0, st, d, finish);
     }
+    return 0;
   };
-  if (first_pairs) pairs(1, 0, 1);
+  if (first_pairs) TRY(pairs(1, 0, 1));
   if (timed) HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(k_reset_queue<T>, dim3(1), dim3(64), 0, st, d);
   const int chunks = (ctx->max_ccap + kBlock - 1) / kBlock;
@@ -3617,7 +3683,9 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   hipLaunchKernelGGL(k_collide<T>, dim3((unsigned)(((nw + 7) / 8) * 8 * nch)), dim3(kBlock), 0, st, d, nch);
   if (timed) HIPCHK(hipEventRecord(ev[1], st));
   if (dt > 0.0) {
-    hipLaunchKernelGGL(k_islands<T>, dim3(nw), dim3(kBlock), ctx->smem_islands, st, d);
+    TRY(fan(ctx->isl_runs, [&](const WorldRun& r, hipStream_t s) {
+      hipLaunchKernelGGL(k_islands<T>, dim3(r.count), dim3(kBlock), r.smem, s, d, r.first);
+    }));
     if (nbig > 0) hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), ctx->smem_big, st, d);
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
@@ -3632,14 +3700,14 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     if (timed) HIPCHK(hipEventRecord(ev[3], st));
     hipLaunchKernelGGL(k_sync<T>, dim3(std::max(1, (ctx->nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[4], st));
-    pairs(0, 1, 1);
+    TRY(pairs(0, 1, 1));
   } else {
     if (timed) {
       HIPCHK(hipEventRecord(ev[2], st));
       HIPCHK(hipEventRecord(ev[3], st));
       HIPCHK(hipEventRecord(ev[4], st));
     }
-    pairs(0, 1, 0);
+    TRY(pairs(0, 1, 0));
   }
   if (timed) HIPCHK(hipEventRecord(ev[5], st));
   HIPCHK(hipGetLastError());
@@ -3799,6 +3867,15 @@ extern "C" {
 
 B2G_API int b2g_version(void) { return B2G_VERSION; }
 
+static void destroy_streams(b2g_ctx* ctx) {
+  for (int r = 0; r < kMaxRuns - 1; ++r) {
+    if (ctx->join_ev[r]) hipEventDestroy(ctx->join_ev[r]);
+    if (ctx->aux[r]) hipStreamDestroy(ctx->aux[r]);
+  }
+  if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+}
+
 B2G_API int b2g_create(const b2g_desc* desc, b2g_ctx** out) {
   if (!out) return B2G_E_INVALID;
   *out = nullptr;
@@ -3822,7 +3899,13 @@ B2G_API int b2g_create(const b2g_desc* desc, b2g_ctx** out) {
     delete ctx;
     return B2G_E_NODEVICE;
   }
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+  bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) == hipSuccess;
+  for (int r = 0; ok && r < kMaxRuns - 1; ++r)
+    ok = hipStreamCreateWithFlags(&ctx->aux[r], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&ctx->join_ev[r], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    destroy_streams(ctx);
     delete ctx;
     return B2G_E_HIP;
   }
@@ -3838,7 +3921,7 @@ B2G_API int b2g_destroy(b2g_ctx* ctx) {
   free_device(ctx);
   for (auto& e : ctx->pev) hipEventDestroy(e);
   ctx->pev.clear();
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  destroy_streams(ctx);
   delete ctx;
   return B2G_OK;
 }
