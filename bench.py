@@ -302,15 +302,16 @@ STAGE_KERNELS = {"k_collide": ("k_reset_queue", "k_collide"), "k_islands": ("k_i
 
 
 def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of a stage (the sum over its kernels) from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of the default workload (FP32 pipeline), or None."""
+    """HBM bytes per step of a stage (the sum over its kernels and their dispatches) from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the default workload (FP32 pipeline),
+    or None."""
     path = os.path.join(HERE, "profiles", "traffic_mixed4096_fp32.md")
     if workload != "mixed" or not os.path.exists(path):
         return None
     per = {}
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch "):
-            name, val = line[len("traffic_bytes_per_launch "):].split(":", 1)
+        if line.startswith("traffic_bytes_per_step "):
+            name, val = line[len("traffic_bytes_per_step "):].split(":", 1)
             per[name.strip()] = float(val)
     parts = STAGE_KERNELS.get(kernel, (kernel,))
     if kernel not in per:
