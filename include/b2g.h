@@ -117,7 +117,11 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
                                 end time (b2g_island_times); 0 (default): off
      B2G_PARAM_SOLVE_WAVES_PER_CU  read only: solver wavefronts resident per
                                 compute unit under the current launch shape
-                                (the lesser of the VGPR and LDS limits) */
+                                (the lesser of the VGPR and LDS limits)
+     B2G_PARAM_SEARCH_RUNS      read only: launches per step of the island
+                                search (low byte) and of the pair search (next
+                                byte); each covers a run of worlds of similar
+                                LDS footprint, set at b2g_upload, up to 4 */
 #define B2G_PARAM_TEAM_LDS 1
 #define B2G_PARAM_HEAVY_CLASS 2
 #define B2G_PARAM_SOLVE_GRID 3
@@ -126,6 +130,7 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
 #define B2G_PARAM_ISLAND_TIMES 6
 #define B2G_PARAM_SOLVE_WAVES_PER_CU 7
 #define B2G_PARAM_TEAM_MAX 8
+#define B2G_PARAM_SEARCH_RUNS 9
 B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value);
 B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
 

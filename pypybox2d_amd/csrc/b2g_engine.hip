@@ -4050,6 +4050,8 @@ B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value) {
     case B2G_PARAM_TEAM_MAX: slot = &ctx->team_max; lo = 8; hi = kSolveBlock; break;
     case B2G_PARAM_SOLVE_WAVES_PER_CU:
       return set_error(ctx, B2G_E_INVALID, "b2g_set_param: B2G_PARAM_SOLVE_WAVES_PER_CU is read-only");
+    case B2G_PARAM_SEARCH_RUNS:
+      return set_error(ctx, B2G_E_INVALID, "b2g_set_param: B2G_PARAM_SEARCH_RUNS is read-only");
     default: return set_error(ctx, B2G_E_INVALID, "b2g_set_param: unknown key " + std::to_string(key));
   }
   if (key == B2G_PARAM_TEAM_MAX && (value & (value - 1)) != 0)
@@ -4080,6 +4082,7 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value) {
     case B2G_PARAM_SOLVE_GRID: *value = ctx->solve_grid; return B2G_OK;
     case B2G_PARAM_ISLAND_TIMES: *value = ctx->island_times; return B2G_OK;
     case B2G_PARAM_TEAM_MAX: *value = ctx->team_max; return B2G_OK;
+    case B2G_PARAM_SEARCH_RUNS: *value = (int)ctx->isl_runs.size() | (int)ctx->pair_runs.size() << 8; return B2G_OK;
     case B2G_PARAM_SOLVE_WAVES_PER_CU: {
       // resident solver wavefronts per compute unit: 4 SIMDs x (512 VGPRs / VGPRs per lane), and
       // the LDS the launch reserves per one-wavefront workgroup

@@ -244,11 +244,53 @@ def test_island_launch_shapes_agree_bit_exactly(prec):
         assert np.array_equal(st.con_d, ref.con_d) and np.array_equal(st.con_i, ref.con_i), name
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_worlds_of_many_sizes_launch_in_runs(prec):
+    """Worlds of 13 to 1021 bodies: the island and the pair search each launch in four runs
+    (worlds grouped by LDS footprint, side by side on separate streams). One step tracks the
+    oracle, and the runs, eager or graph-captured, give bit for bit what the global-memory
+    search gives."""
+    counts = [1020, 40, 700, 300, 520, 60, 900, 12]
+    batch = builder.scenes_batch([scenes.batch_world(11, w, n) for w, n in enumerate(counts)])
+    e = N.Engine(prec)
+    e.upload(batch)
+    assert e.get_param("search_runs") == 4 | 4 << 8
+    e.step_n(120, DT, 8, 3, graph=True)
+    state = e.download()
+    e.close()
+    state.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    ref = oracle_step(state)
+    got = gpu_step(state, prec)
+    assert PU.same_contact_lists(got, ref)
+    be = PU.body_errors(got, ref)
+    print(prec, "worlds in runs, one step", be, flush=True)
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    for q in ("position", "angle", "velocity", "angular_velocity"):
+        assert be[q] <= tol, (q, be[q])
+    results = {}
+    for name, params, runs in (("runs", {}, 4 | 4 << 8), ("large_worlds", {"large_worlds": 1}, 1 | 1 << 8)):
+        e = N.Engine(prec)
+        for k, v in params.items():
+            e.set_param(k, v)
+        e.upload(state)
+        assert e.get_param("search_runs") == runs
+        for _ in range(3):
+            e.step(DT, 8, 3)
+        e.step_n(3, DT, 8, 3, graph=True)
+        results[name] = e.download()
+        print(prec, name, "done", flush=True)
+        e.close()
+    a, b = results["runs"], results["large_worlds"]
+    assert np.array_equal(a.body_d, b.body_d) and np.array_equal(a.body_i, b.body_i)
+    assert np.array_equal(a.con_d, b.con_d) and np.array_equal(a.con_i, b.con_i)
+
+
 def test_set_param_rejects_bad_values():
     e = N.Engine("fp32")
     assert e.get_param("heavy_class") >= 0
     for k, v in (("heavy_class", 17), ("team_lds", 200000), ("team_lds", 1000), ("solve_grid", 0),
-                 ("team_full_class", -1), ("team_max", 24), ("team_max", 4), ("team_max", 128)):
+                 ("team_full_class", -1), ("team_max", 24), ("team_max", 4), ("team_max", 128),
+                 ("search_runs", 1), ("solve_waves_per_cu", 8)):
         with pytest.raises(N.B2GError):
             e.set_param(k, v)
     e.close()

@@ -34,3 +34,9 @@ def test_create_without_gpu_fails_cleanly_or_succeeds():
         assert lib.b2g_destroy(h) == N.B2G_OK
     bad = N.CDesc(7, 0, 8.0, 64, 0)
     assert lib.b2g_create(ctypes.byref(bad), ctypes.byref(h)) in (N.B2G_E_INVALID, N.B2G_E_NODEVICE)
+
+
+def test_parameter_keys_agree():
+    text = open(os.path.join(REPO, "include", "b2g.h")).read()
+    keys = {k.lower(): int(v) for k, v in re.findall(r"#define B2G_PARAM_(\w+) (\d+)", text)}
+    assert keys == N.PARAMS
