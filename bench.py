@@ -150,12 +150,14 @@ def solver_report(rows: np.ndarray, waves_per_cu: int) -> dict:
     for lanes in sorted(set(rows[:, 4].tolist()), reverse=True):
         m = rows[:, 4] == lanes
         groups[str(int(lanes))] = {"islands": int(m.sum()), "contacts": [int(rows[m, 2].min()), int(rows[m, 2].max())],
+                                   "sweep_period": [int(rows[m, 5].min()), int(rows[m, 5].max())],
                                    "latency_us": {"mean": float(lat[m].mean()), "p50": float(np.median(lat[m])),
                                                   "max": float(lat[m].max())}}
     c = int(np.argmax(end))
     return {"islands": int(rows.shape[0]), "span_us": float(span), "mean_islands_in_flight": float(lat.sum() / span),
             "resident_wavefronts_per_cu": int(waves_per_cu), "by_lanes": groups,
             "last_island": {"contacts": int(rows[c, 2]), "bodies": int(rows[c, 3]), "lanes": int(rows[c, 4]),
+                            "sweep_period": int(rows[c, 5]),
                             "start_us": float((start[c] - start.min()) / 1e3), "latency_us": float(lat[c])}}
 
 

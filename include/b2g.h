@@ -143,9 +143,11 @@ B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int 
 
 /* With B2G_PARAM_ISLAND_TIMES on: the islands the last step solved, in the
    solver's queue order, as 4 int64 each: start and end (ns, device wall
-   clock), contacts | bodies << 32, and the lanes that solved it (a team's
-   size, 1 for one lane per island, 0 for a team island whose state did not
-   fit the team's LDS share).  *count receives the number of islands; at most
+   clock), contacts | bodies << 32, and lanes | period << 32: the lanes that
+   solved it (a team's size, 1 for one lane per island, 0 for a team island
+   whose state did not fit the team's LDS share) and its sweep period (the
+   lag in dependency levels between consecutive sweeps, which with the
+   iteration counts predicts the island's passes).  *count receives the number of islands; at most
    cap rows are written.  No counterpart in the reference (per-island latency
    of the parallel solver). */
 B2G_API int b2g_island_times(b2g_ctx* ctx, int64_t* out, int32_t cap, int32_t* count);

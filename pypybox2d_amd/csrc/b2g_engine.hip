@@ -1868,12 +1868,13 @@ __global__ void __launch_bounds__(kBlock) k_prep_contacts(Dev<T> d, int chunks) 
 // the lanes that solved it (a team's size, 1 for the lane path, 0 for a team island solved by
 // one lane because its state did not fit the team's LDS share).
 template <typename T>
-__device__ __forceinline__ void record_island_time(const Dev<T>& d, int q, long long t0, int nci, int nbi, int lanes) {
+__device__ __forceinline__ void record_island_time(const Dev<T>& d, int q, long long t0, int nci, int nbi, int lanes,
+                                                   int period) {
   long long* o = d.i_time + 4 * (size_t)q;
   o[0] = t0;
   o[1] = (long long)wall_clock64();
   o[2] = (long long)nci | (long long)nbi << 32;
-  o[3] = lanes;
+  o[3] = (long long)lanes | (long long)period << 32;
 }
 
 // One wavefront per block.  The queue is ordered largest class first.  Its
@@ -1905,18 +1906,18 @@ __global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_e
       const int qend = d.q_cls[kQBase + cls] + d.q_cls[cls];
       if (q < qend) {
         const int4 in = d.qs_isl[q];
+        const int nbi = in.z & 0xFFFF, period = (int)((uint32_t)in.z >> 16);
         const int nc = d.qs_nc[q];
         const int nci = nc & 0x3FFFFFFF, last_allow = (nc >> 30) & 1;
         const int share = d.team_lds / per;
         const long long t0 = d.i_time ? (long long)wall_clock64() : 0;
-        const int nbi = in.z & 0xFFFF, period = (int)((uint32_t)in.z >> 16);
         const bool fits = team_lds_bytes(nbi) <= share;
         if (fits) {
           solve_island_team(d, in.x, in.y, nbi, in.w, nci, last_allow, period, team_lds + team * share, t, TS);
         } else if (t == 0) {
           solve_island(d, in.x, in.y, nbi, in.w, nci, last_allow);
         }
-        if (d.i_time && t == 0) record_island_time(d, q, t0, nci, nbi, fits ? TS : 0);
+        if (d.i_time && t == 0) record_island_time(d, q, t0, nci, nbi, fits ? TS : 0, period);
       }
       team_sync();
     } else {
@@ -1926,7 +1927,7 @@ __global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_e
         const int4 in = d.qs_isl[q];
         const int nc = d.qs_nc[q];
         solve_island(d, in.x, in.y, in.z & 0xFFFF, in.w, nc & 0x3FFFFFFF, (nc >> 30) & 1);
-        if (d.i_time) record_island_time(d, q, t0, nc & 0x3FFFFFFF, in.z & 0xFFFF, 1);
+        if (d.i_time) record_island_time(d, q, t0, nc & 0x3FFFFFFF, in.z & 0xFFFF, 1, (int)((uint32_t)in.z >> 16));
       }
     }
   }
