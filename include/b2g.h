@@ -52,7 +52,9 @@ typedef struct b2g_ctx b2g_ctx;
 
 B2G_API int b2g_version(void);
 
-/* Create / destroy a context (one GPU, one stream).  Replaces the construction
+/* Create / destroy a context (one GPU; one stream for the step and three
+   helper streams on which the island and pair searches run side by side,
+   forked from and joined back into the step's stream).  Replaces the construction
    of World (world.py:64-91) and its ContactManager/BroadPhase
    (contact_manager.py:34-44, broadphase.py:36-41) as far as the device state
    goes. */
