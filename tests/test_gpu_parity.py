@@ -189,6 +189,31 @@ def test_batched_mixed_worlds_one_step(prec):
         assert be[q] <= tol, (q, be[q])
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_lognormal_batch_worlds_one_step(prec):
+    """Worlds of the lognormal batch (16 384 worlds, body counts clip(round(64 exp(0.75 N)),
+    16, 1024), seed 2): its first 40 worlds and its largest, stepped 200 times on the GPU so
+    that piles form, then one step from that state against the oracle."""
+    counts = scenes.lognormal_batch_counts()
+    pick = list(range(40)) + [int(np.argmax(counts))]
+    batch = builder.scenes_batch([scenes.batch_world(2, w, counts[w]) for w in pick])
+    assert counts[pick[-1]] == 1024
+    e = N.Engine(prec)
+    e.upload(batch)
+    e.step_n(200, DT, 8, 3, graph=True)
+    state = e.download()
+    e.close()
+    state.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    ref = oracle_step(state)
+    got = gpu_step(state, prec)
+    assert PU.same_contact_lists(got, ref)
+    be = PU.body_errors(got, ref)
+    print(prec, "lognormal worlds one step", be, flush=True)
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    for q in ("position", "angle", "velocity", "angular_velocity"):
+        assert be[q] <= tol, (q, be[q])
+
+
 def test_graph_replay_matches_eager():
     rng = np.random.default_rng(4)
     batch = builder.scenes_batch([scenes.mixed_world(rng, n, half_width=10.0, height=15.0) for n in (30, 50, 40)])
