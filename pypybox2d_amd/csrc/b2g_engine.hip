@@ -522,8 +522,12 @@ __device__ int block_exclusive_scan(int* a, int n, int* scratch) {
   return total;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_islands(Dev<T> d, int w_first) {
+// B threads per workgroup: 256, or 128 for worlds whose LDS footprint lets more than five
+// workgroups share a CU (the one-lane walk dominates the kernel, so more worlds in flight
+// per CU outweigh the slower wide phases).
+template <typename T, int B>
+__global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) k_islands(Dev<T> d, int w_first) {
+  constexpr int kBlock = B;
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = d.w_order[w_first + blockIdx.x], tid = threadIdx.x;
   if (d.w_big && d.w_big[w]) return;  // k_islands_big
@@ -595,7 +599,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
           cfl[idx] = (uint16_t)(((st & CS_PROCESSED) ? 1 : 0) | ((st & CS_WOKE) ? 2 : 0));
         }
         __syncthreads();
-        if (tid == 0) L.misc[4] += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        if (tid == 0) {
+          int add = 0;
+          for (int j = 0; j < kBlock / 64; ++j) add += wcount[j];
+          L.misc[4] += add;
+        }
         __syncthreads();
       }
       if (tid == 0) {
@@ -634,7 +642,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     }
   }
   __syncthreads();
-  int total = block_exclusive_scan(L.off, nb, L.scan);
+  int total = block_exclusive_scan<B>(L.off, nb, L.scan);
   if (total > nadj) {  // more touching contacts than the island search has room for
     if (tid == 0) {
       atomicOr(&d.w_err[w], WERR_CAPACITY);
@@ -799,7 +807,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
   __syncthreads();
   for (int k = tid; k < nci_tot; k += kBlock) atomicAdd(&hist[ent[k] >> 16], 1);
   __syncthreads();
-  block_exclusive_scan(hist, nci_tot, L.scan);
+  block_exclusive_scan<B>(hist, nci_tot, L.scan);
   for (int k = tid; k < nci_tot; k += kBlock) {
     const uint32_t e = ent[k];
     d.i_order[cb + atomicAdd(&hist[e >> 16], 1)] = e;
@@ -3094,7 +3102,8 @@ struct WorldRun {
   size_t smem;
 };
 constexpr int kMaxRuns = 4;
-constexpr int kIslandsPerCu = 5;  // k_islands workgroups per CU allowed by its registers
+constexpr int kIslandsPerCu = 10;  // k_islands workgroups per CU allowed by its registers (128 threads)
+constexpr int kIslandsWidePerCu = 5;  // ... with 256 threads
 constexpr int kPairsPerCu = 7;    // k_pairs workgroups per CU allowed by its registers
 
 // Runs of `order` (at most kMaxRuns): a run ends where the next world's workgroups would
@@ -3278,7 +3287,9 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   }
   HIPCHK(hipFuncSetAttribute((const void*)k_solve<T>, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
 
-  HIPCHK(hipFuncSetAttribute((const void*)k_islands<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  HIPCHK(hipFuncSetAttribute((const void*)k_islands<T, kBlock>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)ctx->smem_islands));
+  HIPCHK(hipFuncSetAttribute((const void*)k_islands<T, kBlock / 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)ctx->smem_islands));
   HIPCHK(hipFuncSetAttribute((const void*)k_pairs<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)ctx->smem_pairs));
@@ -3685,7 +3696,10 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   if (timed) HIPCHK(hipEventRecord(ev[1], st));
   if (dt > 0.0) {
     TRY(fan(ctx->isl_runs, [&](const WorldRun& r, hipStream_t s) {
-      hipLaunchKernelGGL(k_islands<T>, dim3(r.count), dim3(kBlock), r.smem, s, d, r.first);
+      if (r.smem > 0 && (size_t)kMaxLds / r.smem <= (size_t)kIslandsWidePerCu)
+        hipLaunchKernelGGL((k_islands<T, kBlock>), dim3(r.count), dim3(kBlock), r.smem, s, d, r.first);
+      else
+        hipLaunchKernelGGL((k_islands<T, kBlock / 2>), dim3(r.count), dim3(kBlock / 2), r.smem, s, d, r.first);
     }));
     if (nbig > 0) hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), ctx->smem_big, st, d);
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
