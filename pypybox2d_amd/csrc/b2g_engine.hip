@@ -2305,6 +2305,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int w_first, int fro
 // canonical order and created by one workgroup per world.
 // ---------------------------------------------------------------------------
 constexpr int kBigBlock = 1024;
+constexpr size_t kBigSortLds = 65536;  // k_islands_big: at least 512 contacts per wavefront's level sort
 constexpr int kIntMax = 0x7fffffff;
 
 __device__ __forceinline__ int g_ld(const int* p) {
@@ -2382,11 +2383,19 @@ __device__ __forceinline__ void touching_ends(const Dev<T>& d, int cb, int f0, i
   }
 }
 
+// LDS written by some lanes of a wavefront is read by others: keep the accesses in order.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 template <typename T>
-__global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
+__global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_cap) {
   __shared__ int scan[kBigBlock];
   __shared__ int sh[8];
-  extern __shared__ uint32_t uf_lds[];  // the union-find parents, 16 bits each
+  // the union-find parents, 16 bits each; once the components are stored, each wavefront's
+  // 2 * sort_cap words for the level sort of islands of up to sort_cap contacts
+  extern __shared__ uint32_t uf_lds[];
   const int w = d.big_list[blockIdx.x], tid = threadIdx.x;
   const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
   const int f0 = d.w_fix_off[w];
@@ -2679,6 +2688,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
       nbi = nci = 0;
     }
     d.i_info[b0 + k] = make_int4(bstart, nbi | sweep_period(span, depth) << 16, cstart, nci | (last_allow << 30));
+    if (nci <= sort_cap) continue;  // sorted by a wavefront in LDS below
     // the island's contacts by level (counting sort in the stack area)
     int* hist = stack;
     for (int q = 0; q < nci; ++q) hist[q] = 0;
@@ -2695,6 +2705,47 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d) {
       hist[lv] += 1;
     }
     for (int q = 0; q < nci; ++q) d.i_order[cb + cstart + q] = (uint32_t)stack[nci + q];
+  }
+  __syncthreads();
+  // Islands of up to sort_cap contacts: their contacts by level, one wavefront per island, a
+  // counting sort in LDS (levels 0 .. nci - 1).  Contacts of one level share no dynamic body,
+  // so their order within the level does not matter.
+  {
+    const int lane = tid & 63, wv = tid >> 6;
+    int* h = (int*)uf_lds + (size_t)wv * 2 * sort_cap;
+    int* o = h + sort_cap;
+    for (int k = wv; k < nisl; k += kBigBlock / 64) {
+      const int4 info = d.i_info[b0 + k];
+      const int nci = info.w & 0x3FFFFFFF;
+      if (nci == 0 || nci > sort_cap) continue;
+      uint32_t* ord = d.i_order + cb + info.z;
+      for (int q = lane; q < nci; q += 64) h[q] = 0;
+      wave_lds_sync();
+      for (int q = lane; q < nci; q += 64) atomicAdd(&h[ord[q]], 1);
+      wave_lds_sync();
+      const int per = (nci + 63) / 64, lo = min(nci, lane * per), hi = min(nci, lo + per);
+      int sum = 0;
+      for (int i = lo; i < hi; ++i) sum += h[i];
+      int incl = sum;
+      for (int sh_ = 1; sh_ < 64; sh_ <<= 1) {
+        const int v = __shfl_up(incl, sh_, 64);
+        if (lane >= sh_) incl += v;
+      }
+      int run = incl - sum;
+      for (int i = lo; i < hi; ++i) {
+        const int v = h[i];
+        h[i] = run;
+        run += v;
+      }
+      wave_lds_sync();
+      for (int q = lane; q < nci; q += 64) {
+        const int lv = (int)ord[q];
+        o[atomicAdd(&h[lv], 1)] = q | lv << 16;  // island-local index | level key << 16
+      }
+      wave_lds_sync();
+      for (int q = lane; q < nci; q += 64) ord[q] = (uint32_t)o[q];
+      wave_lds_sync();
+    }
   }
   __syncthreads();
   // --- queue, island-local contact ends, ranks oriented as (body A, body B) ---
@@ -3280,6 +3331,7 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
       w_big[w] = 1;
       ctx->big.push_back(w);
       ctx->smem_big = std::max(ctx->smem_big, (size_t)((wnb + 1) / 2) * 4);  // 16-bit union-find parents
+      ctx->smem_big = std::max(ctx->smem_big, kBigSortLds);                 // level sorts, reusing them
     } else {
       ctx->smem_islands = std::max(ctx->smem_islands, si);
       ctx->smem_pairs = std::max(ctx->smem_pairs, sp);
@@ -3701,7 +3753,9 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
       else
         hipLaunchKernelGGL((k_islands<T, kBlock / 2>), dim3(r.count), dim3(kBlock / 2), r.smem, s, d, r.first);
     }));
-    if (nbig > 0) hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), ctx->smem_big, st, d);
+    if (nbig > 0)
+      hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), ctx->smem_big, st, d,
+                         (int)(ctx->smem_big / (kBigBlock / 64 * 2 * sizeof(int))));
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[2], st));
