@@ -214,6 +214,41 @@ def test_lognormal_batch_worlds_one_step(prec):
         assert be[q] <= tol, (q, be[q])
 
 
+def test_ragged_batch_with_empty_and_fixtureless_worlds():
+    """Worlds with no dynamic body (ground only), one body, 40 bodies, and one whose bodies
+    have no fixtures (the dynamic one falls with the default unit mass): FP64 tracks the
+    oracle for 60 steps with equal contact lists, then one FP32 step from that state."""
+    rng = np.random.default_rng(5)
+    ground_only = scenes.mixed_world(rng, 0, half_width=10.0, height=15.0)
+    one = scenes.mixed_world(rng, 1, half_width=10.0, height=15.0)
+    forty = scenes.mixed_world(rng, 40, half_width=10.0, height=15.0)
+    bare = {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True,
+            "bodies": [{"type": "dynamic", "position": (0.0, 3.0), "angle": 0.0, "linear_velocity": (1.0, 0.0),
+                        "angular_velocity": 0.5, "fixtures": []},
+                       {"type": "static", "position": (0.0, 0.0), "angle": 0.0, "linear_velocity": (0.0, 0.0),
+                        "angular_velocity": 0.0, "fixtures": []}]}
+    batch = builder.scenes_batch([ground_only, one, forty, bare, ground_only])
+    assert list(np.diff(batch.fix_off)) == [1, 2, 41, 0, 1]
+    o = O.Oracle(batch)
+    e = N.Engine("fp64")
+    e.upload(batch)
+    for _ in range(60):
+        assert o.step(DT, 8, 3) == 0
+        e.step(DT, 8, 3)
+    got, ref = e.download(), o.state()
+    e.close()
+    assert PU.same_contact_lists(got, ref)
+    be = PU.body_errors(got, ref)
+    print("ragged fp64, 60 steps", be, flush=True)
+    assert max(be[q] for q in ("position", "angle", "velocity", "angular_velocity")) <= 1e-9
+    ref.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    got32, ref1 = gpu_step(ref, "fp32"), oracle_step(ref)
+    assert PU.same_contact_lists(got32, ref1)
+    be = PU.body_errors(got32, ref1)
+    print("ragged fp32, one step", be, flush=True)
+    assert max(be[q] for q in ("position", "angle", "velocity", "angular_velocity")) <= 1e-5
+
+
 def test_graph_replay_matches_eager():
     rng = np.random.default_rng(4)
     batch = builder.scenes_batch([scenes.mixed_world(rng, n, half_width=10.0, height=15.0) for n in (30, 50, 40)])
