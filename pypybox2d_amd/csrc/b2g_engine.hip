@@ -2306,10 +2306,41 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int w_first, int fro
 // ---------------------------------------------------------------------------
 constexpr int kBigBlock = 1024;
 constexpr size_t kBigSortLds = 65536;  // k_islands_big: at least 512 contacts per wavefront's level sort
+constexpr size_t kBigWalkLdsMax = 147456;  // k_islands_big: most dynamic LDS for the walks' state
 constexpr int kIntMax = 0x7fffffff;
 
 __device__ __forceinline__ int g_ld(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// State of the large-world per-island walks: body state bits, rank counters (rank | first
+// level << 16), the level of each body's last contact, and the contact-seen flags; in global
+// memory, or in LDS when the world's fits (then one global round trip per adjacency entry,
+// which the walk prefetches ahead, is all that is left of the walk's memory latency).
+struct WalkGlobal {
+  int* bst; int* cur; int* lev; int* cflag;
+  __device__ int bst_ld(int i) const { return g_ld(&bst[i]); }
+  __device__ void bst_st(int i, int v) const { bst[i] = v; }
+  __device__ int cur_ld(int i) const { return g_ld(&cur[i]); }
+  __device__ void cur_st(int i, int v) const { cur[i] = v; }
+  __device__ int lev_ld(int i) const { return g_ld(&lev[i]); }
+  __device__ void lev_st(int i, int v) const { lev[i] = v; }
+  __device__ int cf_ld(int sl) const { return g_ld(&cflag[sl]); }
+  __device__ void cf_set(int sl) const { cflag[sl] = 1; }
+};
+struct WalkLds {
+  int* bst; int* cur; uint16_t* lev; uint32_t* cf;
+  __device__ int bst_ld(int i) const { return bst[i]; }
+  __device__ void bst_st(int i, int v) const { bst[i] = v; }
+  __device__ int cur_ld(int i) const { return cur[i]; }
+  __device__ void cur_st(int i, int v) const { cur[i] = v; }
+  __device__ int lev_ld(int i) const { return lev[i]; }
+  __device__ void lev_st(int i, int v) const { lev[i] = (uint16_t)v; }
+  __device__ int cf_ld(int sl) const { return (int)((cf[sl >> 5] >> (sl & 31)) & 1u); }
+  __device__ void cf_set(int sl) const { atomicOr(&cf[sl >> 5], 1u << (sl & 31)); }  // words shared by islands
+};
+__host__ __device__ inline size_t walk_lds_bytes(int nb, int ccap) {
+  return ((size_t)nb * 4 + 15) / 16 * 16 * 2 + ((size_t)nb * 2 + 15) / 16 * 16 + ((size_t)(ccap + 31) / 32 * 4 + 15) / 16 * 16;
 }
 // Union-find over a large world's bodies (lock-free hooking of the larger root under the
 // smaller, path halving), on 16-bit parents packed two to a word in LDS (a world has fewer
@@ -2390,7 +2421,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_cap) {
+__global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_cap, int smem_dyn) {
   __shared__ int scan[kBigBlock];
   __shared__ int sh[8];
   // the union-find parents, 16 bits each; once the components are stored, each wavefront's
@@ -2608,6 +2639,23 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
   const int nisl = sh[1];
   const int nbi_tot = (int)(*alloc >> 32), nci_tot = (int)(*alloc & 0xffffffffu);
   // --- depth-first search of each island (world.py:494-580), one lane per island ---
+  const bool lds_walk = walk_lds_bytes(nb, ccap) <= (size_t)smem_dyn;
+  WalkLds LW{};
+  if (lds_walk) {  // the union-find parents are dead: the walk state moves into LDS
+    unsigned char* p = (unsigned char*)uf_lds;
+    LW.bst = (int*)p; p += ((size_t)nb * 4 + 15) / 16 * 16;
+    LW.cur = (int*)p; p += ((size_t)nb * 4 + 15) / 16 * 16;
+    LW.lev = (uint16_t*)p; p += ((size_t)nb * 2 + 15) / 16 * 16;
+    LW.cf = (uint32_t*)p;
+    for (int i = tid; i < nb; i += kBigBlock) {
+      LW.bst[i] = bst[i];
+      LW.cur[i] = 0;
+      LW.lev[i] = 0;
+    }
+    for (int i = tid; i < (ccap + 31) / 32; i += kBigBlock) LW.cf[i] = 0u;
+    __syncthreads();
+  }
+  auto walk = [&](const auto& S) {
   for (int k = tid; k < nisl; k += kBigBlock) {
     const int4 r = isl[k];
     const int bstart = r.y, cstart = r.z, s0 = r.w;
@@ -2616,37 +2664,43 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
     int* statics = stack + cap_b + 2 * cap_c;                              // grows downwards
     int nstat = 0, sp = 0, nbi = 0, nci = 0, last_allow = 0, span = 0, depth = 0;
     stack[sp++] = s0;
-    bst[s0] |= 16;
+    S.bst_st(s0, S.bst_ld(s0) | 16);
     bool bad = false;
     while (sp > 0 && !bad) {
       const int b = stack[--sp];
-      int bs = bst[b];
+      int bs = S.bst_ld(b);
       last_allow = (bs >> 2) & 1;
       if (!(bs & 2)) bs |= 2 | 32;  // body.awake = True
-      bst[b] = bs;
+      S.bst_st(b, bs);
       if (bs & 1) continue;  // static bodies join but do not propagate
       if (nbi >= cap_b) { bad = true; break; }
       bidx[b] = bstart + nbi;  // position in the world's island-body list, like the LDS search
       wake[b] = bstart;        // the start of its island (the wake slots are free again)
       isl_b[bstart + nbi++] = b;
       // The expanding body's counters stay in registers (no other lane touches this island);
-      // cur = rank | first level << 16.  The next entry is read one step ahead, and an entry's
-      // contact flag and other body's state are read together by relaxed atomic loads, which
-      // the compiler keeps ahead of the test.
-      int cur_b = cur[b], lev_b = lev[b];
-      const int eend = off[b + 1];
-      int2 a = off[b] < eend ? adj[off[b]] : make_int2(0, 0);
-      for (int e = off[b]; e < eend; ++e) {
-        const int2 an = e + 1 < eend ? adj[e + 1] : make_int2(0, 0);
-        const int sl_ = a.x, o = a.y;
-        a = an;
-        const int fw = g_ld(&cflag[sl_]);
-        const int so = g_ld(&bst[o]);
-        const int co = g_ld(&cur[o]);
-        const int lo_read = g_ld(&lev[o]);
+      // cur = rank | first level << 16.  Adjacency entries (read only) are read four steps
+      // ahead, and an entry's contact flag and other body's state are read together (relaxed
+      // atomic loads in global memory, which the compiler keeps ahead of the test).
+      int cur_b = S.cur_ld(b), lev_b = S.lev_ld(b);
+      const int ebeg = off[b], eend = off[b + 1];
+      int2 a0 = ebeg < eend ? adj[ebeg] : make_int2(0, 0);
+      int2 a1 = ebeg + 1 < eend ? adj[ebeg + 1] : make_int2(0, 0);
+      int2 a2 = ebeg + 2 < eend ? adj[ebeg + 2] : make_int2(0, 0);
+      int2 a3 = ebeg + 3 < eend ? adj[ebeg + 3] : make_int2(0, 0);
+      for (int e = ebeg; e < eend; ++e) {
+        const int2 a4 = e + 4 < eend ? adj[e + 4] : make_int2(0, 0);
+        const int sl_ = a0.x, o = a0.y;
+        a0 = a1;
+        a1 = a2;
+        a2 = a3;
+        a3 = a4;
+        const int fw = S.cf_ld(sl_);
+        const int so = S.bst_ld(o);
+        const int co = S.cur_ld(o);
+        const int lo_read = S.lev_ld(o);
         if (fw) continue;
         if (nci >= cap_c || sp + nstat + 2 > cap_b + 2 * cap_c) { bad = true; break; }
-        cflag[sl_] = 1;
+        S.cf_set(sl_);
         const bool o_dyn = !(so & 1);
         const int lo = o_dyn ? lo_read : 0;
         const int lvl = 1 + (lev_b > lo ? lev_b : lo);
@@ -2658,9 +2712,9 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
         if (o_dyn) {
           rank_o = (uint32_t)(co & 0xFFFF);
           const int fo = rank_o == 0 ? lvl : (int)((uint32_t)co >> 16);
-          cur[o] = (int)(rank_o + 1) | fo << 16;
+          S.cur_st(o, (int)(rank_o + 1) | fo << 16);
           span = max(span, lvl - fo);
-          lev[o] = lvl;
+          S.lev_st(o, lvl);
         }
         depth = max(depth, lvl);
         lev_b = lvl;
@@ -2669,7 +2723,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
         isl_c[cstart + nci++] = sl_;
         if (o_dyn) {
           if (so & 16) continue;
-          bst[o] = so | 16;
+          S.bst_st(o, so | 16);
         } else {
           bool seen = false;
           for (int q = 1; q <= nstat; ++q) seen = seen || statics[-q] == o;
@@ -2678,8 +2732,8 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
         }
         stack[sp++] = o;
       }
-      cur[b] = cur_b;
-      lev[b] = lev_b;
+      S.cur_st(b, cur_b);
+      S.lev_st(b, lev_b);
     }
     if (bad || nbi != cap_b || nci != cap_c) {  // the component sizes and the search disagree
       atomicOr(&d.w_err[w], WERR_SCHEDULE);
@@ -2705,6 +2759,17 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
       hist[lv] += 1;
     }
     for (int q = 0; q < nci; ++q) d.i_order[cb + cstart + q] = (uint32_t)stack[nci + q];
+  }
+  };
+  if (lds_walk) {
+    walk(LW);
+    __syncthreads();
+    for (int i = tid; i < nb; i += kBigBlock) {  // what the later phases read
+      bst[i] = LW.bst[i];
+      cur[i] = LW.cur[i];
+    }
+  } else {
+    walk(WalkGlobal{bst, cur, lev, cflag});
   }
   __syncthreads();
   // Islands of up to sort_cap contacts: their contacts by level, one wavefront per island, a
@@ -3332,6 +3397,8 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
       ctx->big.push_back(w);
       ctx->smem_big = std::max(ctx->smem_big, (size_t)((wnb + 1) / 2) * 4);  // 16-bit union-find parents
       ctx->smem_big = std::max(ctx->smem_big, kBigSortLds);                 // level sorts, reusing them
+      const size_t wl = walk_lds_bytes(wnb, ctx->con_cap[w]);              // the walks' state, if it fits
+      if (wl <= kBigWalkLdsMax) ctx->smem_big = std::max(ctx->smem_big, wl);
     } else {
       ctx->smem_islands = std::max(ctx->smem_islands, si);
       ctx->smem_pairs = std::max(ctx->smem_pairs, sp);
@@ -3755,7 +3822,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     }));
     if (nbig > 0)
       hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), ctx->smem_big, st, d,
-                         (int)(ctx->smem_big / (kBigBlock / 64 * 2 * sizeof(int))));
+                         (int)(ctx->smem_big / (kBigBlock / 64 * 2 * sizeof(int))), (int)ctx->smem_big);
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[2], st));
