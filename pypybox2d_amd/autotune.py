@@ -23,6 +23,7 @@ CANDIDATES: Sequence[Dict[str, int]] = (
     {"heavy_class": 5, "team_full_class": 9, "team_max": 64, "team_lds": 20480},
     {"heavy_class": 5, "team_full_class": 8, "team_max": 64, "team_lds": 20480},
     {"heavy_class": 5, "team_full_class": 10, "team_max": 32, "team_lds": 24576},
+    {"heavy_class": 5, "team_full_class": 7, "team_max": 32, "team_lds": 20480},
 )
 
 
