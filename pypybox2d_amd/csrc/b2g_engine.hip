@@ -59,6 +59,7 @@ enum : int { WF_ALLOW_SLEEP = 1, WF_WARM = 2, WF_NEW_FIXTURE = 4, WF_KEEP_FORCES
 enum : int { WERR_CAPACITY = 1, WERR_UNSUPPORTED = 2, WERR_NEWPAIRS = 4, WERR_SCHEDULE = 8 };
 
 constexpr int kBlock = 256;
+constexpr int kBigPairsBlock = 1024;  // k_pairs workgroup for the largest worlds
 constexpr int kMaxLds = 163840;  // LDS per workgroup on gfx950
 constexpr int kProfileBatch = 128;  // steps per event read-back in b2g_profile_step
 constexpr int kSizeClasses = 16;
@@ -2016,7 +2017,7 @@ __host__ __device__ inline size_t pairs_smem_bytes(int nf, int ccap) {
   s += ((size_t)nf * 2 + 15) & ~(size_t)15;                 // mlist
   s += (size_t)next_pow2(ccap + 1) * 4;                     // hash
   s += (size_t)next_pow2(8 * nf + 64) * 4;                  // new pairs
-  s += (size_t)kBlock * 4 + 64;
+  s += (size_t)kBigPairsBlock * 4 + 64;  // scan scratch for the widest workgroup
   return s;
 }
 
@@ -2101,8 +2102,11 @@ __device__ void pair_candidate(const Dev<T>& d, int w, int f0, int b0, int P, in
   else atomicOr(&misc[1], 1);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int w_first, int from_upload, int finish, int do_pairs) {
+// B threads per workgroup: 256, or 1024 for the largest worlds (those whose LDS footprint
+// allows at most two workgroups per CU), whose search sets the kernel's tail.
+template <typename T, int B>
+__global__ void __launch_bounds__(B) k_pairs(Dev<T> d, int w_first, int from_upload, int finish, int do_pairs) {
+  constexpr int kBlock = B;
   extern __shared__ __align__(16) unsigned char smem[];
   const int w = d.w_order[w_first + blockIdx.x], tid = threadIdx.x;  // largest worlds first
   if (d.w_big && d.w_big[w]) return;  // k_pairs_big_*
@@ -2138,7 +2142,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs(Dev<T> d, int w_first, int fro
     // moved fixtures in synchronisation order (bodies in order, fixtures body-major)
     for (int i = tid; i < nf; i += kBlock) mpos[i] = d.f_moved[f0 + i] ? 1 : 0;
     __syncthreads();
-    nmove = block_exclusive_scan(mpos, nf, scan);
+    nmove = block_exclusive_scan<B>(mpos, nf, scan);
     for (int i = tid; i < nf; i += kBlock) {
       if (d.f_moved[f0 + i]) mlist[mpos[i]] = (uint16_t)i;
       else mpos[i] = -1;
@@ -3221,6 +3225,7 @@ constexpr int kMaxRuns = 4;
 constexpr int kIslandsPerCu = 10;  // k_islands workgroups per CU allowed by its registers (128 threads)
 constexpr int kIslandsWidePerCu = 5;  // ... with 256 threads
 constexpr int kPairsPerCu = 7;    // k_pairs workgroups per CU allowed by its registers
+constexpr int kPairsWideMaxPerCu = 2;  // runs this LDS-heavy launch 1024-thread workgroups
 
 // Runs of `order` (at most kMaxRuns): a run ends where the next world's workgroups would
 // fit more often per CU (`cap` = the kernel's limit from registers) than the run's largest;
@@ -3410,7 +3415,9 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
                               (int)ctx->smem_islands));
   HIPCHK(hipFuncSetAttribute((const void*)k_islands<T, kBlock / 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)ctx->smem_islands));
-  HIPCHK(hipFuncSetAttribute((const void*)k_pairs<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  HIPCHK(hipFuncSetAttribute((const void*)k_pairs<T, kBlock>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)ctx->smem_pairs));
+  HIPCHK(hipFuncSetAttribute((const void*)k_pairs<T, kBigPairsBlock>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)ctx->smem_pairs));
   HIPCHK(hipFuncSetAttribute((const void*)k_islands_big<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)ctx->smem_big));
@@ -3796,8 +3803,12 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   };
   auto pairs = [&](int from_upload, int finish, int do_pairs) -> int {
     TRY(fan(ctx->pair_runs, [&](const WorldRun& r, hipStream_t s) {
-      hipLaunchKernelGGL(k_pairs<T>, dim3(r.count), dim3(kBlock), r.smem, s, d, r.first, from_upload, finish,
-                         do_pairs);
+      if (r.smem > 0 && (size_t)kMaxLds / r.smem <= (size_t)kPairsWideMaxPerCu)
+        hipLaunchKernelGGL((k_pairs<T, kBigPairsBlock>), dim3(r.count), dim3(kBigPairsBlock), r.smem, s, d, r.first,
+                           from_upload, finish, do_pairs);
+      else
+        hipLaunchKernelGGL((k_pairs<T, kBlock>), dim3(r.count), dim3(kBlock), r.smem, s, d, r.first, from_upload,
+                           finish, do_pairs);
     }));
     if (nbig > 0) {
       hipLaunchKernelGGL(k_pairs_big_prep<T>, dim3(nbig), dim3(kBigBlock), 0, st, d, from_upload, do_pairs);
