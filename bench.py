@@ -58,7 +58,8 @@ def workload_counts(state: L.BatchArrays):
 
 
 def algorithmic_bytes(kernel: str, c: dict, fp64: bool) -> float:
-    """Minimum HBM bytes one launch of `kernel` must move (per-unit figures in DESIGN.md)."""
+    """Minimum HBM bytes one launch of `kernel` must move: the per-unit figures below times the
+    counts of the end state."""
     t = 8 if fp64 else 4
     man = 4 + 4 * 2 * t + 4 * t + 8      # state word + ln,lp,p0,p1 + 4 impulses + 2 ids
     if kernel == "k_collide":

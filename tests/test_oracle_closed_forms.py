@@ -1,6 +1,6 @@
 """Closed forms and invariants that pin the FP64 oracle beyond the recorded runs.
 
-Each check is derived from the mathematics of the step (SURVEY-style pins),
+Each check is derived from the mathematics of the step,
 not from the oracle's own formulas: free flight (semi-implicit Euler,
 island.py:71-138), the restitution bias (contact.py:451-456) exchanging the
 velocities of equal head-on circles, the friction cone and non-negative normal
