@@ -102,11 +102,11 @@ def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, tune:
     done = 0
     while done < steps:
         left = steps - done
-        if tuner is not None and left > len(tuner.candidates):
+        if tuner is not None and left > tuner.steps_per_tune:
             tuner.tune(1.0 / 60, 8, 3)
             sums += tuner.last_ms.sum(axis=0)
-            done += len(tuner.candidates)
-            n = min(tuner.interval() - len(tuner.candidates), steps - done)
+            done += tuner.steps_per_tune
+            n = min(tuner.interval() - tuner.steps_per_tune, steps - done)
         else:
             n = min(200, left)
         seg = None

@@ -133,6 +133,20 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
 #define B2G_PARAM_SOLVE_WAVES_PER_CU 7
 #define B2G_PARAM_TEAM_MAX 8
 #define B2G_PARAM_SEARCH_RUNS 9
+/* Launch shapes of the searches and of the narrow phase (results never change):
+     B2G_PARAM_PAIRS_WIDE       pair-search runs whose LDS footprint lets at most
+                                this many workgroups share a CU launch
+                                1024-thread workgroups, the others 256 (0..32,
+                                default 2)
+     B2G_PARAM_ISLANDS_WIDE     island-search runs whose LDS footprint lets at
+                                most this many workgroups share a CU launch
+                                256-thread workgroups, the others 128 (0..32,
+                                default 5)
+     B2G_PARAM_COLLIDE_BLOCK    contact slots per narrow-phase workgroup: 64,
+                                128, 256 (default) or 512 */
+#define B2G_PARAM_PAIRS_WIDE 10
+#define B2G_PARAM_ISLANDS_WIDE 11
+#define B2G_PARAM_COLLIDE_BLOCK 12
 B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value);
 B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
 

@@ -27,7 +27,11 @@ EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step",
 # keys of b2g_set_param / b2g_get_param: the solver's launch shape, island timing, and the
 # resident solver wavefronts per compute unit and the search launches per step (both read only)
 PARAMS = {"team_lds": 1, "heavy_class": 2, "solve_grid": 3, "team_full_class": 4, "large_worlds": 5,
-          "island_times": 6, "solve_waves_per_cu": 7, "team_max": 8, "search_runs": 9}
+          "island_times": 6, "solve_waves_per_cu": 7, "team_max": 8, "search_runs": 9, "pairs_wide": 10,
+          "islands_wide": 11, "collide_block": 12}
+
+# the stages b2g_profile_step times, in order
+STAGES = ("collide", "islands", "solve", "sync", "pairs")
 
 _ERRNAMES = {B2G_E_INVALID: "invalid argument", B2G_E_CAPACITY: "capacity exceeded", B2G_E_HIP: "HIP error",
              B2G_E_NAN: "non-finite state", B2G_E_UNSUPPORTED: "unsupported feature",

@@ -3,32 +3,50 @@
 All worlds of the batch share dt and the iteration counts; each keeps its own
 gravity, sleep and warm-starting flags.  The state stays on the GPU between
 steps; ``state()`` copies it back in the batch layout.
+
+Steps are graph-captured by default.  While the launch tuner is on, it takes
+one eager step per candidate launch shape at the start of each tuning period
+(real steps: the shape changes the speed, never the results).  After every
+call the device's error flags are checked: a capacity error re-creates the
+batch with more contact slots and replays the steps from the last good state
+(``Stepper``); a non-finite body state raises ``B2GError`` (B2G_E_NAN).
 """
 from __future__ import annotations
 
-from typing import List, Optional
+from typing import List
 
 import numpy as np
 
 from . import layout as L
-from ._native import Engine
 from .autotune import LaunchTuner
 from .builder import scenes_batch
+from .stepper import Stepper
 
 
 class BatchedWorld(object):
     def __init__(self, batch: L.BatchArrays, precision: str = "fp32", device: int = 0,
-                 contact_factor: float = 8.0, autotune: bool = True):
+                 contact_factor: float = 8.0, contact_extra: int = 64, autotune: bool = True):
         self.batch = batch.normalized()
-        self.engine = Engine(precision, device, contact_factor=contact_factor)
-        self.engine.upload(self.batch)
         self.precision = precision
-        # the solver's launch shape changes its speed, never its results
-        self.tuner = LaunchTuner(self.engine) if autotune else None
+        self.stepper = Stepper(precision, device, contact_factor, contact_extra)
+        self.stepper.upload(self.batch)
+        self.tuner = LaunchTuner(self.stepper.engine) if autotune else None
+        self.stepper.on_regrow = self._regrown
+        self.steps = 0
+
+    def _regrown(self, engine) -> None:
+        if self.tuner is not None:
+            self.tuner.engine = engine
+            if self.tuner.best:
+                self.tuner.apply(self.tuner.best)
 
     @classmethod
-    def from_scenes(cls, scenes: List[dict], precision: str = "fp32", device: int = 0) -> "BatchedWorld":
-        return cls(scenes_batch(scenes), precision, device)
+    def from_scenes(cls, scenes: List[dict], precision: str = "fp32", device: int = 0, **kw) -> "BatchedWorld":
+        return cls(scenes_batch(scenes), precision, device, **kw)
+
+    @property
+    def engine(self):
+        return self.stepper.engine
 
     @property
     def n_worlds(self) -> int:
@@ -38,20 +56,26 @@ class BatchedWorld(object):
     def dynamic_bodies(self) -> int:
         return int(np.count_nonzero(self.batch.body_i[:, L.BI_TYPE] == L.DYNAMIC))
 
+    def set_param(self, name: str, value: int) -> None:
+        """A launch-shape or debugging parameter of the engine (Engine.set_param)."""
+        self.stepper.set_param(name, value)
+
     def step(self, dt: float = 1.0 / 60, vel_iters: int = 8, pos_iters: int = 3, n: int = 1,
              graph: bool = True) -> None:
-        if n == 1 and not graph:
-            self.engine.step(dt, vel_iters, pos_iters)
-        elif self.tuner is not None and graph:
+        """n steps of every world (World.step(dt, vel_iters, pos_iters) each)."""
+        if self.tuner is not None and graph:
             self.tuner.step_n(n, dt, vel_iters, pos_iters)
+            self.stepper.record(n, dt, vel_iters, pos_iters)
+            self.stepper.check()
         else:
-            self.engine.step_n(n, dt, vel_iters, pos_iters, graph)
+            self.stepper.step(dt, vel_iters, pos_iters, n, graph)
+        self.steps += n
 
     def sync(self) -> None:
-        self.engine.sync()
+        self.stepper.check()
 
     def state(self) -> L.BatchArrays:
-        return self.engine.download()
+        return self.stepper.download()
 
     def close(self) -> None:
-        self.engine.close()
+        self.stepper.close()

@@ -56,7 +56,14 @@ enum : int {
 enum : int { BF_AWAKE = 1, BF_ALLOW_SLEEP = 2, BF_FIXED_ROT = 4, BF_ACTIVE = 8, BF_BULLET = 16 };
 enum : int { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
 enum : int { WF_ALLOW_SLEEP = 1, WF_WARM = 2, WF_NEW_FIXTURE = 4, WF_KEEP_FORCES = 8 };
-enum : int { WERR_CAPACITY = 1, WERR_UNSUPPORTED = 2, WERR_NEWPAIRS = 4, WERR_SCHEDULE = 8 };
+enum : int {
+  WERR_CAPACITY = 1,
+  WERR_UNSUPPORTED = 2,
+  WERR_NEWPAIRS = 4,
+  WERR_SCHEDULE = 8,
+  WERR_NAN = 16,          // a body's position, angle or velocity became non-finite
+  WERR_ISLAND_SIZE = 32,  // an island of more than 65535 contacts (16-bit island-local indices)
+};
 
 constexpr int kBlock = 256;
 constexpr int kBigPairsBlock = 1024;  // k_pairs workgroup for the largest worlds
@@ -132,16 +139,22 @@ struct PV {
   double cx, cy, a, vx, vy, w;
 };
 
-// Velocity-constraint point: FP32 (or FP64) geometry and effective masses,
-// FP64 accumulated impulses.
+// Velocity-constraint point: FP64 lever arms and accumulated impulses, T
+// effective masses and bias.  The lever arms stay FP64 in the FP32 pipeline:
+// in statically indeterminate piles the load split after the velocity
+// iterations moves with the two points' lever arms (through K), and FP32 arms
+// alone put the one-step impulse error near 1e-5 of m g dt; rounding the
+// masses, effective masses or bias to FP32 moves it by < 1e-6.
 template <typename T>
 struct VCP {
-  T rax, ray, rbx, rby, nm, tm, bias, pad;
+  double rax, ray, rbx, rby;
+  T nm, tm, bias, pad;
   double ni, ti;
 };
 template <typename T>
 struct VC {
-  T nx, ny, friction, restitution, ma, mb, Ia, Ib;
+  double nx, ny;  // world normal, FP64 for the same reason as the lever arms
+  T friction, restitution, ma, mb, Ia, Ib;
   double k11, k12, k22, n11, n12, n21, n22;  // K and K^-1 (n11 = c1x, n21 = c1y, n12 = c2x, n22 = c2y)
   int ia, ib;
   int rka, rkb;  // team path: rank | degree << 16 of the contact in body A's (B's) island sequence
@@ -218,7 +231,7 @@ struct Dev {
   V2<T>* c_mat;
   V4<T>* c_lnlp;
   V4<T>* c_pts;
-  V4<T>* c_imp;
+  V4<double>* c_imp;  // accumulated impulses: FP64 in both pipelines (warm-start values)
   uint2* c_id;
   // islands
   int* i_bodies;     // per-world segment at w_body_off[w]
@@ -334,7 +347,8 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
   Man<double> m;
   m.type = (st >> CS_MTYPE_SHIFT) & 3;
   m.pc = (st >> CS_PC_SHIFT) & 3;
-  V4<T> lnlp = d.c_lnlp[s], pts = d.c_pts[s], imp = d.c_imp[s];
+  V4<T> lnlp = d.c_lnlp[s], pts = d.c_pts[s];
+  const V4<double> imp = d.c_imp[s];
   uint2 ids = d.c_id[s];
   m.ln = v2<double>(lnlp.x, lnlp.y);
   m.lp = v2<double>(lnlp.z, lnlp.w);
@@ -358,7 +372,7 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
     return st;
   }
   // warm start: match ids (the dict keeps the last duplicate)
-  T n0 = 0, t0 = 0, n1 = 0, t1 = 0;
+  double n0 = 0, t0 = 0, n1 = 0, t1 = 0;
   if (m.pc >= 1) {
     if (old_pc >= 1 && oid0 == m.id0) { n0 = imp.x; t0 = imp.y; }
     if (old_pc >= 2 && oid1 == m.id0) { n0 = imp.z; t0 = imp.w; }
@@ -367,7 +381,7 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
     if (old_pc >= 1 && oid0 == m.id1) { n1 = imp.x; t1 = imp.y; }
     if (old_pc >= 2 && oid1 == m.id1) { n1 = imp.z; t1 = imp.w; }
   }
-  V4<T> nimp;
+  V4<double> nimp;
   nimp.x = m.pc >= 1 ? n0 : imp.x;
   nimp.y = m.pc >= 1 ? t0 : imp.y;
   nimp.z = m.pc >= 2 ? n1 : imp.z;
@@ -387,8 +401,9 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
   return nst;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_collide(Dev<T> d, int chunks) {
+template <typename T, int B>
+__global__ void __launch_bounds__(B) k_collide(Dev<T> d, int chunks) {
+  constexpr int kBlock = B;  // contact slots per workgroup (B2G_PARAM_COLLIDE_BLOCK)
   // Workgroups are dealt to the 8 XCDs round-robin; all chunks of world w get
   // workgroup ids = w (mod 8), so a world's fixtures and bodies are fetched
   // into one XCD's L2 only.
@@ -1324,7 +1339,7 @@ __device__ bool position_sweep(const Dev<T>& d, int bstart, int cstart, int nci)
 
 // Per-island constants.
 struct IslandK {
-  int b0, f0, cbase, wflags, pvb;
+  int w, b0, f0, cbase, wflags, pvb;
   double dt, dt_ratio;
   V2d grav;
   bool warm;
@@ -1333,6 +1348,7 @@ struct IslandK {
 template <typename T>
 __device__ __forceinline__ IslandK island_k(const Dev<T>& d, int w, int bstart) {
   IslandK c;
+  c.w = w;
   c.b0 = d.w_body_off[w];
   c.f0 = d.w_fix_off[w];
   c.cbase = d.w_con_base[w];
@@ -1391,7 +1407,8 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
   const int ba = b0 + (int)(fbx.y & 0xFFFFu), bb = b0 + (int)(fbx.y >> 16);
   const int st = d.c_state[gs_];
   const int mtype = (st >> CS_MTYPE_SHIFT) & 3, pc = (st >> CS_PC_SHIFT) & 3;
-  V4<T> lnlp = d.c_lnlp[gs_], pts = d.c_pts[gs_], imp = d.c_imp[gs_];
+  V4<T> lnlp = d.c_lnlp[gs_], pts = d.c_pts[gs_];
+  const V4<double> imp = d.c_imp[gs_];
   V2<T> mat = d.c_mat[gs_];
   VC<T> vc;
   PC<T> pcn;
@@ -1437,10 +1454,10 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
   double wa, wb;
   load_vel(d, bstart, b0, ia, va, wa);
   load_vel(d, bstart, b0, ib, vb, wb);
-  // The constraint geometry is computed in FP64 in both pipelines and stored as
-  // T: it runs once per contact and step, and FP32 arithmetic here (rotations,
-  // world manifold, effective masses) is what would otherwise set the FP32
-  // pipeline's one-step error; rounding the results to FP32 is harmless.
+  // The constraint geometry is computed in FP64 in both pipelines: it runs once
+  // per contact and step, and FP32 arithmetic here (rotations, world manifold,
+  // effective masses) would set the FP32 pipeline's one-step error.  Normal and
+  // lever arms are stored in FP64 (VCP), effective masses and bias as T.
   const Rot<double> qa = rot_of<double>(aa), qb = rot_of<double>(ab);
   const V2d dcb = v2<double>(cbx - cax, cby - cay);
   XF<double> xfa, xfb;
@@ -1451,8 +1468,8 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
   V2d w0 = v2<double>(0, 0), w1 = v2<double>(0, 0);
   const V2d normal = world_manifold<double>(mtype, pc, dv2(lnlp.x, lnlp.y), dv2(lnlp.z, lnlp.w), dv2(pts.x, pts.y),
                                             dv2(pts.z, pts.w), xfa, d.f_rad64[fa], xfb, d.f_rad64[fb], w0, w1);
-  vc.nx = (T)normal.x;
-  vc.ny = (T)normal.y;
+  vc.nx = normal.x;
+  vc.ny = normal.y;
   const T ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
   const V2d nd = dv2(vc.nx, vc.ny);
   auto init_point = [&](VCP<T>& p, V2d wp) {
@@ -1463,14 +1480,14 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
     V2d tangent = cross_vs(normal, 1.0);
     double rta = sq(cross(ra, tangent)), rtb = sq(cross(rb, tangent));
     double kt = mad + mbd + Iad * rta + Ibd * rtb;
-    p.rax = (T)ra.x; p.ray = (T)ra.y; p.rbx = (T)rb.x; p.rby = (T)rb.y;
+    p.rax = ra.x; p.ray = ra.y; p.rbx = rb.x; p.rby = rb.y;
     double vrel = dot(nd, vb + cross_sv(wb, dv2(p.rbx, p.rby)) - va - cross_sv(wa, dv2(p.rax, p.ray)));
     p.nm = kn > 0.0 ? (T)(1.0 / kn) : (T)0;
     p.tm = kt > 0.0 ? (T)(1.0 / kt) : (T)0;
     p.bias = vrel < -VELOCITY_THRESHOLD ? (T)(-(double)vc.restitution * vrel) : (T)0;
     p.pad = 0;
   };
-  vc.p[1].rax = vc.p[1].ray = vc.p[1].rbx = vc.p[1].rby = (T)0;
+  vc.p[1].rax = vc.p[1].ray = vc.p[1].rbx = vc.p[1].rby = 0.0;
   vc.p[1].nm = vc.p[1].tm = vc.p[1].bias = vc.p[1].pad = (T)0;
   if (pc >= 1) init_point(vc.p[0], w0);
   if (pc >= 2) init_point(vc.p[1], w1);
@@ -1504,9 +1521,9 @@ __device__ __forceinline__ void store_impulse(const Dev<T>& d, const IslandK& c,
   const int gs_ = c.cbase + d.i_contacts[cstart + k];
   const int pc = (d.c_state[gs_] >> CS_PC_SHIFT) & 3;
   const int n = pc < vc.npts ? pc : vc.npts;
-  V4<T> imp = d.c_imp[gs_];
-  if (n >= 1) { imp.x = (T)vc.p[0].ni; imp.y = (T)vc.p[0].ti; }
-  if (n >= 2) { imp.z = (T)vc.p[1].ni; imp.w = (T)vc.p[1].ti; }
+  V4<double> imp = d.c_imp[gs_];
+  if (n >= 1) { imp.x = vc.p[0].ni; imp.y = vc.p[0].ti; }
+  if (n >= 2) { imp.z = vc.p[1].ni; imp.w = vc.p[1].ti; }
   d.c_imp[gs_] = imp;
 }
 
@@ -1546,6 +1563,10 @@ __device__ __forceinline__ void write_back(const Dev<T>& d, const IslandK& c, in
   sincos(p.a, &sn, &cs);
   double2 lc = d.b_lc[gb];
   d.b_xf[gb] = make_double4(p.cx - (cs * lc.x + (-sn) * lc.y), p.cy - (sn * lc.x + cs * lc.y), cs, sn);
+  // non-finite state is reported (B2G_E_NAN at the next sync); the step's kernels still end
+  // normally, since every loop on this path is bounded by counts, not by values
+  if (!(isfinite(p.cx) && isfinite(p.cy) && isfinite(p.a) && isfinite(p.vx) && isfinite(p.vy) && isfinite(p.w)))
+    atomicOr(&d.w_err[c.w], WERR_NAN);
 }
 
 // 9. sleep timer of island body i (island.py:161-179, quirk Q7: the stale
@@ -2058,7 +2079,7 @@ __device__ __forceinline__ void create_contact(const Dev<T>& d, int f0, int s, i
   d.c_mat[s] = v2<T>(sqrt(ma.x * mb.x), pmax(ma.y, mb.y));
   d.c_lnlp[s] = V4<T>{0, 0, 0, 0};
   d.c_pts[s] = V4<T>{0, 0, 0, 0};
-  d.c_imp[s] = V4<T>{0, 0, 0, 0};
+  d.c_imp[s] = V4<double>{0, 0, 0, 0};
   d.c_id[s] = make_uint2(0, 0);
 }
 
@@ -2230,7 +2251,8 @@ __global__ void __launch_bounds__(B) k_pairs(Dev<T> d, int w_first, int from_upl
         int2 pr = make_int2(0, 0);
         int st = 0;
         V2<T> mat = v2<T>(0, 0);
-        V4<T> lnlp{}, pts{}, imp{};
+        V4<T> lnlp{}, pts{};
+        V4<double> imp{};
         uint2 id = make_uint2(0, 0);
         if (live) {
           pr = d.c_pair[cb + sl]; st = d.c_state[cb + sl]; mat = d.c_mat[cb + sl];
@@ -2739,8 +2761,11 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
       S.cur_st(b, cur_b);
       S.lev_st(b, lev_b);
     }
-    if (bad || nbi != cap_b || nci != cap_c) {  // the component sizes and the search disagree
-      atomicOr(&d.w_err[w], WERR_SCHEDULE);
+    // Island-local contact indices and level keys are 16-bit (i_order, the solver's ranks):
+    // an island of more than 65535 contacts is reported and not solved.
+    const bool too_big = nci > 0xFFFF;
+    if (bad || nbi != cap_b || nci != cap_c || too_big) {  // the component sizes and the search disagree
+      atomicOr(&d.w_err[w], too_big && !bad && nbi == cap_b && nci == cap_c ? WERR_ISLAND_SIZE : WERR_SCHEDULE);
       for (int q = nci; q < cap_c; ++q) isl_c[cstart + q] = 0;  // keep later passes in range
       for (int q = nbi; q < cap_b; ++q) isl_b[bstart + q] = s0;
       nbi = nci = 0;
@@ -3018,7 +3043,8 @@ __global__ void __launch_bounds__(kBigBlock) k_pairs_big_finish(Dev<T> d, int fi
         int2 pr = make_int2(0, 0);
         int st = 0;
         V2<T> mat = v2<T>(0, 0);
-        V4<T> lnlp{}, pts{}, imp{};
+        V4<T> lnlp{}, pts{};
+        V4<double> imp{};
         uint2 id = make_uint2(0, 0);
         if (live) {
           pr = d.c_pair[cb + sl]; st = d.c_state[cb + sl]; mat = d.c_mat[cb + sl];
@@ -3261,6 +3287,9 @@ struct b2g_ctx {
   int team_full_class = 9;
   int team_lds = 24576;
   int team_max = 32;
+  int pairs_wide = kPairsWideMaxPerCu;   // B2G_PARAM_PAIRS_WIDE
+  int islands_wide = kIslandsWidePerCu;  // B2G_PARAM_ISLANDS_WIDE
+  int collide_block = kBlock;            // B2G_PARAM_COLLIDE_BLOCK
   int solve_grid = 16384;
   hipStream_t stream = nullptr;
   std::string err;
@@ -3543,7 +3572,8 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   std::vector<int2> c_pair(ncap, make_int2(0, 0));
   std::vector<int> c_state(ncap, 0);
   std::vector<V2<T>> c_mat(ncap);
-  std::vector<V4<T>> c_lnlp(ncap), c_pts(ncap), c_imp(ncap);
+  std::vector<V4<T>> c_lnlp(ncap), c_pts(ncap);
+  std::vector<V4<double>> c_imp(ncap);
   std::vector<uint2> c_id(ncap, make_uint2(0, 0));
   for (int w = 0; w < nw; ++w) {
     const int wnf = b->fix_off[w + 1] - b->fix_off[w];
@@ -3562,7 +3592,7 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
       c_mat[s] = v2<T>((T)r[B2G_CD_FRICTION], (T)r[B2G_CD_RESTITUTION]);
       c_lnlp[s] = V4<T>{(T)r[B2G_CD_LN_X], (T)r[B2G_CD_LN_Y], (T)r[B2G_CD_LP_X], (T)r[B2G_CD_LP_Y]};
       c_pts[s] = V4<T>{(T)r[B2G_CD_P0_X], (T)r[B2G_CD_P0_Y], (T)r[B2G_CD_P1_X], (T)r[B2G_CD_P1_Y]};
-      c_imp[s] = V4<T>{(T)r[B2G_CD_P0_NI], (T)r[B2G_CD_P0_TI], (T)r[B2G_CD_P1_NI], (T)r[B2G_CD_P1_TI]};
+      c_imp[s] = V4<double>{r[B2G_CD_P0_NI], r[B2G_CD_P0_TI], r[B2G_CD_P1_NI], r[B2G_CD_P1_TI]};
       c_id[s] = make_uint2((uint32_t)ri[B2G_CI_ID0], (uint32_t)ri[B2G_CI_ID1]);
     }
   }
@@ -3803,7 +3833,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   };
   auto pairs = [&](int from_upload, int finish, int do_pairs) -> int {
     TRY(fan(ctx->pair_runs, [&](const WorldRun& r, hipStream_t s) {
-      if (r.smem > 0 && (size_t)kMaxLds / r.smem <= (size_t)kPairsWideMaxPerCu)
+      if (r.smem > 0 && (size_t)kMaxLds / r.smem <= (size_t)ctx->pairs_wide)
         hipLaunchKernelGGL((k_pairs<T, kBigPairsBlock>), dim3(r.count), dim3(kBigPairsBlock), r.smem, s, d, r.first,
                            from_upload, finish, do_pairs);
       else
@@ -3820,13 +3850,19 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   if (first_pairs) TRY(pairs(1, 0, 1));
   if (timed) HIPCHK(hipEventRecord(ev[0], st));
   hipLaunchKernelGGL(k_reset_queue<T>, dim3(1), dim3(64), 0, st, d);
-  const int chunks = (ctx->max_ccap + kBlock - 1) / kBlock;
-  const int nch = std::max(chunks, 1);
-  hipLaunchKernelGGL(k_collide<T>, dim3((unsigned)(((nw + 7) / 8) * 8 * nch)), dim3(kBlock), 0, st, d, nch);
+  {
+    const int cbk = ctx->collide_block;
+    const int nch = std::max((ctx->max_ccap + cbk - 1) / cbk, 1);
+    const dim3 grid((unsigned)(((nw + 7) / 8) * 8 * nch));
+    if (cbk == 64) hipLaunchKernelGGL((k_collide<T, 64>), grid, dim3(64), 0, st, d, nch);
+    else if (cbk == 128) hipLaunchKernelGGL((k_collide<T, 128>), grid, dim3(128), 0, st, d, nch);
+    else if (cbk == 512) hipLaunchKernelGGL((k_collide<T, 512>), grid, dim3(512), 0, st, d, nch);
+    else hipLaunchKernelGGL((k_collide<T, 256>), grid, dim3(256), 0, st, d, nch);
+  }
   if (timed) HIPCHK(hipEventRecord(ev[1], st));
   if (dt > 0.0) {
     TRY(fan(ctx->isl_runs, [&](const WorldRun& r, hipStream_t s) {
-      if (r.smem > 0 && (size_t)kMaxLds / r.smem <= (size_t)kIslandsWidePerCu)
+      if (r.smem > 0 && (size_t)kMaxLds / r.smem <= (size_t)ctx->islands_wide)
         hipLaunchKernelGGL((k_islands<T, kBlock>), dim3(r.count), dim3(kBlock), r.smem, s, d, r.first);
       else
         hipLaunchKernelGGL((k_islands<T, kBlock / 2>), dim3(r.count), dim3(kBlock / 2), r.smem, s, d, r.first);
@@ -3920,7 +3956,8 @@ static int download_batch(b2g_ctx* ctx, b2g_batch_out* o) {
   std::vector<int2> c_pair(ncap);
   std::vector<int> c_state(ncap);
   std::vector<V2<T>> c_mat(ncap);
-  std::vector<V4<T>> c_lnlp(ncap), c_pts(ncap), c_imp(ncap);
+  std::vector<V4<T>> c_lnlp(ncap), c_pts(ncap);
+  std::vector<V4<double>> c_imp(ncap);
   std::vector<uint2> c_id(ncap);
   TRY(download(ctx, c_pair.data(), d.c_pair, ncap));
   TRY(download(ctx, c_state.data(), d.c_state, ncap));
@@ -4144,6 +4181,9 @@ B2G_API int b2g_sync(b2g_ctx* ctx) {
   if (e & WERR_CAPACITY) return set_error(ctx, B2G_E_CAPACITY, "contact segment capacity exceeded");
   if (e & WERR_NEWPAIRS) return set_error(ctx, B2G_E_CAPACITY, "too many new pairs in one step");
   if (e & WERR_UNSUPPORTED) return set_error(ctx, B2G_E_UNSUPPORTED, "unsupported shape pair");
+  if (e & WERR_NAN) return set_error(ctx, B2G_E_NAN, "non-finite body position, angle or velocity");
+  if (e & WERR_ISLAND_SIZE)
+    return set_error(ctx, B2G_E_UNSUPPORTED, "an island of more than 65535 contacts (not solved)");
   if (e & WERR_SCHEDULE) return set_error(ctx, B2G_E_HIP, "island solver pass limit reached (internal error)");
   return B2G_OK;
 }
@@ -4195,12 +4235,17 @@ B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value) {
     case B2G_PARAM_SOLVE_GRID: slot = &ctx->solve_grid; lo = 1; hi = 1 << 30; break;
     case B2G_PARAM_ISLAND_TIMES: slot = &ctx->island_times; lo = 0; hi = 1; break;
     case B2G_PARAM_TEAM_MAX: slot = &ctx->team_max; lo = 8; hi = kSolveBlock; break;
+    case B2G_PARAM_PAIRS_WIDE: slot = &ctx->pairs_wide; lo = 0; hi = 32; break;
+    case B2G_PARAM_ISLANDS_WIDE: slot = &ctx->islands_wide; lo = 0; hi = 32; break;
+    case B2G_PARAM_COLLIDE_BLOCK: slot = &ctx->collide_block; lo = 64; hi = 512; break;
     case B2G_PARAM_SOLVE_WAVES_PER_CU:
       return set_error(ctx, B2G_E_INVALID, "b2g_set_param: B2G_PARAM_SOLVE_WAVES_PER_CU is read-only");
     case B2G_PARAM_SEARCH_RUNS:
       return set_error(ctx, B2G_E_INVALID, "b2g_set_param: B2G_PARAM_SEARCH_RUNS is read-only");
     default: return set_error(ctx, B2G_E_INVALID, "b2g_set_param: unknown key " + std::to_string(key));
   }
+  if (key == B2G_PARAM_COLLIDE_BLOCK && (value & (value - 1)) != 0)
+    return set_error(ctx, B2G_E_INVALID, "b2g_set_param: the collide workgroup must be 64, 128, 256 or 512 slots");
   if (key == B2G_PARAM_TEAM_MAX && (value & (value - 1)) != 0)
     return set_error(ctx, B2G_E_INVALID, "b2g_set_param: the largest team must be 8, 16, 32 or 64 lanes");
   if (key == B2G_PARAM_TEAM_LDS && value % 256 != 0)
@@ -4229,6 +4274,9 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value) {
     case B2G_PARAM_SOLVE_GRID: *value = ctx->solve_grid; return B2G_OK;
     case B2G_PARAM_ISLAND_TIMES: *value = ctx->island_times; return B2G_OK;
     case B2G_PARAM_TEAM_MAX: *value = ctx->team_max; return B2G_OK;
+    case B2G_PARAM_PAIRS_WIDE: *value = ctx->pairs_wide; return B2G_OK;
+    case B2G_PARAM_ISLANDS_WIDE: *value = ctx->islands_wide; return B2G_OK;
+    case B2G_PARAM_COLLIDE_BLOCK: *value = ctx->collide_block; return B2G_OK;
     case B2G_PARAM_SEARCH_RUNS: *value = (int)ctx->isl_runs.size() | (int)ctx->pair_runs.size() << 8; return B2G_OK;
     case B2G_PARAM_SOLVE_WAVES_PER_CU: {
       // resident solver wavefronts per compute unit: 4 SIMDs x (512 VGPRs / VGPRs per lane), and

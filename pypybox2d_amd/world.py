@@ -156,7 +156,8 @@ def fat_aabb(shape, xf):
 class World(object):
     def __init__(self, gravity=(0, -10), do_sleep=True, warm_starting=True, continuous_physics=True,
                  sub_stepping=False, auto_clear_forces=True, joint_destruction_listener=None,
-                 fixture_destruction_listener=None, contact_filter=None, precision="fp64", device=0):
+                 fixture_destruction_listener=None, contact_filter=None, precision="fp64", device=0,
+                 contact_factor=8.0):
         self._gravity = Vec2(*gravity)
         self.bodies: List[Body] = []
         self.joints = []
@@ -175,7 +176,8 @@ class World(object):
         self._locked = False
         self._precision = precision
         self._device = device
-        self._engine = None
+        self._contact_factor = contact_factor
+        self._stepper = None        # the device batch (a Stepper: grows its contact slots when needed)
         self._device_ahead = False  # device holds state newer than the Body objects
         self._host_dirty = True     # Body objects changed since the last upload
         self._move: List = []       # broad-phase move buffer (fixtures)
@@ -383,18 +385,18 @@ class World(object):
                 check_device_shape(f._shape)
 
     def _push(self):
-        from ._native import Engine
+        from .stepper import Stepper
         self._check_supported()
-        if self._engine is None:
-            self._engine = Engine(self._precision, self._device)
-        self._engine.upload(self._batch())
+        if self._stepper is None:
+            self._stepper = Stepper(self._precision, self._device, self._contact_factor)
+        self._stepper.upload(self._batch())
         self._host_dirty = False
 
     def _pull(self):
         """Copy the device state into the Body objects (after a step)."""
         if not self._device_ahead:
             return
-        st = self._engine.download()
+        st = self._stepper.download()
         self._device_ahead = False
         fixtures = self._fixture_list()
         for k, b in enumerate(self.bodies):
@@ -424,12 +426,14 @@ class World(object):
         """One World.step (world.py:307-354) on the GPU."""
         if self._locked:
             raise LockedError
-        if self._host_dirty or self._engine is None:
+        if self._host_dirty or self._stepper is None:
             self._pull()
             self._push()
         self._locked = True
         try:
-            self._engine.step(float(dt), int(vel_iters), int(pos_iters))
+            # steps, then checks the device's error flags: a capacity error grows the contact
+            # slots and replays from the last synchronised state; non-finite state raises
+            self._stepper.step(float(dt), int(vel_iters), int(pos_iters))
             self._device_ahead = True
             self._new_fixture = False
             if dt > 0.0:

@@ -74,3 +74,23 @@ def contact_errors(gpu: L.BatchArrays, ref: L.BatchArrays):
 
 def same_contact_lists(a: L.BatchArrays, b: L.BatchArrays) -> bool:
     return contact_keys(a) == contact_keys(b)
+
+
+QUANTITIES = ("position", "angle", "velocity", "angular_velocity")
+IMPULSES = ("normal_impulse", "tangent_impulse")
+
+
+def assert_one_step(got: L.BatchArrays, ref: L.BatchArrays, tol: float, what=""):
+    """The one-step bar: equal contact lists, no contact whose discrete state (point count,
+    manifold type, touching/enabled) differs, and every body quantity and accumulated impulse
+    within `tol` relative (floors as in the module docstring).  Returns both error dicts."""
+    assert same_contact_lists(got, ref), ("contact lists differ", what)
+    be, ce = body_errors(got, ref), contact_errors(got, ref)
+    print("  one step", what, {k: "%.2e" % be[k] for k in QUANTITIES},
+          {k: "%.2e" % ce[k] for k in IMPULSES}, "matched", ce["matched_contacts"], flush=True)
+    assert ce["discrete_flips"] == 0, (what, ce)
+    for q in QUANTITIES:
+        assert be[q] <= tol, (what, q, be[q])
+    for q in IMPULSES:
+        assert ce[q] <= tol, (what, q, ce[q])
+    return be, ce

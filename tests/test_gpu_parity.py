@@ -6,8 +6,9 @@ Bars:
     FP32 (FP64 geometry and evaluation, FP32 manifold storage): integer fields
     exact, floats within 2e-6;
   * one step from an identical state: relative error (floors: 1 m, 1 rad,
-    |g| dt) <= 1e-10 in FP64 and <= 1e-5 in FP32 for positions, angles and
-    velocities; contact lists identical;
+    |g| dt, m_min |g| dt) <= 1e-10 in FP64 and <= 1e-5 in FP32 for positions,
+    angles, velocities and accumulated normal and tangent impulses; contact
+    lists and every contact's discrete state identical;
   * graph replay equals eager stepping bit for bit.
 """
 import os
@@ -112,12 +113,7 @@ def test_one_step_from_reference_states(run, prec, tol):
         print("one step", run, "from checkpoint", s, prec, flush=True)
         ref = oracle_step(b)
         got = gpu_step(b, prec)
-        assert PU.same_contact_lists(got, ref), (run, s)
-        be = PU.body_errors(got, ref)
-        ce = PU.contact_errors(got, ref)
-        print("  body", be, "contacts", ce, flush=True)
-        for q in ("position", "angle", "velocity", "angular_velocity"):
-            assert be[q] <= tol, (run, s, q, be[q])
+        PU.assert_one_step(got, ref, tol, (run, s))
         assert np.array_equal(got.body_i[:, :2], ref.body_i[:, :2]), (run, s)
 
 
@@ -137,10 +133,7 @@ def test_one_step_from_a_near_tie_state(prec, tol):
     assert np.array_equal(got.con_i[:, cols], ref.con_i[:, cols])
     k = np.nonzero((ref.con_i[:, L.CI_FA] == 229) & (ref.con_i[:, L.CI_FB] == 290))[0][0]
     assert ref.con_i[k, L.CI_PC] == 1
-    be = PU.body_errors(got, ref)
-    print("  body", be, flush=True)
-    for q in ("position", "angle", "velocity", "angular_velocity"):
-        assert be[q] <= tol, (q, be[q])
+    PU.assert_one_step(got, ref, tol, "near tie")
 
 
 def test_first_step_from_construction_fp64():
@@ -180,13 +173,7 @@ def test_batched_mixed_worlds_one_step(prec):
     settled.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
     ref = oracle_step(settled)
     got = gpu_step(settled, prec)
-    assert PU.same_contact_lists(got, ref)
-    be = PU.body_errors(got, ref)
-    ce = PU.contact_errors(got, ref)
-    print(prec, "mixed batch one step", be, ce, flush=True)
-    tol = 1e-10 if prec == "fp64" else 1e-5
-    for q in ("position", "angle", "velocity", "angular_velocity"):
-        assert be[q] <= tol, (q, be[q])
+    PU.assert_one_step(got, ref, 1e-10 if prec == "fp64" else 1e-5, (prec, "mixed batch"))
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -206,12 +193,38 @@ def test_lognormal_batch_worlds_one_step(prec):
     state.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
     ref = oracle_step(state)
     got = gpu_step(state, prec)
-    assert PU.same_contact_lists(got, ref)
-    be = PU.body_errors(got, ref)
-    print(prec, "lognormal worlds one step", be, flush=True)
-    tol = 1e-10 if prec == "fp64" else 1e-5
-    for q in ("position", "angle", "velocity", "angular_velocity"):
-        assert be[q] <= tol, (q, be[q])
+    PU.assert_one_step(got, ref, 1e-10 if prec == "fp64" else 1e-5, (prec, "lognormal worlds"))
+
+
+def settled_mixed_sample(steps=1000, n=64):
+    """The worlds that bench.py samples from the 4096-world mixed batch (seed 1), stepped `steps`
+    times on the GPU's FP32 pipeline (graph-captured) so that piles have formed and settled."""
+    counts = scenes.mixed_batch_counts(4096, 1)
+    worlds = sorted(np.random.default_rng(0).choice(4096, size=n, replace=False).tolist())
+    batch = builder.scenes_batch([scenes.batch_world(1, w, int(counts[w])) for w in worlds])
+    e = N.Engine("fp32")
+    e.upload(batch)
+    e.step_n(steps, DT, 8, 3, graph=True)
+    state = e.download()
+    e.close()
+    state.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    return state
+
+
+def test_settled_mixed_batch_sample_one_step():
+    """64 worlds of the mixed batch after 1000 steps (about 18 000 bodies, most of them in piles):
+    one step of each pipeline from that state against the oracle. Both pipelines keep the
+    oracle's pair lists and discrete contact states; FP32 holds every body quantity and every
+    accumulated normal and tangent impulse within 1e-5 relative, FP64 within 1e-10."""
+    state = settled_mixed_sample()
+    touching = int(np.count_nonzero(state.con_i[:, L.CI_FLAGS] & L.CF_TOUCHING))
+    print("settled sample: bodies", state.body_d.shape[0], "contacts", state.con_d.shape[0], "touching", touching,
+          flush=True)
+    assert touching > state.body_d.shape[0]  # piles: more touching contacts than bodies
+    ref = oracle_step(state)
+    for prec, tol in (("fp64", 1e-10), ("fp32", 1e-5)):
+        got = gpu_step(state, prec)
+        PU.assert_one_step(got, ref, tol, (prec, "settled mixed batch"))
 
 
 def test_ragged_batch_with_empty_and_fixtureless_worlds():
@@ -243,10 +256,7 @@ def test_ragged_batch_with_empty_and_fixtureless_worlds():
     assert max(be[q] for q in ("position", "angle", "velocity", "angular_velocity")) <= 1e-9
     ref.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
     got32, ref1 = gpu_step(ref, "fp32"), oracle_step(ref)
-    assert PU.same_contact_lists(got32, ref1)
-    be = PU.body_errors(got32, ref1)
-    print("ragged fp32, one step", be, flush=True)
-    assert max(be[q] for q in ("position", "angle", "velocity", "angular_velocity")) <= 1e-5
+    PU.assert_one_step(got32, ref1, 1e-5, "ragged fp32")
 
 
 def test_graph_replay_matches_eager():
@@ -286,7 +296,9 @@ def test_island_launch_shapes_agree_bit_exactly(prec):
                          ("teams8", {"heavy_class": 0, "team_full_class": 16}), ("default", {}),
                          ("fallback", {"heavy_class": 3, "team_lds": 4096}), ("large_worlds", {"large_worlds": 1}),
                          ("pairs", {"heavy_class": 5, "team_full_class": 9, "team_max": 32, "team_lds": 24576}),
-                         ("whole_wave", {"heavy_class": 3, "team_full_class": 9, "team_max": 16, "team_lds": 8192})):
+                         ("whole_wave", {"heavy_class": 3, "team_full_class": 9, "team_max": 16, "team_lds": 8192}),
+                         ("narrow_searches", {"collide_block": 64, "islands_wide": 0, "pairs_wide": 0}),
+                         ("wide_searches", {"collide_block": 512, "islands_wide": 32, "pairs_wide": 32})):
         e = N.Engine(prec)
         for k, v in params.items():
             e.set_param(k, v)
@@ -321,12 +333,7 @@ def test_worlds_of_many_sizes_launch_in_runs(prec):
     state.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
     ref = oracle_step(state)
     got = gpu_step(state, prec)
-    assert PU.same_contact_lists(got, ref)
-    be = PU.body_errors(got, ref)
-    print(prec, "worlds in runs, one step", be, flush=True)
-    tol = 1e-10 if prec == "fp64" else 1e-5
-    for q in ("position", "angle", "velocity", "angular_velocity"):
-        assert be[q] <= tol, (q, be[q])
+    PU.assert_one_step(got, ref, 1e-10 if prec == "fp64" else 1e-5, (prec, "worlds in runs"))
     results = {}
     for name, params, runs in (("runs", {}, 4 | 4 << 8), ("large_worlds", {"large_worlds": 1}, 1 | 1 << 8)):
         e = N.Engine(prec)
@@ -350,7 +357,8 @@ def test_set_param_rejects_bad_values():
     assert e.get_param("heavy_class") >= 0
     for k, v in (("heavy_class", 17), ("team_lds", 200000), ("team_lds", 1000), ("solve_grid", 0),
                  ("team_full_class", -1), ("team_max", 24), ("team_max", 4), ("team_max", 128),
-                 ("search_runs", 1), ("solve_waves_per_cu", 8)):
+                 ("search_runs", 1), ("solve_waves_per_cu", 8), ("collide_block", 96), ("collide_block", 1024),
+                 ("collide_block", 32), ("pairs_wide", 33), ("islands_wide", -1)):
         with pytest.raises(N.B2GError):
             e.set_param(k, v)
     e.close()
@@ -379,11 +387,7 @@ def test_restitution_trajectory_fp64_and_one_step_fp32():
     ref.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
     one = oracle_step(ref)
     g32 = gpu_step(ref, "fp32")
-    be = PU.body_errors(g32, one)
-    print("restitution fp32 one step", be, flush=True)
-    assert PU.same_contact_lists(g32, one)
-    for q in ("position", "angle", "velocity", "angular_velocity"):
-        assert be[q] <= 1e-5, (q, be[q])
+    PU.assert_one_step(g32, one, 1e-5, "restitution fp32")
 
 
 def test_drop_in_world_api_tracks_oracle():
@@ -487,11 +491,7 @@ def test_single_large_world_200_pyramids():
     ref.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
     one = oracle_step(ref)
     g32 = gpu_step(ref, "fp32")
-    assert PU.same_contact_lists(g32, one)
-    be = PU.body_errors(g32, one)
-    print("200 pyramids fp32 one step", be, flush=True)
-    for q in ("position", "angle", "velocity", "angular_velocity"):
-        assert be[q] <= 1e-5, (q, be[q])
+    PU.assert_one_step(g32, one, 1e-5, "200 pyramids fp32")
 
 
 def test_single_world_with_body_indices_above_32767():
