@@ -6,16 +6,23 @@ Workload (default): 4096 independent mixed worlds (body counts uniform in
 edge), dt = 1/60, 8 velocity and 3 position iterations, continuous physics
 off.  A step is one World.step of every world.
 
-    python bench.py [--gpus 1] [--steps 1000] [--warmup 10]
+    python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--settle 1000]
 
-Prints one JSON line: value = dynamic bodies x timed steps / wall seconds of
-the timed loop (FP32-bulk pipeline, state resident in HBM), plus the FP64
-pipeline's rate, the dominant kernel's roofline, one-step errors against the
-FP64 oracle on a sample of worlds, long-horizon statistics, and the oracle's
-rate on the host cores for the same worlds and step count.  Timed steps are
-launched with HIP events between their kernels (for the per-kernel averages);
-the rate of HIP-graph-captured steps, measured right after, is reported
-beside it.
+The worlds are first stepped --settle times (untimed, graph-captured, FP32) so
+that the timed window sees settled piles, not bodies in free fall.  From that
+state each pipeline (FP32 bulk, then all-FP64) takes one launch-tuning pass
+and --warmup graph steps, then --steps graph-captured steps are timed by the
+wall clock.  value = dynamic bodies x timed steps / seconds (FP32 pipeline,
+state resident in HBM).  The same window is then re-run eagerly from its saved
+start with HIP events between the kernels for the per-stage times (the
+pipeline is deterministic; the re-run's end state is checked to be identical).
+The line also gives the dominant stage's roofline, the whole step's
+algorithmic bytes from the device's work counters, one-step errors of both
+pipelines against the FP64 oracle on 64 worlds of the settled state, and the
+oracle's rate on the host cores over the same worlds and window.
+
+Under ``rocprofv3 --selected-regions`` only the timed window and its eager
+re-run are recorded (roctx pause/resume).
 """
 from __future__ import annotations
 
@@ -33,7 +40,6 @@ sys.path.insert(0, HERE)
 from pypybox2d_amd import builder, layout as L, scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-GRAPH_CHECK_STEPS = 100
 KERNELS = ("k_collide", "k_islands", "k_solve", "k_sync", "k_pairs")
 
 
@@ -57,87 +63,147 @@ def workload_counts(state: L.BatchArrays):
             "island_bodies": island_bodies, "geometry_bytes": int(geo.sum())}
 
 
-def algorithmic_bytes(kernel: str, c: dict, fp64: bool) -> float:
-    """Minimum HBM bytes one launch of `kernel` must move: the per-unit figures below times the
-    counts of the end state."""
+def per_step(stats: dict) -> dict:
+    """Work counters of the timed window, per step."""
+    n = max(1, stats["steps"])
+    return {k: v / n for k, v in stats.items() if k != "steps"}
+
+
+def step_bytes(c: dict, w: dict) -> float:
+    """Algorithmic bytes of one whole step by the survey's definition: per dynamic body 148 + G
+    (state read and write, sweep write, constants, geometry, fat AABB), per persistent contact
+    128 (pair, manifold read and write), per new contact 8, per re-inserted proxy 32."""
+    return (148 * c["dynamic_bodies"] + c["geometry_bytes"] + 128 * w["live_contacts"] + 8 * w["new_contacts"]
+            + 32 * w["moved_proxies"])
+
+
+def stage_bytes(stage: str, c: dict, w: dict, fp64: bool) -> float:
+    """Minimum HBM bytes of one launch of a stage's kernels, from the per-step work counters `w`
+    and the batch's counts `c`.  t = bytes of the bulk type."""
     t = 8 if fp64 else 4
-    man = 4 + 4 * 2 * t + 4 * t + 8      # state word + ln,lp,p0,p1 + 4 impulses + 2 ids
-    if kernel == "k_collide":
-        # per contact: pair + manifold read + write; per body: xf and fat AABB and geometry once
-        return c["contacts"] * (8 + 2 * man) + c["dynamic_bodies"] * (32 + 32) + c["geometry_bytes"]
-    if kernel == "k_solve":
-        # per island body: c, a, v, w read + write, c0/a0 write, constants; per touching contact:
-        # pair + manifold read, impulses write
-        return c["island_bodies"] * (48 + 48 + 24 + 4 * t + 16) + c["touching"] * (8 + man + 4 * t)
-    if kernel == "k_islands":
-        return c["contacts"] * (8 + 4) + c["bodies"] * 8 + c["touching"] * 8
-    if kernel == "k_sync":
-        return c["fixtures"] * (32 + 32) + c["island_bodies"] * (56 + 32) + c["geometry_bytes"]
-    if kernel == "k_pairs":
-        return c["contacts"] * 12 + c["fixtures"] * 4
+    man = 8 * t + 32 + 8 + 4  # local normal/point and two points (T), four FP64 impulses, two ids, state
+    geo_per_fix = c["geometry_bytes"] / max(1, c["fixtures"])
+    if stage == "k_collide":
+        # live contacts: pair + state; evaluated: manifold read and write, both fixtures' records
+        # and bodies' transforms; geometry of the fixtures once
+        return (w["live_contacts"] * (8 + 8) + w["evaluated_contacts"] * (2 * man + 2 * (16 + 32))
+                + c["geometry_bytes"])
+    if stage == "k_solve":
+        # island body: c, a, v, w read and write, c0/a0 write, inverse mass/inertia and local centre;
+        # island contact: fixture/body refs, manifold and material read, impulses read and written
+        return (w["island_bodies"] * (48 + 48 + 24 + 2 * t + 16)
+                + w["island_contacts"] * (8 + 4 + 8 * t + 2 * t + 32 + 32))
+    if stage == "k_islands":
+        return w["live_contacts"] * (8 + 4) + c["bodies"] * 8 + w["island_contacts"] * 8 + w["island_bodies"] * 8
+    if stage == "k_sync":
+        return (c["fixtures"] * (8 + 4 + 8) + w["island_bodies"] * (72 + 32 + geo_per_fix)
+                + w["moved_proxies"] * 32)
+    if stage == "k_pairs":
+        return (w["live_contacts"] * 12 + c["fixtures"] * 4 + w["moved_proxies"] * 32
+                + w["new_contacts"] * (8 + 4 + 2 * t + man))
     return 0.0
 
 
-def run_gpu(batch: L.BatchArrays, precision: str, steps: int, warmup: int, tune: bool = True,
-            solver_times: bool = True, graph_check: bool = True):
-    """Times `steps` steps and the average duration of each kernel over them.
+class Roctx(object):
+    """rocprofv3 --selected-regions markers: only the timed windows are profiled.  Without a
+    profiler attached (or without the library) the calls do nothing."""
 
-    Every timed step is launched with HIP events between its kernels on the
-    library's stream (b2g_profile_step), so the per-kernel averages cover exactly
-    the timed launches.  Every 200 steps the launch-shape tuner takes the next
-    steps, one per candidate shape (real steps, timed with the rest).
+    def __init__(self):
+        self.lib = None
+        try:
+            import ctypes
+            self.lib = ctypes.CDLL("/opt/rocm/lib/librocprofiler-sdk-roctx.so")
+            self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
+            self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
+        except OSError:
+            self.lib = None
+
+    def pause(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+
+    def resume(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+
+
+ROCTX = Roctx()
+
+
+def settle(batch: L.BatchArrays, steps: int) -> L.BatchArrays:
+    """The untimed prefix: `steps` graph-captured steps of the FP32 pipeline from the initial state,
+    so that the timed window measures piles that have formed and settled."""
+    from pypybox2d_amd._native import Engine
+    eng = Engine("fp32")
+    eng.upload(batch)
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        n = min(200, steps - done)
+        eng.step_n(n, graph=True)
+        eng.sync()
+        done += n
+        log("[settle] %d/%d steps, %.1f s" % (done, steps, time.perf_counter() - t0))
+    state = eng.download()
+    eng.close()
+    state.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    return state
+
+
+def run_gpu(start: L.BatchArrays, precision: str, steps: int, warmup: int, tune: bool = True,
+            solver_times: bool = True):
+    """Times `steps` graph-captured steps from `start` (after one tuning pass and `warmup` graph steps).
+
+    value: wall clock of the timed steps, with the state resident in HBM.  Per-stage averages: the
+    same window stepped again eagerly with HIP events between the kernels (b2g_profile_step),
+    from the state saved at the window's start; the pipeline is deterministic, so this re-run does
+    exactly the work the timed steps did (checked: its end state equals theirs bit for bit).
+    Work counters (b2g_stats) are summed over the timed steps.
     """
     from pypybox2d_amd._native import Engine
     from pypybox2d_amd.autotune import LaunchTuner
     eng = Engine(precision)
-    eng.upload(batch)
-    tuner = LaunchTuner(eng) if tune else None
-    log("[%s] uploaded; warmup %d steps" % (precision, warmup))
+    eng.upload(start)
+    tuned = None
+    if tune:
+        tuner = LaunchTuner(eng)
+        tuned = dict(tuner.tune(1.0 / 60, 8, 3))
+        log("[%s] launch shape %s (solve ms per candidate %s)" % (precision, tuned,
+                                                                  np.round(tuner.last_times, 2).tolist()))
     if warmup > 0:
         eng.step_n(warmup, graph=True)
     eng.sync()
-    log("[%s] timing %d steps%s" % (precision, steps, " (launch shape tuned at the start, after 50, 50 and 100 steps, then every 200)" if tune else ""))
-    sums = np.zeros(5)
+    window = eng.download()
+    eng.stats(reset=True)
+    log("[%s] timing %d graph-captured steps" % (precision, steps))
+    ROCTX.resume()
     t0 = time.perf_counter()
-    done = 0
-    while done < steps:
-        left = steps - done
-        if tuner is not None and left > tuner.steps_per_tune:
-            tuner.tune(1.0 / 60, 8, 3)
-            sums += tuner.last_ms.sum(axis=0)
-            done += tuner.steps_per_tune
-            n = min(tuner.interval() - tuner.steps_per_tune, steps - done)
-        else:
-            n = min(200, left)
-        seg = None
-        if n > 0:
-            seg = eng.profile(n)
-            sums += seg * n
-            done += n
-        log("[%s] %d/%d steps, %.2f s%s%s" % (
-            precision, done, steps, time.perf_counter() - t0,
-            (", launch shape %s (solve ms per candidate %s)" % (tuner.best, np.round(tuner.last_times, 2).tolist()))
-            if tuner and tuner.last_times is not None else "",
-            (", solve %.2f ms over the next %d steps" % (seg[2], n)) if seg is not None else ""))
+    eng.step_n(steps, graph=True)
     eng.sync()
     elapsed = time.perf_counter() - t0
-    state = eng.download()
-    # the same launch sequence captured in a HIP graph, continuing from here
-    graph_ms = None
-    if graph_check:
-        g0 = time.perf_counter()
-        eng.step_n(GRAPH_CHECK_STEPS, graph=True)
-        eng.sync()
-        graph_ms = 1e3 * (time.perf_counter() - g0) / GRAPH_CHECK_STEPS
-    # one more (untimed) step with per-island timing, for the solver's latency report (last,
-    # since switching the timing on drops the captured graph)
+    ROCTX.pause()
+    stats = eng.stats(reset=True)
+    end = eng.download()
+    # the same window again, eager, with events between the kernels
+    eng.upload(window)
+    ROCTX.resume()
+    ms = eng.profile(steps)
+    eng.sync()
+    ROCTX.pause()
+    again = eng.download()
+    identical = bool(np.array_equal(again.body_d, end.body_d) and np.array_equal(again.con_d, end.con_d)
+                     and np.array_equal(again.con_i, end.con_i))
+    log("[%s] %.3f ms/step; stages %s ms; eager re-run identical: %s" % (
+        precision, 1e3 * elapsed / steps, np.round(ms, 3).tolist(), identical))
+    # one more (untimed) step with per-island timing, for the solver's latency report
     solver = None
     if solver_times:
         eng.set_param("island_times", 1)
         eng.profile(1)
         solver = solver_report(eng.island_times(), eng.get_param("solve_waves_per_cu"))
     eng.close()
-    return elapsed, sums / steps, state, (dict(tuner.best) if tuner and tuner.best else None), graph_ms, solver
+    return {"elapsed": elapsed, "stage_ms": ms, "state": end, "launch_shape": tuned, "solver": solver,
+            "stats": stats, "eager_rerun_identical": identical}
 
 
 def solver_report(rows: np.ndarray, waves_per_cu: int) -> dict:
@@ -259,7 +325,8 @@ _CPU_BATCH = None
 
 
 def _cpu_worker(args):
-    """Whole trajectories of worlds p, p + P, ... (the GPU's step count each) until `seconds` pass."""
+    """Worlds p, p + P, ... of the settled state, each stepped `steps` times (the GPU's timed window),
+    until `seconds` pass."""
     p, nproc, steps, seconds = args
     from oracle import oracle as O
     t0 = time.perf_counter()
@@ -276,13 +343,22 @@ def _cpu_worker(args):
     return body_steps, time.perf_counter() - t0, worlds
 
 
-def cpu_baseline(batch: L.BatchArrays, steps: int, seconds: float, procs: int):
+def host_cores() -> dict:
+    """The cores this process may use, and the host's count."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"usable": usable, "host_cpu_count": os.cpu_count()}
+
+
+def cpu_baseline(state: L.BatchArrays, steps: int, seconds: float, procs: int):
     """The FP64 oracle (oracle/b2oracle.c) on the host cores: one world per process at a time, each
-    world stepped from the same initial state for the same number of steps as the GPU run."""
+    world of the settled state stepped for the GPU's timed window."""
     import multiprocessing as mp
     global _CPU_BATCH
-    _CPU_BATCH = batch
-    procs = max(1, min(procs, batch.n_worlds))  # one world per process
+    _CPU_BATCH = state
+    procs = max(1, min(procs, state.n_worlds))  # one world per process
     jobs = [(p, procs, steps, seconds) for p in range(procs)]
     if procs > 1:
         with mp.get_context("fork").Pool(procs) as pool:
@@ -291,11 +367,14 @@ def cpu_baseline(batch: L.BatchArrays, steps: int, seconds: float, procs: int):
         res = [_cpu_worker(j) for j in jobs]
     rate = sum(bs / el for bs, el, _ in res)
     nw = sum(r[2] for r in res)
+    hc = host_cores()
     return {"value": rate, "unit": "body-steps/s", "cores": procs, "kind": "port",
-            "sample": "FP64 oracle (oracle/b2oracle.c, a C restatement of the reference's step) on %d host cores, "
-                      "one process per core, each stepping whole worlds of the same batch (worlds p, p+%d, ...) "
-                      "for %d steps from the initial state until %.0f s had passed: %d worlds in total"
-                      % (procs, procs, steps, seconds, nw)}
+            "host_cpu_count": hc["host_cpu_count"], "usable_cores": hc["usable"],
+            "sample": "FP64 oracle (oracle/b2oracle.c, a C restatement of the reference's step) in %d processes, "
+                      "one world per process at a time: worlds p, p+%d, ... of the same settled state, each "
+                      "stepped for the GPU's %d timed steps, until %.0f s had passed (%d worlds). The host has %s "
+                      "cores, %d usable by this process; the pool is held to 16, this GPU's share of the host"
+                      % (procs, procs, steps, seconds, nw, hc["host_cpu_count"], hc["usable"])}
 
 
 # the kernels of each timed stage (the stage names are those of the stage's main kernel)
@@ -304,37 +383,50 @@ STAGE_KERNELS = {"k_collide": ("k_reset_queue", "k_collide"), "k_islands": ("k_i
                  "k_pairs": ("k_pairs",)}
 
 
-def measured_traffic(kernel: str, workload: str):
-    """HBM bytes per step of a stage (the sum over its kernels and their dispatches) from the
-    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the default workload (FP32 pipeline),
-    or None."""
-    path = os.path.join(HERE, "profiles", "traffic_mixed4096_fp32.md")
-    if workload != "mixed" or not os.path.exists(path):
+def measured_traffic(stage: str, args) -> dict:
+    """HBM bytes per step of a stage (FETCH_SIZE + WRITE_SIZE over its kernels' dispatches, scaled
+    by the copy-kernel calibration) from the committed rocprofv3 --pmc summary of this same command
+    (same workload, settle prefix, warmup and timed steps), or None when there is none."""
+    path = os.path.join(HERE, "profiles", "traffic_%s.md" % workload_tag(args))
+    if not os.path.exists(path):
         return None
-    per = {}
+    per, cmd_ok = {}, False
     for line in open(path):
+        if line.startswith("window: ") and line.strip() == "window: " + window_tag(args):
+            cmd_ok = True
         if line.startswith("traffic_bytes_per_step "):
             name, val = line[len("traffic_bytes_per_step "):].split(":", 1)
             per[name.strip()] = float(val)
-    parts = STAGE_KERNELS.get(kernel, (kernel,))
-    if kernel not in per:
+    parts = STAGE_KERNELS.get(stage, (stage,))
+    if not cmd_ok or not any(k in per for k in parts):
         return None
     return sum(per.get(k, 0.0) for k in parts)
 
 
-def roofline(kernel: str, ms: float, counts: dict, fp64: bool, workload: str = "mixed") -> dict:
-    ab = algorithmic_bytes(kernel, counts, fp64)
+def workload_tag(args) -> str:
+    return {"mixed": "mixed%d" % args.worlds, "lognormal": "lognormal%d" % args.worlds,
+            "pyramids": "pyramids200"}[args.workload] + "_fp32"
+
+
+def window_tag(args) -> str:
+    return "workload=%s seed=%d settle=%d warmup=%d steps=%d" % (args.workload, args.seed, args.settle, args.warmup,
+                                                                 args.steps)
+
+
+def roofline(stage: str, ms: float, c: dict, w: dict, fp64: bool, args) -> dict:
+    ab = stage_bytes(stage, c, w, fp64)
     achieved = ab / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    out = {"bound": "hbm", "kernel": kernel, "kernels": list(STAGE_KERNELS.get(kernel, (kernel,))),
-           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, workload), "algorithmic_bytes": ab,
-           "avg_ms": ms}
-    if kernel in LATENCY_LIMITS:
-        out["limit"] = LATENCY_LIMITS[kernel]
+    traffic = measured_traffic(stage, args) if not fp64 else None
+    out = {"bound": "hbm", "kernel": stage, "kernels": list(STAGE_KERNELS.get(stage, (stage,))),
+           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+           "traffic": traffic, "algorithmic_bytes": ab, "avg_ms": ms,
+           "traffic_over_algorithmic": (traffic / ab) if traffic and ab else None}
+    if stage in LATENCY_LIMITS:
+        out["limit"] = LATENCY_LIMITS[stage]
     return out
 
 
-# Kernels whose time is set by serial dependency chains rather than by bytes or flops; their HBM
+# Stages whose time is set by serial dependency chains rather than by bytes or flops; their HBM
 # fraction is reported for completeness and the "solver" report gives the solve's own measure.
 LATENCY_LIMITS = {
     "k_solve": "dependency chains inside the largest islands: passes (about the Gauss-Seidel dependency depth "
@@ -343,40 +435,70 @@ LATENCY_LIMITS = {
 }
 
 
+def copy_calibration(mib: int = 256, reps: int = 5) -> None:
+    """A plain device-to-device copy kernel (torch's) of known size inside a selected region: what
+    rocprofv3's FETCH_SIZE and WRITE_SIZE read for it against the bytes it must move gives the
+    scale of those counters."""
+    import torch
+    ROCTX.pause()
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    ROCTX.resume()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ROCTX.pause()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_dispatch": mib << 20, "bytes_written_per_dispatch":
+                                           mib << 20, "dispatches": reps}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="default 1000 (600 for pyramids)")
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=100, help="timed steps (graph-captured)")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed graph steps before the timed window")
+    ap.add_argument("--settle", type=int, default=None,
+                    help="untimed graph steps of the FP32 pipeline from the initial state before the window "
+                         "(default 1000; 300 for pyramids)")
     ap.add_argument("--workload", choices=("mixed", "lognormal", "pyramids"), default="mixed",
                     help="mixed: 4096 worlds, body counts U[32,512]; lognormal: 16384 worlds, median 64, sigma 0.75, "
                          "clipped to [16,1024]; pyramids: one world of 200 ten-row box pyramids")
     ap.add_argument("--worlds", type=int, default=None, help="default 4096 (mixed) or 16384 (lognormal)")
     ap.add_argument("--seed", type=int, default=None, help="default 1 (mixed) or 2 (lognormal)")
-    ap.add_argument("--fp64-steps", type=int, default=None, help="timed steps of the FP64 run (default: --steps)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default solver launch shape")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--error-worlds", type=int, default=64)
     ap.add_argument("--no-solver-report", action="store_true",
                     help="skip the extra step with per-island solver timing (used for profiler runs)")
-    ap.add_argument("--no-graph-check", action="store_true",
-                    help="skip the graph-replayed steps after the timed region (used for profiler runs)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-procs", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-procs", type=int, default=None, help="default: min(16, usable cores)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB five times with a plain copy kernel (calibrates rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE against known bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
-        print("bench.py runs on one GPU only (the batch is one GPU's share; worlds would shard across GPUs "
-              "with no communication); got --gpus %d, WORLD_SIZE=%d" % (args.gpus, world_size))
+        print("bench.py runs on one GPU only: every GPU box here has one GPU, and the batch is one GPU's share "
+              "(worlds would shard across GPUs with no communication); got --gpus %d, WORLD_SIZE=%d"
+              % (args.gpus, world_size))
         sys.exit(2)
+    ROCTX.pause()  # under rocprofv3 --selected-regions only the timed windows are recorded
 
     if args.worlds is None:
         args.worlds = {"mixed": 4096, "lognormal": 16384, "pyramids": 1}[args.workload]
     if args.seed is None:
         args.seed = 1 if args.workload == "mixed" else 2
-    if args.steps is None:
-        args.steps = 600 if args.workload == "pyramids" else 1000
+    if args.settle is None:
+        args.settle = 300 if args.workload == "pyramids" else 1000
+    if args.cpu_procs is None:
+        args.cpu_procs = min(16, host_cores()["usable"])
     t0 = time.perf_counter()
     if args.workload == "pyramids":
         args.worlds = 1
@@ -396,60 +518,71 @@ def main():
         batch = builder.batch_from_counts(counts, args.seed)
     log("built in %.1f s" % (time.perf_counter() - t0))
     dyn = int(np.count_nonzero(batch.body_i[:, L.BI_TYPE] == L.DYNAMIC))
+    desc += ("; timed window: steps %d..%d after an untimed prefix of %d graph-captured FP32 steps from the "
+             "initial state (settled piles), then one launch-tuning pass (7 steps) and %d warm-up steps"
+             % (args.settle + 7 + args.warmup + 1, args.settle + 7 + args.warmup + args.steps, args.settle,
+                args.warmup))
 
-    el32, ms32, st32, tuned32, gms32, solver32 = run_gpu(batch, "fp32", args.steps, args.warmup, not args.no_tune,
-                                                         not args.no_solver_report, not args.no_graph_check)
-    value = dyn * args.steps / el32
-    log("fp32: %.3e body-steps/s (%.3f ms/step)" % (value, 1e3 * el32 / args.steps))
-    c = workload_counts(st32)
+    settled = settle(batch, args.settle) if args.settle > 0 else batch
+    c = workload_counts(settled)
+    log("settled state: %d contacts, %d touching" % (c["contacts"], c["touching"]))
+    r32 = run_gpu(settled, "fp32", args.steps, args.warmup, not args.no_tune, not args.no_solver_report)
+    value = dyn * args.steps / r32["elapsed"]
+    w32 = per_step(r32["stats"])
+    ms32 = r32["stage_ms"]
     k = int(np.argmax(ms32))
+    sb = step_bytes(c, w32)
     line = {
         "metric": "body-steps/sec, 4096 mixed worlds, FP32 vs FP64 on GPU; one-step err vs FP64",
         "value": value, "unit": "body-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * el32 / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 bulk (f64 positions/angles/velocities/impulses/AABBs)",
+        "ms_per_step": 1e3 * r32["elapsed"] / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 bulk (f64 positions/angles/velocities/impulses/lever arms/AABBs)",
         "data": "synthetic",
-        "config": {"workload": desc,
-                   "worlds": args.worlds, "dynamic_bodies": dyn, "contacts_at_end": c["contacts"],
-                   "touching_at_end": c["touching"]},
+        "config": {"workload": desc, "worlds": args.worlds, "dynamic_bodies": dyn, "settle_steps": args.settle,
+                   "contacts_at_window_start": c["contacts"], "touching_at_window_start": c["touching"]},
         "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms32)},
-        "roofline": roofline(KERNELS[k], float(ms32[k]), c, False, args.workload),
-        "roofline_other": {n: roofline(n, float(ms32[i]), c, False, args.workload) for i, n in enumerate(KERNELS)
-                           if n in ("k_collide", "k_pairs", "k_sync")},
-        "solver": solver32,
-        "launch_shape": tuned32,
-        "graph_ms_per_step_next_%d" % GRAPH_CHECK_STEPS: gms32,
-        "timing": "wall clock of the timed steps; per-kernel averages from HIP events around every timed launch",
+        "roofline": roofline(KERNELS[k], float(ms32[k]), c, w32, False, args),
+        "roofline_other": {n: roofline(n, float(ms32[i]), c, w32, False, args) for i, n in enumerate(KERNELS)
+                           if i != k},
+        "step_bytes": {"algorithmic": sb, "achieved_GBps": sb / (1e-3 * r32["elapsed"] / args.steps) / 1e9,
+                       "frac": sb / (1e-3 * r32["elapsed"] / args.steps) / 1e9 / HBM_PEAK_GBS,
+                       "per_step_counters": w32,
+                       "definition": "sum over dynamic bodies of (148 + G) + 128 C_persist + 8 P_new + 32 R_reinsert"},
+        "solver": r32["solver"],
+        "launch_shape": r32["launch_shape"],
+        "eager_rerun_identical": r32["eager_rerun_identical"],
+        "timing": "value: wall clock of the timed graph-captured steps (barrier = stream sync on both sides); "
+                  "kernel_ms: HIP events around every kernel of the same window re-run eagerly from its saved "
+                  "start state",
     }
     if not args.no_fp64:
-        n64 = args.fp64_steps or args.steps
-        el64, ms64, st64, tuned64, gms64, solver64 = run_gpu(batch, "fp64", n64, args.warmup, not args.no_tune,
-                                                             not args.no_solver_report, not args.no_graph_check)
-        v64 = dyn * n64 / el64
-        line["fp64"] = {"value": v64, "ms_per_step": 1e3 * el64 / n64, "steps": n64,
-                        "kernel_ms": {n: float(v) for n, v in zip(KERNELS, ms64)},
-                        "launch_shape": tuned64, "graph_ms_per_step_next_%d" % GRAPH_CHECK_STEPS: gms64,
-                        "solver": solver64}
+        r64 = run_gpu(settled, "fp64", args.steps, args.warmup, not args.no_tune, not args.no_solver_report)
+        v64 = dyn * args.steps / r64["elapsed"]
+        w64 = per_step(r64["stats"])
+        line["fp64"] = {"value": v64, "ms_per_step": 1e3 * r64["elapsed"] / args.steps,
+                        "kernel_ms": {n: float(v) for n, v in zip(KERNELS, r64["stage_ms"])},
+                        "launch_shape": r64["launch_shape"], "solver": r64["solver"],
+                        "roofline": roofline(KERNELS[k], float(r64["stage_ms"][k]), c, w64, True, args),
+                        "eager_rerun_identical": r64["eager_rerun_identical"]}
         line["fp32_over_fp64"] = value / v64
         log("fp64: %.3e body-steps/s; fp32/fp64 = %.2f" % (v64, value / v64))
-        if n64 == args.steps:
-            line["long_horizon"] = {"fp32": long_horizon(st32), "fp64": long_horizon(st64),
-                                    "steps": args.warmup + args.steps}
+        line["long_horizon"] = {"fp32": long_horizon(r32["state"]), "fp64": long_horizon(r64["state"]),
+                                "steps": args.settle + 7 + args.warmup + args.steps}
     if args.workload == "pyramids" and args.error_worlds > 0:
-        line["one_step_error_fp32_vs_fp64_oracle_every_50"] = sampled_errors(batch, args.warmup + args.steps, 50)
+        line["one_step_error_fp32_vs_fp64_oracle_every_50"] = sampled_errors(batch, 600, 50)
         log("sampled one-step errors:", line["one_step_error_fp32_vs_fp64_oracle_every_50"]["max"])
     rng = np.random.default_rng(0)
     worlds = sorted(rng.choice(args.worlds, size=min(args.error_worlds, args.worlds), replace=False).tolist())
-    for prec, st in ((("fp32", st32),) + ((("fp64", st64),) if not args.no_fp64 else ())) if worlds else ():
+    for prec in (("fp32",) + (() if args.no_fp64 else ("fp64",))) if worlds else ():
         key = "one_step_error_%s_vs_fp64_oracle" % prec
         try:
-            line[key] = one_step_error(st, prec, worlds)
-            line[key]["from_step"] = args.warmup + ((args.fp64_steps or args.steps) if prec == "fp64" else args.steps)
+            line[key] = one_step_error(settled, prec, worlds)
+            line[key]["from_step"] = args.settle
             log("%s one-step error:" % prec, line[key])
         except Exception as exc:  # the error check never hides the measurement
             line[key] = {"error": str(exc)}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(batch, args.warmup + args.steps, args.cpu_seconds, args.cpu_procs)
+        line["cpu_baseline"] = cpu_baseline(settled, args.steps, args.cpu_seconds, args.cpu_procs)
         log("cpu baseline: %.3e body-steps/s" % line["cpu_baseline"]["value"])
     print(json.dumps(line), flush=True)
 

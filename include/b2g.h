@@ -168,6 +168,19 @@ B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int 
    of the parallel solver). */
 B2G_API int b2g_island_times(b2g_ctx* ctx, int64_t* out, int32_t cap, int32_t* count);
 
+/* Work counters of the steps since the last reset (summed over steps):
+   out[0] steps, out[1] live contacts at the narrow phase (fat-AABB pairs present
+   in the step: ContactManager.collide's loop, contact_manager.py:174-223),
+   out[2] of those evaluated (an awake non-static body), out[3] contacts created
+   (find_new_contacts, contact_manager.py:102-158), out[4] proxies re-fattened
+   and moved (DynamicTree.move_proxy returning true, dynamictree.py:152-188),
+   out[5] island bodies and out[6] island contacts solved (the lengths of
+   Island.bodies and Island.contacts, world.py:494-580, summed over islands).
+   At most n values are written; reset != 0 zeroes the counters.  No
+   counterpart in the reference (the per-step counts that set the step's
+   algorithmic bytes). */
+B2G_API int b2g_stats(b2g_ctx* ctx, int64_t* out, int32_t n, int32_t reset);
+
 /* Parity seam, narrow phase: evaluates n contacts (fixture records A and B
    already oriented as the contact registry orders them, contact.py:305-326,
    at body transforms xf = (px, py, cos, sin)) from zeroed manifolds, like

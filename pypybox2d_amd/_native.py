@@ -22,7 +22,7 @@ B2G_FP32, B2G_FP64 = 0, 1
 
 EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step", "b2g_step_n", "b2g_sync",
            "b2g_counts", "b2g_download", "b2g_last_error", "b2g_set_param", "b2g_get_param", "b2g_profile_step",
-           "b2g_island_times", "b2g_narrowphase", "b2g_pairs")
+           "b2g_island_times", "b2g_narrowphase", "b2g_pairs", "b2g_stats")
 
 # keys of b2g_set_param / b2g_get_param: the solver's launch shape, island timing, and the
 # resident solver wavefronts per compute unit and the search launches per step (both read only)
@@ -78,6 +78,7 @@ def load(path: Optional[str] = None):
     lib.b2g_set_param.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
     lib.b2g_get_param.argtypes = [vp, ctypes.c_int32, vp]
     lib.b2g_island_times.argtypes = [vp, vp, ctypes.c_int32, vp]
+    lib.b2g_stats.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32]
     _ = i32
     _lib = lib
     return lib
@@ -158,6 +159,14 @@ class Engine:
         ms = np.zeros(5, np.float32)
         self._check(self.lib.b2g_profile_step(self.h, n, dt, vel_iters, pos_iters, ms.ctypes.data), "profile")
         return ms
+
+    def stats(self, reset: bool = False) -> dict:
+        """Work counters summed over the steps since the last reset (b2g_stats)."""
+        v = np.zeros(7, np.int64)
+        self._check(self.lib.b2g_stats(self.h, v.ctypes.data, 7, 1 if reset else 0), "stats")
+        return {"steps": int(v[0]), "live_contacts": int(v[1]), "evaluated_contacts": int(v[2]),
+                "new_contacts": int(v[3]), "moved_proxies": int(v[4]), "island_bodies": int(v[5]),
+                "island_contacts": int(v[6])}
 
     def island_times(self) -> np.ndarray:
         """Islands of the last step (island timing on): rows of start_ns, end_ns, contacts,

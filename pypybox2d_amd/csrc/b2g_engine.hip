@@ -65,6 +65,9 @@ enum : int {
   WERR_ISLAND_SIZE = 32,  // an island of more than 65535 contacts (16-bit island-local indices)
 };
 
+// work counters (b2g_stats): summed over the steps since the last reset
+enum : int { S_LIVE = 0, S_EVAL = 1, S_NEW = 2, S_MOVED = 3, S_IBODIES = 4, S_ICONTACTS = 5, kStats = 8 };
+
 constexpr int kBlock = 256;
 constexpr int kBigPairsBlock = 1024;  // k_pairs workgroup for the largest worlds
 constexpr int kMaxLds = 163840;  // LDS per workgroup on gfx950
@@ -189,6 +192,7 @@ struct Dev {
   const double* w_grav;
   double* w_inv_dt0;
   int* w_err;
+  unsigned long long* stat;  // per-step work counters, summed over steps (S_* below)
   int* w_move_n;
   int* w_asleep;
   int* w_nisl;
@@ -409,21 +413,25 @@ __global__ void __launch_bounds__(B) k_collide(Dev<T> d, int chunks) {
   // into one XCD's L2 only.
   // The chunk's contacts that need a manifold are then regrouped by shape pair (a counting
   // sort in LDS), so that a wavefront runs one collider instead of all of them in turn.
-  __shared__ int count[16], base[16], total_s;
+  __shared__ int count[16], base[16], total_s, live_s;
   __shared__ uint16_t sorted[kBlock];
   const int b = blockIdx.x, xcd = b & 7, k = b >> 3;
   const int w = (k / chunks) * 8 + xcd, chunk = k % chunks;
   if (w >= d.nw) return;  // the whole workgroup
+  const int used = d.w_con_used[w];
+  if (chunk * kBlock >= used) return;  // the whole workgroup: past the world's used slots
   const int tid = threadIdx.x;
   if (tid < 16) count[tid] = 0;
+  if (tid == 0) live_s = 0;
   __syncthreads();
   const int sl = chunk * kBlock + tid;
   const int f0 = d.w_fix_off[w], b0 = d.w_body_off[w], cbase = d.w_con_base[w];
   int key = -1, rank = 0;
-  if (sl < d.w_con_used[w]) {
+  if (sl < used) {
     const int s = cbase + sl;
     int st = d.c_state[s];
     if (st & CS_LIVE) {
+      atomicAdd(&live_s, 1);
       const int2 pr = d.c_pair[s];
       const int4 ia = d.f_info[f0 + pr.x], ib = d.f_info[f0 + pr.y];
       const int ba = b0 + ia.x, bb = b0 + ib.x;
@@ -445,6 +453,8 @@ __global__ void __launch_bounds__(B) k_collide(Dev<T> d, int chunks) {
       run += count[q];
     }
     total_s = run;
+    atomicAdd(&d.stat[S_LIVE], (unsigned long long)live_s);
+    atomicAdd(&d.stat[S_EVAL], (unsigned long long)run);
   }
   __syncthreads();
   const int total = total_s;
@@ -1873,6 +1883,11 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_prep_bodies(Dev<T> d, int chunks) {
   const int w = blockIdx.x / chunks, i = (blockIdx.x % chunks) * kBlock + threadIdx.x;
+  if (i == 0) {  // the world's island bodies and contacts, for the work counters
+    const int2 t = d.w_itot[w];
+    if (t.x > 0) atomicAdd(&d.stat[S_IBODIES], (unsigned long long)t.x);
+    if (t.y > 0) atomicAdd(&d.stat[S_ICONTACTS], (unsigned long long)t.y);
+  }
   if (i >= d.w_itot[w].x) return;
   const int b0 = d.w_body_off[w];
   const IslandK c = island_k(d, w, b0);
@@ -1990,13 +2005,23 @@ __device__ __forceinline__ void compute_aabb64(int type, int n, const double* g,
 }
 
 template <typename T>
+__device__ __forceinline__ bool sync_fixture(const Dev<T>& d, int f);
+
+template <typename T>
 __global__ void __launch_bounds__(kBlock) k_sync(Dev<T> d) {
   const int f = blockIdx.x * kBlock + threadIdx.x;
-  if (f >= d.nf) return;
+  const bool moved = f < d.nf && sync_fixture(d, f);
+  const unsigned long long m = __ballot(moved);  // every lane of the wavefront gets here
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&d.stat[S_MOVED], (unsigned long long)__popcll(m));
+}
+
+// Body._synchronize_fixtures for fixture f: returns whether its proxy moved (was re-fattened).
+template <typename T>
+__device__ __forceinline__ bool sync_fixture(const Dev<T>& d, int f) {
   int2 wb = d.f_wb[f];
   const int gb = wb.y;
   d.f_moved[f] = 0;
-  if (d.b_island[gb] < 0 || d.b_type[gb] == BT_STATIC) return;
+  if (d.b_island[gb] < 0 || d.b_type[gb] == BT_STATIC) return false;
   int4 info = d.f_info[f];
   const int type = info.y & 255, n = (info.y >> 8) & 255;
   const double* g = d.f_geo64 + (size_t)f * 16;
@@ -2012,7 +2037,7 @@ __global__ void __launch_bounds__(kBlock) k_sync(Dev<T> d) {
   double lx = pmin(alx, blx), ly = pmin(aly, bly), ux = pmax(aux, bux), uy = pmax(auy, buy);
   double dx = xf.x - x1, dy = xf.y - y1;
   double4 fat = d.f_fat[f];
-  if (fat.x <= lx && fat.y <= ly && ux <= fat.z && uy <= fat.w) return;  // contains
+  if (fat.x <= lx && fat.y <= ly && ux <= fat.z && uy <= fat.w) return false;  // contains
   double nlx = lx - AABB_EXTENSION, nly = ly - AABB_EXTENSION, nux = ux + AABB_EXTENSION, nuy = uy + AABB_EXTENSION;
   double mdx = AABB_MULTIPLIER * dx, mdy = AABB_MULTIPLIER * dy;
   if (mdx < 0.0) { nlx = nlx + mdx; nly = nly + 0.0; }
@@ -2021,6 +2046,7 @@ __global__ void __launch_bounds__(kBlock) k_sync(Dev<T> d) {
   else { nux = nux + 0.0; nuy = nuy + mdy; }
   d.f_fat[f] = make_double4(nlx, nly, nux, nuy);
   d.f_moved[f] = 1;
+  return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -2288,7 +2314,10 @@ __global__ void __launch_bounds__(B) k_pairs(Dev<T> d, int w_first, int from_upl
       create_contact(d, f0, cb + used + k, (int)mlist[key >> 16], (int)(key & 0xFFFFu));
     }
     __syncthreads();
-    if (tid == 0) d.w_con_used[w] = used + nnew;
+    if (tid == 0) {
+      d.w_con_used[w] = used + nnew;
+      if (nnew > 0) atomicAdd(&d.stat[S_NEW], (unsigned long long)nnew);
+    }
   }
   if (tid == 0) d.w_move_n[w] = 0;
   if (!finish) return;
@@ -3078,7 +3107,10 @@ __global__ void __launch_bounds__(kBigBlock) k_pairs_big_finish(Dev<T> d, int fi
     for (int k = tid; k < nnew; k += kBigBlock) create_contact(d, f0, cb + used + k, mlist[nlist[k] >> 16],
                                                                (int)(nlist[k] & 0xFFFFu));
     __syncthreads();
-    if (tid == 0) d.w_con_used[w] = used + nnew;
+    if (tid == 0) {
+      d.w_con_used[w] = used + nnew;
+      if (nnew > 0) atomicAdd(&d.stat[S_NEW], (unsigned long long)nnew);
+    }
   }
   if (tid == 0) d.w_move_n[w] = 0;
   if (!finish) return;
@@ -3318,6 +3350,7 @@ struct b2g_ctx {
   // timing
   std::vector<hipEvent_t> pev;  // b2g_profile_step
   int island_times = 0;         // B2G_PARAM_ISLAND_TIMES
+  long long stat_steps = 0;     // steps launched since the work counters were reset
   long long* itime = nullptr;   // 4 values per island slot (nb slots)
 };
 
@@ -3626,6 +3659,7 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   d.w_grav = w_grav_d;
   TRY(dalloc(ctx, &d.w_inv_dt0, nw));
   TRY(dalloc(ctx, &d.w_err, nw));
+  TRY(dalloc(ctx, &d.stat, kStats));
   TRY(dalloc(ctx, &d.w_move_n, nw));
   TRY(dalloc(ctx, &d.w_asleep, nw));
   TRY(dalloc(ctx, &d.w_nisl, nw));
@@ -3759,6 +3793,11 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(upload(ctx, w_grav_d, w_grav.data(), 2 * (size_t)nw));
   TRY(upload(ctx, d.w_inv_dt0, w_inv_dt0.data(), nw));
   TRY(upload(ctx, d.w_err, zeros_w.data(), nw));
+  {
+    const std::vector<unsigned long long> z(kStats, 0ull);
+    TRY(upload(ctx, d.stat, z.data(), kStats));
+    ctx->stat_steps = 0;
+  }
   TRY(upload(ctx, d.w_move_n, w_move_n.data(), nw));
   TRY(upload(ctx, d.w_asleep, w_asleep.data(), nw));
   TRY(upload(ctx, d.w_nisl, zeros_w.data(), nw));
@@ -4127,7 +4166,9 @@ B2G_API int b2g_step(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters) {
   TRY(ensure_island_times(ctx));
   bool first = ctx->pending_new_fixture;
   ctx->pending_new_fixture = false;
-  return dispatch_step(ctx, dt, vel_iters, pos_iters, first);
+  TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, first));
+  ctx->stat_steps += 1;
+  return B2G_OK;
 }
 
 B2G_API int b2g_step_n(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, int use_graph) {
@@ -4141,6 +4182,7 @@ B2G_API int b2g_step_n(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_it
     TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, true));
     ++i;
   }
+  ctx->stat_steps += n;
   if (!use_graph) {
     for (; i < n; ++i) TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, false));
     return B2G_OK;
@@ -4299,10 +4341,10 @@ B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int 
   if (!ctx || !ctx->dev || !ms_out || n <= 0) return set_error(ctx, B2G_E_INVALID, "bad arguments");
   hipSetDevice(ctx->device);
   TRY(ensure_island_times(ctx));
-  if (ctx->pending_new_fixture) {
-    ctx->pending_new_fixture = false;
-    TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, true));
-  }
+  // a pending new-fixture pair search runs at the start of the first step, before its first
+  // event (as b2g_step does), so exactly n steps run
+  bool first = ctx->pending_new_fixture;
+  ctx->pending_new_fixture = false;
   if (ctx->pev.empty()) {  // a ring of 6 events per step, read once per batch of steps
     ctx->pev.resize(6 * kProfileBatch);
     for (auto& e : ctx->pev) HIPCHK(hipEventCreate(&e));
@@ -4310,7 +4352,10 @@ B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int 
   double acc[5] = {0, 0, 0, 0, 0};
   for (int done = 0; done < n;) {
     const int m = std::min(kProfileBatch, n - done);
-    for (int i = 0; i < m; ++i) TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, false, &ctx->pev[6 * i]));
+    for (int i = 0; i < m; ++i) {
+      TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, first, &ctx->pev[6 * i]));
+      first = false;
+    }
     HIPCHK(hipEventSynchronize(ctx->pev[6 * (m - 1) + 5]));
     for (int i = 0; i < m; ++i)
       for (int k = 0; k < 5; ++k) {
@@ -4321,6 +4366,23 @@ B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int 
     done += m;
   }
   for (int k = 0; k < 5; ++k) ms_out[k] = (float)(acc[k] / n);
+  ctx->stat_steps += n;
+  return B2G_OK;
+}
+
+B2G_API int b2g_stats(b2g_ctx* ctx, int64_t* out, int32_t n, int32_t reset) {
+  if (!ctx || !ctx->dev || !out || n < 1) return set_error(ctx, B2G_E_INVALID, "bad arguments");
+  hipSetDevice(ctx->device);
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  unsigned long long* st = ctx->precision == B2G_FP64 ? ((Dev<double>*)ctx->dev)->stat : ((Dev<float>*)ctx->dev)->stat;
+  unsigned long long v[kStats];
+  HIPCHK(hipMemcpy(v, st, sizeof(v), hipMemcpyDeviceToHost));
+  out[0] = ctx->stat_steps;
+  for (int k = 0; k < 6 && k + 1 < n; ++k) out[k + 1] = (int64_t)v[k];
+  if (reset) {
+    HIPCHK(hipMemset(st, 0, sizeof(v)));
+    ctx->stat_steps = 0;
+  }
   return B2G_OK;
 }
 
