@@ -21,8 +21,10 @@ algorithmic bytes from the device's work counters, one-step errors of both
 pipelines against the FP64 oracle on 64 worlds of the settled state, and the
 oracle's rate on the host cores over the same worlds and window.
 
-Under ``rocprofv3 --selected-regions`` only the timed window and its eager
-re-run are recorded (roctx pause/resume).
+Profiles: every step of the process runs k_reset_queue first, so a rocprofv3
+trace of this command splits into steps at those dispatches; the line's
+"profile_window" gives the step ordinals (0-based, counted over the whole
+process) of each pipeline's timed window and eager re-run.
 """
 from __future__ import annotations
 
@@ -104,30 +106,8 @@ def stage_bytes(stage: str, c: dict, w: dict, fp64: bool) -> float:
     return 0.0
 
 
-class Roctx(object):
-    """rocprofv3 --selected-regions markers: only the timed windows are profiled.  Without a
-    profiler attached (or without the library) the calls do nothing."""
-
-    def __init__(self):
-        self.lib = None
-        try:
-            import ctypes
-            self.lib = ctypes.CDLL("/opt/rocm/lib/librocprofiler-sdk-roctx.so")
-            self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
-            self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
-        except OSError:
-            self.lib = None
-
-    def pause(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerPause(0)
-
-    def resume(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerResume(0)
-
-
-ROCTX = Roctx()
+# steps launched so far in this process, over all engines (the step ordinals of a profile)
+STEP_COUNT = [0]
 
 
 def settle(batch: L.BatchArrays, steps: int) -> L.BatchArrays:
@@ -146,6 +126,7 @@ def settle(batch: L.BatchArrays, steps: int) -> L.BatchArrays:
         log("[settle] %d/%d steps, %.1f s" % (done, steps, time.perf_counter() - t0))
     state = eng.download()
     eng.close()
+    STEP_COUNT[0] += steps
     state.world_i[:, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
     return state
 
@@ -168,28 +149,29 @@ def run_gpu(start: L.BatchArrays, precision: str, steps: int, warmup: int, tune:
     if tune:
         tuner = LaunchTuner(eng)
         tuned = dict(tuner.tune(1.0 / 60, 8, 3))
+        STEP_COUNT[0] += tuner.steps_per_tune
         log("[%s] launch shape %s (solve ms per candidate %s)" % (precision, tuned,
                                                                   np.round(tuner.last_times, 2).tolist()))
     if warmup > 0:
         eng.step_n(warmup, graph=True)
     eng.sync()
+    STEP_COUNT[0] += warmup
+    first = STEP_COUNT[0]
+    profile_window = {"graph": [first, first + steps - 1], "eager": [first + steps, first + 2 * steps - 1]}
+    STEP_COUNT[0] += 2 * steps
     window = eng.download()
     eng.stats(reset=True)
     log("[%s] timing %d graph-captured steps" % (precision, steps))
-    ROCTX.resume()
     t0 = time.perf_counter()
     eng.step_n(steps, graph=True)
     eng.sync()
     elapsed = time.perf_counter() - t0
-    ROCTX.pause()
     stats = eng.stats(reset=True)
     end = eng.download()
     # the same window again, eager, with events between the kernels
     eng.upload(window)
-    ROCTX.resume()
     ms = eng.profile(steps)
     eng.sync()
-    ROCTX.pause()
     again = eng.download()
     identical = bool(np.array_equal(again.body_d, end.body_d) and np.array_equal(again.con_d, end.con_d)
                      and np.array_equal(again.con_i, end.con_i))
@@ -200,10 +182,11 @@ def run_gpu(start: L.BatchArrays, precision: str, steps: int, warmup: int, tune:
     if solver_times:
         eng.set_param("island_times", 1)
         eng.profile(1)
+        STEP_COUNT[0] += 1
         solver = solver_report(eng.island_times(), eng.get_param("solve_waves_per_cu"))
     eng.close()
     return {"elapsed": elapsed, "stage_ms": ms, "state": end, "launch_shape": tuned, "solver": solver,
-            "stats": stats, "eager_rerun_identical": identical}
+            "stats": stats, "eager_rerun_identical": identical, "profile_window": profile_window}
 
 
 def solver_report(rows: np.ndarray, waves_per_cu: int) -> dict:
@@ -440,17 +423,14 @@ def copy_calibration(mib: int = 256, reps: int = 5) -> None:
     rocprofv3's FETCH_SIZE and WRITE_SIZE read for it against the bytes it must move gives the
     scale of those counters."""
     import torch
-    ROCTX.pause()
     n = mib * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    ROCTX.resume()
     for _ in range(reps):
         dst.copy_(src)
     torch.cuda.synchronize()
-    ROCTX.pause()
     print(json.dumps({"copy_calibration": {"bytes_read_per_dispatch": mib << 20, "bytes_written_per_dispatch":
                                            mib << 20, "dispatches": reps}}), flush=True)
 
@@ -489,7 +469,6 @@ def main():
               "(worlds would shard across GPUs with no communication); got --gpus %d, WORLD_SIZE=%d"
               % (args.gpus, world_size))
         sys.exit(2)
-    ROCTX.pause()  # under rocprofv3 --selected-regions only the timed windows are recorded
 
     if args.worlds is None:
         args.worlds = {"mixed": 4096, "lognormal": 16384, "pyramids": 1}[args.workload]
@@ -544,13 +523,14 @@ def main():
         "roofline": roofline(KERNELS[k], float(ms32[k]), c, w32, False, args),
         "roofline_other": {n: roofline(n, float(ms32[i]), c, w32, False, args) for i, n in enumerate(KERNELS)
                            if i != k},
-        "step_bytes": {"algorithmic": sb, "achieved_GBps": sb / (1e-3 * r32["elapsed"] / args.steps) / 1e9,
-                       "frac": sb / (1e-3 * r32["elapsed"] / args.steps) / 1e9 / HBM_PEAK_GBS,
+        "step_bytes": {"algorithmic": sb, "achieved_GBps": sb / (r32["elapsed"] / args.steps) / 1e9,
+                       "frac": sb / (r32["elapsed"] / args.steps) / 1e9 / HBM_PEAK_GBS,
                        "per_step_counters": w32,
                        "definition": "sum over dynamic bodies of (148 + G) + 128 C_persist + 8 P_new + 32 R_reinsert"},
         "solver": r32["solver"],
         "launch_shape": r32["launch_shape"],
         "eager_rerun_identical": r32["eager_rerun_identical"],
+        "profile_window": {"fp32": r32["profile_window"]},
         "timing": "value: wall clock of the timed graph-captured steps (barrier = stream sync on both sides); "
                   "kernel_ms: HIP events around every kernel of the same window re-run eagerly from its saved "
                   "start state",
@@ -564,6 +544,7 @@ def main():
                         "launch_shape": r64["launch_shape"], "solver": r64["solver"],
                         "roofline": roofline(KERNELS[k], float(r64["stage_ms"][k]), c, w64, True, args),
                         "eager_rerun_identical": r64["eager_rerun_identical"]}
+        line["profile_window"]["fp64"] = r64["profile_window"]
         line["fp32_over_fp64"] = value / v64
         log("fp64: %.3e body-steps/s; fp32/fp64 = %.2f" % (v64, value / v64))
         line["long_horizon"] = {"fp32": long_horizon(r32["state"]), "fp64": long_horizon(r64["state"]),
