@@ -70,7 +70,6 @@ static double pysq(double x) {
 }
 
 static int g_init = 0;
-static double g_norm_eps = EPSILON; /* common.py:216-222 (C flavour: FLT_EPSILON) */
 static void init_consts(void) {
   if (g_init) return;
   ANGULAR_SLOP = 2.0 / 180.0 * PI_;        /* settings.py:65 */
@@ -108,7 +107,7 @@ static inline double length_(V2 a) { return sqrt(pysq(a.x) + pysq(a.y)); }
 static inline double length_sq(V2 a) { return pysq(a.x) + pysq(a.y); }
 static inline V2 normalized(V2 a) {
   double l = length_(a);
-  if (l < g_norm_eps) return a;
+  if (l < EPSILON) return a; /* common.py:216-222 (pure-Python flavour: DBL_EPSILON) */
   double il = 1.0 / l;
   return v2(a.x * il, a.y * il);
 }
@@ -1634,7 +1633,6 @@ static int world_load(World* w, const b2g_batch* d, int wi) {
 /* ------------------------------------------------------------------ */
 /* exported entry points (ctypes)                                      */
 /* ------------------------------------------------------------------ */
-ORC_EXPORT void orc_set_normalize_eps(double eps) { init_consts(); g_norm_eps = eps; }
 
 ORC_EXPORT void* orc_load(const b2g_batch* d) {
   init_consts();

@@ -45,7 +45,6 @@ def lib():
         l.orc_evaluate.argtypes = [vp] * 8
         l.orc_pairs.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int]
         l.orc_compute_aabb.argtypes = [vp, vp, vp, vp]
-        l.orc_set_normalize_eps.argtypes = [f64]
         _ = i32
         _lib = l
     return _lib

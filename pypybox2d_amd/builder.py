@@ -6,6 +6,7 @@ from typing import List
 from . import layout as L
 from .fixture import Fixture
 from .shapes import Circle, Edge, Polygon
+from .scenes import fixture_filter
 from .world import World
 
 
@@ -28,7 +29,9 @@ def build_world(scene, precision: str = "fp64", device: int = 0) -> World:
                   continuous_physics=False, precision=precision, device=device)
     for bs in scene["bodies"]:
         fixtures = [Fixture(make_shape(f["shape"]), density=f["density"], friction=f["friction"],
-                            restitution=f["restitution"]) for f in bs["fixtures"]]
+                            restitution=f["restitution"], category_bits=fixture_filter(f)[0],
+                            mask_bits=fixture_filter(f)[1], group_index=fixture_filter(f)[2])
+                    for f in bs["fixtures"]]
         kw = dict(position=bs["position"], angle=bs["angle"], linear_velocity=bs["linear_velocity"],
                   angular_velocity=bs["angular_velocity"], fixtures=fixtures)
         if bs["type"] == "static":

@@ -1,9 +1,10 @@
 """Synthetic scenes of the benchmark and parity workloads, as plain data.
 
 A scene is a dict ``{"gravity", "do_sleep", "warm_starting", "bodies"}``;
-every body is a dict with ``type`` ("static" / "dynamic"), ``position``,
+every body is a dict with ``type`` ("static" / "dynamic" / "kinematic"), ``position``,
 ``angle``, ``linear_velocity``, ``angular_velocity`` and a list of
-``fixtures`` (``shape``, ``density``, ``friction``, ``restitution``).  Shapes
+``fixtures`` (``shape``, ``density``, ``friction``, ``restitution`` and, when
+not the defaults, ``category``, ``mask`` and ``group``).  Shapes
 are tuples: ``("box", hx, hy)``, ``("circle", r)``, ``("polygon", [(x, y),
 ...])`` (convex, counter-clockwise) or ``("edge", (x1, y1), (x2, y2))``.
 
@@ -17,6 +18,7 @@ Workloads (gravity (0, -10), dt = 1/60, 8 velocity / 3 position iterations):
   mixed_batch  many mixed worlds with body counts uniform in [32, 512]
   pyramids     N pyramids side by side on their own ground edges
   lognormal_batch  many mixed worlds with lognormal body counts
+  filtered_world   mixed bodies with category/mask/group filtering and a kinematic platform
 """
 from __future__ import annotations
 
@@ -32,8 +34,16 @@ def _body(type_, position, fixtures, angle=0.0, linear_velocity=(0.0, 0.0), angu
             "fixtures": fixtures}
 
 
-def _fix(shape, density=0.0, friction=0.2, restitution=0.0):
-    return {"shape": shape, "density": density, "friction": friction, "restitution": restitution}
+def _fix(shape, density=0.0, friction=0.2, restitution=0.0, category=0x0001, mask=0xFFFF, group=0):
+    f = {"shape": shape, "density": density, "friction": friction, "restitution": restitution}
+    if (category, mask, group) != (0x0001, 0xFFFF, 0):
+        f.update(category=category, mask=mask, group=group)
+    return f
+
+
+def fixture_filter(f):
+    """(category_bits, mask_bits, group_index) of a fixture spec (defaults as Fixture's)."""
+    return f.get("category", 0x0001), f.get("mask", 0xFFFF), f.get("group", 0)
 
 
 def pyramid(rows: int = 10, x0: float = 0.0, ground_half: float = 40.0, with_ground: bool = True):
@@ -163,6 +173,40 @@ def lognormal_batch(n_worlds: int = 16384, seed: int = 2, median: float = 64.0, 
     """Mixed worlds with lognormal body counts (see lognormal_batch_counts)."""
     counts = lognormal_batch_counts(n_worlds, seed, median, sigma, lo, hi)
     return [batch_world(seed, w, n) for w, n in enumerate(counts)]
+
+
+def filtered_world(seed: int = 5, n_bodies: int = 36):
+    """Mixed bodies with collision filtering and a kinematic platform.
+
+    Fixtures take category bits 1, 2, 4 or 8 in turn.  Category 2 ignores category 4 and
+    4 ignores 2 (mask bits); category 8 ignores its own category, but its bodies are in group
+    +1 and collide with each other anyway (a positive group overrides the masks), and every
+    seventh body (from the fourth) is in group -1, whose members never collide with each
+    other.  A kinematic box
+    platform (6 x 0.3 half-extents) at (0, 5) moves right at 1 m/s and turns at 0.2 rad/s;
+    bodies fall on it and on the ground edge."""
+    rng = np.random.default_rng(seed)
+    bodies = [_body("static", (0.0, 0.0), [_fix(("edge", (-30.0, 0.0), (30.0, 0.0)))]),
+              _body("kinematic", (0.0, 5.0), [_fix(("box", 6.0, 0.3), friction=0.5)],
+                    linear_velocity=(1.0, 0.0), angular_velocity=0.2)]
+    cats = (0x0001, 0x0002, 0x0004, 0x0008)
+    masks = {0x0001: 0xFFFF, 0x0002: 0xFFFF & ~0x0004, 0x0004: 0xFFFF & ~0x0002, 0x0008: 0xFFFF & ~0x0008}
+    for i in range(n_bodies):
+        shape, _ = _random_shape(rng)
+        if shape[0] == "box":
+            shape = ("box", min(shape[1], 0.6), min(shape[2], 0.6))
+        elif shape[0] == "circle":
+            shape = ("circle", min(shape[1], 0.6))
+        cat = cats[i % 4]
+        group = -1 if i % 7 == 3 else (1 if cat == 0x0008 else 0)
+        x = float(rng.uniform(-3.0, 3.0)) if cat == 0x0008 else float(rng.uniform(-6.0, 6.0))
+        y = 7.0 + 1.4 * i
+        bodies.append(_body("dynamic", (x, y), [_fix(shape, density=float(rng.uniform(0.5, 2.0)),
+                                                     friction=float(rng.uniform(0.2, 0.8)), category=cat,
+                                                     mask=masks[cat], group=group)],
+                            angle=float(rng.uniform(-np.pi, np.pi)),
+                            linear_velocity=(float(rng.normal(0, 0.5)), 0.0)))
+    return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies}
 
 
 def pyramids(count: int = 200, spacing: float = 30.0):

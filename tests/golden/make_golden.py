@@ -1,6 +1,7 @@
 """Writes the golden fixtures of the contact pipeline (tests/golden/*.npz).
 
-Run once in the build container:  python tests/golden/make_golden.py
+Run once in the build container:  python tests/golden/make_golden.py [names]
+(names: pyramid stacks mixed filtered narrowphase; default all)
 
 It imports pypybox2d from /root/reference in its default pure-Python flavour
 (the reference tree ships no compiled _common) and records, for small scenes:
@@ -57,11 +58,15 @@ def build_ref_world(scene):
                      warm_starting=scene["warm_starting"], continuous_physics=False)
     for bs in scene["bodies"]:
         fixtures = [b2.Fixture(ref_shape(f["shape"]), density=f["density"], friction=f["friction"],
-                               restitution=f["restitution"]) for f in bs["fixtures"]]
+                               restitution=f["restitution"], category_bits=scenes.fixture_filter(f)[0],
+                               mask_bits=scenes.fixture_filter(f)[1], group_index=scenes.fixture_filter(f)[2])
+                    for f in bs["fixtures"]]
         kw = dict(position=bs["position"], angle=bs["angle"], linear_velocity=bs["linear_velocity"],
                   angular_velocity=bs["angular_velocity"], fixtures=fixtures)
         if bs["type"] == "static":
             world.create_static_body(**kw)
+        elif bs["type"] == "kinematic":
+            world.create_kinematic_body(**kw)
         else:
             world.create_dynamic_body(**kw)
     return world
@@ -345,24 +350,32 @@ def run_narrowphase(cases):
                 kinds=np.array(kinds))
 
 
-def main():
-    pyr_ck = {1, 2, 3, 4, 5, 10, 20, 50, 100, 150, 200, 300, 400, 500, 600}
-    init, hints, states = record_run(scenes.pyramid(), 600, pyr_ck)
-    save_run(os.path.join(HERE, "pyramid_600.npz"), init, hints, states, {"steps": 600})
-
-    st_ck = {1, 2, 3, 5, 10, 30, 60, 100, 200, 300, 400, 500, 600}
-    init, hints, states = record_run(scenes.stacks(seed=0), 600, st_ck)
-    save_run(os.path.join(HERE, "stacks_600.npz"), init, hints, states, {"steps": 600})
-
-    mix_ck = {1, 2, 3, 5, 10, 20, 40, 60, 80, 100, 150, 200, 250, 300}
-    rng = np.random.default_rng(7)
-    init, hints, states = record_run(scenes.mixed_world(rng, 24, half_width=8.0, height=12.0), 300, mix_ck)
-    save_run(os.path.join(HERE, "mixed24_300.npz"), init, hints, states, {"steps": 300})
-
-    cases = narrowphase_cases(np.random.default_rng(11))
-    np.savez_compressed(os.path.join(HERE, "narrowphase.npz"), **run_narrowphase(cases))
-    print("narrowphase cases:", len(cases))
+def main(which=None):
+    """Records every fixture, or only those named (pyramid, stacks, mixed, filtered, narrowphase)."""
+    want = set(which or ("pyramid", "stacks", "mixed", "filtered", "narrowphase"))
+    if "pyramid" in want:
+        pyr_ck = {1, 2, 3, 4, 5, 10, 20, 50, 100, 150, 200, 300, 400, 500, 600}
+        init, hints, states = record_run(scenes.pyramid(), 600, pyr_ck)
+        save_run(os.path.join(HERE, "pyramid_600.npz"), init, hints, states, {"steps": 600})
+    if "stacks" in want:
+        st_ck = {1, 2, 3, 5, 10, 30, 60, 100, 200, 300, 400, 500, 600}
+        init, hints, states = record_run(scenes.stacks(seed=0), 600, st_ck)
+        save_run(os.path.join(HERE, "stacks_600.npz"), init, hints, states, {"steps": 600})
+    if "mixed" in want:
+        mix_ck = {1, 2, 3, 5, 10, 20, 40, 60, 80, 100, 150, 200, 250, 300}
+        rng = np.random.default_rng(7)
+        init, hints, states = record_run(scenes.mixed_world(rng, 24, half_width=8.0, height=12.0), 300, mix_ck)
+        save_run(os.path.join(HERE, "mixed24_300.npz"), init, hints, states, {"steps": 300})
+    if "filtered" in want:
+        # collision filtering (category / mask / group bits) and a kinematic body
+        filt_ck = {1, 2, 3, 5, 10, 20, 40, 60, 80, 100, 120, 150, 200, 250, 300}
+        init, hints, states = record_run(scenes.filtered_world(), 300, filt_ck)
+        save_run(os.path.join(HERE, "filtered_kinematic_300.npz"), init, hints, states, {"steps": 300})
+    if "narrowphase" in want:
+        cases = narrowphase_cases(np.random.default_rng(11))
+        np.savez_compressed(os.path.join(HERE, "narrowphase.npz"), **run_narrowphase(cases))
+        print("narrowphase cases:", len(cases))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

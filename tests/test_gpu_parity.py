@@ -105,7 +105,7 @@ def test_pair_seam_matches_oracle_exactly():
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-5)])
-@pytest.mark.parametrize("run", ["pyramid_600.npz", "stacks_600.npz", "mixed24_300.npz"])
+@pytest.mark.parametrize("run", ["pyramid_600.npz", "stacks_600.npz", "mixed24_300.npz", "filtered_kinematic_300.npz"])
 def test_one_step_from_reference_states(run, prec, tol):
     init, z = load_init(run)
     for s in [int(x) for x in z["checkpoints"]][::3]:
@@ -138,7 +138,7 @@ def test_one_step_from_a_near_tie_state(prec, tol):
 
 def test_first_step_from_construction_fp64():
     """Start from the freshly built world (new-fixture pair search on the device)."""
-    for run in ["pyramid_600.npz", "stacks_600.npz", "mixed24_300.npz"]:
+    for run in ["pyramid_600.npz", "stacks_600.npz", "mixed24_300.npz", "filtered_kinematic_300.npz"]:
         init, z = load_init(run)
         ref = oracle_step(init, 3)
         got = gpu_step(init, "fp64", 3)
@@ -161,6 +161,29 @@ def test_pyramid_trajectory_fp64_tracks_oracle():
             be = PU.body_errors(got, ref)
             print("pyramid fp64 step", s, be, flush=True)
             assert be["position"] <= 1e-6, (s, be)
+    e.close()
+
+
+@pytest.mark.parametrize("large_worlds", [0, 1])
+def test_filtered_kinematic_trajectory_fp64_tracks_oracle(large_worlds):
+    """The recorded run with category/mask/group filtering and a kinematic platform, 300 steps
+    from construction on the FP64 pipeline (LDS searches, then the global-memory searches),
+    against the oracle: identical contact lists (the filter decides which overlapping pairs
+    become contacts) and body states within 1e-9."""
+    init, z = load_init("filtered_kinematic_300.npz")
+    o = O.Oracle(init)
+    e = N.Engine("fp64")
+    e.set_param("large_worlds", large_worlds)
+    e.upload(init)
+    for s in range(1, 301):
+        assert o.step(DT, 8, 3) == 0
+        e.step(DT, 8, 3)
+        if s in (1, 20, 100, 200, 300):
+            ref, got = o.state(), e.download()
+            assert PU.same_contact_lists(got, ref), s
+            be = PU.body_errors(got, ref)
+            print("filtered/kinematic fp64 step", s, "large_worlds", large_worlds, be, flush=True)
+            assert max(be[q] for q in PU.QUANTITIES) <= 1e-9, (s, be)
     e.close()
 
 

@@ -16,7 +16,7 @@ from conftest import GOLDEN
 from oracle import oracle as O
 from pypybox2d_amd import layout as L
 
-RUNS = ["pyramid_600.npz", "stacks_600.npz", "mixed24_300.npz"]
+RUNS = ["pyramid_600.npz", "stacks_600.npz", "mixed24_300.npz", "filtered_kinematic_300.npz"]
 
 
 def load_init(z):
@@ -73,3 +73,51 @@ def test_pyramid_run_reaches_sleep_quirk():
     # sleeping or not, the pyramid stands: top box above 9 m
     assert bd[:, L.BD_C_Y].max() > 9.0
     assert slept or np.any(bd[1:, L.BD_SLEEP_TIME] > 0)
+
+
+def _default_filter(fa, fb):
+    """world.py:958-970 (the reference's default contact filter) on (category, mask, group)."""
+    if fa[2] == fb[2] and fa[2] != 0:
+        return fa[2] > 0
+    return (fa[1] & fb[0]) != 0 and (fa[0] & fb[1]) != 0
+
+
+def test_filtered_run_exercises_every_filter_branch_and_a_kinematic_body():
+    """The recorded run with collision filtering and a kinematic platform: at its checkpoints
+    there are fat-AABB-overlapping pairs that the mask bits reject, that a negative group rejects
+    and that a positive group admits against the masks, and none of the rejected pairs is a
+    contact, while every admitted overlapping pair with a dynamic body is; the kinematic body
+    moves and touches dynamic bodies."""
+    z = np.load(os.path.join(GOLDEN, "filtered_kinematic_300.npz"))
+    fi = z["init_fix_i"]
+    bi0 = z["init_body_i"]
+    filt = [(int(r[L.FI_CATEGORY]), int(r[L.FI_MASK]), int(r[L.FI_GROUP])) for r in fi]
+    kin = np.nonzero(bi0[:, L.BI_TYPE] == L.KINEMATIC)[0]
+    assert len(kin) == 1
+    seen = {"mask_reject": 0, "group_reject": 0, "group_admit": 0, "kinematic_touch": 0}
+    for s in z["checkpoints"]:
+        fat = z["s%d_fix_d" % s][:, [L.FD_FAT_LX, L.FD_FAT_LY, L.FD_FAT_UX, L.FD_FAT_UY]]
+        ci = z["s%d_con_i" % s]
+        cons = set((int(a), int(b)) for a, b in ci[:, :2]) | set((int(b), int(a)) for a, b in ci[:, :2])
+        for a in range(len(fi)):
+            for b in range(a + 1, len(fi)):
+                ba, bb = fi[a, L.FI_BODY], fi[b, L.FI_BODY]
+                if ba == bb or L.DYNAMIC not in (bi0[ba, L.BI_TYPE], bi0[bb, L.BI_TYPE]):
+                    continue
+                if fat[b, 0] - fat[a, 2] > 0 or fat[b, 1] - fat[a, 3] > 0 or fat[a, 0] - fat[b, 2] > 0 or \
+                        fat[a, 1] - fat[b, 3] > 0:
+                    continue
+                ok = _default_filter(filt[a], filt[b])
+                assert ((a, b) in cons) == ok, (int(s), a, b, filt[a], filt[b])
+                fa, fb = filt[a], filt[b]
+                if fa[2] == fb[2] and fa[2] != 0:
+                    seen["group_admit" if fa[2] > 0 else "group_reject"] += int(((fa[1] & fb[0]) == 0) or fa[2] < 0)
+                elif not ok:
+                    seen["mask_reject"] += 1
+        k_touch = [k for k in range(ci.shape[0]) if ci[k, L.CI_FLAGS] & L.CF_TOUCHING and
+                   kin[0] in (fi[ci[k, L.CI_FA], L.FI_BODY], fi[ci[k, L.CI_FB], L.FI_BODY])]
+        seen["kinematic_touch"] += len(k_touch)
+    print(seen)
+    assert all(v > 0 for v in seen.values()), seen
+    bd = z["s300_body_d"]
+    assert abs(bd[kin[0], L.BD_C_X] - 5.0) < 1e-9  # moved 300 steps at 1 m/s, untouched by contacts
