@@ -379,7 +379,7 @@ def measured_traffic(stage: str, args) -> dict:
             cmd_ok = True
         if line.startswith("traffic_bytes_per_step "):
             name, val = line[len("traffic_bytes_per_step "):].split(":", 1)
-            per[name.strip()] = float(val)
+            per[name.strip()] = float(val.split()[0])
     parts = STAGE_KERNELS.get(stage, (stage,))
     if not cmd_ok or not any(k in per for k in parts):
         return None

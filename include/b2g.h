@@ -147,6 +147,12 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
 #define B2G_PARAM_PAIRS_WIDE 10
 #define B2G_PARAM_ISLANDS_WIDE 11
 #define B2G_PARAM_COLLIDE_BLOCK 12
+/* B2G_PARAM_TREE_PAIRS   1: the large worlds' pair search (worlds on the
+                          global-memory path) traverses an LBVH built over the
+                          world's fat AABBs after each synchronisation instead of
+                          testing every moved proxy against every proxy; 0
+                          (default): the all-pairs test.  Same pairs, same order. */
+#define B2G_PARAM_TREE_PAIRS 13
 B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value);
 B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
 
@@ -167,6 +173,30 @@ B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int 
    cap rows are written.  No counterpart in the reference (per-island latency
    of the parallel solver). */
 B2G_API int b2g_island_times(b2g_ctx* ctx, int64_t* out, int32_t cap, int32_t* count);
+
+/* World.query_aabb (world.py:371-384) for n boxes at once: for query k, box
+   box[4k .. 4k+3] = (lower x, lower y, upper x, upper y) in world world[k],
+   every fixture whose fat AABB overlaps it (DynamicTree.query,
+   dynamictree.py:190-207, with PyAABB.test_overlap), as out[2j] = k,
+   out[2j + 1] = world-local fixture index, sorted by query then fixture.  The
+   tree is an LBVH built on the device over the state of the last step.
+   *count receives the number of results; with more than cap, the first cap are
+   written and B2G_E_CAPACITY is returned. */
+B2G_API int b2g_query_aabb(b2g_ctx* ctx, int n, const int32_t* world, const double* box, int32_t* out, int cap,
+                           int32_t* count);
+
+/* World.ray_cast (world.py:386-440) for n segments at once: seg[4k .. 4k+3] =
+   (p1 x, p1 y, p2 x, p2 y) in world world[k].  A fixture is hit when its proxy
+   passes DynamicTree.ray_cast's leaf test (dynamictree.py:209-273) and its shape's
+   ray cast (shapes.py:147-174, 265-307, 655-702) hits with max_fraction 1.
+   Rows: out_i[2j] = ray, out_i[2j + 1] = world-local fixture (-1: no hit);
+   out_d[5j ..] = fraction, point x, y ((1 - f) p1 + f p2), normal x, y.
+   closest != 0: row k is ray k's hit of least fraction (ties: lower fixture
+   index), cap >= n.  closest == 0: every hit, sorted by ray, fraction, fixture;
+   more than cap hits: the first cap are written, B2G_E_CAPACITY.  A segment
+   with p1 == p2 is rejected (B2G_E_INVALID; the reference raises ValueError). */
+B2G_API int b2g_ray_cast(b2g_ctx* ctx, int n, const int32_t* world, const double* seg, int32_t closest, int32_t* out_i,
+                         double* out_d, int cap, int32_t* count);
 
 /* Work counters of the steps since the last reset (summed over steps):
    out[0] steps, out[1] live contacts at the narrow phase (fat-AABB pairs present

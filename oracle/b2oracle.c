@@ -1825,3 +1825,162 @@ ORC_EXPORT void orc_compute_aabb(const double* fd, const int32_t* fi, const doub
   AABB a = compute_aabb(&s, t);
   out[0] = a.lo.x; out[1] = a.lo.y; out[2] = a.hi.x; out[3] = a.hi.y;
 }
+
+/* ------------------------------------------------------------------ */
+/* Queries and ray casts: World.query_aabb / World.ray_cast            */
+/* (world.py:371-440), by brute force over a world's proxies.          */
+/* ------------------------------------------------------------------ */
+/* Circle.ray_cast (shapes.py:147-174) */
+static int ray_circle(const Shape* sh, XF xf, V2 p1, V2 p2, double maxf, double* frac, V2* normal) {
+  V2 position = add(xf.p, mulmv(xf.R, sh->p));
+  V2 s = sub(p1, position);
+  double b = dot(s, s) - pysq(sh->radius);
+  V2 r = sub(p2, p1);
+  double c = dot(s, r);
+  double rr = dot(r, r);
+  double sigma = c * c - rr * b;
+  if (sigma < 0.0 || rr < EPSILON) return 0;
+  double a = -(c + sqrt(sigma));
+  if (0.0 <= a && a <= maxf * rr) {
+    a /= rr;
+    *frac = a;
+    *normal = normalized(add(s, scale(a, r)));
+    return 1;
+  }
+  return 0;
+}
+
+/* Polygon.ray_cast (shapes.py:265-307) */
+static int ray_polygon(const Shape* sh, XF xf, V2 p1w, V2 p2w, double maxf, double* frac, V2* normal) {
+  V2 p1 = mulmtv(xf.R, sub(p1w, xf.p));
+  V2 p2 = mulmtv(xf.R, sub(p2w, xf.p));
+  V2 d = sub(p2, p1);
+  double lower = 0.0, upper = maxf;
+  int tagged = -1;
+  for (int i = 0; i < sh->n; ++i) {
+    double numerator = dot(sh->nrm[i], sub(sh->v[i], p1));
+    double denominator = dot(sh->nrm[i], d);
+    if (denominator == 0.0) {
+      if (numerator < 0.0) return 0;
+    } else {
+      if (denominator < 0.0 && numerator < lower * denominator) {
+        lower = numerator / denominator;
+        tagged = i;
+      } else if (denominator > 0.0 && numerator < upper * denominator) {
+        upper = numerator / denominator;
+      }
+    }
+    if (upper < lower) return 0;
+  }
+  if (tagged < 0) return 0;
+  *normal = mulmv(xf.R, sh->nrm[tagged]);
+  *frac = lower;
+  return 1;
+}
+
+/* Edge.ray_cast (shapes.py:655-702): the normal stays in the edge's frame */
+static int ray_edge(const Shape* sh, XF xf, V2 p1w, V2 p2w, double* frac, V2* normal) {
+  V2 p1 = mulmtv(xf.R, sub(p1w, xf.p));
+  V2 p2 = mulmtv(xf.R, sub(p2w, xf.p));
+  V2 d = sub(p2, p1);
+  V2 v1 = sh->e1, v2_ = sh->e2;
+  V2 e = sub(v2_, v1);
+  V2 n = normalized(v2(e.y, -e.x));
+  double numerator = dot(n, sub(v1, p1));
+  double denominator = dot(n, d);
+  if (denominator == 0.0) return 0;
+  double t = numerator / denominator;
+  if (t < 0.0 || 1.0 < t) return 0;
+  V2 q = add(p1, scale(t, d));
+  V2 r = sub(v2_, v1);
+  double rr = dot(r, r);
+  if (rr == 0.0) return 0;
+  double sv = dot(sub(q, v1), r) / rr;
+  if (sv < 0.0 || 1.0 < sv) return 0;
+  *frac = t;
+  if (numerator > 0.0) n = neg(n);
+  *normal = n;
+  return 1;
+}
+
+static int shape_ray_cast(const Shape* sh, XF xf, V2 p1, V2 p2, double maxf, double* frac, V2* normal) {
+  if (sh->type == B2G_SHAPE_CIRCLE) return ray_circle(sh, xf, p1, p2, maxf, frac, normal);
+  if (sh->type == B2G_SHAPE_POLYGON) return ray_polygon(sh, xf, p1, p2, maxf, frac, normal);
+  if (sh->type == B2G_SHAPE_EDGE) return ray_edge(sh, xf, p1, p2, frac, normal);
+  return 0;
+}
+
+/* One shape's ray cast at a transform (px, py, c, s) for seg = (p1 x, p1 y, p2 x, p2 y) and
+   max_fraction; out = fraction, normal x, normal y.  Returns 1 on a hit. */
+ORC_EXPORT int orc_shape_ray_cast(const double* fd, const int32_t* fi, const double* xf, const double* seg,
+                                  double max_fraction, double* out) {
+  init_consts();
+  Shape sh;
+  shape_load(&sh, fd, fi);
+  XF t;
+  t.p = v2(xf[0], xf[1]); t.R.c1x = xf[2]; t.R.c1y = xf[3]; t.R.c2x = -xf[3]; t.R.c2y = xf[2];
+  double f = 0.0;
+  V2 n = v2(0.0, 0.0);
+  int hit = shape_ray_cast(&sh, t, v2(seg[0], seg[1]), v2(seg[2], seg[3]), max_fraction, &f, &n);
+  out[0] = f; out[1] = n.x; out[2] = n.y;
+  return hit;
+}
+
+/* World.query_aabb: the fixtures of world wi whose fat AABB overlaps box (lx, ly, ux, uy), in
+   fixture order (node.aabb.test_overlap(aabb), dynamictree.py:190-207). */
+ORC_EXPORT int orc_query_aabb(void* h, int wi, const double* box, int32_t* out, int cap) {
+  Batch* B = (Batch*)h;
+  if (!B || wi < 0 || wi >= B->nw) return -1;
+  World* w = &B->w[wi];
+  AABB q;
+  q.lo = v2(box[0], box[1]);
+  q.hi = v2(box[2], box[3]);
+  int n = 0;
+  for (int f = 0; f < w->nf; ++f) {
+    if (!aabb_overlap(&w->f[f].fat, &q)) continue;
+    if (n < cap) out[n] = f;
+    ++n;
+  }
+  return n;
+}
+
+/* World.ray_cast with a callback that returns 1 (every hit, no clipping): the fixtures of world
+   wi whose proxy passes DynamicTree.ray_cast's test at max_fraction 1 (dynamictree.py:209-273)
+   and whose shape the ray hits, in fixture order; out_d = fraction, point x, y, normal x, y
+   per hit.  Returns the number of hits, or -1 for p1 == p2 (the reference's ValueError). */
+ORC_EXPORT int orc_ray_cast(void* h, int wi, const double* seg, int32_t* out_f, double* out_d, int cap) {
+  Batch* B = (Batch*)h;
+  if (!B || wi < 0 || wi >= B->nw) return -1;
+  init_consts();
+  World* w = &B->w[wi];
+  V2 p1 = v2(seg[0], seg[1]), p2 = v2(seg[2], seg[3]);
+  V2 r = sub(p2, p1);
+  if (length_sq(r) <= 0.0) return -1;
+  r = normalized(r);
+  V2 v = cross_sv(1.0, r);
+  V2 abs_v = v2(fabs(v.x), fabs(v.y));
+  V2 t = add(p1, scale(1.0, sub(p2, p1)));
+  AABB segbox;
+  segbox.lo = vmin(p1, t);
+  segbox.hi = vmax(p1, t);
+  int n = 0;
+  for (int f = 0; f < w->nf; ++f) {
+    const AABB* fat = &w->f[f].fat;
+    if (!aabb_overlap(fat, &segbox)) continue;
+    V2 c = scale(0.5, add(fat->lo, fat->hi));
+    V2 hh = scale(0.5, sub(fat->hi, fat->lo));
+    double separation = fabs(dot(v, sub(p1, c))) - dot(abs_v, hh);
+    if (separation > 0.0) continue;
+    double fr = 0.0;
+    V2 nrm = v2(0.0, 0.0);
+    if (!shape_ray_cast(&w->f[f].sh, w->b[w->f[f].body].xf, p1, p2, 1.0, &fr, &nrm)) continue;
+    if (n < cap) {
+      V2 pt = add(scale(1.0 - fr, p1), scale(fr, p2));
+      out_f[n] = f;
+      double* o = out_d + 5 * (size_t)n;
+      o[0] = fr; o[1] = pt.x; o[2] = pt.y; o[3] = nrm.x; o[4] = nrm.y;
+    }
+    ++n;
+  }
+  return n;
+}

@@ -45,6 +45,9 @@ def lib():
         l.orc_evaluate.argtypes = [vp] * 8
         l.orc_pairs.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int]
         l.orc_compute_aabb.argtypes = [vp, vp, vp, vp]
+        l.orc_shape_ray_cast.argtypes = [vp, vp, vp, vp, f64, vp]
+        l.orc_query_aabb.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int]
+        l.orc_ray_cast.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.c_int]
         _ = i32
         _lib = l
     return _lib
@@ -80,6 +83,34 @@ class Oracle:
         rc = lib().orc_set_hint(self.h, w, p.ctypes.data if p.size else None, int(p.shape[0]))
         if rc:
             raise RuntimeError("orc_set_hint failed")
+
+    def query_aabb(self, w: int, box) -> np.ndarray:
+        """World.query_aabb by brute force: fixtures of world w whose fat AABB overlaps box."""
+        b = np.ascontiguousarray(box, np.float64)
+        cap = 64
+        while True:
+            out = np.zeros(cap, np.int32)
+            n = lib().orc_query_aabb(self.h, w, b.ctypes.data, out.ctypes.data, cap)
+            if n < 0:
+                raise ValueError("bad world")
+            if n <= cap:
+                return out[:n].copy()
+            cap = n
+
+    def ray_cast(self, w: int, seg):
+        """World.ray_cast, every hit unclipped, in fixture order: (fixtures, rows of fraction, point x, y,
+        normal x, y)."""
+        sg = np.ascontiguousarray(seg, np.float64)
+        cap = 64
+        while True:
+            of = np.zeros(cap, np.int32)
+            od = np.zeros((cap, 5))
+            n = lib().orc_ray_cast(self.h, w, sg.ctypes.data, of.ctypes.data, od.ctypes.data, cap)
+            if n < 0:
+                raise ValueError("p1 == p2 or bad world")
+            if n <= cap:
+                return of[:n].copy(), od[:n].copy()
+            cap = n
 
     def error(self) -> int:
         return lib().orc_error(self.h)
@@ -130,3 +161,13 @@ def compute_aabb(fd, fi, xf):
          np.ascontiguousarray(xf, np.float64)]
     lib().orc_compute_aabb(a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data, out.ctypes.data)
     return out
+
+
+def shape_ray_cast(fd, fi, xf, seg, max_fraction=1.0):
+    """Shape.ray_cast of one fixture's shape at transform xf = (px, py, c, s): (hit, fraction, nx, ny)."""
+    out = np.zeros(3)
+    a = [np.ascontiguousarray(fd, np.float64), np.ascontiguousarray(fi, np.int32),
+         np.ascontiguousarray(xf, np.float64), np.ascontiguousarray(seg, np.float64)]
+    hit = lib().orc_shape_ray_cast(a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data, a[3].ctypes.data,
+                                   float(max_fraction), out.ctypes.data)
+    return bool(hit), float(out[0]), float(out[1]), float(out[2])

@@ -22,13 +22,13 @@ B2G_FP32, B2G_FP64 = 0, 1
 
 EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step", "b2g_step_n", "b2g_sync",
            "b2g_counts", "b2g_download", "b2g_last_error", "b2g_set_param", "b2g_get_param", "b2g_profile_step",
-           "b2g_island_times", "b2g_narrowphase", "b2g_pairs", "b2g_stats")
+           "b2g_island_times", "b2g_narrowphase", "b2g_pairs", "b2g_stats", "b2g_query_aabb", "b2g_ray_cast")
 
 # keys of b2g_set_param / b2g_get_param: the solver's launch shape, island timing, and the
 # resident solver wavefronts per compute unit and the search launches per step (both read only)
 PARAMS = {"team_lds": 1, "heavy_class": 2, "solve_grid": 3, "team_full_class": 4, "large_worlds": 5,
           "island_times": 6, "solve_waves_per_cu": 7, "team_max": 8, "search_runs": 9, "pairs_wide": 10,
-          "islands_wide": 11, "collide_block": 12}
+          "islands_wide": 11, "collide_block": 12, "tree_pairs": 13}
 
 # the stages b2g_profile_step times, in order
 STAGES = ("collide", "islands", "solve", "sync", "pairs")
@@ -79,6 +79,8 @@ def load(path: Optional[str] = None):
     lib.b2g_get_param.argtypes = [vp, ctypes.c_int32, vp]
     lib.b2g_island_times.argtypes = [vp, vp, ctypes.c_int32, vp]
     lib.b2g_stats.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32]
+    lib.b2g_query_aabb.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]
+    lib.b2g_ray_cast.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int, vp]
     _ = i32
     _lib = lib
     return lib
@@ -167,6 +169,48 @@ class Engine:
         return {"steps": int(v[0]), "live_contacts": int(v[1]), "evaluated_contacts": int(v[2]),
                 "new_contacts": int(v[3]), "moved_proxies": int(v[4]), "island_bodies": int(v[5]),
                 "island_contacts": int(v[6])}
+
+    def query_aabb(self, worlds, boxes) -> np.ndarray:
+        """World.query_aabb for many boxes: rows (query, world-local fixture) of every fixture whose fat
+        AABB overlaps box k = (lower x, lower y, upper x, upper y) of world worlds[k], by query then
+        fixture."""
+        w = np.ascontiguousarray(worlds, np.int32).reshape(-1)
+        b = np.ascontiguousarray(boxes, np.float64).reshape(-1, 4)
+        if w.shape[0] != b.shape[0]:
+            raise ValueError("one world index per box")
+        cap = max(64, 8 * w.shape[0])
+        while True:
+            out = np.zeros((cap, 2), np.int32)
+            n = np.zeros(1, np.int32)
+            rc = self.lib.b2g_query_aabb(self.h, w.shape[0], w.ctypes.data, b.ctypes.data, out.ctypes.data, cap,
+                                         n.ctypes.data)
+            if rc == B2G_E_CAPACITY and n[0] > cap:
+                cap = int(n[0])
+                continue
+            self._check(rc, "query_aabb")
+            return out[:int(n[0])].copy()
+
+    def ray_cast(self, worlds, segments, closest: bool = True):
+        """World.ray_cast for many segments (p1 x, p1 y, p2 x, p2 y) of worlds[k].  Returns (rows, data):
+        rows (ray, world-local fixture; -1 for no hit), data (fraction, point x, y, normal x, y); one
+        row per ray (its closest hit) or every hit sorted by ray, fraction, fixture."""
+        w = np.ascontiguousarray(worlds, np.int32).reshape(-1)
+        sg = np.ascontiguousarray(segments, np.float64).reshape(-1, 4)
+        if w.shape[0] != sg.shape[0]:
+            raise ValueError("one world index per segment")
+        cap = max(w.shape[0], 64 if closest else 8 * w.shape[0])
+        while True:
+            oi = np.zeros((cap, 2), np.int32)
+            od = np.zeros((cap, 5))
+            n = np.zeros(1, np.int32)
+            rc = self.lib.b2g_ray_cast(self.h, w.shape[0], w.ctypes.data, sg.ctypes.data, 1 if closest else 0,
+                                       oi.ctypes.data, od.ctypes.data, cap, n.ctypes.data)
+            if rc == B2G_E_CAPACITY and n[0] > cap:
+                cap = int(n[0])
+                continue
+            self._check(rc, "ray_cast")
+            m = int(n[0])
+            return oi[:m].copy(), od[:m].copy()
 
     def island_times(self) -> np.ndarray:
         """Islands of the last step (island timing on): rows of start_ns, end_ns, contacts,

@@ -71,6 +71,16 @@ class BatchedWorld(object):
             self.stepper.step(dt, vel_iters, pos_iters, n, graph)
         self.steps += n
 
+    def query_aabb(self, worlds, boxes):
+        """World.query_aabb for many boxes at once: rows (query, world-local fixture) of every fixture
+        whose fat AABB overlaps box k = (lower x, lower y, upper x, upper y) of world worlds[k]."""
+        return self.stepper.engine.query_aabb(worlds, boxes)
+
+    def ray_cast(self, worlds, segments, closest: bool = True):
+        """World.ray_cast for many segments (p1 x, p1 y, p2 x, p2 y) at once: each ray's closest hit, or
+        every hit (Engine.ray_cast)."""
+        return self.stepper.engine.ray_cast(worlds, segments, closest)
+
     def sync(self) -> None:
         self.stepper.check()
 

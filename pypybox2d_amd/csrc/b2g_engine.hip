@@ -38,6 +38,7 @@
 
 #include "../../include/b2g.h"
 #include "b2g_collide.hpp"
+#include "b2g_raycast.hpp"
 
 namespace b2g {
 
@@ -291,6 +292,15 @@ struct Dev {
   uint32_t* g_nlist;   // per world next_pow2(ccap) at g_nbase[w]
   const long long* g_hbase;
   const long long* g_nbase;
+  // LBVH over each world's fat AABBs (queries, ray casts, the large worlds' tree pair search);
+  // per-fixture arrays at w_fix_off[w]
+  unsigned long long* t_key;  // leaves in Morton order: 30-bit Morton code << 32 | world-local fixture
+  int2* t_child;              // internal node i (0 .. n - 2): children; >= 0 internal, < 0 leaf ~position
+  double4* t_box;             // internal node i: the union of its leaves' fat AABBs
+  int* t_flag;                // refit state of internal node i
+  int* t_root;                // per world: 0 (internal root), ~0 (one leaf) or kTreeEmpty
+  unsigned long long* t_sort; // sort scratch of worlds over kTreeLdsKeys fixtures, at t_sbase[w]
+  const long long* t_sbase;
 };
 
 // ---------------------------------------------------------------------------
@@ -2971,6 +2981,223 @@ __global__ void __launch_bounds__(kBigBlock) k_pairs_big_prep(Dev<T> d, int from
   }
 }
 
+// ---------------------------------------------------------------------------
+// LBVH over a world's fat AABBs (the tree the reference keeps in DynamicTree,
+// dynamictree.py:115-207, built from scratch instead of updated incrementally):
+// 30-bit Morton codes of the proxies' centres, a bitonic sort of (code, proxy)
+// keys in LDS (in global scratch for worlds over kTreeLdsKeys proxies), Karras'
+// radix-tree construction from the sorted keys (the proxy index in the low bits
+// makes every key distinct), and a bottom-up refit in sweeps.  The tree's shape is
+// not observable: queries and the pair search test the proxies' own fat AABBs
+// exactly, and internal nodes only prune (their boxes are exact unions).
+// ---------------------------------------------------------------------------
+constexpr int kTreeEmpty = 0x7FFFFFFF;
+constexpr int kTreeLdsKeys = 16384;  // 128 KB of LDS
+constexpr int kTreeBlock = 256;
+constexpr int kTreeStack = 64;       // 62-bit distinct keys: at most 62 levels
+
+__device__ __forceinline__ uint32_t morton_spread15(uint32_t v) {
+  v &= 0x7FFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+__device__ __forceinline__ int tree_delta(const unsigned long long* k, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  return __clzll(k[i] ^ k[j]);
+}
+
+// In-place bitonic sort of keys[0, np) (np a power of two) by one workgroup.
+__device__ void block_bitonic_sort(unsigned long long* keys, int np) {
+  for (int size = 2; size <= np; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < (np >> 1); t += blockDim.x) {
+        const int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const unsigned long long a = keys[lo], b = keys[hi];
+        if ((a > b) == up) {
+          keys[lo] = b;
+          keys[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTreeBlock) k_tree_build(Dev<T> d, const int* wlist) {
+  extern __shared__ unsigned long long tkeys[];
+  __shared__ double red[4][kTreeBlock / 64];
+  __shared__ int sweep_done;
+  const int w = wlist[blockIdx.x], tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int f0 = d.w_fix_off[w], n = d.w_fix_off[w + 1] - f0;
+  if (n < 2) {
+    if (tid == 0) d.t_root[w] = n == 1 ? ~0 : kTreeEmpty;
+    if (n == 1 && tid == 0) d.t_key[f0] = 0ull;
+    return;
+  }
+  // bounds of the proxies' centres
+  double mnx = 1e300, mny = 1e300, mxx = -1e300, mxy = -1e300;
+  for (int i = tid; i < n; i += kTreeBlock) {
+    const double4 b = d.f_fat[f0 + i];
+    const double cx = 0.5 * (b.x + b.z), cy = 0.5 * (b.y + b.w);
+    mnx = fmin(mnx, cx); mny = fmin(mny, cy); mxx = fmax(mxx, cx); mxy = fmax(mxy, cy);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mnx = fmin(mnx, __shfl_xor(mnx, o, 64)); mny = fmin(mny, __shfl_xor(mny, o, 64));
+    mxx = fmax(mxx, __shfl_xor(mxx, o, 64)); mxy = fmax(mxy, __shfl_xor(mxy, o, 64));
+  }
+  if (lane == 0) { red[0][wv] = mnx; red[1][wv] = mny; red[2][wv] = mxx; red[3][wv] = mxy; }
+  __syncthreads();
+  mnx = red[0][0]; mny = red[1][0]; mxx = red[2][0]; mxy = red[3][0];
+  for (int k = 1; k < kTreeBlock / 64; ++k) {
+    mnx = fmin(mnx, red[0][k]); mny = fmin(mny, red[1][k]); mxx = fmax(mxx, red[2][k]); mxy = fmax(mxy, red[3][k]);
+  }
+  const double sx = mxx > mnx ? 32767.0 / (mxx - mnx) : 0.0, sy = mxy > mny ? 32767.0 / (mxy - mny) : 0.0;
+  // keys
+  int np = 1;
+  while (np < n) np <<= 1;
+  const bool in_lds = np <= kTreeLdsKeys;
+  unsigned long long* keys = in_lds ? tkeys : d.t_sort + d.t_sbase[w];
+  for (int i = tid; i < np; i += kTreeBlock) {
+    unsigned long long key = ~0ull;
+    if (i < n) {
+      const double4 b = d.f_fat[f0 + i];
+      const double cx = 0.5 * (b.x + b.z), cy = 0.5 * (b.y + b.w);
+      const uint32_t qx = (uint32_t)fmin(32767.0, fmax(0.0, (cx - mnx) * sx));
+      const uint32_t qy = (uint32_t)fmin(32767.0, fmax(0.0, (cy - mny) * sy));
+      const uint32_t code = morton_spread15(qx) | (morton_spread15(qy) << 1);
+      key = ((unsigned long long)code << 32) | (unsigned long long)i;
+    }
+    keys[i] = key;
+  }
+  __syncthreads();
+  block_bitonic_sort(keys, np);
+  for (int i = tid; i < n; i += kTreeBlock) d.t_key[f0 + i] = keys[i];
+  // hierarchy (Karras 2012): internal node i covers the sorted range [min(i, j), max(i, j)]
+  for (int i = tid; i < n - 1; i += kTreeBlock) {
+    const int dir = tree_delta(keys, n, i, i + 1) - tree_delta(keys, n, i, i - 1) >= 0 ? 1 : -1;
+    const int dmin = tree_delta(keys, n, i, i - dir);
+    int lmax = 2;
+    while (tree_delta(keys, n, i, i + lmax * dir) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+      if (tree_delta(keys, n, i, i + (l + t) * dir) > dmin) l += t;
+    const int j = i + l * dir;
+    const int dnode = tree_delta(keys, n, i, j);
+    int sp = 0;
+    for (int t = (l + 1) >> 1;; t = (t + 1) >> 1) {
+      if (tree_delta(keys, n, i, i + (sp + t) * dir) > dnode) sp += t;
+      if (t == 1) break;
+    }
+    const int g = i + sp * dir + (dir < 0 ? -1 : 0);
+    const int left = (i < j ? i : j) == g ? ~g : g;
+    const int right = (i > j ? i : j) == g + 1 ? ~(g + 1) : g + 1;
+    d.t_child[f0 + i] = make_int2(left, right);
+    d.t_flag[f0 + i] = 0;
+  }
+  __syncthreads();
+  // refit in sweeps: a node whose children are done takes the union of their boxes (phase 1,
+  // flag -1), and becomes done after the barrier (phase 2), until the root is done
+  for (;;) {
+    for (int i = tid; i < n - 1; i += kTreeBlock) {
+      if (d.t_flag[f0 + i] != 0) continue;
+      const int2 c = d.t_child[f0 + i];
+      if ((c.x >= 0 && d.t_flag[f0 + c.x] != 1) || (c.y >= 0 && d.t_flag[f0 + c.y] != 1)) continue;
+      double4 a, b;
+      if (c.x < 0) a = d.f_fat[f0 + (int)(keys[~c.x] & 0xFFFFFFFFu)];
+      else a = d.t_box[f0 + c.x];
+      if (c.y < 0) b = d.f_fat[f0 + (int)(keys[~c.y] & 0xFFFFFFFFu)];
+      else b = d.t_box[f0 + c.y];
+      d.t_box[f0 + i] = make_double4(fmin(a.x, b.x), fmin(a.y, b.y), fmax(a.z, b.z), fmax(a.w, b.w));
+      d.t_flag[f0 + i] = -1;
+    }
+    __syncthreads();
+    for (int i = tid; i < n - 1; i += kTreeBlock)
+      if (d.t_flag[f0 + i] == -1) d.t_flag[f0 + i] = 1;
+    __syncthreads();
+    if (tid == 0) sweep_done = d.t_flag[f0] == 1;
+    __syncthreads();
+    if (sweep_done) break;
+    __syncthreads();
+  }
+  if (tid == 0) d.t_root[w] = 0;
+}
+
+// Visits every proxy of world w whose subtree boxes overlap the box (inclusive), with the leaf's
+// world-local fixture index; the visitor applies the exact test on the proxy's own fat AABB.
+// Returns false if the stack overflowed (a tree deeper than its keys allow: never).
+template <typename T, typename Visit>
+__device__ __forceinline__ bool tree_visit(const Dev<T>& d, int w, double lx, double ly, double ux, double uy,
+                                           Visit&& visit) {
+  const int f0 = d.w_fix_off[w];
+  const int root = d.t_root[w];
+  if (root == kTreeEmpty) return true;
+  int stack[kTreeStack];
+  int sp = 0;
+  stack[sp++] = root;
+  while (sp > 0) {
+    const int node = stack[--sp];
+    if (node < 0) {
+      visit((int)(d.t_key[f0 + ~node] & 0xFFFFFFFFu));
+      continue;
+    }
+    const double4 b = d.t_box[f0 + node];
+    if (b.x - ux > 0.0 || b.y - uy > 0.0 || lx - b.z > 0.0 || ly - b.w > 0.0) continue;
+    if (sp + 2 > kTreeStack) return false;
+    const int2 c = d.t_child[f0 + node];
+    stack[sp++] = c.y;
+    stack[sp++] = c.x;
+  }
+  return true;
+}
+
+// One candidate (moved proxy P at move-buffer position i, partner R) of a large world's pair
+// search: the rules of BroadPhase.update_pairs and find_new_contacts (exact FP64 overlap, not
+// already paired, same body, active, an existing contact, Body._should_collide, the default
+// filter, the contact registry), appended to the world's new-pair list.
+template <typename T>
+__device__ __forceinline__ void big_pair_candidate(const Dev<T>& d, int b0, int f0, const int* mpos,
+                                                   const uint32_t* hash, int hcap, uint32_t* nlist, int ncap,
+                                                   int* misc, int i, int P, int R) {
+  if (R == P) return;
+  const int mr = mpos[R];
+  if (mr >= 0 && mr < i) return;  // (R, P) is already in the pair set
+  const double4 fp = d.f_fat[f0 + P], fr = d.f_fat[f0 + R];
+  if (!aabb_overlap(fr.x, fr.y, fr.z, fr.w, fp.x, fp.y, fp.z, fp.w)) return;
+  const int4 ip = d.f_info[f0 + P], ir = d.f_info[f0 + R];
+  if (ip.x == ir.x) return;  // same body
+  const int tb_p = d.b_type[b0 + ip.x], tb_r = d.b_type[b0 + ir.x];
+  if (!(d.b_flags[b0 + ip.x] & BF_ACTIVE) || !(d.b_flags[b0 + ir.x] & BF_ACTIVE)) return;
+  {
+    const uint32_t k = hkey(P, R);
+    uint32_t h = hhash(k) & (uint32_t)(hcap - 1);
+    for (;;) {
+      const uint32_t v = hash[h];
+      if (v == 0xFFFFFFFFu) break;
+      if (v == k) return;
+      h = (h + 1) & (uint32_t)(hcap - 1);
+    }
+  }
+  if (tb_r != BT_DYNAMIC && tb_p != BT_DYNAMIC) return;  // Body._should_collide
+  bool ok;  // World.default_contact_filter(fixture P, fixture R)
+  if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
+  else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
+  if (!ok) return;
+  const int ta = ip.y & 255, tb = ir.y & 255;
+  const bool reg = (ta == SHAPE_CIRCLE || ta == SHAPE_POLYGON || ta == SHAPE_EDGE) &&
+                   (tb == SHAPE_CIRCLE || tb == SHAPE_POLYGON || tb == SHAPE_EDGE) &&
+                   !(ta == SHAPE_EDGE && tb == SHAPE_EDGE);
+  if (!reg) return;
+  const int slot = atomicAdd(&misc[3], 1);
+  if (slot < ncap) nlist[slot] = ((uint32_t)i << 16) | (uint32_t)R;
+  else atomicOr(&misc[4], 1);
+}
+
 // Every moved proxy of a large world against every proxy of the world (exact
 // FP64 tests), with the filters of find_new_contacts; grid (chunks, worlds).
 template <typename T>
@@ -2990,41 +3217,35 @@ __global__ void __launch_bounds__(kBlock) k_pairs_big_find(Dev<T> d) {
   const long long work = (long long)nmove * nf;
   for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < work; t += (long long)gridDim.x * kBlock) {
     const int i = (int)(t / nf), R = (int)(t % nf);
+    big_pair_candidate(d, b0, f0, mpos, hash, hcap, nlist, ncap, misc, i, mlist[i], R);
+  }
+}
+
+// The same search by traversal of the world's LBVH (built over its fat AABBs after the
+// synchronisation): one lane per moved proxy visits only the proxies whose subtree boxes overlap
+// its fat AABB; the candidate rules and the new-pair list are the same, and the list is put in
+// canonical order afterwards, so the contacts created are identical.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pairs_big_find_tree(Dev<T> d) {
+  const int w = d.big_list[blockIdx.y];
+  const int b0 = d.w_body_off[w];
+  const int f0 = d.w_fix_off[w];
+  const int ccap = d.w_con_cap[w];
+  int* misc = d.g_wmisc + 16 * w;
+  const int nmove = misc[2];
+  const int* mpos = d.g_mpos + f0;
+  const int* mlist = d.g_mlist + f0;
+  const int hcap = next_pow2(ccap + 1);
+  const uint32_t* hash = d.g_hash + d.g_hbase[w];
+  const int ncap = next_pow2(ccap);
+  uint32_t* nlist = d.g_nlist + d.g_nbase[w];
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nmove; i += gridDim.x * kBlock) {
     const int P = mlist[i];
-    if (R == P) continue;
-    const int mr = mpos[R];
-    if (mr >= 0 && mr < i) continue;  // (R, P) is already in the pair set
-    const double4 fp = d.f_fat[f0 + P], fr = d.f_fat[f0 + R];
-    if (!aabb_overlap(fr.x, fr.y, fr.z, fr.w, fp.x, fp.y, fp.z, fp.w)) continue;
-    const int4 ip = d.f_info[f0 + P], ir = d.f_info[f0 + R];
-    if (ip.x == ir.x) continue;  // same body
-    const int tb_p = d.b_type[b0 + ip.x], tb_r = d.b_type[b0 + ir.x];
-    if (!(d.b_flags[b0 + ip.x] & BF_ACTIVE) || !(d.b_flags[b0 + ir.x] & BF_ACTIVE)) continue;
-    {
-      const uint32_t k = hkey(P, R);
-      uint32_t h = hhash(k) & (uint32_t)(hcap - 1);
-      bool found = false;
-      for (;;) {
-        const uint32_t v = hash[h];
-        if (v == 0xFFFFFFFFu) break;
-        if (v == k) { found = true; break; }
-        h = (h + 1) & (uint32_t)(hcap - 1);
-      }
-      if (found) continue;
-    }
-    if (tb_r != BT_DYNAMIC && tb_p != BT_DYNAMIC) continue;  // Body._should_collide
-    bool ok;  // World.default_contact_filter(fixture P, fixture R)
-    if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
-    else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
-    if (!ok) continue;
-    const int ta = ip.y & 255, tb = ir.y & 255;
-    const bool reg = (ta == SHAPE_CIRCLE || ta == SHAPE_POLYGON || ta == SHAPE_EDGE) &&
-                     (tb == SHAPE_CIRCLE || tb == SHAPE_POLYGON || tb == SHAPE_EDGE) &&
-                     !(ta == SHAPE_EDGE && tb == SHAPE_EDGE);
-    if (!reg) continue;
-    const int slot = atomicAdd(&misc[3], 1);
-    if (slot < ncap) nlist[slot] = ((uint32_t)i << 16) | (uint32_t)R;
-    else atomicOr(&misc[4], 1);
+    const double4 fp = d.f_fat[f0 + P];
+    const bool ok = tree_visit(d, w, fp.x, fp.y, fp.z, fp.w, [&](int R) {
+      big_pair_candidate(d, b0, f0, mpos, hash, hcap, nlist, ncap, misc, i, P, R);
+    });
+    if (!ok) atomicOr(&d.w_err[w], WERR_SCHEDULE);
   }
 }
 
@@ -3134,6 +3355,77 @@ __global__ void __launch_bounds__(kBigBlock) k_pairs_big_finish(Dev<T> d, int fi
   if (tid == 0) {
     d.w_asleep[w] = sh[0];
     if (d.w_flags[w] & WF_NEW_FIXTURE) d.w_flags[w] &= ~WF_NEW_FIXTURE;
+  }
+}
+
+
+// World.query_aabb (world.py:371-384) for query q: every fixture of world qw[q] whose fat AABB
+// overlaps the box (DynamicTree.query, dynamictree.py:190-207, PyAABB.test_overlap with the
+// proxy's box first), appended as (q, fixture).
+template <typename T>
+__global__ void __launch_bounds__(256) k_query_aabb(Dev<T> d, int nq, const int* qw, const double4* qbox, int2* out,
+                                                    int cap, int* count) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const int w = qw[q];
+  const int f0 = d.w_fix_off[w];
+  const double4 a = qbox[q];
+  const bool ok = tree_visit(d, w, a.x, a.y, a.z, a.w, [&](int f) {
+    const double4 b = d.f_fat[f0 + f];
+    if (a.x - b.z > 0.0 || a.y - b.w > 0.0 || b.x - a.z > 0.0 || b.y - a.w > 0.0) return;
+    const int slot = atomicAdd(count, 1);
+    if (slot < cap) out[slot] = make_int2(q, f);
+  });
+  if (!ok) atomicOr(&d.w_err[w], WERR_SCHEDULE);
+}
+
+// World.ray_cast (world.py:386-440) for ray r (p1 -> p2 = seg[r]) in world rw[r]: every fixture
+// whose proxy passes the tree's leaf test and whose shape the ray hits.  closest != 0: one row
+// per ray, the hit of least fraction (ties: the lower fixture index), fixture -1 if none;
+// otherwise every hit is appended.  Rows: (ray, fixture) and (fraction, point x, y, normal x, y),
+// point = (1 - f) p1 + f p2.
+template <typename T>
+__global__ void __launch_bounds__(256) k_ray_cast(Dev<T> d, int nr, const int* rw, const double4* seg, int closest,
+                                                  int2* out_i, double* out_d, int cap, int* count) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const int w = rw[r];
+  const int f0 = d.w_fix_off[w], b0 = d.w_body_off[w];
+  const double4 p = seg[r];
+  const RaySeg s = ray_seg(p.x, p.y, p.z, p.w);
+  int best = -1;
+  RayHit bh{0.0, 0.0, 0.0};
+  auto emit = [&](int f, const RayHit& h) {
+    const double px = (1.0 - h.fraction) * p.x + h.fraction * p.z, py = (1.0 - h.fraction) * p.y + h.fraction * p.w;
+    const int slot = closest ? r : atomicAdd(count, 1);
+    if (slot < cap) {
+      out_i[slot] = make_int2(r, f);
+      double* o = out_d + 5 * (size_t)slot;
+      o[0] = h.fraction; o[1] = px; o[2] = py; o[3] = h.nx; o[4] = h.ny;
+    }
+  };
+  const bool ok = tree_visit(d, w, s.lx, s.ly, s.ux, s.uy, [&](int f) {
+    const double4 b = d.f_fat[f0 + f];
+    if (!ray_leaf_passes(s, b.x, b.y, b.z, b.w)) return;
+    const int4 info = d.f_info[f0 + f];
+    const double4 x = d.b_xf[b0 + info.x];
+    const XF64 xf{x.x, x.y, x.z, x.w};
+    RayHit h;
+    if (!ray_shape(geo_of(d, f0 + f), xf, p.x, p.y, p.z, p.w, h)) return;
+    if (!closest) emit(f, h);
+    else if (best < 0 || h.fraction < bh.fraction || (h.fraction == bh.fraction && f < best)) {
+      best = f;
+      bh = h;
+    }
+  });
+  if (!ok) atomicOr(&d.w_err[w], WERR_SCHEDULE);
+  if (closest) {
+    if (best >= 0) emit(best, bh);
+    else if (r < cap) {
+      out_i[r] = make_int2(r, -1);
+      double* o = out_d + 5 * (size_t)r;
+      o[0] = o[1] = o[2] = o[3] = o[4] = 0.0;
+    }
   }
 }
 
@@ -3254,6 +3546,7 @@ __global__ void k_pairs_seam(int n, const double4* fat, int nmove, const int* mo
   }
 }
 
+
 }  // namespace b2g
 
 // ===========================================================================
@@ -3333,6 +3626,9 @@ struct b2g_ctx {
   hipStream_t aux[kMaxRuns - 1] = {};  // streams of runs 1.. (run 0 stays on `stream`)
   hipEvent_t fork_ev = nullptr, join_ev[kMaxRuns - 1] = {};
   bool pending_new_fixture = false;
+  size_t tree_lds = 8;      // dynamic LDS of k_tree_build
+  bool tree_dirty = true;   // the fat AABBs changed since the trees were built
+  int tree_pairs = 0;       // B2G_PARAM_TREE_PAIRS
   // host mirrors of constant per-body / per-world data
   std::vector<int> body_off, fix_off, con_base, con_cap;
   std::vector<double> mass, inertia;
@@ -3472,6 +3768,8 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
     }
   }
   HIPCHK(hipFuncSetAttribute((const void*)k_solve<T>, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds));
+  HIPCHK(hipFuncSetAttribute((const void*)k_tree_build<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(kTreeLdsKeys * sizeof(unsigned long long))));
 
   HIPCHK(hipFuncSetAttribute((const void*)k_islands<T, kBlock>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)ctx->smem_islands));
@@ -3716,7 +4014,7 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.pv, nb));
   TRY(dalloc(ctx, &d.vc, ncap));
   TRY(dalloc(ctx, &d.pc, ncap));
-  std::vector<long long> hb(nw, 0), nbse(nw, 0);  // kept alive until the copies below complete
+  std::vector<long long> hb(nw, 0), nbse(nw, 0), tsb(nw, -1);  // kept alive until the copies below complete
   std::vector<int> zero16(16 * (size_t)nw, 0);
   std::vector<int> w_order(nw);
   for (int w = 0; w < nw; ++w) w_order[w] = w;
@@ -3782,6 +4080,32 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
     TRY(dalloc(ctx, &d.g_hash, (size_t)ht));
     TRY(dalloc(ctx, &d.g_nlist, (size_t)nt));
     TRY(upload(ctx, d.g_wmisc, zero16.data(), 16 * (size_t)nw));
+  }
+  // LBVH arrays; worlds whose keys do not fit the LDS sort get global sort scratch
+  TRY(dalloc(ctx, &d.t_key, nf));
+  TRY(dalloc(ctx, &d.t_child, nf));
+  TRY(dalloc(ctx, &d.t_box, nf));
+  TRY(dalloc(ctx, &d.t_flag, nf));
+  TRY(dalloc(ctx, &d.t_root, nw));
+  {
+    long long st = 0;
+    int lds_np = 1;
+    for (int w = 0; w < nw; ++w) {
+      const int np = next_pow2(std::max(1, ctx->fix_off[w + 1] - ctx->fix_off[w]));
+      if (np > kTreeLdsKeys) {
+        tsb[w] = st;
+        st += np;
+      } else {
+        lds_np = std::max(lds_np, np);
+      }
+    }
+    ctx->tree_lds = (size_t)lds_np * sizeof(unsigned long long);
+    long long* tsbd = nullptr;
+    TRY(dalloc(ctx, &tsbd, nw));
+    TRY(upload(ctx, tsbd, tsb.data(), nw));
+    d.t_sbase = tsbd;
+    TRY(dalloc(ctx, &d.t_sort, (size_t)std::max<long long>(st, 1)));
+    ctx->tree_dirty = true;
   }
 
   TRY(upload(ctx, w_body_off, b->body_off, nw + 1));
@@ -3881,7 +4205,12 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     }));
     if (nbig > 0) {
       hipLaunchKernelGGL(k_pairs_big_prep<T>, dim3(nbig), dim3(kBigBlock), 0, st, d, from_upload, do_pairs);
-      hipLaunchKernelGGL(k_pairs_big_find<T>, dim3(256, nbig), dim3(kBlock), 0, st, d);
+      if (ctx->tree_pairs && do_pairs) {  // LBVH over the large worlds' current fat AABBs
+        hipLaunchKernelGGL(k_tree_build<T>, dim3(nbig), dim3(kTreeBlock), ctx->tree_lds, st, d, d.big_list);
+        hipLaunchKernelGGL(k_pairs_big_find_tree<T>, dim3(64, nbig), dim3(kBlock), 0, st, d);
+      } else {
+        hipLaunchKernelGGL(k_pairs_big_find<T>, dim3(256, nbig), dim3(kBlock), 0, st, d);
+      }
       hipLaunchKernelGGL(k_pairs_big_finish<T>, dim3(nbig), dim3(kBigBlock), 0, st, d, finish);
     }
     return 0;
@@ -3948,6 +4277,7 @@ static int ensure_island_times(b2g_ctx* ctx) {
 }
 
 static int dispatch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, hipEvent_t* ev = nullptr) {
+  ctx->tree_dirty = true;
   if (ctx->precision == B2G_FP64) return launch_step<double>(ctx, dt, vi, pi, first_pairs, ev);
   return launch_step<float>(ctx, dt, vi, pi, first_pairs, ev);
 }
@@ -4086,6 +4416,116 @@ static int download_batch(b2g_ctx* ctx, b2g_batch_out* o) {
 // ===========================================================================
 // C-ABI
 // ===========================================================================
+
+// The trees of every world, rebuilt when a step or an upload changed the fat AABBs.
+template <typename T>
+static int ensure_trees_t(b2g_ctx* ctx) {
+  if (!ctx->tree_dirty) return 0;
+  const Dev<T>& d = *(Dev<T>*)ctx->dev;
+  if (ctx->nw > 0)
+    hipLaunchKernelGGL(k_tree_build<T>, dim3(ctx->nw), dim3(kTreeBlock), ctx->tree_lds, ctx->stream, d, d.w_order);
+  HIPCHK(hipGetLastError());
+  ctx->tree_dirty = false;
+  return 0;
+}
+static int ensure_trees(b2g_ctx* ctx) {
+  return ctx->precision == B2G_FP64 ? ensure_trees_t<double>(ctx) : ensure_trees_t<float>(ctx);
+}
+
+// Device staging for a query call: inputs copied in, outputs copied back, all freed on return.
+struct Staging {
+  std::vector<void*> p;
+  ~Staging() {
+    for (void* q : p) hipFree(q);
+  }
+  void* get(size_t bytes) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+    p.push_back(q);
+    return q;
+  }
+};
+
+template <typename T>
+static int query_aabb_t(b2g_ctx* ctx, int n, const int32_t* world, const double* box, int32_t* out, int cap,
+                        int32_t* count) {
+  const Dev<T>& d = *(Dev<T>*)ctx->dev;
+  Staging sg;
+  int* dw = (int*)sg.get((size_t)n * 4);
+  double* db = (double*)sg.get((size_t)n * 32);
+  int2* dout = (int2*)sg.get((size_t)std::max(cap, 1) * 8);
+  int* dcnt = (int*)sg.get(4);
+  if (!dw || !db || !dout || !dcnt) return set_error(ctx, B2G_E_HIP, "hipMalloc failed (query staging)");
+  TRY(upload(ctx, dw, world, (size_t)n));
+  TRY(upload(ctx, db, box, 4 * (size_t)n));
+  HIPCHK(hipMemsetAsync(dcnt, 0, 4, ctx->stream));
+  TRY(ensure_trees(ctx));
+  hipLaunchKernelGGL(k_query_aabb<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d, n, (const int*)dw,
+                     (const double4*)db, dout, cap, dcnt);
+  HIPCHK(hipGetLastError());
+  int total = 0;
+  HIPCHK(hipMemcpyAsync(&total, dcnt, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *count = total;
+  const int m = std::min(total, cap);
+  std::vector<int2> h((size_t)m);
+  if (m > 0) HIPCHK(hipMemcpy(h.data(), dout, (size_t)m * 8, hipMemcpyDeviceToHost));
+  std::sort(h.begin(), h.end(), [](int2 a, int2 b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+  for (int k = 0; k < m; ++k) {
+    out[2 * k] = h[k].x;
+    out[2 * k + 1] = h[k].y;
+  }
+  return total > cap ? set_error(ctx, B2G_E_CAPACITY, "query results exceed the output capacity") : B2G_OK;
+}
+
+template <typename T>
+static int ray_cast_t(b2g_ctx* ctx, int n, const int32_t* world, const double* seg, int closest, int32_t* out_i,
+                      double* out_d, int cap, int32_t* count) {
+  const Dev<T>& d = *(Dev<T>*)ctx->dev;
+  Staging sg;
+  const int rows = closest ? n : std::max(cap, 1);
+  int* dw = (int*)sg.get((size_t)n * 4);
+  double* ds = (double*)sg.get((size_t)n * 32);
+  int2* di = (int2*)sg.get((size_t)rows * 8);
+  double* dd = (double*)sg.get((size_t)rows * 40);
+  int* dcnt = (int*)sg.get(4);
+  if (!dw || !ds || !di || !dd || !dcnt) return set_error(ctx, B2G_E_HIP, "hipMalloc failed (ray cast staging)");
+  TRY(upload(ctx, dw, world, (size_t)n));
+  TRY(upload(ctx, ds, seg, 4 * (size_t)n));
+  HIPCHK(hipMemsetAsync(dcnt, 0, 4, ctx->stream));
+  TRY(ensure_trees(ctx));
+  hipLaunchKernelGGL(k_ray_cast<T>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d, n, (const int*)dw,
+                     (const double4*)ds, closest, di, dd, closest ? n : cap, dcnt);
+  HIPCHK(hipGetLastError());
+  int total = 0;
+  HIPCHK(hipMemcpyAsync(&total, dcnt, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (closest) total = n;
+  *count = total;
+  const int m = std::min(total, closest ? n : cap);
+  std::vector<int2> hi((size_t)m);
+  std::vector<double> hd((size_t)m * 5);
+  if (m > 0) {
+    HIPCHK(hipMemcpy(hi.data(), di, (size_t)m * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hd.data(), dd, (size_t)m * 40, hipMemcpyDeviceToHost));
+  }
+  std::vector<int> ord((size_t)m);
+  for (int k = 0; k < m; ++k) ord[k] = k;
+  if (!closest)  // by ray, then fraction, then fixture
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) {
+      if (hi[a].x != hi[b].x) return hi[a].x < hi[b].x;
+      if (hd[5 * a] != hd[5 * b]) return hd[5 * a] < hd[5 * b];
+      return hi[a].y < hi[b].y;
+    });
+  for (int k = 0; k < m; ++k) {
+    out_i[2 * k] = hi[ord[k]].x;
+    out_i[2 * k + 1] = hi[ord[k]].y;
+    for (int j = 0; j < 5; ++j) out_d[5 * k + j] = hd[5 * ord[k] + j];
+  }
+  return (!closest && total > cap) ? set_error(ctx, B2G_E_CAPACITY, "ray-cast hits exceed the output capacity")
+                                   : B2G_OK;
+}
+
 extern "C" {
 
 B2G_API int b2g_version(void) { return B2G_VERSION; }
@@ -4280,6 +4720,7 @@ B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value) {
     case B2G_PARAM_PAIRS_WIDE: slot = &ctx->pairs_wide; lo = 0; hi = 32; break;
     case B2G_PARAM_ISLANDS_WIDE: slot = &ctx->islands_wide; lo = 0; hi = 32; break;
     case B2G_PARAM_COLLIDE_BLOCK: slot = &ctx->collide_block; lo = 64; hi = 512; break;
+    case B2G_PARAM_TREE_PAIRS: slot = &ctx->tree_pairs; lo = 0; hi = 1; break;
     case B2G_PARAM_SOLVE_WAVES_PER_CU:
       return set_error(ctx, B2G_E_INVALID, "b2g_set_param: B2G_PARAM_SOLVE_WAVES_PER_CU is read-only");
     case B2G_PARAM_SEARCH_RUNS:
@@ -4319,6 +4760,7 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value) {
     case B2G_PARAM_PAIRS_WIDE: *value = ctx->pairs_wide; return B2G_OK;
     case B2G_PARAM_ISLANDS_WIDE: *value = ctx->islands_wide; return B2G_OK;
     case B2G_PARAM_COLLIDE_BLOCK: *value = ctx->collide_block; return B2G_OK;
+    case B2G_PARAM_TREE_PAIRS: *value = ctx->tree_pairs; return B2G_OK;
     case B2G_PARAM_SEARCH_RUNS: *value = (int)ctx->isl_runs.size() | (int)ctx->pair_runs.size() << 8; return B2G_OK;
     case B2G_PARAM_SOLVE_WAVES_PER_CU: {
       // resident solver wavefronts per compute unit: 4 SIMDs x (512 VGPRs / VGPRs per lane), and
@@ -4368,6 +4810,39 @@ B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int 
   for (int k = 0; k < 5; ++k) ms_out[k] = (float)(acc[k] / n);
   ctx->stat_steps += n;
   return B2G_OK;
+}
+
+B2G_API int b2g_query_aabb(b2g_ctx* ctx, int n, const int32_t* world, const double* box, int32_t* out, int cap,
+                           int32_t* count) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (n < 0 || cap < 0 || !count || (n > 0 && (!world || !box)) || (cap > 0 && !out))
+    return set_error(ctx, B2G_E_INVALID, "b2g_query_aabb: bad arguments");
+  *count = 0;
+  for (int k = 0; k < n; ++k)
+    if (world[k] < 0 || world[k] >= ctx->nw) return set_error(ctx, B2G_E_INVALID, "b2g_query_aabb: bad world index");
+  if (n == 0) return B2G_OK;
+  hipSetDevice(ctx->device);
+  return ctx->precision == B2G_FP64 ? query_aabb_t<double>(ctx, n, world, box, out, cap, count)
+                                    : query_aabb_t<float>(ctx, n, world, box, out, cap, count);
+}
+
+B2G_API int b2g_ray_cast(b2g_ctx* ctx, int n, const int32_t* world, const double* seg, int32_t closest, int32_t* out_i,
+                         double* out_d, int cap, int32_t* count) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (n < 0 || cap < 0 || !count || (n > 0 && (!world || !seg)) || ((closest ? n : cap) > 0 && (!out_i || !out_d)))
+    return set_error(ctx, B2G_E_INVALID, "b2g_ray_cast: bad arguments");
+  if (closest && cap < n) return set_error(ctx, B2G_E_INVALID, "b2g_ray_cast: closest hits need cap >= n");
+  *count = 0;
+  for (int k = 0; k < n; ++k) {
+    if (world[k] < 0 || world[k] >= ctx->nw) return set_error(ctx, B2G_E_INVALID, "b2g_ray_cast: bad world index");
+    const double rx = seg[4 * k + 2] - seg[4 * k], ry = seg[4 * k + 3] - seg[4 * k + 1];
+    if (!(rx * rx + ry * ry > 0.0))  // DynamicTree.ray_cast raises ValueError (dynamictree.py:225-227)
+      return set_error(ctx, B2G_E_INVALID, "b2g_ray_cast: ||p2 - p1||^2 must be > 0");
+  }
+  if (n == 0) return B2G_OK;
+  hipSetDevice(ctx->device);
+  return ctx->precision == B2G_FP64 ? ray_cast_t<double>(ctx, n, world, seg, closest, out_i, out_d, cap, count)
+                                    : ray_cast_t<float>(ctx, n, world, seg, closest, out_i, out_d, cap, count);
 }
 
 B2G_API int b2g_stats(b2g_ctx* ctx, int64_t* out, int32_t n, int32_t reset) {

@@ -8,6 +8,9 @@ World.add_body / Body._attached_to_world / Fixture._create_proxies compute
 them (world.py:183-224, body.py:288-308, fixture.py:250-261,
 dynamictree.py:115-134).
 
+``World.query_aabb`` and ``World.ray_cast`` run on an LBVH the device builds
+over the fat AABBs of the current state.
+
 Outside the device path, and therefore refused with NotImplementedError when a
 world holding them is stepped: continuous physics (TOI), joints, controllers,
 sensors, loop shapes, contact listeners and custom contact filters.
@@ -440,6 +443,70 @@ class World(object):
                 self._inv_dt0 = 1.0 / dt
         finally:
             self._locked = False
+
+    # ---- queries (world.py:371-440), on the device's LBVH over the fat AABBs ----
+    def _device_state(self):
+        """The device batch of this world, current with the Body objects."""
+        if self._host_dirty or self._stepper is None:
+            self._pull()
+            self._push()
+        return self._stepper.engine
+
+    def query_aabb(self, aabb, callback=None):
+        """Fixtures whose fat AABB (the broad-phase proxy) overlaps `aabb` (World.query_aabb,
+        world.py:371-384).  As in the reference, the result is a generator: without a callback it
+        yields the fixtures; with one, iterating it calls the callback with each fixture until the
+        callback returns a false value, and yields nothing.  Fixtures come in fixture order (the
+        reference's order is that of its tree, which is not part of the result)."""
+        lo, hi = (aabb.lower_bound, aabb.upper_bound) if hasattr(aabb, "lower_bound") else aabb
+        box = np.array([[lo[0], lo[1], hi[0], hi[1]]], np.float64)
+        rows = self._device_state().query_aabb(np.zeros(1, np.int32), box)
+        fixtures = self._fixture_list()
+        found = [fixtures[int(f)] for f in rows[:, 1]]
+        return self._query_hits(found, callback)
+
+    @staticmethod
+    def _query_hits(found, callback):
+        for f in found:
+            if callback is None:
+                yield f
+            elif not callback(f):
+                break
+
+    def ray_cast(self, point1, point2, callback=None):
+        """Fixtures hit by the segment point1 -> point2 (World.ray_cast, world.py:386-440): each hit as
+        [fixture, point, normal, fraction].  As in the reference, the result is a generator: without a
+        callback it yields each hit's list, whose last entry the caller may set; with one, iterating
+        it calls the callback with each hit and yields nothing.  That value (or the callback's return
+        value) steers the cast as in the reference: -1 ignores the hit, 0 terminates, a fraction f
+        clips the ray so that hits beyond f are not reported, 1 continues.  Hits are reported in order
+        of increasing fraction (ties: fixture order); shapes that contain point1 are not hit.
+        Raises ValueError when point1 == point2."""
+        p1, p2 = Vec2(*point1), Vec2(*point2)
+        if (p2 - p1).length_squared <= 0.0:
+            raise ValueError("||p2 - p1||^2 > 0.0")
+        seg = np.array([[p1.x, p1.y, p2.x, p2.y]], np.float64)
+        rows, data = self._device_state().ray_cast(np.zeros(1, np.int32), seg, closest=False)
+        fixtures = self._fixture_list()
+        hits = [(fixtures[int(f)], Vec2(d[1], d[2]), Vec2(d[3], d[4]), float(d[0])) for (_, f), d in zip(rows, data)]
+        return self._ray_hits(hits, callback)
+
+    @staticmethod
+    def _ray_hits(hits, callback):
+        max_fraction = 1.0
+        for fixture, point, normal, fraction in hits:
+            if fraction > max_fraction:
+                break
+            ret = [fixture, point, normal, fraction]
+            if callback is not None:
+                value = callback(*ret)
+            else:
+                yield ret
+                value = ret[-1]
+            if value == 0.0:
+                return
+            if value > 0.0:  # DynamicTree.ray_cast: the returned value becomes the max fraction
+                max_fraction = value
 
     def clear_forces(self):
         self._pull()

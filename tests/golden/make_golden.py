@@ -1,7 +1,7 @@
 """Writes the golden fixtures of the contact pipeline (tests/golden/*.npz).
 
 Run once in the build container:  python tests/golden/make_golden.py [names]
-(names: pyramid stacks mixed filtered narrowphase; default all)
+(names: pyramid stacks mixed filtered queries narrowphase; default all)
 
 It imports pypybox2d from /root/reference in its default pure-Python flavour
 (the reference tree ships no compiled _common) and records, for small scenes:
@@ -350,9 +350,89 @@ def run_narrowphase(cases):
                 kinds=np.array(kinds))
 
 
+# --------------------------------------------------------------------------
+# queries and ray casts
+# --------------------------------------------------------------------------
+def shape_ray_cases(rng, n_per_kind=120):
+    """Random shapes at random transforms and random segments near them, through the reference's
+    Shape.ray_cast(p1, p2, 1.0, xf, 0) (shapes.py:147-174, 265-307, 655-702)."""
+    rows = []
+    for kind in ("circle", "polygon", "edge"):
+        for i in range(n_per_kind):
+            if kind == "circle":
+                sh = b2.Circle(radius=float(rng.uniform(0.2, 1.5)), position=tuple(rng.uniform(-0.5, 0.5, 2)))
+            elif kind == "polygon":
+                verts, _ = _rand_polygon(rng)
+                sh = copy(b2.Polygon(vertices=verts))
+            else:
+                h = float(rng.uniform(0.5, 4.0))
+                sh = b2.Edge((-h, float(rng.uniform(-0.5, 0.5))), (h, float(rng.uniform(-0.5, 0.5))))
+            pos = tuple(rng.uniform(-5, 5, 2))
+            xf = b2.Transform(position=pos, angle=float(rng.uniform(-np.pi, np.pi)))
+            th = float(rng.uniform(-np.pi, np.pi))
+            p2 = np.array(pos) + rng.uniform(-1.0, 1.0, 2)
+            p1 = p2 + float(rng.uniform(1.0, 6.0)) * np.array([math.cos(th), math.sin(th)])
+            if i % 4 == 0:  # through the shape's origin
+                p2 = np.array(pos) - (p1 - np.array(pos)) * 0.5
+            if i % 9 == 0:  # starting inside
+                p1 = np.array(pos) + rng.uniform(-0.1, 0.1, 2)
+            fx = b2.Fixture(sh)
+            fd, fi = fixture_record(fx)
+            hit, normal, fraction = sh.ray_cast(b2.Vec2(*p1), b2.Vec2(*p2), 1.0, xf, 0)
+            rows.append((fd, fi, (xf._position.x, xf._position.y, xf._rotation._col1.x, xf._rotation._col1.y),
+                         (float(p1[0]), float(p1[1]), float(p2[0]), float(p2[1])), bool(hit),
+                         (float(fraction), normal.x if hit else 0.0, normal.y if hit else 0.0)))
+    return {"fd": np.array([r[0] for r in rows]), "fi": np.array([r[1] for r in rows]),
+            "xf": np.array([r[2] for r in rows]), "seg": np.array([r[3] for r in rows]),
+            "hit": np.array([r[4] for r in rows]), "out": np.array([r[5] for r in rows])}
+
+
+def world_queries(scene, steps, rng, nq=200, nr=200):
+    """A reference world after `steps` steps, with World.query_aabb over random boxes (the set of
+    fixtures yielded) and World.ray_cast over random segments with every hit continued unclipped
+    (ret[-1] = 1): (fixture, point, normal, fraction) of each hit, in the reference's order."""
+    world = build_ref_world(scene)
+    for _ in range(steps):
+        world.step(1.0 / 60, 8, 3)
+    state, fidx = snapshot(world)
+    bodies = [b for b in world.bodies]
+    xs = [b.position.x for b in bodies]
+    ys = [b.position.y for b in bodies]
+    lo, hi = (min(xs) - 3.0, min(ys) - 3.0), (max(xs) + 3.0, max(ys) + 3.0)
+    boxes, qoff, qfix = [], [0], []
+    for _ in range(nq):
+        c = (float(rng.uniform(lo[0], hi[0])), float(rng.uniform(lo[1], hi[1])))
+        e = rng.uniform(0.1, 4.0, 2)
+        box = (c[0] - float(e[0]), c[1] - float(e[1]), c[0] + float(e[0]), c[1] + float(e[1]))
+        boxes.append(box)
+        aabb = b2.AABB(lower_bound=(box[0], box[1]), upper_bound=(box[2], box[3]))
+        found = sorted(fidx[id(f)] for f in world.query_aabb(aabb))
+        qfix.extend(found)
+        qoff.append(len(qfix))
+    segs, roff, rfix, rdat = [], [0], [], []
+    for _ in range(nr):
+        p1 = (float(rng.uniform(lo[0], hi[0])), float(rng.uniform(lo[1], hi[1])))
+        p2 = (float(rng.uniform(lo[0], hi[0])), float(rng.uniform(lo[1], hi[1])))
+        segs.append(p1 + p2)
+        hits = []
+        for ret in world.ray_cast(p1, p2):
+            f, point, normal, fraction = ret
+            hits.append((fidx[id(f)], fraction, point.x, point.y, normal.x, normal.y))
+            ret[-1] = 1.0
+        hits.sort()
+        rfix.extend(h[0] for h in hits)
+        rdat.extend(h[1:] for h in hits)
+        roff.append(len(rfix))
+    out = {"init_" + n: getattr(state, n) for n in L._FIELDS}
+    out.update(boxes=np.array(boxes), q_off=np.array(qoff, np.int32), q_fix=np.array(qfix, np.int32),
+               segs=np.array(segs), r_off=np.array(roff, np.int32), r_fix=np.array(rfix, np.int32),
+               r_dat=np.array(rdat).reshape(-1, 5))
+    return out
+
+
 def main(which=None):
-    """Records every fixture, or only those named (pyramid, stacks, mixed, filtered, narrowphase)."""
-    want = set(which or ("pyramid", "stacks", "mixed", "filtered", "narrowphase"))
+    """Records every fixture, or only those named (pyramid, stacks, mixed, filtered, queries, narrowphase)."""
+    want = set(which or ("pyramid", "stacks", "mixed", "filtered", "queries", "narrowphase"))
     if "pyramid" in want:
         pyr_ck = {1, 2, 3, 4, 5, 10, 20, 50, 100, 150, 200, 300, 400, 500, 600}
         init, hints, states = record_run(scenes.pyramid(), 600, pyr_ck)
@@ -371,6 +451,15 @@ def main(which=None):
         filt_ck = {1, 2, 3, 5, 10, 20, 40, 60, 80, 100, 120, 150, 200, 250, 300}
         init, hints, states = record_run(scenes.filtered_world(), 300, filt_ck)
         save_run(os.path.join(HERE, "filtered_kinematic_300.npz"), init, hints, states, {"steps": 300})
+    if "queries" in want:
+        np.savez_compressed(os.path.join(HERE, "shape_raycast.npz"), **shape_ray_cases(np.random.default_rng(17)))
+        rng = np.random.default_rng(19)
+        np.savez_compressed(os.path.join(HERE, "queries_mixed24.npz"),
+                            **world_queries(scenes.mixed_world(np.random.default_rng(7), 24, half_width=8.0,
+                                                               height=12.0), 120, rng))
+        np.savez_compressed(os.path.join(HERE, "queries_filtered.npz"),
+                            **world_queries(scenes.filtered_world(), 150, rng))
+        print("query and ray-cast fixtures written")
     if "narrowphase" in want:
         cases = narrowphase_cases(np.random.default_rng(11))
         np.savez_compressed(os.path.join(HERE, "narrowphase.npz"), **run_narrowphase(cases))

