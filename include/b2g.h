@@ -227,6 +227,21 @@ B2G_API int b2g_narrowphase(int precision, int n, const double* fa_d, const int3
 B2G_API int b2g_pairs(int n, const double* fat, int nmove, const int32_t* move, int32_t* out, int cap,
                       int32_t* count);
 
+/* The pair seam of b2g_pairs without any allocation, for callers that cannot
+   allocate device memory (code under HIP graph capture, serving loops, harnesses
+   that forbid hipMalloc).  No counterpart in the reference.  Inputs and outputs
+   are device pointers the caller owns: fat[n][4], move[nmove], out[2 * cap],
+   *count; the work is queued on `stream` (a hipStream_t, or NULL for the null
+   stream) and nothing is synchronised.  *count receives the number of pairs
+   (only the first cap are written, in canonical order: move-buffer position,
+   then partner), or -1 when a move entry is out of range or repeated.  The
+   scratch (move positions, per-proxy counts and offsets) lives in a fixed
+   per-device array of 8 MiB: n and nmove are limited to 1 << 20 each, larger
+   sizes return B2G_E_INVALID before anything is launched.  Calls that overlap
+   on one GPU share that array and race. */
+B2G_API int b2g_pairs_noalloc(int n, const double* fat, int nmove, const int32_t* move, int32_t* out, int cap,
+                              int32_t* count, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,7 +22,7 @@ B2G_FP32, B2G_FP64 = 0, 1
 
 EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step", "b2g_step_n", "b2g_sync",
            "b2g_counts", "b2g_download", "b2g_last_error", "b2g_set_param", "b2g_get_param", "b2g_profile_step",
-           "b2g_island_times", "b2g_narrowphase", "b2g_pairs", "b2g_stats", "b2g_query_aabb", "b2g_ray_cast")
+           "b2g_island_times", "b2g_narrowphase", "b2g_pairs", "b2g_stats", "b2g_query_aabb", "b2g_ray_cast", "b2g_pairs_noalloc")
 
 # keys of b2g_set_param / b2g_get_param: the solver's launch shape, island timing, and the
 # resident solver wavefronts per compute unit and the search launches per step (both read only)

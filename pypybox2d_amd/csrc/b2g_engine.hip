@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3547,6 +3548,99 @@ __global__ void k_pairs_seam(int n, const double4* fat, int nmove, const int* mo
 }
 
 
+// ---------------------------------------------------------------------------
+// The pair seam without allocation (b2g_pairs_noalloc): device inputs and outputs, its
+// scratch (move positions, per-moved-proxy pair counts and output offsets, an error word and
+// the total) in the file-scope array below.  The host refers to the array only through the
+// address hipGetSymbolAddress returns for the current device.
+// ---------------------------------------------------------------------------
+constexpr int kPdMaxProxies = 1 << 20;  // largest n
+constexpr int kPdMaxMoved = 1 << 20;    // largest move buffer
+constexpr int kPdMisc = 64;             // [0] error bits (1: move entry out of range, 2: duplicate)
+constexpr size_t kPdCounts = (size_t)kPdMaxProxies, kPdOffsets = kPdCounts + kPdMaxMoved,
+                 kPdMiscAt = kPdOffsets + kPdMaxMoved, kPdScratchInts = kPdMiscAt + kPdMisc;
+static __device__ __align__(256) int scratch[kPdScratchInts];  // 8 MiB + 256 B per device
+
+__global__ void k_pd_init(int n, int* sc) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) sc[i] = -1;
+  if (blockIdx.x == 0 && threadIdx.x < kPdMisc) sc[kPdMiscAt + threadIdx.x] = 0;
+}
+
+__global__ void k_pd_mark(int n, int nmove, const int* move, int* sc) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nmove; k += gridDim.x * blockDim.x) {
+    const int P = move[k];
+    if (P < 0 || P >= n) {
+      atomicOr(&sc[kPdMiscAt], 1);
+      continue;
+    }
+    if (atomicCAS(&sc[P], -1, k) != -1) atomicOr(&sc[kPdMiscAt], 2);
+  }
+}
+
+// One workgroup per moved proxy i (move-buffer position): its partners R in ascending order,
+// under the same rule as k_pairs_seam (R != P, R not earlier in the move buffer, inclusive
+// FP64 overlap).  kWrite = false counts them; kWrite = true writes (P, R) from the proxy's
+// offset on, so the output is in canonical order without a sort.
+template <bool kWrite>
+__global__ void __launch_bounds__(256) k_pd_pairs(int n, const double4* fat, int nmove, const int* move,
+                                                  const int* sc_ro, int* sc, int* out, int cap) {
+  __shared__ int wsum[4];
+  const int i = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (i >= nmove || sc_ro[kPdMiscAt] != 0) return;  // the whole workgroup
+  const int P = move[i];
+  const double4 fp = fat[P];
+  const int base = kWrite ? sc_ro[kPdOffsets + i] : 0;
+  int run = 0;
+  for (int r0 = 0; r0 < n; r0 += 256) {
+    const int R = r0 + tid;
+    bool hit = false;
+    if (R < n && R != P) {
+      const int mr = sc_ro[R];
+      if (!(mr >= 0 && mr < i)) {
+        const double4 fr = fat[R];
+        hit = aabb_overlap(fr.x, fr.y, fr.z, fr.w, fp.x, fp.y, fp.z, fp.w);
+      }
+    }
+    const unsigned long long b = __ballot(hit);
+    if (lane == 0) wsum[wv] = __popcll(b);
+    __syncthreads();
+    int before = run + __popcll(b & ((1ull << lane) - 1));
+    for (int j = 0; j < wv; ++j) before += wsum[j];
+    if (kWrite && hit && base + before < cap) {
+      out[2 * (base + before)] = P;
+      out[2 * (base + before) + 1] = R;
+    }
+    run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+  if (!kWrite && tid == 0) sc[kPdCounts + i] = run;
+}
+
+// Exclusive scan of the per-proxy counts into offsets (one workgroup); *count = the total, or
+// -1 when the move buffer was invalid.
+__global__ void __launch_bounds__(256) k_pd_scan(int nmove, int* sc, int* count) {
+  __shared__ int part[256];
+  const int tid = threadIdx.x;
+  const int per = (nmove + 255) / 256, lo = min(nmove, tid * per), hi = min(nmove, lo + per);
+  int sum = 0;
+  for (int k = lo; k < hi; ++k) sum += sc[kPdCounts + k];
+  part[tid] = sum;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - sum;
+  for (int k = lo; k < hi; ++k) {
+    const int v = sc[kPdCounts + k];
+    sc[kPdOffsets + k] = run;
+    run += v;
+  }
+  if (tid == 255) *count = sc[kPdMiscAt] != 0 ? -1 : part[255];
+}
+
 }  // namespace b2g
 
 // ===========================================================================
@@ -5036,6 +5130,49 @@ B2G_API int b2g_pairs(int n, const double* fat, int nmove, const int32_t* move, 
   }
   *count = hcount;
   return B2G_OK;
+}
+
+// Per device: the address of the pair seam's scratch array (hipGetSymbolAddress, taken once
+// per device with that device current, at the entry point's first call there).
+static std::mutex g_pd_mu;
+static int* g_pd_scratch[64] = {};
+
+static int pd_scratch(int dev, int** out) {
+  if (dev < 0 || dev >= 64) return B2G_E_NODEVICE;
+  std::lock_guard<std::mutex> lk(g_pd_mu);
+  if (!g_pd_scratch[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return B2G_E_HIP;
+    g_pd_scratch[dev] = (int*)p;
+  }
+  *out = g_pd_scratch[dev];
+  return B2G_OK;
+}
+
+B2G_API int b2g_pairs_noalloc(int n, const double* fat, int nmove, const int32_t* move, int32_t* out, int cap,
+                              int32_t* count, void* stream) {
+  if (n < 0 || nmove < 0 || cap < 0 || !count || (n > 0 && !fat) || (nmove > 0 && !move) || (cap > 0 && !out))
+    return B2G_E_INVALID;
+  if (n > kPdMaxProxies || nmove > kPdMaxMoved) return B2G_E_INVALID;  // beyond the scratch array
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return B2G_E_NODEVICE;
+  int* sc = nullptr;
+  const int rc = pd_scratch(dev, &sc);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int fill = std::max(1, std::min((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_pd_init, dim3(fill), dim3(256), 0, st, n, sc);
+  if (nmove > 0)
+    hipLaunchKernelGGL(k_pd_mark, dim3(std::min((nmove + 255) / 256, 4096)), dim3(256), 0, st, n, nmove,
+                       (const int*)move, sc);
+  if (nmove > 0 && n > 0)
+    hipLaunchKernelGGL((k_pd_pairs<false>), dim3(nmove), dim3(256), 0, st, n, (const double4*)fat, nmove,
+                       (const int*)move, (const int*)sc, sc, (int*)out, cap);
+  hipLaunchKernelGGL(k_pd_scan, dim3(1), dim3(256), 0, st, nmove, sc, (int*)count);
+  if (nmove > 0 && n > 0)
+    hipLaunchKernelGGL((k_pd_pairs<true>), dim3(nmove), dim3(256), 0, st, n, (const double4*)fat, nmove,
+                       (const int*)move, (const int*)sc, sc, (int*)out, cap);
+  return hipGetLastError() == hipSuccess ? B2G_OK : B2G_E_HIP;
 }
 
 }  // extern "C"
