@@ -153,6 +153,13 @@ B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);
                           testing every moved proxy against every proxy; 0
                           (default): the all-pairs test.  Same pairs, same order. */
 #define B2G_PARAM_TREE_PAIRS 13
+/* B2G_PARAM_EVENTS       the contact listener's callbacks to record, a mask of
+                          1 << B2G_EV_BEGIN / END / PRE_SOLVE / POST_SOLVE
+                          (0, default: none).  Records are appended by the step's
+                          kernels and read with b2g_events; the buffers (4 records
+                          of 224 bytes per contact slot) are allocated when a mask
+                          is first set. */
+#define B2G_PARAM_EVENTS 14
 B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value);
 B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
 
@@ -210,6 +217,62 @@ B2G_API int b2g_ray_cast(b2g_ctx* ctx, int n, const int32_t* world, const double
    counterpart in the reference (the per-step counts that set the step's
    algorithmic bytes). */
 B2G_API int b2g_stats(b2g_ctx* ctx, int64_t* out, int32_t n, int32_t reset);
+
+/* The contact listener of the last step (ContactManager.begin_contact,
+   end_contact, pre_solve, post_solve; contact_manager.py:47-98): the records of
+   the kinds enabled by B2G_PARAM_EVENTS (layout in b2g_layout.h), in the order
+   the reference calls the listener: by world, then phase (ContactManager.collide's
+   loop over the contacts, contact.py:262-268; the end_contact of the contacts it
+   destroys, contact_manager.py:160-172; Island.report, island.py:245-252).
+   *count receives the number of records; with more than cap nothing is written
+   and B2G_E_CAPACITY is returned.  b2g_step_n keeps its last step's records. */
+B2G_API int b2g_events(b2g_ctx* ctx, int32_t* out_i, double* out_d, int32_t cap, int32_t* count);
+
+/* One step in two parts, so that the contact listener's begin_contact,
+   end_contact and pre_solve run between them as in the reference, where
+   pre_solve may disable a contact for this step (Contact.enabled = False,
+   contact.py:116-127; testbed/one_sided_platform.py): b2g_step_collide runs the
+   step up to and with ContactManager.collide (the pending new-fixture pair
+   search, the narrow phase; world.py:316-338), b2g_events then holds that
+   part's records, b2g_disable_contacts applies the listener's verdicts, and
+   b2g_step_solve runs the rest (world.py:340-354: islands, solve, Island.report,
+   synchronisation, find_new_contacts).  The two parts give the same state as
+   b2g_step.  No other step call is accepted in between. */
+B2G_API int b2g_step_collide(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters);
+B2G_API int b2g_step_solve(b2g_ctx* ctx);
+
+/* contact.enabled = False for this step: contact slot slot[k] of world world[k]
+   (the slot is a phase-0 event's key / 4, its place in the world's contact list
+   during collide).  Only between b2g_step_collide and b2g_step_solve. */
+B2G_API int b2g_disable_contacts(b2g_ctx* ctx, int32_t n, const int32_t* world, const int32_t* slot);
+
+/* The new-fixture pair search of the next step run ahead of it
+   (World.step -> ContactManager.find_new_contacts when fixtures were added,
+   world.py:316-319), so that a custom contact filter can see the new contacts
+   before the step uses them.  The next b2g_step does not repeat it.  With no
+   search pending, nothing is new. */
+B2G_API int b2g_pair_search(b2g_ctx* ctx);
+
+/* The contacts created by each world's last pair search (the step's
+   ContactManager.find_new_contacts, contact_manager.py:102-158, or
+   b2g_pair_search), 4 int32 per contact: world, fixture A, fixture B (world-
+   local, as the contact holds them), flags: B2G_CF_SWAPPED when the contact
+   registry swapped the broad phase's pair (contact.py:305-326), so that the
+   pair the reference hands its contact_filter is (B, A) instead of (A, B);
+   B2G_NC_DROPPED once dropped.  Rows by world, then creation order.  *count
+   receives the number of rows; with more than cap nothing is written and
+   B2G_E_CAPACITY is returned.  A custom contact_filter (world.py:124-136,
+   contact_manager.py:144-146) runs on these rows on the host; with
+   B2G_WF_CUSTOM_FILTER set the pair search applies no filter of its own. */
+B2G_API int b2g_new_contacts(b2g_ctx* ctx, int32_t* out, int32_t cap, int32_t* count);
+
+/* Removes contacts created by the last pair search, before the next step:
+   contact index[k] (0-based among world[k]'s rows of b2g_new_contacts) of
+   world world[k].  The rest of the contact list keeps its order.  The
+   counterpart of the reference's filter returning False (contact_manager.py:
+   144-146): that contact is never created.  B2G_E_INVALID for an index outside
+   the world's new contacts. */
+B2G_API int b2g_drop_contacts(b2g_ctx* ctx, int32_t n, const int32_t* world, const int32_t* index);
 
 /* Parity seam, narrow phase: evaluates n contacts (fixture records A and B
    already oriented as the contact registry orders them, contact.py:305-326,

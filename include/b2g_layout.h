@@ -100,15 +100,60 @@ enum {
 #define B2G_CF_TOUCHING 1u
 #define B2G_CF_ENABLED 2u
 #define B2G_CF_FILTER 4u
+/* b2g_new_contacts only: the contact registry swapped the broad phase's pair when the
+   contact was created (contact.py:305-326): the pair was reported as (fixture B, fixture A) */
+#define B2G_CF_SWAPPED 8u
+/* with B2G_WF_CUSTOM_FILTER, on a contact flagged B2G_CF_FILTER: the host ran the
+   custom filter for it (ContactManager.collide, contact_manager.py:200-205) and
+   it returned False; collide destroys the contact */
+#define B2G_CF_REJECT 16u
+/* b2g_new_contacts: the contact has been dropped (b2g_drop_contacts) */
+#define B2G_NC_DROPPED 16
 /* a contact-point id (contact_util.py:78-79) packed as 4 bytes:
    id[0] | id[1] << 8 | id[2] << 16 | id[3] << 24 */
 
 /* ---- world record ---- */
 enum { B2G_WD_GX = 0, B2G_WD_GY, B2G_WD_INV_DT0, B2G_WD_PAD, B2G_WORLD_ND = 4 };
-enum { B2G_WI_FLAGS = 0, B2G_WI_PAD1, B2G_WI_PAD2, B2G_WI_PAD3, B2G_WORLD_NI = 4 };
+/* B2G_WI_NEW_CONTACTS (output only): contacts created by the world's last pair search
+   (find_new_contacts); they are the last ones of its contact list */
+enum { B2G_WI_FLAGS = 0, B2G_WI_NEW_CONTACTS, B2G_WI_PAD2, B2G_WI_PAD3, B2G_WORLD_NI = 4 };
 #define B2G_WF_ALLOW_SLEEP 1u
 #define B2G_WF_WARM_STARTING 2u
 #define B2G_WF_NEW_FIXTURE 4u
+#define B2G_WF_KEEP_FORCES 8u     /* auto_clear_forces off */
+/* a custom contact_filter replaces World.default_contact_filter (world.py:86-89): the pair
+   search creates every pair that passes Body._should_collide, and the host runs its filter on
+   the new contacts and drops the rejected ones (b2g_drop_contacts) before the next step, and
+   decides the contacts flagged for filtering (B2G_CF_REJECT) */
+#define B2G_WF_CUSTOM_FILTER 16u
+
+/* ---- contact event record (b2g_events): int32 x B2G_EV_NI, float64 x B2G_EV_ND ----
+   One callback of the contact listener (contact.py:210-268, contact_manager.py:160-172,
+   island.py:245-252).  kind: B2G_EV_BEGIN / END / PRE_SOLVE / POST_SOLVE.  Events of one
+   step are ordered by (world, phase, key): phase 0, key 4 x contact + 0/1/2 for begin, end,
+   pre_solve in ContactManager.collide's loop; phase 1, key contact for the end_contact of a
+   contact that collide destroys while touching; phase 2, key = the contact's position in the
+   island lists for post_solve.  The manifold fields hold the contact's manifold at the call;
+   the old_* fields hold, for pre_solve, the manifold before Contact.update, and for
+   post_solve the impulses handed to the listener: old pc of them (one per velocity-constraint
+   point), in the old P0/P1 normal and tangent impulse fields. */
+#define B2G_EV_BEGIN 0
+#define B2G_EV_END 1
+#define B2G_EV_PRE_SOLVE 2
+#define B2G_EV_POST_SOLVE 3
+enum {
+  B2G_EI_WORLD = 0, B2G_EI_KIND, B2G_EI_PHASE, B2G_EI_KEY, B2G_EI_FA, B2G_EI_FB, B2G_EI_FLAGS,
+  B2G_EI_MTYPE, B2G_EI_PC, B2G_EI_ID0, B2G_EI_ID1,
+  B2G_EI_OLD_MTYPE, B2G_EI_OLD_PC, B2G_EI_OLD_ID0, B2G_EI_OLD_ID1, B2G_EI_PAD,
+  B2G_EV_NI = 16
+};
+enum {
+  B2G_ED_LN_X = 0, B2G_ED_LN_Y, B2G_ED_LP_X, B2G_ED_LP_Y,
+  B2G_ED_P0_X, B2G_ED_P0_Y, B2G_ED_P0_NI, B2G_ED_P0_TI,
+  B2G_ED_P1_X, B2G_ED_P1_Y, B2G_ED_P1_NI, B2G_ED_P1_TI,
+  B2G_ED_OLD = 12, /* the old manifold, the same 12 fields */
+  B2G_EV_ND = 24
+};
 
 /* A batch description.  Every pointer is a caller-owned host array; sizes are
    given by the offset arrays (each of length n_worlds + 1). */

@@ -28,8 +28,13 @@
  * ascending partner index, or, when a hint is installed, in the hint's order
  * (used to replay a recorded reference run).
  *
+ * The contact listener (begin_contact, end_contact, pre_solve, post_solve)
+ * is recorded as event records in call order (orc_events), and a custom
+ * contact_filter is given as its verdict for every ordered fixture pair
+ * (orc_set_filter).
+ *
  * Not supported (reported through the error word): sensors, loop shapes,
- * joints, controllers, listeners, custom filters, continuous physics.
+ * joints, controllers, continuous physics.
  */
 #include <float.h>
 #include <math.h>
@@ -222,6 +227,7 @@ typedef struct {
   double friction, restitution;
   Manifold m;
   int touching, enabled, filter, island_flag;
+  int swapped; /* the registry swapped the reported pair (fa, fb) = (B, A) */
 } Contact;
 
 typedef struct {
@@ -239,6 +245,22 @@ typedef struct {
   int hint_n;
   int64_t* hint;
   int err;
+  int index;              /* the world's index in its batch (event records) */
+  /* contact listener: kinds recorded (1 << B2G_EV_*), the last step's records */
+  int ev_mask, ev_n, ev_cap, post_key;
+  int32_t* ev_i;
+  double* ev_d;
+  /* custom contact_filter: accept[a * nf + b] = filter(fixture a, fixture b); NULL: default */
+  unsigned char* accept;
+  int32_t* new_rows;      /* the contacts the last find_new_contacts created: fa, fb, swapped */
+  int new_n, new_cap;
+  /* a step in two parts (orc_step_begin / orc_step_end): its time step, and the list index
+     after collide's removals of each contact's index during collide (-1: removed) */
+  int split;
+  double split_dt;
+  int split_vi, split_pi;
+  int* slot_map;
+  int slot_n;
 } World;
 
 #define ORC_E_UNSUPPORTED 1
@@ -831,6 +853,54 @@ static int hint_rank(int64_t key) {
   return g_sort_rank_n;
 }
 
+/* One listener call as an event record (b2g_layout.h): the contact's pair, flags and
+   manifold at the call.  Returns the record's index, or -1. */
+static void ev_manifold(const Manifold* m, double* o, int32_t* oi) {
+  o[0] = m->ln.x; o[1] = m->ln.y; o[2] = m->lp.x; o[3] = m->lp.y;
+  o[4] = m->pts[0].lp.x; o[5] = m->pts[0].lp.y; o[6] = m->pts[0].ni; o[7] = m->pts[0].ti;
+  o[8] = m->pts[1].lp.x; o[9] = m->pts[1].lp.y; o[10] = m->pts[1].ni; o[11] = m->pts[1].ti;
+  oi[0] = m->type;
+  oi[1] = m->pc;
+  oi[2] = mp_id_pack(m->pts[0].id);
+  oi[3] = mp_id_pack(m->pts[1].id);
+}
+
+static int ev_push(World* w, int kind, int phase, int key, const Contact* c) {
+  if (!(w->ev_mask & (1 << kind))) return -1;
+  if (w->ev_n == w->ev_cap) {
+    int cap = w->ev_cap ? 2 * w->ev_cap : 64;
+    int32_t* ni = (int32_t*)realloc(w->ev_i, sizeof(int32_t) * B2G_EV_NI * (size_t)cap);
+    if (ni) w->ev_i = ni;
+    double* nd = (double*)realloc(w->ev_d, sizeof(double) * B2G_EV_ND * (size_t)cap);
+    if (nd) w->ev_d = nd;
+    if (!ni || !nd) { w->err |= ORC_E_ALLOC; return -1; }
+    w->ev_cap = cap;
+  }
+  int k = w->ev_n++;
+  int32_t* ei = w->ev_i + (size_t)k * B2G_EV_NI;
+  double* ed = w->ev_d + (size_t)k * B2G_EV_ND;
+  memset(ei, 0, sizeof(int32_t) * B2G_EV_NI);
+  memset(ed, 0, sizeof(double) * B2G_EV_ND);
+  ei[B2G_EI_WORLD] = w->index;
+  ei[B2G_EI_KIND] = kind;
+  ei[B2G_EI_PHASE] = phase;
+  ei[B2G_EI_KEY] = key;
+  ei[B2G_EI_FA] = c->fa;
+  ei[B2G_EI_FB] = c->fb;
+  ei[B2G_EI_FLAGS] = (c->touching ? B2G_CF_TOUCHING : 0) | (c->enabled ? B2G_CF_ENABLED : 0);
+  ev_manifold(&c->m, ed, ei + B2G_EI_MTYPE);
+  return k;
+}
+
+/* contact_filter(fixture a, fixture b): the custom filter's verdict, or World.default_contact_filter
+   (world.py:958-970) */
+static int filter_ok(const World* w, int a, int b) {
+  if (w->accept) return w->accept[(size_t)a * (size_t)w->nf + (size_t)b] != 0;
+  const Fix *A = &w->f[a], *B = &w->f[b];
+  if (A->group == B->group && A->group != 0) return A->group > 0;
+  return ((A->mask & B->category) != 0) && ((A->category & B->mask) != 0);
+}
+
 static void find_new_contacts(World* w) { /* contact_manager.py:102-158 */
   int* pairs = NULL;
   int np = 0;
@@ -851,12 +921,9 @@ static void find_new_contacts(World* w) { /* contact_manager.py:102-158 */
     if (hs_has(&exist, ukey(fa, fb))) continue;
     /* Body._should_collide (body.py:910-928), no joints */
     if (w->b[bb].type != B2G_DYNAMIC && w->b[ba].type != B2G_DYNAMIC) continue;
-    /* World.default_contact_filter (world.py:958-970) */
+    /* the contact filter (contact_manager.py:144-146) */
     const Fix *A = &w->f[fa], *B = &w->f[fb];
-    int ok;
-    if (A->group == B->group && A->group != 0) ok = A->group > 0;
-    else ok = ((A->mask & B->category) != 0) && ((A->category & B->mask) != 0);
-    if (!ok) continue;
+    if (!filter_ok(w, fa, fb)) continue;
     /* Contact._create + registry (contact.py:294-326) */
     int ta = A->sh.type, tb = B->sh.type, swap;
     if (ta == B2G_SHAPE_LOOP || tb == B2G_SHAPE_LOOP) { w->err |= ORC_E_UNSUPPORTED; continue; }
@@ -869,6 +936,7 @@ static void find_new_contacts(World* w) { /* contact_manager.py:102-158 */
     memset(c, 0, sizeof(*c));
     c->fa = swap ? fb : fa;
     c->fb = swap ? fa : fb;
+    c->swapped = swap;
     /* mix_friction / mix_restitution (contact_util.py:36-48) */
     c->friction = sqrt(w->f[c->fa].friction * w->f[c->fb].friction);
     c->restitution = pmax(w->f[c->fa].restitution, w->f[c->fb].restitution);
@@ -899,12 +967,27 @@ static void find_new_contacts(World* w) { /* contact_manager.py:102-158 */
     free(rank);
     free(tmp);
   }
+  w->new_n = 0;
+  int nn = w->nc - first_new;
+  if (nn > w->new_cap) {
+    int32_t* r = (int32_t*)realloc(w->new_rows, sizeof(int32_t) * 3 * (size_t)nn);
+    if (!r) { w->err |= ORC_E_ALLOC; return; }
+    w->new_rows = r;
+    w->new_cap = nn;
+  }
+  for (int i = 0; i < nn; ++i) {
+    const Contact* c = &w->c[first_new + i];
+    w->new_rows[3 * i] = c->fa;
+    w->new_rows[3 * i + 1] = c->fb;
+    w->new_rows[3 * i + 2] = c->swapped ? (int32_t)B2G_CF_SWAPPED : 0;
+  }
+  w->new_n = nn;
 }
 
 /* ------------------------------------------------------------------ */
 /* ContactManager.collide + Contact.update                             */
 /* ------------------------------------------------------------------ */
-static void contact_update(World* w, Contact* c) { /* contact.py:210-268 */
+static void contact_update(World* w, Contact* c, int slot) { /* contact.py:210-268 */
   Manifold old = c->m;
   c->enabled = 1;
   int was_touching = c->touching;
@@ -924,6 +1007,14 @@ static void contact_update(World* w, Contact* c) { /* contact.py:210-268 */
   }
   if (touching != was_touching) { set_awake(A, 1); set_awake(B, 1); }
   c->touching = touching;
+  /* the listener calls (contact.py:262-268), keyed by the contact's place in the list */
+  if (!was_touching && touching) ev_push(w, B2G_EV_BEGIN, 0, 4 * slot, c);
+  if (was_touching && !touching) ev_push(w, B2G_EV_END, 0, 4 * slot + 1, c);
+  if (touching) {
+    int k = ev_push(w, B2G_EV_PRE_SOLVE, 0, 4 * slot + 2, c);
+    if (k >= 0) ev_manifold(&old, w->ev_d + (size_t)k * B2G_EV_ND + B2G_ED_OLD,
+                            w->ev_i + (size_t)k * B2G_EV_NI + B2G_EI_OLD_MTYPE);
+  }
 }
 
 static void collide(World* w) { /* contact_manager.py:174-223 */
@@ -939,19 +1030,25 @@ static void collide(World* w) { /* contact_manager.py:174-223 */
     if (!active_a && !active_b) continue;
     if (c->filter) {
       int ok = (A->type == B2G_DYNAMIC || B->type == B2G_DYNAMIC);
-      if (ok) {
-        if (FA->group == FB->group && FA->group != 0) ok = FA->group > 0;
-        else ok = ((FA->mask & FB->category) != 0) && ((FA->category & FB->mask) != 0);
-      }
+      if (ok) ok = filter_ok(w, c->fa, c->fb);
       if (!ok) { remove[i] = 1; ++nremove; continue; }
       c->filter = 0;
     }
     if (!aabb_overlap(&FA->fat, &FB->fat)) { remove[i] = 1; ++nremove; continue; }
-    contact_update(w, c);
+    contact_update(w, c, i);
   }
-  if (nremove) { /* ContactManager.destroy keeps the order of the others */
+  for (int i = 0; i < w->nc; ++i) /* ContactManager.destroy: end_contact (contact_manager.py:160-172) */
+    if (remove[i] && w->c[i].touching) ev_push(w, B2G_EV_END, 1, i, &w->c[i]);
+  free(w->slot_map);
+  w->slot_map = (int*)malloc(sizeof(int) * (size_t)(w->nc > 0 ? w->nc : 1));
+  w->slot_n = w->slot_map ? w->nc : 0;
+  if (!w->slot_map) w->err |= ORC_E_ALLOC;
+  { /* ContactManager.destroy keeps the order of the others */
     int k = 0;
-    for (int i = 0; i < w->nc; ++i) if (!remove[i]) w->c[k++] = w->c[i];
+    for (int i = 0; i < w->nc; ++i) {
+      if (w->slot_map) w->slot_map[i] = remove[i] ? -1 : k;
+      if (!remove[i]) w->c[k++] = w->c[i];
+    }
     w->nc = k;
   }
   free(remove);
@@ -1288,6 +1385,20 @@ static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py
     int n = c->m.pc < vc->npts ? c->m.pc : vc->npts;
     for (int j = 0; j < n; ++j) { c->m.pts[j].ni = vc->p[j].ni; c->m.pts[j].ti = vc->p[j].ti; }
   }
+  /* Island.report (island.py:245-252): post_solve with the velocity constraints' impulses */
+  for (int k = 0; k < nvc; ++k) {
+    const VC* vc = &is->vc[k];
+    int e = ev_push(w, B2G_EV_POST_SOLVE, 2, w->post_key + k, &w->c[vc->contact]);
+    if (e < 0) continue;
+    int32_t* ei = w->ev_i + (size_t)e * B2G_EV_NI;
+    double* ed = w->ev_d + (size_t)e * B2G_EV_ND;
+    ei[B2G_EI_OLD_PC] = vc->npts; /* the impulses handed to the listener */
+    ed[B2G_ED_OLD + B2G_ED_P0_NI] = vc->p[0].ni;
+    ed[B2G_ED_OLD + B2G_ED_P0_TI] = vc->p[0].ti;
+    ed[B2G_ED_OLD + B2G_ED_P1_NI] = vc->npts >= 2 ? vc->p[1].ni : 0.0;
+    ed[B2G_ED_OLD + B2G_ED_P1_TI] = vc->npts >= 2 ? vc->p[1].ti : 0.0;
+  }
+  w->post_key += nvc;
   /* integrate positions (island.py:120-138) */
   for (int i = 0; i < is->nbodies; ++i) {
     V2 v = pv[i].v;
@@ -1339,6 +1450,7 @@ static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py
 
 static void solve(World* w, const Step* st) { /* world.py:475-592 */
   int nb = w->nb, nc = w->nc;
+  w->post_key = 0;
   for (int i = 0; i < nb; ++i) { w->b[i].island_flag = 0; w->b[i].island_id = -1; }
   for (int i = 0; i < nc; ++i) w->c[i].island_flag = 0;
   /* per-body contact lists in ContactManager order (= body._contacts order) */
@@ -1450,23 +1562,40 @@ done:
   free(is.bodies); free(is.contacts); free(is.pv); free(is.vc); free(is.pc);
 }
 
-static void world_step(World* w, double dt, int vel_iters, int pos_iters) { /* world.py:307-354 */
+/* World.step (world.py:307-354) in two parts: up to and with collide, and the rest, so that
+   a listener's pre_solve verdicts (Contact.enabled = False) can be applied in between. */
+static void world_step_begin(World* w, double dt, int vel_iters, int pos_iters) {
+  w->ev_n = 0;
   if (w->new_fixture) {
     find_new_contacts(w);
     w->new_fixture = 0;
   }
+  collide(w);
+  w->split = 1;
+  w->split_dt = dt;
+  w->split_vi = vel_iters;
+  w->split_pi = pos_iters;
+}
+
+static void world_step_end(World* w) {
+  double dt = w->split_dt;
+  w->split = 0;
   Step st;
   st.dt = dt;
   st.inv_dt = dt > 0.0 ? 1.0 / dt : 0.0;
   st.dt_ratio = w->inv_dt0 * dt;
-  st.vel_iters = vel_iters;
-  st.pos_iters = pos_iters;
+  st.vel_iters = w->split_vi;
+  st.pos_iters = w->split_pi;
   st.warm_starting = w->warm_starting;
-  collide(w);
   if (st.dt > 0.0) solve(w, &st);
   if (st.dt > 0.0) w->inv_dt0 = st.inv_dt;
   if (!w->keep_forces)
     for (int i = 0; i < w->nb; ++i) { w->b[i].force = v2(0.0, 0.0); w->b[i].torque = 0.0; }
+}
+
+static void world_step(World* w, double dt, int vel_iters, int pos_iters) { /* world.py:307-354 */
+  world_step_begin(w, dt, vel_iters, pos_iters);
+  world_step_end(w);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1523,6 +1652,7 @@ static void manifold_save(const Manifold* m, double* d, int32_t* ci) {
 static void world_free(World* w) {
   free(w->b); free(w->f); free(w->c); free(w->move);
   free(w->bfix_off); free(w->bfix); free(w->hint);
+  free(w->ev_i); free(w->ev_d); free(w->accept); free(w->slot_map); free(w->new_rows);
   memset(w, 0, sizeof(*w));
 }
 
@@ -1648,6 +1778,7 @@ ORC_EXPORT void* orc_load(const b2g_batch* d) {
       free(B);
       return NULL;
     }
+    B->w[i].index = i;
   }
   return B;
 }
@@ -1765,6 +1896,79 @@ ORC_EXPORT int orc_set_hint(void* h, int wi, const int32_t* pairs, int n) {
   for (int i = 0; i < n; ++i) w->hint[i] = ukey(pairs[2 * i], pairs[2 * i + 1]);
   w->hint_n = n;
   return 0;
+}
+
+/* World wi's step in two parts: orc_step_begin (the pending pair search and collide), then
+   orc_disable for the contacts a pre_solve disabled (slot: the contact's index during collide,
+   a phase-0 record's key / 4), then orc_step_end.  Each returns the world's error word, or -1. */
+ORC_EXPORT int orc_step_begin(void* h, int wi, double dt, int vel_iters, int pos_iters) {
+  Batch* B = (Batch*)h;
+  if (wi < 0 || wi >= B->nw || B->w[wi].split) return -1;
+  world_step_begin(&B->w[wi], dt, vel_iters, pos_iters);
+  return B->w[wi].err;
+}
+
+ORC_EXPORT int orc_disable(void* h, int wi, int slot) {
+  Batch* B = (Batch*)h;
+  if (wi < 0 || wi >= B->nw) return -1;
+  World* w = &B->w[wi];
+  if (!w->split || slot < 0 || slot >= w->slot_n || w->slot_map[slot] < 0) return -1;
+  w->c[w->slot_map[slot]].enabled = 0;
+  return 0;
+}
+
+ORC_EXPORT int orc_step_end(void* h, int wi) {
+  Batch* B = (Batch*)h;
+  if (wi < 0 || wi >= B->nw || !B->w[wi].split) return -1;
+  world_step_end(&B->w[wi]);
+  return B->w[wi].err;
+}
+
+/* The contact listener kinds to record (1 << B2G_EV_*), for every world. */
+ORC_EXPORT void orc_set_events(void* h, int mask) {
+  Batch* B = (Batch*)h;
+  for (int i = 0; i < B->nw; ++i) B->w[i].ev_mask = mask;
+}
+
+/* The last step's listener records of world wi, in call order.  Returns the count,
+   -1 (bad world) or -2 (more than cap records: nothing written). */
+ORC_EXPORT int orc_events(void* h, int wi, int32_t* out_i, double* out_d, int cap) {
+  Batch* B = (Batch*)h;
+  if (wi < 0 || wi >= B->nw) return -1;
+  const World* w = &B->w[wi];
+  if (w->ev_n > cap) return -2;
+  if (w->ev_n > 0) {
+    memcpy(out_i, w->ev_i, sizeof(int32_t) * B2G_EV_NI * (size_t)w->ev_n);
+    memcpy(out_d, w->ev_d, sizeof(double) * B2G_EV_ND * (size_t)w->ev_n);
+  }
+  return w->ev_n;
+}
+
+/* A custom contact_filter for world wi as its verdicts: accept[a * nf + b] != 0 when
+   filter(fixture a, fixture b) returns True; accept == NULL restores the default filter. */
+ORC_EXPORT int orc_set_filter(void* h, int wi, const unsigned char* accept, int nf) {
+  Batch* B = (Batch*)h;
+  if (wi < 0 || wi >= B->nw) return -1;
+  World* w = &B->w[wi];
+  free(w->accept);
+  w->accept = NULL;
+  if (!accept) return 0;
+  if (nf != w->nf) return -1;
+  w->accept = (unsigned char*)malloc((size_t)nf * (size_t)nf + 1);
+  if (!w->accept) return -1;
+  memcpy(w->accept, accept, (size_t)nf * (size_t)nf);
+  return 0;
+}
+
+/* The contacts world wi's last find_new_contacts created, 3 ints each: fixture A, fixture B,
+   B2G_CF_SWAPPED when the registry swapped the reported pair.  Returns the count or -2. */
+ORC_EXPORT int orc_new_contacts(void* h, int wi, int32_t* out, int cap) {
+  Batch* B = (Batch*)h;
+  if (wi < 0 || wi >= B->nw) return -1;
+  const World* w = &B->w[wi];
+  if (w->new_n > cap) return -2;
+  if (w->new_n > 0) memcpy(out, w->new_rows, sizeof(int32_t) * 3 * (size_t)w->new_n);
+  return w->new_n;
 }
 
 ORC_EXPORT int orc_error(void* h) {

@@ -48,6 +48,13 @@ def lib():
         l.orc_shape_ray_cast.argtypes = [vp, vp, vp, vp, f64, vp]
         l.orc_query_aabb.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int]
         l.orc_ray_cast.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.c_int]
+        l.orc_set_events.argtypes = [vp, ctypes.c_int]
+        l.orc_events.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int]
+        l.orc_set_filter.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
+        l.orc_new_contacts.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int]
+        l.orc_step_begin.argtypes = [vp, ctypes.c_int, f64, ctypes.c_int, ctypes.c_int]
+        l.orc_disable.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        l.orc_step_end.argtypes = [vp, ctypes.c_int]
         _ = i32
         _lib = l
     return _lib
@@ -83,6 +90,65 @@ class Oracle:
         rc = lib().orc_set_hint(self.h, w, p.ctypes.data if p.size else None, int(p.shape[0]))
         if rc:
             raise RuntimeError("orc_set_hint failed")
+
+    def step_begin(self, w: int, dt=1.0 / 60, vel_iters=8, pos_iters=3) -> int:
+        """World w's step up to and with collide (the listener's begin/end/pre_solve calls)."""
+        rc = lib().orc_step_begin(self.h, w, dt, vel_iters, pos_iters)
+        if rc < 0:
+            raise ValueError("orc_step_begin: bad world or a step already begun")
+        return rc
+
+    def disable(self, w: int, slot: int) -> None:
+        """Contact.enabled = False for the contact at index `slot` during this step's collide."""
+        if lib().orc_disable(self.h, w, int(slot)) < 0:
+            raise ValueError("orc_disable: no such contact")
+
+    def step_end(self, w: int) -> int:
+        rc = lib().orc_step_end(self.h, w)
+        if rc < 0:
+            raise ValueError("orc_step_end: no step begun")
+        return rc
+
+    def set_events(self, mask: int) -> None:
+        """Records the listener calls of the kinds in mask (1 << layout.EV_*), for every world."""
+        lib().orc_set_events(self.h, int(mask))
+
+    def events(self, w: int):
+        """The last step's listener records of world w in call order: (int32 rows of EV_NI, float64 rows
+        of EV_ND)."""
+        cap = 256
+        while True:
+            ei = np.zeros((cap, L.EV_NI), np.int32)
+            ed = np.zeros((cap, L.EV_ND))
+            n = lib().orc_events(self.h, w, ei.ctypes.data, ed.ctypes.data, cap)
+            if n == -1:
+                raise ValueError("bad world")
+            if n >= 0:
+                return ei[:n].copy(), ed[:n].copy()
+            cap *= 4
+
+    def set_filter(self, w: int, accept) -> None:
+        """A custom contact_filter for world w as its verdicts accept[a, b] = filter(fixture a, fixture b)
+        (world-local fixtures); None restores World.default_contact_filter."""
+        if accept is None:
+            rc = lib().orc_set_filter(self.h, w, None, 0)
+        else:
+            a = np.ascontiguousarray(accept, np.uint8)
+            rc = lib().orc_set_filter(self.h, w, a.ctypes.data, int(a.shape[0]))
+        if rc:
+            raise ValueError("orc_set_filter failed")
+
+    def new_contacts(self, w: int) -> np.ndarray:
+        """Rows (fixture A, fixture B, flags) of the contacts world w's last pair search created."""
+        cap = 256
+        while True:
+            out = np.zeros((cap, 3), np.int32)
+            n = lib().orc_new_contacts(self.h, w, out.ctypes.data, cap)
+            if n == -1:
+                raise ValueError("bad world")
+            if n >= 0:
+                return out[:n].copy()
+            cap *= 4
 
     def query_aabb(self, w: int, box) -> np.ndarray:
         """World.query_aabb by brute force: fixtures of world w whose fat AABB overlaps box."""

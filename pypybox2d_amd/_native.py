@@ -22,13 +22,15 @@ B2G_FP32, B2G_FP64 = 0, 1
 
 EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step", "b2g_step_n", "b2g_sync",
            "b2g_counts", "b2g_download", "b2g_last_error", "b2g_set_param", "b2g_get_param", "b2g_profile_step",
-           "b2g_island_times", "b2g_narrowphase", "b2g_pairs", "b2g_stats", "b2g_query_aabb", "b2g_ray_cast", "b2g_pairs_noalloc")
+           "b2g_island_times", "b2g_narrowphase", "b2g_pairs", "b2g_stats", "b2g_query_aabb", "b2g_ray_cast",
+           "b2g_pairs_noalloc", "b2g_events", "b2g_pair_search", "b2g_new_contacts", "b2g_drop_contacts",
+           "b2g_step_collide", "b2g_step_solve", "b2g_disable_contacts")
 
 # keys of b2g_set_param / b2g_get_param: the solver's launch shape, island timing, and the
 # resident solver wavefronts per compute unit and the search launches per step (both read only)
 PARAMS = {"team_lds": 1, "heavy_class": 2, "solve_grid": 3, "team_full_class": 4, "large_worlds": 5,
           "island_times": 6, "solve_waves_per_cu": 7, "team_max": 8, "search_runs": 9, "pairs_wide": 10,
-          "islands_wide": 11, "collide_block": 12, "tree_pairs": 13}
+          "islands_wide": 11, "collide_block": 12, "tree_pairs": 13, "events": 14}
 
 # the stages b2g_profile_step times, in order
 STAGES = ("collide", "islands", "solve", "sync", "pairs")
@@ -81,6 +83,13 @@ def load(path: Optional[str] = None):
     lib.b2g_stats.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32]
     lib.b2g_query_aabb.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]
     lib.b2g_ray_cast.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int32, vp, vp, ctypes.c_int, vp]
+    lib.b2g_events.argtypes = [vp, vp, vp, ctypes.c_int32, vp]
+    lib.b2g_pair_search.argtypes = [vp]
+    lib.b2g_new_contacts.argtypes = [vp, vp, ctypes.c_int32, vp]
+    lib.b2g_drop_contacts.argtypes = [vp, ctypes.c_int32, vp, vp]
+    lib.b2g_step_collide.argtypes = [vp, f64, ctypes.c_int, ctypes.c_int]
+    lib.b2g_step_solve.argtypes = [vp]
+    lib.b2g_disable_contacts.argtypes = [vp, ctypes.c_int32, vp, vp]
     _ = i32
     _lib = lib
     return lib
@@ -211,6 +220,72 @@ class Engine:
             self._check(rc, "ray_cast")
             m = int(n[0])
             return oi[:m].copy(), od[:m].copy()
+
+    def events(self):
+        """The contact listener's records of the last step (b2g_events; kinds enabled with
+        set_param("events", mask)), in the reference's call order: (int32 rows of layout.EV_NI,
+        float64 rows of layout.EV_ND)."""
+        n = np.zeros(1, np.int32)
+        cap = 256
+        while True:
+            ei = np.zeros((cap, L.EV_NI), np.int32)
+            ed = np.zeros((cap, L.EV_ND))
+            rc = self.lib.b2g_events(self.h, ei.ctypes.data, ed.ctypes.data, cap, n.ctypes.data)
+            if rc == B2G_E_CAPACITY and n[0] > cap:
+                cap = int(n[0])
+                continue
+            self._check(rc, "events")
+            m = int(n[0])
+            return ei[:m].copy(), ed[:m].copy()
+
+    def step_collide(self, dt: float = 1.0 / 60, vel_iters: int = 8, pos_iters: int = 3) -> None:
+        """The first part of a step, up to and with the narrow phase (b2g_step_collide); its listener
+        records are then readable (events) and contacts can be disabled for the step."""
+        self._check(self.lib.b2g_step_collide(self.h, dt, vel_iters, pos_iters), "step_collide")
+
+    def disable_contacts(self, worlds, slots) -> None:
+        """contact.enabled = False for this step: slot slots[k] (a phase-0 record's key // 4) of world
+        worlds[k], between step_collide and step_solve (b2g_disable_contacts)."""
+        w = np.ascontiguousarray(worlds, np.int32).reshape(-1)
+        j = np.ascontiguousarray(slots, np.int32).reshape(-1)
+        if w.shape[0] != j.shape[0]:
+            raise ValueError("one world index per slot")
+        if w.shape[0] == 0:
+            return
+        self._check(self.lib.b2g_disable_contacts(self.h, w.shape[0], w.ctypes.data, j.ctypes.data), "disable_contacts")
+
+    def step_solve(self) -> None:
+        """The rest of the step begun by step_collide (b2g_step_solve)."""
+        self._check(self.lib.b2g_step_solve(self.h), "step_solve")
+
+    def pair_search(self) -> None:
+        """Runs the pending new-fixture pair search ahead of the next step (b2g_pair_search)."""
+        self._check(self.lib.b2g_pair_search(self.h), "pair_search")
+
+    def new_contacts(self) -> np.ndarray:
+        """Rows (world, fixture A, fixture B, flags) of the contacts each world's last pair search created
+        (b2g_new_contacts); flags: layout.CF_SWAPPED, layout.NC_DROPPED."""
+        n = np.zeros(1, np.int32)
+        cap = 64
+        while True:
+            out = np.zeros((cap, 4), np.int32)
+            rc = self.lib.b2g_new_contacts(self.h, out.ctypes.data, cap, n.ctypes.data)
+            if rc == B2G_E_CAPACITY and n[0] > cap:
+                cap = int(n[0])
+                continue
+            self._check(rc, "new_contacts")
+            return out[:int(n[0])].copy()
+
+    def drop_contacts(self, worlds, index) -> None:
+        """Removes new contacts index[k] of world worlds[k] (indices among that world's rows of
+        new_contacts) before the next step (b2g_drop_contacts)."""
+        w = np.ascontiguousarray(worlds, np.int32).reshape(-1)
+        j = np.ascontiguousarray(index, np.int32).reshape(-1)
+        if w.shape[0] != j.shape[0]:
+            raise ValueError("one world index per contact")
+        if w.shape[0] == 0:
+            return
+        self._check(self.lib.b2g_drop_contacts(self.h, w.shape[0], w.ctypes.data, j.ctypes.data), "drop_contacts")
 
     def island_times(self) -> np.ndarray:
         """Islands of the last step (island timing on): rows of start_ns, end_ns, contacts,

@@ -35,6 +35,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/b2g.h"
@@ -53,11 +54,16 @@ enum : int {
   CS_PC_SHIFT = 6,
   CS_PROCESSED = 256,
   CS_WOKE = 512,
+  CS_SWAPPED = 1024,  // the registry swapped the broad phase's pair at creation
+  CS_REJECT = 2048,   // with a custom filter: the host's filter rejected this flagged contact
 };
 // body flag bits (B2G_BF_*)
 enum : int { BF_AWAKE = 1, BF_ALLOW_SLEEP = 2, BF_FIXED_ROT = 4, BF_ACTIVE = 8, BF_BULLET = 16 };
 enum : int { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
-enum : int { WF_ALLOW_SLEEP = 1, WF_WARM = 2, WF_NEW_FIXTURE = 4, WF_KEEP_FORCES = 8 };
+enum : int { WF_ALLOW_SLEEP = 1, WF_WARM = 2, WF_NEW_FIXTURE = 4, WF_KEEP_FORCES = 8, WF_CUSTOM_FILTER = 16 };
+// the parts of a step one launch sequence runs: all of it; the new-fixture pair search alone
+// (b2g_pair_search); up to and with collide (b2g_step_collide); the rest (b2g_step_solve)
+enum : int { PART_ALL = 0, PART_PAIRS = 1, PART_COLLIDE = 2, PART_SOLVE = 3 };
 enum : int {
   WERR_CAPACITY = 1,
   WERR_UNSUPPORTED = 2,
@@ -195,6 +201,12 @@ struct Dev {
   double* w_inv_dt0;
   int* w_err;
   unsigned long long* stat;  // per-step work counters, summed over steps (S_* below)
+  int* w_new;                // contacts created by the world's last pair search (the last ones of its list)
+  // contact events (b2g_events): kinds enabled (1 << B2G_EV_*), ring of ev_cap records
+  int ev_mask, ev_cap;
+  int* ev_count;
+  int* ev_i;
+  double* ev_d;
   int* w_move_n;
   int* w_asleep;
   int* w_nisl;
@@ -334,6 +346,51 @@ __device__ __forceinline__ Geo<double, double> geo_of(const Dev<T>& d, int fg) {
 
 __device__ __forceinline__ bool is_static(int type) { return type == BT_STATIC; }
 
+
+// ---------------------------------------------------------------------------
+// contact events: one record per listener call, appended with an atomic and put in the
+// reference's call order on the host by (world, phase, key)
+// ---------------------------------------------------------------------------
+enum : int { EV_BEGIN = 0, EV_END = 1, EV_PRE_SOLVE = 2, EV_POST_SOLVE = 3 };
+
+template <typename T>
+__device__ __forceinline__ int ev_open(const Dev<T>& d, int kind) {
+  if (!(d.ev_mask & (1 << kind))) return -1;
+  const int k = atomicAdd(d.ev_count, 1);
+  return k < d.ev_cap ? k : -1;  // the host sees the overflow in the count
+}
+
+template <typename T>
+__device__ __forceinline__ void ev_manifold(double* o, int* oi, int mtype, int pc, V4<T> lnlp, V4<T> pts,
+                                            V4<double> imp, uint2 ids) {
+  o[0] = (double)lnlp.x; o[1] = (double)lnlp.y; o[2] = (double)lnlp.z; o[3] = (double)lnlp.w;
+  o[4] = (double)pts.x; o[5] = (double)pts.y; o[6] = imp.x; o[7] = imp.y;
+  o[8] = (double)pts.z; o[9] = (double)pts.w; o[10] = imp.z; o[11] = imp.w;
+  oi[0] = mtype;
+  oi[1] = pc;
+  oi[2] = (int)ids.x;
+  oi[3] = (int)ids.y;
+}
+
+// An event of contact slot s of world w: its pair, state flags and current manifold.
+template <typename T>
+__device__ __forceinline__ void ev_write(const Dev<T>& d, int k, int w, int kind, int phase, int key, int s, int st) {
+  int* ei = d.ev_i + (size_t)k * B2G_EV_NI;
+  double* ed = d.ev_d + (size_t)k * B2G_EV_ND;
+  const int2 pr = d.c_pair[s];
+  ei[B2G_EI_WORLD] = w;
+  ei[B2G_EI_KIND] = kind;
+  ei[B2G_EI_PHASE] = phase;
+  ei[B2G_EI_KEY] = key;
+  ei[B2G_EI_FA] = pr.x;
+  ei[B2G_EI_FB] = pr.y;
+  ei[B2G_EI_FLAGS] = ((st & CS_TOUCHING) ? B2G_CF_TOUCHING : 0) | ((st & CS_ENABLED) ? B2G_CF_ENABLED : 0);
+  ev_manifold<T>(ed, ei + B2G_EI_MTYPE, (st >> CS_MTYPE_SHIFT) & 3, (st >> CS_PC_SHIFT) & 3, d.c_lnlp[s], d.c_pts[s],
+                 d.c_imp[s], d.c_id[s]);
+  for (int k = B2G_EI_OLD_MTYPE; k < B2G_EV_NI; ++k) ei[k] = 0;  // the record slot holds an older step's
+  for (int k = B2G_ED_OLD; k < B2G_EV_ND; ++k) ed[k] = 0.0;      // record: the callers fill what they use
+}
+
 // ---------------------------------------------------------------------------
 // narrow phase for one contact slot (Contact.update, contact.py:210-268)
 // returns the new state word; manifold written in place
@@ -345,17 +402,28 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
   const int fa = f0 + pr.x, fb = f0 + pr.y;
   int4 ia = d.f_info[fa], ib = d.f_info[fb];
   const int ba = b0 + ia.x, bb = b0 + ib.x;
+  const int slot = s - d.w_con_base[w];
+  // a contact destroyed by collide while touching: end_contact (ContactManager.destroy,
+  // contact_manager.py:160-172), after the loop's updates
+  auto destroyed = [&]() {
+    if (st & CS_TOUCHING) {
+      const int k = ev_open(d, EV_END);
+      if (k >= 0) ev_write(d, k, w, EV_END, 1, slot, s, st);
+    }
+    return 0;
+  };
   if (st & CS_FILTER) {  // refilter (contact_manager.py:200-208)
     bool ok = d.b_type[ba] == BT_DYNAMIC || d.b_type[bb] == BT_DYNAMIC;
     if (ok) {
-      if (ia.w == ib.w && ia.w != 0) ok = ia.w > 0;
+      if (d.w_flags[w] & WF_CUSTOM_FILTER) ok = !(st & CS_REJECT);  // the host ran the custom filter
+      else if (ia.w == ib.w && ia.w != 0) ok = ia.w > 0;  // World.default_contact_filter
       else ok = ((ia.z >> 16) & (ib.z & 0xFFFF)) != 0 && ((ia.z & 0xFFFF) & (ib.z >> 16)) != 0;
     }
-    if (!ok) return 0;
-    st &= ~CS_FILTER;
+    if (!ok) return destroyed();
+    st &= ~(CS_FILTER | CS_REJECT);
   }
   double4 A = d.f_fat[fa], B = d.f_fat[fb];
-  if (!aabb_overlap(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w)) return 0;  // destroyed
+  if (!aabb_overlap(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w)) return destroyed();
   // The manifold is evaluated in FP64 from the stored (T) geometry and stored
   // back as T: FP32 arithmetic in the clipping and frame changes would set the
   // FP32 pipeline's one-step error (5e-5 on stacked boxes), FP32 storage does not.
@@ -373,6 +441,7 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
   m.id1 = ids.y;
   const int old_pc = m.pc;
   const uint32_t oid0 = ids.x, oid1 = ids.y;
+  const V4<T> old_lnlp = lnlp, old_pts = pts;
   double4 xa = d.b_xf[ba], xb = d.b_xf[bb];
   XF<double> xfa, xfb;  // relative to body A's position
   xfa.p = v2<double>(0, 0);
@@ -412,7 +481,25 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
   const bool touching = m.pc > 0;
   const bool was = (st & CS_TOUCHING) != 0;
   int nst = CS_LIVE | CS_ENABLED | (touching ? CS_TOUCHING : 0) | (m.type << CS_MTYPE_SHIFT) | (m.pc << CS_PC_SHIFT) |
-            CS_PROCESSED | (touching != was ? CS_WOKE : 0) | (st & CS_FILTER);
+            CS_PROCESSED | (touching != was ? CS_WOKE : 0) | (st & (CS_FILTER | CS_SWAPPED));
+  if (d.ev_mask) {  // the listener calls of Contact.update (contact.py:262-268), in that order
+    if (!was && touching) {
+      const int k = ev_open(d, EV_BEGIN);
+      if (k >= 0) ev_write(d, k, w, EV_BEGIN, 0, 4 * slot, s, nst);
+    }
+    if (was && !touching) {
+      const int k = ev_open(d, EV_END);
+      if (k >= 0) ev_write(d, k, w, EV_END, 0, 4 * slot + 1, s, nst);
+    }
+    if (touching) {
+      const int k = ev_open(d, EV_PRE_SOLVE);
+      if (k >= 0) {
+        ev_write(d, k, w, EV_PRE_SOLVE, 0, 4 * slot + 2, s, nst);
+        ev_manifold<T>(d.ev_d + (size_t)k * B2G_EV_ND + B2G_ED_OLD, d.ev_i + (size_t)k * B2G_EV_NI + B2G_EI_OLD_MTYPE,
+                       (st >> CS_MTYPE_SHIFT) & 3, old_pc, old_lnlp, old_pts, imp, make_uint2(oid0, oid1));
+      }
+    }
+  }
   return nst;
 }
 
@@ -1546,6 +1633,21 @@ __device__ __forceinline__ void store_impulse(const Dev<T>& d, const IslandK& c,
   if (n >= 1) { imp.x = vc.p[0].ni; imp.y = vc.p[0].ti; }
   if (n >= 2) { imp.z = vc.p[1].ni; imp.w = vc.p[1].ti; }
   d.c_imp[gs_] = imp;
+  if (d.ev_mask & (1 << EV_POST_SOLVE)) {  // Island.report (island.py:245-252): the points' impulses
+    const int e = ev_open(d, EV_POST_SOLVE);
+    if (e >= 0) {
+      ev_write(d, e, c.w, EV_POST_SOLVE, 2, cstart + k - c.cbase, gs_, d.c_state[gs_]);
+      double* ed = d.ev_d + (size_t)e * B2G_EV_ND;
+      int* ei = d.ev_i + (size_t)e * B2G_EV_NI;
+      // the impulses handed to the listener: one per velocity-constraint point (quirk Q6 may
+      // have dropped the second), in the record's old-manifold fields
+      ei[B2G_EI_OLD_PC] = vc.npts;
+      ed[B2G_ED_OLD + B2G_ED_P0_NI] = vc.p[0].ni;
+      ed[B2G_ED_OLD + B2G_ED_P0_TI] = vc.p[0].ti;
+      ed[B2G_ED_OLD + B2G_ED_P1_NI] = vc.npts >= 2 ? vc.p[1].ni : 0.0;
+      ed[B2G_ED_OLD + B2G_ED_P1_TI] = vc.npts >= 2 ? vc.p[1].ti : 0.0;
+    }
+  }
 }
 
 // 6. integrate positions (island.py:120-138)
@@ -2111,7 +2213,7 @@ __device__ __forceinline__ void create_contact(const Dev<T>& d, int f0, int s, i
                     (ta == SHAPE_POLYGON && tb == SHAPE_EDGE);
   const int fa = swap ? R : P, fb = swap ? P : R;
   d.c_pair[s] = make_int2(fa, fb);
-  d.c_state[s] = CS_LIVE | (M_CIRCLES << CS_MTYPE_SHIFT);
+  d.c_state[s] = CS_LIVE | (M_CIRCLES << CS_MTYPE_SHIFT) | (swap ? CS_SWAPPED : 0);
   V2<T> ma = d.f_mat[f0 + fa], mb = d.f_mat[f0 + fb];
   d.c_mat[s] = v2<T>(sqrt(ma.x * mb.x), pmax(ma.y, mb.y));
   d.c_lnlp[s] = V4<T>{0, 0, 0, 0};
@@ -2124,6 +2226,47 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// The move buffer as BroadPhase.update_pairs walks it (broadphase.py:61-68, 86-99): first the
+// entries the batch brought that no search has taken yet (proxies created, or touched by
+// Fixture.refilter), in their order; after a solve, then the proxies the synchronisation moved,
+// in fixture order (bodies in order, fixtures body-major).  A proxy keeps its first place: a later
+// entry of it adds no pair (the pair set).  mpos[f] = place of fixture f or -1, mlist[place] = f.
+// list_batch_moves (every thread of the block) lists the batch's entries and returns their count.
+template <int B, typename T, typename IdxT>
+__device__ int list_batch_moves(const Dev<T>& d, int w, int f0, int nf, int* mpos, IdxT* mlist) {
+  for (int i = threadIdx.x; i < nf; i += B) mpos[i] = -1;
+  __syncthreads();
+  const int nup = d.w_move_n[w];
+  if (threadIdx.x == 0)
+    for (int k = 0; k < nup; ++k) {
+      const int f = d.mv[f0 + k];  // distinct and in range (checked at upload)
+      mpos[f] = k;
+      mlist[k] = (IdxT)f;
+    }
+  __syncthreads();
+  return nup;
+}
+
+// The synchronisation's moved proxies not yet listed, after the nup listed ones; f_moved (set by
+// k_sync in this step, read by nothing after the pair search) serves as the scan's array.
+// Returns the list's length.
+template <int B, typename T, typename IdxT>
+__device__ int append_synced_moves(const Dev<T>& d, int f0, int nf, int* mpos, IdxT* mlist, int nup, int* scan) {
+  int* fm = d.f_moved + f0;
+  for (int i = threadIdx.x; i < nf; i += B) fm[i] = (fm[i] && mpos[i] < 0) ? 1 : 0;
+  __syncthreads();
+  const int total = block_exclusive_scan<B>(fm, nf, scan);
+  for (int i = threadIdx.x; i < nf; i += B) {
+    const int next = i + 1 < nf ? fm[i + 1] : total;
+    if (next != fm[i]) {
+      mpos[i] = nup + fm[i];
+      mlist[nup + fm[i]] = (IdxT)i;
+    }
+  }
+  __syncthreads();
+  return nup + total;
 }
 
 // A pair of overlapping fat AABBs (moved proxy P at move position i, proxy R): the rest of
@@ -2146,10 +2289,12 @@ __device__ void pair_candidate(const Dev<T>& d, int w, int f0, int b0, int P, in
     }
   }
   if (tb_r != BT_DYNAMIC && tb_p != BT_DYNAMIC) return;  // Body._should_collide
-  bool ok;  // World.default_contact_filter(fixture P, fixture R)
-  if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
-  else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
-  if (!ok) return;
+  if (!(d.w_flags[w] & WF_CUSTOM_FILTER)) {  // World.default_contact_filter(fixture P, fixture R)
+    bool ok;
+    if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
+    else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
+    if (!ok) return;
+  }
   const int ta = ip.y & 255, tb = ir.y & 255;
   const bool reg = (ta == SHAPE_CIRCLE || ta == SHAPE_POLYGON || ta == SHAPE_EDGE) &&
                    (tb == SHAPE_CIRCLE || tb == SHAPE_POLYGON || tb == SHAPE_EDGE) &&
@@ -2183,29 +2328,9 @@ __global__ void __launch_bounds__(B) k_pairs(Dev<T> d, int w_first, int from_upl
 
   // ---- the move buffer (broadphase.py:61-68) ----
   int nmove = 0;
-  if (!do_pairs) {
-    // dt == 0: World._solve (and with it find_new_contacts) does not run
-  } else if (from_upload) {
-    nmove = d.w_move_n[w];
-    for (int i = tid; i < nf; i += kBlock) mpos[i] = -1;
-    __syncthreads();
-    if (tid == 0)
-      for (int k = 0; k < nmove; ++k) {
-        int f = d.mv[f0 + k];
-        mlist[k] = (uint16_t)f;
-        if (mpos[f] < 0) mpos[f] = k;
-      }
-    __syncthreads();
-  } else {
-    // moved fixtures in synchronisation order (bodies in order, fixtures body-major)
-    for (int i = tid; i < nf; i += kBlock) mpos[i] = d.f_moved[f0 + i] ? 1 : 0;
-    __syncthreads();
-    nmove = block_exclusive_scan<B>(mpos, nf, scan);
-    for (int i = tid; i < nf; i += kBlock) {
-      if (d.f_moved[f0 + i]) mlist[mpos[i]] = (uint16_t)i;
-      else mpos[i] = -1;
-    }
-    __syncthreads();
+  if (do_pairs) {  // dt == 0 without new fixtures: World._solve (and find_new_contacts) does not run
+    nmove = list_batch_moves<B>(d, w, f0, nf, mpos, mlist);
+    if (!from_upload) nmove = append_synced_moves<B>(d, f0, nf, mpos, mlist, nmove, scan);
   }
   if (tid == 0) { misc[0] = 0; misc[1] = 0; }
   __syncthreads();
@@ -2330,7 +2455,10 @@ __global__ void __launch_bounds__(B) k_pairs(Dev<T> d, int w_first, int from_upl
       if (nnew > 0) atomicAdd(&d.stat[S_NEW], (unsigned long long)nnew);
     }
   }
-  if (tid == 0) d.w_move_n[w] = 0;
+  if (tid == 0) {
+    if (do_pairs) d.w_move_n[w] = 0;  // update_pairs empties the buffer (broadphase.py:101-102)
+    d.w_new[w] = nnew;
+  }
   if (!finish) return;
   // ---- end-of-step bookkeeping (world.py:348-352) ----
   if (tid == 0) {
@@ -2937,25 +3065,9 @@ __global__ void __launch_bounds__(kBigBlock) k_pairs_big_prep(Dev<T> d, int from
   int* mlist = d.g_mlist + f0;
   int* misc = d.g_wmisc + 16 * w;  // [2] moves, [3] new pairs, [4] overflow
   int nmove = 0;
-  if (!do_pairs) {
-  } else if (from_upload) {
-    nmove = d.w_move_n[w];
-    for (int i = tid; i < nf; i += kBigBlock) mpos[i] = -1;
-    __syncthreads();
-    if (tid == 0)
-      for (int k = 0; k < nmove; ++k) {
-        const int f = d.mv[f0 + k];
-        mlist[k] = f;
-        if (mpos[f] < 0) mpos[f] = k;
-      }
-  } else {
-    for (int i = tid; i < nf; i += kBigBlock) mpos[i] = d.f_moved[f0 + i] ? 1 : 0;
-    __syncthreads();
-    nmove = block_exclusive_scan<kBigBlock>(mpos, nf, scan);
-    for (int i = tid; i < nf; i += kBigBlock) {
-      if (d.f_moved[f0 + i]) mlist[mpos[i]] = i;
-      else mpos[i] = -1;
-    }
+  if (do_pairs) {  // the move buffer as in k_pairs
+    nmove = list_batch_moves<kBigBlock>(d, w, f0, nf, mpos, mlist);
+    if (!from_upload) nmove = append_synced_moves<kBigBlock>(d, f0, nf, mpos, mlist, nmove, scan);
   }
   if (tid == 0) {
     misc[2] = nmove;
@@ -3164,7 +3276,7 @@ __device__ __forceinline__ bool tree_visit(const Dev<T>& d, int w, double lx, do
 template <typename T>
 __device__ __forceinline__ void big_pair_candidate(const Dev<T>& d, int b0, int f0, const int* mpos,
                                                    const uint32_t* hash, int hcap, uint32_t* nlist, int ncap,
-                                                   int* misc, int i, int P, int R) {
+                                                   int* misc, int i, int P, int R, bool custom_filter) {
   if (R == P) return;
   const int mr = mpos[R];
   if (mr >= 0 && mr < i) return;  // (R, P) is already in the pair set
@@ -3185,10 +3297,12 @@ __device__ __forceinline__ void big_pair_candidate(const Dev<T>& d, int b0, int 
     }
   }
   if (tb_r != BT_DYNAMIC && tb_p != BT_DYNAMIC) return;  // Body._should_collide
-  bool ok;  // World.default_contact_filter(fixture P, fixture R)
-  if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
-  else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
-  if (!ok) return;
+  if (!custom_filter) {  // World.default_contact_filter(fixture P, fixture R)
+    bool ok;
+    if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
+    else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
+    if (!ok) return;
+  }
   const int ta = ip.y & 255, tb = ir.y & 255;
   const bool reg = (ta == SHAPE_CIRCLE || ta == SHAPE_POLYGON || ta == SHAPE_EDGE) &&
                    (tb == SHAPE_CIRCLE || tb == SHAPE_POLYGON || tb == SHAPE_EDGE) &&
@@ -3218,7 +3332,8 @@ __global__ void __launch_bounds__(kBlock) k_pairs_big_find(Dev<T> d) {
   const long long work = (long long)nmove * nf;
   for (long long t = (long long)blockIdx.x * kBlock + threadIdx.x; t < work; t += (long long)gridDim.x * kBlock) {
     const int i = (int)(t / nf), R = (int)(t % nf);
-    big_pair_candidate(d, b0, f0, mpos, hash, hcap, nlist, ncap, misc, i, mlist[i], R);
+    big_pair_candidate(d, b0, f0, mpos, hash, hcap, nlist, ncap, misc, i, mlist[i], R,
+                       (d.w_flags[w] & WF_CUSTOM_FILTER) != 0);
   }
 }
 
@@ -3244,7 +3359,8 @@ __global__ void __launch_bounds__(kBlock) k_pairs_big_find_tree(Dev<T> d) {
     const int P = mlist[i];
     const double4 fp = d.f_fat[f0 + P];
     const bool ok = tree_visit(d, w, fp.x, fp.y, fp.z, fp.w, [&](int R) {
-      big_pair_candidate(d, b0, f0, mpos, hash, hcap, nlist, ncap, misc, i, P, R);
+      big_pair_candidate(d, b0, f0, mpos, hash, hcap, nlist, ncap, misc, i, P, R,
+                         (d.w_flags[w] & WF_CUSTOM_FILTER) != 0);
     });
     if (!ok) atomicOr(&d.w_err[w], WERR_SCHEDULE);
   }
@@ -3252,7 +3368,7 @@ __global__ void __launch_bounds__(kBlock) k_pairs_big_find_tree(Dev<T> d) {
 
 // Canonical order, contact creation and end-of-step bookkeeping of each large world.
 template <typename T>
-__global__ void __launch_bounds__(kBigBlock) k_pairs_big_finish(Dev<T> d, int finish) {
+__global__ void __launch_bounds__(kBigBlock) k_pairs_big_finish(Dev<T> d, int finish, int do_pairs) {
   __shared__ int scan[kBigBlock];
   __shared__ int sh[4];
   const int w = d.big_list[blockIdx.x], tid = threadIdx.x;
@@ -3334,7 +3450,10 @@ __global__ void __launch_bounds__(kBigBlock) k_pairs_big_finish(Dev<T> d, int fi
       if (nnew > 0) atomicAdd(&d.stat[S_NEW], (unsigned long long)nnew);
     }
   }
-  if (tid == 0) d.w_move_n[w] = 0;
+  if (tid == 0) {
+    if (do_pairs) d.w_move_n[w] = 0;
+    d.w_new[w] = nnew;
+  }
   if (!finish) return;
   if (tid == 0) {
     if (d.dt > 0.0) d.w_inv_dt0[w] = d.inv_dt;
@@ -3434,8 +3553,60 @@ template <typename T>
 __global__ void k_reset_queue(Dev<T> d) {
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) *d.q_count = 0;
+    if (threadIdx.x == 0 && d.ev_count) *d.ev_count = 0;  // events are those of the last step
     if (threadIdx.x < kQWaveBase) d.q_cls[threadIdx.x] = 0;
   }
+}
+
+// b2g_pair_search ran the new-fixture search ahead of the step: the flag is done with
+// (World.step clears _new_fixture right after that search, world.py:316-319).
+template <typename T>
+__global__ void k_clear_new_fixture(Dev<T> d) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < d.nw) d.w_flags[w] &= ~WF_NEW_FIXTURE;
+}
+
+// b2g_new_contacts: row r of the listing is new contact r - off[w] of world w, where
+// off[w] <= r < off[w + 1] (off: exclusive prefix of w_new, nw + 1 entries).  The new
+// contacts of a world are the last w_new[w] slots of its list.
+template <typename T>
+__global__ void k_new_contacts(Dev<T> d, const int* off, int total, int* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= total) return;
+  int lo = 0, hi = d.nw;  // off[lo] <= r < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= r) lo = mid;
+    else hi = mid;
+  }
+  const int w = lo, j = r - off[w];
+  const int s = d.w_con_base[w] + d.w_con_used[w] - d.w_new[w] + j;
+  const int2 pr = d.c_pair[s];
+  const int st = d.c_state[s];
+  out[4 * r] = w;
+  out[4 * r + 1] = pr.x;
+  out[4 * r + 2] = pr.y;
+  out[4 * r + 3] = ((st & CS_LIVE) ? 0 : B2G_NC_DROPPED) | ((st & CS_SWAPPED) ? (int)B2G_CF_SWAPPED : 0);
+}
+
+// b2g_drop_contacts: (world, j) pairs, j < w_new[world] (checked by the host)
+template <typename T>
+__global__ void k_drop_new(Dev<T> d, const int* wj, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int w = wj[2 * i], j = wj[2 * i + 1];
+  d.c_state[d.w_con_base[w] + d.w_con_used[w] - d.w_new[w] + j] = 0;
+}
+
+// b2g_disable_contacts: contact.enabled = False for (world, slot) pairs (slot < w_con_used[world],
+// checked by the host); a destroyed slot stays as it is
+template <typename T>
+__global__ void k_disable(Dev<T> d, const int* ws, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = d.w_con_base[ws[2 * i]] + ws[2 * i + 1];
+  const int st = d.c_state[s];
+  if (st & CS_LIVE) d.c_state[s] = st & ~CS_ENABLED;
 }
 
 // Orders the island queue by size class, largest first, so that the lanes of
@@ -3723,6 +3894,8 @@ struct b2g_ctx {
   size_t tree_lds = 8;      // dynamic LDS of k_tree_build
   bool tree_dirty = true;   // the fat AABBs changed since the trees were built
   int tree_pairs = 0;       // B2G_PARAM_TREE_PAIRS
+  int ev_mask = 0;          // B2G_PARAM_EVENTS
+  long long ev_cap = 0;     // event records the buffers hold (allocated when events are first enabled)
   // host mirrors of constant per-body / per-world data
   std::vector<int> body_off, fix_off, con_base, con_cap;
   std::vector<double> mass, inertia;
@@ -3742,6 +3915,11 @@ struct b2g_ctx {
   int island_times = 0;         // B2G_PARAM_ISLAND_TIMES
   long long stat_steps = 0;     // steps launched since the work counters were reset
   long long* itime = nullptr;   // 4 values per island slot (nb slots)
+  bool split = false;           // b2g_step_collide ran; b2g_step_solve finishes that step
+  double split_dt = 0;
+  int split_vi = 0, split_pi = 0;
+  int* xbuf = nullptr;          // argument and result staging of the between-step calls
+  size_t xcap = 0;              // ints in xbuf
 };
 
 static void ctx_error(b2g_ctx* ctx, const std::string& s) {
@@ -3782,6 +3960,9 @@ static void free_device(b2g_ctx* ctx) {
   ctx->allocs.clear();
   if (ctx->itime) hipFree(ctx->itime);
   ctx->itime = nullptr;
+  if (ctx->xbuf) hipFree(ctx->xbuf);
+  ctx->xbuf = nullptr;
+  ctx->xcap = 0;
   if (ctx->dev) {
     if (ctx->precision == B2G_FP64) delete (Dev<double>*)ctx->dev;
     else delete (Dev<float>*)ctx->dev;
@@ -3807,6 +3988,18 @@ static int download(b2g_ctx* ctx, X* dst, const X* src, size_t n) {
     int rc_ = (x);       \
     if (rc_) return rc_; \
   } while (0)
+
+// Event records for ev_cap events per step (4 per contact slot: begin/end/pre-solve in collide,
+// post-solve in the solve; an end at destruction replaces the update's events).
+template <typename T>
+static int ensure_event_buffers(b2g_ctx* ctx, Dev<T>& d) {
+  if (d.ev_i) return 0;
+  const long long cap = std::min<long long>(ctx->ev_cap, 1ll << 30);
+  TRY(dalloc(ctx, &d.ev_i, (size_t)cap * B2G_EV_NI));
+  TRY(dalloc(ctx, &d.ev_d, (size_t)cap * B2G_EV_ND));
+  d.ev_cap = (int)cap;
+  return 0;
+}
 
 template <typename T>
 static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
@@ -4013,7 +4206,8 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
       if (pc < 0 || pc > 2 || mt < 0 || mt > 2) return set_error(ctx, B2G_E_INVALID, "bad manifold");
       c_pair[s] = make_int2(ri[B2G_CI_FA], ri[B2G_CI_FB]);
       c_state[s] = CS_LIVE | ((fl & B2G_CF_TOUCHING) ? CS_TOUCHING : 0) | ((fl & B2G_CF_ENABLED) ? CS_ENABLED : 0) |
-                   ((fl & B2G_CF_FILTER) ? CS_FILTER : 0) | (mt << CS_MTYPE_SHIFT) | (pc << CS_PC_SHIFT);
+                   ((fl & B2G_CF_FILTER) ? CS_FILTER : 0) | ((fl & B2G_CF_REJECT) ? CS_REJECT : 0) |
+                   (mt << CS_MTYPE_SHIFT) | (pc << CS_PC_SHIFT);
       c_mat[s] = v2<T>((T)r[B2G_CD_FRICTION], (T)r[B2G_CD_RESTITUTION]);
       c_lnlp[s] = V4<T>{(T)r[B2G_CD_LN_X], (T)r[B2G_CD_LN_Y], (T)r[B2G_CD_LP_X], (T)r[B2G_CD_LP_Y]};
       c_pts[s] = V4<T>{(T)r[B2G_CD_P0_X], (T)r[B2G_CD_P0_Y], (T)r[B2G_CD_P1_X], (T)r[B2G_CD_P1_Y]};
@@ -4052,6 +4246,12 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.w_inv_dt0, nw));
   TRY(dalloc(ctx, &d.w_err, nw));
   TRY(dalloc(ctx, &d.stat, kStats));
+  TRY(dalloc(ctx, &d.w_new, nw));
+  TRY(dalloc(ctx, &d.ev_count, 1));
+  d.ev_i = nullptr;
+  d.ev_d = nullptr;
+  d.ev_cap = 0;
+  ctx->ev_cap = 4 * (long long)ncap + 64;
   TRY(dalloc(ctx, &d.w_move_n, nw));
   TRY(dalloc(ctx, &d.w_asleep, nw));
   TRY(dalloc(ctx, &d.w_nisl, nw));
@@ -4211,6 +4411,12 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(upload(ctx, w_grav_d, w_grav.data(), 2 * (size_t)nw));
   TRY(upload(ctx, d.w_inv_dt0, w_inv_dt0.data(), nw));
   TRY(upload(ctx, d.w_err, zeros_w.data(), nw));
+  TRY(upload(ctx, d.w_new, zeros_w.data(), nw));
+  {
+    static const int zero1 = 0;
+    TRY(upload(ctx, d.ev_count, &zero1, 1));
+  }
+  if (ctx->ev_mask) TRY(ensure_event_buffers(ctx, d));
   {
     const std::vector<unsigned long long> z(kStats, 0ull);
     TRY(upload(ctx, d.stat, z.data(), kStats));
@@ -4260,7 +4466,8 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
 }
 
 template <typename T>
-static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, hipEvent_t* ev) {
+static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, hipEvent_t* ev,
+                       int part = PART_ALL) {
   const bool timed = ev != nullptr;
   Dev<T> d = *(Dev<T>*)ctx->dev;
   d.dt = dt;
@@ -4271,6 +4478,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   d.team_full_class = ctx->team_full_class;
   d.team_lds = ctx->team_lds;
   d.team_max = ctx->team_max;
+  d.ev_mask = ctx->ev_mask;
   d.i_time = ctx->island_times ? ctx->itime : nullptr;
   const int nw = ctx->nw;
   hipStream_t st = ctx->stream;
@@ -4305,23 +4513,33 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
       } else {
         hipLaunchKernelGGL(k_pairs_big_find<T>, dim3(256, nbig), dim3(kBlock), 0, st, d);
       }
-      hipLaunchKernelGGL(k_pairs_big_finish<T>, dim3(nbig), dim3(kBigBlock), 0, st, d, finish);
+      hipLaunchKernelGGL(k_pairs_big_finish<T>, dim3(nbig), dim3(kBigBlock), 0, st, d, finish, do_pairs);
     }
     return 0;
   };
-  if (first_pairs) TRY(pairs(1, 0, 1));
-  if (timed) HIPCHK(hipEventRecord(ev[0], st));
-  hipLaunchKernelGGL(k_reset_queue<T>, dim3(1), dim3(64), 0, st, d);
-  {
-    const int cbk = ctx->collide_block;
-    const int nch = std::max((ctx->max_ccap + cbk - 1) / cbk, 1);
-    const dim3 grid((unsigned)(((nw + 7) / 8) * 8 * nch));
-    if (cbk == 64) hipLaunchKernelGGL((k_collide<T, 64>), grid, dim3(64), 0, st, d, nch);
-    else if (cbk == 128) hipLaunchKernelGGL((k_collide<T, 128>), grid, dim3(128), 0, st, d, nch);
-    else if (cbk == 512) hipLaunchKernelGGL((k_collide<T, 512>), grid, dim3(512), 0, st, d, nch);
-    else hipLaunchKernelGGL((k_collide<T, 256>), grid, dim3(256), 0, st, d, nch);
+  if (part != PART_SOLVE) {
+    if (first_pairs) TRY(pairs(1, 0, 1));
+    if (part == PART_PAIRS) {  // the new-fixture pair search alone (b2g_pair_search)
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
+    if (timed) HIPCHK(hipEventRecord(ev[0], st));
+    hipLaunchKernelGGL(k_reset_queue<T>, dim3(1), dim3(64), 0, st, d);
+    {
+      const int cbk = ctx->collide_block;
+      const int nch = std::max((ctx->max_ccap + cbk - 1) / cbk, 1);
+      const dim3 grid((unsigned)(((nw + 7) / 8) * 8 * nch));
+      if (cbk == 64) hipLaunchKernelGGL((k_collide<T, 64>), grid, dim3(64), 0, st, d, nch);
+      else if (cbk == 128) hipLaunchKernelGGL((k_collide<T, 128>), grid, dim3(128), 0, st, d, nch);
+      else if (cbk == 512) hipLaunchKernelGGL((k_collide<T, 512>), grid, dim3(512), 0, st, d, nch);
+      else hipLaunchKernelGGL((k_collide<T, 256>), grid, dim3(256), 0, st, d, nch);
+    }
+    if (timed) HIPCHK(hipEventRecord(ev[1], st));
+    if (part == PART_COLLIDE) {  // the listener runs between collide and the solve (b2g_step_collide)
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
   }
-  if (timed) HIPCHK(hipEventRecord(ev[1], st));
   if (dt > 0.0) {
     TRY(fan(ctx->isl_runs, [&](const WorldRun& r, hipStream_t s) {
       if (r.smem > 0 && (size_t)kMaxLds / r.smem <= (size_t)ctx->islands_wide)
@@ -4370,10 +4588,11 @@ static int ensure_island_times(b2g_ctx* ctx) {
   return 0;
 }
 
-static int dispatch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, hipEvent_t* ev = nullptr) {
+static int dispatch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs, hipEvent_t* ev = nullptr,
+                         int part = PART_ALL) {
   ctx->tree_dirty = true;
-  if (ctx->precision == B2G_FP64) return launch_step<double>(ctx, dt, vi, pi, first_pairs, ev);
-  return launch_step<float>(ctx, dt, vi, pi, first_pairs, ev);
+  if (ctx->precision == B2G_FP64) return launch_step<double>(ctx, dt, vi, pi, first_pairs, ev, part);
+  return launch_step<float>(ctx, dt, vi, pi, first_pairs, ev, part);
 }
 
 template <typename T>
@@ -4381,8 +4600,9 @@ static int download_batch(b2g_ctx* ctx, b2g_batch_out* o) {
   Dev<T>& d = *(Dev<T>*)ctx->dev;
   const int nw = ctx->nw, nb = ctx->nb, nf = ctx->nf, ncap = ctx->ncap;
   if (o->n_worlds != nw) return set_error(ctx, B2G_E_INVALID, "n_worlds mismatch");
-  std::vector<int> w_used(nw), w_flags(nw), w_move_n(nw);
+  std::vector<int> w_used(nw), w_flags(nw), w_move_n(nw), w_new(nw);
   std::vector<double> w_inv_dt0(nw);
+  TRY(download(ctx, w_new.data(), d.w_new, nw));
   TRY(download(ctx, w_used.data(), d.w_con_used, nw));
   TRY(download(ctx, w_flags.data(), d.w_flags, nw));
   TRY(download(ctx, w_inv_dt0.data(), d.w_inv_dt0, nw));
@@ -4440,6 +4660,7 @@ static int download_batch(b2g_ctx* ctx, b2g_batch_out* o) {
     for (int k = 0; k < B2G_WORLD_NI; ++k) wi[k] = ctx->world_i[(size_t)w * B2G_WORLD_NI + k];
     wd[B2G_WD_INV_DT0] = w_inv_dt0[w];
     wi[B2G_WI_FLAGS] = w_flags[w];
+    wi[B2G_WI_NEW_CONTACTS] = w_new[w];
   }
   for (int i = 0; i < nb; ++i) {
     double* r = o->body_d + (size_t)i * B2G_BODY_ND;
@@ -4483,7 +4704,7 @@ static int download_batch(b2g_ctx* ctx, b2g_batch_out* o) {
       ri[B2G_CI_FA] = c_pair[s].x;
       ri[B2G_CI_FB] = c_pair[s].y;
       ri[B2G_CI_FLAGS] = ((st & CS_TOUCHING) ? B2G_CF_TOUCHING : 0) | ((st & CS_ENABLED) ? B2G_CF_ENABLED : 0) |
-                         ((st & CS_FILTER) ? B2G_CF_FILTER : 0);
+                         ((st & CS_FILTER) ? B2G_CF_FILTER : 0) | ((st & CS_REJECT) ? B2G_CF_REJECT : 0);
       ri[B2G_CI_MTYPE] = (st >> CS_MTYPE_SHIFT) & 3;
       ri[B2G_CI_PC] = (st >> CS_PC_SHIFT) & 3;
       ri[B2G_CI_ID0] = (int32_t)c_id[s].x;
@@ -4620,6 +4841,126 @@ static int ray_cast_t(b2g_ctx* ctx, int n, const int32_t* world, const double* s
                                    : B2G_OK;
 }
 
+// ---------------------------------------------------------------------------
+// between-step calls of the contact listener and the custom filter
+// ---------------------------------------------------------------------------
+static int ensure_xbuf(b2g_ctx* ctx, size_t n) {
+  if (ctx->xcap >= n) return 0;
+  if (ctx->xbuf) hipFree(ctx->xbuf);
+  ctx->xbuf = nullptr;
+  ctx->xcap = 0;
+  void* q = nullptr;
+  const size_t want = std::max<size_t>(n, 1024);
+  const hipError_t e = hipMalloc(&q, want * sizeof(int));
+  if (e != hipSuccess) return set_error(ctx, B2G_E_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  ctx->xbuf = (int*)q;
+  ctx->xcap = want;
+  return 0;
+}
+
+template <typename T>
+static int new_contacts_t(b2g_ctx* ctx, int32_t* out, int cap, int32_t* count) {
+  const Dev<T>& d = *(const Dev<T>*)ctx->dev;
+  const int nw = ctx->nw;
+  std::vector<int> nnew(nw), off(nw + 1, 0);
+  TRY(download(ctx, nnew.data(), d.w_new, nw));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int w = 0; w < nw; ++w) off[w + 1] = off[w] + nnew[w];
+  const int total = off[nw];
+  *count = total;
+  if (total > cap) return set_error(ctx, B2G_E_CAPACITY, "b2g_new_contacts: output capacity too small");
+  if (total == 0) return B2G_OK;
+  TRY(ensure_xbuf(ctx, (size_t)(nw + 1) + 4 * (size_t)total));
+  int* d_off = ctx->xbuf;
+  int* d_out = ctx->xbuf + (nw + 1);
+  TRY(upload(ctx, d_off, off.data(), nw + 1));
+  hipLaunchKernelGGL(k_new_contacts<T>, dim3((total + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, d, d_off,
+                     total, d_out);
+  HIPCHK(hipGetLastError());
+  TRY(download(ctx, out, d_out, 4 * (size_t)total));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return B2G_OK;
+}
+
+template <typename T>
+static int drop_contacts_t(b2g_ctx* ctx, int n, const int32_t* world, const int32_t* index) {
+  const Dev<T>& d = *(const Dev<T>*)ctx->dev;
+  const int nw = ctx->nw;
+  std::vector<int> nnew(nw);
+  TRY(download(ctx, nnew.data(), d.w_new, nw));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::vector<int> wj(2 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    if (world[i] < 0 || world[i] >= nw) return set_error(ctx, B2G_E_INVALID, "b2g_drop_contacts: bad world index");
+    if (index[i] < 0 || index[i] >= nnew[world[i]])
+      return set_error(ctx, B2G_E_INVALID, "b2g_drop_contacts: not a contact of the world's last pair search");
+    wj[2 * i] = world[i];
+    wj[2 * i + 1] = index[i];
+  }
+  TRY(ensure_xbuf(ctx, 2 * (size_t)n));
+  TRY(upload(ctx, ctx->xbuf, wj.data(), wj.size()));
+  hipLaunchKernelGGL(k_drop_new<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, d, ctx->xbuf, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));  // the staging buffer is reused by the next call
+  ctx->tree_dirty = true;
+  return B2G_OK;
+}
+
+template <typename T>
+static int disable_contacts_t(b2g_ctx* ctx, int n, const int32_t* world, const int32_t* slot) {
+  const Dev<T>& d = *(const Dev<T>*)ctx->dev;
+  const int nw = ctx->nw;
+  std::vector<int> used(nw);
+  TRY(download(ctx, used.data(), d.w_con_used, nw));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::vector<int> ws(2 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    if (world[i] < 0 || world[i] >= nw) return set_error(ctx, B2G_E_INVALID, "b2g_disable_contacts: bad world index");
+    if (slot[i] < 0 || slot[i] >= used[world[i]])
+      return set_error(ctx, B2G_E_INVALID, "b2g_disable_contacts: no such contact slot");
+    ws[2 * i] = world[i];
+    ws[2 * i + 1] = slot[i];
+  }
+  TRY(ensure_xbuf(ctx, 2 * (size_t)n));
+  TRY(upload(ctx, ctx->xbuf, ws.data(), ws.size()));
+  hipLaunchKernelGGL(k_disable<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, d, ctx->xbuf, n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return B2G_OK;
+}
+
+template <typename T>
+static int events_t(b2g_ctx* ctx, int32_t* out_i, double* out_d, int cap, int32_t* count) {
+  const Dev<T>& d = *(const Dev<T>*)ctx->dev;
+  int n = 0;
+  if (d.ev_i) {
+    TRY(download(ctx, &n, d.ev_count, 1));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  }
+  if (n > d.ev_cap) return set_error(ctx, B2G_E_CAPACITY, "b2g_events: event buffer overflow");
+  *count = n;
+  if (n > cap) return set_error(ctx, B2G_E_CAPACITY, "b2g_events: output capacity too small");
+  if (n == 0) return B2G_OK;
+  std::vector<int32_t> ei((size_t)n * B2G_EV_NI);
+  std::vector<double> ed((size_t)n * B2G_EV_ND);
+  TRY(download(ctx, ei.data(), d.ev_i, ei.size()));
+  TRY(download(ctx, ed.data(), d.ev_d, ed.size()));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // records arrive in completion order; the listener's call order is (world, phase, key)
+  std::vector<int> idx(n);
+  for (int k = 0; k < n; ++k) idx[k] = k;
+  auto key = [&](int k) {
+    const int32_t* r = &ei[(size_t)k * B2G_EV_NI];
+    return std::make_tuple(r[B2G_EI_WORLD], r[B2G_EI_PHASE], r[B2G_EI_KEY]);
+  };
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
+  for (int k = 0; k < n; ++k) {
+    std::memcpy(out_i + (size_t)k * B2G_EV_NI, &ei[(size_t)idx[k] * B2G_EV_NI], sizeof(int32_t) * B2G_EV_NI);
+    std::memcpy(out_d + (size_t)k * B2G_EV_ND, &ed[(size_t)idx[k] * B2G_EV_ND], sizeof(double) * B2G_EV_ND);
+  }
+  return B2G_OK;
+}
+
 extern "C" {
 
 B2G_API int b2g_version(void) { return B2G_VERSION; }
@@ -4688,6 +5029,7 @@ B2G_API int b2g_upload(b2g_ctx* ctx, const b2g_batch* batch) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   free_device(ctx);
+  ctx->split = false;
   int rc = ctx->precision == B2G_FP64 ? upload_batch<double>(ctx, batch) : upload_batch<float>(ctx, batch);
   if (rc) free_device(ctx);
   return rc;
@@ -4695,6 +5037,7 @@ B2G_API int b2g_upload(b2g_ctx* ctx, const b2g_batch* batch) {
 
 B2G_API int b2g_step(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters) {
   if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (ctx->split) return set_error(ctx, B2G_E_INVALID, "a step begun by b2g_step_collide is not finished");
   if (vel_iters < 0 || pos_iters < 0) return set_error(ctx, B2G_E_INVALID, "negative iteration count");
   hipSetDevice(ctx->device);
   TRY(ensure_island_times(ctx));
@@ -4707,6 +5050,7 @@ B2G_API int b2g_step(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters) {
 
 B2G_API int b2g_step_n(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, int use_graph) {
   if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (ctx->split) return set_error(ctx, B2G_E_INVALID, "a step begun by b2g_step_collide is not finished");
   if (n < 0 || vel_iters < 0 || pos_iters < 0) return set_error(ctx, B2G_E_INVALID, "negative count");
   hipSetDevice(ctx->device);
   TRY(ensure_island_times(ctx));  // before any capture
@@ -4815,6 +5159,7 @@ B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value) {
     case B2G_PARAM_ISLANDS_WIDE: slot = &ctx->islands_wide; lo = 0; hi = 32; break;
     case B2G_PARAM_COLLIDE_BLOCK: slot = &ctx->collide_block; lo = 64; hi = 512; break;
     case B2G_PARAM_TREE_PAIRS: slot = &ctx->tree_pairs; lo = 0; hi = 1; break;
+    case B2G_PARAM_EVENTS: slot = &ctx->ev_mask; lo = 0; hi = 15; break;
     case B2G_PARAM_SOLVE_WAVES_PER_CU:
       return set_error(ctx, B2G_E_INVALID, "b2g_set_param: B2G_PARAM_SOLVE_WAVES_PER_CU is read-only");
     case B2G_PARAM_SEARCH_RUNS:
@@ -4838,6 +5183,11 @@ B2G_API int b2g_set_param(b2g_ctx* ctx, int32_t key, int32_t value) {
     *slot = value;
     drop_graph(ctx);  // kernel arguments are baked into a captured step
   }
+  if (key == B2G_PARAM_EVENTS && value && ctx->dev) {
+    hipSetDevice(ctx->device);
+    TRY(ctx->precision == B2G_FP64 ? ensure_event_buffers(ctx, *(Dev<double>*)ctx->dev)
+                                   : ensure_event_buffers(ctx, *(Dev<float>*)ctx->dev));
+  }
   return B2G_OK;
 }
 
@@ -4855,6 +5205,7 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value) {
     case B2G_PARAM_ISLANDS_WIDE: *value = ctx->islands_wide; return B2G_OK;
     case B2G_PARAM_COLLIDE_BLOCK: *value = ctx->collide_block; return B2G_OK;
     case B2G_PARAM_TREE_PAIRS: *value = ctx->tree_pairs; return B2G_OK;
+    case B2G_PARAM_EVENTS: *value = ctx->ev_mask; return B2G_OK;
     case B2G_PARAM_SEARCH_RUNS: *value = (int)ctx->isl_runs.size() | (int)ctx->pair_runs.size() << 8; return B2G_OK;
     case B2G_PARAM_SOLVE_WAVES_PER_CU: {
       // resident solver wavefronts per compute unit: 4 SIMDs x (512 VGPRs / VGPRs per lane), and
@@ -4875,6 +5226,7 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value) {
 
 B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, float* ms_out) {
   if (!ctx || !ctx->dev || !ms_out || n <= 0) return set_error(ctx, B2G_E_INVALID, "bad arguments");
+  if (ctx->split) return set_error(ctx, B2G_E_INVALID, "a step begun by b2g_step_collide is not finished");
   hipSetDevice(ctx->device);
   TRY(ensure_island_times(ctx));
   // a pending new-fixture pair search runs at the start of the first step, before its first
@@ -4937,6 +5289,94 @@ B2G_API int b2g_ray_cast(b2g_ctx* ctx, int n, const int32_t* world, const double
   hipSetDevice(ctx->device);
   return ctx->precision == B2G_FP64 ? ray_cast_t<double>(ctx, n, world, seg, closest, out_i, out_d, cap, count)
                                     : ray_cast_t<float>(ctx, n, world, seg, closest, out_i, out_d, cap, count);
+}
+
+B2G_API int b2g_pair_search(b2g_ctx* ctx) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (ctx->split) return set_error(ctx, B2G_E_INVALID, "a step begun by b2g_step_collide is not finished");
+  hipSetDevice(ctx->device);
+  if (!ctx->pending_new_fixture) {  // no search pending: nothing is new
+    const int* w_new = ctx->precision == B2G_FP64 ? ((Dev<double>*)ctx->dev)->w_new : ((Dev<float>*)ctx->dev)->w_new;
+    HIPCHK(hipMemsetAsync((void*)w_new, 0, sizeof(int) * ctx->nw, ctx->stream));
+    return B2G_OK;
+  }
+  ctx->pending_new_fixture = false;
+  TRY(dispatch_step(ctx, 0.0, 0, 0, true, nullptr, PART_PAIRS));
+  const dim3 grid((unsigned)((ctx->nw + kBlock - 1) / kBlock));
+  if (ctx->precision == B2G_FP64)
+    hipLaunchKernelGGL(k_clear_new_fixture<double>, grid, dim3(kBlock), 0, ctx->stream, *(Dev<double>*)ctx->dev);
+  else
+    hipLaunchKernelGGL(k_clear_new_fixture<float>, grid, dim3(kBlock), 0, ctx->stream, *(Dev<float>*)ctx->dev);
+  HIPCHK(hipGetLastError());
+  return B2G_OK;
+}
+
+B2G_API int b2g_step_collide(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (vel_iters < 0 || pos_iters < 0) return set_error(ctx, B2G_E_INVALID, "negative iteration count");
+  if (ctx->split) return set_error(ctx, B2G_E_INVALID, "b2g_step_collide: the last split step was not finished");
+  hipSetDevice(ctx->device);
+  TRY(ensure_island_times(ctx));
+  const bool first = ctx->pending_new_fixture;
+  ctx->pending_new_fixture = false;
+  TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, first, nullptr, PART_COLLIDE));
+  ctx->split = true;
+  ctx->split_dt = dt;
+  ctx->split_vi = vel_iters;
+  ctx->split_pi = pos_iters;
+  return B2G_OK;
+}
+
+B2G_API int b2g_step_solve(b2g_ctx* ctx) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (!ctx->split) return set_error(ctx, B2G_E_INVALID, "b2g_step_solve: no step begun with b2g_step_collide");
+  hipSetDevice(ctx->device);
+  ctx->split = false;
+  TRY(dispatch_step(ctx, ctx->split_dt, ctx->split_vi, ctx->split_pi, false, nullptr, PART_SOLVE));
+  ctx->stat_steps += 1;
+  return B2G_OK;
+}
+
+B2G_API int b2g_disable_contacts(b2g_ctx* ctx, int32_t n, const int32_t* world, const int32_t* slot) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (n < 0 || (n > 0 && (!world || !slot))) return set_error(ctx, B2G_E_INVALID, "b2g_disable_contacts: bad arguments");
+  if (!ctx->split) return set_error(ctx, B2G_E_INVALID, "b2g_disable_contacts: only between collide and solve");
+  if (n == 0) return B2G_OK;
+  hipSetDevice(ctx->device);
+  TRY(b2g_sync(ctx));
+  return ctx->precision == B2G_FP64 ? disable_contacts_t<double>(ctx, n, world, slot)
+                                    : disable_contacts_t<float>(ctx, n, world, slot);
+}
+
+B2G_API int b2g_new_contacts(b2g_ctx* ctx, int32_t* out, int32_t cap, int32_t* count) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (!count || cap < 0 || (cap > 0 && !out)) return set_error(ctx, B2G_E_INVALID, "b2g_new_contacts: bad arguments");
+  *count = 0;
+  hipSetDevice(ctx->device);
+  TRY(b2g_sync(ctx));  // a step that ran out of contact slots listed nothing usable
+  return ctx->precision == B2G_FP64 ? new_contacts_t<double>(ctx, out, cap, count)
+                                    : new_contacts_t<float>(ctx, out, cap, count);
+}
+
+B2G_API int b2g_drop_contacts(b2g_ctx* ctx, int32_t n, const int32_t* world, const int32_t* index) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (n < 0 || (n > 0 && (!world || !index))) return set_error(ctx, B2G_E_INVALID, "b2g_drop_contacts: bad arguments");
+  if (n == 0) return B2G_OK;
+  hipSetDevice(ctx->device);
+  TRY(b2g_sync(ctx));
+  return ctx->precision == B2G_FP64 ? drop_contacts_t<double>(ctx, n, world, index)
+                                    : drop_contacts_t<float>(ctx, n, world, index);
+}
+
+B2G_API int b2g_events(b2g_ctx* ctx, int32_t* out_i, double* out_d, int32_t cap, int32_t* count) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (!count || cap < 0 || (cap > 0 && (!out_i || !out_d)))
+    return set_error(ctx, B2G_E_INVALID, "b2g_events: bad arguments");
+  *count = 0;
+  hipSetDevice(ctx->device);
+  TRY(b2g_sync(ctx));
+  return ctx->precision == B2G_FP64 ? events_t<double>(ctx, out_i, out_d, cap, count)
+                                    : events_t<float>(ctx, out_i, out_d, cap, count);
 }
 
 B2G_API int b2g_stats(b2g_ctx* ctx, int64_t* out, int32_t n, int32_t reset) {

@@ -101,13 +101,38 @@ class Fixture(object):
     def category_bits(self):
         return self._category_bits
 
+    @category_bits.setter
+    def category_bits(self, category_bits):
+        self._category_bits = category_bits
+        self._refilter()
+
     @property
     def mask_bits(self):
         return self._mask_bits
 
+    @mask_bits.setter
+    def mask_bits(self, mask_bits):
+        self._mask_bits = mask_bits
+        self._refilter()
+
     @property
     def group_index(self):
         return self._group_index
+
+    @group_index.setter
+    def group_index(self, group_index):
+        self._group_index = group_index
+        self._refilter()
+
+    def _refilter(self):
+        """fixture.py:269-285: the fixture's contacts are tested against the filter again at the next
+        step where one of their bodies is awake, and the pair search reconsiders its pairs."""
+        world = self.world
+        if world is not None:
+            if world._locked:  # inside a listener callback: it takes effect from the next step on
+                world._deferred_refilter.append(self)
+            else:
+                world._refilter(self)
 
     def test_point(self, p):
         return self._shape.test_point(self._body._xf, p)

@@ -49,12 +49,26 @@ CON_ND = 16
 CI_FA, CI_FB, CI_FLAGS, CI_MTYPE, CI_PC, CI_ID0, CI_ID1, CI_PAD = range(8)
 CON_NI = 8
 CF_TOUCHING, CF_ENABLED, CF_FILTER = 1, 2, 4
+CF_SWAPPED = 8   # b2g_new_contacts rows: the registry swapped the reported pair (B, A)
+CF_REJECT = 16   # with WF_CUSTOM_FILTER: the host's filter rejected this flagged contact
+NC_DROPPED = 16  # b2g_new_contacts rows: dropped since
 
 WD_GX, WD_GY, WD_INV_DT0 = range(3)
 WORLD_ND = 4
-WI_FLAGS = 0
+WI_FLAGS, WI_NEW_CONTACTS = 0, 1
 WORLD_NI = 4
-WF_ALLOW_SLEEP, WF_WARM_STARTING, WF_NEW_FIXTURE, WF_KEEP_FORCES = 1, 2, 4, 8
+WF_ALLOW_SLEEP, WF_WARM_STARTING, WF_NEW_FIXTURE, WF_KEEP_FORCES, WF_CUSTOM_FILTER = 1, 2, 4, 8, 16
+
+# contact listener records (b2g_events)
+EV_BEGIN, EV_END, EV_PRE_SOLVE, EV_POST_SOLVE = range(4)
+EV_ALL = 15
+(EI_WORLD, EI_KIND, EI_PHASE, EI_KEY, EI_FA, EI_FB, EI_FLAGS, EI_MTYPE, EI_PC, EI_ID0, EI_ID1,
+ EI_OLD_MTYPE, EI_OLD_PC, EI_OLD_ID0, EI_OLD_ID1, EI_PAD) = range(16)
+EV_NI = 16
+(ED_LN_X, ED_LN_Y, ED_LP_X, ED_LP_Y, ED_P0_X, ED_P0_Y, ED_P0_NI, ED_P0_TI, ED_P1_X, ED_P1_Y, ED_P1_NI,
+ ED_P1_TI) = range(12)
+ED_OLD = 12
+EV_ND = 24
 
 
 def pack_id(ia: int, ib: int, ta: int, tb: int) -> int:

@@ -209,6 +209,26 @@ def filtered_world(seed: int = 5, n_bodies: int = 36):
     return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies}
 
 
+def listener_filter_verdict(index_a: int, bits_a, index_b: int, bits_b) -> bool:
+    """The custom contact filter of the listener scenes, as a function of the two fixtures'
+    indices and (category, mask, group) bits: World.default_contact_filter on the bits, and
+    then a rule on the indices that is not symmetric ((a + 2 b) mod 5 != 0), so that a call
+    with the fixtures swapped would often give another answer."""
+    ca, ma, ga = bits_a
+    cb, mb, gb = bits_b
+    if ga == gb and ga != 0:
+        ok = ga > 0
+    else:
+        ok = (ma & cb) != 0 and (ca & mb) != 0
+    return ok and (index_a + 2 * index_b) % 5 != 0
+
+
+def listener_disable_rule(step: int, index_a: int, index_b: int) -> bool:
+    """The pre_solve of the listener scenes sets contact.enabled = False for the step when this
+    holds for the step number and the contact's fixtures' indices."""
+    return (step + index_a + 2 * index_b) % 6 == 0
+
+
 def pyramids(count: int = 200, spacing: float = 30.0):
     """One world holding `count` 10-row pyramids, each on its own static ground edge."""
     bodies = []

@@ -11,9 +11,22 @@ dynamictree.py:115-134).
 ``World.query_aabb`` and ``World.ray_cast`` run on an LBVH the device builds
 over the fat AABBs of the current state.
 
+The contact listener (``world.contact_manager.begin_contact``, ``end_contact``,
+``pre_solve``, ``post_solve``) is called with the device's records of the step,
+in the reference's call order: the step runs in two parts, and the callbacks
+of ContactManager.collide (begin, end, pre_solve) run in between, so that
+``contact.enabled = False`` keeps a contact out of the step's solve as in the
+reference; post_solve follows the solve.  Inside a callback the world is
+locked: creating or destroying bodies raises LockedError as in the reference,
+and so do Body setters (the rest of the step is already on the device).  A
+custom ``contact_filter`` runs on the host on the contacts each pair search
+creates (the device creates every pair Body._should_collide admits; the
+rejected ones are dropped before the next step) and on the contacts flagged
+by Fixture.refilter, before the step that would test them.
+
 Outside the device path, and therefore refused with NotImplementedError when a
 world holding them is stepped: continuous physics (TOI), joints, controllers,
-sensors, loop shapes, contact listeners and custom contact filters.
+sensors and loop shapes.
 """
 from __future__ import annotations
 
@@ -25,10 +38,11 @@ import numpy as np
 from . import layout as L
 from .body import Body
 from .common import LockedError, Vec2
+from .contact import Contact, ContactManager, Manifold
 from .settings import AABB_EXTENSION
 from .shapes import Circle, Edge, Loop, Polygon, check_device_shape
 
-__all__ = ("World", "ContactView", "TimeStep")
+__all__ = ("World", "TimeStep")
 
 
 class TimeStep(object):
@@ -37,53 +51,6 @@ class TimeStep(object):
     def __init__(self, dt=0.0, inv_dt=0.0, dt_ratio=0.0, vel_iters=0, pos_iters=0, warm_starting=True):
         self.dt, self.inv_dt, self.dt_ratio = float(dt), float(inv_dt), float(dt_ratio)
         self.vel_iters, self.pos_iters, self.warm_starting = int(vel_iters), int(pos_iters), bool(warm_starting)
-
-
-class ManifoldPointView(object):
-    __slots__ = ("local_point", "normal_impulse", "tangent_impulse", "id")
-
-    def __init__(self, lp, ni, ti, id_):
-        self.local_point, self.normal_impulse, self.tangent_impulse, self.id = lp, ni, ti, id_
-
-
-class ManifoldView(object):
-    __slots__ = ("type", "point_count", "local_normal", "local_point", "points")
-
-    @property
-    def used_points(self):
-        return self.points[:self.point_count]
-
-
-class ContactView(object):
-    """Read-only view of one contact (contact.py:49-160) after the last step."""
-
-    def __init__(self, world, k, fa, fb):
-        self._fixture_a, self._fixture_b = fa, fb
-        cd, ci = world._con_d[k], world._con_i[k]
-        self._touching = bool(ci[L.CI_FLAGS] & L.CF_TOUCHING)
-        self._enabled = bool(ci[L.CI_FLAGS] & L.CF_ENABLED)
-        self.friction = float(cd[L.CD_FRICTION])
-        self.restitution = float(cd[L.CD_RESTITUTION])
-        m = ManifoldView()
-        m.type = int(ci[L.CI_MTYPE])
-        m.point_count = int(ci[L.CI_PC])
-        m.local_normal = Vec2(cd[L.CD_LN_X], cd[L.CD_LN_Y])
-        m.local_point = Vec2(cd[L.CD_LP_X], cd[L.CD_LP_Y])
-        m.points = [ManifoldPointView(Vec2(cd[L.CD_P0_X], cd[L.CD_P0_Y]), float(cd[L.CD_P0_NI]),
-                                      float(cd[L.CD_P0_TI]), L.unpack_id(ci[L.CI_ID0])),
-                    ManifoldPointView(Vec2(cd[L.CD_P1_X], cd[L.CD_P1_Y]), float(cd[L.CD_P1_NI]),
-                                      float(cd[L.CD_P1_TI]), L.unpack_id(ci[L.CI_ID1]))]
-        self.manifold = m
-
-    fixture_a = property(lambda self: self._fixture_a)
-    fixture_b = property(lambda self: self._fixture_b)
-    body_a = property(lambda self: self._fixture_a._body)
-    body_b = property(lambda self: self._fixture_b._body)
-    touching = property(lambda self: self._touching)
-    enabled = property(lambda self: self._enabled)
-
-    def other_body(self, body):
-        return self.body_b if body is self.body_a else self.body_a
 
 
 def body_record(b: Body):
@@ -172,8 +139,11 @@ class World(object):
         self._auto_clear_forces = auto_clear_forces
         self.joint_destruction_listener = joint_destruction_listener
         self.fixture_destruction_listener = fixture_destruction_listener
-        self._contact_filter = contact_filter
-        self.begin_contact = self.end_contact = self.pre_solve = self.post_solve = None
+        self.contact_manager = ContactManager(self)
+        self._contact_filter = self.default_contact_filter if contact_filter is None else contact_filter
+        self._views = {}              # (id(fixture a), id(fixture b)) -> Contact, while the contact exists
+        self._refilter_pending = False  # contacts flagged for filtering wait for a custom filter's verdict
+        self._deferred_refilter: List = []  # fixtures whose filter bits a listener callback changed
         self._inv_dt0 = 0.0
         self._new_fixture = False
         self._locked = False
@@ -247,7 +217,17 @@ class World(object):
 
     @contact_filter.setter
     def contact_filter(self, fcn):
+        """world.py:124-136.  None turns filtering off (every pair collides)."""
+        self._pull()
         self._contact_filter = fcn
+        self._refilter_pending = True
+        self._host_dirty = True
+
+    def _custom_filter(self) -> bool:
+        return self._contact_filter is not World.default_contact_filter
+
+    def _listening(self) -> bool:
+        return any(self.contact_manager.contact_listener)
 
     # ---- bodies ----
     def add_body(self, body: Body) -> Body:
@@ -292,7 +272,12 @@ class World(object):
             raise ValueError("Body not in this world")
         self._pull()
         fixtures = set(id(f) for f in body.fixtures)
-        self._drop_contacts(lambda fa, fb: id(fa) in fixtures or id(fb) in fixtures)
+        # Body._detached_from_world (body.py:310-345): its contacts are destroyed (end_contact for the
+        # touching ones, ContactManager.destroy), then the fixture destruction listener is called
+        self._destroy_contacts(lambda fa, fb: id(fa) in fixtures or id(fb) in fixtures)
+        if self.fixture_destruction_listener is not None:
+            for f in list(body.fixtures):
+                self.fixture_destruction_listener(f)
         self._move = [f for f in self._move if id(f) not in fixtures]
         self.bodies.remove(body)
         body._world = None
@@ -307,8 +292,22 @@ class World(object):
         self._host_dirty = True
 
     def _fixture_removed(self, body, fixture):
-        self._drop_contacts(lambda fa, fb: fa is fixture or fb is fixture)
+        """Body.destroy_fixture (body.py:258-286): the fixture's contacts are destroyed."""
+        self._destroy_contacts(lambda fa, fb: fa is fixture or fb is fixture)
         self._move = [f for f in self._move if f is not fixture]
+        self._host_dirty = True
+
+    def _refilter(self, fixture):
+        """Fixture.refilter (fixture.py:269-285): the fixture's contacts are flagged for filtering
+        at the next step where a body of theirs is awake, and its proxy is touched so that the pair
+        search reconsiders its pairs."""
+        self._pull()
+        for k, (fa, fb) in enumerate(self._con_fix):
+            if fa is fixture or fb is fixture:
+                self._con_i[k, L.CI_FLAGS] |= L.CF_FILTER
+        if fixture._body._active and fixture not in self._move:
+            self._move.append(fixture)
+        self._refilter_pending = True
         self._host_dirty = True
 
     def _topology_changed(self):
@@ -336,6 +335,33 @@ class World(object):
         self._con_d = self._con_d[keep]
         self._con_i = self._con_i[keep]
         self._con_fix = [self._con_fix[k] for k in keep]
+
+    def _destroy_contacts(self, pred):
+        """ContactManager.destroy (contact_manager.py:160-172) of every contact matching pred, in
+        list order: end_contact for those touching, then removal."""
+        end = self.contact_manager.end_contact
+        gone = [k for k, (fa, fb) in enumerate(self._con_fix) if pred(fa, fb)]
+        views = [self._row_view(k) for k in gone]
+        self._drop_contacts(pred)
+        for c in views:
+            self._views.pop((id(c._fixture_a), id(c._fixture_b)), None)
+            if end is not None and c._touching_flag:
+                end(c)
+
+    def _view(self, fa, fb) -> Contact:
+        """The contact object of a fixture pair (the same object while the contact exists)."""
+        key = (id(fa), id(fb))
+        c = self._views.get(key)
+        if c is None:
+            c = Contact(self, fa, fb)
+            self._views[key] = c
+        return c
+
+    def _row_view(self, k) -> Contact:
+        fa, fb = self._con_fix[k]
+        c = self._view(fa, fb)
+        c._load_row(self._con_d[k], self._con_i[k])
+        return c
 
     # ---- device state ----
     def _fixture_list(self):
@@ -367,7 +393,8 @@ class World(object):
         wi[0, L.WI_FLAGS] = ((L.WF_ALLOW_SLEEP if self._allow_sleep else 0)
                              | (L.WF_WARM_STARTING if self._warm_starting else 0)
                              | (L.WF_NEW_FIXTURE if self._new_fixture else 0)
-                             | (0 if self._auto_clear_forces else L.WF_KEEP_FORCES))
+                             | (0 if self._auto_clear_forces else L.WF_KEEP_FORCES)
+                             | (L.WF_CUSTOM_FILTER if self._custom_filter() else 0))
         return L.BatchArrays(np.array([0, nb], np.int32), np.array([0, nf], np.int32),
                              np.array([0, len(ci)], np.int32), np.array([0, len(move)], np.int32),
                              wd, wi, bd, bi, fd, fi, self._con_d.copy(), ci, move)
@@ -378,9 +405,6 @@ class World(object):
                                       "create the world with continuous_physics=False")
         if self.joints or self.controllers:
             raise NotImplementedError("joints and controllers are outside the device contact pipeline")
-        if self._contact_filter is not None or any((self.begin_contact, self.end_contact, self.pre_solve,
-                                                    self.post_solve)):
-            raise NotImplementedError("contact listeners and custom contact filters are not supported")
         for b in self.bodies:
             for f in b.fixtures:
                 if f._sensor:
@@ -429,6 +453,10 @@ class World(object):
         """One World.step (world.py:307-354) on the GPU."""
         if self._locked:
             raise LockedError
+        custom = self._custom_filter()
+        if custom and self._refilter_pending:
+            self._pull()
+            self._filter_flagged()
         if self._host_dirty or self._stepper is None:
             self._pull()
             self._push()
@@ -436,13 +464,114 @@ class World(object):
         try:
             # steps, then checks the device's error flags: a capacity error grows the contact
             # slots and replays from the last synchronised state; non-finite state raises
-            self._stepper.step(float(dt), int(vel_iters), int(pos_iters))
+            if custom or self._listening():
+                self._step_listened(float(dt), int(vel_iters), int(pos_iters), custom)
+            else:
+                self._stepper.step(float(dt), int(vel_iters), int(pos_iters))
             self._device_ahead = True
             self._new_fixture = False
             if dt > 0.0:
                 self._inv_dt0 = 1.0 / dt
         finally:
             self._locked = False
+        deferred, self._deferred_refilter = self._deferred_refilter, []
+        for f in deferred:
+            if f.world is self:
+                self._refilter(f)
+
+    # ---- contact listener and custom filter ----
+    def _filter_flagged(self):
+        """ContactManager.collide's test of the contacts flagged for filtering (contact_manager.py:
+        192-208) with the custom filter, for the flagged contacts the coming step will test (a body
+        of theirs awake and not static): Body._should_collide, then the filter.  The device destroys
+        the rejected ones in that step's collide (B2G_CF_REJECT); the others keep no flag."""
+        pending = False
+        for k, (fa, fb) in enumerate(self._con_fix):
+            fl = int(self._con_i[k, L.CI_FLAGS])
+            if not fl & L.CF_FILTER or fl & L.CF_REJECT:
+                continue
+            ba, bb = fa._body, fb._body
+            if not ((ba._awake and ba._type != Body.STATIC) or (bb._awake and bb._type != Body.STATIC)):
+                pending = True
+                continue
+            ok = Body.DYNAMIC in (ba._type, bb._type) and (self._contact_filter is None or
+                                                            bool(self._contact_filter(fa, fb)))
+            self._con_i[k, L.CI_FLAGS] = (fl & ~L.CF_FILTER) if ok else (fl | L.CF_REJECT)
+            self._host_dirty = True
+        self._refilter_pending = pending
+
+    def _step_listened(self, dt, vel_iters, pos_iters, custom):
+        st = self._stepper
+        cm = self.contact_manager
+        mask = ((1 << L.EV_BEGIN) | (1 << L.EV_END) | ((1 << L.EV_PRE_SOLVE) if cm.pre_solve else 0)
+                | ((1 << L.EV_POST_SOLVE) if cm.post_solve else 0))
+        if st.params.get("events") != mask:
+            st.set_param("events", mask)
+        fixtures = self._fixture_list()
+        if custom and self._new_fixture:  # the new fixtures' pairs, filtered before the step uses them
+            st.pair_search()
+            self._new_contacts(fixtures, custom)
+        st.collide(dt, vel_iters, pos_iters)
+        ei, ed = st.events()
+        disable = self._dispatch(ei, ed, fixtures, (0, 1))
+        if disable:
+            st.disable([0] * len(disable), disable)
+        st.solve()
+        self._device_ahead = True
+        if cm.post_solve is not None:
+            ei, ed = st.events()
+            self._dispatch(ei, ed, fixtures, (2,))
+        self._new_contacts(fixtures, custom)
+
+    def _dispatch(self, ei, ed, fixtures, phases):
+        """Calls the listener for the records of the given phases, in order; returns the contact
+        slots whose contact a callback disabled."""
+        cm = self.contact_manager
+        updated = {}
+        for r in range(ei.shape[0]):
+            row = ei[r]
+            phase = int(row[L.EI_PHASE])
+            if phase not in phases:
+                continue
+            c = self._view(fixtures[row[L.EI_FA]], fixtures[row[L.EI_FB]])
+            c._load_event(row, ed[r])
+            kind = int(row[L.EI_KIND])
+            if phase == 0:
+                updated[int(row[L.EI_KEY]) >> 2] = c
+            if kind == L.EV_BEGIN:
+                if cm.begin_contact is not None:
+                    cm.begin_contact(c)
+            elif kind == L.EV_END:
+                if cm.end_contact is not None:
+                    cm.end_contact(c)
+                if phase == 1:  # destroyed
+                    self._views.pop((id(c._fixture_a), id(c._fixture_b)), None)
+            elif kind == L.EV_PRE_SOLVE:
+                if cm.pre_solve is not None:
+                    cm.pre_solve(c, Manifold.from_record(ed[r, L.ED_OLD:], row[L.EI_OLD_MTYPE:L.EI_OLD_MTYPE + 4]))
+            elif kind == L.EV_POST_SOLVE:
+                if cm.post_solve is not None:
+                    o = L.ED_OLD
+                    cm.post_solve(c, [(float(ed[r, o + L.ED_P0_NI + 4 * j]), float(ed[r, o + L.ED_P0_TI + 4 * j]))
+                                      for j in range(int(row[L.EI_OLD_PC]))])
+        return sorted(slot for slot, c in updated.items() if not c._enabled_flag)
+
+    def _new_contacts(self, fixtures, custom):
+        """The contacts the last pair search created: each is a new contact object, and with a custom
+        filter (contact_manager.py:144-146) each is offered to it as the pair search reported it
+        (registry swap undone); the rejected ones are dropped."""
+        rows = self._stepper.new_contacts()
+        drop = []
+        f = self._contact_filter
+        for j in range(rows.shape[0]):
+            a, b = fixtures[rows[j, 1]], fixtures[rows[j, 2]]
+            self._views.pop((id(a), id(b)), None)
+            if custom and f is not None:
+                p, q = (b, a) if rows[j, 3] & L.CF_SWAPPED else (a, b)
+                if not f(p, q):
+                    drop.append(j)
+        if drop:
+            self._stepper.drop([0] * len(drop), drop)
 
     # ---- queries (world.py:371-440), on the device's LBVH over the fat AABBs ----
     def _device_state(self):
@@ -516,9 +645,13 @@ class World(object):
         self._host_dirty = True
 
     @property
-    def contacts(self) -> List[ContactView]:
+    def contacts(self) -> List[Contact]:
+        """The contact list (contact_manager.contacts), in the device's list order."""
         self._pull()
-        return [ContactView(self, k, fa, fb) for k, (fa, fb) in enumerate(self._con_fix)]
+        out = [self._row_view(k) for k in range(len(self._con_fix))]
+        if not self._locked:  # forget the contacts that no longer exist
+            self._views = {(id(c._fixture_a), id(c._fixture_b)): c for c in out}
+        return out
 
     @property
     def contact_count(self):

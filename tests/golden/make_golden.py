@@ -1,7 +1,7 @@
 """Writes the golden fixtures of the contact pipeline (tests/golden/*.npz).
 
 Run once in the build container:  python tests/golden/make_golden.py [names]
-(names: pyramid stacks mixed filtered queries narrowphase; default all)
+(names: pyramid stacks mixed filtered listeners queries narrowphase; default all)
 
 It imports pypybox2d from /root/reference in its default pure-Python flavour
 (the reference tree ships no compiled _common) and records, for small scenes:
@@ -236,6 +236,110 @@ def save_run(path, init, hints, states, meta):
 
 
 # --------------------------------------------------------------------------
+# contact listener and custom contact filter
+# --------------------------------------------------------------------------
+def _ev_manifold(m, o, oi):
+    o[0], o[1], o[2], o[3] = m.local_normal.x, m.local_normal.y, m.local_point.x, m.local_point.y
+    p0, p1 = m.points[0], m.points[1]
+    o[4], o[5], o[6], o[7] = p0.local_point.x, p0.local_point.y, p0.normal_impulse, p0.tangent_impulse
+    o[8], o[9], o[10], o[11] = p1.local_point.x, p1.local_point.y, p1.normal_impulse, p1.tangent_impulse
+    oi[0], oi[1], oi[2], oi[3] = m.type, m.point_count, L.pack_id(*p0.id), L.pack_id(*p1.id)
+
+
+def record_listener_run(scene, steps, checkpoints):
+    """A reference run with every contact listener set (contact_manager.begin_contact, end_contact,
+    pre_solve, post_solve) and a custom contact_filter (scenes.listener_filter_verdict on the
+    fixtures' indices, kept in user_data, and their filter bits).  pre_solve disables contacts
+    by scenes.listener_disable_rule.  Each call is recorded in the event layout of b2g_layout.h (the
+    phase and key fields, which order the device's records, are -1 here: the list is in call
+    order).  After steps 60, 110 and 150 the filter bits of fixtures change (Fixture.refilter):
+    a fixture stops colliding, collides again, and two touching fixtures join a negative group;
+    the state right after each change is recorded too."""
+    world = build_ref_world(scene)
+    fixtures = [f for b in world.bodies for f in b.fixtures]
+    for k, f in enumerate(fixtures):
+        f.user_data = k
+
+    def bits(f):
+        return (f._category_bits, f._mask_bits, f._group_index)
+
+    world.contact_filter = lambda fa, fb: scenes.listener_filter_verdict(fa.user_data, bits(fa), fb.user_data, bits(fb))
+    cur = {"step": 0}
+    rows = []
+
+    def ev(kind, c, old=None, impulses=None):
+        ei = np.zeros(L.EV_NI, np.int32)
+        ed = np.zeros(L.EV_ND)
+        ei[L.EI_KIND], ei[L.EI_PHASE], ei[L.EI_KEY] = kind, -1, -1
+        ei[L.EI_FA], ei[L.EI_FB] = c._fixture_a.user_data, c._fixture_b.user_data
+        ei[L.EI_FLAGS] = (L.CF_TOUCHING if c._touching_flag else 0) | (L.CF_ENABLED if c._enabled_flag else 0)
+        _ev_manifold(c._manifold, ed[:12], ei[L.EI_MTYPE:L.EI_MTYPE + 4])
+        if old is not None:
+            _ev_manifold(old, ed[L.ED_OLD:], ei[L.EI_OLD_MTYPE:L.EI_OLD_MTYPE + 4])
+        if impulses is not None:
+            ei[L.EI_OLD_PC] = len(impulses)
+            for j, (n, t) in enumerate(impulses):
+                ed[L.ED_OLD + L.ED_P0_NI + 4 * j], ed[L.ED_OLD + L.ED_P0_TI + 4 * j] = n, t
+        rows.append((cur["step"], ei, ed))
+
+    def pre_solve(c, old):
+        ev(L.EV_PRE_SOLVE, c, old=old)
+        if scenes.listener_disable_rule(cur["step"], c._fixture_a.user_data, c._fixture_b.user_data):
+            c.enabled = False
+
+    cm = world.contact_manager
+    cm.begin_contact = lambda c: ev(L.EV_BEGIN, c)
+    cm.end_contact = lambda c: ev(L.EV_END, c)
+    cm.pre_solve = pre_solve
+    cm.post_solve = lambda c, imp: ev(L.EV_POST_SOLVE, c, impulses=imp)
+    init, fidx = snapshot(world)
+    hints, states, edit_states, edits = [], {}, {}, []
+
+    def touching_partners(f):
+        return [c for c in cm.contacts if c._touching_flag and f in (c._fixture_a, c._fixture_b)]
+
+    picked = {}
+    for s in range(1, steps + 1):
+        cur["step"] = s
+        prev = list(cm.contacts)
+        prev_ids = set(id(c) for c in prev)
+        world.step(1.0 / 60, 8, 3)
+        new = [c for c in cm.contacts if id(c) not in prev_ids]
+        hints.append([(fidx[id(c._fixture_a)], fidx[id(c._fixture_b)]) for c in new])
+        del prev
+        if s in checkpoints:
+            states[s] = snapshot(world)[0]
+        changed = []
+        if s == 60:  # the dynamic fixture touching most others stops colliding
+            dyn = [f for f in fixtures if f._body._type == b2.Body.DYNAMIC]
+            f = max(dyn, key=lambda f: (len(touching_partners(f)), -f.user_data))
+            picked["x"], picked["mask"] = f, f._mask_bits
+            f.mask_bits = 0
+            changed.append(f)
+        elif s == 110:  # ... and collides again
+            f = picked["x"]
+            f.mask_bits = picked["mask"]
+            changed.append(f)
+        elif s == 150:  # two touching dynamic fixtures join group -3 (never collide)
+            pairs = [c for c in cm.contacts if c._touching_flag and c._fixture_a._body._type == b2.Body.DYNAMIC
+                     and c._fixture_b._body._type == b2.Body.DYNAMIC]
+            c = min(pairs, key=lambda c: (c._fixture_a.user_data, c._fixture_b.user_data))
+            for f in (c._fixture_a, c._fixture_b):
+                f.group_index = -3
+                changed.append(f)
+        if changed:
+            edit_states[s] = snapshot(world)[0]
+            edits.extend((s, f.user_data, f._category_bits, f._mask_bits, f._group_index) for f in changed)
+    off = np.zeros(steps + 1, np.int32)
+    for st, _, _ in rows:
+        off[st] += 1
+    off = np.concatenate([[0], np.cumsum(off[1:])]).astype(np.int32)
+    return (init, hints, states, edit_states, np.array(edits, np.int32), off,
+            np.array([r[1] for r in rows], np.int32).reshape(-1, L.EV_NI),
+            np.array([r[2] for r in rows]).reshape(-1, L.EV_ND))
+
+
+# --------------------------------------------------------------------------
 # narrow phase cases
 # --------------------------------------------------------------------------
 def _rand_polygon(rng, k=None, rad=None):
@@ -432,7 +536,7 @@ def world_queries(scene, steps, rng, nq=200, nr=200):
 
 def main(which=None):
     """Records every fixture, or only those named (pyramid, stacks, mixed, filtered, queries, narrowphase)."""
-    want = set(which or ("pyramid", "stacks", "mixed", "filtered", "queries", "narrowphase"))
+    want = set(which or ("pyramid", "stacks", "mixed", "filtered", "listeners", "queries", "narrowphase"))
     if "pyramid" in want:
         pyr_ck = {1, 2, 3, 4, 5, 10, 20, 50, 100, 150, 200, 300, 400, 500, 600}
         init, hints, states = record_run(scenes.pyramid(), 600, pyr_ck)
@@ -451,6 +555,19 @@ def main(which=None):
         filt_ck = {1, 2, 3, 5, 10, 20, 40, 60, 80, 100, 120, 150, 200, 250, 300}
         init, hints, states = record_run(scenes.filtered_world(), 300, filt_ck)
         save_run(os.path.join(HERE, "filtered_kinematic_300.npz"), init, hints, states, {"steps": 300})
+    if "listeners" in want:
+        ck = {1, 2, 5, 10, 20, 40, 60, 61, 80, 100, 110, 111, 112, 130, 150, 151, 170, 200}
+        init, hints, states, edit_states, edits, ev_off, ev_i, ev_d = record_listener_run(scenes.filtered_world(), 200,
+                                                                                         ck)
+        path = os.path.join(HERE, "listeners_filtered_200.npz")
+        save_run(path, init, hints, states, {"steps": 200})
+        z = dict(np.load(path))
+        for s, st in edit_states.items():
+            for name in L._FIELDS:
+                z["e%d_%s" % (s, name)] = getattr(st, name)
+        z.update(edits=edits, ev_off=ev_off, ev_i=ev_i, ev_d=ev_d)
+        np.savez_compressed(path, **z)
+        print("listener run:", len(ev_i), "calls,", len(edits), "filter edits")
     if "queries" in want:
         np.savez_compressed(os.path.join(HERE, "shape_raycast.npz"), **shape_ray_cases(np.random.default_rng(17)))
         rng = np.random.default_rng(19)

@@ -188,6 +188,51 @@ def test_filtered_kinematic_trajectory_fp64_tracks_oracle(large_worlds):
     e.close()
 
 
+@pytest.mark.parametrize("large_worlds", [0, 1])
+def test_a_touched_proxy_is_searched_at_the_end_of_the_step(large_worlds):
+    """A move-buffer entry the batch brings without new fixtures (Fixture.refilter touching a proxy,
+    fixture.py:282-285) stays in the buffer until the end-of-step pair search, ahead of the proxies
+    the synchronisation moves (broadphase.py:61-68, 86-99); a dt = 0 step keeps it (no
+    find_new_contacts runs).  From the recorded filtered run at step 150, the first category-2
+    fixture becomes category 1 with a full mask, its contacts are flagged for filtering and its
+    proxy touched: the pairs it gains (two, which the untouched buffer would miss) must be created
+    as the oracle creates them."""
+    init, z = load_init("filtered_kinematic_300.npz")
+
+    def edited(touch):
+        b = golden_state(z, init, 150)
+        f = [k for k in range(b.fix_i.shape[0]) if b.fix_i[k, L.FI_CATEGORY] == 2][0]
+        b.fix_i[f, L.FI_CATEGORY], b.fix_i[f, L.FI_MASK] = 1, 0xFFFF
+        for k in range(b.con_i.shape[0]):
+            if f in (b.con_i[k, L.CI_FA], b.con_i[k, L.CI_FB]):
+                b.con_i[k, L.CI_FLAGS] |= L.CF_FILTER
+        if touch:
+            b.move = np.array([f], np.int32)
+            b.move_off = np.array([0, 1], np.int32)
+        return b, f
+    b, f = edited(True)
+    untouched = O.Oracle(edited(False)[0])
+    untouched.step(DT, 8, 3)
+    o = O.Oracle(b)
+    e = N.Engine("fp64")
+    e.set_param("large_worlds", large_worlds)
+    e.upload(b)
+    o.step(0.0, 8, 3)
+    e.step(0.0, 8, 3)
+    ref, got = o.state(), e.download()
+    assert list(got.move) == [f] and list(ref.move) == [f]  # a dt = 0 step keeps the buffer
+    for s in range(1, 31):
+        assert o.step(DT, 8, 3) == 0
+        e.step(DT, 8, 3)
+        ref, got = o.state(), e.download()
+        assert PU.same_contact_lists(got, ref), s
+        if s == 1:
+            assert ref.con_i.shape[0] > untouched.state().con_i.shape[0]
+    be = PU.body_errors(got, ref)
+    assert max(be[q] for q in PU.QUANTITIES) <= 1e-12, be
+    e.close()
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_batched_mixed_worlds_one_step(prec):
     rng = np.random.default_rng(3)

@@ -2,6 +2,7 @@
 that the product path has no CPU fallback."""
 import os
 
+import numpy as np
 import pytest
 
 import pypybox2d_amd as b2
@@ -34,11 +35,40 @@ def test_sensor_is_rejected():
         world.step(1.0 / 60, 8, 3)
 
 
-def test_contact_listener_is_rejected():
+def test_contact_listener_and_filter_are_set_on_the_contact_manager():
+    """contact_manager.py:47-100 and world.py:86-136: the listener's callbacks live on the world's
+    contact manager; the contact filter is the world's, World.default_contact_filter unless set."""
     world = _pyramid_world(continuous_physics=False)
-    world.begin_contact = lambda contact: None
-    with pytest.raises(NotImplementedError):
-        world.step(1.0 / 60, 8, 3)
+    cm = world.contact_manager
+    assert cm.contact_listener == [None, None, None, None]
+    assert world.contact_filter is b2.World.default_contact_filter and not world._custom_filter()
+    f = lambda fa, fb: True  # noqa: E731
+    cm.begin_contact = f
+    cm.contact_filter = f
+    assert cm.contact_listener[0] is f and world.contact_filter is f and world._custom_filter()
+    world.contact_filter = None  # the reference's "no filtering"
+    assert world._custom_filter() and cm.contact_filter is None
+    world2 = b2.World(contact_filter=f)
+    assert world2.contact_filter is f
+
+
+def test_fixture_filter_setters_flag_the_fixtures_contacts():
+    """Fixture.refilter (fixture.py:269-285) on the host: the fixture's contacts are flagged and its
+    proxy is touched (the world's move buffer)."""
+    world = _pyramid_world(continuous_physics=False)
+    body = world.bodies[1]
+    fx = body.fixtures[0]
+    world._con_fix = [(world.bodies[0].fixtures[0], fx), (world.bodies[2].fixtures[0], world.bodies[3].fixtures[0])]
+    world._con_i = np.zeros((2, b2.layout.CON_NI), np.int32)
+    world._con_d = np.zeros((2, b2.layout.CON_ND))
+    world._move = []
+    fx.mask_bits = 0x0002
+    assert fx.mask_bits == 0x0002
+    assert world._con_i[0, b2.layout.CI_FLAGS] & b2.layout.CF_FILTER and not world._con_i[1, b2.layout.CI_FLAGS]
+    assert world._move == [fx] and world._refilter_pending
+    world._locked = True  # inside a listener callback the change waits for the step's end
+    fx.group_index = -2
+    assert fx.group_index == -2 and world._deferred_refilter == [fx]
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")

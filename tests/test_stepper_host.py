@@ -39,6 +39,12 @@ class FakeEngine:
     def set_param(self, k, v):
         self.params[k] = v
 
+    def pair_search(self):
+        self.calls.append(("pairs",))
+
+    def drop_contacts(self, w, j):
+        self.calls.append(("drop", list(w), list(j)))
+
     def download(self):
         b = start_state()
         b.body_d[0, 0] = self.steps  # the "state" is the number of steps taken
@@ -71,7 +77,7 @@ def test_capacity_error_regrows_and_replays_from_the_last_good_state():
     assert s.download().body_d[0, 0] == 5
     # a new checkpoint: later replays start from the downloaded state
     s.step(0.02, 6, 2)
-    assert s._log == [(1, 0.02, 6, 2)] and int(s._ckpt.body_d[0, 0]) == 5
+    assert s._log == [("step", 1, 0.02, 6, 2)] and int(s._ckpt.body_d[0, 0]) == 5
 
 
 def test_other_errors_are_raised_and_not_retried():
@@ -95,3 +101,30 @@ def test_regrow_gives_up_after_a_bounded_number_of_attempts():
     with pytest.raises(B2GError) as e:
         s.step(0.01, 8, 3)
     assert e.value.code == B2G_E_CAPACITY
+
+
+def test_replay_repeats_pair_searches_and_drops_in_order():
+    """A custom filter's work between steps (a pair search run ahead of a step, drops of new
+    contacts) is part of the replay, in the order it happened."""
+    FakeEngine.made = []
+    s = Stepper("fp32", 0, contact_factor=4.0, engine_factory=lambda *a, **k: FakeEngine(*a, need=8.0, at=3, **k))
+    s.upload(start_state())
+    s.pair_search()
+    s.drop([0, 0], [1, 3])
+    s.step(0.01, 8, 3)
+    s.drop([0], [0])
+    s.step(0.01, 8, 3, n=2)  # reaches step 3 -> regrow to factor 8
+    assert s.regrows == 1
+    assert FakeEngine.made[-1].calls == [("upload", 0), ("pairs",), ("drop", [0, 0], [1, 3]), ("step", 1, 0.01, 8, 3),
+                                         ("drop", [0], [0]), ("step", 2, 0.01, 8, 3)]
+
+
+def test_a_long_log_is_checkpointed():
+    from pypybox2d_amd import stepper
+    FakeEngine.made = []
+    s = Stepper("fp32", 0, engine_factory=FakeEngine)
+    s.upload(start_state())
+    for k in range(stepper.MAX_LOG + 1):
+        s.drop([0], [0])
+        s.step(0.01, 8, 3)
+    assert len(s._log) <= stepper.MAX_LOG and int(s._ckpt.body_d[0, 0]) > 0
