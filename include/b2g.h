@@ -212,7 +212,9 @@ B2G_API int b2g_ray_cast(b2g_ctx* ctx, int n, const int32_t* world, const double
    (find_new_contacts, contact_manager.py:102-158), out[4] proxies re-fattened
    and moved (DynamicTree.move_proxy returning true, dynamictree.py:152-188),
    out[5] island bodies and out[6] island contacts solved (the lengths of
-   Island.bodies and Island.contacts, world.py:494-580, summed over islands).
+   Island.bodies and Island.contacts, world.py:494-580, summed over islands),
+   with continuous physics out[7] time_of_impact calls (toi.py:215) and out[8]
+   TOI events (sub-step island solves, world.py:801).
    At most n values are written; reset != 0 zeroes the counters.  No
    counterpart in the reference (the per-step counts that set the step's
    algorithmic bytes). */

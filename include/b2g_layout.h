@@ -126,6 +126,9 @@ enum { B2G_WI_FLAGS = 0, B2G_WI_NEW_CONTACTS, B2G_WI_PAD2, B2G_WI_PAD3, B2G_WORL
    the new contacts and drops the rejected ones (b2g_drop_contacts) before the next step, and
    decides the contacts flagged for filtering (B2G_CF_REJECT) */
 #define B2G_WF_CUSTOM_FILTER 16u
+/* continuous physics on (World.continuous_physics, world.py:345-346): after each step with
+   dt > 0, World.solve_toi (world.py:594-821) sub-steps the first impacts of fast bodies */
+#define B2G_WF_CONTINUOUS 32u
 
 /* ---- contact event record (b2g_events): int32 x B2G_EV_NI, float64 x B2G_EV_ND ----
    One callback of the contact listener (contact.py:210-268, contact_manager.py:160-172,

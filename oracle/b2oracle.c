@@ -217,6 +217,8 @@ typedef struct {
   V2 force; double torque;
   double mass, inv_mass, I, invI;
   double gscale, ldamp, adamp, sleep_time;
+  double alpha0;  /* Sweep.alpha0 (common.py:870-923): the sweep's start within the step */
+  int toi_flag;   /* in the TOI island being built */
 } Body;
 
 typedef struct { V2 lp; double ni, ti; int id[4]; } MPoint;
@@ -228,6 +230,8 @@ typedef struct {
   Manifold m;
   int touching, enabled, filter, island_flag;
   int swapped; /* the registry swapped the reported pair (fa, fb) = (B, A) */
+  double toi;                        /* cached time of impact (contact.py:64-81) */
+  int toi_flag, toi_count, toi_island;
 } Contact;
 
 typedef struct {
@@ -261,6 +265,11 @@ typedef struct {
   int split_vi, split_pi;
   int* slot_map;
   int slot_n;
+  /* continuous physics (World.solve_toi, world.py:594-821): on/off, listener records keyed in
+     call order while it runs, and counts of time_of_impact calls and TOI events (sub-step
+     island solves) in the last step */
+  int continuous, toi_mode, toi_seq;
+  long long toi_calls, toi_events;
 } World;
 
 #define ORC_E_UNSUPPORTED 1
@@ -867,6 +876,10 @@ static void ev_manifold(const Manifold* m, double* o, int32_t* oi) {
 
 static int ev_push(World* w, int kind, int phase, int key, const Contact* c) {
   if (!(w->ev_mask & (1 << kind))) return -1;
+  if (w->toi_mode) { /* inside World.solve_toi: records come after the step's others, in call order */
+    phase = 3;
+    key = w->toi_seq++;
+  }
   if (w->ev_n == w->ev_cap) {
     int cap = w->ev_cap ? 2 * w->ev_cap : 64;
     int32_t* ni = (int32_t*)realloc(w->ev_i, sizeof(int32_t) * B2G_EV_NI * (size_t)cap);
@@ -1251,26 +1264,8 @@ typedef struct {
   PV* pv; VC* vc; PC* pc;
 } IslandBuf;
 
-static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py:65-179 */
-  double dt = st->dt;
-  PV* pv = is->pv;
-  for (int i = 0; i < is->nbodies; ++i) {
-    Body* b = &w->b[is->bodies[i]];
-    V2 c = b->c;
-    double a = b->a;
-    V2 v = b->v;
-    double wv = b->w;
-    b->c0 = c;
-    b->a0 = a;
-    if (b->type == B2G_DYNAMIC) {
-      v = add(v, scale(dt, add(scale(b->gscale, w->gravity), scale(b->inv_mass, b->force))));
-      wv += dt * b->invI * b->torque;
-      v = scale(pclamp(1.0 - dt * b->ldamp, 0.0, 1.0), v);
-      wv *= pclamp(1.0 - dt * b->adamp, 0.0, 1.0);
-    }
-    pv[i].c = c; pv[i].a = a; pv[i].v = v; pv[i].w = wv;
-  }
-  /* ContactSolver.__init__ (contact.py:376-412) */
+/* ContactSolver.__init__ (contact.py:376-412) over the island's contacts */
+static void cs_build(World* w, IslandBuf* is, const Step* st) {
   int nvc = is->ncontacts;
   for (int k = 0; k < nvc; ++k) {
     Contact* c = &w->c[is->contacts[k]];
@@ -1307,7 +1302,12 @@ static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py
     pc->npts = c->m.pc;
     for (int j = 0; j < c->m.pc; ++j) pc->lpts[j] = c->m.pts[j].lp;
   }
-  /* initialize_velocity_constraints (contact.py:414-496) */
+}
+
+/* ContactSolver.initialize_velocity_constraints (contact.py:414-496) at the island's positions */
+static void cs_init_velocity(World* w, IslandBuf* is) {
+  PV* pv = is->pv;
+  int nvc = is->ncontacts;
   for (int k = 0; k < nvc; ++k) {
     VC* vc = &is->vc[k];
     PC* pc = &is->pc[k];
@@ -1358,6 +1358,47 @@ static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py
       }
     }
   }
+}
+
+/* Island.report (island.py:245-252): post_solve with the velocity constraints' impulses, keyed
+   from key0 in island order */
+static void cs_report(World* w, IslandBuf* is, int phase, int key0) {
+  for (int k = 0; k < is->ncontacts; ++k) {
+    const VC* vc = &is->vc[k];
+    int e = ev_push(w, B2G_EV_POST_SOLVE, phase, key0 + k, &w->c[vc->contact]);
+    if (e < 0) continue;
+    int32_t* ei = w->ev_i + (size_t)e * B2G_EV_NI;
+    double* ed = w->ev_d + (size_t)e * B2G_EV_ND;
+    ei[B2G_EI_OLD_PC] = vc->npts; /* the impulses handed to the listener */
+    ed[B2G_ED_OLD + B2G_ED_P0_NI] = vc->p[0].ni;
+    ed[B2G_ED_OLD + B2G_ED_P0_TI] = vc->p[0].ti;
+    ed[B2G_ED_OLD + B2G_ED_P1_NI] = vc->npts >= 2 ? vc->p[1].ni : 0.0;
+    ed[B2G_ED_OLD + B2G_ED_P1_TI] = vc->npts >= 2 ? vc->p[1].ti : 0.0;
+  }
+}
+
+static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py:65-179 */
+  double dt = st->dt;
+  PV* pv = is->pv;
+  for (int i = 0; i < is->nbodies; ++i) {
+    Body* b = &w->b[is->bodies[i]];
+    V2 c = b->c;
+    double a = b->a;
+    V2 v = b->v;
+    double wv = b->w;
+    b->c0 = c;
+    b->a0 = a;
+    if (b->type == B2G_DYNAMIC) {
+      v = add(v, scale(dt, add(scale(b->gscale, w->gravity), scale(b->inv_mass, b->force))));
+      wv += dt * b->invI * b->torque;
+      v = scale(pclamp(1.0 - dt * b->ldamp, 0.0, 1.0), v);
+      wv *= pclamp(1.0 - dt * b->adamp, 0.0, 1.0);
+    }
+    pv[i].c = c; pv[i].a = a; pv[i].v = v; pv[i].w = wv;
+  }
+  int nvc = is->ncontacts;
+  cs_build(w, is, st);
+  cs_init_velocity(w, is);
   /* warm_start (contact.py:498-525) */
   if (st->warm_starting) {
     for (int k = 0; k < nvc; ++k) {
@@ -1385,19 +1426,7 @@ static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py
     int n = c->m.pc < vc->npts ? c->m.pc : vc->npts;
     for (int j = 0; j < n; ++j) { c->m.pts[j].ni = vc->p[j].ni; c->m.pts[j].ti = vc->p[j].ti; }
   }
-  /* Island.report (island.py:245-252): post_solve with the velocity constraints' impulses */
-  for (int k = 0; k < nvc; ++k) {
-    const VC* vc = &is->vc[k];
-    int e = ev_push(w, B2G_EV_POST_SOLVE, 2, w->post_key + k, &w->c[vc->contact]);
-    if (e < 0) continue;
-    int32_t* ei = w->ev_i + (size_t)e * B2G_EV_NI;
-    double* ed = w->ev_d + (size_t)e * B2G_EV_ND;
-    ei[B2G_EI_OLD_PC] = vc->npts; /* the impulses handed to the listener */
-    ed[B2G_ED_OLD + B2G_ED_P0_NI] = vc->p[0].ni;
-    ed[B2G_ED_OLD + B2G_ED_P0_TI] = vc->p[0].ti;
-    ed[B2G_ED_OLD + B2G_ED_P1_NI] = vc->npts >= 2 ? vc->p[1].ni : 0.0;
-    ed[B2G_ED_OLD + B2G_ED_P1_TI] = vc->npts >= 2 ? vc->p[1].ti : 0.0;
-  }
+  cs_report(w, is, 2, w->post_key);
   w->post_key += nvc;
   /* integrate positions (island.py:120-138) */
   for (int i = 0; i < is->nbodies; ++i) {
@@ -1446,6 +1475,45 @@ static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py
     }
     if (min_sleep >= TIME_TO_SLEEP && position_solved && last_ns >= 0) set_awake(&w->b[last_ns], 0);
   }
+}
+
+/* Body._synchronize_fixtures (body.py:898-904): each fixture's swept AABB from the sweep's
+   start (c0, a0) to the current transform; a proxy that leaves its fat AABB is re-fattened and
+   appended to the move buffer (Fixture._synchronize, fixture.py:287-298; DynamicTree.move_proxy,
+   dynamictree.py:152-188).  Returns nonzero on allocation failure. */
+static int sync_body_fixtures(World* w, int bi) {
+  Body* b = &w->b[bi];
+  XF xf1;
+  xf1.R = rot(b->a0);
+  xf1.p = sub(b->c0, mulmv(xf1.R, b->lc));
+  for (int k = w->bfix_off[bi]; k < w->bfix_off[bi + 1]; ++k) {
+    int fi = w->bfix[k];
+    Fix* f = &w->f[fi];
+    AABB a1 = compute_aabb(&f->sh, xf1);
+    AABB a2 = compute_aabb(&f->sh, b->xf);
+    AABB aabb = aabb_union(a1, a2);
+    V2 disp = sub(b->xf.p, xf1.p);
+    if (aabb_contains(&f->fat, &aabb)) continue;
+    AABB nb_;
+    V2 r = v2(AABB_EXTENSION, AABB_EXTENSION);
+    nb_.lo = sub(aabb.lo, r);
+    nb_.hi = add(aabb.hi, r);
+    V2 d = scale(AABB_MULTIPLIER, disp);
+    if (d.x < 0.0) nb_.lo = add(nb_.lo, v2(d.x, 0.0));
+    else nb_.hi = add(nb_.hi, v2(d.x, 0.0));
+    if (d.y < 0.0) nb_.lo = add(nb_.lo, v2(0.0, d.y));
+    else nb_.hi = add(nb_.hi, v2(0.0, d.y));
+    f->fat = nb_;
+    if (w->nmove == w->mcap) {
+      int cap = w->mcap ? 2 * w->mcap : 16;
+      int* nm = (int*)realloc(w->move, sizeof(int) * (size_t)cap);
+      if (!nm) { w->err |= ORC_E_ALLOC; return -1; }
+      w->move = nm;
+      w->mcap = cap;
+    }
+    w->move[w->nmove++] = fi;
+  }
+  return 0;
 }
 
 static void solve(World* w, const Step* st) { /* world.py:475-592 */
@@ -1523,42 +1591,545 @@ static void solve(World* w, const Step* st) { /* world.py:475-592 */
   for (int bi = 0; bi < nb; ++bi) {
     Body* b = &w->b[bi];
     if (!b->island_flag || b->type == B2G_STATIC) continue;
-    XF xf1;
-    xf1.R = rot(b->a0);
-    xf1.p = sub(b->c0, mulmv(xf1.R, b->lc));
-    for (int k = w->bfix_off[bi]; k < w->bfix_off[bi + 1]; ++k) {
-      int fi = w->bfix[k];
-      Fix* f = &w->f[fi];
-      /* Fixture._synchronize (fixture.py:287-298) */
-      AABB a1 = compute_aabb(&f->sh, xf1);
-      AABB a2 = compute_aabb(&f->sh, b->xf);
-      AABB aabb = aabb_union(a1, a2);
-      V2 disp = sub(b->xf.p, xf1.p);
-      /* DynamicTree.move_proxy (dynamictree.py:152-188) */
-      if (aabb_contains(&f->fat, &aabb)) continue;
-      AABB nb_;
-      V2 r = v2(AABB_EXTENSION, AABB_EXTENSION);
-      nb_.lo = sub(aabb.lo, r);
-      nb_.hi = add(aabb.hi, r);
-      V2 d = scale(AABB_MULTIPLIER, disp);
-      if (d.x < 0.0) nb_.lo = add(nb_.lo, v2(d.x, 0.0));
-      else nb_.hi = add(nb_.hi, v2(d.x, 0.0));
-      if (d.y < 0.0) nb_.lo = add(nb_.lo, v2(0.0, d.y));
-      else nb_.hi = add(nb_.hi, v2(0.0, d.y));
-      f->fat = nb_;
-      if (w->nmove == w->mcap) {
-        int cap = w->mcap ? 2 * w->mcap : 16;
-        int* nm = (int*)realloc(w->move, sizeof(int) * (size_t)cap);
-        if (!nm) { w->err |= ORC_E_ALLOC; goto done; }
-        w->move = nm;
-        w->mcap = cap;
-      }
-      w->move[w->nmove++] = fi;
-    }
+    if (sync_body_fixtures(w, bi)) goto done;
   }
   find_new_contacts(w);
 done:
   free(off); free(adj); free(fill); free(stack);
+  free(is.bodies); free(is.contacts); free(is.pv); free(is.vc); free(is.pc);
+}
+
+
+/* ------------------------------------------------------------------ */
+/* Continuous physics: GJK distance (distance.py), time of impact      */
+/* (toi.py), Island.solve_toi (island.py:181-243), World.solve_toi     */
+/* (world.py:594-821).                                                 */
+/* ------------------------------------------------------------------ */
+static const int DISTANCE_MAX_ITERS = 20;  /* settings.py:129 */
+static const int TOI_MAX_ITERS = 20;       /* settings.py:82 */
+static const int MAX_SUB_STEPS = 8;        /* settings.py:73 */
+static const int MAX_TOI_CONTACTS = 32;    /* settings.py:79 */
+static const double TOI_BAUMGARTE = 0.75;  /* settings.py:110 */
+/* The reference's inner TOI loop has no bound on its push-back iterations (toi.py:335-339
+   leaves the check commented out); the oracle stops at this many and flags the world. */
+static const int TOI_PUSH_BACK_LIMIT = 1000;
+#define ORC_E_TOI 8
+
+typedef struct { V2 v[B2G_MAX_POLY_VERTS]; int n; double radius; } DProxy;
+
+/* DistanceProxy (distance.py:355-400): circles give their centre, polygons their vertices,
+   edges their two vertices */
+static void dproxy_init(DProxy* p, const Shape* s) {
+  p->radius = s->radius;
+  if (s->type == B2G_SHAPE_CIRCLE) { p->v[0] = s->p; p->n = 1; }
+  else if (s->type == B2G_SHAPE_POLYGON) { p->n = s->n; for (int i = 0; i < s->n; ++i) p->v[i] = s->v[i]; }
+  else { p->v[0] = s->e1; p->v[1] = s->e2; p->n = 2; }
+}
+
+/* get_support: the first vertex of largest dot (distance.py:402-405) */
+static int dproxy_support(const DProxy* p, V2 d) {
+  int best = 0;
+  double bv = dot(p->v[0], d);
+  for (int i = 1; i < p->n; ++i) {
+    double x = dot(p->v[i], d);
+    if (x > bv) { bv = x; best = i; }
+  }
+  return best;
+}
+
+typedef struct { V2 wa, wb, w; double a; int ia, ib; } SVert;
+typedef struct { SVert v[3]; int count; } Simplex;
+typedef struct { double metric; int count; int ia[3], ib[3]; } SCache;
+
+static double simplex_metric(const Simplex* s) { /* distance.py:180-193 */
+  if (s->count == 1) return 0.0;
+  if (s->count == 2) return length_(sub(s->v[0].w, s->v[1].w));
+  return cross(sub(s->v[1].w, s->v[0].w), sub(s->v[2].w, s->v[0].w));
+}
+
+static void simplex_vertex(SVert* v, const DProxy* pa, XF xfa, const DProxy* pb, XF xfb) {
+  v->wa = xmul(xfa, pa->v[v->ia]);
+  v->wb = xmul(xfb, pb->v[v->ib]);
+  v->w = sub(v->wb, v->wa);
+}
+
+static void simplex_read(Simplex* s, const SCache* c, const DProxy* pa, XF xfa, const DProxy* pb, XF xfb) {
+  s->count = c->count; /* distance.py:88-120 */
+  for (int i = 0; i < s->count; ++i) {
+    SVert* v = &s->v[i];
+    v->ia = c->ia[i];
+    v->ib = c->ib[i];
+    simplex_vertex(v, pa, xfa, pb, xfb);
+    v->a = 0.0;
+  }
+  if (s->count > 1) {
+    double m1 = c->metric, m2 = simplex_metric(s);
+    if (m2 < 0.5 * m1 || 2.0 * m1 < m2 || m2 < EPSILON) s->count = 0;
+  }
+  if (s->count == 0) {
+    s->v[0].ia = 0;
+    s->v[0].ib = 0;
+    simplex_vertex(&s->v[0], pa, xfa, pb, xfb);
+    s->count = 1;
+  }
+}
+
+static V2 simplex_closest(const Simplex* s) { /* distance.py:128-138 */
+  if (s->count == 1) return s->v[0].w;
+  if (s->count == 2) return add(scale(s->v[0].a, s->v[0].w), scale(s->v[1].a, s->v[1].w));
+  return v2(0.0, 0.0);
+}
+
+static V2 simplex_direction(const Simplex* s) { /* distance.py:140-154 */
+  if (s->count == 1) return neg(s->v[0].w);
+  V2 e12 = sub(s->v[1].w, s->v[0].w);
+  double sgn = cross(e12, neg(s->v[0].w));
+  return sgn > 0.0 ? cross_sv(1.0, e12) : cross_vs(e12, 1.0);
+}
+
+static void simplex_witness(const Simplex* s, V2* pa, V2* pb) { /* distance.py:156-178 */
+  if (s->count == 1) { *pa = s->v[0].wa; *pb = s->v[0].wb; return; }
+  if (s->count == 2) {
+    *pa = add(scale(s->v[0].a, s->v[0].wa), scale(s->v[1].a, s->v[1].wa));
+    *pb = add(scale(s->v[0].a, s->v[0].wb), scale(s->v[1].a, s->v[1].wb));
+    return;
+  }
+  V2 p = add(add(scale(s->v[0].a, s->v[0].wa), scale(s->v[1].a, s->v[1].wa)), scale(s->v[2].a, s->v[2].wa));
+  *pa = p;
+  *pb = p;
+}
+
+static void simplex_solve2(Simplex* s) { /* distance.py:195-245 */
+  V2 w1 = s->v[0].w, w2 = s->v[1].w, e12 = sub(w2, w1);
+  double d12_2 = -dot(w1, e12);
+  if (d12_2 <= 0.0) { s->v[0].a = 1.0; s->count = 1; return; }
+  double d12_1 = dot(w2, e12);
+  if (d12_1 <= 0.0) { s->v[1].a = 1.0; s->count = 1; s->v[0] = s->v[1]; return; }
+  double inv = 1.0 / (d12_1 + d12_2);
+  s->v[0].a = d12_1 * inv;
+  s->v[1].a = d12_2 * inv;
+  s->count = 2;
+}
+
+static void simplex_solve3(Simplex* s) { /* distance.py:247-353 */
+  V2 w1 = s->v[0].w, w2 = s->v[1].w, w3 = s->v[2].w;
+  V2 e12 = sub(w2, w1);
+  double d12_1 = dot(w2, e12), d12_2 = -dot(w1, e12);
+  V2 e13 = sub(w3, w1);
+  double d13_1 = dot(w3, e13), d13_2 = -dot(w1, e13);
+  V2 e23 = sub(w3, w2);
+  double d23_1 = dot(w3, e23), d23_2 = -dot(w2, e23);
+  double n123 = cross(e12, e13);
+  double d123_1 = n123 * cross(w2, w3), d123_2 = n123 * cross(w3, w1), d123_3 = n123 * cross(w1, w2);
+  if (d12_2 <= 0.0 && d13_2 <= 0.0) { s->v[0].a = 1.0; s->count = 1; return; }
+  if (d12_1 > 0.0 && d12_2 > 0.0 && d123_3 <= 0.0) {
+    double inv = 1.0 / (d12_1 + d12_2);
+    s->v[0].a = d12_1 * inv; s->v[1].a = d12_2 * inv; s->count = 2;
+    return;
+  }
+  if (d13_1 > 0.0 && d13_2 > 0.0 && d123_2 <= 0.0) {
+    double inv = 1.0 / (d13_1 + d13_2);
+    s->v[0].a = d13_1 * inv; s->v[2].a = d13_2 * inv; s->count = 2;
+    s->v[1] = s->v[2];
+    return;
+  }
+  if (d12_1 <= 0.0 && d23_2 <= 0.0) { s->v[1].a = 1.0; s->count = 1; s->v[0] = s->v[1]; return; }
+  if (d13_1 <= 0.0 && d23_1 <= 0.0) { s->v[2].a = 1.0; s->count = 1; s->v[0] = s->v[2]; return; }
+  if (d23_1 > 0.0 && d23_2 > 0.0 && d123_1 <= 0.0) {
+    double inv = 1.0 / (d23_1 + d23_2);
+    s->v[1].a = d23_1 * inv; s->v[2].a = d23_2 * inv; s->count = 2;
+    s->v[0] = s->v[2];
+    return;
+  }
+  double inv = 1.0 / (d123_1 + d123_2 + d123_3);
+  s->v[0].a = d123_1 * inv; s->v[1].a = d123_2 * inv; s->v[2].a = d123_3 * inv;
+  s->count = 3;
+}
+
+/* shape_distance (distance.py:424-561): GJK; the simplex cache is read and written.
+   Returns the distance (without the radii: use_radii False). */
+static double gjk_distance(const DProxy* pa, const DProxy* pb, XF xfa, XF xfb, SCache* cache) {
+  Simplex s;
+  memset(&s, 0, sizeof(s));
+  simplex_read(&s, cache, pa, xfa, pb, xfb);
+  int iter = 0;
+  while (iter < DISTANCE_MAX_ITERS) {
+    int save_n = s.count, save_a[3], save_b[3];
+    for (int i = 0; i < s.count; ++i) { save_a[i] = s.v[i].ia; save_b[i] = s.v[i].ib; }
+    if (s.count == 2) simplex_solve2(&s);
+    else if (s.count == 3) { simplex_solve3(&s); break; }
+    V2 d = simplex_direction(&s);
+    if (length_sq(d) < EPSILON_SQR) break;
+    SVert* v = &s.v[s.count];
+    v->ia = dproxy_support(pa, mulmtv(xfa.R, neg(d)));
+    v->wa = xmul(xfa, pa->v[v->ia]);
+    v->ib = dproxy_support(pb, mulmtv(xfb.R, d));
+    v->wb = xmul(xfb, pb->v[v->ib]);
+    v->w = sub(v->wb, v->wa);
+    ++iter;
+    int dup = 0;
+    for (int i = 0; i < save_n; ++i) if (save_a[i] == v->ia && save_b[i] == v->ib) dup = 1;
+    if (dup) break;
+    ++s.count;
+  }
+  V2 wa, wb;
+  simplex_witness(&s, &wa, &wb);
+  double dist = length_(sub(wa, wb));
+  cache->metric = simplex_metric(&s);
+  cache->count = s.count;
+  for (int i = 0; i < s.count; ++i) { cache->ia[i] = s.v[i].ia; cache->ib[i] = s.v[i].ib; }
+  return dist;
+}
+
+typedef struct { V2 c0, c, lc; double a0, a, alpha0; } SweepT;
+
+static XF sweep_xf(const SweepT* sw, double beta) { /* Sweep.get_transform (common.py:895-906) */
+  V2 p = add(scale(1.0 - beta, sw->c0), scale(beta, sw->c));
+  double ang = (1.0 - beta) * sw->a0 + beta * sw->a;
+  return xf_from(p, ang, sw->lc);
+}
+
+static void sweep_advance(SweepT* sw, double alpha) { /* Sweep.advance (common.py:908-917) */
+  double beta = (alpha - sw->alpha0) / (1.0 - sw->alpha0);
+  sw->c0 = add(scale(1.0 - beta, sw->c0), scale(beta, sw->c));
+  sw->a0 = (1.0 - beta) * sw->a0 + beta * sw->a;
+  sw->alpha0 = alpha;
+}
+
+static SweepT body_sweep(const Body* b) {
+  SweepT s;
+  s.c0 = b->c0; s.c = b->c; s.lc = b->lc; s.a0 = b->a0; s.a = b->a; s.alpha0 = b->alpha0;
+  return s;
+}
+
+static void set_body_sweep(Body* b, const SweepT* s) {
+  b->c0 = s->c0; b->c = s->c; b->a0 = s->a0; b->a = s->a; b->alpha0 = s->alpha0;
+}
+
+enum { SEP_POINTS, SEP_FACE_A, SEP_FACE_B };
+typedef struct { const DProxy *pa, *pb; SweepT sa, sb; int type; V2 axis, lp; } SepFn;
+
+static void sep_init(SepFn* f, const SCache* cache, double t1) { /* toi.py:54-122 */
+  XF xfa = sweep_xf(&f->sa, t1), xfb = sweep_xf(&f->sb, t1);
+  int ia0 = cache->ia[0], ib0 = cache->ib[0];
+  if (cache->count == 1) {
+    f->type = SEP_POINTS;
+    /* the axis is left unnormalised: `s = self.axis.normalized` discards the unit vector */
+    f->axis = sub(xmul(xfb, f->pb->v[ib0]), xmul(xfa, f->pa->v[ia0]));
+    return;
+  }
+  int ia1 = cache->ia[1], ib1 = cache->ib[1];
+  if (ia0 == ia1) {
+    f->type = SEP_FACE_B;
+    V2 b1 = f->pb->v[ib0], b2 = f->pb->v[ib1];
+    f->axis = normalized(cross_vs(sub(b2, b1), 1.0));
+    V2 n = mulmv(xfb.R, f->axis);
+    f->lp = scale(0.5, add(b1, b2));
+    V2 pb = xmul(xfb, f->lp), pa = xmul(xfa, f->pa->v[ia0]);
+    if (dot(sub(pa, pb), n) < 0.0) f->axis = neg(f->axis);
+  } else {
+    f->type = SEP_FACE_A;
+    V2 a1 = f->pa->v[ia0], a2 = f->pa->v[ia1];
+    f->axis = normalized(cross_vs(sub(a2, a1), 1.0));
+    V2 n = mulmv(xfa.R, f->axis);
+    f->lp = scale(0.5, add(a1, a2));
+    V2 pa = xmul(xfa, f->lp), pb = xmul(xfb, f->pb->v[ib0]);
+    if (dot(sub(pb, pa), n) < 0.0) f->axis = neg(f->axis);
+  }
+}
+
+/* find_min_separation (toi.py:124-175) and evaluate (177-213): the witness indices are
+   found (find) or given (evaluate) */
+static double sep_eval(const SepFn* f, int find, int* ia, int* ib, double t) {
+  XF xfa = sweep_xf(&f->sa, t), xfb = sweep_xf(&f->sb, t);
+  if (f->type == SEP_POINTS) {
+    if (find) {
+      *ia = dproxy_support(f->pa, mulmtv(xfa.R, f->axis));
+      *ib = dproxy_support(f->pb, mulmtv(xfb.R, neg(f->axis)));
+    }
+    V2 pa = xmul(xfa, f->pa->v[*ia]), pb = xmul(xfb, f->pb->v[*ib]);
+    return dot(sub(pb, pa), f->axis);
+  }
+  if (f->type == SEP_FACE_A) {
+    V2 n = mulmv(xfa.R, f->axis);
+    V2 pa = xmul(xfa, f->lp);
+    if (find) { *ia = -1; *ib = dproxy_support(f->pb, mulmtv(xfb.R, neg(n))); }
+    V2 pb = xmul(xfb, f->pb->v[*ib]);
+    return dot(sub(pb, pa), n);
+  }
+  V2 n = mulmv(xfb.R, f->axis);
+  V2 pb = xmul(xfb, f->lp);
+  if (find) { *ib = -1; *ia = dproxy_support(f->pa, mulmtv(xfa.R, neg(n))); }
+  V2 pa = xmul(xfa, f->pa->v[*ia]);
+  return dot(sub(pa, pb), n);
+}
+
+enum { TOI_UNKNOWN, TOI_FAILED, TOI_OVERLAPPED, TOI_TOUCHING, TOI_SEPARATED };
+
+/* time_of_impact (toi.py:215-391) over [0, 1]; returns the state, *t_out the time */
+static int time_of_impact(World* w, const DProxy* pa, const DProxy* pb, const SweepT* swa, const SweepT* swb,
+                          double* t_out) {
+  const double t_max = 1.0;
+  w->toi_calls++;
+  SepFn f;
+  f.pa = pa; f.pb = pb;
+  f.sa = *swa; f.sb = *swb;
+  /* Sweep.normalize (common.py:919-923) on the copies */
+  double d = 2.0 * PI_ * floor(f.sa.a0 / (2.0 * PI_));
+  f.sa.a0 -= d; f.sa.a -= d;
+  d = 2.0 * PI_ * floor(f.sb.a0 / (2.0 * PI_));
+  f.sb.a0 -= d; f.sb.a -= d;
+  double total_radius = pa->radius + pb->radius;
+  double target = pmax(LINEAR_SLOP, total_radius - 3.0 * LINEAR_SLOP);
+  double tolerance = 0.25 * LINEAR_SLOP;
+  double t1 = 0.0, out_t = t_max;
+  int iter = 0, state = TOI_UNKNOWN;
+  SCache cache;
+  memset(&cache, 0, sizeof(cache));
+  for (;;) {
+    XF xfa = sweep_xf(&f.sa, t1), xfb = sweep_xf(&f.sb, t1);
+    double dist = gjk_distance(pa, pb, xfa, xfb, &cache);
+    if (dist <= 0.0) { state = TOI_OVERLAPPED; out_t = 0.0; break; }
+    if (dist < target + tolerance) { state = TOI_TOUCHING; out_t = t1; break; }
+    sep_init(&f, &cache, t1);
+    int done = 0, push_back = 0;
+    double t2 = t_max;
+    for (;;) {
+      int ia = 0, ib = 0;
+      double s2 = sep_eval(&f, 1, &ia, &ib, t2);
+      if (s2 > target + tolerance) { state = TOI_SEPARATED; out_t = t_max; done = 1; break; }
+      if (s2 > target - tolerance) { t1 = t2; break; }
+      double s1 = sep_eval(&f, 0, &ia, &ib, t1);
+      if (s1 < target - tolerance) { state = TOI_FAILED; out_t = t1; done = 1; break; }
+      if (s1 <= target + tolerance) { state = TOI_TOUCHING; out_t = t1; done = 1; break; }
+      int rc = 0;
+      double a1 = t1, a2 = t2;
+      for (;;) {
+        double t = (rc & 1) ? a1 + (target - s1) * (a2 - a1) / (s2 - s1) : 0.5 * (a1 + a2);
+        double sv = sep_eval(&f, 0, &ia, &ib, t);
+        if (fabs(sv - target) < tolerance) { t2 = t; break; }
+        if (sv > target) { a1 = t; s1 = sv; }
+        else { a2 = t; s2 = sv; }
+        ++rc;
+        if (rc == 50) break;
+      }
+      if (++push_back == TOI_PUSH_BACK_LIMIT) { w->err |= ORC_E_TOI; state = TOI_FAILED; out_t = t1; done = 1; break; }
+    }
+    ++iter;
+    if (done) break;
+    if (iter == TOI_MAX_ITERS) { state = TOI_FAILED; out_t = t1; break; }
+  }
+  *t_out = out_t;
+  return state;
+}
+
+/* Body._advance (body.py:930-937) */
+static void body_advance(Body* b, double alpha) {
+  SweepT sw = body_sweep(b);
+  sweep_advance(&sw, alpha);
+  set_body_sweep(b, &sw);
+  b->c = b->c0;
+  b->a = b->a0;
+  b->xf = xf_from(b->c, b->a, b->lc);
+}
+
+/* ContactSolver.solve_toi_position_constraints (contact.py:866-933): only the two TOI bodies
+   move */
+static int solve_toi_position(PC* pcs, int npc, PV* pv, int toi_a, int toi_b) {
+  double min_sep = 0.0;
+  for (int k = 0; k < npc; ++k) {
+    PC* pc = &pcs[k];
+    V2 ca = pv[pc->ia].c, cb = pv[pc->ib].c;
+    double aa = pv[pc->ia].a, ab = pv[pc->ib].a;
+    int ma = pc->ia == toi_a || pc->ia == toi_b, mb = pc->ib == toi_a || pc->ib == toi_b;
+    double ima = ma ? pc->ma : 0.0, iIa = ma ? pc->Ia : 0.0;
+    double imb = mb ? pc->mb : 0.0, iIb = mb ? pc->Ib : 0.0;
+    for (int j = 0; j < pc->npts; ++j) {
+      XF xfa = xf_from(ca, aa, pc->lca), xfb = xf_from(cb, ab, pc->lcb);
+      V2 normal, point;
+      double sep;
+      solver_manifold(pc, xfa, xfb, j, &normal, &point, &sep);
+      V2 ra = sub(point, ca), rb = sub(point, cb);
+      min_sep = pmin(min_sep, sep);
+      double C = pclamp(TOI_BAUMGARTE * (sep + LINEAR_SLOP), -MAX_LINEAR_CORRECTION, 0.0);
+      double rna = pysq(cross(ra, normal)), rnb = pysq(cross(rb, normal));
+      double K = ima + imb + iIa * rna + iIb * rnb;
+      double impulse = K > 0.0 ? -C / K : 0.0;
+      V2 P = scale(impulse, normal);
+      ca = sub(ca, scale(ima, P));
+      aa -= iIa * cross(ra, P);
+      cb = add(cb, scale(imb, P));
+      ab += iIb * cross(rb, P);
+    }
+    pv[pc->ia].c = ca; pv[pc->ia].a = aa;
+    pv[pc->ib].c = cb; pv[pc->ib].a = ab;
+  }
+  return min_sep >= -1.5 * LINEAR_SLOP;
+}
+
+/* Island.solve_toi (island.py:181-243) */
+static void island_solve_toi(World* w, IslandBuf* is, const Step* st, int toi_a, int toi_b) {
+  PV* pv = is->pv;
+  for (int i = 0; i < is->nbodies; ++i) {
+    Body* b = &w->b[is->bodies[i]];
+    pv[i].c = b->c; pv[i].a = b->a; pv[i].v = b->v; pv[i].w = b->w;
+  }
+  cs_build(w, is, st);
+  for (int it = 0; it < st->pos_iters; ++it)
+    if (solve_toi_position(is->pc, is->ncontacts, pv, toi_a, toi_b)) break;
+  Body *A = &w->b[is->bodies[toi_a]], *B = &w->b[is->bodies[toi_b]];
+  A->c0 = pv[toi_a].c; A->a0 = pv[toi_a].a;
+  B->c0 = pv[toi_b].c; B->a0 = pv[toi_b].a;
+  cs_init_velocity(w, is);
+  for (int it = 0; it < st->vel_iters; ++it) solve_velocity(is->vc, is->ncontacts, pv);
+  double dt = st->dt;
+  for (int i = 0; i < is->nbodies; ++i) {
+    Body* b = &w->b[is->bodies[i]];
+    V2 v = pv[i].v;
+    double wv = pv[i].w;
+    V2 tr = scale(dt, v);
+    if (length_sq(tr) > MAX_TRANSLATION_SQR) v = scale(MAX_TRANSLATION / length_(tr), v);
+    double rotation = dt * wv;
+    if (pysq(rotation) > MAX_ROTATION_SQR) wv *= MAX_ROTATION / fabs(rotation);
+    pv[i].c = add(pv[i].c, scale(dt, v));
+    pv[i].a += dt * wv;
+    b->c = pv[i].c; b->a = pv[i].a; b->v = v; b->w = wv;
+    b->xf = xf_from(b->c, b->a, b->lc);
+  }
+  cs_report(w, is, 3, 0);
+}
+
+/* World.solve_toi (world.py:594-821) */
+static void solve_toi(World* w, const Step* st) {
+  IslandBuf is;
+  memset(&is, 0, sizeof(is));
+  is.bodies = (int*)malloc(sizeof(int) * 2 * (size_t)MAX_TOI_CONTACTS);
+  is.contacts = (int*)malloc(sizeof(int) * (size_t)MAX_TOI_CONTACTS);
+  is.pv = (PV*)malloc(sizeof(PV) * 2 * (size_t)MAX_TOI_CONTACTS);
+  is.vc = (VC*)malloc(sizeof(VC) * (size_t)MAX_TOI_CONTACTS);
+  is.pc = (PC*)malloc(sizeof(PC) * (size_t)MAX_TOI_CONTACTS);
+  if (!is.bodies || !is.contacts || !is.pv || !is.vc || !is.pc) { w->err |= ORC_E_ALLOC; goto out; }
+  w->toi_mode = 1;
+  for (int i = 0; i < w->nb; ++i) { w->b[i].toi_flag = 0; w->b[i].alpha0 = 0.0; }
+  for (int i = 0; i < w->nc; ++i) {
+    Contact* c = &w->c[i];
+    c->toi_flag = 0; c->toi_island = 0; c->toi_count = 0; c->toi = 1.0;
+  }
+  for (;;) {
+    int min_c = -1;
+    double min_alpha = 1.0;
+    for (int i = 0; i < w->nc; ++i) {
+      Contact* c = &w->c[i];
+      if (!c->enabled || c->toi_count > MAX_SUB_STEPS) continue;
+      double alpha = 1.0;
+      if (c->toi_flag) {
+        alpha = c->toi;
+      } else {
+        Fix *FA = &w->f[c->fa], *FB = &w->f[c->fb];
+        if (FA->sensor || FB->sensor) continue;
+        Body *A = &w->b[FA->body], *B = &w->b[FB->body];
+        int active_a = A->awake && A->type != B2G_STATIC, active_b = B->awake && B->type != B2G_STATIC;
+        if (!active_a && !active_b) continue;
+        int collide_a = A->bullet || A->type != B2G_DYNAMIC, collide_b = B->bullet || B->type != B2G_DYNAMIC;
+        if (!collide_a && !collide_b) continue;
+        double alpha0 = A->alpha0;
+        if (A->alpha0 < B->alpha0) {
+          alpha0 = B->alpha0;
+          SweepT sw = body_sweep(A); sweep_advance(&sw, alpha0); set_body_sweep(A, &sw);
+        } else if (B->alpha0 < A->alpha0) {
+          SweepT sw = body_sweep(B); sweep_advance(&sw, alpha0); set_body_sweep(B, &sw);
+        }
+        DProxy pa, pb;
+        dproxy_init(&pa, &FA->sh);
+        dproxy_init(&pb, &FB->sh);
+        SweepT sa = body_sweep(A), sb = body_sweep(B);
+        double beta;
+        int state = time_of_impact(w, &pa, &pb, &sa, &sb, &beta);
+        alpha = state == TOI_TOUCHING ? pmin(alpha0 + (1.0 - alpha0) * beta, 1.0) : 1.0;
+        c->toi = alpha;
+        c->toi_flag = 1;
+      }
+      if (alpha < min_alpha) { min_c = i; min_alpha = alpha; }
+    }
+    if (min_c < 0 || (1.0 - 10.0 * EPSILON) < min_alpha) break;
+    Contact* mc = &w->c[min_c];
+    int ba = w->f[mc->fa].body, bb = w->f[mc->fb].body;
+    Body *A = &w->b[ba], *B = &w->b[bb];
+    SweepT backup_a = body_sweep(A), backup_b = body_sweep(B);
+    body_advance(A, min_alpha);
+    body_advance(B, min_alpha);
+    contact_update(w, mc, min_c);
+    mc->toi_flag = 0;
+    mc->toi_count += 1;
+    if (!mc->enabled || !mc->touching) {
+      mc->enabled = 0;
+      set_body_sweep(A, &backup_a); A->xf = xf_from(A->c, A->a, A->lc);
+      set_body_sweep(B, &backup_b); B->xf = xf_from(B->c, B->a, B->lc);
+      continue;
+    }
+    set_awake(A, 1);
+    set_awake(B, 1);
+    is.nbodies = 0;
+    is.ncontacts = 0;
+    A->island_index = is.nbodies; is.bodies[is.nbodies++] = ba;
+    B->island_index = is.nbodies; is.bodies[is.nbodies++] = bb;
+    is.contacts[is.ncontacts++] = min_c;
+    A->toi_flag = 1; B->toi_flag = 1; mc->toi_island = 1;
+    for (int side = 0; side < 2; ++side) {
+      int bi = side == 0 ? ba : bb;
+      Body* body = &w->b[bi];
+      if (body->type != B2G_DYNAMIC) continue;
+      for (int k = 0; k < w->nc; ++k) { /* body._contacts, in creation order */
+        Contact* c = &w->c[k];
+        if (w->f[c->fa].body != bi && w->f[c->fb].body != bi) continue;
+        if (is.nbodies == 2 * MAX_TOI_CONTACTS) break; /* island body capacity */
+        if (is.ncontacts == MAX_TOI_CONTACTS) break;   /* island contact capacity */
+        if (c->toi_island) continue;
+        int oi = other_body(w, c, bi);
+        Body* other = &w->b[oi];
+        if (other->type == B2G_DYNAMIC && !body->bullet && !other->bullet) continue;
+        if (w->f[c->fa].sensor || w->f[c->fb].sensor) continue;
+        SweepT backup = body_sweep(other);
+        if (!other->toi_flag) body_advance(other, min_alpha);
+        contact_update(w, c, k);
+        if (!c->enabled || !c->touching) {
+          set_body_sweep(other, &backup);
+          other->xf = xf_from(other->c, other->a, other->lc);
+          continue;
+        }
+        c->toi_island = 1;
+        is.contacts[is.ncontacts++] = k;
+        if (other->toi_flag) continue;
+        other->toi_flag = 1;
+        if (other->type != B2G_STATIC) set_awake(other, 1);
+        other->island_index = is.nbodies;
+        is.bodies[is.nbodies++] = oi;
+      }
+    }
+    Step sub;
+    sub.dt = (1.0 - min_alpha) * st->dt;
+    sub.inv_dt = 1.0 / sub.dt;
+    sub.dt_ratio = 1.0;
+    sub.pos_iters = 20;
+    sub.vel_iters = st->vel_iters;
+    sub.warm_starting = 0;
+    w->toi_events++;
+    island_solve_toi(w, &is, &sub, A->island_index, B->island_index);
+    for (int i = 0; i < is.nbodies; ++i) {
+      int bi = is.bodies[i];
+      Body* b = &w->b[bi];
+      b->toi_flag = 0;
+      if (b->type != B2G_DYNAMIC) continue;
+      if (sync_body_fixtures(w, bi)) goto out;
+      for (int k = 0; k < w->nc; ++k) {
+        Contact* c = &w->c[k];
+        if (w->f[c->fa].body == bi || w->f[c->fb].body == bi) { c->toi_flag = 0; c->toi_island = 0; }
+      }
+    }
+    find_new_contacts(w); /* new contacts start disabled (Contact.__init__): no TOI until updated */
+  }
+out:
+  w->toi_mode = 0;
   free(is.bodies); free(is.contacts); free(is.pv); free(is.vc); free(is.pc);
 }
 
@@ -1588,6 +2159,9 @@ static void world_step_end(World* w) {
   st.pos_iters = w->split_pi;
   st.warm_starting = w->warm_starting;
   if (st.dt > 0.0) solve(w, &st);
+  w->toi_calls = 0;
+  w->toi_events = 0;
+  if (w->continuous && st.dt > 0.0) solve_toi(w, &st);
   if (st.dt > 0.0) w->inv_dt0 = st.inv_dt;
   if (!w->keep_forces)
     for (int i = 0; i < w->nb; ++i) { w->b[i].force = v2(0.0, 0.0); w->b[i].torque = 0.0; }
@@ -1670,6 +2244,7 @@ static int world_load(World* w, const b2g_batch* d, int wi) {
   w->warm_starting = (wf & B2G_WF_WARM_STARTING) != 0;
   w->new_fixture = (wf & B2G_WF_NEW_FIXTURE) != 0;
   w->keep_forces = (wf & 8) != 0;
+  w->continuous = (wf & B2G_WF_CONTINUOUS) != 0;
   w->nb = b1 - b0;
   w->nf = f1 - f0;
   w->nc = c1 - c0;
@@ -1829,7 +2404,8 @@ ORC_EXPORT int orc_save(void* h, b2g_batch_out* o) {
     int32_t* wfl = o->world_i + (size_t)wi * B2G_WORLD_NI;
     wd[B2G_WD_GX] = w->gravity.x; wd[B2G_WD_GY] = w->gravity.y; wd[B2G_WD_INV_DT0] = w->inv_dt0;
     wfl[B2G_WI_FLAGS] = (w->allow_sleep ? B2G_WF_ALLOW_SLEEP : 0) | (w->warm_starting ? B2G_WF_WARM_STARTING : 0) |
-                        (w->new_fixture ? B2G_WF_NEW_FIXTURE : 0) | (w->keep_forces ? 8 : 0);
+                        (w->new_fixture ? B2G_WF_NEW_FIXTURE : 0) | (w->keep_forces ? 8 : 0) |
+                        (w->continuous ? B2G_WF_CONTINUOUS : 0);
     for (int i = 0; i < w->nb; ++i) {
       double* bd = o->body_d + (size_t)(bo + i) * B2G_BODY_ND;
       int32_t* bi = o->body_i + (size_t)(bo + i) * B2G_BODY_NI;
@@ -1922,6 +2498,16 @@ ORC_EXPORT int orc_step_end(void* h, int wi) {
   if (wi < 0 || wi >= B->nw || !B->w[wi].split) return -1;
   world_step_end(&B->w[wi]);
   return B->w[wi].err;
+}
+
+/* World wi's continuous-physics counts of its last step: time_of_impact calls and TOI events
+   (sub-step island solves). */
+ORC_EXPORT int orc_toi_stats(void* h, int wi, int64_t* out) {
+  Batch* B = (Batch*)h;
+  if (wi < 0 || wi >= B->nw) return -1;
+  out[0] = B->w[wi].toi_calls;
+  out[1] = B->w[wi].toi_events;
+  return 0;
 }
 
 /* The contact listener kinds to record (1 << B2G_EV_*), for every world. */

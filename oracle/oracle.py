@@ -55,12 +55,13 @@ def lib():
         l.orc_step_begin.argtypes = [vp, ctypes.c_int, f64, ctypes.c_int, ctypes.c_int]
         l.orc_disable.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         l.orc_step_end.argtypes = [vp, ctypes.c_int]
+        l.orc_toi_stats.argtypes = [vp, ctypes.c_int, vp]
         _ = i32
         _lib = l
     return _lib
 
 
-ORC_E_UNSUPPORTED, ORC_E_HINT, ORC_E_ALLOC = 1, 2, 4
+ORC_E_UNSUPPORTED, ORC_E_HINT, ORC_E_ALLOC, ORC_E_TOI = 1, 2, 4, 8
 
 
 class Oracle:
@@ -108,6 +109,13 @@ class Oracle:
         if rc < 0:
             raise ValueError("orc_step_end: no step begun")
         return rc
+
+    def toi_stats(self, w: int = 0):
+        """(time_of_impact calls, TOI events) of world w's last step (continuous physics)."""
+        out = np.zeros(2, np.int64)
+        if lib().orc_toi_stats(self.h, w, out.ctypes.data):
+            raise ValueError("bad world")
+        return int(out[0]), int(out[1])
 
     def set_events(self, mask: int) -> None:
         """Records the listener calls of the kinds in mask (1 << layout.EV_*), for every world."""

@@ -173,11 +173,11 @@ class Engine:
 
     def stats(self, reset: bool = False) -> dict:
         """Work counters summed over the steps since the last reset (b2g_stats)."""
-        v = np.zeros(7, np.int64)
-        self._check(self.lib.b2g_stats(self.h, v.ctypes.data, 7, 1 if reset else 0), "stats")
+        v = np.zeros(9, np.int64)
+        self._check(self.lib.b2g_stats(self.h, v.ctypes.data, 9, 1 if reset else 0), "stats")
         return {"steps": int(v[0]), "live_contacts": int(v[1]), "evaluated_contacts": int(v[2]),
                 "new_contacts": int(v[3]), "moved_proxies": int(v[4]), "island_bodies": int(v[5]),
-                "island_contacts": int(v[6])}
+                "island_contacts": int(v[6]), "toi_calls": int(v[7]), "toi_events": int(v[8])}
 
     def query_aabb(self, worlds, boxes) -> np.ndarray:
         """World.query_aabb for many boxes: rows (query, world-local fixture) of every fixture whose fat

@@ -26,14 +26,14 @@ def make_shape(spec):
 def build_world(scene, precision: str = "fp64", device: int = 0) -> World:
     """Builds the scene through the public API, in the spec's body order."""
     world = World(gravity=scene["gravity"], do_sleep=scene["do_sleep"], warm_starting=scene["warm_starting"],
-                  continuous_physics=False, precision=precision, device=device)
+                  continuous_physics=scene.get("continuous", False), precision=precision, device=device)
     for bs in scene["bodies"]:
         fixtures = [Fixture(make_shape(f["shape"]), density=f["density"], friction=f["friction"],
                             restitution=f["restitution"], category_bits=fixture_filter(f)[0],
                             mask_bits=fixture_filter(f)[1], group_index=fixture_filter(f)[2])
                     for f in bs["fixtures"]]
         kw = dict(position=bs["position"], angle=bs["angle"], linear_velocity=bs["linear_velocity"],
-                  angular_velocity=bs["angular_velocity"], fixtures=fixtures)
+                  angular_velocity=bs["angular_velocity"], fixtures=fixtures, bullet=bs.get("bullet", False))
         if bs["type"] == "static":
             world.create_static_body(**kw)
         elif bs["type"] == "kinematic":

@@ -41,6 +41,7 @@
 #include "../../include/b2g.h"
 #include "b2g_collide.hpp"
 #include "b2g_raycast.hpp"
+#include "b2g_toi.hpp"
 
 namespace b2g {
 
@@ -60,7 +61,9 @@ enum : int {
 // body flag bits (B2G_BF_*)
 enum : int { BF_AWAKE = 1, BF_ALLOW_SLEEP = 2, BF_FIXED_ROT = 4, BF_ACTIVE = 8, BF_BULLET = 16 };
 enum : int { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
-enum : int { WF_ALLOW_SLEEP = 1, WF_WARM = 2, WF_NEW_FIXTURE = 4, WF_KEEP_FORCES = 8, WF_CUSTOM_FILTER = 16 };
+enum : int {
+  WF_ALLOW_SLEEP = 1, WF_WARM = 2, WF_NEW_FIXTURE = 4, WF_KEEP_FORCES = 8, WF_CUSTOM_FILTER = 16, WF_CONTINUOUS = 32
+};
 // the parts of a step one launch sequence runs: all of it; the new-fixture pair search alone
 // (b2g_pair_search); up to and with collide (b2g_step_collide); the rest (b2g_step_solve)
 enum : int { PART_ALL = 0, PART_PAIRS = 1, PART_COLLIDE = 2, PART_SOLVE = 3 };
@@ -71,10 +74,14 @@ enum : int {
   WERR_SCHEDULE = 8,
   WERR_NAN = 16,          // a body's position, angle or velocity became non-finite
   WERR_ISLAND_SIZE = 32,  // an island of more than 65535 contacts (16-bit island-local indices)
+  WERR_TOI = 64,          // a time-of-impact search that the reference would not end (toi.py:335-339)
 };
 
 // work counters (b2g_stats): summed over the steps since the last reset
-enum : int { S_LIVE = 0, S_EVAL = 1, S_NEW = 2, S_MOVED = 3, S_IBODIES = 4, S_ICONTACTS = 5, kStats = 8 };
+enum : int {
+  S_LIVE = 0, S_EVAL = 1, S_NEW = 2, S_MOVED = 3, S_IBODIES = 4, S_ICONTACTS = 5, S_TOI_CALLS = 6, S_TOI_EVENTS = 7,
+  kStats = 8
+};
 
 constexpr int kBlock = 256;
 constexpr int kBigPairsBlock = 1024;  // k_pairs workgroup for the largest worlds
@@ -222,6 +229,8 @@ struct Dev {
   double4* b_xf;
   double2* b_lc;
   double* b_sleep;
+  double* b_alpha0;  // continuous physics: Sweep.alpha0, and the TOI island index + 1 (0: not in it)
+  int* b_tflag;
   V2<double>* b_v;
   double* b_w;
   T* b_im;
@@ -251,6 +260,8 @@ struct Dev {
   V4<T>* c_pts;
   V4<double>* c_imp;  // accumulated impulses: FP64 in both pipelines (warm-start values)
   uint2* c_id;
+  double* c_toi;  // continuous physics: the cached time of impact, and toi flag | island flag << 1 |
+  int* c_tmeta;   // sub-step count << 8 (contact.py:64-81)
   // islands
   int* i_bodies;     // per-world segment at w_body_off[w]
   int* i_contacts;   // per-world segment at w_con_base[w]
@@ -291,6 +302,9 @@ struct Dev {
   int* g_lev;          // per body
   int* g_parent;       // per body: union-find
   int* g_seed;         // per body (by component root)
+  const int* toi_worlds;  // the worlds with continuous physics (k_toi's grid)
+  unsigned char* toi_scr; // per such world: the TOI island's scratch (ToiScratch)
+  int* toi_move;          // per world at w_fix_off[w]: the move buffer of a TOI event
   int* g_cntb;         // per body (by component root)
   int* g_cntc;         // per body (by component root)
   int* g_bidx;         // per body: island-local index
@@ -392,38 +406,29 @@ __device__ __forceinline__ void ev_write(const Dev<T>& d, int k, int w, int kind
 }
 
 // ---------------------------------------------------------------------------
-// narrow phase for one contact slot (Contact.update, contact.py:210-268)
-// returns the new state word; manifold written in place
+// Contact.update (contact.py:210-268) of contact slot s of world w, state word st: the manifold
+// at the bodies' transforms, warm-start impulses matched by point id, touching, the listener's
+// records; returns the new state word (the manifold is written in place).  In the narrow phase
+// (toi_seq null) a change of touching is marked CS_WOKE for the island search, which wakes the
+// bodies in collide's order, and records are keyed by slot; inside World.solve_toi (toi_seq:
+// the world's record counter) the bodies are woken here and records are keyed in call order.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
-  const int f0 = d.w_fix_off[w], b0 = d.w_body_off[w];
-  int2 pr = d.c_pair[s];
-  const int fa = f0 + pr.x, fb = f0 + pr.y;
-  int4 ia = d.f_info[fa], ib = d.f_info[fb];
-  const int ba = b0 + ia.x, bb = b0 + ib.x;
-  const int slot = s - d.w_con_base[w];
-  // a contact destroyed by collide while touching: end_contact (ContactManager.destroy,
-  // contact_manager.py:160-172), after the loop's updates
-  auto destroyed = [&]() {
-    if (st & CS_TOUCHING) {
-      const int k = ev_open(d, EV_END);
-      if (k >= 0) ev_write(d, k, w, EV_END, 1, slot, s, st);
-    }
-    return 0;
-  };
-  if (st & CS_FILTER) {  // refilter (contact_manager.py:200-208)
-    bool ok = d.b_type[ba] == BT_DYNAMIC || d.b_type[bb] == BT_DYNAMIC;
-    if (ok) {
-      if (d.w_flags[w] & WF_CUSTOM_FILTER) ok = !(st & CS_REJECT);  // the host ran the custom filter
-      else if (ia.w == ib.w && ia.w != 0) ok = ia.w > 0;  // World.default_contact_filter
-      else ok = ((ia.z >> 16) & (ib.z & 0xFFFF)) != 0 && ((ia.z & 0xFFFF) & (ib.z >> 16)) != 0;
-    }
-    if (!ok) return destroyed();
-    st &= ~(CS_FILTER | CS_REJECT);
+__device__ void wake_body(const Dev<T>& d, int gb) {  // Body.awake = True (body.py:800-812)
+  const int fl = d.b_flags[gb];
+  if (!(fl & BF_AWAKE)) {
+    d.b_sleep[gb] = 0.0;
+    d.b_flags[gb] = fl | BF_AWAKE;
   }
-  double4 A = d.f_fat[fa], B = d.f_fat[fb];
-  if (!aabb_overlap(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w)) return destroyed();
+}
+
+template <typename T>
+__device__ int update_contact(const Dev<T>& d, int w, int s, int st, int* toi_seq) {
+  const int f0 = d.w_fix_off[w], b0 = d.w_body_off[w];
+  const int2 pr = d.c_pair[s];
+  const int fa = f0 + pr.x, fb = f0 + pr.y;
+  const int ba = b0 + d.f_info[fa].x, bb = b0 + d.f_info[fb].x;
+  const int slot = s - d.w_con_base[w];
   // The manifold is evaluated in FP64 from the stored (T) geometry and stored
   // back as T: FP32 arithmetic in the clipping and frame changes would set the
   // FP32 pipeline's one-step error (5e-5 on stacked boxes), FP32 storage does not.
@@ -481,26 +486,72 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
   const bool touching = m.pc > 0;
   const bool was = (st & CS_TOUCHING) != 0;
   int nst = CS_LIVE | CS_ENABLED | (touching ? CS_TOUCHING : 0) | (m.type << CS_MTYPE_SHIFT) | (m.pc << CS_PC_SHIFT) |
-            CS_PROCESSED | (touching != was ? CS_WOKE : 0) | (st & (CS_FILTER | CS_SWAPPED));
+            (st & (CS_FILTER | CS_SWAPPED));
+  if (toi_seq) {
+    if (touching != was) {
+      wake_body(d, ba);
+      wake_body(d, bb);
+    }
+  } else {
+    nst |= CS_PROCESSED | (touching != was ? CS_WOKE : 0);
+  }
   if (d.ev_mask) {  // the listener calls of Contact.update (contact.py:262-268), in that order
+    const int ph = toi_seq ? 3 : 0;
     if (!was && touching) {
       const int k = ev_open(d, EV_BEGIN);
-      if (k >= 0) ev_write(d, k, w, EV_BEGIN, 0, 4 * slot, s, nst);
+      if (k >= 0) ev_write(d, k, w, EV_BEGIN, ph, toi_seq ? (*toi_seq)++ : 4 * slot, s, nst);
     }
     if (was && !touching) {
       const int k = ev_open(d, EV_END);
-      if (k >= 0) ev_write(d, k, w, EV_END, 0, 4 * slot + 1, s, nst);
+      if (k >= 0) ev_write(d, k, w, EV_END, ph, toi_seq ? (*toi_seq)++ : 4 * slot + 1, s, nst);
     }
     if (touching) {
       const int k = ev_open(d, EV_PRE_SOLVE);
       if (k >= 0) {
-        ev_write(d, k, w, EV_PRE_SOLVE, 0, 4 * slot + 2, s, nst);
+        ev_write(d, k, w, EV_PRE_SOLVE, ph, toi_seq ? (*toi_seq)++ : 4 * slot + 2, s, nst);
         ev_manifold<T>(d.ev_d + (size_t)k * B2G_EV_ND + B2G_ED_OLD, d.ev_i + (size_t)k * B2G_EV_NI + B2G_EI_OLD_MTYPE,
                        (st >> CS_MTYPE_SHIFT) & 3, old_pc, old_lnlp, old_pts, imp, make_uint2(oid0, oid1));
       }
     }
   }
   return nst;
+}
+
+// ---------------------------------------------------------------------------
+// narrow phase for one contact slot (ContactManager.collide's loop, contact_manager.py:174-223):
+// the filter test of a flagged contact, the broad-phase overlap test, then Contact.update;
+// returns the new state word (0: destroyed)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
+  const int f0 = d.w_fix_off[w], b0 = d.w_body_off[w];
+  int2 pr = d.c_pair[s];
+  const int fa = f0 + pr.x, fb = f0 + pr.y;
+  int4 ia = d.f_info[fa], ib = d.f_info[fb];
+  const int ba = b0 + ia.x, bb = b0 + ib.x;
+  const int slot = s - d.w_con_base[w];
+  // a contact destroyed by collide while touching: end_contact (ContactManager.destroy,
+  // contact_manager.py:160-172), after the loop's updates
+  auto destroyed = [&]() {
+    if (st & CS_TOUCHING) {
+      const int k = ev_open(d, EV_END);
+      if (k >= 0) ev_write(d, k, w, EV_END, 1, slot, s, st);
+    }
+    return 0;
+  };
+  if (st & CS_FILTER) {  // refilter (contact_manager.py:200-208)
+    bool ok = d.b_type[ba] == BT_DYNAMIC || d.b_type[bb] == BT_DYNAMIC;
+    if (ok) {
+      if (d.w_flags[w] & WF_CUSTOM_FILTER) ok = !(st & CS_REJECT);  // the host ran the custom filter
+      else if (ia.w == ib.w && ia.w != 0) ok = ia.w > 0;  // World.default_contact_filter
+      else ok = ((ia.z >> 16) & (ib.z & 0xFFFF)) != 0 && ((ia.z & 0xFFFF) & (ib.z >> 16)) != 0;
+    }
+    if (!ok) return destroyed();
+    st &= ~(CS_FILTER | CS_REJECT);
+  }
+  double4 A = d.f_fat[fa], B = d.f_fat[fb];
+  if (!aabb_overlap(A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w)) return destroyed();
+  return update_contact(d, w, s, st, nullptr);
 }
 
 template <typename T, int B>
@@ -2128,13 +2179,10 @@ __global__ void __launch_bounds__(kBlock) k_sync(Dev<T> d) {
   if ((threadIdx.x & 63) == 0 && m) atomicAdd(&d.stat[S_MOVED], (unsigned long long)__popcll(m));
 }
 
-// Body._synchronize_fixtures for fixture f: returns whether its proxy moved (was re-fattened).
+// Fixture._synchronize (fixture.py:287-298) of fixture f on body gb, with DynamicTree.move_proxy
+// (dynamictree.py:152-188): returns whether its proxy was re-fattened.
 template <typename T>
-__device__ __forceinline__ bool sync_fixture(const Dev<T>& d, int f) {
-  int2 wb = d.f_wb[f];
-  const int gb = wb.y;
-  d.f_moved[f] = 0;
-  if (d.b_island[gb] < 0 || d.b_type[gb] == BT_STATIC) return false;
+__device__ __forceinline__ bool sync_fixture_aabb(const Dev<T>& d, int f, int gb) {
   int4 info = d.f_info[f];
   const int type = info.y & 255, n = (info.y >> 8) & 255;
   const double* g = d.f_geo64 + (size_t)f * 16;
@@ -2158,6 +2206,16 @@ __device__ __forceinline__ bool sync_fixture(const Dev<T>& d, int f) {
   if (mdy < 0.0) { nlx = nlx + 0.0; nly = nly + mdy; }
   else { nux = nux + 0.0; nuy = nuy + mdy; }
   d.f_fat[f] = make_double4(nlx, nly, nux, nuy);
+  return true;
+}
+
+// Body._synchronize_fixtures for fixture f after the solve: returns whether its proxy moved.
+template <typename T>
+__device__ __forceinline__ bool sync_fixture(const Dev<T>& d, int f) {
+  const int gb = d.f_wb[f].y;
+  d.f_moved[f] = 0;
+  if (d.b_island[gb] < 0 || d.b_type[gb] == BT_STATIC) return false;
+  if (!sync_fixture_aabb(d, f, gb)) return false;
   d.f_moved[f] = 1;
   return true;
 }
@@ -3549,6 +3607,477 @@ __global__ void __launch_bounds__(256) k_ray_cast(Dev<T> d, int nr, const int* r
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Continuous physics: World.solve_toi (world.py:594-821) after the step's final pair search,
+// one lane per world in the reference's sequential order.  Each round finds the contact with
+// the earliest time of impact (contacts with a bullet or a static or kinematic body; the
+// lagging body's sweep advanced to the other's start first), advances its two bodies to that
+// time, updates it, gathers the contacts of its dynamic bodies with static, kinematic or
+// bullet bodies into a sub-step island (at most 32 contacts, 64 bodies), solves that island for
+// the rest of the step (Island.solve_toi, island.py:181-243: 20 position iterations moving
+// only the two bodies, then the velocity iterations without warm starting), re-synchronises
+// the island bodies' proxies and runs the pair search on those moved proxies.  FP64 throughout.
+// ---------------------------------------------------------------------------
+constexpr int kToiContacts = 32;     // settings.py:79 (MAX_TOI_CONTACTS)
+constexpr int kToiBodies = 64;       // 2 * MAX_TOI_CONTACTS
+constexpr int kToiMaxSubSteps = 8;   // settings.py:73
+constexpr double kToiBaumgarte = 0.75;  // settings.py:110
+
+struct ToiPC {
+  V2<double> lp0, lp1, ln, lpp, lca, lcb;
+  double ra, rb, ma, mb, Ia, Ib;
+  int type, npts, ia, ib;
+};
+struct ToiPV {
+  V2<double> c, v;
+  double a, w;
+};
+struct ToiScratch {
+  VC<double> vc[kToiContacts];
+  ToiPC pc[kToiContacts];
+  ToiPV pv[kToiBodies];
+  int bodies[kToiBodies];
+  int contacts[kToiContacts];
+};
+
+template <typename T>
+__device__ TSweep load_sweep(const Dev<T>& d, int gb) {
+  TSweep s;
+  const double2 c0 = d.b_c0[gb], c = d.b_c[gb], lc = d.b_lc[gb];
+  s.c0 = v2<double>(c0.x, c0.y);
+  s.c = v2<double>(c.x, c.y);
+  s.lc = v2<double>(lc.x, lc.y);
+  s.a0 = d.b_a0[gb];
+  s.a = d.b_a[gb];
+  s.alpha0 = d.b_alpha0[gb];
+  return s;
+}
+
+template <typename T>
+__device__ void store_sweep(const Dev<T>& d, int gb, const TSweep& s) {
+  d.b_c0[gb] = make_double2(s.c0.x, s.c0.y);
+  d.b_c[gb] = make_double2(s.c.x, s.c.y);
+  d.b_a0[gb] = s.a0;
+  d.b_a[gb] = s.a;
+  d.b_alpha0[gb] = s.alpha0;
+  const XF<double> xf = xf_at(s.c, s.a, s.lc);  // Body._synchronize_transform
+  d.b_xf[gb] = make_double4(xf.p.x, xf.p.y, xf.q.c, xf.q.s);
+}
+
+template <typename T>
+__device__ void body_advance(const Dev<T>& d, int gb, double alpha) {  // Body._advance (body.py:930-937)
+  TSweep s = load_sweep(d, gb);
+  sweep_advance(s, alpha);
+  s.c = s.c0;
+  s.a = s.a0;
+  store_sweep(d, gb, s);
+}
+
+// ContactSolver.solve_toi_position_constraints (contact.py:866-933): only the TOI bodies
+// (island indices 0 and 1) move.
+__device__ inline bool toi_position_sweep(ToiScratch& S, int nc) {
+  double min_sep = 0.0;
+  for (int k = 0; k < nc; ++k) {
+    const ToiPC& pc = S.pc[k];
+    V2<double> ca = S.pv[pc.ia].c, cb = S.pv[pc.ib].c;
+    double aa = S.pv[pc.ia].a, ab = S.pv[pc.ib].a;
+    const bool mva = pc.ia < 2, mvb = pc.ib < 2;
+    const double ima = mva ? pc.ma : 0.0, iIa = mva ? pc.Ia : 0.0;
+    const double imb = mvb ? pc.mb : 0.0, iIb = mvb ? pc.Ib : 0.0;
+    for (int j = 0; j < pc.npts; ++j) {
+      const XF<double> xfa = xf_at(ca, aa, pc.lca), xfb = xf_at(cb, ab, pc.lcb);
+      V2<double> normal, point;
+      double sep;
+      const V2<double> lpj = j == 0 ? pc.lp0 : pc.lp1;
+      if (pc.type == M_CIRCLES) {  // PositionSolverManifold (contact_util.py)
+        const V2<double> pa = xmul(xfa, pc.lpp), pb = xmul(xfb, pc.lp0);
+        normal = normalized(pb - pa);
+        point = scale(0.5, pa + pb);
+        sep = dot(pb - pa, normal) - pc.ra - pc.rb;
+      } else if (pc.type == M_FACE_A) {
+        normal = rmul(xfa.q, pc.ln);
+        const V2<double> plane = xmul(xfa, pc.lpp), clip = xmul(xfb, lpj);
+        sep = dot(clip - plane, normal) - pc.ra - pc.rb;
+        point = clip;
+      } else {
+        normal = rmul(xfb.q, pc.ln);
+        const V2<double> plane = xmul(xfb, pc.lpp), clip = xmul(xfa, lpj);
+        sep = dot(clip - plane, normal) - pc.ra - pc.rb;
+        point = clip;
+        normal = -normal;
+      }
+      const V2<double> ra = point - ca, rb = point - cb;
+      min_sep = pmin(min_sep, sep);
+      const double C = pclamp(kToiBaumgarte * (sep + LINEAR_SLOP), -MAX_LINEAR_CORRECTION, 0.0);
+      const double rna = sq(cross(ra, normal)), rnb = sq(cross(rb, normal));
+      const double K = ima + imb + iIa * rna + iIb * rnb;
+      const double impulse = K > 0.0 ? -C / K : 0.0;
+      const V2<double> P = scale(impulse, normal);
+      ca = ca - scale(ima, P);
+      aa -= iIa * cross(ra, P);
+      cb = cb + scale(imb, P);
+      ab += iIb * cross(rb, P);
+    }
+    S.pv[pc.ia].c = ca;
+    S.pv[pc.ia].a = aa;
+    S.pv[pc.ib].c = cb;
+    S.pv[pc.ib].a = ab;
+  }
+  return min_sep >= -1.5 * LINEAR_SLOP;
+}
+
+// ContactSolver.initialize_velocity_constraints (contact.py:414-496) of island contact k at
+// the island's positions, in world coordinates.
+__device__ inline void toi_init_velocity(ToiScratch& S, int k) {
+  VC<double>& vc = S.vc[k];
+  const ToiPC& pc = S.pc[k];
+  const ToiPV &A = S.pv[pc.ia], &B = S.pv[pc.ib];
+  const XF<double> xfa = xf_at(A.c, A.a, pc.lca), xfb = xf_at(B.c, B.a, pc.lcb);
+  V2<double> w0 = v2<double>(0, 0), w1 = v2<double>(0, 0);
+  const V2<double> normal = world_manifold<double>(pc.type, pc.npts, pc.ln, pc.lpp, pc.lp0, pc.lp1, xfa, pc.ra, xfb,
+                                                   pc.rb, w0, w1);
+  vc.nx = normal.x;
+  vc.ny = normal.y;
+  const double ma = vc.ma, mb = vc.mb, Ia = vc.Ia, Ib = vc.Ib;
+  for (int j = 0; j < vc.npts; ++j) {
+    VCP<double>& p = vc.p[j];
+    const V2<double> wp = j == 0 ? w0 : w1;
+    const V2<double> ra = wp - A.c, rb = wp - B.c;
+    p.rax = ra.x; p.ray = ra.y; p.rbx = rb.x; p.rby = rb.y;
+    const double kn = ma + mb + Ia * sq(cross(ra, normal)) + Ib * sq(cross(rb, normal));
+    p.nm = kn > 0.0 ? 1.0 / kn : 0.0;
+    const V2<double> tangent = cross_vs(normal, 1.0);
+    const double kt = ma + mb + Ia * sq(cross(ra, tangent)) + Ib * sq(cross(rb, tangent));
+    p.tm = kt > 0.0 ? 1.0 / kt : 0.0;
+    p.bias = 0.0;
+    const double vrel = dot(normal, B.v + cross_sv(B.w, rb) - A.v - cross_sv(A.w, ra));
+    if (vrel < -VELOCITY_THRESHOLD) p.bias = -vc.restitution * vrel;
+  }
+  vc.k11 = 1; vc.k12 = 0; vc.k22 = 1; vc.n11 = 1; vc.n12 = 0; vc.n21 = 0; vc.n22 = 1;
+  if (vc.npts == 2) {
+    const V2<double> ra1 = dv2(vc.p[0].rax, vc.p[0].ray), rb1 = dv2(vc.p[0].rbx, vc.p[0].rby);
+    const V2<double> ra2 = dv2(vc.p[1].rax, vc.p[1].ray), rb2 = dv2(vc.p[1].rbx, vc.p[1].rby);
+    const double rn1a = cross(ra1, normal), rn1b = cross(rb1, normal);
+    const double rn2a = cross(ra2, normal), rn2b = cross(rb2, normal);
+    const double k11 = ma + mb + Ia * rn1a * rn1a + Ib * rn1b * rn1b;
+    const double k22 = ma + mb + Ia * rn2a * rn2a + Ib * rn2b * rn2b;
+    const double k12 = ma + mb + Ia * rn1a * rn2a + Ib * rn1b * rn2b;
+    if (sq(k11) < 1000.0 * (k11 * k22 - sq(k12))) {
+      vc.k11 = k11; vc.k12 = k12; vc.k22 = k22;
+      double det = k11 * k22 - k12 * k12;
+      if (det != 0.0) det = 1.0 / det;
+      vc.n11 = det * k22; vc.n21 = -det * k12; vc.n12 = -det * k12; vc.n22 = det * k11;
+    } else {
+      vc.npts = 1;  // quirk Q6
+    }
+  }
+}
+
+// One World.solve_toi of world w (lane 0 of its wavefront).
+template <typename T>
+__device__ void world_solve_toi(const Dev<T>& d, int w, ToiScratch& S) {
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
+  const int f0 = d.w_fix_off[w], nf = d.w_fix_off[w + 1] - f0;
+  const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
+  int* mv = d.toi_move + f0;
+  int seq = 0;  // listener records of this phase, in call order
+  unsigned long long calls = 0, events = 0;
+  for (int i = 0; i < nb; ++i) {
+    d.b_alpha0[b0 + i] = 0.0;
+    d.b_tflag[b0 + i] = 0;
+  }
+  for (int sl = 0; sl < d.w_con_used[w]; ++sl) {
+    d.c_toi[cb + sl] = 1.0;
+    d.c_tmeta[cb + sl] = 0;
+  }
+  const double eps = NormEps<double>::value;
+  for (;;) {
+    const int used = d.w_con_used[w];
+    int min_s = -1;
+    double min_alpha = 1.0;
+    for (int sl = 0; sl < used; ++sl) {
+      const int s = cb + sl, st = d.c_state[s];
+      if (!(st & CS_LIVE) || !(st & CS_ENABLED)) continue;
+      const int meta = d.c_tmeta[s];
+      if ((meta >> 8) > kToiMaxSubSteps) continue;
+      double alpha = 1.0;
+      if (meta & 1) {
+        alpha = d.c_toi[s];
+      } else {
+        const int2 pr = d.c_pair[s];
+        const int fa = f0 + pr.x, fb = f0 + pr.y;
+        const int ba = b0 + d.f_info[fa].x, bb = b0 + d.f_info[fb].x;
+        const int ta = d.b_type[ba], tb = d.b_type[bb], fla = d.b_flags[ba], flb = d.b_flags[bb];
+        const bool active_a = (fla & BF_AWAKE) && ta != BT_STATIC, active_b = (flb & BF_AWAKE) && tb != BT_STATIC;
+        if (!active_a && !active_b) continue;
+        const bool col_a = (fla & BF_BULLET) || ta != BT_DYNAMIC, col_b = (flb & BF_BULLET) || tb != BT_DYNAMIC;
+        if (!col_a && !col_b) continue;
+        double alpha0 = d.b_alpha0[ba];
+        if (d.b_alpha0[ba] < d.b_alpha0[bb]) {
+          alpha0 = d.b_alpha0[bb];
+          TSweep sw = load_sweep(d, ba);
+          sweep_advance(sw, alpha0);
+          store_sweep(d, ba, sw);
+        } else if (d.b_alpha0[bb] < d.b_alpha0[ba]) {
+          TSweep sw = load_sweep(d, bb);
+          sweep_advance(sw, alpha0);
+          store_sweep(d, bb, sw);
+        }
+        TProxy pa, pb;
+        tproxy_init(pa, geo_of(d, fa));
+        tproxy_init(pb, geo_of(d, fb));
+        double beta;
+        int state = time_of_impact(pa, pb, load_sweep(d, ba), load_sweep(d, bb), beta);
+        ++calls;
+        if (state == TOI_STUCK) atomicOr(&d.w_err[w], WERR_TOI);
+        alpha = state == TOI_TOUCHING ? pmin(alpha0 + (1.0 - alpha0) * beta, 1.0) : 1.0;
+        d.c_toi[s] = alpha;
+        d.c_tmeta[s] = meta | 1;
+      }
+      if (alpha < min_alpha) {
+        min_s = s;
+        min_alpha = alpha;
+      }
+    }
+    if (min_s < 0 || (1.0 - 10.0 * eps) < min_alpha) break;
+    // advance the two bodies to the time of impact and update their contact
+    const int2 mpr = d.c_pair[min_s];
+    const int ba = b0 + d.f_info[f0 + mpr.x].x, bb = b0 + d.f_info[f0 + mpr.y].x;
+    const TSweep back_a = load_sweep(d, ba), back_b = load_sweep(d, bb);
+    body_advance(d, ba, min_alpha);
+    body_advance(d, bb, min_alpha);
+    int st = update_contact(d, w, min_s, d.c_state[min_s], &seq);
+    d.c_state[min_s] = st;
+    const int m0 = d.c_tmeta[min_s];  // its cached time is spent, one more sub-step
+    d.c_tmeta[min_s] = (m0 & 2) | (((m0 >> 8) + 1) << 8);
+    if (!(st & CS_ENABLED) || !(st & CS_TOUCHING)) {
+      d.c_state[min_s] = st & ~CS_ENABLED;
+      store_sweep(d, ba, back_a);
+      store_sweep(d, bb, back_b);
+      continue;
+    }
+    wake_body(d, ba);
+    wake_body(d, bb);
+    // the sub-step island: the two bodies, their contact, and the contacts of their dynamic
+    // bodies with static, kinematic or bullet bodies (in each body's contact order)
+    int nbi = 0, nci = 0;
+    S.bodies[nbi++] = ba;
+    d.b_tflag[ba] = nbi;
+    S.bodies[nbi++] = bb;
+    d.b_tflag[bb] = nbi;
+    S.contacts[nci++] = min_s;
+    d.c_tmeta[min_s] |= 2;
+    for (int side = 0; side < 2; ++side) {
+      const int body = side == 0 ? ba : bb;
+      if (d.b_type[body] != BT_DYNAMIC) continue;
+      const bool bullet_b = (d.b_flags[body] & BF_BULLET) != 0;
+      for (int sl = 0; sl < d.w_con_used[w]; ++sl) {
+        const int s = cb + sl;
+        if (!(d.c_state[s] & CS_LIVE)) continue;
+        const int2 pr = d.c_pair[s];
+        const int ga = b0 + d.f_info[f0 + pr.x].x, gbd = b0 + d.f_info[f0 + pr.y].x;
+        if (ga != body && gbd != body) continue;
+        if (nbi == kToiBodies || nci == kToiContacts) break;  // the island's capacity
+        if (d.c_tmeta[s] & 2) continue;
+        const int other = ga == body ? gbd : ga;
+        if (d.b_type[other] == BT_DYNAMIC && !bullet_b && !(d.b_flags[other] & BF_BULLET)) continue;
+        const TSweep back = load_sweep(d, other);
+        if (!d.b_tflag[other]) body_advance(d, other, min_alpha);
+        const int cst = update_contact(d, w, s, d.c_state[s], &seq);
+        d.c_state[s] = cst;
+        if (!(cst & CS_ENABLED) || !(cst & CS_TOUCHING)) {
+          store_sweep(d, other, back);
+          continue;
+        }
+        d.c_tmeta[s] |= 2;
+        S.contacts[nci++] = s;
+        if (d.b_tflag[other]) continue;
+        if (d.b_type[other] != BT_STATIC) wake_body(d, other);
+        S.bodies[nbi++] = other;
+        d.b_tflag[other] = nbi;
+      }
+    }
+    // Island.solve_toi over the rest of the step
+    const double sub_dt = (1.0 - min_alpha) * d.dt;
+    for (int i = 0; i < nbi; ++i) {
+      const int gb = S.bodies[i];
+      const double2 c = d.b_c[gb];
+      S.pv[i].c = v2<double>(c.x, c.y);
+      S.pv[i].a = d.b_a[gb];
+      S.pv[i].v = d.b_v[gb];
+      S.pv[i].w = d.b_w[gb];
+    }
+    for (int k = 0; k < nci; ++k) {  // ContactSolver.__init__ (contact.py:376-412), no warm starting
+      const int s = S.contacts[k], cst = d.c_state[s];
+      const int2 pr = d.c_pair[s];
+      const int fa = f0 + pr.x, fb = f0 + pr.y;
+      const int ga = b0 + d.f_info[fa].x, gbd = b0 + d.f_info[fb].x;
+      const V2<T> mat = d.c_mat[s];
+      const V4<T> lnlp = d.c_lnlp[s], pts = d.c_pts[s];
+      const V4<double> imp = d.c_imp[s];
+      VC<double>& vc = S.vc[k];
+      ToiPC& pc = S.pc[k];
+      vc.friction = (double)mat.x;
+      vc.restitution = (double)mat.y;
+      vc.ia = d.b_tflag[ga] - 1;
+      vc.ib = d.b_tflag[gbd] - 1;
+      vc.ma = (double)d.b_im[ga];
+      vc.mb = (double)d.b_im[gbd];
+      vc.Ia = (double)d.b_iI[ga];
+      vc.Ib = (double)d.b_iI[gbd];
+      vc.npts = (cst >> CS_PC_SHIFT) & 3;
+      vc.rka = vc.rkb = vc.pad = 0;
+      vc.p[0].ni = imp.x; vc.p[0].ti = imp.y;  // the stored impulses (quirk Q8)
+      vc.p[1].ni = imp.z; vc.p[1].ti = imp.w;
+      vc.p[0].pad = vc.p[1].pad = 0.0;
+      pc.lp0 = dv2(pts.x, pts.y);
+      pc.lp1 = dv2(pts.z, pts.w);
+      pc.ln = dv2(lnlp.x, lnlp.y);
+      pc.lpp = dv2(lnlp.z, lnlp.w);
+      const double2 lca = d.b_lc[ga], lcb = d.b_lc[gbd];
+      pc.lca = v2<double>(lca.x, lca.y);
+      pc.lcb = v2<double>(lcb.x, lcb.y);
+      pc.ra = d.f_rad64[fa];
+      pc.rb = d.f_rad64[fb];
+      pc.ma = vc.ma; pc.mb = vc.mb; pc.Ia = vc.Ia; pc.Ib = vc.Ib;
+      pc.type = (cst >> CS_MTYPE_SHIFT) & 3;
+      pc.npts = vc.npts;
+      pc.ia = vc.ia;
+      pc.ib = vc.ib;
+    }
+    for (int it = 0; it < 20; ++it)
+      if (toi_position_sweep(S, nci)) break;
+    {  // the two TOI bodies' sweeps start at the solved positions ("leap of faith")
+      TSweep sa = load_sweep(d, ba), sb = load_sweep(d, bb);
+      sa.c0 = S.pv[0].c; sa.a0 = S.pv[0].a;
+      sb.c0 = S.pv[1].c; sb.a0 = S.pv[1].a;
+      d.b_c0[ba] = make_double2(sa.c0.x, sa.c0.y);
+      d.b_a0[ba] = sa.a0;
+      d.b_c0[bb] = make_double2(sb.c0.x, sb.c0.y);
+      d.b_a0[bb] = sb.a0;
+    }
+    for (int k = 0; k < nci; ++k) toi_init_velocity(S, k);
+    for (int it = 0; it < d.vel_iters; ++it)
+      for (int k = 0; k < nci; ++k) {
+        VC<double>& vc = S.vc[k];
+        V2<double> va = S.pv[vc.ia].v, vb = S.pv[vc.ib].v;
+        double wa = S.pv[vc.ia].w, wbv = S.pv[vc.ib].w;
+        solve_contact_velocity(vc, va, wa, vb, wbv);
+        S.pv[vc.ia].v = va; S.pv[vc.ia].w = wa;
+        S.pv[vc.ib].v = vb; S.pv[vc.ib].w = wbv;
+      }
+    for (int i = 0; i < nbi; ++i) {  // integrate positions
+      const int gb = S.bodies[i];
+      V2<double> v = S.pv[i].v;
+      double wv = S.pv[i].w;
+      const V2<double> tr = scale(sub_dt, v);
+      if (tr.x * tr.x + tr.y * tr.y > MAX_TRANSLATION * MAX_TRANSLATION)
+        v = scale(MAX_TRANSLATION / sqrt(tr.x * tr.x + tr.y * tr.y), v);
+      const double rotation = sub_dt * wv;
+      if (rotation * rotation > MAX_ROTATION * MAX_ROTATION) wv *= MAX_ROTATION / fabs(rotation);
+      const V2<double> c = S.pv[i].c + scale(sub_dt, v);
+      const double a = S.pv[i].a + sub_dt * wv;
+      TSweep sw = load_sweep(d, gb);
+      sw.c = c;
+      sw.a = a;
+      store_sweep(d, gb, sw);
+      d.b_v[gb] = v;
+      d.b_w[gb] = wv;
+    }
+    if (d.ev_mask & (1 << EV_POST_SOLVE))  // Island.report
+      for (int k = 0; k < nci; ++k) {
+        const int e = ev_open(d, EV_POST_SOLVE);
+        if (e < 0) continue;
+        const VC<double>& vc = S.vc[k];
+        ev_write(d, e, w, EV_POST_SOLVE, 3, seq++, S.contacts[k], d.c_state[S.contacts[k]]);
+        int* ei = d.ev_i + (size_t)e * B2G_EV_NI;
+        double* ed = d.ev_d + (size_t)e * B2G_EV_ND;
+        ei[B2G_EI_OLD_PC] = vc.npts;
+        ed[B2G_ED_OLD + B2G_ED_P0_NI] = vc.p[0].ni;
+        ed[B2G_ED_OLD + B2G_ED_P0_TI] = vc.p[0].ti;
+        ed[B2G_ED_OLD + B2G_ED_P1_NI] = vc.npts >= 2 ? vc.p[1].ni : 0.0;
+        ed[B2G_ED_OLD + B2G_ED_P1_TI] = vc.npts >= 2 ? vc.p[1].ti : 0.0;
+      }
+    ++events;
+    // re-synchronise the island's dynamic bodies, forget their contacts' times of impact
+    int nmove = 0;
+    for (int i = 0; i < nbi; ++i) {
+      const int gb = S.bodies[i];
+      d.b_tflag[gb] = 0;
+      if (d.b_type[gb] != BT_DYNAMIC) continue;
+      const int lb = gb - b0;
+      for (int f = 0; f < nf; ++f)
+        if (d.f_info[f0 + f].x == lb && sync_fixture_aabb(d, f0 + f, gb)) mv[nmove++] = f;
+      for (int sl = 0; sl < d.w_con_used[w]; ++sl) {
+        const int s = cb + sl;
+        if (!(d.c_state[s] & CS_LIVE)) continue;
+        const int2 pr = d.c_pair[s];
+        if (b0 + d.f_info[f0 + pr.x].x == gb || b0 + d.f_info[f0 + pr.y].x == gb) d.c_tmeta[s] &= ~3;
+      }
+    }
+    // ContactManager.find_new_contacts over the moved proxies (move order, then partner index)
+    for (int i = 0; i < nmove; ++i) {
+      const int P = mv[i];
+      const double4 fp = d.f_fat[f0 + P];
+      const int4 ip = d.f_info[f0 + P];
+      for (int R = 0; R < nf; ++R) {
+        if (R == P) continue;
+        bool earlier = false;
+        for (int j = 0; j < i; ++j) earlier |= mv[j] == R;
+        if (earlier) continue;  // (R, P) is in the pair set already
+        const double4 fr = d.f_fat[f0 + R];
+        if (!aabb_overlap(fr.x, fr.y, fr.z, fr.w, fp.x, fp.y, fp.z, fp.w)) continue;
+        const int4 ir = d.f_info[f0 + R];
+        if (ip.x == ir.x) continue;
+        const int gp = b0 + ip.x, gr = b0 + ir.x;
+        if (!(d.b_flags[gp] & BF_ACTIVE) || !(d.b_flags[gr] & BF_ACTIVE)) continue;
+        bool exists = false;
+        const int used2 = d.w_con_used[w];
+        for (int sl = 0; sl < used2 && !exists; ++sl) {
+          const int s = cb + sl;
+          if (!(d.c_state[s] & CS_LIVE)) continue;
+          const int2 pr = d.c_pair[s];
+          exists = (pr.x == P && pr.y == R) || (pr.x == R && pr.y == P);
+        }
+        if (exists) continue;
+        if (d.b_type[gp] != BT_DYNAMIC && d.b_type[gr] != BT_DYNAMIC) continue;  // Body._should_collide
+        bool ok;
+        if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;  // World.default_contact_filter
+        else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
+        if (!ok) continue;
+        const int ta = ip.y & 255, tb = ir.y & 255;
+        if (ta == SHAPE_EDGE && tb == SHAPE_EDGE) continue;  // no collider registered
+        if (used2 >= ccap) {
+          atomicOr(&d.w_err[w], WERR_CAPACITY);
+          continue;
+        }
+        create_contact(d, f0, cb + used2, P, R);
+        d.c_toi[cb + used2] = 0.0;
+        d.c_tmeta[cb + used2] = 0;
+        d.w_con_used[w] = used2 + 1;
+      }
+    }
+  }
+  if (calls) atomicAdd(&d.stat[S_TOI_CALLS], calls);
+  if (events) {
+    atomicAdd(&d.stat[S_TOI_EVENTS], events);
+    int asleep = 0;  // the events woke bodies
+    for (int i = 0; i < nb; ++i)
+      if (d.b_type[b0 + i] != BT_STATIC && !(d.b_flags[b0 + i] & BF_AWAKE)) ++asleep;
+    d.w_asleep[w] = asleep;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_toi(Dev<T> d, int nworlds) {
+  const int k = blockIdx.x;
+  if (k >= nworlds || threadIdx.x != 0) return;
+  const int w = d.toi_worlds[k];
+  if (!(d.w_flags[w] & WF_CONTINUOUS) || !(d.dt > 0.0)) return;
+  world_solve_toi(d, w, *(ToiScratch*)(d.toi_scr + (size_t)k * sizeof(ToiScratch)));
+}
+
 template <typename T>
 __global__ void k_reset_queue(Dev<T> d) {
   if (blockIdx.x == 0) {
@@ -3873,6 +4402,7 @@ struct b2g_ctx {
   // solver launch shape (b2g_set_param)
   int large_worlds = 0;  // 1: every world takes the global-memory island and pair search
   std::vector<int> big;  // worlds on that path
+  std::vector<int> toi_list;  // worlds with continuous physics (k_toi)
   int heavy_class = 5;
   int team_full_class = 9;
   int team_lds = 24576;
@@ -4089,6 +4619,9 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
     w_move_n[w] = b->move_off[w + 1] - b->move_off[w];
     w_con_used[w] = b->con_off[w + 1] - b->con_off[w];
     if (w_flags[w] & WF_NEW_FIXTURE) ctx->pending_new_fixture = true;
+    if ((w_flags[w] & WF_CONTINUOUS) && (w_flags[w] & WF_CUSTOM_FILTER))
+      return set_error(ctx, B2G_E_UNSUPPORTED,
+                       "continuous physics with a custom contact filter (the filter would run inside the kernel)");
     if (w_move_n[w] > b->fix_off[w + 1] - b->fix_off[w]) return set_error(ctx, B2G_E_INVALID, "move buffer too long");
   }
   std::vector<int> b_type(nb), b_flags(nb), b_island(nb, -1);
@@ -4332,6 +4865,29 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
     ctx->isl_runs = plan_runs(w_order, ni, kIslandsPerCu);
     ctx->pair_runs = plan_runs(w_order, np, kPairsPerCu);
   }
+  // continuous physics: the worlds that run World.solve_toi after each step, and their state
+  ctx->toi_list.clear();
+  for (int w = 0; w < nw; ++w)
+    if (w_flags[w] & WF_CONTINUOUS) ctx->toi_list.push_back(w);
+  d.toi_worlds = nullptr;
+  d.toi_scr = nullptr;
+  d.toi_move = nullptr;
+  d.b_alpha0 = nullptr;
+  d.b_tflag = nullptr;
+  d.c_toi = nullptr;
+  d.c_tmeta = nullptr;
+  if (!ctx->toi_list.empty()) {
+    int* tl = nullptr;
+    TRY(dalloc(ctx, &tl, ctx->toi_list.size()));
+    TRY(upload(ctx, tl, ctx->toi_list.data(), ctx->toi_list.size()));
+    d.toi_worlds = tl;
+    TRY(dalloc(ctx, &d.toi_scr, ctx->toi_list.size() * sizeof(ToiScratch)));
+    TRY(dalloc(ctx, &d.toi_move, nf));
+    TRY(dalloc(ctx, &d.b_alpha0, nb));
+    TRY(dalloc(ctx, &d.b_tflag, nb));
+    TRY(dalloc(ctx, &d.c_toi, ncap));
+    TRY(dalloc(ctx, &d.c_tmeta, ncap));
+  }
   if (!ctx->big.empty()) {  // scratch of the large-world path
     int *wb = nullptr, *bl = nullptr;
     TRY(dalloc(ctx, &wb, nw));
@@ -4564,6 +5120,8 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     hipLaunchKernelGGL(k_sync<T>, dim3(std::max(1, (ctx->nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[4], st));
     TRY(pairs(0, 1, 1));
+    const int ntoi = (int)ctx->toi_list.size();
+    if (ntoi > 0) hipLaunchKernelGGL(k_toi<T>, dim3(ntoi), dim3(64), 0, st, d, ntoi);
   } else {
     if (timed) {
       HIPCHK(hipEventRecord(ev[2], st));
@@ -5105,6 +5663,9 @@ B2G_API int b2g_sync(b2g_ctx* ctx) {
   if (e & WERR_ISLAND_SIZE)
     return set_error(ctx, B2G_E_UNSUPPORTED, "an island of more than 65535 contacts (not solved)");
   if (e & WERR_SCHEDULE) return set_error(ctx, B2G_E_HIP, "island solver pass limit reached (internal error)");
+  if (e & WERR_TOI)
+    return set_error(ctx, B2G_E_UNSUPPORTED,
+                     "a time-of-impact search did not end (the reference's push-back loop has no bound)");
   return B2G_OK;
 }
 
@@ -5387,7 +5948,7 @@ B2G_API int b2g_stats(b2g_ctx* ctx, int64_t* out, int32_t n, int32_t reset) {
   unsigned long long v[kStats];
   HIPCHK(hipMemcpy(v, st, sizeof(v), hipMemcpyDeviceToHost));
   out[0] = ctx->stat_steps;
-  for (int k = 0; k < 6 && k + 1 < n; ++k) out[k + 1] = (int64_t)v[k];
+  for (int k = 0; k < 8 && k + 1 < n; ++k) out[k + 1] = (int64_t)v[k];
   if (reset) {
     HIPCHK(hipMemset(st, 0, sizeof(v)));
     ctx->stat_steps = 0;

@@ -58,6 +58,7 @@ WORLD_ND = 4
 WI_FLAGS, WI_NEW_CONTACTS = 0, 1
 WORLD_NI = 4
 WF_ALLOW_SLEEP, WF_WARM_STARTING, WF_NEW_FIXTURE, WF_KEEP_FORCES, WF_CUSTOM_FILTER = 1, 2, 4, 8, 16
+WF_CONTINUOUS = 32  # continuous physics on (World.solve_toi after each step)
 
 # contact listener records (b2g_events)
 EV_BEGIN, EV_END, EV_PRE_SOLVE, EV_POST_SOLVE = range(4)

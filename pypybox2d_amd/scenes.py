@@ -4,7 +4,8 @@ A scene is a dict ``{"gravity", "do_sleep", "warm_starting", "bodies"}``;
 every body is a dict with ``type`` ("static" / "dynamic" / "kinematic"), ``position``,
 ``angle``, ``linear_velocity``, ``angular_velocity`` and a list of
 ``fixtures`` (``shape``, ``density``, ``friction``, ``restitution`` and, when
-not the defaults, ``category``, ``mask`` and ``group``).  Shapes
+not the defaults, ``category``, ``mask`` and ``group``); a body may carry
+``bullet``, and the scene ``continuous`` (continuous physics on).  Shapes
 are tuples: ``("box", hx, hy)``, ``("circle", r)``, ``("polygon", [(x, y),
 ...])`` (convex, counter-clockwise) or ``("edge", (x1, y1), (x2, y2))``.
 
@@ -19,6 +20,8 @@ Workloads (gravity (0, -10), dt = 1/60, 8 velocity / 3 position iterations):
   pyramids     N pyramids side by side on their own ground edges
   lognormal_batch  many mixed worlds with lognormal body counts
   filtered_world   mixed bodies with category/mask/group filtering and a kinematic platform
+  projectiles  fast bodies (some bullets) shot at a thin wall and into the ground
+               (continuous physics)
 """
 from __future__ import annotations
 
@@ -28,10 +31,13 @@ from typing import List
 import numpy as np
 
 
-def _body(type_, position, fixtures, angle=0.0, linear_velocity=(0.0, 0.0), angular_velocity=0.0):
-    return {"type": type_, "position": tuple(position), "angle": angle,
-            "linear_velocity": tuple(linear_velocity), "angular_velocity": angular_velocity,
-            "fixtures": fixtures}
+def _body(type_, position, fixtures, angle=0.0, linear_velocity=(0.0, 0.0), angular_velocity=0.0, bullet=False):
+    b = {"type": type_, "position": tuple(position), "angle": angle,
+         "linear_velocity": tuple(linear_velocity), "angular_velocity": angular_velocity,
+         "fixtures": fixtures}
+    if bullet:
+        b["bullet"] = True
+    return b
 
 
 def _fix(shape, density=0.0, friction=0.2, restitution=0.0, category=0x0001, mask=0xFFFF, group=0):
@@ -206,6 +212,36 @@ def filtered_world(seed: int = 5, n_bodies: int = 36):
                                                      mask=masks[cat], group=group)],
                             angle=float(rng.uniform(-np.pi, np.pi)),
                             linear_velocity=(float(rng.normal(0, 0.5)), 0.0)))
+    return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies}
+
+
+def projectiles(seed: int = 3):
+    """Fast bodies that tunnel without continuous physics: a thin static wall (0.1 m half
+    width) at x = 10 on an edge ground, a column of dynamic boxes in front of it, and 10
+    projectiles (circles, boxes, a triangle; every other one a bullet) shot at it at 60-150 m/s
+    (1-2.5 m per step), plus 3 bodies thrown into the ground at 50-90 m/s and a kinematic
+    paddle sweeping through."""
+    rng = np.random.default_rng(seed)
+    bodies = [_body("static", (0.0, 0.0), [_fix(("edge", (-30.0, 0.0), (30.0, 0.0)))]),
+              _body("static", (10.0, 4.0), [_fix(("box", 0.1, 4.0))]),
+              _body("kinematic", (0.0, 9.0), [_fix(("box", 2.0, 0.15))], linear_velocity=(0.0, -1.5),
+                    angular_velocity=1.0)]
+    for k in range(4):  # dynamic boxes against the wall: bullets hit them (bullet vs dynamic)
+        bodies.append(_body("dynamic", (9.3, 0.5 + 1.01 * k), [_fix(("box", 0.5, 0.5), density=1.0, friction=0.6)]))
+    shapes = [("circle", 0.2), ("box", 0.2, 0.15), ("polygon", [(-0.2, -0.15), (0.25, -0.1), (0.0, 0.2)]),
+              ("circle", 0.12), ("box", 0.1, 0.3)]
+    for k in range(10):
+        sh = shapes[k % len(shapes)]
+        speed = float(rng.uniform(60.0, 150.0))
+        y = 0.6 + 0.75 * k
+        ang = float(rng.uniform(-0.05, 0.05))
+        bodies.append(_body("dynamic", (-4.0 - 1.5 * k, y), [_fix(sh, density=float(rng.uniform(0.5, 3.0)),
+                                                                     friction=0.3, restitution=0.1 * (k % 3))],
+                            angle=float(rng.uniform(-1, 1)), linear_velocity=(speed * math.cos(ang), speed * math.sin(ang)),
+                            angular_velocity=float(rng.uniform(-5, 5)), bullet=(k % 2 == 0)))
+    for k in range(3):
+        bodies.append(_body("dynamic", (-12.0 + 4.0 * k, 6.0), [_fix(shapes[k], density=1.0, friction=0.4)],
+                            linear_velocity=(float(rng.uniform(-5, 5)), -float(rng.uniform(50, 90)))))
     return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies}
 
 

@@ -24,9 +24,14 @@ creates (the device creates every pair Body._should_collide admits; the
 rejected ones are dropped before the next step) and on the contacts flagged
 by Fixture.refilter, before the step that would test them.
 
+Continuous physics (``continuous_physics=True``, the reference's default) runs
+on the device too: after each step the time-of-impact sub-steps of
+World.solve_toi (world.py:594-821) run per world.  Its listener calls come
+after the step's others.
+
 Outside the device path, and therefore refused with NotImplementedError when a
-world holding them is stepped: continuous physics (TOI), joints, controllers,
-sensors and loop shapes.
+world holding them is stepped: joints, controllers, sensors, loop shapes,
+sub-stepping, and a custom contact filter together with continuous physics.
 """
 from __future__ import annotations
 
@@ -199,7 +204,17 @@ class World(object):
 
     @continuous_physics.setter
     def continuous_physics(self, v):
+        self._pull()
         self._continuous_physics = bool(v)
+        self._host_dirty = True
+
+    @property
+    def sub_stepping(self):
+        return self._sub_stepping
+
+    @sub_stepping.setter
+    def sub_stepping(self, v):
+        self._sub_stepping = bool(v)
 
     @property
     def auto_clear_forces(self):
@@ -394,15 +409,18 @@ class World(object):
                              | (L.WF_WARM_STARTING if self._warm_starting else 0)
                              | (L.WF_NEW_FIXTURE if self._new_fixture else 0)
                              | (0 if self._auto_clear_forces else L.WF_KEEP_FORCES)
-                             | (L.WF_CUSTOM_FILTER if self._custom_filter() else 0))
+                             | (L.WF_CUSTOM_FILTER if self._custom_filter() else 0)
+                             | (L.WF_CONTINUOUS if self._continuous_physics else 0))
         return L.BatchArrays(np.array([0, nb], np.int32), np.array([0, nf], np.int32),
                              np.array([0, len(ci)], np.int32), np.array([0, len(move)], np.int32),
                              wd, wi, bd, bi, fd, fi, self._con_d.copy(), ci, move)
 
     def _check_supported(self):
-        if self._continuous_physics:
-            raise NotImplementedError("continuous physics (TOI) is not part of the device contact pipeline; "
-                                      "create the world with continuous_physics=False")
+        if self._continuous_physics and self._sub_stepping:
+            raise NotImplementedError("sub-stepping is not supported")
+        if self._continuous_physics and self._custom_filter():
+            raise NotImplementedError("a custom contact filter with continuous physics is not supported (the "
+                                      "time-of-impact sub-steps create contacts inside the device step)")
         if self.joints or self.controllers:
             raise NotImplementedError("joints and controllers are outside the device contact pipeline")
         for b in self.bodies:
@@ -518,9 +536,9 @@ class World(object):
             st.disable([0] * len(disable), disable)
         st.solve()
         self._device_ahead = True
-        if cm.post_solve is not None:
-            ei, ed = st.events()
-            self._dispatch(ei, ed, fixtures, (2,))
+        if cm.post_solve is not None or self._continuous_physics:
+            ei, ed = st.events()  # post_solve, then the calls of the time-of-impact sub-steps
+            self._dispatch(ei, ed, fixtures, (2, 3))
         self._new_contacts(fixtures, custom)
 
     def _dispatch(self, ei, ed, fixtures, phases):

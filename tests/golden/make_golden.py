@@ -1,7 +1,7 @@
 """Writes the golden fixtures of the contact pipeline (tests/golden/*.npz).
 
 Run once in the build container:  python tests/golden/make_golden.py [names]
-(names: pyramid stacks mixed filtered listeners queries narrowphase; default all)
+(names: pyramid stacks mixed filtered listeners continuous queries narrowphase; default all)
 
 It imports pypybox2d from /root/reference in its default pure-Python flavour
 (the reference tree ships no compiled _common) and records, for small scenes:
@@ -53,16 +53,16 @@ def ref_shape(spec):
     raise ValueError(kind)
 
 
-def build_ref_world(scene):
+def build_ref_world(scene, continuous_physics=False):
     world = b2.World(gravity=scene["gravity"], do_sleep=scene["do_sleep"],
-                     warm_starting=scene["warm_starting"], continuous_physics=False)
+                     warm_starting=scene["warm_starting"], continuous_physics=continuous_physics)
     for bs in scene["bodies"]:
         fixtures = [b2.Fixture(ref_shape(f["shape"]), density=f["density"], friction=f["friction"],
                                restitution=f["restitution"], category_bits=scenes.fixture_filter(f)[0],
                                mask_bits=scenes.fixture_filter(f)[1], group_index=scenes.fixture_filter(f)[2])
                     for f in bs["fixtures"]]
         kw = dict(position=bs["position"], angle=bs["angle"], linear_velocity=bs["linear_velocity"],
-                  angular_velocity=bs["angular_velocity"], fixtures=fixtures)
+                  angular_velocity=bs["angular_velocity"], fixtures=fixtures, bullet=bs.get("bullet", False))
         if bs["type"] == "static":
             world.create_static_body(**kw)
         elif bs["type"] == "kinematic":
@@ -189,21 +189,50 @@ def snapshot(world):
     wi[0, L.WI_FLAGS] = ((L.WF_ALLOW_SLEEP if world._allow_sleep else 0)
                          | (L.WF_WARM_STARTING if world._warm_starting else 0)
                          | (L.WF_NEW_FIXTURE if world._new_fixture else 0)
-                         | (0 if world._auto_clear_forces else L.WF_KEEP_FORCES))
+                         | (0 if world._auto_clear_forces else L.WF_KEEP_FORCES)
+                         | (L.WF_CONTINUOUS if world._continuous_physics else 0))
     return L.BatchArrays(np.array([0, nb], np.int32), np.array([0, nf], np.int32),
                          np.array([0, nc], np.int32), np.array([0, len(move)], np.int32),
                          wd, wi, bd, bi, fd, fi, cd, ci, np.array(move, np.int32)), fidx
 
 
-def record_run(scene, steps, checkpoints, dt=1.0 / 60, vel_iters=8, pos_iters=3):
-    world = build_ref_world(scene)
+_toi_events = [0]
+
+
+def _count_toi_events():
+    """Counts Island.solve_toi calls (one per TOI event) for the recording; the reference's own
+    method runs unchanged."""
+    from pypybox2d import island as b2island
+    if getattr(b2island.Island.solve_toi, "_counted", False):
+        return
+    orig = b2island.Island.solve_toi
+
+    def counted(self, *a, **k):
+        _toi_events[0] += 1
+        return orig(self, *a, **k)
+    counted._counted = True
+    b2island.Island.solve_toi = counted
+
+
+def record_run(scene, steps, checkpoints, dt=1.0 / 60, vel_iters=8, pos_iters=3, continuous=False, stats=None):
+    """The run's initial state, each step's new contacts in the reference's creation order (hints)
+    and the states at the checkpoints.  With continuous physics, `stats` receives per step the
+    time_of_impact calls and the TOI events (sub-step island solves)."""
+    world = build_ref_world(scene, continuous_physics=continuous)
     init, fidx = snapshot(world)
     hints = []
     states = {}
+    if continuous:
+        from pypybox2d import toi as b2toi
+        _count_toi_events()
     for s in range(1, steps + 1):
         prev = list(world.contact_manager.contacts)  # hold refs: no address reuse
         prev_ids = set(id(c) for c in prev)
+        if continuous:
+            calls0, events0 = b2toi.toi_calls, _toi_events[0]
         world.step(dt, vel_iters, pos_iters)
+        if continuous and stats is not None:
+            stats.append((b2toi.toi_calls - calls0, _toi_events[0] - events0))
         new = [c for c in world.contact_manager.contacts if id(c) not in prev_ids]
         hints.append([(fidx[id(c._fixture_a)], fidx[id(c._fixture_b)]) for c in new])
         if s in checkpoints:
@@ -337,6 +366,49 @@ def record_listener_run(scene, steps, checkpoints):
     return (init, hints, states, edit_states, np.array(edits, np.int32), off,
             np.array([r[1] for r in rows], np.int32).reshape(-1, L.EV_NI),
             np.array([r[2] for r in rows]).reshape(-1, L.EV_ND))
+
+
+# --------------------------------------------------------------------------
+# continuous physics: what World.step does with it on
+# --------------------------------------------------------------------------
+def record_continuous_run(scene, steps):
+    """World.step with continuous_physics on (the reference's default), `steps` times, each call
+    caught: the exception it raises (type name and message, or empty), then the world's state,
+    _inv_dt0 and _locked; the body states of the same world stepped with continuous physics off;
+    and the same for a world without bodies."""
+    world = build_ref_world(scene, continuous_physics=True)
+    for b in world.bodies:
+        b._force = b2.Vec2(1.5, -2.0)  # shows whether the step clears forces
+    init, _ = snapshot(world)
+    out = {"init_" + n: getattr(init, n) for n in L._FIELDS}
+    for s in range(1, steps + 1):
+        err = ""
+        try:
+            world.step(1.0 / 60, 8, 3)
+        except Exception as e:  # recorded, not handled
+            err = "%s: %s" % (type(e).__name__, e)
+        st, _ = snapshot(world)
+        for n in ("body_d", "body_i", "fix_d", "con_d", "con_i", "move"):
+            out["s%d_%s" % (s, n)] = getattr(st, n)
+        out["s%d_error" % s] = np.array(err)
+        out["s%d_inv_dt0" % s] = np.array(world._inv_dt0)
+        out["s%d_locked" % s] = np.array(bool(world._locked))
+    twin = build_ref_world(scene, continuous_physics=False)  # the same world, discrete
+    for b in twin.bodies:
+        b._force = b2.Vec2(1.5, -2.0)
+    for s in range(1, steps + 1):
+        twin.step(1.0 / 60, 8, 3)
+        out["d%d_body_d" % s] = snapshot(twin)[0].body_d
+    empty = b2.World((0, -10), True)
+    err = ""
+    try:
+        empty.step(1.0 / 60, 8, 3)
+    except Exception as e:
+        err = "%s: %s" % (type(e).__name__, e)
+    out["empty_error"] = np.array(err)
+    out["empty_inv_dt0"] = np.array(empty._inv_dt0)
+    out["meta_steps"] = np.array(steps)
+    return out
 
 
 # --------------------------------------------------------------------------
@@ -536,7 +608,8 @@ def world_queries(scene, steps, rng, nq=200, nr=200):
 
 def main(which=None):
     """Records every fixture, or only those named (pyramid, stacks, mixed, filtered, queries, narrowphase)."""
-    want = set(which or ("pyramid", "stacks", "mixed", "filtered", "listeners", "queries", "narrowphase"))
+    want = set(which or ("pyramid", "stacks", "mixed", "filtered", "listeners", "continuous", "queries",
+                         "narrowphase"))
     if "pyramid" in want:
         pyr_ck = {1, 2, 3, 4, 5, 10, 20, 50, 100, 150, 200, 300, 400, 500, 600}
         init, hints, states = record_run(scenes.pyramid(), 600, pyr_ck)
@@ -568,6 +641,21 @@ def main(which=None):
         z.update(edits=edits, ev_off=ev_off, ev_i=ev_i, ev_d=ev_d)
         np.savez_compressed(path, **z)
         print("listener run:", len(ev_i), "calls,", len(edits), "filter edits")
+    if "continuous" in want:
+        out = record_continuous_run(scenes.pyramid(rows=4), 3)
+        np.savez_compressed(os.path.join(HERE, "continuous_pyramid4.npz"), **out)
+        print("continuous physics:", [str(out["s%d_error" % s]) for s in (1, 2, 3)], str(out["empty_error"]))
+        for name, scene, steps in (("projectiles", scenes.projectiles(), 150), ("pyramid", scenes.pyramid(), 200)):
+            stats = []
+            ck = {1, 2, 3, 5, 10, 20, 30, 50, 75, 100, 125, 150, 200}
+            init, hints, states = record_run(scene, steps, {c for c in ck if c <= steps}, continuous=True, stats=stats)
+            path = os.path.join(HERE, "continuous_%s_%d.npz" % (name, steps))
+            save_run(path, init, hints, states, {"steps": steps, "continuous": 1})
+            z = dict(np.load(path))
+            z["toi_stats"] = np.array(stats, np.int64).reshape(-1, 2)
+            np.savez_compressed(path, **z)
+            print(name, "toi calls", int(z["toi_stats"][:, 0].sum()), "toi events", int(z["toi_stats"][:, 1].sum()),
+                  "steps with events", int(np.count_nonzero(z["toi_stats"][:, 1])))
     if "queries" in want:
         np.savez_compressed(os.path.join(HERE, "shape_raycast.npz"), **shape_ray_cases(np.random.default_rng(17)))
         rng = np.random.default_rng(19)

@@ -20,10 +20,15 @@ def _pyramid_world(**kw):
     return world
 
 
-def test_continuous_physics_is_rejected_before_device_work():
+def test_continuous_physics_with_a_custom_filter_or_sub_stepping_is_rejected_before_device_work():
     world = _pyramid_world(continuous_physics=True)
+    world.contact_filter = lambda fa, fb: True
     with pytest.raises(NotImplementedError):
         world.step(1.0 / 60, 8, 3)
+    world = _pyramid_world(continuous_physics=True, sub_stepping=True)
+    with pytest.raises(NotImplementedError):
+        world.step(1.0 / 60, 8, 3)
+    assert _pyramid_world()._batch().world_i[0, b2.layout.WI_FLAGS] & b2.layout.WF_CONTINUOUS
 
 
 def test_sensor_is_rejected():
