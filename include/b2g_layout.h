@@ -158,8 +158,101 @@ enum {
   B2G_EV_ND = 24
 };
 
+/* ---- joint records: int32 x B2G_JOINT_NI, float64 x B2G_JOINT_ND ----
+   One joint of World.joints (world.py:252-283), in the world's joint-list order, which is
+   also the order of each body's joint list (Body.joints: add_joint appends to body_a's list,
+   then body_b's).  The double record holds the joint object's numbers: its parameters,
+   its accumulated impulses and the values its solver computes each step (the reference
+   keeps them as attributes; some are read in later steps or by the reaction getters). */
+#define B2G_JT_REVOLUTE 1  /* joints/revolute.py */
+#define B2G_JT_PRISMATIC 2 /* joints/prismatic.py */
+#define B2G_JT_DISTANCE 3  /* joints/distance.py */
+#define B2G_JT_PULLEY 4    /* joints/pulley.py */
+#define B2G_JT_MOUSE 5     /* joints/mouse.py: body_a is none (-1) */
+#define B2G_JT_GEAR 6      /* joints/gear.py: bodies a, b, c, d as the constructor sets them */
+#define B2G_JT_WHEEL 7     /* joints/wheel.py */
+#define B2G_JT_WELD 8      /* joints/weld.py */
+#define B2G_JT_FRICTION 9  /* joints/friction.py */
+#define B2G_JT_ROPE 10     /* joints/rope.py */
+enum {
+  B2G_JI_TYPE = 0, B2G_JI_BODY_A, B2G_JI_BODY_B, /* world-local body indices, -1 for none */
+  B2G_JI_FLAGS, B2G_JI_STATE,                    /* limit state (joint.py:35), rope _state */
+  B2G_JI_BODY_C, B2G_JI_BODY_D,                  /* gear joint: bodies c and d */
+  B2G_JI_PAD,
+  B2G_JOINT_NI = 8
+};
+#define B2G_JF_COLLIDE_CONNECTED 1u
+#define B2G_JF_LIMIT 2u      /* limit_enabled */
+#define B2G_JF_MOTOR 4u      /* motor_enabled */
+#define B2G_JF_REVOLUTE_A 8u /* gear: joint_a is a revolute joint */
+#define B2G_JF_REVOLUTE_B 16u
+/* limit states (joint.py:35) */
+#define B2G_LIMIT_INACTIVE 0
+#define B2G_LIMIT_AT_LOWER 1
+#define B2G_LIMIT_AT_UPPER 2
+#define B2G_LIMIT_EQUAL 3
+enum {
+  /* parameters and accumulated impulses */
+  B2G_JD_LA_X = 0, B2G_JD_LA_Y, B2G_JD_LB_X, B2G_JD_LB_Y, /* local anchors a, b */
+  B2G_JD_REF,                      /* reference angle; gear: reference_angle_a */
+  B2G_JD_IMP_X, B2G_JD_IMP_Y, B2G_JD_IMP_Z,
+  /* impulse: Vec3 (revolute, prismatic, weld), Vec2 (mouse), scalar in X (distance, pulley,
+     gear, wheel, rope); friction: linear impulse in X, Y and angular impulse in Z */
+  B2G_JD_MOTOR_IMP,                /* 8: revolute, prismatic, wheel */
+  B2G_JD_MOTOR_SPEED,
+  B2G_JD_MAX_MOTOR,                /* 10: max motor torque (revolute, wheel), force (prismatic);
+                                      max_force (friction, mouse) */
+  B2G_JD_LOWER, B2G_JD_UPPER,      /* limits */
+  B2G_JD_AXIS_X, B2G_JD_AXIS_Y,    /* 13: local x axis (prismatic, wheel); gear: local_axis_c */
+  B2G_JD_PERP_X, B2G_JD_PERP_Y,    /* 15: local y axis (prismatic, wheel); gear: local_axis_d */
+  B2G_JD_FREQ, B2G_JD_DAMP,        /* 17: frequency, damping ratio (distance, wheel, mouse) */
+  B2G_JD_LENGTH,                   /* 19: distance length, rope max length; pulley, gear constant */
+  B2G_JD_RATIO,                    /* 20: pulley, gear */
+  B2G_JD_G0_X, B2G_JD_G0_Y, B2G_JD_G1_X, B2G_JD_G1_Y,
+  /* 21: pulley ground anchors a, b; mouse target (G0); gear local anchors c, d */
+  B2G_JD_SPRING_IMP,               /* 25: wheel */
+  B2G_JD_MAX_TORQUE,               /* 26: friction */
+  B2G_JD_REF_B,                    /* 27: gear reference_angle_b */
+  B2G_JD_LEN_A, B2G_JD_LEN_B,      /* 28: pulley length_a, length_b */
+  /* the solver's values of the last step */
+  B2G_JD_RA_X = 32, B2G_JD_RA_Y, B2G_JD_RB_X, B2G_JD_RB_Y, /* lever arms _ra, _rb */
+  B2G_JD_S = 36,                   /* per-type values: */
+  /* revolute: _mass (Mat33 by columns), _motor_mass */
+  B2G_JD_REV_MASS = B2G_JD_S, B2G_JD_REV_MOTOR_MASS = B2G_JD_S + 9,
+  /* prismatic: _axis, _perp, _a1, _a2, _s1, _s2, _motor_mass, _k (Mat33 by columns) */
+  B2G_JD_PRI_AXIS_X = B2G_JD_S, B2G_JD_PRI_AXIS_Y, B2G_JD_PRI_PERP_X, B2G_JD_PRI_PERP_Y,
+  B2G_JD_PRI_A1, B2G_JD_PRI_A2, B2G_JD_PRI_S1, B2G_JD_PRI_S2, B2G_JD_PRI_MOTOR_MASS, B2G_JD_PRI_K,
+  /* distance: _u, _mass, _gamma, _bias */
+  B2G_JD_DIS_U_X = B2G_JD_S, B2G_JD_DIS_U_Y, B2G_JD_DIS_MASS, B2G_JD_DIS_GAMMA, B2G_JD_DIS_BIAS,
+  /* pulley: _ua, _ub, _mass */
+  B2G_JD_PUL_UA_X = B2G_JD_S, B2G_JD_PUL_UA_Y, B2G_JD_PUL_UB_X, B2G_JD_PUL_UB_Y, B2G_JD_PUL_MASS,
+  /* mouse: _mass (Mat22 by columns), _gamma, _beta, _c */
+  B2G_JD_MOU_MASS = B2G_JD_S, B2G_JD_MOU_GAMMA = B2G_JD_S + 4, B2G_JD_MOU_BETA, B2G_JD_MOU_C_X,
+  B2G_JD_MOU_C_Y,
+  /* wheel: _ax, _ay, _s_ax, _s_bx, _s_ay, _s_by, _mass, _motor_mass, _spring_mass, _bias, _gamma */
+  B2G_JD_WHE_AX_X = B2G_JD_S, B2G_JD_WHE_AX_Y, B2G_JD_WHE_AY_X, B2G_JD_WHE_AY_Y, B2G_JD_WHE_S_AX,
+  B2G_JD_WHE_S_BX, B2G_JD_WHE_S_AY, B2G_JD_WHE_S_BY, B2G_JD_WHE_MASS, B2G_JD_WHE_MOTOR_MASS,
+  B2G_JD_WHE_SPRING_MASS, B2G_JD_WHE_BIAS, B2G_JD_WHE_GAMMA,
+  /* weld: _mass (Mat33 by columns) */
+  B2G_JD_WEL_MASS = B2G_JD_S,
+  /* friction: _linear_mass (Mat22 by columns), _angular_mass */
+  B2G_JD_FRI_LMASS = B2G_JD_S, B2G_JD_FRI_AMASS = B2G_JD_S + 4,
+  /* rope: _u, _length, _mass */
+  B2G_JD_ROP_U_X = B2G_JD_S, B2G_JD_ROP_U_Y, B2G_JD_ROP_LENGTH, B2G_JD_ROP_MASS,
+  /* gear (uses the lever-arm slots too): _j_vac, _j_vbd, _j_wa, _j_wb, _j_wc, _j_wd, _mass,
+     then the body states it captured at its velocity-constraint setup: linear velocities,
+     centres and angles of bodies a, b, c, d */
+  B2G_JD_GEA_JVAC_X = B2G_JD_RA_X, B2G_JD_GEA_JVAC_Y, B2G_JD_GEA_JVBD_X, B2G_JD_GEA_JVBD_Y,
+  B2G_JD_GEA_JWA, B2G_JD_GEA_JWB, B2G_JD_GEA_JWC, B2G_JD_GEA_JWD, B2G_JD_GEA_MASS,
+  B2G_JD_GEA_V = B2G_JD_GEA_MASS + 1, /* 4 x (x, y) */
+  B2G_JD_GEA_C = B2G_JD_GEA_V + 8,    /* 4 x (x, y) */
+  B2G_JD_GEA_A = B2G_JD_GEA_C + 8,    /* 4 */
+  B2G_JOINT_ND = 64
+};
+
 /* A batch description.  Every pointer is a caller-owned host array; sizes are
-   given by the offset arrays (each of length n_worlds + 1). */
+   given by the offset arrays (each of length n_worlds + 1).  The joint arrays
+   may be NULL (no joints). */
 typedef struct b2g_batch {
   int32_t n_worlds;
   const int32_t *body_off, *fix_off, *con_off, *move_off;
@@ -172,6 +265,9 @@ typedef struct b2g_batch {
   const double *con_d;    /* n_contacts x B2G_CON_ND */
   const int32_t *con_i;   /* n_contacts x B2G_CON_NI */
   const int32_t *move;    /* broad-phase move buffer, world-local fixture indices */
+  const int32_t *joint_off; /* n_worlds + 1, or NULL */
+  const double *joint_d;    /* n_joints x B2G_JOINT_ND */
+  const int32_t *joint_i;   /* n_joints x B2G_JOINT_NI */
 } b2g_batch;
 
 /* Output view of a batch: the same arrays, writable.  Body/fixture counts do
@@ -190,6 +286,8 @@ typedef struct b2g_batch_out {
   double *con_d;
   int32_t *con_i;
   int32_t *move;
+  double *joint_d;  /* n_joints x B2G_JOINT_ND (joints do not change during stepping), or NULL */
+  int32_t *joint_i; /* n_joints x B2G_JOINT_NI, or NULL */
 } b2g_batch_out;
 
 #endif /* B2G_LAYOUT_H */

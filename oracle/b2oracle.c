@@ -33,8 +33,12 @@
  * contact_filter is given as its verdict for every ordered fixture pair
  * (orc_set_filter).
  *
+ * Joints (pypybox2d/joints/) are solved in the island's Gauss-Seidel order
+ * (island.py:107-148), their quirks reproduced where the reference has them
+ * (cited at each joint type).
+ *
  * Not supported (reported through the error word): sensors, loop shapes,
- * joints, controllers, continuous physics.
+ * controllers.
  */
 #include <float.h>
 #include <math.h>
@@ -60,6 +64,7 @@ static const double MAX_LINEAR_CORRECTION = 0.2;
 static const double MAX_TRANSLATION = 2.0;
 static const double BAUMGARTE = 0.2;
 static const double TIME_TO_SLEEP = 0.5;
+static double MAX_ANGULAR_CORRECTION;
 static double ANGULAR_SLOP, POLYGON_RADIUS, MAX_ROTATION, MAX_ROTATION_SQR,
     MAX_TRANSLATION_SQR, LINEAR_SLEEP_TOLERANCE_SQR, ANGULAR_SLEEP_TOLERANCE_SQR,
     EPSILON_SQR;
@@ -78,6 +83,7 @@ static int g_init = 0;
 static void init_consts(void) {
   if (g_init) return;
   ANGULAR_SLOP = 2.0 / 180.0 * PI_;        /* settings.py:65 */
+  MAX_ANGULAR_CORRECTION = 8.0 / 180.0 * PI_; /* settings.py:94 */
   POLYGON_RADIUS = 2.0 * LINEAR_SLOP;      /* settings.py:70 */
   MAX_TRANSLATION_SQR = pysq(MAX_TRANSLATION);
   MAX_ROTATION = 0.5 * PI_;                /* settings.py:103 */
@@ -234,6 +240,13 @@ typedef struct {
   int toi_flag, toi_count, toi_island;
 } Contact;
 
+/* a joint: its record (b2g_layout.h B2G_JI_* / B2G_JD_*) stands for the joint object */
+typedef struct {
+  int32_t i[B2G_JOINT_NI];
+  double d[B2G_JOINT_ND];
+  int island_flag;
+} Joint;
+
 typedef struct {
   V2 gravity;
   double inv_dt0;
@@ -270,11 +283,18 @@ typedef struct {
      island solves) in the last step */
   int continuous, toi_mode, toi_seq;
   long long toi_calls, toi_events;
+  /* joints in World.joints order; per-body joint lists (Body.joints: the world's joints
+     touching the body, in order, a joint listed twice when both its ends are the body) */
+  int nj;
+  Joint* j;
+  int* bj_off;
+  int* bj;
 } World;
 
 #define ORC_E_UNSUPPORTED 1
 #define ORC_E_HINT 2
 #define ORC_E_ALLOC 4
+#define ORC_E_JOINT 16 /* a mouse joint's spring was too weak (the reference asserts, mouse.py:173) */
 
 typedef struct { int nw; World* w; } Batch;
 
@@ -905,6 +925,18 @@ static int ev_push(World* w, int kind, int phase, int key, const Contact* c) {
   return k;
 }
 
+/* Body._should_collide's joint test (body.py:920-926): a joint between the two bodies that does
+   not let them collide */
+static int joint_blocks(const World* w, int ba, int bb) {
+  for (int k = w->bj_off[bb]; k < w->bj_off[bb + 1]; ++k) {
+    const Joint* J = &w->j[w->bj[k]];
+    int a = J->i[B2G_JI_BODY_A], b = J->i[B2G_JI_BODY_B];
+    if ((a == bb && b == ba) || (b == bb && a == ba))
+      if (!(J->i[B2G_JI_FLAGS] & B2G_JF_COLLIDE_CONNECTED)) return 1;
+  }
+  return 0;
+}
+
 /* contact_filter(fixture a, fixture b): the custom filter's verdict, or World.default_contact_filter
    (world.py:958-970) */
 static int filter_ok(const World* w, int a, int b) {
@@ -932,8 +964,9 @@ static void find_new_contacts(World* w) { /* contact_manager.py:102-158 */
     int ba = w->f[fa].body, bb = w->f[fb].body;
     if (ba == bb) continue;
     if (hs_has(&exist, ukey(fa, fb))) continue;
-    /* Body._should_collide (body.py:910-928), no joints */
+    /* body_b._should_collide(body_a) (body.py:910-928) */
     if (w->b[bb].type != B2G_DYNAMIC && w->b[ba].type != B2G_DYNAMIC) continue;
+    if (joint_blocks(w, ba, bb)) continue;
     /* the contact filter (contact_manager.py:144-146) */
     const Fix *A = &w->f[fa], *B = &w->f[fb];
     if (!filter_ok(w, fa, fb)) continue;
@@ -1043,6 +1076,7 @@ static void collide(World* w) { /* contact_manager.py:174-223 */
     if (!active_a && !active_b) continue;
     if (c->filter) {
       int ok = (A->type == B2G_DYNAMIC || B->type == B2G_DYNAMIC);
+      if (ok) ok = !joint_blocks(w, FA->body, FB->body);
       if (ok) ok = filter_ok(w, c->fa, c->fb);
       if (!ok) { remove[i] = 1; ++nremove; continue; }
       c->filter = 0;
@@ -1261,6 +1295,7 @@ static int solve_position(PC* pcs, int npc, PV* pv) { /* contact.py:809-864 */
 typedef struct {
   int* bodies; int nbodies;
   int* contacts; int ncontacts;
+  int* joints; int njoints;
   PV* pv; VC* vc; PC* pc;
 } IslandBuf;
 
@@ -1377,6 +1412,1325 @@ static void cs_report(World* w, IslandBuf* is, int phase, int key0) {
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* Joints (pypybox2d/joints/): velocity-constraint setup, velocity and  */
+/* position solves, called by Island.solve (island.py:107-148).  Each  */
+/* joint's numbers live in its record (b2g_layout.h B2G_JD_*), which   */
+/* stands for the joint object's attributes: values the reference      */
+/* leaves stale between steps stay stale here too.                     */
+/* ------------------------------------------------------------------ */
+typedef struct { double x, y, z; } V3;
+typedef struct { V3 c1, c2, c3; } M33;
+static inline V3 v3(double x, double y, double z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
+static inline V3 add3(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline V3 sub3(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline V3 neg3(V3 a) { return v3(-a.x, -a.y, -a.z); }
+static inline V3 scale3(double s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }
+static inline double dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; } /* common.py:342-344 */
+static inline V3 cross3(V3 s, V3 v) {                                              /* common.py:353-358 */
+  return v3(s.y * v.z - s.z * v.y, s.z * v.x - s.x * v.z, s.x * v.y - s.y * v.x);
+}
+static V3 solve33(M33 m, V3 b) { /* Mat33.solve3x3 (common.py:587-599) */
+  double det = dot3(m.c1, cross3(m.c2, m.c3));
+  if (det != 0.0) det = 1.0 / det;
+  return v3(det * dot3(b, cross3(m.c2, m.c3)), det * dot3(m.c1, cross3(b, m.c3)), det * dot3(m.c1, cross3(m.c2, b)));
+}
+static V2 solve22(M33 m, V2 v) { /* Mat33.solve2x2 (common.py:601-614) */
+  double a11 = m.c1.x, a12 = m.c2.x, a21 = m.c1.y, a22 = m.c2.y;
+  double det = a11 * a22 - a12 * a21;
+  if (det != 0.0) det = 1.0 / det;
+  return v2(det * (a22 * v.x - a12 * v.y), det * (a11 * v.y - a21 * v.x));
+}
+static V2 m22_solve(M22 m, V2 v) { /* Mat22.solve (common.py:521-531) */
+  double a = m.c1x, b = m.c2x, c = m.c1y, d = m.c2y;
+  double det = a * d - b * c;
+  if (det != 0.0) det = 1.0 / det;
+  return v2(det * (d * v.x - b * v.y), det * (a * v.y - c * v.x));
+}
+static M22 m22_inverse(M22 m) { /* Mat22.inverse (common.py:506-519) */
+  double a = m.c1x, b = m.c2x, c = m.c1y, d = m.c2y;
+  double det = a * d - b * c;
+  if (det != 0.0) det = 1.0 / det;
+  M22 r;
+  r.c1x = det * d; r.c1y = -det * c;
+  r.c2x = -det * b; r.c2y = det * a;
+  return r;
+}
+static inline V2 vdiv(V2 a, double s) { return v2(a.x / s, a.y / s); } /* Vec2.__div__ */
+
+static inline V2 jget2(const Joint* J, int k) { return v2(J->d[k], J->d[k + 1]); }
+static inline void jset2(Joint* J, int k, V2 v) { J->d[k] = v.x; J->d[k + 1] = v.y; }
+static inline V3 jget3(const Joint* J, int k) { return v3(J->d[k], J->d[k + 1], J->d[k + 2]); }
+static inline void jset3(Joint* J, int k, V3 v) { J->d[k] = v.x; J->d[k + 1] = v.y; J->d[k + 2] = v.z; }
+static inline M33 jget33(const Joint* J, int k) {
+  M33 m; m.c1 = jget3(J, k); m.c2 = jget3(J, k + 3); m.c3 = jget3(J, k + 6); return m;
+}
+static inline void jset33(Joint* J, int k, M33 m) { jset3(J, k, m.c1); jset3(J, k + 3, m.c2); jset3(J, k + 6, m.c3); }
+static inline M22 jget22(const Joint* J, int k) {
+  M22 m; m.c1x = J->d[k]; m.c1y = J->d[k + 1]; m.c2x = J->d[k + 2]; m.c2y = J->d[k + 3]; return m;
+}
+static inline void jset22(Joint* J, int k, M22 m) {
+  J->d[k] = m.c1x; J->d[k + 1] = m.c1y; J->d[k + 2] = m.c2x; J->d[k + 3] = m.c2y;
+}
+
+/* the two-body joints' common view of their bodies */
+typedef struct {
+  int ia, ib;               /* island indices */
+  double ma, mb, Ia, Ib;    /* inverse masses and inertias */
+  V2 lca, lcb;              /* local centres */
+} JB;
+static JB joint_bodies(const World* w, const Joint* J) {
+  JB b;
+  const Body* A = &w->b[J->i[B2G_JI_BODY_A]];
+  const Body* B = &w->b[J->i[B2G_JI_BODY_B]];
+  b.ia = A->island_index; b.ib = B->island_index;
+  b.ma = A->inv_mass; b.mb = B->inv_mass;
+  b.Ia = A->invI; b.Ib = B->invI;
+  b.lca = A->lc; b.lcb = B->lc;
+  return b;
+}
+#define JFLAG(J, f) (((J)->i[B2G_JI_FLAGS] & (f)) != 0)
+
+/* ---- revolute (joints/revolute.py:214-453) ---- */
+static void revolute_init(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  jset2(J, B2G_JD_RA_X, ra); jset2(J, B2G_JD_RB_X, rb);
+  M33 m;
+  m.c3 = v3(-ra.y * ia - rb.y * ib, ra.x * ia + rb.x * ib, ia + ib);
+  m.c2 = v3(-ra.y * ra.x * ia - rb.y * rb.x * ib, ma + mb + ra.x * ra.x * ia + rb.x * rb.x * ib, m.c3.y);
+  m.c1 = v3(ma + mb + ra.y * ra.y * ia + rb.y * rb.y * ib, m.c2.x, m.c3.x);
+  jset33(J, B2G_JD_REV_MASS, m);
+  double motor_mass = ia + ib;
+  int fixed_rotation = (motor_mass == 0.0);
+  if (motor_mass > 0.0) motor_mass = 1.0 / motor_mass;
+  J->d[B2G_JD_REV_MOTOR_MASS] = motor_mass;
+  if (!JFLAG(J, B2G_JF_MOTOR) || fixed_rotation) J->d[B2G_JD_MOTOR_IMP] = 0.0;
+  double* imp_z = &J->d[B2G_JD_IMP_Z];
+  int* state = &J->i[B2G_JI_STATE];
+  if (JFLAG(J, B2G_JF_LIMIT) && !fixed_rotation) {
+    double joint_angle = ab - aa - J->d[B2G_JD_REF];
+    double lower = J->d[B2G_JD_LOWER], upper = J->d[B2G_JD_UPPER];
+    if (fabs(upper - lower) < 2.0 * ANGULAR_SLOP) {
+      *state = B2G_LIMIT_EQUAL;
+    } else if (joint_angle <= lower) {
+      if (*state != B2G_LIMIT_AT_LOWER) *imp_z = *imp_z - *imp_z;
+      *state = B2G_LIMIT_AT_LOWER;
+    } else if (joint_angle >= upper) {
+      if (*state != B2G_LIMIT_AT_UPPER) *imp_z = *imp_z - *imp_z;
+      *state = B2G_LIMIT_AT_UPPER;
+    } else {
+      *state = B2G_LIMIT_INACTIVE;
+      *imp_z = *imp_z - *imp_z;
+    }
+  } else {
+    *state = B2G_LIMIT_INACTIVE;
+  }
+  if (st->warm_starting) {
+    V3 imp = scale3(st->dt_ratio, jget3(J, B2G_JD_IMP_X));
+    jset3(J, B2G_JD_IMP_X, imp);
+    double mi = J->d[B2G_JD_MOTOR_IMP] * st->dt_ratio;
+    J->d[B2G_JD_MOTOR_IMP] = mi;
+    V2 P = v2(imp.x, imp.y);
+    va = sub(va, scale(ma, P));
+    wa -= ia * (cross(ra, P) + mi + imp.z);
+    vb = add(vb, scale(mb, P));
+    wb += ib * (cross(rb, P) + mi + imp.z);
+  } else {
+    jset3(J, B2G_JD_IMP_X, v3(0.0, 0.0, 0.0));
+    J->d[B2G_JD_MOTOR_IMP] = 0.0;
+  }
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static void revolute_solve_velocity(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  V2 ra = jget2(J, B2G_JD_RA_X), rb = jget2(J, B2G_JD_RB_X);
+  M33 mass = jget33(J, B2G_JD_REV_MASS);
+  int fixed_rotation = ((ia + ib) == 0.0);
+  int state = J->i[B2G_JI_STATE];
+  if (JFLAG(J, B2G_JF_MOTOR) && state != B2G_LIMIT_EQUAL && !fixed_rotation) {
+    double Cdot = wb - wa - J->d[B2G_JD_MOTOR_SPEED];
+    double impulse = J->d[B2G_JD_REV_MOTOR_MASS] * (-Cdot);
+    double old = J->d[B2G_JD_MOTOR_IMP];
+    double max_impulse = st->dt * J->d[B2G_JD_MAX_MOTOR];
+    J->d[B2G_JD_MOTOR_IMP] = pclamp(old + impulse, -max_impulse, max_impulse);
+    impulse = J->d[B2G_JD_MOTOR_IMP] - old;
+    wa -= ia * impulse;
+    wb += ib * impulse;
+  }
+  V3 acc = jget3(J, B2G_JD_IMP_X);
+  if (JFLAG(J, B2G_JF_LIMIT) && state != B2G_LIMIT_INACTIVE && !fixed_rotation) {
+    V2 Cdot1 = sub(sub(add(vb, cross_sv(wb, rb)), va), cross_sv(wa, ra));
+    double Cdot2 = wb - wa;
+    V3 Cdot = v3(Cdot1.x, Cdot1.y, Cdot2);
+    V3 impulse = neg3(solve33(mass, Cdot));
+    if (state == B2G_LIMIT_EQUAL) {
+      acc = add3(acc, impulse);
+    } else if (state == B2G_LIMIT_AT_LOWER) {
+      double new_impulse = acc.z + impulse.z;
+      if (new_impulse < 0.0) {
+        V2 rhs = add(neg(Cdot1), scale(acc.z, v2(mass.c3.x, mass.c3.y)));
+        V2 reduced = solve22(mass, rhs);
+        impulse = v3(reduced.x, reduced.y, -acc.z);
+        acc = add3(acc, v3(reduced.x, reduced.y, -acc.z));
+      } else {
+        acc = add3(acc, impulse);
+      }
+    } else if (state == B2G_LIMIT_AT_UPPER) {
+      double new_impulse = acc.z + impulse.z;
+      if (new_impulse > 0.0) {
+        V2 rhs = add(neg(Cdot1), scale(acc.z, v2(mass.c3.x, mass.c3.y)));
+        V2 reduced = solve22(mass, rhs);
+        impulse = v3(reduced.x, reduced.y, -acc.z);
+        acc = add3(acc, v3(reduced.x, reduced.y, -acc.z));
+      } else {
+        acc = add3(acc, impulse);
+      }
+    }
+    V2 P = v2(impulse.x, impulse.y);
+    va = sub(va, scale(ma, P));
+    wa -= ia * (cross(ra, P) + impulse.z);
+    vb = add(vb, scale(mb, P));
+    wb += ib * (cross(rb, P) + impulse.z);
+  } else {
+    V2 Cdot = sub(sub(add(vb, cross_sv(wb, rb)), va), cross_sv(wa, ra));
+    V2 impulse = solve22(mass, neg(Cdot));
+    acc = add3(acc, v3(impulse.x, impulse.y, 0.0));
+    va = sub(va, scale(ma, impulse));
+    wa -= ia * cross(ra, impulse);
+    vb = add(vb, scale(mb, impulse));
+    wb += ib * cross(rb, impulse);
+  }
+  jset3(J, B2G_JD_IMP_X, acc);
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static int revolute_solve_position(World* w, Joint* J, PV* pv) {
+  JB b = joint_bodies(w, J);
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  int fixed_rotation = ((ia + ib) == 0.0);
+  double angular_error = 0.0, position_error = 0.0;
+  int state = J->i[B2G_JI_STATE];
+  if (JFLAG(J, B2G_JF_LIMIT) && state != B2G_LIMIT_INACTIVE && !fixed_rotation) {
+    double angle = ab - aa - J->d[B2G_JD_REF];
+    double limit_impulse = 0.0, C;
+    double motor_mass = J->d[B2G_JD_REV_MOTOR_MASS];
+    if (state == B2G_LIMIT_EQUAL) {
+      C = pclamp(angle - J->d[B2G_JD_LOWER], -MAX_ANGULAR_CORRECTION, MAX_ANGULAR_CORRECTION);
+      limit_impulse = -motor_mass * C;
+      angular_error = fabs(C);
+    } else if (state == B2G_LIMIT_AT_LOWER) {
+      C = angle - J->d[B2G_JD_LOWER];
+      angular_error = -C;
+      C = pclamp(C + ANGULAR_SLOP, -MAX_ANGULAR_CORRECTION, 0.0);
+      limit_impulse = -motor_mass * C;
+    } else if (state == B2G_LIMIT_AT_UPPER) {
+      C = angle - J->d[B2G_JD_UPPER];
+      angular_error = C;
+      C = pclamp(C - ANGULAR_SLOP, 0.0, MAX_ANGULAR_CORRECTION);
+      limit_impulse = -motor_mass * C;
+    }
+    aa -= ia * limit_impulse;
+    ab += ib * limit_impulse;
+  }
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  V2 C = sub(sub(add(cb, rb), ca), ra);
+  position_error = length_(C);
+  double rax = ra.x, ray = ra.y, rbx = rb.x, rby = rb.y;
+  M22 K;
+  K.c1x = ma + mb + ia * ray * ray + ib * rby * rby;
+  K.c1y = -ia * rax * ray - ib * rbx * rby;
+  K.c2x = K.c1y;
+  K.c2y = ma + mb + ia * rax * rax + ib * rbx * rbx;
+  V2 impulse = neg(m22_solve(K, C));
+  ca = sub(ca, scale(ma, impulse));
+  aa -= ia * cross(ra, impulse);
+  cb = add(cb, scale(mb, impulse));
+  ab += ib * cross(rb, impulse);
+  pv[b.ia].c = ca; pv[b.ia].a = aa;
+  pv[b.ib].c = cb; pv[b.ib].a = ab;
+  return position_error <= LINEAR_SLOP && angular_error <= ANGULAR_SLOP;
+}
+
+/* ---- prismatic (joints/prismatic.py:271-513) ---- */
+static M33 prismatic_k(double ma, double mb, double ia, double ib, double s1, double s2, double a1, double a2) {
+  double k11 = ma + mb + ia * pysq(s1) + ib * pysq(s2);
+  double k12 = ia * s1 + ib * s2;
+  double k13 = ia * s1 * a1 + ib * s2 * a2;
+  double k22 = ia + ib;
+  if (k22 == 0.0) k22 = 1.0;
+  double k23 = ia * a1 + ib * a2;
+  double k33 = ma + mb + ia * pysq(a1) + ib * pysq(a2);
+  M33 k;
+  k.c1 = v3(k11, k12, k13);
+  k.c2 = v3(k12, k22, k23);
+  k.c3 = v3(k13, k23, k33);
+  return k;
+}
+
+static void prismatic_init(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  jset2(J, B2G_JD_RA_X, ra); jset2(J, B2G_JD_RB_X, rb);
+  V2 d = sub(sub(add(cb, rb), ca), ra);
+  V2 axis = mulmv(qa, jget2(J, B2G_JD_AXIS_X));
+  jset2(J, B2G_JD_PRI_AXIS_X, axis);
+  double a1 = cross(add(d, ra), axis), a2 = cross(rb, axis);
+  J->d[B2G_JD_PRI_A1] = a1; J->d[B2G_JD_PRI_A2] = a2;
+  double motor_mass = ma + mb + ia * pysq(a1) + ib * pysq(a2);
+  if (motor_mass > 0.0) motor_mass = 1.0 / motor_mass;
+  J->d[B2G_JD_PRI_MOTOR_MASS] = motor_mass;
+  V2 perp = mulmv(qa, jget2(J, B2G_JD_PERP_X));
+  jset2(J, B2G_JD_PRI_PERP_X, perp);
+  double s1 = cross(add(d, ra), perp), s2 = cross(rb, perp);
+  J->d[B2G_JD_PRI_S1] = s1; J->d[B2G_JD_PRI_S2] = s2;
+  jset33(J, B2G_JD_PRI_K, prismatic_k(ma, mb, ia, ib, s1, s2, a1, a2));
+  double* imp_z = &J->d[B2G_JD_IMP_Z];
+  int* state = &J->i[B2G_JI_STATE];
+  if (JFLAG(J, B2G_JF_LIMIT)) {
+    double jt = dot(axis, d);
+    double lower = J->d[B2G_JD_LOWER], upper = J->d[B2G_JD_UPPER];
+    if (fabs(upper - lower) < 2.0 * LINEAR_SLOP) {
+      *state = B2G_LIMIT_EQUAL;
+    } else if (jt <= lower) {
+      if (*state != B2G_LIMIT_AT_LOWER) { *state = B2G_LIMIT_AT_LOWER; *imp_z = *imp_z - *imp_z; }
+    } else if (jt >= upper) {
+      if (*state != B2G_LIMIT_AT_UPPER) { *state = B2G_LIMIT_AT_UPPER; *imp_z = *imp_z - *imp_z; }
+    } else {
+      *state = B2G_LIMIT_INACTIVE;
+      *imp_z = *imp_z - *imp_z;
+    }
+  } else {
+    *state = B2G_LIMIT_INACTIVE;
+    *imp_z = *imp_z - *imp_z;
+  }
+  if (!JFLAG(J, B2G_JF_MOTOR)) J->d[B2G_JD_MOTOR_IMP] = 0.0;
+  if (st->warm_starting) {
+    V3 imp = scale3(st->dt_ratio, jget3(J, B2G_JD_IMP_X));
+    jset3(J, B2G_JD_IMP_X, imp);
+    double mi = J->d[B2G_JD_MOTOR_IMP] * st->dt_ratio;
+    J->d[B2G_JD_MOTOR_IMP] = mi;
+    V2 P = add(scale(imp.x, perp), scale(mi + imp.z, axis));
+    double L1 = imp.x * s1 + imp.y + (mi + imp.z) * a1;
+    double L2 = imp.x * s2 + imp.y + (mi + imp.z) * a2;
+    va = sub(va, scale(ma, P));
+    wa -= ia * L1;
+    vb = add(vb, scale(mb, P));
+    wb += ib * L2;
+  } else {
+    jset3(J, B2G_JD_IMP_X, v3(0.0, 0.0, 0.0));
+    J->d[B2G_JD_MOTOR_IMP] = 0.0;
+  }
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static void prismatic_solve_velocity(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  double a1 = J->d[B2G_JD_PRI_A1], a2 = J->d[B2G_JD_PRI_A2];
+  double s1 = J->d[B2G_JD_PRI_S1], s2 = J->d[B2G_JD_PRI_S2];
+  V2 perp = jget2(J, B2G_JD_PRI_PERP_X), axis = jget2(J, B2G_JD_PRI_AXIS_X);
+  M33 k = jget33(J, B2G_JD_PRI_K);
+  int state = J->i[B2G_JI_STATE];
+  if (JFLAG(J, B2G_JF_MOTOR) && state != B2G_LIMIT_EQUAL) {
+    double Cdot = dot(axis, sub(vb, va)) + a2 * wb - a1 * wa;
+    double impulse = J->d[B2G_JD_PRI_MOTOR_MASS] * (J->d[B2G_JD_MOTOR_SPEED] - Cdot);
+    double old = J->d[B2G_JD_MOTOR_IMP];
+    double max_impulse = st->dt * J->d[B2G_JD_MAX_MOTOR];
+    J->d[B2G_JD_MOTOR_IMP] = pclamp(old + impulse, -max_impulse, max_impulse);
+    impulse = J->d[B2G_JD_MOTOR_IMP] - old;
+    V2 P = scale(impulse, axis);
+    double L1 = impulse * a1, L2 = impulse * a2;
+    va = sub(va, scale(ma, P));
+    wa -= ia * L1;
+    vb = add(vb, scale(mb, P));
+    wb += ib * L2;
+  }
+  V2 Cdot1 = v2(dot(perp, sub(vb, va)) + s2 * wb - s1 * wa, wb - wa);
+  V2 P;
+  double L1, L2;
+  V3 acc = jget3(J, B2G_JD_IMP_X);
+  if (JFLAG(J, B2G_JF_LIMIT) && state != B2G_LIMIT_INACTIVE) {
+    double Cdot2 = dot(axis, sub(vb, va)) + a2 * wb - a1 * wa;
+    V3 Cdot = v3(Cdot1.x, Cdot1.y, Cdot2);
+    V3 old = acc;
+    V3 df = solve33(k, neg3(Cdot));
+    acc = add3(acc, df);
+    double iz;
+    if (state == B2G_LIMIT_AT_LOWER) iz = pmax(acc.z, 0.0);
+    else if (state == B2G_LIMIT_AT_UPPER) iz = pmin(acc.z, 0.0);
+    else iz = acc.z;
+    V2 bb = sub(neg(Cdot1), scale(acc.z - old.z, v2(k.c3.x, k.c3.y)));
+    V2 f2r = add(solve22(k, bb), v2(old.x, old.y));
+    acc = v3(f2r.x, f2r.y, iz);
+    df = sub3(acc, old);
+    P = add(scale(df.x, perp), scale(df.z, axis));
+    L1 = df.x * s1 + df.y + df.z * a1;
+    L2 = df.x * s2 + df.y + df.z * a2;
+  } else {
+    V2 df = solve22(k, neg(Cdot1));
+    acc = add3(acc, v3(df.x, df.y, 0.0));
+    P = scale(df.x, perp);
+    L1 = df.x * s1 + df.y;
+    L2 = df.x * s2 + df.y;
+  }
+  jset3(J, B2G_JD_IMP_X, acc);
+  va = sub(va, scale(ma, P));
+  wa -= ia * L1;
+  vb = add(vb, scale(mb, P));
+  wb += ib * L2;
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static int prismatic_solve_position(World* w, Joint* J, PV* pv) {
+  JB b = joint_bodies(w, J);
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  V2 d = sub(sub(add(cb, rb), ca), ra);
+  V2 axis = mulmv(qa, jget2(J, B2G_JD_AXIS_X));
+  double a1 = cross(add(d, ra), axis), a2 = cross(rb, axis);
+  V2 perp = mulmv(qa, jget2(J, B2G_JD_PERP_X));
+  double s1 = cross(add(d, ra), perp), s2 = cross(rb, perp);
+  V2 C1 = v2(dot(perp, d), ab - aa - J->d[B2G_JD_REF]);
+  double linear_error = fabs(C1.x), angular_error = fabs(C1.y);
+  int active = 0;
+  double C2 = 0.0;
+  if (JFLAG(J, B2G_JF_LIMIT)) {
+    double translation = dot(axis, d);
+    double lower = J->d[B2G_JD_LOWER], upper = J->d[B2G_JD_UPPER];
+    if (fabs(upper - lower) < 2.0 * LINEAR_SLOP) {
+      C2 = pclamp(translation, -MAX_LINEAR_CORRECTION, MAX_LINEAR_CORRECTION);
+      linear_error = pmax(linear_error, translation);
+      active = 1;
+    } else if (translation <= lower) {
+      C2 = pclamp(translation - lower + LINEAR_SLOP, -MAX_LINEAR_CORRECTION, 0.0);
+      linear_error = pmax(linear_error, lower - translation);
+      active = 1;
+    } else if (translation >= upper) {
+      C2 = pclamp(translation - upper - LINEAR_SLOP, 0.0, MAX_LINEAR_CORRECTION);
+      linear_error = pmax(linear_error, translation - upper);
+      active = 1;
+    }
+  }
+  V3 impulse;
+  if (active) {
+    M33 k = prismatic_k(ma, mb, ia, ib, s1, s2, a1, a2);
+    jset33(J, B2G_JD_PRI_K, k);
+    impulse = solve33(k, neg3(v3(C1.x, C1.y, C2)));
+  } else {
+    double k11 = ma + mb + ia * s1 * s1 + ib * s2 * s2;
+    double k12 = ia * s1 + ib * s2;
+    double k22 = ia + ib;
+    if (k22 == 0.0) k22 = 1.0;
+    M33 k = jget33(J, B2G_JD_PRI_K);
+    k.c1 = v3(k11, k12, 0.0);
+    k.c2 = v3(k12, k22, 0.0);
+    jset33(J, B2G_JD_PRI_K, k);
+    V2 impulse1 = solve22(k, neg(C1));
+    impulse = v3(impulse1.x, impulse1.y, 0.0);
+  }
+  V2 P = add(scale(impulse.x, perp), scale(impulse.z, axis));
+  double L1 = impulse.x * s1 + impulse.y + impulse.z * a1;
+  double L2 = impulse.x * s2 + impulse.y + impulse.z * a2;
+  ca = sub(ca, scale(ma, P));
+  cb = add(cb, scale(mb, P));
+  aa -= ia * L1;
+  ab += ib * L2;
+  pv[b.ia].c = ca; pv[b.ia].a = aa;
+  pv[b.ib].c = cb; pv[b.ib].a = ab;
+  return (linear_error <= LINEAR_SLOP) && (angular_error <= ANGULAR_SLOP);
+}
+
+/* ---- distance (joints/distance.py:170-311); the effective mass uses the separation before
+   normalisation: `self._u *= 1/length` rebinds the attribute, not the local u ---- */
+static void distance_init(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  jset2(J, B2G_JD_RA_X, ra); jset2(J, B2G_JD_RB_X, rb);
+  V2 u = sub(sub(add(cb, rb), ca), ra);
+  double length = length_(u);
+  V2 un = length > LINEAR_SLOP ? scale(1.0 / length, u) : v2(0.0, 0.0);
+  jset2(J, B2G_JD_DIS_U_X, un);
+  double cr_au = pysq(cross(ra, u)), cr_bu = pysq(cross(rb, u));
+  double inv_mass = ma + ia * cr_au + mb + ib * cr_bu;
+  double mass = inv_mass != 0.0 ? 1.0 / inv_mass : 0.0;
+  J->d[B2G_JD_DIS_MASS] = mass;
+  if (J->d[B2G_JD_FREQ] > 0.0) {
+    double C = length - J->d[B2G_JD_LENGTH];
+    double omega = 2.0 * PI_ * J->d[B2G_JD_FREQ];
+    double dd = 2.0 * mass * J->d[B2G_JD_DAMP] * omega;
+    double k = mass * pysq(omega);
+    double dt = st->dt;
+    double gamma = dt * (dd + dt * k);
+    gamma = gamma != 0.0 ? 1.0 / gamma : 0.0;
+    J->d[B2G_JD_DIS_GAMMA] = gamma;
+    J->d[B2G_JD_DIS_BIAS] = C * dt * k * gamma;
+    double m2 = inv_mass + gamma;
+    J->d[B2G_JD_DIS_MASS] = m2 != 0.0 ? 1.0 / m2 : 0.0;
+  }
+  if (st->warm_starting) {
+    double imp = J->d[B2G_JD_IMP_X] * st->dt_ratio;
+    J->d[B2G_JD_IMP_X] = imp;
+    V2 P = scale(imp, un);
+    va = sub(va, scale(ma, P));
+    wa -= ia * cross(ra, P);
+    vb = add(vb, scale(mb, P));
+    wb += ib * cross(rb, P);
+  } else {
+    J->d[B2G_JD_IMP_X] = 0.0;
+  }
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static void distance_solve_velocity(World* w, Joint* J, PV* pv) {
+  JB b = joint_bodies(w, J);
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  V2 ra = jget2(J, B2G_JD_RA_X), rb = jget2(J, B2G_JD_RB_X);
+  V2 u = jget2(J, B2G_JD_DIS_U_X);
+  V2 vpa = add(va, cross_sv(wa, ra)), vpb = add(vb, cross_sv(wb, rb));
+  double Cdot = dot(u, sub(vpb, vpa));
+  double impulse = -J->d[B2G_JD_DIS_MASS] * (Cdot + J->d[B2G_JD_DIS_BIAS] + J->d[B2G_JD_DIS_GAMMA] * J->d[B2G_JD_IMP_X]);
+  J->d[B2G_JD_IMP_X] += impulse;
+  V2 P = scale(impulse, u);
+  va = sub(va, scale(b.ma, P));
+  wa -= b.Ia * cross(ra, P);
+  vb = add(vb, scale(b.mb, P));
+  wb += b.Ib * cross(rb, P);
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static int distance_solve_position(World* w, Joint* J, PV* pv) {
+  if (J->d[B2G_JD_FREQ] > 0.0) return 1; /* no position correction for soft constraints */
+  JB b = joint_bodies(w, J);
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  V2 u = sub(sub(add(cb, rb), ca), ra);
+  double length = length_(u);
+  u = normalized(u);
+  double C = length - J->d[B2G_JD_LENGTH];
+  double impulse = -J->d[B2G_JD_DIS_MASS] * pclamp(C, -MAX_LINEAR_CORRECTION, MAX_LINEAR_CORRECTION);
+  V2 P = scale(impulse, u);
+  ca = sub(ca, scale(b.ma, P));
+  aa -= b.Ia * cross(ra, P);
+  cb = add(cb, scale(b.mb, P));
+  ab += b.Ib * cross(rb, P);
+  pv[b.ia].c = ca; pv[b.ia].a = aa;
+  pv[b.ib].c = cb; pv[b.ib].a = ab;
+  return fabs(C) < LINEAR_SLOP;
+}
+
+/* ---- pulley (joints/pulley.py:181-320); the warm start and the position correction move the
+   bodies by the effective masses (the local ma, mb are rebound before they are used) ---- */
+static void pulley_init(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  jset2(J, B2G_JD_RA_X, ra); jset2(J, B2G_JD_RB_X, rb);
+  V2 ua = sub(add(ca, ra), jget2(J, B2G_JD_G0_X));
+  V2 ub = sub(add(cb, rb), jget2(J, B2G_JD_G1_X));
+  double length_a = length_(ua), length_b = length_(ub);
+  ua = length_a > 10.0 * LINEAR_SLOP ? vdiv(ua, length_a) : v2(0.0, 0.0);
+  ub = length_b > 10.0 * LINEAR_SLOP ? vdiv(ub, length_b) : v2(0.0, 0.0);
+  jset2(J, B2G_JD_PUL_UA_X, ua); jset2(J, B2G_JD_PUL_UB_X, ub);
+  double rua = cross(ra, ua), rub = cross(rb, ub);
+  ma = ma + ia * pysq(rua);
+  mb = mb + ib * pysq(rub);
+  double ratio = J->d[B2G_JD_RATIO];
+  double mass = ma + ratio * ratio * mb;
+  if (mass > 0.0) mass = 1.0 / mass;
+  J->d[B2G_JD_PUL_MASS] = mass;
+  if (st->warm_starting) {
+    double imp = J->d[B2G_JD_IMP_X] * st->dt_ratio;
+    J->d[B2G_JD_IMP_X] = imp;
+    V2 Pa = scale(-imp, ua);
+    V2 Pb = scale(-ratio * imp, ub);
+    va = add(va, scale(ma, Pa));
+    wa += ia * cross(ra, Pa);
+    vb = add(vb, scale(mb, Pb));
+    wb += ib * cross(rb, Pb);
+  } else {
+    J->d[B2G_JD_IMP_X] = 0.0;
+  }
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static void pulley_solve_velocity(World* w, Joint* J, PV* pv) {
+  JB b = joint_bodies(w, J);
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  V2 ra = jget2(J, B2G_JD_RA_X), rb = jget2(J, B2G_JD_RB_X);
+  V2 ua = jget2(J, B2G_JD_PUL_UA_X), ub = jget2(J, B2G_JD_PUL_UB_X);
+  double ratio = J->d[B2G_JD_RATIO];
+  V2 vpa = add(va, cross_sv(wa, ra)), vpb = add(vb, cross_sv(wb, rb));
+  double Cdot = -dot(ua, vpa) - ratio * dot(ub, vpb);
+  double impulse = -J->d[B2G_JD_PUL_MASS] * Cdot;
+  J->d[B2G_JD_IMP_X] += impulse;
+  V2 Pa = scale(-impulse, ua);
+  V2 Pb = scale(-ratio * impulse, ub);
+  va = add(va, scale(b.ma, Pa));
+  wa += b.Ia * cross(ra, Pa);
+  vb = add(vb, scale(b.mb, Pb));
+  wb += b.Ib * cross(rb, Pb);
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static int pulley_solve_position(World* w, Joint* J, PV* pv) {
+  JB b = joint_bodies(w, J);
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  V2 ua = sub(add(ca, ra), jget2(J, B2G_JD_G0_X));
+  V2 ub = sub(add(cb, rb), jget2(J, B2G_JD_G1_X));
+  double length_a = length_(ua), length_b = length_(ub);
+  ua = length_a > 10.0 * LINEAR_SLOP ? vdiv(ua, length_a) : v2(0.0, 0.0);
+  ub = length_b > 10.0 * LINEAR_SLOP ? vdiv(ub, length_b) : v2(0.0, 0.0);
+  double rua = cross(ra, ua), rub = cross(rb, ub);
+  ma = ma + ia * pysq(rua);
+  mb = mb + ib * pysq(rub);
+  double ratio = J->d[B2G_JD_RATIO];
+  double mass = ma + pysq(ratio) * mb;
+  if (mass > 0.0) mass = 1.0 / mass;
+  double C = J->d[B2G_JD_LENGTH] - length_a - ratio * length_b;
+  double linear_error = fabs(C);
+  double impulse = -mass * C;
+  V2 Pa = scale(-impulse, ua);
+  V2 Pb = scale(-ratio * impulse, ub);
+  ca = add(ca, scale(ma, Pa));
+  aa += ia * cross(ra, Pa);
+  cb = add(cb, scale(mb, Pb));
+  ab += ib * cross(rb, Pb);
+  pv[b.ia].c = ca; pv[b.ia].a = aa;
+  pv[b.ib].c = cb; pv[b.ib].a = ab;
+  return linear_error < LINEAR_SLOP;
+}
+
+/* ---- mouse (joints/mouse.py:150-229): one body (body_b) ---- */
+static void mouse_init(World* w, Joint* J, PV* pv, const Step* st) {
+  const Body* B = &w->b[J->i[B2G_JI_BODY_B]];
+  int ix = B->island_index;
+  V2 cb = pv[ix].c;
+  double ab = pv[ix].a;
+  V2 vb = pv[ix].v;
+  double wb = pv[ix].w;
+  double mb = B->inv_mass, ib = B->invI;
+  M22 qb = rot(ab);
+  double mass = B->mass;
+  double omega = 2.0 * PI_ * J->d[B2G_JD_FREQ];
+  double d = 2.0 * mass * J->d[B2G_JD_DAMP] * omega;
+  double k = mass * pysq(omega);
+  double dt = st->dt;
+  if (!(d + dt * k > EPSILON)) w->err |= ORC_E_JOINT; /* the reference asserts (mouse.py:173) */
+  double gamma = dt * (d + dt * k);
+  if (gamma != 0.0) gamma = 1.0 / gamma;
+  double beta = dt * k * gamma;
+  J->d[B2G_JD_MOU_GAMMA] = gamma;
+  J->d[B2G_JD_MOU_BETA] = beta;
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), B->lc));
+  jset2(J, B2G_JD_RB_X, rb);
+  M22 K;
+  K.c1x = mb + ib * pysq(rb.y) + gamma;
+  K.c1y = -ib * rb.x * rb.y;
+  K.c2x = K.c1y;
+  K.c2y = mb + ib * pysq(rb.x) + gamma;
+  jset22(J, B2G_JD_MOU_MASS, m22_inverse(K));
+  jset2(J, B2G_JD_MOU_C_X, scale(beta, sub(add(cb, rb), jget2(J, B2G_JD_G0_X))));
+  wb *= 0.98;
+  if (st->warm_starting) {
+    V2 imp = scale(st->dt_ratio, jget2(J, B2G_JD_IMP_X));
+    jset2(J, B2G_JD_IMP_X, imp);
+    vb = add(vb, scale(mb, imp));
+    wb += ib * cross(rb, imp);
+  } else {
+    jset2(J, B2G_JD_IMP_X, v2(0.0, 0.0));
+  }
+  pv[ix].v = vb; pv[ix].w = wb;
+}
+
+static void mouse_solve_velocity(World* w, Joint* J, PV* pv, const Step* st) {
+  const Body* B = &w->b[J->i[B2G_JI_BODY_B]];
+  int ix = B->island_index;
+  V2 vb = pv[ix].v;
+  double wb = pv[ix].w;
+  V2 rb = jget2(J, B2G_JD_RB_X);
+  V2 Cdot = add(vb, cross_sv(wb, rb));
+  V2 acc = jget2(J, B2G_JD_IMP_X);
+  V2 impulse = mulmv(jget22(J, B2G_JD_MOU_MASS),
+                     neg(add(add(Cdot, jget2(J, B2G_JD_MOU_C_X)), scale(J->d[B2G_JD_MOU_GAMMA], acc))));
+  V2 old = acc;
+  acc = add(acc, impulse);
+  double max_impulse = st->dt * J->d[B2G_JD_MAX_MOTOR];
+  if (length_sq(acc) > pysq(max_impulse)) acc = scale(max_impulse / length_(acc), acc);
+  jset2(J, B2G_JD_IMP_X, acc);
+  impulse = sub(acc, old);
+  vb = add(vb, scale(B->inv_mass, impulse));
+  wb += B->invI * cross(rb, impulse);
+  pv[ix].v = vb; pv[ix].w = wb;
+}
+
+/* ---- gear (joints/gear.py:262-388).  The constructor takes bodies b and d from joint_a as
+   well (gear.py:91-92), so a == b and c == d here; the velocity solve keeps the linear
+   velocities it captured at setup (only the angular ones are re-read, gear.py:337-340), and the
+   position solve starts from the positions captured at setup (gear.py:356-358) and writes them
+   back with its correction. ---- */
+typedef struct { int idx[4]; double m[4], I[4]; V2 lc[4]; } GB;
+static GB gear_bodies(const World* w, const Joint* J) {
+  static const int slot[4] = {B2G_JI_BODY_A, B2G_JI_BODY_B, B2G_JI_BODY_C, B2G_JI_BODY_D};
+  GB g;
+  for (int k = 0; k < 4; ++k) {
+    const Body* b = &w->b[J->i[slot[k]]];
+    g.idx[k] = b->island_index;
+    g.m[k] = b->inv_mass;
+    g.I[k] = b->invI;
+    g.lc[k] = b->lc;
+  }
+  return g;
+}
+
+static void gear_init(World* w, Joint* J, PV* pv, const Step* st) {
+  GB g = gear_bodies(w, J);
+  V2 Cs[4], Vs[4];
+  double As[4], Ws[4];
+  for (int k = 0; k < 4; ++k) {
+    Cs[k] = pv[g.idx[k]].c; As[k] = pv[g.idx[k]].a;
+    Vs[k] = pv[g.idx[k]].v; Ws[k] = pv[g.idx[k]].w;
+  }
+  M22 qa = rot(As[0]), qb = rot(As[1]), qc = rot(As[2]), qd = rot(As[3]);
+  double ia = g.I[0], ib = g.I[1], ic = g.I[2], id_ = g.I[3];
+  double ma = g.m[0], mb = g.m[1], mc = g.m[2], md = g.m[3];
+  double ratio = J->d[B2G_JD_RATIO];
+  double mass = 0.0;
+  V2 j_vac, j_vbd;
+  double j_wa, j_wb, j_wc, j_wd;
+  if (JFLAG(J, B2G_JF_REVOLUTE_A)) {
+    j_vac = v2(0.0, 0.0);
+    j_wa = 1.0;
+    j_wc = 1.0;
+    mass += ia + ic;
+  } else {
+    V2 u = mulmv(qc, jget2(J, B2G_JD_AXIS_X));
+    V2 rc = mulmv(qc, sub(jget2(J, B2G_JD_G0_X), g.lc[2]));
+    V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), g.lc[0]));
+    j_vac = u;
+    j_wc = cross(rc, u);
+    j_wa = cross(ra, u);
+    mass += mc + ma + ic * j_wc * j_wc + ia * j_wa * j_wa;
+  }
+  if (JFLAG(J, B2G_JF_REVOLUTE_B)) {
+    j_vbd = v2(0.0, 0.0);
+    j_wb = 1.0;
+    j_wd = 1.0;
+    mass += pysq(ratio) * (ib + id_);
+  } else {
+    V2 u = mulmv(qd, jget2(J, B2G_JD_PERP_X));
+    V2 rd = mulmv(qd, sub(jget2(J, B2G_JD_G1_X), g.lc[3]));
+    V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), g.lc[1]));
+    j_vbd = u;
+    j_wd = cross(rd, u);
+    j_wb = cross(rb, u);
+    mass += pysq(ratio) * (md + mb) + id_ * pysq(j_wd) + ib * j_wb * j_wb;
+  }
+  mass = mass > 0.0 ? 1.0 / mass : 0.0;
+  jset2(J, B2G_JD_GEA_JVAC_X, j_vac); jset2(J, B2G_JD_GEA_JVBD_X, j_vbd);
+  J->d[B2G_JD_GEA_JWA] = j_wa; J->d[B2G_JD_GEA_JWB] = j_wb;
+  J->d[B2G_JD_GEA_JWC] = j_wc; J->d[B2G_JD_GEA_JWD] = j_wd;
+  J->d[B2G_JD_GEA_MASS] = mass;
+  if (st->warm_starting) { /* no dt_ratio scaling here (gear.py:319-328) */
+    double impulse = J->d[B2G_JD_IMP_X];
+    Vs[0] = add(Vs[0], scale(ma * impulse, j_vac));
+    Ws[0] += ia * impulse * j_wa;
+    Vs[1] = add(Vs[1], scale(mb * impulse, j_vbd));
+    Ws[1] += ib * impulse * j_wb;
+    Vs[2] = sub(Vs[2], scale(mc * impulse, j_vac));
+    Ws[2] -= ic * impulse * j_wc;
+    Vs[3] = sub(Vs[3], scale(md * impulse, j_vbd));
+    Ws[3] -= id_ * impulse * j_wd;
+  } else {
+    J->d[B2G_JD_IMP_X] = 0.0;
+  }
+  for (int k = 0; k < 4; ++k) {
+    jset2(J, B2G_JD_GEA_V + 2 * k, Vs[k]);
+    jset2(J, B2G_JD_GEA_C + 2 * k, Cs[k]);
+    J->d[B2G_JD_GEA_A + k] = As[k];
+    pv[g.idx[k]].v = Vs[k];
+    pv[g.idx[k]].w = Ws[k];
+  }
+}
+
+static void gear_solve_velocity(World* w, Joint* J, PV* pv) {
+  GB g = gear_bodies(w, J);
+  V2 Vs[4];
+  double Ws[4];
+  for (int k = 0; k < 4; ++k) {
+    Vs[k] = jget2(J, B2G_JD_GEA_V + 2 * k);
+    Ws[k] = pv[g.idx[k]].w;
+  }
+  V2 j_vac = jget2(J, B2G_JD_GEA_JVAC_X), j_vbd = jget2(J, B2G_JD_GEA_JVBD_X);
+  double j_wa = J->d[B2G_JD_GEA_JWA], j_wb = J->d[B2G_JD_GEA_JWB];
+  double j_wc = J->d[B2G_JD_GEA_JWC], j_wd = J->d[B2G_JD_GEA_JWD];
+  double Cdot = dot(j_vac, sub(Vs[0], Vs[2])) + dot(j_vbd, sub(Vs[1], Vs[3]));
+  Cdot += (j_wa * Ws[0] - j_wc * Ws[2]) + (j_wb * Ws[1] - j_wd * Ws[3]);
+  double impulse = -J->d[B2G_JD_GEA_MASS] * Cdot;
+  J->d[B2G_JD_IMP_X] += impulse;
+  V2 lvj[4] = {j_vac, j_vbd, neg(j_vac), neg(j_vbd)};
+  double avj[4] = {j_wa, j_wb, -j_wc, -j_wd};
+  for (int k = 0; k < 4; ++k) {
+    Vs[k] = add(Vs[k], scale(g.m[k] * impulse, lvj[k]));
+    Ws[k] += (g.I[k] * impulse) * avj[k];
+  }
+  for (int k = 0; k < 4; ++k) {
+    jset2(J, B2G_JD_GEA_V + 2 * k, Vs[k]);
+    pv[g.idx[k]].v = Vs[k];
+    pv[g.idx[k]].w = Ws[k];
+  }
+}
+
+static int gear_solve_position(World* w, Joint* J, PV* pv) {
+  GB g = gear_bodies(w, J);
+  double ratio = J->d[B2G_JD_RATIO];
+  V2 ca = jget2(J, B2G_JD_GEA_C), cb = jget2(J, B2G_JD_GEA_C + 2);
+  V2 cc = jget2(J, B2G_JD_GEA_C + 4), cd = jget2(J, B2G_JD_GEA_C + 6);
+  double aa = J->d[B2G_JD_GEA_A], ab = J->d[B2G_JD_GEA_A + 1];
+  double ac = J->d[B2G_JD_GEA_A + 2], ad = J->d[B2G_JD_GEA_A + 3];
+  M22 qa = rot(aa), qb = rot(ab), qc = rot(ac), qd = rot(ad);
+  double ia = g.I[0], ib = g.I[1], ic = g.I[2], id_ = g.I[3];
+  double ma = g.m[0], mb = g.m[1], mc = g.m[2], md = g.m[3];
+  double mass = 0.0, coordinate_a, coordinate_b;
+  V2 j_vac, j_vbd;
+  double j_wa, j_wb, j_wc, j_wd;
+  if (JFLAG(J, B2G_JF_REVOLUTE_A)) {
+    j_vac = v2(0.0, 0.0);
+    j_wa = 1.0;
+    j_wc = 1.0;
+    mass += ia + ic;
+    coordinate_a = aa - ac - J->d[B2G_JD_REF];
+  } else {
+    V2 axis_c = jget2(J, B2G_JD_AXIS_X);
+    V2 u = mulmv(qc, axis_c);
+    V2 rc = mulmv(qc, sub(jget2(J, B2G_JD_G0_X), g.lc[2]));
+    V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), g.lc[0]));
+    j_vac = u;
+    j_wc = cross(rc, u);
+    j_wa = cross(ra, u);
+    mass += mc + ma + ic * pysq(j_wc) + ia * pysq(j_wa);
+    V2 pc = sub(jget2(J, B2G_JD_G0_X), g.lc[2]);
+    V2 pa = mulmtv(qc, add(ra, sub(ca, cc)));
+    coordinate_a = dot(sub(pa, pc), axis_c);
+  }
+  if (JFLAG(J, B2G_JF_REVOLUTE_B)) {
+    j_vbd = v2(0.0, 0.0);
+    j_wb = 1.0;
+    j_wd = 1.0;
+    mass += ib + id_;
+    coordinate_b = ab - ad - J->d[B2G_JD_REF_B];
+  } else {
+    V2 axis_d = jget2(J, B2G_JD_PERP_X);
+    V2 u = mulmv(qd, axis_d);
+    V2 rd = mulmv(qd, sub(jget2(J, B2G_JD_G1_X), g.lc[3]));
+    V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), g.lc[1]));
+    j_vbd = scale(ratio, u);
+    j_wd = ratio * cross(rd, u);
+    j_wb = ratio * cross(rb, u);
+    mass += pysq(ratio) * (md + mb) + id_ * pysq(j_wd) + ib * pysq(j_wb);
+    V2 pd = sub(jget2(J, B2G_JD_G1_X), g.lc[3]);
+    V2 pb = mulmtv(qd, add(rb, sub(cb, cd)));
+    coordinate_b = dot(sub(pb, pd), axis_d);
+  }
+  double C = (coordinate_a + ratio * coordinate_b) - J->d[B2G_JD_LENGTH];
+  if (mass > 0.0) {
+    double impulse = -C / mass;
+    ca = add(ca, scale(ma * impulse, j_vac));
+    cb = add(cb, scale(mb * impulse, j_vbd));
+    cc = sub(cc, scale(mc * impulse, j_vac));
+    cd = sub(cd, scale(md * impulse, j_vbd));
+    aa += ia * impulse * j_wa;
+    ab += ib * impulse * j_wb;
+    ac -= ic * impulse * j_wc;
+    ad -= id_ * impulse * j_wd;
+    V2 cs[4] = {ca, cb, cc, cd};
+    double as[4] = {aa, ab, ac, ad};
+    for (int k = 0; k < 4; ++k) { pv[g.idx[k]].c = cs[k]; pv[g.idx[k]].a = as[k]; }
+  }
+  return 1;
+}
+
+/* ---- wheel (joints/wheel.py:241-417) ---- */
+static void wheel_init(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  jset2(J, B2G_JD_RA_X, ra); jset2(J, B2G_JD_RB_X, rb);
+  V2 d = sub(sub(add(cb, rb), ca), ra);
+  V2 ay = mulmv(qa, jget2(J, B2G_JD_PERP_X));
+  jset2(J, B2G_JD_WHE_AY_X, ay);
+  double s_ay = cross(add(d, ra), ay), s_by = cross(rb, ay);
+  J->d[B2G_JD_WHE_S_AY] = s_ay; J->d[B2G_JD_WHE_S_BY] = s_by;
+  double mass = ma + mb + ia * pysq(s_ay) + ib * pysq(s_by);
+  if (mass > 0.0) mass = 1.0 / mass;
+  J->d[B2G_JD_WHE_MASS] = mass;
+  J->d[B2G_JD_WHE_SPRING_MASS] = 0.0;
+  J->d[B2G_JD_WHE_BIAS] = 0.0;
+  J->d[B2G_JD_WHE_GAMMA] = 0.0;
+  if (J->d[B2G_JD_FREQ] > 0.0) {
+    V2 ax = mulmv(qa, jget2(J, B2G_JD_AXIS_X));
+    jset2(J, B2G_JD_WHE_AX_X, ax);
+    double s_ax = cross(add(d, ra), ax), s_bx = cross(rb, ax);
+    J->d[B2G_JD_WHE_S_AX] = s_ax; J->d[B2G_JD_WHE_S_BX] = s_bx;
+    double inv_mass = ma + mb + ia * pysq(s_ax) + ib * pysq(s_bx);
+    if (inv_mass > 0.0) {
+      double spring_mass = 1.0 / inv_mass;
+      double C = dot(d, ax);
+      double omega = 2.0 * PI_ * J->d[B2G_JD_FREQ];
+      double dd = 2.0 * spring_mass * J->d[B2G_JD_DAMP] * omega;
+      double k = spring_mass * pysq(omega);
+      double dt = st->dt;
+      double gamma = dt * (dd + dt * k);
+      if (gamma > 0.0) gamma = 1.0 / gamma;
+      J->d[B2G_JD_WHE_GAMMA] = gamma;
+      J->d[B2G_JD_WHE_BIAS] = C * dt * k * gamma;
+      spring_mass = inv_mass + gamma;
+      if (spring_mass > 0.0) spring_mass = 1.0 / spring_mass;
+      J->d[B2G_JD_WHE_SPRING_MASS] = spring_mass;
+    }
+  } else {
+    J->d[B2G_JD_SPRING_IMP] = 0.0;
+  }
+  if (JFLAG(J, B2G_JF_MOTOR)) {
+    double motor_mass = ia + ib;
+    if (motor_mass > 0.0) motor_mass = 1.0 / motor_mass;
+    J->d[B2G_JD_WHE_MOTOR_MASS] = motor_mass;
+  } else {
+    J->d[B2G_JD_WHE_MOTOR_MASS] = 0.0;
+    J->d[B2G_JD_MOTOR_IMP] = 0.0;
+  }
+  if (st->warm_starting) {
+    double r = st->dt_ratio;
+    double impulse = J->d[B2G_JD_IMP_X] * r;
+    double spring_impulse = J->d[B2G_JD_SPRING_IMP] * r;
+    double motor_impulse = J->d[B2G_JD_MOTOR_IMP] * r;
+    J->d[B2G_JD_IMP_X] = impulse;
+    J->d[B2G_JD_SPRING_IMP] = spring_impulse;
+    J->d[B2G_JD_MOTOR_IMP] = motor_impulse;
+    V2 P = add(scale(impulse, ay), scale(spring_impulse, jget2(J, B2G_JD_WHE_AX_X)));
+    double La = impulse * s_ay + spring_impulse * J->d[B2G_JD_WHE_S_AX] + motor_impulse;
+    double Lb = impulse * s_by + spring_impulse * J->d[B2G_JD_WHE_S_BX] + motor_impulse;
+    va = sub(va, scale(ma, P));
+    wa -= ia * La;
+    vb = add(vb, scale(mb, P));
+    wb += ib * Lb;
+  } else {
+    J->d[B2G_JD_IMP_X] = 0.0;
+    J->d[B2G_JD_SPRING_IMP] = 0.0;
+    J->d[B2G_JD_MOTOR_IMP] = 0.0;
+  }
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static void wheel_solve_velocity(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  V2 ax = jget2(J, B2G_JD_WHE_AX_X), ay = jget2(J, B2G_JD_WHE_AY_X);
+  double s_ax = J->d[B2G_JD_WHE_S_AX], s_bx = J->d[B2G_JD_WHE_S_BX];
+  double s_ay = J->d[B2G_JD_WHE_S_AY], s_by = J->d[B2G_JD_WHE_S_BY];
+  double Cdot = dot(ax, sub(vb, va)) + s_bx * wb - s_ax * wa;
+  double impulse = -J->d[B2G_JD_WHE_SPRING_MASS] *
+                   (Cdot + J->d[B2G_JD_WHE_BIAS] + J->d[B2G_JD_WHE_GAMMA] * J->d[B2G_JD_SPRING_IMP]);
+  J->d[B2G_JD_SPRING_IMP] += impulse;
+  V2 P = scale(impulse, ax);
+  double La = impulse * s_ax, Lb = impulse * s_bx;
+  va = sub(va, scale(ma, P));
+  wa -= ia * La;
+  vb = add(vb, scale(mb, P));
+  wb += ib * Lb;
+  Cdot = wb - wa - J->d[B2G_JD_MOTOR_SPEED];
+  impulse = -J->d[B2G_JD_WHE_MOTOR_MASS] * Cdot;
+  double old = J->d[B2G_JD_MOTOR_IMP];
+  double max_impulse = st->dt * J->d[B2G_JD_MAX_MOTOR];
+  J->d[B2G_JD_MOTOR_IMP] = pclamp(old + impulse, -max_impulse, max_impulse);
+  impulse = J->d[B2G_JD_MOTOR_IMP] - old;
+  wa -= ia * impulse;
+  wb += ib * impulse;
+  Cdot = dot(ay, sub(vb, va)) + s_by * wb - s_ay * wa;
+  impulse = -J->d[B2G_JD_WHE_MASS] * Cdot;
+  J->d[B2G_JD_IMP_X] += impulse;
+  P = scale(impulse, ay);
+  La = impulse * s_ay;
+  Lb = impulse * s_by;
+  va = sub(va, scale(ma, P));
+  wa -= ia * La;
+  vb = add(vb, scale(mb, P));
+  wb += ib * Lb;
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static int wheel_solve_position(World* w, Joint* J, PV* pv) {
+  JB b = joint_bodies(w, J);
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  V2 d = sub(add(sub(cb, ca), rb), ra);
+  /* the effective mass from the setup's s_ay, s_by (wheel.py:402) */
+  double k = ma + mb + ia * pysq(J->d[B2G_JD_WHE_S_AY]) + ib * pysq(J->d[B2G_JD_WHE_S_BY]);
+  V2 ay = mulmv(qa, jget2(J, B2G_JD_PERP_X));
+  double s_ay = cross(add(d, ra), ay), s_by = cross(rb, ay);
+  double C = dot(d, ay);
+  double impulse = k != 0.0 ? -C / k : 0.0;
+  V2 P = scale(impulse, ay);
+  double La = impulse * s_ay, Lb = impulse * s_by;
+  ca = sub(ca, scale(ma, P));
+  aa -= ia * La;
+  cb = add(cb, scale(mb, P));
+  ab += ib * Lb;
+  pv[b.ia].c = ca; pv[b.ia].a = aa;
+  pv[b.ib].c = cb; pv[b.ib].a = ab;
+  return fabs(C) <= LINEAR_SLOP;
+}
+
+/* ---- weld (joints/weld.py:127-257); the setup's mass matrix has col2.z = c3x (weld.py:162) ---- */
+static void weld_init(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  jset2(J, B2G_JD_RA_X, ra); jset2(J, B2G_JD_RB_X, rb);
+  double c2x = -ra.y * ra.x * ia - rb.y * rb.x * ib;
+  double c3x = -ra.y * ia - rb.y * ib;
+  M33 m;
+  m.c1 = v3(ma + mb + ra.y * ra.y * ia + rb.y * rb.y * ib, c2x, c3x);
+  m.c2 = v3(c2x, ma + mb + ra.x * ra.x * ia + rb.x * rb.x * ib, c3x);
+  m.c3 = v3(c3x, ra.x * ia + rb.x * ib, ia + ib);
+  jset33(J, B2G_JD_WEL_MASS, m);
+  if (st->warm_starting) {
+    V3 imp = scale3(st->dt_ratio, jget3(J, B2G_JD_IMP_X));
+    jset3(J, B2G_JD_IMP_X, imp);
+    V2 P = v2(imp.x, imp.y);
+    va = sub(va, scale(ma, P));
+    wa -= ia * (cross(ra, P) + imp.z);
+    vb = add(vb, scale(mb, P));
+    wb += ib * (cross(rb, P) + imp.z);
+  } else {
+    jset3(J, B2G_JD_IMP_X, v3(0.0, 0.0, 0.0));
+  }
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static void weld_solve_velocity(World* w, Joint* J, PV* pv) {
+  JB b = joint_bodies(w, J);
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  V2 ra = jget2(J, B2G_JD_RA_X), rb = jget2(J, B2G_JD_RB_X);
+  V2 Cdot1 = sub(sub(add(vb, cross_sv(wb, rb)), va), cross_sv(wa, ra));
+  double Cdot2 = wb - wa;
+  V3 impulse = solve33(jget33(J, B2G_JD_WEL_MASS), neg3(v3(Cdot1.x, Cdot1.y, Cdot2)));
+  jset3(J, B2G_JD_IMP_X, add3(jget3(J, B2G_JD_IMP_X), impulse));
+  V2 P = v2(impulse.x, impulse.y);
+  va = sub(va, scale(b.ma, P));
+  wa -= b.Ia * (cross(ra, P) + impulse.z);
+  vb = add(vb, scale(b.mb, P));
+  wb += b.Ib * (cross(rb, P) + impulse.z);
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static int weld_solve_position(World* w, Joint* J, PV* pv) {
+  JB b = joint_bodies(w, J);
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  V2 C1 = sub(sub(add(cb, rb), ca), ra);
+  double C2 = ab - aa - J->d[B2G_JD_REF];
+  double position_error = length_(C1), angular_error = fabs(C2);
+  double c2x = -ra.y * ra.x * ia - rb.y * rb.x * ib;
+  double c3x = -ra.y * ia - rb.y * ib;
+  double c3y = ra.x * ia + rb.x * ib;
+  M33 m;
+  m.c1 = v3(ma + mb + ra.y * ra.y * ia + rb.y * rb.y * ib, c2x, c3x);
+  m.c2 = v3(c2x, ma + mb + ra.x * ra.x * ia + rb.x * rb.x * ib, c3y);
+  m.c3 = v3(-ra.y * ia - rb.y * ib, ra.x * ia + rb.x * ib, ia + ib);
+  jset33(J, B2G_JD_WEL_MASS, m); /* the joint's _mass is rewritten here (weld.py:230-242) */
+  V3 impulse = neg3(solve33(m, v3(C1.x, C1.y, C2)));
+  V2 P = v2(impulse.x, impulse.y);
+  ca = sub(ca, scale(ma, P));
+  aa -= ia * (cross(ra, P) + impulse.z);
+  cb = add(cb, scale(mb, P));
+  ab += ib * (cross(rb, P) + impulse.z);
+  pv[b.ia].c = ca; pv[b.ia].a = aa;
+  pv[b.ib].c = cb; pv[b.ib].a = ab;
+  return position_error <= LINEAR_SLOP && angular_error <= ANGULAR_SLOP;
+}
+
+/* ---- friction (joints/friction.py:115-233); the velocity solve stores its results into the
+   position list (friction.py:228-229), so only its impulses (through the warm start) act ---- */
+static void friction_init(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  jset2(J, B2G_JD_RA_X, ra); jset2(J, B2G_JD_RB_X, rb);
+  M22 K;
+  K.c1x = ma + mb + ia * ra.y * ra.y + ib * rb.y * rb.y;
+  K.c1y = -ia * ra.x * ra.y - ib * rb.x * rb.y;
+  K.c2x = K.c1y;
+  K.c2y = ma + mb + ia * ra.x * ra.x + ib * rb.x * rb.x;
+  jset22(J, B2G_JD_FRI_LMASS, m22_inverse(K));
+  double angular_mass = ia + ib;
+  if (angular_mass > 0.0) angular_mass = 1.0 / angular_mass;
+  J->d[B2G_JD_FRI_AMASS] = angular_mass;
+  if (st->warm_starting) {
+    V2 li = scale(st->dt_ratio, jget2(J, B2G_JD_IMP_X));
+    double ang = J->d[B2G_JD_IMP_Z] * st->dt_ratio;
+    jset2(J, B2G_JD_IMP_X, li);
+    J->d[B2G_JD_IMP_Z] = ang;
+    va = sub(va, scale(ma, li));
+    wa -= ia * (cross(ra, li) + ang);
+    vb = add(vb, scale(mb, li));
+    wb += ib * (cross(rb, li) + ang);
+  } else {
+    jset2(J, B2G_JD_IMP_X, v2(0.0, 0.0));
+    J->d[B2G_JD_IMP_Z] = 0.0;
+  }
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static void friction_solve_velocity(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  V2 ra = jget2(J, B2G_JD_RA_X), rb = jget2(J, B2G_JD_RB_X);
+  double dt = st->dt;
+  double Cdot = wb - wa;
+  double impulse = -J->d[B2G_JD_FRI_AMASS] * Cdot;
+  double old = J->d[B2G_JD_IMP_Z];
+  double max_impulse = dt * J->d[B2G_JD_MAX_TORQUE];
+  J->d[B2G_JD_IMP_Z] = pclamp(old + impulse, -max_impulse, max_impulse);
+  impulse = J->d[B2G_JD_IMP_Z] - old;
+  wa -= b.Ia * impulse;
+  wb += b.Ib * impulse;
+  V2 Cdot2 = sub(sub(add(vb, cross_sv(wb, rb)), va), cross_sv(wa, ra));
+  V2 limp = neg(mulmv(jget22(J, B2G_JD_FRI_LMASS), Cdot2));
+  V2 lold = jget2(J, B2G_JD_IMP_X);
+  V2 li = add(lold, limp);
+  double lmax = dt * J->d[B2G_JD_MAX_MOTOR];
+  if (length_sq(li) > pysq(lmax)) li = scale(lmax, normalized(li));
+  jset2(J, B2G_JD_IMP_X, li);
+  (void)pv; /* the updated velocities are dropped (friction.py:228-229) */
+}
+
+/* ---- rope (joints/rope.py:104-248); the attribute _u keeps the unnormalised separation
+   (`u /= length` rebinds the local u), and the warm start and velocity solve use it ---- */
+static void rope_init(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  double ma = b.ma, mb = b.mb, ia = b.Ia, ib = b.Ib;
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  jset2(J, B2G_JD_RA_X, ra); jset2(J, B2G_JD_RB_X, rb);
+  V2 u = sub(sub(add(cb, rb), ca), ra);
+  jset2(J, B2G_JD_ROP_U_X, u);
+  double length = length_(u);
+  J->d[B2G_JD_ROP_LENGTH] = length;
+  double C = length - J->d[B2G_JD_LENGTH];
+  J->i[B2G_JI_STATE] = C > 0.0 ? B2G_LIMIT_AT_UPPER : B2G_LIMIT_INACTIVE;
+  if (length > LINEAR_SLOP) {
+    u = vdiv(u, length);
+  } else {
+    J->d[B2G_JD_ROP_MASS] = 0.0;
+    J->d[B2G_JD_IMP_X] = 0.0;
+    return;
+  }
+  double cra = cross(ra, u), crb = cross(rb, u);
+  double inv_mass = ma + ia * cra * cra + mb + ib * crb * crb;
+  J->d[B2G_JD_ROP_MASS] = inv_mass != 0.0 ? 1.0 / inv_mass : 0.0;
+  if (st->warm_starting) {
+    double imp = J->d[B2G_JD_IMP_X] * st->dt_ratio;
+    J->d[B2G_JD_IMP_X] = imp;
+    V2 P = scale(imp, jget2(J, B2G_JD_ROP_U_X));
+    va = sub(va, scale(ma, P));
+    wa -= ia * cross(ra, P);
+    vb = add(vb, scale(mb, P));
+    wb += ib * cross(rb, P);
+  } else {
+    J->d[B2G_JD_IMP_X] = 0.0;
+  }
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static void rope_solve_velocity(World* w, Joint* J, PV* pv, const Step* st) {
+  JB b = joint_bodies(w, J);
+  V2 va = pv[b.ia].v, vb = pv[b.ib].v;
+  double wa = pv[b.ia].w, wb = pv[b.ib].w;
+  V2 ra = jget2(J, B2G_JD_RA_X), rb = jget2(J, B2G_JD_RB_X);
+  V2 u = jget2(J, B2G_JD_ROP_U_X);
+  V2 vpa = add(va, cross_sv(wa, ra)), vpb = add(vb, cross_sv(wb, rb));
+  double C = J->d[B2G_JD_ROP_LENGTH] - J->d[B2G_JD_LENGTH];
+  double Cdot = dot(u, sub(vpb, vpa));
+  if (C < 0.0) Cdot += st->inv_dt * C;
+  double impulse = -J->d[B2G_JD_ROP_MASS] * Cdot;
+  double old = J->d[B2G_JD_IMP_X];
+  J->d[B2G_JD_IMP_X] = pmin(0.0, old + impulse);
+  impulse = J->d[B2G_JD_IMP_X] - old;
+  V2 P = scale(impulse, u);
+  va = sub(va, scale(b.ma, P));
+  wa -= b.Ia * cross(ra, P);
+  vb = add(vb, scale(b.mb, P));
+  wb += b.Ib * cross(rb, P);
+  pv[b.ia].v = va; pv[b.ia].w = wa;
+  pv[b.ib].v = vb; pv[b.ib].w = wb;
+}
+
+static int rope_solve_position(World* w, Joint* J, PV* pv) {
+  JB b = joint_bodies(w, J);
+  V2 ca = pv[b.ia].c, cb = pv[b.ib].c;
+  double aa = pv[b.ia].a, ab = pv[b.ib].a;
+  M22 qa = rot(aa), qb = rot(ab);
+  V2 ra = mulmv(qa, sub(jget2(J, B2G_JD_LA_X), b.lca));
+  V2 rb = mulmv(qb, sub(jget2(J, B2G_JD_LB_X), b.lcb));
+  V2 u = sub(sub(add(cb, rb), ca), ra);
+  double length = length_(u);
+  u = normalized(u);
+  double C = length - J->d[B2G_JD_LENGTH];
+  C = pclamp(C, 0.0, MAX_LINEAR_CORRECTION);
+  double impulse = -J->d[B2G_JD_ROP_MASS] * C;
+  V2 P = scale(impulse, u);
+  ca = sub(ca, scale(b.ma, P));
+  aa -= b.Ia * cross(ra, P);
+  cb = add(cb, scale(b.mb, P));
+  ab += b.Ib * cross(rb, P);
+  pv[b.ia].c = ca; pv[b.ia].a = aa;
+  pv[b.ib].c = cb; pv[b.ib].a = ab;
+  return length - J->d[B2G_JD_LENGTH] < LINEAR_SLOP;
+}
+
+/* Joint._init_velocity_constraints / _solve_velocity_constraints / _solve_position_constraints */
+static void joint_init(World* w, Joint* J, PV* pv, const Step* st) {
+  switch (J->i[B2G_JI_TYPE]) {
+    case B2G_JT_REVOLUTE: revolute_init(w, J, pv, st); break;
+    case B2G_JT_PRISMATIC: prismatic_init(w, J, pv, st); break;
+    case B2G_JT_DISTANCE: distance_init(w, J, pv, st); break;
+    case B2G_JT_PULLEY: pulley_init(w, J, pv, st); break;
+    case B2G_JT_MOUSE: mouse_init(w, J, pv, st); break;
+    case B2G_JT_GEAR: gear_init(w, J, pv, st); break;
+    case B2G_JT_WHEEL: wheel_init(w, J, pv, st); break;
+    case B2G_JT_WELD: weld_init(w, J, pv, st); break;
+    case B2G_JT_FRICTION: friction_init(w, J, pv, st); break;
+    case B2G_JT_ROPE: rope_init(w, J, pv, st); break;
+    default: w->err |= ORC_E_UNSUPPORTED;
+  }
+}
+static void joint_solve_velocity(World* w, Joint* J, PV* pv, const Step* st) {
+  switch (J->i[B2G_JI_TYPE]) {
+    case B2G_JT_REVOLUTE: revolute_solve_velocity(w, J, pv, st); break;
+    case B2G_JT_PRISMATIC: prismatic_solve_velocity(w, J, pv, st); break;
+    case B2G_JT_DISTANCE: distance_solve_velocity(w, J, pv); break;
+    case B2G_JT_PULLEY: pulley_solve_velocity(w, J, pv); break;
+    case B2G_JT_MOUSE: mouse_solve_velocity(w, J, pv, st); break;
+    case B2G_JT_GEAR: gear_solve_velocity(w, J, pv); break;
+    case B2G_JT_WHEEL: wheel_solve_velocity(w, J, pv, st); break;
+    case B2G_JT_WELD: weld_solve_velocity(w, J, pv); break;
+    case B2G_JT_FRICTION: friction_solve_velocity(w, J, pv, st); break;
+    case B2G_JT_ROPE: rope_solve_velocity(w, J, pv, st); break;
+    default: break;
+  }
+}
+static int joint_solve_position(World* w, Joint* J, PV* pv) {
+  switch (J->i[B2G_JI_TYPE]) {
+    case B2G_JT_REVOLUTE: return revolute_solve_position(w, J, pv);
+    case B2G_JT_PRISMATIC: return prismatic_solve_position(w, J, pv);
+    case B2G_JT_DISTANCE: return distance_solve_position(w, J, pv);
+    case B2G_JT_PULLEY: return pulley_solve_position(w, J, pv);
+    case B2G_JT_GEAR: return gear_solve_position(w, J, pv);
+    case B2G_JT_WHEEL: return wheel_solve_position(w, J, pv);
+    case B2G_JT_WELD: return weld_solve_position(w, J, pv);
+    case B2G_JT_ROPE: return rope_solve_position(w, J, pv);
+    default: return 1; /* mouse, friction (mouse.py:228-229, friction.py:231-233) */
+  }
+}
+
 static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py:65-179 */
   double dt = st->dt;
   PV* pv = is->pv;
@@ -1418,7 +2772,11 @@ static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py
       pv[vc->ib].v = vb; pv[vc->ib].w = wb;
     }
   }
-  for (int it = 0; it < st->vel_iters; ++it) solve_velocity(is->vc, nvc, pv);
+  for (int k = 0; k < is->njoints; ++k) joint_init(w, &w->j[is->joints[k]], pv, st); /* island.py:107-108 */
+  for (int it = 0; it < st->vel_iters; ++it) { /* island.py:111-114: joints, then contacts */
+    for (int k = 0; k < is->njoints; ++k) joint_solve_velocity(w, &w->j[is->joints[k]], pv, st);
+    solve_velocity(is->vc, nvc, pv);
+  }
   /* store_impulses (contact.py:801-806): zip(manifold.used_points, vc.points) */
   for (int k = 0; k < nvc; ++k) {
     VC* vc = &is->vc[k];
@@ -1448,8 +2806,14 @@ static void island_solve(World* w, IslandBuf* is, const Step* st) { /* island.py
     pv[i].w = wv;
   }
   int position_solved = 0;
-  for (int it = 0; it < st->pos_iters; ++it) {
-    if (solve_position(is->pc, nvc, pv)) { position_solved = 1; break; }
+  for (int it = 0; it < st->pos_iters; ++it) { /* island.py:141-152: contacts, then every joint */
+    int contacts_okay = solve_position(is->pc, nvc, pv);
+    int joints_okay = 1;
+    for (int k = 0; k < is->njoints; ++k) {
+      int ok = joint_solve_position(w, &w->j[is->joints[k]], pv);
+      joints_okay = joints_okay && ok;
+    }
+    if (contacts_okay && joints_okay) { position_solved = 1; break; }
   }
   for (int i = 0; i < is->nbodies; ++i) {
     Body* b = &w->b[is->bodies[i]];
@@ -1521,6 +2885,7 @@ static void solve(World* w, const Step* st) { /* world.py:475-592 */
   w->post_key = 0;
   for (int i = 0; i < nb; ++i) { w->b[i].island_flag = 0; w->b[i].island_id = -1; }
   for (int i = 0; i < nc; ++i) w->c[i].island_flag = 0;
+  for (int i = 0; i < w->nj; ++i) w->j[i].island_flag = 0;
   /* per-body contact lists in ContactManager order (= body._contacts order) */
   int* off = (int*)calloc((size_t)nb + 1, sizeof(int));
   int* adj = (int*)malloc(sizeof(int) * (size_t)(2 * nc + 1));
@@ -1528,11 +2893,12 @@ static void solve(World* w, const Step* st) { /* world.py:475-592 */
   IslandBuf is;
   is.bodies = (int*)malloc(sizeof(int) * (size_t)(nb + 1));
   is.contacts = (int*)malloc(sizeof(int) * (size_t)(nc + 1));
+  is.joints = (int*)malloc(sizeof(int) * (size_t)(w->nj + 1));
   is.pv = (PV*)malloc(sizeof(PV) * (size_t)(nb + 1));
   is.vc = (VC*)malloc(sizeof(VC) * (size_t)(nc + 1));
   is.pc = (PC*)malloc(sizeof(PC) * (size_t)(nc + 1));
   int* stack = (int*)malloc(sizeof(int) * (size_t)(nb + 1));
-  if (!off || !adj || !fill || !is.bodies || !is.contacts || !is.pv || !is.vc || !is.pc || !stack) {
+  if (!off || !adj || !fill || !is.bodies || !is.contacts || !is.joints || !is.pv || !is.vc || !is.pc || !stack) {
     w->err |= ORC_E_ALLOC;
     goto done;
   }
@@ -1554,6 +2920,7 @@ static void solve(World* w, const Step* st) { /* world.py:475-592 */
     if (sb->type == B2G_STATIC) continue;
     is.nbodies = 0;
     is.ncontacts = 0;
+    is.njoints = 0;
     int sp = 0;
     stack[sp++] = seed;
     sb->island_flag = 1;
@@ -1581,6 +2948,24 @@ static void solve(World* w, const Step* st) { /* world.py:475-592 */
         stack[sp++] = other;
         w->b[other].island_flag = 1;
       }
+      /* the body's joints (world.py:548-572) */
+      for (int k = w->bj_off[bi]; k < w->bj_off[bi + 1]; ++k) {
+        Joint* J = &w->j[w->bj[k]];
+        if (J->island_flag) continue;
+        int other = J->i[B2G_JI_BODY_A] == bi ? J->i[B2G_JI_BODY_B] : J->i[B2G_JI_BODY_A]; /* joint.py:76-81 */
+        if (other < 0) { /* a joint on one body (mouse) */
+          is.joints[is.njoints++] = w->bj[k];
+          J->island_flag = 1;
+          continue;
+        } else if (!w->b[other].active) {
+          continue;
+        }
+        is.joints[is.njoints++] = w->bj[k];
+        J->island_flag = 1;
+        if (w->b[other].island_flag) continue;
+        stack[sp++] = other;
+        w->b[other].island_flag = 1;
+      }
     }
     island_solve(w, &is, st);
     for (int i = 0; i < is.nbodies; ++i)
@@ -1596,7 +2981,7 @@ static void solve(World* w, const Step* st) { /* world.py:475-592 */
   find_new_contacts(w);
 done:
   free(off); free(adj); free(fill); free(stack);
-  free(is.bodies); free(is.contacts); free(is.pv); free(is.vc); free(is.pc);
+  free(is.bodies); free(is.contacts); free(is.joints); free(is.pv); free(is.vc); free(is.pc);
 }
 
 
@@ -2227,6 +3612,7 @@ static void world_free(World* w) {
   free(w->b); free(w->f); free(w->c); free(w->move);
   free(w->bfix_off); free(w->bfix); free(w->hint);
   free(w->ev_i); free(w->ev_d); free(w->accept); free(w->slot_map); free(w->new_rows);
+  free(w->j); free(w->bj_off); free(w->bj);
   memset(w, 0, sizeof(*w));
 }
 
@@ -2332,6 +3718,38 @@ static int world_load(World* w, const b2g_batch* d, int wi) {
     manifold_load(&c->m, cd, ci);
   }
   for (int i = 0; i < w->nmove; ++i) w->move[i] = d->move[m0 + i];
+  /* joints, and each body's joint list (world.py:268-273) */
+  int j0 = d->joint_off ? d->joint_off[wi] : 0, j1 = d->joint_off ? d->joint_off[wi + 1] : 0;
+  w->nj = j1 - j0;
+  w->j = (Joint*)calloc((size_t)w->nj + 1, sizeof(Joint));
+  w->bj_off = (int*)calloc((size_t)w->nb + 1, sizeof(int));
+  w->bj = (int*)calloc((size_t)(2 * w->nj) + 1, sizeof(int));
+  if (!w->j || !w->bj_off || !w->bj) return -1;
+  for (int k = 0; k < w->nj; ++k) {
+    Joint* J = &w->j[k];
+    memcpy(J->i, d->joint_i + (size_t)(j0 + k) * B2G_JOINT_NI, sizeof(int32_t) * B2G_JOINT_NI);
+    memcpy(J->d, d->joint_d + (size_t)(j0 + k) * B2G_JOINT_ND, sizeof(double) * B2G_JOINT_ND);
+    int t = J->i[B2G_JI_TYPE], a = J->i[B2G_JI_BODY_A], b = J->i[B2G_JI_BODY_B];
+    if (t < B2G_JT_REVOLUTE || t > B2G_JT_ROPE) return -4;
+    if (a < -1 || a >= w->nb || b < 0 || b >= w->nb || (a < 0 && t != B2G_JT_MOUSE)) return -2;
+    if (t == B2G_JT_GEAR) {
+      int c = J->i[B2G_JI_BODY_C], dd = J->i[B2G_JI_BODY_D];
+      if (a < 0 || c < 0 || c >= w->nb || dd < 0 || dd >= w->nb) return -2;
+    }
+    if (a >= 0) w->bj_off[a + 1]++;
+    w->bj_off[b + 1]++;
+  }
+  for (int i = 0; i < w->nb; ++i) w->bj_off[i + 1] += w->bj_off[i];
+  {
+    int* fill = (int*)calloc((size_t)w->nb + 1, sizeof(int));
+    if (!fill) return -1;
+    for (int k = 0; k < w->nj; ++k) {
+      int a = w->j[k].i[B2G_JI_BODY_A], b = w->j[k].i[B2G_JI_BODY_B];
+      if (a >= 0) w->bj[w->bj_off[a] + fill[a]++] = k;
+      w->bj[w->bj_off[b] + fill[b]++] = k;
+    }
+    free(fill);
+  }
   return 0;
 }
 
@@ -2395,7 +3813,7 @@ ORC_EXPORT void orc_counts(void* h, int32_t* counts) {
 ORC_EXPORT int orc_save(void* h, b2g_batch_out* o) {
   Batch* B = (Batch*)h;
   if (o->n_worlds != B->nw) return -1;
-  int bo = 0, fo = 0, co = 0, mo = 0;
+  int bo = 0, fo = 0, co = 0, mo = 0, jo = 0;
   o->con_off[0] = 0;
   o->move_off[0] = 0;
   for (int wi = 0; wi < B->nw; ++wi) {
@@ -2450,6 +3868,12 @@ ORC_EXPORT int orc_save(void* h, b2g_batch_out* o) {
       manifold_save(&c->m, cd, ci);
     }
     for (int i = 0; i < w->nmove; ++i) o->move[mo + i] = w->move[i];
+    if (o->joint_d && o->joint_i)
+      for (int k = 0; k < w->nj; ++k) {
+        memcpy(o->joint_i + (size_t)(jo + k) * B2G_JOINT_NI, w->j[k].i, sizeof(int32_t) * B2G_JOINT_NI);
+        memcpy(o->joint_d + (size_t)(jo + k) * B2G_JOINT_ND, w->j[k].d, sizeof(double) * B2G_JOINT_ND);
+      }
+    jo += w->nj;
     bo += w->nb; fo += w->nf; co += w->nc; mo += w->nmove;
     o->con_off[wi + 1] = co;
     o->move_off[wi + 1] = mo;

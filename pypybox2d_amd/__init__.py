@@ -12,11 +12,15 @@ from .shapes import Circle, Edge, Loop, MassData, Polygon, Shape
 from .fixture import Fixture
 from .body import Body
 from .world import World
+from . import joints
+from .joints import (DistanceJoint, FrictionJoint, GearJoint, MouseJoint, PrismaticJoint, PulleyJoint, RevoluteJoint,
+                     RopeJoint, WeldJoint, WheelJoint)
 
 __all__ = ["World", "Body", "Fixture", "Circle", "Polygon", "Edge", "Loop", "Shape", "MassData", "Vec2", "Mat22",
            "Transform", "AABB", "Sweep", "PhysicsError", "LockedError", "EmptyFixtureError", "PI", "EPSILON",
            "MAX_FLOAT", "NUMBER_TYPES", "scalar_cross", "min_vector", "max_vector", "clamp", "clamp_vector",
-           "BatchedWorld"]
+           "BatchedWorld", "DistanceJoint", "RevoluteJoint", "FrictionJoint", "PrismaticJoint", "WeldJoint",
+           "RopeJoint", "WheelJoint", "MouseJoint", "PulleyJoint", "GearJoint"]
 
 
 def __getattr__(name):

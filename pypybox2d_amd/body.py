@@ -450,6 +450,17 @@ class Body(object):
             return []
         return [c for c in self._world.contacts if c.body_a is self or c.body_b is self]
 
+    def _should_collide(self, other):
+        """body.py:910-928: at least one body dynamic, and no joint between the two that keeps
+        them from colliding."""
+        if self._type != Body.DYNAMIC and other._type != Body.DYNAMIC:
+            return False
+        for joint in self.joints:
+            if (joint.body_a is self and joint.body_b is other) or (joint.body_b is self and joint.body_a is other):
+                if not joint.collide_connected:
+                    return False
+        return True
+
     def get_world_point(self, local_point):
         self._pull()
         return self._xf * local_point

@@ -40,6 +40,14 @@ def build_world(scene, precision: str = "fp64", device: int = 0) -> World:
             world.create_kinematic_body(**kw)
         else:
             world.create_dynamic_body(**kw)
+    made = []
+    for js in scene.get("joints", ()):
+        kw = {k: v for k, v in js.items() if k not in ("type", "bodies", "joints")}
+        create = getattr(world, "create_%s_joint" % js["type"])
+        if js["type"] == "gear":
+            made.append(create(made[js["joints"][0]], made[js["joints"][1]], **kw))
+        else:
+            made.append(create(*[world.bodies[i] for i in js["bodies"]], **kw))
     return world
 
 

@@ -5584,6 +5584,8 @@ B2G_API int b2g_destroy(b2g_ctx* ctx) {
 
 B2G_API int b2g_upload(b2g_ctx* ctx, const b2g_batch* batch) {
   if (!ctx || !batch || batch->n_worlds <= 0) return set_error(ctx, B2G_E_INVALID, "bad arguments");
+  if (batch->joint_off && batch->joint_off[batch->n_worlds] > 0)
+    return set_error(ctx, B2G_E_UNSUPPORTED, "joints are not solved on the device yet");
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   free_device(ctx);

@@ -72,6 +72,39 @@ ED_OLD = 12
 EV_ND = 24
 
 
+# joint records (b2g_layout.h: world joints in World.joints order)
+JT_REVOLUTE, JT_PRISMATIC, JT_DISTANCE, JT_PULLEY, JT_MOUSE, JT_GEAR, JT_WHEEL, JT_WELD, JT_FRICTION, \
+    JT_ROPE = range(1, 11)
+JI_TYPE, JI_BODY_A, JI_BODY_B, JI_FLAGS, JI_STATE, JI_BODY_C, JI_BODY_D, JI_PAD = range(8)
+JOINT_NI = 8
+JF_COLLIDE_CONNECTED, JF_LIMIT, JF_MOTOR, JF_REVOLUTE_A, JF_REVOLUTE_B = 1, 2, 4, 8, 16
+LIMIT_INACTIVE, LIMIT_AT_LOWER, LIMIT_AT_UPPER, LIMIT_EQUAL = range(4)
+(JD_LA_X, JD_LA_Y, JD_LB_X, JD_LB_Y, JD_REF, JD_IMP_X, JD_IMP_Y, JD_IMP_Z, JD_MOTOR_IMP, JD_MOTOR_SPEED,
+ JD_MAX_MOTOR, JD_LOWER, JD_UPPER, JD_AXIS_X, JD_AXIS_Y, JD_PERP_X, JD_PERP_Y, JD_FREQ, JD_DAMP, JD_LENGTH,
+ JD_RATIO, JD_G0_X, JD_G0_Y, JD_G1_X, JD_G1_Y, JD_SPRING_IMP, JD_MAX_TORQUE, JD_REF_B, JD_LEN_A,
+ JD_LEN_B) = range(30)
+JD_RA_X, JD_RA_Y, JD_RB_X, JD_RB_Y = range(32, 36)
+JD_S = 36
+JD_REV_MASS, JD_REV_MOTOR_MASS = JD_S, JD_S + 9
+(JD_PRI_AXIS_X, JD_PRI_AXIS_Y, JD_PRI_PERP_X, JD_PRI_PERP_Y, JD_PRI_A1, JD_PRI_A2, JD_PRI_S1, JD_PRI_S2,
+ JD_PRI_MOTOR_MASS, JD_PRI_K) = range(JD_S, JD_S + 10)
+JD_DIS_U_X, JD_DIS_U_Y, JD_DIS_MASS, JD_DIS_GAMMA, JD_DIS_BIAS = range(JD_S, JD_S + 5)
+JD_PUL_UA_X, JD_PUL_UA_Y, JD_PUL_UB_X, JD_PUL_UB_Y, JD_PUL_MASS = range(JD_S, JD_S + 5)
+JD_MOU_MASS = JD_S
+JD_MOU_GAMMA, JD_MOU_BETA, JD_MOU_C_X, JD_MOU_C_Y = range(JD_S + 4, JD_S + 8)
+(JD_WHE_AX_X, JD_WHE_AX_Y, JD_WHE_AY_X, JD_WHE_AY_Y, JD_WHE_S_AX, JD_WHE_S_BX, JD_WHE_S_AY, JD_WHE_S_BY,
+ JD_WHE_MASS, JD_WHE_MOTOR_MASS, JD_WHE_SPRING_MASS, JD_WHE_BIAS, JD_WHE_GAMMA) = range(JD_S, JD_S + 13)
+JD_WEL_MASS = JD_S
+JD_FRI_LMASS, JD_FRI_AMASS = JD_S, JD_S + 4
+JD_ROP_U_X, JD_ROP_U_Y, JD_ROP_LENGTH, JD_ROP_MASS = range(JD_S, JD_S + 4)
+(JD_GEA_JVAC_X, JD_GEA_JVAC_Y, JD_GEA_JVBD_X, JD_GEA_JVBD_Y, JD_GEA_JWA, JD_GEA_JWB, JD_GEA_JWC, JD_GEA_JWD,
+ JD_GEA_MASS) = range(JD_RA_X, JD_RA_X + 9)
+JD_GEA_V = JD_GEA_MASS + 1
+JD_GEA_C = JD_GEA_V + 8
+JD_GEA_A = JD_GEA_C + 8
+JOINT_ND = 64
+
+
 def pack_id(ia: int, ib: int, ta: int, tb: int) -> int:
     """Pack a contact-point id tuple (contact_util.py:78-79) into one int32."""
     u = (ia & 255) | ((ib & 255) << 8) | ((ta & 255) << 16) | ((tb & 255) << 24)
@@ -93,6 +126,7 @@ class CBatch(ctypes.Structure):
         ("fix_d", ctypes.c_void_p), ("fix_i", ctypes.c_void_p),
         ("con_d", ctypes.c_void_p), ("con_i", ctypes.c_void_p),
         ("move", ctypes.c_void_p),
+        ("joint_off", ctypes.c_void_p), ("joint_d", ctypes.c_void_p), ("joint_i", ctypes.c_void_p),
     ]
 
 
@@ -105,6 +139,7 @@ class CBatchOut(ctypes.Structure):
         ("fix_d", ctypes.c_void_p), ("fix_i", ctypes.c_void_p),
         ("con_d", ctypes.c_void_p), ("con_i", ctypes.c_void_p),
         ("move", ctypes.c_void_p),
+        ("joint_d", ctypes.c_void_p), ("joint_i", ctypes.c_void_p),
     ]
 
 
@@ -128,7 +163,22 @@ class BatchArrays:
     con_d: np.ndarray
     con_i: np.ndarray
     move: np.ndarray
+    joint_off: Optional[np.ndarray] = None  # n_worlds + 1 (None: no joints)
+    joint_d: Optional[np.ndarray] = None
+    joint_i: Optional[np.ndarray] = None
     _keep: list = field(default_factory=list, repr=False)
+
+    def __post_init__(self):
+        if self.joint_off is None:
+            self.joint_off = np.zeros(int(np.asarray(self.world_d).reshape(-1, WORLD_ND).shape[0]) + 1, np.int32)
+        if self.joint_d is None:
+            self.joint_d = np.zeros((0, JOINT_ND))
+        if self.joint_i is None:
+            self.joint_i = np.zeros((0, JOINT_NI), np.int32)
+
+    @property
+    def n_joints(self) -> int:
+        return int(np.asarray(self.joint_d).reshape(-1, JOINT_ND).shape[0])
 
     @property
     def n_worlds(self) -> int:
@@ -159,13 +209,15 @@ class BatchArrays:
                            f(self.body_d).reshape(-1, BODY_ND), i(self.body_i).reshape(-1, BODY_NI),
                            f(self.fix_d).reshape(-1, FIX_ND), i(self.fix_i).reshape(-1, FIX_NI),
                            f(self.con_d).reshape(-1, CON_ND), i(self.con_i).reshape(-1, CON_NI),
-                           i(self.move))
+                           i(self.move), i(self.joint_off), f(self.joint_d).reshape(-1, JOINT_ND),
+                           i(self.joint_i).reshape(-1, JOINT_NI))
 
     def as_struct(self) -> CBatch:
         b = self.normalized()
         s = CBatch(b.n_worlds, _ptr(b.body_off), _ptr(b.fix_off), _ptr(b.con_off), _ptr(b.move_off),
                    _ptr(b.world_d), _ptr(b.world_i), _ptr(b.body_d), _ptr(b.body_i),
-                   _ptr(b.fix_d), _ptr(b.fix_i), _ptr(b.con_d), _ptr(b.con_i), _ptr(b.move))
+                   _ptr(b.fix_d), _ptr(b.fix_i), _ptr(b.con_d), _ptr(b.con_i), _ptr(b.move),
+                   _ptr(b.joint_off), _ptr(b.joint_d), _ptr(b.joint_i))
         self._keep = [b, s]
         return s
 
@@ -175,20 +227,32 @@ class BatchArrays:
         f0, f1 = self.fix_off[w], self.fix_off[w + 1]
         c0, c1 = self.con_off[w], self.con_off[w + 1]
         m0, m1 = self.move_off[w], self.move_off[w + 1]
+        j0, j1 = self.joint_off[w], self.joint_off[w + 1]
         return BatchArrays(np.array([0, b1 - b0], np.int32), np.array([0, f1 - f0], np.int32),
                            np.array([0, c1 - c0], np.int32), np.array([0, m1 - m0], np.int32),
                            self.world_d[w:w + 1].copy(), self.world_i[w:w + 1].copy(),
                            self.body_d[b0:b1].copy(), self.body_i[b0:b1].copy(),
                            self.fix_d[f0:f1].copy(), self.fix_i[f0:f1].copy(),
                            self.con_d[c0:c1].copy(), self.con_i[c0:c1].copy(),
-                           self.move[m0:m1].copy())
+                           self.move[m0:m1].copy(), np.array([0, j1 - j0], np.int32),
+                           self.joint_d[j0:j1].copy(), self.joint_i[j0:j1].copy())
 
     def copy(self) -> "BatchArrays":
-        return BatchArrays(*(getattr(self, n).copy() for n in _FIELDS))
+        return BatchArrays(*(getattr(self, n).copy() for n in ALL_FIELDS))
 
 
 _FIELDS = ("body_off", "fix_off", "con_off", "move_off", "world_d", "world_i", "body_d",
            "body_i", "fix_d", "fix_i", "con_d", "con_i", "move")
+_JOINT_FIELDS = ("joint_off", "joint_d", "joint_i")
+ALL_FIELDS = _FIELDS + _JOINT_FIELDS
+
+
+def load_batch(z, prefix: str = "") -> BatchArrays:
+    """A batch from a mapping of arrays named prefix + field (an npz file); the joint arrays
+    are optional."""
+    names = getattr(z, "files", None) or list(z.keys())
+    return BatchArrays(*[z[prefix + n] for n in _FIELDS],
+                       *[z[prefix + n] if prefix + n in names else None for n in _JOINT_FIELDS])
 
 
 def concat(worlds: List[BatchArrays]) -> BatchArrays:
@@ -209,7 +273,9 @@ def concat(worlds: List[BatchArrays]) -> BatchArrays:
                        np.concatenate([w.fix_i for w in worlds]).reshape(-1, FIX_NI),
                        np.concatenate([w.con_d for w in worlds]).reshape(-1, CON_ND),
                        np.concatenate([w.con_i for w in worlds]).reshape(-1, CON_NI),
-                       np.concatenate([w.move for w in worlds]).astype(np.int32))
+                       np.concatenate([w.move for w in worlds]).astype(np.int32), offs("joint_off"),
+                       np.concatenate([w.joint_d for w in worlds]).reshape(-1, JOINT_ND),
+                       np.concatenate([w.joint_i for w in worlds]).reshape(-1, JOINT_NI).astype(np.int32))
 
 
 class OutBuffers:
@@ -234,10 +300,13 @@ class OutBuffers:
         self.move = np.zeros(self.move_cap, np.int32)
         self.body_off = like.body_off.copy()
         self.fix_off = like.fix_off.copy()
+        self.joint_off = np.ascontiguousarray(like.joint_off, dtype=np.int32).copy()
+        self.joint_d = np.ascontiguousarray(like.joint_d, dtype=np.float64).reshape(-1, JOINT_ND).copy()
+        self.joint_i = np.ascontiguousarray(like.joint_i, dtype=np.int32).reshape(-1, JOINT_NI).copy()
         self.struct = CBatchOut(nw, self.con_cap, self.move_cap, _ptr(self.con_off), _ptr(self.move_off),
                                 _ptr(self.world_d), _ptr(self.world_i), _ptr(self.body_d), _ptr(self.body_i),
                                 _ptr(self.fix_d), _ptr(self.fix_i), _ptr(self.con_d), _ptr(self.con_i),
-                                _ptr(self.move))
+                                _ptr(self.move), _ptr(self.joint_d), _ptr(self.joint_i))
 
     def to_batch(self) -> BatchArrays:
         nc = int(self.con_off[-1])
@@ -245,4 +314,5 @@ class OutBuffers:
         return BatchArrays(self.body_off.copy(), self.fix_off.copy(), self.con_off.copy(),
                            self.move_off.copy(), self.world_d.copy(), self.world_i.copy(),
                            self.body_d.copy(), self.body_i.copy(), self.fix_d.copy(), self.fix_i.copy(),
-                           self.con_d[:nc].copy(), self.con_i[:nc].copy(), self.move[:nm].copy())
+                           self.con_d[:nc].copy(), self.con_i[:nc].copy(), self.move[:nm].copy(),
+                           self.joint_off.copy(), self.joint_d.copy(), self.joint_i.copy())

@@ -22,6 +22,15 @@ Workloads (gravity (0, -10), dt = 1/60, 8 velocity / 3 position iterations):
   filtered_world   mixed bodies with category/mask/group filtering and a kinematic platform
   projectiles  fast bodies (some bullets) shot at a thin wall and into the ground
                (continuous physics)
+  joint_bridge    planks pinned end to end by revolute joints, shapes dropped onto them
+  joint_machines  one mechanism per joint type side by side (limits, motors, springs, a
+                  pulley, a rope, a car on wheel joints, a friction joint, a mouse joint)
+  joint_gears     gear joints over revolute and prismatic joints
+
+A scene may carry ``joints``: each a dict with ``type`` (revolute, prismatic, distance,
+pulley, mouse, gear, wheel, weld, friction, rope), ``bodies`` (indices into the body list;
+one body for a mouse joint) or, for a gear joint, ``joints`` (indices into the joint list),
+and the keyword arguments of the reference's create_<type>_joint.
 """
 from __future__ import annotations
 
@@ -243,6 +252,142 @@ def projectiles(seed: int = 3):
         bodies.append(_body("dynamic", (-12.0 + 4.0 * k, 6.0), [_fix(shapes[k], density=1.0, friction=0.4)],
                             linear_velocity=(float(rng.uniform(-5, 5)), -float(rng.uniform(50, 90)))))
     return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies}
+
+
+def _joint(kind, *bodies, **params):
+    j = {"type": kind, "bodies": tuple(bodies)}
+    j.update(params)
+    return j
+
+
+def joint_bridge(planks: int = 30):
+    """The reference testbed's bridge (testbed/bridge.py): planks of half extents 0.5 x 0.125
+    (density 20, friction 0.2) pinned end to end by revolute joints between the ground and the
+    ground again, with triangles and circles dropped onto it."""
+    bodies = [_body("static", (0.0, 0.0), [_fix(("edge", (-40.0, 0.0), (40.0, 0.0)))])]
+    joints = []
+    prev = 0
+    for i in range(planks):
+        bodies.append(_body("dynamic", (-14.5 + i, 5.0), [_fix(("box", 0.5, 0.125), density=20.0, friction=0.2)]))
+        joints.append(_joint("revolute", prev, len(bodies) - 1, anchor=(-15.0 + i, 5.0)))
+        prev = len(bodies) - 1
+    joints.append(_joint("revolute", prev, 0, anchor=(-15.0 + planks, 5.0)))
+    for i in range(2):
+        bodies.append(_body("dynamic", (-8.0 + 8.0 * i, 12.0),
+                            [_fix(("polygon", [(-0.5, 0.0), (0.5, 0.0), (0.0, 1.5)]), density=1.0)]))
+    for i in range(3):
+        bodies.append(_body("dynamic", (-6.0 + 6.0 * i, 10.0), [_fix(("circle", 0.5), density=1.0)]))
+    return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies, "joints": joints}
+
+
+def joint_machines():
+    """Every joint type in one world, each mechanism on its own stretch of a long ground edge:
+    a pendulum between revolute limits, a motor-driven wheel, a revolute joint with equal limits,
+    a prismatic elevator with limits and a motor, a slanted prismatic slide, a welded cantilever,
+    a chain held by a rope joint, rigid and soft distance joints, a pulley, a car on wheel joints
+    (one motor), a box held back by a friction joint and a box dragged by a mouse joint."""
+    pi = math.pi
+    bodies = [_body("static", (0.0, 0.0), [_fix(("edge", (-80.0, 0.0), (80.0, 0.0)), friction=0.6)])]
+    joints = []
+
+    def add(b):
+        bodies.append(b)
+        return len(bodies) - 1
+    # pendulum between limits (starts swinging)
+    p = add(_body("dynamic", (-60.0, 6.0), [_fix(("box", 0.25, 1.5), density=2.0)], angular_velocity=8.0))
+    joints.append(_joint("revolute", 0, p, anchor=(-60.0, 7.5), lower_angle=-0.15 * pi, upper_angle=0.15 * pi,
+                         limit_enabled=True))
+    p = add(_body("dynamic", (-56.0, 6.0), [_fix(("box", 0.25, 1.5), density=2.0)], angular_velocity=-8.0))
+    joints.append(_joint("revolute", 0, p, anchor=(-56.0, 7.5), lower_angle=-0.15 * pi, upper_angle=0.15 * pi,
+                         limit_enabled=True))
+    # motor-driven wheel with a box on top
+    wh = add(_body("dynamic", (-52.0, 3.0), [_fix(("circle", 1.0), density=1.0, friction=0.8)]))
+    joints.append(_joint("revolute", 0, wh, anchor=(-52.0, 3.0), motor_enabled=True, motor_speed=4.0,
+                         max_motor_torque=50.0))
+    add(_body("dynamic", (-52.0, 4.6), [_fix(("box", 0.5, 0.5), density=1.0, friction=0.8)]))
+    # equal limits: held at 0.3 rad
+    e = add(_body("dynamic", (-46.0, 5.0), [_fix(("box", 1.0, 0.2), density=1.0)]))
+    joints.append(_joint("revolute", 0, e, anchor=(-47.0, 5.0), lower_angle=0.3, upper_angle=0.3,
+                         limit_enabled=True))
+    # prismatic elevator (limits 0..4, motor up) carrying a box
+    el = add(_body("dynamic", (-38.0, 1.0), [_fix(("box", 2.0, 0.2), density=2.0, friction=0.6)]))
+    joints.append(_joint("prismatic", 0, el, anchor=(-38.0, 1.0), axis=(0.0, 1.0), lower_limit=0.0,
+                         upper_limit=4.0, limit_enabled=True, motor_enabled=True, motor_speed=1.5,
+                         max_motor_force=2000.0))
+    add(_body("dynamic", (-38.0, 1.8), [_fix(("box", 0.5, 0.5), density=1.0, friction=0.6)]))
+    # slanted slide, no limit or motor
+    sl = add(_body("dynamic", (-30.0, 4.0), [_fix(("box", 0.5, 0.5), density=1.0)]))
+    joints.append(_joint("prismatic", 0, sl, anchor=(-30.0, 4.0), axis=(1.0, -0.4)))
+    # welded cantilever of six planks
+    prev = 0
+    for i in range(6):
+        b = add(_body("dynamic", (-21.5 + i, 5.0), [_fix(("box", 0.5, 0.125), density=20.0, friction=0.2)]))
+        joints.append(_joint("weld", prev, b, anchor=(-22.0 + i, 5.0)))
+        prev = b
+    # chain of planks under a rope joint (testbed/ropes.py)
+    prev = 0
+    for i in range(8):
+        b = add(_body("dynamic", (-9.5 + i, 15.0), [_fix(("box", 0.5, 0.125), density=20.0, friction=0.2)]))
+        joints.append(_joint("revolute", prev, b, anchor=(-10.0 + i, 15.0)))
+        prev = b
+    joints.append(_joint("rope", 0, prev, local_anchor_a=(-10.0, 15.0), local_anchor_b=(0.5, 0.0),
+                         max_length=7.01))
+    # distance joints: a rigid pair of circles hanging from the ground, and soft springs
+    c1 = add(_body("dynamic", (4.0, 10.0), [_fix(("circle", 0.4), density=1.0)]))
+    c2 = add(_body("dynamic", (5.5, 9.0), [_fix(("circle", 0.4), density=1.0)]))
+    joints.append(_joint("distance", 0, c1, anchor_a=(4.0, 14.0), anchor_b=(4.0, 10.0)))
+    joints.append(_joint("distance", c1, c2, anchor_a=(4.0, 10.0), anchor_b=(5.5, 9.0), collide_connected=True))
+    s1 = add(_body("dynamic", (10.0, 9.0), [_fix(("box", 0.4, 0.4), density=1.0)]))
+    s2 = add(_body("dynamic", (12.0, 9.0), [_fix(("box", 0.4, 0.4), density=1.0)]))
+    joints.append(_joint("distance", 0, s1, anchor_a=(10.0, 13.0), anchor_b=(10.0, 9.4), frequency=3.0,
+                         damping_ratio=0.4))
+    joints.append(_joint("distance", s1, s2, anchor_a=(10.0, 9.0), anchor_b=(12.0, 9.0), frequency=4.0,
+                         damping_ratio=0.1))
+    # pulley (testbed/pulley.py, smaller)
+    pa = add(_body("dynamic", (20.0, 8.0), [_fix(("box", 0.5, 1.0), density=5.0)]))
+    pb = add(_body("dynamic", (26.0, 8.0), [_fix(("box", 0.5, 1.0), density=5.0)]))
+    joints.append(_joint("pulley", pa, pb, ground_anchor_a=(20.0, 14.0), ground_anchor_b=(26.0, 14.0),
+                         anchor_a=(20.0, 9.0), anchor_b=(26.0, 9.0), ratio=1.5))
+    # car: chassis on two wheel joints, rear wheel driven
+    ch = add(_body("dynamic", (40.0, 1.4), [_fix(("box", 1.5, 0.4), density=1.0)]))
+    for k, x in enumerate((38.8, 41.2)):
+        w_ = add(_body("dynamic", (x, 0.6), [_fix(("circle", 0.4), density=1.0, friction=0.9)]))
+        joints.append(_joint("wheel", ch, w_, anchor=(x, 0.6), axis=(0.0, 1.0), frequency=4.0, damping_ratio=0.7,
+                             motor_enabled=(k == 0), motor_speed=-8.0, max_motor_torque=20.0))
+    # friction joint holding back a sliding box
+    fr = add(_body("dynamic", (55.0, 0.5), [_fix(("box", 0.5, 0.5), density=1.0, friction=0.1)],
+                   linear_velocity=(6.0, 0.0), angular_velocity=1.0))
+    joints.append(_joint("friction", 0, fr, local_anchor_a=(55.0, 0.0), local_anchor_b=(0.0, 0.0), max_force=8.0,
+                         max_torque=2.0))
+    # mouse joint dragging a box toward a point
+    mo = add(_body("dynamic", (65.0, 3.0), [_fix(("box", 0.5, 0.5), density=1.0)]))
+    joints.append({"type": "mouse", "bodies": (mo,), "target": (68.0, 7.0), "max_force": 300.0, "frequency": 5.0,
+                   "damping_ratio": 0.7})
+    return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies, "joints": joints}
+
+
+def joint_gears():
+    """The reference testbed's gears (testbed/gear.py): a box pinned to a static circle and to a
+    dynamic circle, coupled by a gear joint; two circles on revolute joints and a box on a
+    prismatic joint to the ground, coupled by gear joints (revolute-revolute and
+    revolute-prismatic)."""
+    bodies = [_body("static", (0.0, 0.0), [_fix(("edge", (50.0, 0.0), (-50.0, 0.0)))])]
+    bodies.append(_body("static", (10.0, 9.0), [_fix(("circle", 1.0))]))                    # 1
+    bodies.append(_body("dynamic", (10.0, 8.0), [_fix(("box", 0.5, 0.5), density=5.0)]))    # 2
+    bodies.append(_body("dynamic", (10.0, 6.0), [_fix(("circle", 2.0), density=5.0)]))     # 3
+    bodies.append(_body("dynamic", (-3.0, 12.0), [_fix(("circle", 1.0), density=5.0)]))    # 4
+    bodies.append(_body("dynamic", (0.0, 12.0), [_fix(("circle", 2.0), density=5.0)]))     # 5
+    bodies.append(_body("dynamic", (2.5, 12.0), [_fix(("box", 0.5, 0.5), density=5.0)]))   # 6
+    joints = [_joint("revolute", 2, 1, anchor=(10.0, 9.0)),                                 # 0
+              _joint("revolute", 2, 3, anchor=(10.0, 6.0)),                                 # 1
+              {"type": "gear", "joints": (0, 1), "ratio": 2.0},                             # 2
+              _joint("revolute", 0, 4, anchor=(-3.0, 12.0)),                                # 3
+              _joint("revolute", 0, 5, anchor=(0.0, 12.0)),                                 # 4
+              _joint("prismatic", 0, 6, anchor=(2.5, 12.0), axis=(0.0, 1.0), lower_limit=-5.0,
+                     upper_limit=5.0, limit_enabled=True),                                   # 5
+              {"type": "gear", "joints": (3, 4), "ratio": 2.0},                             # 6
+              {"type": "gear", "joints": (4, 5), "ratio": -1.0 / 2.0}]                      # 7
+    return {"gravity": (0.0, -10.0), "do_sleep": True, "warm_starting": True, "bodies": bodies, "joints": joints}
 
 
 def listener_filter_verdict(index_a: int, bits_a, index_b: int, bits_b) -> bool:

@@ -40,6 +40,7 @@ from typing import List, Optional
 
 import numpy as np
 
+from . import joints
 from . import layout as L
 from .body import Body
 from .common import LockedError, Vec2
@@ -286,9 +287,17 @@ class World(object):
         if body not in self.bodies:
             raise ValueError("Body not in this world")
         self._pull()
+        # Body._detached_from_world (body.py:310-345): its joints are destroyed first (the joint
+        # destruction listener, then World.destroy_joint)
+        for joint in list(body.joints):
+            if joint in self.joints:
+                if self.joint_destruction_listener is not None:
+                    self.joint_destruction_listener(joint)
+                self.destroy_joint(joint)
+        del body.joints[:]
         fixtures = set(id(f) for f in body.fixtures)
-        # Body._detached_from_world (body.py:310-345): its contacts are destroyed (end_contact for the
-        # touching ones, ContactManager.destroy), then the fixture destruction listener is called
+        # then its contacts (end_contact for the touching ones, ContactManager.destroy), then the
+        # fixture destruction listener is called
         self._destroy_contacts(lambda fa, fb: id(fa) in fixtures or id(fb) in fixtures)
         if self.fixture_destruction_listener is not None:
             for f in list(body.fixtures):
@@ -297,6 +306,81 @@ class World(object):
         self.bodies.remove(body)
         body._world = None
         self._host_dirty = True
+
+    # ---- joints (world.py:252-306, 823-921) ----
+    def add_joint(self, joint, duplicate=False):
+        """Create a joint.  Creating a joint doesn't wake the bodies; contacts between its bodies
+        are flagged for filtering when the joint keeps them from colliding."""
+        if self._locked:
+            raise LockedError("Cannot create a joint while simulating")
+        self._pull()
+        if duplicate:
+            joint = copy(joint)
+        self.joints.append(joint)
+        joint._world = self
+        if joint._body_a is not None:
+            joint._body_a.joints.append(joint)
+        if joint._body_b is not None:
+            joint._body_b.joints.append(joint)
+            if not joint._has(L.JF_COLLIDE_CONNECTED):
+                self._flag_between(joint._body_b, joint._body_a)
+        self._host_dirty = True
+        return joint
+
+    def destroy_joint(self, joint):
+        """Destroy an already created joint; its bodies are woken, and contacts between them
+        are flagged for filtering."""
+        if self._locked:
+            raise LockedError("Cannot destroy a joint while simulating")
+        if joint not in self.joints:
+            raise ValueError("Joint not in world")
+        self._pull()
+        self.joints.remove(joint)
+        for body in joint.bodies:
+            if body is not None:
+                body.awake = True
+                body.joints.remove(joint)
+        if not joint._has(L.JF_COLLIDE_CONNECTED) and joint._body_b is not None:
+            self._flag_between(joint._body_b, joint._body_a)
+        joint._world = None
+        self._host_dirty = True
+
+    def _flag_between(self, body, other):
+        """contact._flag_for_filtering for body's contacts whose other body is `other`."""
+        for k, (fa, fb) in enumerate(self._con_fix):
+            if (fa._body is body and fb._body is other) or (fb._body is body and fa._body is other):
+                self._con_i[k, L.CI_FLAGS] |= L.CF_FILTER
+                self._refilter_pending = True
+
+    def create_distance_joint(self, *args, **kwargs):
+        return self.add_joint(joints.DistanceJoint(*args, **kwargs))
+
+    def create_weld_joint(self, *args, **kwargs):
+        return self.add_joint(joints.WeldJoint(*args, **kwargs))
+
+    def create_friction_joint(self, *args, **kwargs):
+        return self.add_joint(joints.FrictionJoint(*args, **kwargs))
+
+    def create_gear_joint(self, *args, **kwargs):
+        return self.add_joint(joints.GearJoint(*args, **kwargs))
+
+    def create_mouse_joint(self, *args, **kwargs):
+        return self.add_joint(joints.MouseJoint(*args, **kwargs))
+
+    def create_prismatic_joint(self, *args, **kwargs):
+        return self.add_joint(joints.PrismaticJoint(*args, **kwargs))
+
+    def create_pulley_joint(self, *args, **kwargs):
+        return self.add_joint(joints.PulleyJoint(*args, **kwargs))
+
+    def create_revolute_joint(self, *args, **kwargs):
+        return self.add_joint(joints.RevoluteJoint(*args, **kwargs))
+
+    def create_rope_joint(self, *args, **kwargs):
+        return self.add_joint(joints.RopeJoint(*args, **kwargs))
+
+    def create_wheel_joint(self, *args, **kwargs):
+        return self.add_joint(joints.WheelJoint(*args, **kwargs))
 
     # hooks used by Body / Fixture
     def _fixture_added(self, body, fixture):
@@ -411,9 +495,15 @@ class World(object):
                              | (0 if self._auto_clear_forces else L.WF_KEEP_FORCES)
                              | (L.WF_CUSTOM_FILTER if self._custom_filter() else 0)
                              | (L.WF_CONTINUOUS if self._continuous_physics else 0))
+        nj = len(self.joints)
+        jd = np.zeros((nj, L.JOINT_ND))
+        ji = np.zeros((nj, L.JOINT_NI), np.int32)
+        for k, j in enumerate(self.joints):
+            jd[k], ji[k] = j._record(bidx)
         return L.BatchArrays(np.array([0, nb], np.int32), np.array([0, nf], np.int32),
                              np.array([0, len(ci)], np.int32), np.array([0, len(move)], np.int32),
-                             wd, wi, bd, bi, fd, fi, self._con_d.copy(), ci, move)
+                             wd, wi, bd, bi, fd, fi, self._con_d.copy(), ci, move,
+                             np.array([0, nj], np.int32), jd, ji)
 
     def _check_supported(self):
         if self._continuous_physics and self._sub_stepping:
@@ -421,8 +511,8 @@ class World(object):
         if self._continuous_physics and self._custom_filter():
             raise NotImplementedError("a custom contact filter with continuous physics is not supported (the "
                                       "time-of-impact sub-steps create contacts inside the device step)")
-        if self.joints or self.controllers:
-            raise NotImplementedError("joints and controllers are outside the device contact pipeline")
+        if self.controllers:
+            raise NotImplementedError("controllers are outside the device contact pipeline")
         for b in self.bodies:
             for f in b.fixtures:
                 if f._sensor:
@@ -464,6 +554,9 @@ class World(object):
         self._con_i = st.con_i.copy()
         self._con_fix = [(fixtures[a], fixtures[b]) for a, b in zip(st.con_i[:, L.CI_FA], st.con_i[:, L.CI_FB])]
         self._move = [fixtures[k] for k in st.move]
+        for j, d, i in zip(self.joints, st.joint_d, st.joint_i):
+            j._d[:] = d
+            j._i[:] = i
         self._inv_dt0 = float(st.world_d[0, L.WD_INV_DT0])
         self._new_fixture = bool(st.world_i[0, L.WI_FLAGS] & L.WF_NEW_FIXTURE)
 
@@ -512,8 +605,7 @@ class World(object):
             if not ((ba._awake and ba._type != Body.STATIC) or (bb._awake and bb._type != Body.STATIC)):
                 pending = True
                 continue
-            ok = Body.DYNAMIC in (ba._type, bb._type) and (self._contact_filter is None or
-                                                            bool(self._contact_filter(fa, fb)))
+            ok = bb._should_collide(ba) and (self._contact_filter is None or bool(self._contact_filter(fa, fb)))
             self._con_i[k, L.CI_FLAGS] = (fl & ~L.CF_FILTER) if ok else (fl | L.CF_REJECT)
             self._host_dirty = True
         self._refilter_pending = pending

@@ -1,7 +1,7 @@
 """Writes the golden fixtures of the contact pipeline (tests/golden/*.npz).
 
 Run once in the build container:  python tests/golden/make_golden.py [names]
-(names: pyramid stacks mixed filtered listeners continuous queries narrowphase; default all)
+(names: pyramid stacks mixed filtered listeners continuous joints queries narrowphase; default all)
 
 It imports pypybox2d from /root/reference in its default pure-Python flavour
 (the reference tree ships no compiled _common) and records, for small scenes:
@@ -69,7 +69,149 @@ def build_ref_world(scene, continuous_physics=False):
             world.create_kinematic_body(**kw)
         else:
             world.create_dynamic_body(**kw)
+    made = []
+    for js in scene.get("joints", ()):
+        kw = {k: v for k, v in js.items() if k not in ("type", "bodies", "joints")}
+        create = getattr(world, "create_%s_joint" % js["type"])
+        if js["type"] == "gear":
+            made.append(create(made[js["joints"][0]], made[js["joints"][1]], **kw))
+        else:
+            made.append(create(*[world.bodies[i] for i in js["bodies"]], **kw))
     return world
+
+
+# --------------------------------------------------------------------------
+# reference joints -> joint records (b2g_layout.h B2G_JI_* / B2G_JD_*)
+# --------------------------------------------------------------------------
+_JT = {"RevoluteJoint": L.JT_REVOLUTE, "PrismaticJoint": L.JT_PRISMATIC, "DistanceJoint": L.JT_DISTANCE,
+       "PulleyJoint": L.JT_PULLEY, "MouseJoint": L.JT_MOUSE, "GearJoint": L.JT_GEAR, "WheelJoint": L.JT_WHEEL,
+       "WeldJoint": L.JT_WELD, "FrictionJoint": L.JT_FRICTION, "RopeJoint": L.JT_ROPE}
+
+
+def joint_record(j, bidx):
+    """The joint object's attributes in the record layout (attributes a joint sets only in its
+    first velocity-constraint setup read as zero before it)."""
+    jd = np.zeros(L.JOINT_ND)
+    ji = np.zeros(L.JOINT_NI, np.int32)
+    t = _JT[type(j).__name__]
+    ji[L.JI_TYPE] = t
+    ji[L.JI_BODY_A] = bidx[id(j._body_a)] if j._body_a is not None else -1
+    ji[L.JI_BODY_B] = bidx[id(j._body_b)]
+    ji[L.JI_FLAGS] = ((L.JF_COLLIDE_CONNECTED if j._collide_connected else 0)
+                      | (L.JF_LIMIT if getattr(j, "_limit_enabled", False) else 0)
+                      | (L.JF_MOTOR if getattr(j, "_motor_enabled", False) else 0))
+
+    def g(name, default=0.0):
+        return getattr(j, name, default)
+
+    def put2(k, v):
+        if v is not None:
+            jd[k], jd[k + 1] = v[0], v[1]
+
+    def put3(k, v):
+        jd[k], jd[k + 1], jd[k + 2] = v.x, v.y, v.z
+
+    def put33(k, m):
+        if m is not None and not isinstance(m, float):
+            put3(k, m.col1); put3(k + 3, m.col2); put3(k + 6, m.col3)
+
+    def put22(k, m):
+        if m is not None and not isinstance(m, float):
+            jd[k], jd[k + 1], jd[k + 2], jd[k + 3] = m._col1.x, m._col1.y, m._col2.x, m._col2.y
+
+    if t != L.JT_GEAR:
+        put2(L.JD_LA_X, g("_local_anchor_a", None))
+        put2(L.JD_LB_X, g("_local_anchor_b", None))
+        put2(L.JD_RA_X, g("_ra", None))
+        put2(L.JD_RB_X, g("_rb", None))
+    if t in (L.JT_REVOLUTE, L.JT_PRISMATIC, L.JT_WELD):
+        jd[L.JD_REF] = j._reference_angle
+        put3(L.JD_IMP_X, j._impulse)
+    if t in (L.JT_REVOLUTE, L.JT_PRISMATIC):
+        ji[L.JI_STATE] = j._limit_state
+        jd[L.JD_MOTOR_IMP], jd[L.JD_MOTOR_SPEED] = j._motor_impulse, j._motor_speed
+    if t == L.JT_REVOLUTE:
+        jd[L.JD_MAX_MOTOR] = j._max_motor_torque
+        jd[L.JD_LOWER], jd[L.JD_UPPER] = j._lower_angle, j._upper_angle
+        put33(L.JD_REV_MASS, j._mass)
+        jd[L.JD_REV_MOTOR_MASS] = g("_motor_mass")
+    elif t == L.JT_PRISMATIC:
+        jd[L.JD_MAX_MOTOR] = j._max_motor_force
+        jd[L.JD_LOWER], jd[L.JD_UPPER] = j._lower_limit, j._upper_limit
+        put2(L.JD_AXIS_X, j._local_x_axis)
+        put2(L.JD_PERP_X, j._local_y_axis)
+        put2(L.JD_PRI_AXIS_X, j._axis)
+        put2(L.JD_PRI_PERP_X, j._perp)
+        jd[L.JD_PRI_A1], jd[L.JD_PRI_A2], jd[L.JD_PRI_S1], jd[L.JD_PRI_S2] = j._a1, j._a2, j._s1, j._s2
+        jd[L.JD_PRI_MOTOR_MASS] = j._motor_mass
+        put33(L.JD_PRI_K, j._k)
+    elif t == L.JT_DISTANCE:
+        jd[L.JD_LENGTH], jd[L.JD_FREQ], jd[L.JD_DAMP] = j._length, j._frequency, j._damping_ratio
+        jd[L.JD_IMP_X] = j._impulse
+        put2(L.JD_DIS_U_X, j._u)
+        jd[L.JD_DIS_MASS], jd[L.JD_DIS_GAMMA], jd[L.JD_DIS_BIAS] = j._mass, j._gamma, j._bias
+    elif t == L.JT_PULLEY:
+        put2(L.JD_G0_X, j._ground_anchor_a)
+        put2(L.JD_G1_X, j._ground_anchor_b)
+        jd[L.JD_LEN_A], jd[L.JD_LEN_B] = j._length_a, j._length_b
+        jd[L.JD_LENGTH], jd[L.JD_RATIO] = j._constant, j._ratio
+        jd[L.JD_IMP_X] = j._impulse
+        put2(L.JD_PUL_UA_X, g("_ua", None))
+        put2(L.JD_PUL_UB_X, g("_ub", None))
+        jd[L.JD_PUL_MASS] = j._mass
+    elif t == L.JT_MOUSE:
+        put2(L.JD_G0_X, j._target)
+        jd[L.JD_MAX_MOTOR], jd[L.JD_FREQ], jd[L.JD_DAMP] = j._max_force, j._frequency, j._damping_ratio
+        put2(L.JD_IMP_X, j._impulse)
+        put22(L.JD_MOU_MASS, g("_mass", None))
+        jd[L.JD_MOU_GAMMA], jd[L.JD_MOU_BETA] = j._gamma, j._beta
+        put2(L.JD_MOU_C_X, g("_c", None))
+    elif t == L.JT_GEAR:
+        ji[L.JI_BODY_C] = bidx[id(j._body_c)]
+        ji[L.JI_BODY_D] = bidx[id(j._body_d)]
+        ji[L.JI_FLAGS] |= (L.JF_REVOLUTE_A if j._revolute_a else 0) | (L.JF_REVOLUTE_B if j._revolute_b else 0)
+        put2(L.JD_LA_X, j._local_anchor_a)
+        put2(L.JD_LB_X, j._local_anchor_b)
+        put2(L.JD_G0_X, j._local_anchor_c)
+        put2(L.JD_G1_X, j._local_anchor_d)
+        put2(L.JD_AXIS_X, j._local_axis_c)
+        put2(L.JD_PERP_X, j._local_axis_d)
+        jd[L.JD_REF], jd[L.JD_REF_B] = j._reference_angle_a, j._reference_angle_b
+        jd[L.JD_RATIO], jd[L.JD_LENGTH], jd[L.JD_IMP_X] = j._ratio, j._constant, j._impulse
+        if hasattr(j, "_j_vac"):
+            put2(L.JD_GEA_JVAC_X, j._j_vac)
+            put2(L.JD_GEA_JVBD_X, j._j_vbd)
+            jd[L.JD_GEA_JWA], jd[L.JD_GEA_JWB], jd[L.JD_GEA_JWC], jd[L.JD_GEA_JWD] = j._j_wa, j._j_wb, j._j_wc, j._j_wd
+            jd[L.JD_GEA_MASS] = j._mass
+            for k in range(4):
+                put2(L.JD_GEA_V + 2 * k, j._lin_vels[k])
+                put2(L.JD_GEA_C + 2 * k, j._centers[k])
+                jd[L.JD_GEA_A + k] = j._angles[k]
+    elif t == L.JT_WHEEL:
+        put2(L.JD_AXIS_X, j._local_x_axis)
+        put2(L.JD_PERP_X, j._local_y_axis)
+        jd[L.JD_FREQ], jd[L.JD_DAMP] = j._frequency, j._damping_ratio
+        jd[L.JD_MAX_MOTOR], jd[L.JD_MOTOR_SPEED] = j._max_motor_torque, j._motor_speed
+        jd[L.JD_IMP_X], jd[L.JD_MOTOR_IMP], jd[L.JD_SPRING_IMP] = j._impulse, j._motor_impulse, j._spring_impulse
+        put2(L.JD_WHE_AX_X, j._ax)
+        put2(L.JD_WHE_AY_X, j._ay)
+        jd[L.JD_WHE_S_AX], jd[L.JD_WHE_S_BX], jd[L.JD_WHE_S_AY], jd[L.JD_WHE_S_BY] = j._s_ax, j._s_bx, j._s_ay, j._s_by
+        jd[L.JD_WHE_MASS], jd[L.JD_WHE_MOTOR_MASS], jd[L.JD_WHE_SPRING_MASS] = j._mass, j._motor_mass, j._spring_mass
+        jd[L.JD_WHE_BIAS], jd[L.JD_WHE_GAMMA] = j._bias, j._gamma
+    elif t == L.JT_WELD:
+        put33(L.JD_WEL_MASS, j._mass)
+    elif t == L.JT_FRICTION:
+        put2(L.JD_IMP_X, j._linear_impulse)
+        jd[L.JD_IMP_Z] = j._angular_impulse
+        jd[L.JD_MAX_MOTOR], jd[L.JD_MAX_TORQUE] = j._max_force, j._max_torque
+        put22(L.JD_FRI_LMASS, g("_linear_mass", None))
+        jd[L.JD_FRI_AMASS] = g("_angular_mass")
+    elif t == L.JT_ROPE:
+        ji[L.JI_STATE] = j._state
+        jd[L.JD_LENGTH], jd[L.JD_IMP_X] = j._max_length, j._impulse
+        put2(L.JD_ROP_U_X, j._u)
+        jd[L.JD_ROP_LENGTH], jd[L.JD_ROP_MASS] = j._length, j._mass
+    return jd, ji
 
 
 # --------------------------------------------------------------------------
@@ -191,9 +333,15 @@ def snapshot(world):
                          | (L.WF_NEW_FIXTURE if world._new_fixture else 0)
                          | (0 if world._auto_clear_forces else L.WF_KEEP_FORCES)
                          | (L.WF_CONTINUOUS if world._continuous_physics else 0))
+    nj = len(world.joints)
+    jd = np.zeros((nj, L.JOINT_ND))
+    ji = np.zeros((nj, L.JOINT_NI), np.int32)
+    for k, j in enumerate(world.joints):
+        jd[k], ji[k] = joint_record(j, bidx)
     return L.BatchArrays(np.array([0, nb], np.int32), np.array([0, nf], np.int32),
                          np.array([0, nc], np.int32), np.array([0, len(move)], np.int32),
-                         wd, wi, bd, bi, fd, fi, cd, ci, np.array(move, np.int32)), fidx
+                         wd, wi, bd, bi, fd, fi, cd, ci, np.array(move, np.int32),
+                         np.array([0, nj], np.int32), jd, ji), fidx
 
 
 _toi_events = [0]
@@ -243,7 +391,7 @@ def record_run(scene, steps, checkpoints, dt=1.0 / 60, vel_iters=8, pos_iters=3,
 
 def save_run(path, init, hints, states, meta):
     out = {}
-    for name in L._FIELDS:
+    for name in (L.ALL_FIELDS if init.n_joints else L._FIELDS):
         out["init_" + name] = getattr(init, name)
     off = [0]
     flat = []
@@ -258,6 +406,8 @@ def save_run(path, init, hints, states, meta):
         st = states[s]
         for name in ("body_d", "body_i", "fix_d", "con_d", "con_i", "move"):
             out["s%d_%s" % (s, name)] = getattr(st, name)
+        if init.n_joints:
+            out["s%d_joint_d" % s], out["s%d_joint_i" % s] = st.joint_d, st.joint_i
     for k, v in meta.items():
         out["meta_" + k] = np.array(v)
     np.savez_compressed(path, **out)
@@ -608,7 +758,7 @@ def world_queries(scene, steps, rng, nq=200, nr=200):
 
 def main(which=None):
     """Records every fixture, or only those named (pyramid, stacks, mixed, filtered, queries, narrowphase)."""
-    want = set(which or ("pyramid", "stacks", "mixed", "filtered", "listeners", "continuous", "queries",
+    want = set(which or ("pyramid", "stacks", "mixed", "filtered", "listeners", "continuous", "joints", "queries",
                          "narrowphase"))
     if "pyramid" in want:
         pyr_ck = {1, 2, 3, 4, 5, 10, 20, 50, 100, 150, 200, 300, 400, 500, 600}
@@ -656,6 +806,14 @@ def main(which=None):
             np.savez_compressed(path, **z)
             print(name, "toi calls", int(z["toi_stats"][:, 0].sum()), "toi events", int(z["toi_stats"][:, 1].sum()),
                   "steps with events", int(np.count_nonzero(z["toi_stats"][:, 1])))
+    if "joints" in want:
+        ck = {1, 2, 3, 5, 10, 20, 40, 60, 80, 100, 120, 150, 200, 250, 300}
+        # the gear scene runs away in the reference (its gear joints' quirks drive the circles to
+        # the speed cap; a step near 190 overflows), so it is recorded for 120 steps
+        for name, scene, steps in (("bridge", scenes.joint_bridge(), 300), ("machines", scenes.joint_machines(), 300),
+                                   ("gears", scenes.joint_gears(), 120)):
+            init, hints, states = record_run(scene, steps, {c for c in ck if c <= steps})
+            save_run(os.path.join(HERE, "joints_%s_%d.npz" % (name, steps)), init, hints, states, {"steps": steps})
     if "queries" in want:
         np.savez_compressed(os.path.join(HERE, "shape_raycast.npz"), **shape_ray_cases(np.random.default_rng(17)))
         rng = np.random.default_rng(19)
