@@ -63,15 +63,17 @@ B2G_API int b2g_destroy(b2g_ctx* ctx);
 
 /* Copy a batch of worlds to the device (replaces the world-building calls
    World.create_body / Body.attach_fixture / Fixture._create_proxies,
-   world.py:183-233, body.py:185-214, fixture.py:250-261, whose results the
-   caller encodes in the records).  Replaces any previous batch. */
+   world.py:183-233, body.py:185-214, fixture.py:250-261, and World.add_joint,
+   world.py:252-283, whose results the caller encodes in the records).
+   Replaces any previous batch. */
 B2G_API int b2g_upload(b2g_ctx* ctx, const b2g_batch* batch);
 
 /* Advance every world by one World.step(dt, vel_iters, pos_iters)
-   (world.py:307-354) with continuous physics off: ContactManager.collide
-   (contact_manager.py:174-223), World._solve with Island.solve
-   (world.py:475-592, island.py:65-179) and find_new_contacts
-   (contact_manager.py:102-158).  Asynchronous; see b2g_sync. */
+   (world.py:307-354): ContactManager.collide (contact_manager.py:174-223),
+   World._solve with Island.solve and its joints (world.py:475-592,
+   island.py:65-179, joints/), find_new_contacts (contact_manager.py:102-158)
+   and, for worlds with B2G_WF_CONTINUOUS, World.solve_toi (world.py:594-821).
+   Asynchronous; see b2g_sync. */
 B2G_API int b2g_step(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters);
 
 /* n steps; with use_graph != 0 the step is captured once in a HIP graph and
@@ -85,8 +87,8 @@ B2G_API int b2g_sync(b2g_ctx* ctx);
 B2G_API int b2g_counts(b2g_ctx* ctx, int32_t* counts);
 
 /* Copy the full state back (bodies, fat AABBs, ordered contacts with their
-   manifolds and impulses, move buffer) into caller arrays.  Replaces reading
-   Body/Contact/Manifold attributes after World.step. */
+   manifolds and impulses, move buffer, joint records) into caller arrays.
+   Replaces reading Body/Contact/Manifold/Joint attributes after World.step. */
 B2G_API int b2g_download(b2g_ctx* ctx, b2g_batch_out* out);
 
 B2G_API int b2g_last_error(b2g_ctx* ctx, char* buf, size_t n);

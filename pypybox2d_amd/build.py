@@ -18,7 +18,7 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libb2g.so")
 SOURCES = [os.path.join(CSRC, "b2g_engine.hip")]
-DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("b2g_math.hpp", "b2g_collide.hpp", "b2g_raycast.hpp")] + [
+DEPS = SOURCES + [os.path.join(CSRC, f) for f in ("b2g_math.hpp", "b2g_collide.hpp", "b2g_raycast.hpp", "b2g_toi.hpp", "b2g_joints.hpp")] + [
     os.path.join(REPO, "include", f) for f in ("b2g.h", "b2g_layout.h")]
 
 ARCH = os.environ.get("B2G_OFFLOAD_ARCH", "gfx950")

@@ -42,6 +42,7 @@
 #include "b2g_collide.hpp"
 #include "b2g_raycast.hpp"
 #include "b2g_toi.hpp"
+#include "b2g_joints.hpp"
 
 namespace b2g {
 
@@ -75,6 +76,7 @@ enum : int {
   WERR_NAN = 16,          // a body's position, angle or velocity became non-finite
   WERR_ISLAND_SIZE = 32,  // an island of more than 65535 contacts (16-bit island-local indices)
   WERR_TOI = 64,          // a time-of-impact search that the reference would not end (toi.py:335-339)
+  WERR_JOINT = 128,       // a mouse joint's spring too weak for its time step (the reference asserts)
 };
 
 // work counters (b2g_stats): summed over the steps since the last reset
@@ -302,6 +304,20 @@ struct Dev {
   int* g_lev;          // per body
   int* g_parent;       // per body: union-find
   int* g_seed;         // per body (by component root)
+  // joints (b2g_layout.h joint records), per-world segments at w_joint_off[w]; null without joints
+  const int* w_joint_off;
+  double* j_d;            // B2G_JOINT_ND doubles per joint
+  int* j_i;               // B2G_JOINT_NI ints per joint
+  const double4* j_mass;  // 4 per joint: (inverse mass, inverse inertia, mass, 0) of bodies a, b, c, d
+  const int* bj_off;      // per body (global) + 1: the body's joint list in bj
+  const uint32_t* bj;     // world-local joint | other body << 16 (0xFFFF: none), in Body.joints order
+  const int* w_jnc_off;   // per world + 1: its entries in jnc
+  const uint32_t* jnc;    // sorted body pairs (lower | higher << 16) that a joint keeps from colliding
+  int* j_flag;            // per joint: taken by an island this step
+  int* i_joints;          // per-world segment at w_joint_off[w]: the islands' joints (world-local)
+  int4* i_jidx;           // indexed like i_joints: island-local indices of the joint's bodies a, b, c, d
+                          // (-1 - world-local index for a body outside the island's body list)
+  int2* i_jr;             // per island, at its first body (global index): its entries in i_joints
   const int* toi_worlds;  // the worlds with continuous physics (k_toi's grid)
   unsigned char* toi_scr; // per such world: the TOI island's scratch (ToiScratch)
   int* toi_move;          // per world at w_fix_off[w]: the move buffer of a TOI event
@@ -522,6 +538,23 @@ __device__ int update_contact(const Dev<T>& d, int w, int s, int st, int* toi_se
 // the filter test of a flagged contact, the broad-phase overlap test, then Contact.update;
 // returns the new state word (0: destroyed)
 // ---------------------------------------------------------------------------
+// Body._should_collide's joint test (body.py:920-926): does a joint between the two bodies
+// (world-local) keep them from colliding?  A binary search over the world's sorted pairs.
+template <typename T>
+__device__ __forceinline__ bool joint_blocks(const Dev<T>& d, int w, int la, int lb) {
+  if (!d.w_jnc_off) return false;
+  int lo = d.w_jnc_off[w], hi = d.w_jnc_off[w + 1];
+  const uint32_t key = la < lb ? ((uint32_t)la | (uint32_t)lb << 16) : ((uint32_t)lb | (uint32_t)la << 16);
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const uint32_t v = d.jnc[mid];
+    if (v == key) return true;
+    if (v < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return false;
+}
+
 template <typename T>
 __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
   const int f0 = d.w_fix_off[w], b0 = d.w_body_off[w];
@@ -540,7 +573,7 @@ __device__ int narrow_update(const Dev<T>& d, int w, int s, int st) {
     return 0;
   };
   if (st & CS_FILTER) {  // refilter (contact_manager.py:200-208)
-    bool ok = d.b_type[ba] == BT_DYNAMIC || d.b_type[bb] == BT_DYNAMIC;
+    bool ok = (d.b_type[ba] == BT_DYNAMIC || d.b_type[bb] == BT_DYNAMIC) && !joint_blocks(d, w, ia.x, ib.x);
     if (ok) {
       if (d.w_flags[w] & WF_CUSTOM_FILTER) ok = !(st & CS_REJECT);  // the host ran the custom filter
       else if (ia.w == ib.w && ia.w != 0) ok = ia.w > 0;  // World.default_contact_filter
@@ -841,6 +874,9 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) k_i
     }
   }
   __syncthreads();
+  const int j0 = d.w_joint_off ? d.w_joint_off[w] : 0;
+  const int nj = d.w_joint_off ? d.w_joint_off[w + 1] - j0 : 0;
+  for (int k = tid; k < nj; k += kBlock) d.j_flag[j0 + k] = 0;
   for (int i = tid; i < nb; i += kBlock) {  // restore ascending slot order per body
     L.cur[i] = 0;  // from here on: contacts of the body seen by the island search
     L.lev[i] = 0;
@@ -862,7 +898,7 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) k_i
   int* isl_c = d.i_contacts + cb;
   int4* info = d.i_info + b0;
   if (tid == 0) {
-    int nbi_tot = 0, nci_tot = 0, nisl = 0;
+    int nbi_tot = 0, nci_tot = 0, nisl = 0, njt = 0;
     for (int seed = 0; seed < nb; ++seed) {
       uint8_t sb = L.bst[seed];
       if (sb & 16) continue;            // island flag
@@ -871,7 +907,7 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) k_i
       int sp = 0, ns = 0;
       L.stack[sp++] = (uint16_t)seed;
       L.bst[seed] = sb | 16;
-      const int bstart = nbi_tot, cstart = nci_tot;
+      const int bstart = nbi_tot, cstart = nci_tot, jstart = njt;
       int last_allow = 0, span = 0, depth = 0;
       while (sp > 0) {
         const int b = L.stack[--sp];
@@ -930,18 +966,59 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) k_i
           span = max(span, lev_b - (int)((uint32_t)cur_b >> 16));
           depth = max(depth, lev_b);
         }
+        // the body's joints (world.py:548-572), in Body.joints order
+        if (nj > 0) {
+          const int gb = b0 + b;
+          for (int k = d.bj_off[gb]; k < d.bj_off[gb + 1]; ++k) {
+            const uint32_t e = d.bj[k];
+            const int jl = (int)(e & 0xFFFFu), o = (int)(e >> 16);
+            if (d.j_flag[j0 + jl]) continue;
+            if (o == 0xFFFF) {  // a joint on one body (mouse)
+              d.i_joints[j0 + njt++] = jl;
+              d.j_flag[j0 + jl] = 1;
+              continue;
+            }
+            const uint8_t so = L.bst[o];
+            if (!(so & 8)) continue;  // joints to inactive bodies are not simulated
+            d.i_joints[j0 + njt++] = jl;
+            d.j_flag[j0 + jl] = 1;
+            if (so & 16) continue;
+            L.stack[sp++] = (uint16_t)o;
+            L.bst[o] = so | 16;
+          }
+        }
       }
       for (int k = 0; k < ns; ++k) L.bst[L.slist[k]] &= (uint8_t)~16;  // statics may join other islands
       info[nisl++] = make_int4(bstart, (nbi_tot - bstart) | sweep_period(span, depth) << 16, cstart,
                                (nci_tot - cstart) | (last_allow << 30));
+      if (nj > 0) d.i_jr[b0 + bstart] = make_int2(j0 + jstart, njt - jstart);
     }
     L.misc[0] = nisl;
     L.misc[1] = nbi_tot;
     L.misc[2] = nci_tot;
+    L.misc[5] = njt;
     L.misc[3] = nisl > 0 ? atomicAdd(d.q_count, nisl) : 0;
   }
   __syncthreads();
   const int nisl = L.misc[0], nci_tot = L.misc[2], qbase = L.misc[3];
+  // the island joints' bodies as island-local indices (a body outside the island's body list,
+  // a static one, as -1 - world-local index)
+  for (int k = tid; k < L.misc[5]; k += kBlock) {
+    const int* ji = d.j_i + (size_t)(j0 + d.i_joints[j0 + k]) * B2G_JOINT_NI;
+    const int ba = ji[B2G_JI_BODY_A], bb = ji[B2G_JI_BODY_B];
+    const int home = !(L.bst[bb] & 1) ? bb : ba;  // a body of the joint's island
+    const int ist = L.ibs[home];
+    const bool gear = ji[B2G_JI_TYPE] == B2G_JT_GEAR;  // only gear joints have bodies c, d (checked at upload)
+    const int slot[4] = {ba < 0 ? bb : ba, bb, gear ? ji[B2G_JI_BODY_C] : bb, gear ? ji[B2G_JI_BODY_D] : bb};
+    int idx[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int x = slot[q];
+      const uint8_t bs = L.bst[x];
+      idx[q] = (!(bs & 1) && (bs & 16) && L.ibs[x] == ist) ? (int)L.bidx[x] - ist : -1 - x;
+    }
+    d.i_jidx[j0 + k] = make_int4(idx[0], idx[1], idx[2], idx[3]);
+  }
   int* ccount = L.misc + 16;  // islands per size class, then one global add per class
   if (tid < kSizeClasses) ccount[tid] = 0;
   __syncthreads();
@@ -1771,24 +1848,104 @@ __device__ __forceinline__ void sleep_last(const Dev<T>& d, const IslandK& c, in
   d.b_flags[gb] &= ~BF_AWAKE;
 }
 
-// Island.solve on one lane (island.py:65-179).
+// The island's positions and velocities as the joint solver reads and writes them: island
+// bodies in pv, bodies outside the island's body list (static ones) read from the world and
+// never written.
+template <typename T>
+struct IslandIO {
+  const Dev<T>* d;
+  int pvb, b0;
+  __device__ __forceinline__ void pos(int i, V2d& c, double& a) const {
+    double cx, cy;
+    load_pos(*d, pvb, b0, i, cx, cy, a);
+    c = v2<double>(cx, cy);
+  }
+  __device__ __forceinline__ void vel(int i, V2d& v, double& w) const { load_vel(*d, pvb, b0, i, v, w); }
+  __device__ __forceinline__ void set_pos(int i, V2d c, double a) const { store_pos(*d, pvb, i, c.x, c.y, a); }
+  __device__ __forceinline__ void set_vel(int i, V2d v, double w) const { store_vel(*d, pvb, i, v, w); }
+};
+
+// The island's joint k (an entry of i_joints): its record and its bodies.
+template <typename T>
+__device__ __forceinline__ JRec island_joint(const Dev<T>& d, int b0, int j0, int e, JBodies& B) {
+  const int jl = d.i_joints[e];
+  JRec J;
+  J.d = d.j_d + (size_t)(j0 + jl) * B2G_JOINT_ND;
+  J.i = d.j_i + (size_t)(j0 + jl) * B2G_JOINT_NI;
+  const int4 ix = d.i_jidx[e];
+  B.idx[0] = ix.x;
+  B.idx[1] = ix.y;
+  B.idx[2] = ix.z;
+  B.idx[3] = ix.w;
+  const bool gear = J.i[B2G_JI_TYPE] == B2G_JT_GEAR;  // only gear joints have bodies c, d
+  const int bb = J.i[B2G_JI_BODY_B];
+  const int slot[4] = {J.i[B2G_JI_BODY_A] < 0 ? bb : J.i[B2G_JI_BODY_A], bb, gear ? J.i[B2G_JI_BODY_C] : bb,
+                       gear ? J.i[B2G_JI_BODY_D] : bb};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double4 m = d.j_mass[4 * (size_t)(j0 + jl) + q];
+    B.im[q] = m.x;
+    B.iI[q] = m.y;
+    B.mass[q] = m.z;
+    const double2 lc = d.b_lc[b0 + slot[q]];
+    B.lc[q] = v2<double>(lc.x, lc.y);
+  }
+  return J;
+}
+
+// Island.solve on one lane (island.py:65-179), with the island's joints in their places
+// (island.py:107-114 and 141-152).
 template <typename T>
 __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci, int last_allow) {
   const IslandK c = island_k(d, w, bstart);
+  int je = 0, nje = 0, j0 = 0;
+  if (d.w_joint_off && d.w_joint_off[w + 1] > d.w_joint_off[w]) {
+    const int2 jr = d.i_jr[bstart];
+    je = jr.x;
+    nje = jr.y;
+    j0 = d.w_joint_off[w];
+  }
+  const IslandIO<T> io{&d, bstart, c.b0};
+  JStep jst;
+  jst.dt = c.dt;
+  jst.inv_dt = d.inv_dt;
+  jst.dt_ratio = c.dt_ratio;
+  jst.warm = c.warm;
   // 1 + 2. velocities integrated and constraint records built by k_prep_bodies / k_prep_contacts
-  // 3. warm_start (contact.py:498-525) and 4. velocity iterations (contact.py:527-799)
+  // 3. warm_start (contact.py:498-525), the joints' setup, and 4. velocity iterations
+  // (contact.py:527-799), joints before contacts
   if (c.warm) velocity_sweep<T, true>(d, bstart, cstart, nci);
-  for (int it = 0; it < d.vel_iters; ++it) velocity_sweep<T, false>(d, bstart, cstart, nci);
+  for (int k = 0; k < nje; ++k) {
+    JBodies B;
+    const JRec J = island_joint(d, c.b0, j0, je + k, B);
+    if (!joint_init(J, B, io, jst)) atomicOr(&d.w_err[w], WERR_JOINT);
+  }
+  for (int it = 0; it < d.vel_iters; ++it) {
+    for (int k = 0; k < nje; ++k) {
+      JBodies B;
+      const JRec J = island_joint(d, c.b0, j0, je + k, B);
+      joint_solve_velocity(J, B, io, jst);
+    }
+    velocity_sweep<T, false>(d, bstart, cstart, nci);
+  }
   for (int k = 0; k < nci; ++k) store_impulse(d, c, cstart, k);
   for (int i = 0; i < nbi; ++i) {
     PV<T> p = d.pv[bstart + i];
     integrate_position(p, c.dt);
     d.pv[bstart + i] = p;
   }
-  // 7. position iterations with early exit (contact.py:809-864)
+  // 7. position iterations with early exit (contact.py:809-864), contacts then every joint
   bool position_solved = false;
   for (int it = 0; it < d.pos_iters; ++it) {
-    if (position_sweep(d, bstart, cstart, nci)) {
+    const bool contacts_okay = position_sweep(d, bstart, cstart, nci);
+    bool joints_okay = true;
+    for (int k = 0; k < nje; ++k) {
+      JBodies B;
+      const JRec J = island_joint(d, c.b0, j0, je + k, B);
+      const bool ok = joint_solve_position(J, B, io);
+      joints_okay = joints_okay && ok;
+    }
+    if (contacts_okay && joints_okay) {
       position_solved = true;
       break;
     }
@@ -2120,7 +2277,9 @@ __global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_e
         const int nci = nc & 0x3FFFFFFF, last_allow = (nc >> 30) & 1;
         const int share = d.team_lds / per;
         const long long t0 = d.i_time ? (long long)wall_clock64() : 0;
-        const bool fits = team_lds_bytes(nbi) <= share;
+        // an island with joints is solved by the team's first lane (the joints run in sequence)
+        const bool joints = d.w_joint_off && d.w_joint_off[in.x + 1] > d.w_joint_off[in.x] && d.i_jr[in.y].y > 0;
+        const bool fits = team_lds_bytes(nbi) <= share && !joints;
         if (fits) {
           solve_island_team(d, in.x, in.y, nbi, in.w, nci, last_allow, period, team_lds + team * share, t, TS);
         } else if (t == 0) {
@@ -2347,6 +2506,7 @@ __device__ void pair_candidate(const Dev<T>& d, int w, int f0, int b0, int P, in
     }
   }
   if (tb_r != BT_DYNAMIC && tb_p != BT_DYNAMIC) return;  // Body._should_collide
+  if (joint_blocks(d, w, ip.x, ir.x)) return;
   if (!(d.w_flags[w] & WF_CUSTOM_FILTER)) {  // World.default_contact_filter(fixture P, fixture R)
     bool ok;
     if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;
@@ -4042,6 +4202,7 @@ __device__ void world_solve_toi(const Dev<T>& d, int w, ToiScratch& S) {
         }
         if (exists) continue;
         if (d.b_type[gp] != BT_DYNAMIC && d.b_type[gr] != BT_DYNAMIC) continue;  // Body._should_collide
+        if (joint_blocks(d, w, ip.x, ir.x)) continue;
         bool ok;
         if (ip.w == ir.w && ip.w != 0) ok = ip.w > 0;  // World.default_contact_filter
         else ok = ((ip.z >> 16) & (ir.z & 0xFFFF)) != 0 && ((ip.z & 0xFFFF) & (ir.z >> 16)) != 0;
@@ -4450,6 +4611,7 @@ struct b2g_ctx {
   int split_vi = 0, split_pi = 0;
   int* xbuf = nullptr;          // argument and result staging of the between-step calls
   size_t xcap = 0;              // ints in xbuf
+  int nj = 0;                   // joints of the batch
 };
 
 static void ctx_error(b2g_ctx* ctx, const std::string& s) {
@@ -4888,6 +5050,98 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
     TRY(dalloc(ctx, &d.c_toi, ncap));
     TRY(dalloc(ctx, &d.c_tmeta, ncap));
   }
+  // joints: their records, each body's joint list (Body.joints, world.py:268-273), and the body
+  // pairs a joint keeps from colliding (body.py:920-926)
+  ctx->nj = 0;
+  if (b->joint_off && b->joint_off[nw] > 0) {
+    const int nj = b->joint_off[nw];
+    if (!b->joint_d || !b->joint_i) return set_error(ctx, B2G_E_INVALID, "joint records missing");
+    std::vector<double4> j_mass(4 * (size_t)nj, make_double4(0, 0, 0, 0));
+    std::vector<int> bj_cnt((size_t)nb + 1, 0), jnc_off(nw + 1, 0);
+    std::vector<uint32_t> jnc;
+    for (int w = 0; w < nw; ++w) {
+      const int j0 = b->joint_off[w], j1 = b->joint_off[w + 1], b0 = b->body_off[w];
+      const int wnb = b->body_off[w + 1] - b0;
+      if (j1 < j0) return set_error(ctx, B2G_E_INVALID, "negative joint count");
+      if (j1 > j0 && w_big[w])
+        return set_error(ctx, B2G_E_UNSUPPORTED, "joints in a world beyond the LDS island search");
+      if (j1 - j0 >= 65535) return set_error(ctx, B2G_E_UNSUPPORTED, "more than 65534 joints in a world");
+      std::vector<uint32_t> keys;
+      for (int j = j0; j < j1; ++j) {
+        const int32_t* ji = b->joint_i + (size_t)j * B2G_JOINT_NI;
+        const int t = ji[B2G_JI_TYPE], ba = ji[B2G_JI_BODY_A], bb = ji[B2G_JI_BODY_B];
+        if (t < B2G_JT_REVOLUTE || t > B2G_JT_ROPE) return set_error(ctx, B2G_E_INVALID, "bad joint type");
+        if (bb < 0 || bb >= wnb || ba < -1 || ba >= wnb || (ba < 0 && t != B2G_JT_MOUSE))
+          return set_error(ctx, B2G_E_INVALID, "joint body out of range");
+        int bodies[4] = {ba, bb, -1, -1};
+        if (t == B2G_JT_GEAR) {
+          bodies[2] = ji[B2G_JI_BODY_C];
+          bodies[3] = ji[B2G_JI_BODY_D];
+          if (bodies[2] < 0 || bodies[2] >= wnb || bodies[3] < 0 || bodies[3] >= wnb)
+            return set_error(ctx, B2G_E_INVALID, "gear joint body out of range");
+        }
+        for (int q = 0; q < 4; ++q)
+          if (bodies[q] >= 0) {
+            const double* r = b->body_d + (size_t)(b0 + bodies[q]) * B2G_BODY_ND;
+            j_mass[4 * (size_t)j + q] = make_double4(r[B2G_BD_INV_MASS], r[B2G_BD_INV_I], r[B2G_BD_MASS], 0.0);
+          }
+        if (ba >= 0) bj_cnt[b0 + ba + 1]++;
+        bj_cnt[b0 + bb + 1]++;
+        if (!(ji[B2G_JI_FLAGS] & B2G_JF_COLLIDE_CONNECTED) && ba >= 0 && ba != bb)
+          keys.push_back(ba < bb ? ((uint32_t)ba | (uint32_t)bb << 16) : ((uint32_t)bb | (uint32_t)ba << 16));
+      }
+      std::sort(keys.begin(), keys.end());
+      keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+      jnc.insert(jnc.end(), keys.begin(), keys.end());
+      jnc_off[w + 1] = (int)jnc.size();
+    }
+    for (int i = 0; i < nb; ++i) bj_cnt[i + 1] += bj_cnt[i];
+    std::vector<uint32_t> bj(bj_cnt[nb] > 0 ? bj_cnt[nb] : 1);
+    {
+      std::vector<int> fill(bj_cnt.begin(), bj_cnt.end() - 1);
+      for (int w = 0; w < nw; ++w) {  // world joint order gives each body's list its order
+        const int j0 = b->joint_off[w], b0 = b->body_off[w];
+        for (int j = j0; j < b->joint_off[w + 1]; ++j) {
+          const int32_t* ji = b->joint_i + (size_t)j * B2G_JOINT_NI;
+          const int ba = ji[B2G_JI_BODY_A], bb = ji[B2G_JI_BODY_B];
+          const uint32_t jl = (uint32_t)(j - j0);
+          if (ba >= 0) bj[fill[b0 + ba]++] = jl | (uint32_t)bb << 16;
+          bj[fill[b0 + bb]++] = jl | (uint32_t)(ba >= 0 ? ba : 0xFFFF) << 16;
+        }
+      }
+    }
+    int *wjo = nullptr, *bjo = nullptr, *wjn = nullptr;
+    uint32_t *bjd = nullptr, *jncd = nullptr;
+    double4* jm = nullptr;
+    TRY(dalloc(ctx, &wjo, nw + 1));
+    TRY(upload(ctx, wjo, b->joint_off, nw + 1));
+    TRY(dalloc(ctx, &d.j_d, (size_t)nj * B2G_JOINT_ND));
+    TRY(upload(ctx, d.j_d, b->joint_d, (size_t)nj * B2G_JOINT_ND));
+    TRY(dalloc(ctx, &d.j_i, (size_t)nj * B2G_JOINT_NI));
+    TRY(upload(ctx, d.j_i, b->joint_i, (size_t)nj * B2G_JOINT_NI));
+    TRY(dalloc(ctx, &jm, 4 * (size_t)nj));
+    TRY(upload(ctx, jm, j_mass.data(), 4 * (size_t)nj));
+    TRY(dalloc(ctx, &bjo, (size_t)nb + 1));
+    TRY(upload(ctx, bjo, bj_cnt.data(), (size_t)nb + 1));
+    TRY(dalloc(ctx, &bjd, bj.size()));
+    TRY(upload(ctx, bjd, bj.data(), bj.size()));
+    TRY(dalloc(ctx, &wjn, nw + 1));
+    TRY(upload(ctx, wjn, jnc_off.data(), nw + 1));
+    TRY(dalloc(ctx, &jncd, jnc.empty() ? 1 : jnc.size()));
+    if (!jnc.empty()) TRY(upload(ctx, jncd, jnc.data(), jnc.size()));
+    TRY(dalloc(ctx, &d.j_flag, nj));
+    TRY(dalloc(ctx, &d.i_joints, nj));
+    TRY(dalloc(ctx, &d.i_jidx, nj));
+    TRY(dalloc(ctx, &d.i_jr, nb));
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // the staging vectors go out of scope
+    d.w_joint_off = wjo;
+    d.j_mass = jm;
+    d.bj_off = bjo;
+    d.bj = bjd;
+    d.w_jnc_off = wjn;
+    d.jnc = jncd;
+    ctx->nj = nj;
+  }
   if (!ctx->big.empty()) {  // scratch of the large-world path
     int *wb = nullptr, *bl = nullptr;
     TRY(dalloc(ctx, &wb, nw));
@@ -5209,6 +5463,10 @@ static int download_batch(b2g_ctx* ctx, b2g_batch_out* o) {
   TRY(download(ctx, c_id.data(), d.c_id, ncap));
   std::vector<int> mv(nf);
   TRY(download(ctx, mv.data(), d.mv, nf));
+  if (ctx->nj > 0 && o->joint_d && o->joint_i) {  // the joint records (joints do not change count)
+    TRY(download(ctx, o->joint_d, d.j_d, (size_t)ctx->nj * B2G_JOINT_ND));
+    TRY(download(ctx, o->joint_i, d.j_i, (size_t)ctx->nj * B2G_JOINT_NI));
+  }
   HIPCHK(hipStreamSynchronize(ctx->stream));
 
   for (int w = 0; w < nw; ++w) {
@@ -5584,8 +5842,6 @@ B2G_API int b2g_destroy(b2g_ctx* ctx) {
 
 B2G_API int b2g_upload(b2g_ctx* ctx, const b2g_batch* batch) {
   if (!ctx || !batch || batch->n_worlds <= 0) return set_error(ctx, B2G_E_INVALID, "bad arguments");
-  if (batch->joint_off && batch->joint_off[batch->n_worlds] > 0)
-    return set_error(ctx, B2G_E_UNSUPPORTED, "joints are not solved on the device yet");
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   free_device(ctx);
@@ -5668,6 +5924,9 @@ B2G_API int b2g_sync(b2g_ctx* ctx) {
   if (e & WERR_TOI)
     return set_error(ctx, B2G_E_UNSUPPORTED,
                      "a time-of-impact search did not end (the reference's push-back loop has no bound)");
+  if (e & WERR_JOINT)
+    return set_error(ctx, B2G_E_INVALID,
+                     "a mouse joint's spring is too weak for the time step (the reference asserts, mouse.py:173)");
   return B2G_OK;
 }
 
