@@ -245,6 +245,14 @@ B2G_API int b2g_events(b2g_ctx* ctx, int32_t* out_i, double* out_d, int32_t cap,
 B2G_API int b2g_step_collide(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters);
 B2G_API int b2g_step_solve(b2g_ctx* ctx);
 
+/* Parity seam of the solver: the island search and Island.solve of World._solve
+   (world.py:494-580, island.py:65-179, the ContactSolver of contact.py:375-864 and
+   the joints) on the state as it stands, dt > 0: no collide (manifolds as
+   uploaded), no fixture synchronisation, no pair search, no time-of-impact pass,
+   forces not cleared.  Separates a solver difference from narrowphase or
+   pair-search differences. */
+B2G_API int b2g_solve_islands(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters);
+
 /* contact.enabled = False for this step: contact slot slot[k] of world world[k]
    (the slot is a phase-0 event's key / 4, its place in the world's contact list
    during collide).  Only between b2g_step_collide and b2g_step_solve. */

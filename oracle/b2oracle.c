@@ -2880,7 +2880,12 @@ static int sync_body_fixtures(World* w, int bi) {
   return 0;
 }
 
-static void solve(World* w, const Step* st) { /* world.py:475-592 */
+static void solve_islands(World* w, const Step* st, int then_sync);
+static void solve(World* w, const Step* st) { solve_islands(w, st, 1); }
+
+/* world.py:475-592; with then_sync = 0 only the island search and the island solves
+   (the solver seam, orc_solve_islands) */
+static void solve_islands(World* w, const Step* st, int then_sync) {
   int nb = w->nb, nc = w->nc;
   w->post_key = 0;
   for (int i = 0; i < nb; ++i) { w->b[i].island_flag = 0; w->b[i].island_id = -1; }
@@ -2972,6 +2977,7 @@ static void solve(World* w, const Step* st) { /* world.py:475-592 */
       if (w->b[is.bodies[i]].type == B2G_STATIC) w->b[is.bodies[i]].island_flag = 0;
     ++island_id;
   }
+  if (!then_sync) goto done;
   /* Body._synchronize_fixtures for island bodies (world.py:584-589, body.py:898-904) */
   for (int bi = 0; bi < nb; ++bi) {
     Body* b = &w->b[bi];
@@ -3879,6 +3885,28 @@ ORC_EXPORT int orc_save(void* h, b2g_batch_out* o) {
     o->move_off[wi + 1] = mo;
   }
   return 0;
+}
+
+/* The solver seam: the island search and Island.solve of every world on its current state
+   (no collide, no fixture synchronisation or pair search, forces kept); returns the OR of the
+   worlds' error words. */
+ORC_EXPORT int orc_solve_islands(void* h, double dt, int vel_iters, int pos_iters) {
+  Batch* B = (Batch*)h;
+  int err = 0;
+  for (int i = 0; i < B->nw; ++i) {
+    World* w = &B->w[i];
+    Step st;
+    st.dt = dt;
+    st.inv_dt = dt > 0.0 ? 1.0 / dt : 0.0;
+    st.dt_ratio = w->inv_dt0 * dt;
+    st.vel_iters = vel_iters;
+    st.pos_iters = pos_iters;
+    st.warm_starting = w->warm_starting;
+    w->ev_n = 0;
+    if (dt > 0.0) solve_islands(w, &st, 0);
+    err |= w->err;
+  }
+  return err;
 }
 
 /* Replay hint for world wi: n unordered fixture pairs (2n ints) in the

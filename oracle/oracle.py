@@ -56,6 +56,7 @@ def lib():
         l.orc_disable.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         l.orc_step_end.argtypes = [vp, ctypes.c_int]
         l.orc_toi_stats.argtypes = [vp, ctypes.c_int, vp]
+        l.orc_solve_islands.argtypes = [vp, f64, ctypes.c_int, ctypes.c_int]
         _ = i32
         _lib = l
     return _lib
@@ -82,6 +83,10 @@ class Oracle:
 
     def step(self, dt=1.0 / 60, vel_iters=8, pos_iters=3) -> int:
         return lib().orc_step(self.h, dt, vel_iters, pos_iters)
+
+    def solve_islands(self, dt=1.0 / 60, vel_iters=8, pos_iters=3) -> int:
+        """The island search and Island.solve alone on the current state (the solver seam)."""
+        return lib().orc_solve_islands(self.h, dt, vel_iters, pos_iters)
 
     def step_world(self, w: int, dt=1.0 / 60, vel_iters=8, pos_iters=3) -> int:
         return lib().orc_step_world(self.h, w, dt, vel_iters, pos_iters)

@@ -24,7 +24,7 @@ EXPORTS = ("b2g_version", "b2g_create", "b2g_destroy", "b2g_upload", "b2g_step",
            "b2g_counts", "b2g_download", "b2g_last_error", "b2g_set_param", "b2g_get_param", "b2g_profile_step",
            "b2g_island_times", "b2g_narrowphase", "b2g_pairs", "b2g_stats", "b2g_query_aabb", "b2g_ray_cast",
            "b2g_pairs_noalloc", "b2g_events", "b2g_pair_search", "b2g_new_contacts", "b2g_drop_contacts",
-           "b2g_step_collide", "b2g_step_solve", "b2g_disable_contacts")
+           "b2g_step_collide", "b2g_step_solve", "b2g_disable_contacts", "b2g_solve_islands")
 
 # keys of b2g_set_param / b2g_get_param: the solver's launch shape, island timing, and the
 # resident solver wavefronts per compute unit and the search launches per step (both read only)
@@ -90,6 +90,7 @@ def load(path: Optional[str] = None):
     lib.b2g_step_collide.argtypes = [vp, f64, ctypes.c_int, ctypes.c_int]
     lib.b2g_step_solve.argtypes = [vp]
     lib.b2g_disable_contacts.argtypes = [vp, ctypes.c_int32, vp, vp]
+    lib.b2g_solve_islands.argtypes = [vp, f64, ctypes.c_int, ctypes.c_int]
     _ = i32
     _lib = lib
     return lib
@@ -257,6 +258,11 @@ class Engine:
     def step_solve(self) -> None:
         """The rest of the step begun by step_collide (b2g_step_solve)."""
         self._check(self.lib.b2g_step_solve(self.h), "step_solve")
+
+    def solve_islands(self, dt: float, vel_iters: int, pos_iters: int) -> None:
+        """The island search and Island.solve alone on the current state (b2g_solve_islands): no
+        collide, no fixture synchronisation, no pair search (a parity seam for the solver)."""
+        self._check(self.lib.b2g_solve_islands(self.h, float(dt), int(vel_iters), int(pos_iters)), "solve_islands")
 
     def pair_search(self) -> None:
         """Runs the pending new-fixture pair search ahead of the next step (b2g_pair_search)."""

@@ -67,7 +67,7 @@ enum : int {
 };
 // the parts of a step one launch sequence runs: all of it; the new-fixture pair search alone
 // (b2g_pair_search); up to and with collide (b2g_step_collide); the rest (b2g_step_solve)
-enum : int { PART_ALL = 0, PART_PAIRS = 1, PART_COLLIDE = 2, PART_SOLVE = 3 };
+enum : int { PART_ALL = 0, PART_PAIRS = 1, PART_COLLIDE = 2, PART_SOLVE = 3, PART_ISLANDS = 4 };
 enum : int {
   WERR_CAPACITY = 1,
   WERR_UNSUPPORTED = 2,
@@ -213,6 +213,7 @@ struct Dev {
   int* w_new;                // contacts created by the world's last pair search (the last ones of its list)
   // contact events (b2g_events): kinds enabled (1 << B2G_EV_*), ring of ev_cap records
   int ev_mask, ev_cap;
+  int solve_only;  // b2g_solve_islands: the island search without collide's awake gating
   int* ev_count;
   int* ev_i;
   double* ev_d;
@@ -776,7 +777,7 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) k_i
   // touching flag changed (which wakes bodies) can matter.  They are first
   // compacted in slot order, with their bodies, into LDS (the adjacency arrays
   // are free until the graph is built); one lane then walks them in order.
-  if (d.w_asleep[w] > 0) {
+  if (d.w_asleep[w] > 0 && !d.solve_only) {
     uint16_t* cs = (uint16_t*)L.adj;  // the adjacency area holds 4 * kGateChunk halves at least
     uint16_t* cla = cs + kGateChunk;
     uint16_t* clb = cla + kGateChunk;
@@ -2876,7 +2877,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
   __syncthreads();
 
   // --- awake gating (contact_manager.py:191-197) as a fixed point over wake slots ---
-  if (d.w_asleep[w] > 0) {
+  if (d.w_asleep[w] > 0 && !d.solve_only) {
     for (int i = tid; i < nb; i += kBigBlock) wake[i] = (bst[i] & 2) ? -1 : kIntMax;
     __syncthreads();
     for (;;) {
@@ -5289,6 +5290,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
   d.team_lds = ctx->team_lds;
   d.team_max = ctx->team_max;
   d.ev_mask = ctx->ev_mask;
+  d.solve_only = part == PART_ISLANDS;
   d.i_time = ctx->island_times ? ctx->itime : nullptr;
   const int nw = ctx->nw;
   hipStream_t st = ctx->stream;
@@ -5327,7 +5329,9 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     }
     return 0;
   };
-  if (part != PART_SOLVE) {
+  if (part == PART_ISLANDS) {  // the island search and solve alone (b2g_solve_islands)
+    hipLaunchKernelGGL(k_reset_queue<T>, dim3(1), dim3(64), 0, st, d);
+  } else if (part != PART_SOLVE) {
     if (first_pairs) TRY(pairs(1, 0, 1));
     if (part == PART_PAIRS) {  // the new-fixture pair search alone (b2g_pair_search)
       HIPCHK(hipGetLastError());
@@ -5370,6 +5374,10 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     hipLaunchKernelGGL(k_prep_contacts<T>, dim3((unsigned)(cch * nw)), dim3(kBlock), 0, st, d, cch);
     const int swaves = std::max(1, std::min((ctx->nb + kSolveBlock - 1) / kSolveBlock, ctx->solve_grid));
     hipLaunchKernelGGL(k_solve<T>, dim3(swaves), dim3(kSolveBlock), ctx->team_lds, st, d);
+    if (part == PART_ISLANDS) {
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
     if (timed) HIPCHK(hipEventRecord(ev[3], st));
     hipLaunchKernelGGL(k_sync<T>, dim3(std::max(1, (ctx->nf + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[4], st));
@@ -6156,6 +6164,17 @@ B2G_API int b2g_step_solve(b2g_ctx* ctx) {
   ctx->split = false;
   TRY(dispatch_step(ctx, ctx->split_dt, ctx->split_vi, ctx->split_pi, false, nullptr, PART_SOLVE));
   ctx->stat_steps += 1;
+  return B2G_OK;
+}
+
+B2G_API int b2g_solve_islands(b2g_ctx* ctx, double dt, int vel_iters, int pos_iters) {
+  if (!ctx || !ctx->dev) return set_error(ctx, B2G_E_INVALID, "no batch uploaded");
+  if (ctx->split) return set_error(ctx, B2G_E_INVALID, "a step begun by b2g_step_collide is not finished");
+  if (vel_iters < 0 || pos_iters < 0) return set_error(ctx, B2G_E_INVALID, "negative iteration count");
+  if (!(dt > 0.0)) return set_error(ctx, B2G_E_INVALID, "the island solve needs dt > 0");
+  hipSetDevice(ctx->device);
+  TRY(ensure_island_times(ctx));
+  TRY(dispatch_step(ctx, dt, vel_iters, pos_iters, false, nullptr, PART_ISLANDS));
   return B2G_OK;
 }
 

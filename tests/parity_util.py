@@ -94,3 +94,20 @@ def assert_one_step(got: L.BatchArrays, ref: L.BatchArrays, tol: float, what="")
     for q in IMPULSES:
         assert ce[q] <= tol, (what, q, ce[q])
     return be, ce
+
+
+def golden_state(z, init, s, dt=1.0 / 60):
+    """The full batch at checkpoint s of a recorded reference run (a one-world npz written by
+    tests/golden/make_golden.py), joints included when the run has them."""
+    b = init.copy()
+    b.body_d, b.body_i = z["s%d_body_d" % s].copy(), z["s%d_body_i" % s].copy()
+    b.fix_d = z["s%d_fix_d" % s].copy()
+    b.con_d, b.con_i = z["s%d_con_d" % s].copy(), z["s%d_con_i" % s].copy()
+    b.move = z["s%d_move" % s].copy()
+    b.con_off = np.array([0, b.con_d.shape[0]], np.int32)
+    b.move_off = np.array([0, b.move.shape[0]], np.int32)
+    if "s%d_joint_d" % s in z.files:
+        b.joint_d, b.joint_i = z["s%d_joint_d" % s].copy(), z["s%d_joint_i" % s].copy()
+    b.world_d[0, L.WD_INV_DT0] = 1.0 / dt
+    b.world_i[0, L.WI_FLAGS] &= ~L.WF_NEW_FIXTURE
+    return b
