@@ -202,7 +202,11 @@ def solver_report(rows: np.ndarray, waves_per_cu: int) -> dict:
         groups[str(int(lanes))] = {"islands": int(m.sum()), "contacts": [int(rows[m, 2].min()), int(rows[m, 2].max())],
                                    "sweep_period": [int(rows[m, 5].min()), int(rows[m, 5].max())],
                                    "latency_us": {"mean": float(lat[m].mean()), "p50": float(np.median(lat[m])),
-                                                  "max": float(lat[m].max())}}
+                                                  "max": float(lat[m].max())},
+                                   # share of the latency spent in the velocity iterations (the rest:
+                                   # impulse store, integration, position iterations, write-back)
+                                   "velocity_share": float(((rows[m, 6] - start[m]) / np.maximum(end[m] - start[m], 1)).mean()),
+                                   "position_iterations_mean": float(rows[m, 7].mean())}
     c = int(np.argmax(end))
     return {"islands": int(rows.shape[0]), "span_us": float(span), "mean_islands_in_flight": float(lat.sum() / span),
             "resident_wavefronts_per_cu": int(waves_per_cu), "by_lanes": groups,

@@ -173,12 +173,14 @@ B2G_API int b2g_get_param(b2g_ctx* ctx, int32_t key, int32_t* value);
 B2G_API int b2g_profile_step(b2g_ctx* ctx, int n, double dt, int vel_iters, int pos_iters, float* ms_out);
 
 /* With B2G_PARAM_ISLAND_TIMES on: the islands the last step solved, in the
-   solver's queue order, as 4 int64 each: start and end (ns, device wall
-   clock), contacts | bodies << 32, and lanes | period << 32: the lanes that
-   solved it (a team's size, 1 for one lane per island, 0 for a team island
-   whose state did not fit the team's LDS share) and its sweep period (the
-   lag in dependency levels between consecutive sweeps, which with the
-   iteration counts predicts the island's passes).  *count receives the number of islands; at most
+   solver's queue order, as 6 int64 each: start and end (ns, device wall
+   clock), contacts | bodies << 32, lanes | period << 32 (the lanes that
+   solved it: a team's size, 1 for one lane per island, 0 for a team island
+   whose state did not fit the team's LDS share or that has joints; and its
+   sweep period, the lag in dependency levels between consecutive sweeps,
+   which with the iteration counts predicts the island's passes), the end of
+   its velocity iterations (ns) and the number of position iterations it ran
+   before the early exit.  *count receives the number of islands; at most
    cap rows are written.  No counterpart in the reference (per-island latency
    of the parallel solver). */
 B2G_API int b2g_island_times(b2g_ctx* ctx, int64_t* out, int32_t cap, int32_t* count);

@@ -295,16 +295,17 @@ class Engine:
 
     def island_times(self) -> np.ndarray:
         """Islands of the last step (island timing on): rows of start_ns, end_ns, contacts,
-        bodies, lanes, sweep period."""
+        bodies, lanes, sweep period, velocity_end_ns, position iterations."""
         n = np.zeros(1, np.int32)
         self._check(self.lib.b2g_island_times(self.h, None, 0, n.ctypes.data), "island_times")
-        raw = np.zeros((max(int(n[0]), 1), 4), np.int64)
+        raw = np.zeros((max(int(n[0]), 1), 6), np.int64)
         self._check(self.lib.b2g_island_times(self.h, raw.ctypes.data, int(n[0]), n.ctypes.data), "island_times")
         raw = raw[:int(n[0])]
-        out = np.zeros((raw.shape[0], 6), np.int64)
+        out = np.zeros((raw.shape[0], 8), np.int64)
         out[:, 0], out[:, 1] = raw[:, 0], raw[:, 1]
         out[:, 2], out[:, 3] = raw[:, 2] & 0xFFFFFFFF, raw[:, 2] >> 32
         out[:, 4], out[:, 5] = raw[:, 3] & 0xFFFFFFFF, raw[:, 3] >> 32
+        out[:, 6], out[:, 7] = raw[:, 4], raw[:, 5]
         return out
 
 

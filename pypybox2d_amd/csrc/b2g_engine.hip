@@ -103,6 +103,7 @@ __host__ __device__ inline int size_class(int nc) {  // floor(log2(nc + 1)), cap
 }
 
 constexpr int kSolveBlock = 64;  // k_solve: one wavefront per workgroup
+constexpr int kIslandTimeFields = 6;  // int64 per island of b2g_island_times
 
 // Team size of a heavy size class: 64 lanes from full_cls up, halved per class
 // below down to 8, and at most tmax lanes, except in the top class, which holds
@@ -1849,6 +1850,13 @@ __device__ __forceinline__ void sleep_last(const Dev<T>& d, const IslandK& c, in
   d.b_flags[gb] &= ~BF_AWAKE;
 }
 
+// Island timing marks (b2g_island_times): when the velocity iterations ended, and how many
+// position iterations ran before the early exit.
+struct SolveMarks {
+  long long t_vel;
+  int npos;
+};
+
 // The island's positions and velocities as the joint solver reads and writes them: island
 // bodies in pv, bodies outside the island's body list (static ones) read from the world and
 // never written.
@@ -1894,18 +1902,15 @@ __device__ __forceinline__ JRec island_joint(const Dev<T>& d, int b0, int j0, in
   return J;
 }
 
-// Island.solve on one lane (island.py:65-179), with the island's joints in their places
-// (island.py:107-114 and 141-152).
+// Island.solve on one lane (island.py:65-179) for an island with joints, the joints in their
+// places (island.py:107-114 and 141-152).  Runs in k_solve_joints, so that the joint solver's
+// registers stay out of k_solve.
 template <typename T>
-__device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci, int last_allow) {
+__device__ void solve_island_joints(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci,
+                                    int last_allow, SolveMarks& mk) {
   const IslandK c = island_k(d, w, bstart);
-  int je = 0, nje = 0, j0 = 0;
-  if (d.w_joint_off && d.w_joint_off[w + 1] > d.w_joint_off[w]) {
-    const int2 jr = d.i_jr[bstart];
-    je = jr.x;
-    nje = jr.y;
-    j0 = d.w_joint_off[w];
-  }
+  const int2 jr = d.i_jr[bstart];
+  const int je = jr.x, nje = jr.y, j0 = d.w_joint_off[w];
   const IslandIO<T> io{&d, bstart, c.b0};
   JStep jst;
   jst.dt = c.dt;
@@ -1929,6 +1934,7 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
     }
     velocity_sweep<T, false>(d, bstart, cstart, nci);
   }
+  if (d.i_time) mk.t_vel = (long long)wall_clock64();
   for (int k = 0; k < nci; ++k) store_impulse(d, c, cstart, k);
   for (int i = 0; i < nbi; ++i) {
     PV<T> p = d.pv[bstart + i];
@@ -1938,6 +1944,7 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
   // 7. position iterations with early exit (contact.py:809-864), contacts then every joint
   bool position_solved = false;
   for (int it = 0; it < d.pos_iters; ++it) {
+    if (d.i_time) mk.npos = it + 1;
     const bool contacts_okay = position_sweep(d, bstart, cstart, nci);
     bool joints_okay = true;
     for (int k = 0; k < nje; ++k) {
@@ -1957,6 +1964,45 @@ __device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cs
     for (int i = 0; i < nbi; ++i) min_sleep = pmin(min_sleep, sleep_body(d, c, bstart, i, last_allow));
     if (min_sleep >= TIME_TO_SLEEP && position_solved && nbi > 0) sleep_last(d, c, bstart, nbi);
   }
+}
+
+// Island.solve on one lane (island.py:65-179).
+template <typename T>
+__device__ void solve_island(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci, int last_allow,
+                             SolveMarks& mk) {
+  const IslandK c = island_k(d, w, bstart);
+  // 1 + 2. velocities integrated and constraint records built by k_prep_bodies / k_prep_contacts
+  // 3. warm_start (contact.py:498-525) and 4. velocity iterations (contact.py:527-799)
+  if (c.warm) velocity_sweep<T, true>(d, bstart, cstart, nci);
+  for (int it = 0; it < d.vel_iters; ++it) velocity_sweep<T, false>(d, bstart, cstart, nci);
+  if (d.i_time) mk.t_vel = (long long)wall_clock64();
+  for (int k = 0; k < nci; ++k) store_impulse(d, c, cstart, k);
+  for (int i = 0; i < nbi; ++i) {
+    PV<T> p = d.pv[bstart + i];
+    integrate_position(p, c.dt);
+    d.pv[bstart + i] = p;
+  }
+  // 7. position iterations with early exit (contact.py:809-864)
+  bool position_solved = false;
+  for (int it = 0; it < d.pos_iters; ++it) {
+    if (d.i_time) mk.npos = it + 1;
+    if (position_sweep(d, bstart, cstart, nci)) {
+      position_solved = true;
+      break;
+    }
+  }
+  for (int i = 0; i < nbi; ++i) write_back(d, c, bstart, i, d.pv[bstart + i]);
+  if (c.wflags & WF_ALLOW_SLEEP) {
+    double min_sleep = 1.0e300;
+    for (int i = 0; i < nbi; ++i) min_sleep = pmin(min_sleep, sleep_body(d, c, bstart, i, last_allow));
+    if (min_sleep >= TIME_TO_SLEEP && position_solved && nbi > 0) sleep_last(d, c, bstart, nbi);
+  }
+}
+
+// Does the island starting at island-body bstart (global) have joints?
+template <typename T>
+__device__ __forceinline__ bool island_has_joints(const Dev<T>& d, int w, int bstart) {
+  return d.w_joint_off && d.w_joint_off[w + 1] > d.w_joint_off[w] && d.i_jr[bstart].y > 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2011,7 +2057,7 @@ __device__ __forceinline__ X team_min(X v, int T) {
 
 template <typename T>
 __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, int cstart, int nci, int last_allow,
-                                  int period, char* lds, int t, int TS) {
+                                  int period, char* lds, int t, int TS, SolveMarks& mk) {
   const IslandK c = island_k(d, w, bstart);
   double* sx = (double*)lds;        // [3 * nbi]
   int* cnt = (int*)(sx + 3 * nbi);  // [nbi]
@@ -2122,6 +2168,7 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
     if (active) atomicOr(&d.w_err[w], WERR_SCHEDULE);  // pass limit reached
   }
   team_sync();
+  if (d.i_time) mk.t_vel = (long long)wall_clock64();
   // 5. store impulses, 6. integrate positions (positions to LDS)
   for (int k = t; k < nci; k += TS) store_impulse(d, c, cstart, k);
   for (int i = t; i < nbi; i += TS) {
@@ -2139,6 +2186,7 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
   bool position_solved = false;
   for (int it = 0; it < d.pos_iters; ++it) {
     if (!position_solved) {  // team-uniform
+      if (d.i_time) mk.npos = it + 1;
       for (int i = t; i < nbi; i += TS) cnt[i] = 0;
       team_sync();
       T min_sep = 0;
@@ -2236,12 +2284,14 @@ __global__ void __launch_bounds__(kBlock) k_prep_contacts(Dev<T> d, int chunks) 
 // one lane because its state did not fit the team's LDS share).
 template <typename T>
 __device__ __forceinline__ void record_island_time(const Dev<T>& d, int q, long long t0, int nci, int nbi, int lanes,
-                                                   int period) {
-  long long* o = d.i_time + 4 * (size_t)q;
+                                                   int period, const SolveMarks& mk) {
+  long long* o = d.i_time + kIslandTimeFields * (size_t)q;
   o[0] = t0;
   o[1] = (long long)wall_clock64();
   o[2] = (long long)nci | (long long)nbi << 32;
   o[3] = (long long)lanes | (long long)period << 32;
+  o[4] = mk.t_vel;
+  o[5] = mk.npos;
 }
 
 // One wavefront per block.  The queue is ordered largest class first.  Its
@@ -2279,14 +2329,17 @@ __global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_e
         const int share = d.team_lds / per;
         const long long t0 = d.i_time ? (long long)wall_clock64() : 0;
         // an island with joints is solved by the team's first lane (the joints run in sequence)
-        const bool joints = d.w_joint_off && d.w_joint_off[in.x + 1] > d.w_joint_off[in.x] && d.i_jr[in.y].y > 0;
-        const bool fits = team_lds_bytes(nbi) <= share && !joints;
-        if (fits) {
-          solve_island_team(d, in.x, in.y, nbi, in.w, nci, last_allow, period, team_lds + team * share, t, TS);
+        const bool fits = team_lds_bytes(nbi) <= share;
+        SolveMarks mk{0, 0};
+        if (island_has_joints(d, in.x, in.y)) {
+          // k_solve_joints takes it
+        } else if (fits) {
+          solve_island_team(d, in.x, in.y, nbi, in.w, nci, last_allow, period, team_lds + team * share, t, TS, mk);
+          if (d.i_time && t == 0) record_island_time(d, q, t0, nci, nbi, TS, period, mk);
         } else if (t == 0) {
-          solve_island(d, in.x, in.y, nbi, in.w, nci, last_allow);
+          solve_island(d, in.x, in.y, nbi, in.w, nci, last_allow, mk);
+          if (d.i_time) record_island_time(d, q, t0, nci, nbi, 0, period, mk);
         }
-        if (d.i_time && t == 0) record_island_time(d, q, t0, nci, nbi, fits ? TS : 0, period);
       }
       team_sync();
     } else {
@@ -2295,11 +2348,32 @@ __global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_e
         const long long t0 = d.i_time ? (long long)wall_clock64() : 0;
         const int4 in = d.qs_isl[q];
         const int nc = d.qs_nc[q];
-        solve_island(d, in.x, in.y, in.z & 0xFFFF, in.w, nc & 0x3FFFFFFF, (nc >> 30) & 1);
-        if (d.i_time) record_island_time(d, q, t0, nc & 0x3FFFFFFF, in.z & 0xFFFF, 1, (int)((uint32_t)in.z >> 16));
+        SolveMarks mk{0, 0};
+        if (!island_has_joints(d, in.x, in.y)) {  // else k_solve_joints takes it
+          solve_island(d, in.x, in.y, in.z & 0xFFFF, in.w, nc & 0x3FFFFFFF, (nc >> 30) & 1, mk);
+          if (d.i_time)
+            record_island_time(d, q, t0, nc & 0x3FFFFFFF, in.z & 0xFFFF, 1, (int)((uint32_t)in.z >> 16), mk);
+        }
       }
     }
   }
+}
+
+// The islands with joints, one lane each, after k_solve (which leaves them): launched only for
+// batches with joints.  Their bodies and contacts belong to no other island, so the order of the
+// two kernels does not matter.
+template <typename T>
+__global__ void __launch_bounds__(64) k_solve_joints(Dev<T> d) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= *d.q_count) return;
+  const int4 in = d.qs_isl[q];
+  if (!island_has_joints(d, in.x, in.y)) return;
+  const long long t0 = d.i_time ? (long long)wall_clock64() : 0;
+  const int nc = d.qs_nc[q];
+  const int nbi = in.z & 0xFFFF, nci = nc & 0x3FFFFFFF;
+  SolveMarks mk{0, 0};
+  solve_island_joints(d, in.x, in.y, nbi, in.w, nci, (nc >> 30) & 1, mk);
+  if (d.i_time) record_island_time(d, q, t0, nci, nbi, 0, (int)((uint32_t)in.z >> 16), mk);
 }
 
 // ---------------------------------------------------------------------------
@@ -4606,7 +4680,7 @@ struct b2g_ctx {
   std::vector<hipEvent_t> pev;  // b2g_profile_step
   int island_times = 0;         // B2G_PARAM_ISLAND_TIMES
   long long stat_steps = 0;     // steps launched since the work counters were reset
-  long long* itime = nullptr;   // 4 values per island slot (nb slots)
+  long long* itime = nullptr;   // kIslandTimeFields values per island slot (nb slots)
   bool split = false;           // b2g_step_collide ran; b2g_step_solve finishes that step
   double split_dt = 0;
   int split_vi = 0, split_pi = 0;
@@ -5374,6 +5448,8 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     hipLaunchKernelGGL(k_prep_contacts<T>, dim3((unsigned)(cch * nw)), dim3(kBlock), 0, st, d, cch);
     const int swaves = std::max(1, std::min((ctx->nb + kSolveBlock - 1) / kSolveBlock, ctx->solve_grid));
     hipLaunchKernelGGL(k_solve<T>, dim3(swaves), dim3(kSolveBlock), ctx->team_lds, st, d);
+    if (ctx->nj > 0)
+      hipLaunchKernelGGL(k_solve_joints<T>, dim3((unsigned)std::max(1, (ctx->nb + 63) / 64)), dim3(64), 0, st, d);
     if (part == PART_ISLANDS) {
       HIPCHK(hipGetLastError());
       return 0;
@@ -5402,7 +5478,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
 static int ensure_island_times(b2g_ctx* ctx) {
   if (!ctx->island_times || ctx->itime || !ctx->dev) return 0;
   void* q = nullptr;
-  const hipError_t e = hipMalloc(&q, (size_t)std::max(ctx->nb, 1) * 4 * sizeof(long long));
+  const hipError_t e = hipMalloc(&q, (size_t)std::max(ctx->nb, 1) * kIslandTimeFields * sizeof(long long));
   if (e != hipSuccess) return set_error(ctx, B2G_E_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
   ctx->itime = (long long*)q;
   return 0;
@@ -6251,13 +6327,18 @@ B2G_API int b2g_island_times(b2g_ctx* ctx, int64_t* out, int32_t cap, int32_t* c
   int rate_khz = 0;
   HIPCHK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
   if (rate_khz <= 0) return set_error(ctx, B2G_E_HIP, "no wall clock rate");
-  std::vector<long long> raw((size_t)4 * m);
+  constexpr int F = kIslandTimeFields;
+  std::vector<long long> raw((size_t)F * m);
   HIPCHK(hipMemcpy(raw.data(), ctx->itime, raw.size() * sizeof(long long), hipMemcpyDeviceToHost));
   for (int i = 0; i < m; ++i) {
-    out[4 * i] = (int64_t)((double)raw[4 * i] * 1.0e6 / rate_khz);
-    out[4 * i + 1] = (int64_t)((double)raw[4 * i + 1] * 1.0e6 / rate_khz);
-    out[4 * i + 2] = raw[4 * i + 2];
-    out[4 * i + 3] = raw[4 * i + 3];
+    const long long* r = &raw[(size_t)F * i];
+    int64_t* o = out + (size_t)F * i;
+    o[0] = (int64_t)((double)r[0] * 1.0e6 / rate_khz);
+    o[1] = (int64_t)((double)r[1] * 1.0e6 / rate_khz);
+    o[2] = r[2];
+    o[3] = r[3];
+    o[4] = (int64_t)((double)r[4] * 1.0e6 / rate_khz);
+    o[5] = r[5];
   }
   return B2G_OK;
 }

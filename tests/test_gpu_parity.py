@@ -605,6 +605,8 @@ def test_island_times_report_every_island_and_change_nothing():
     assert rows.shape[0] > 0 and 1 <= waves <= 32
     assert np.all(rows[:, 1] >= rows[:, 0])
     assert set(rows[:, 4].tolist()) <= {0, 1, 8, 16, 32, 64}
-    assert rows.shape[1] == 6 and np.all(rows[rows[:, 4] >= 8, 5] >= 1)  # team islands' sweep periods
+    assert rows.shape[1] == 8 and np.all(rows[rows[:, 4] >= 8, 5] >= 1)  # team islands' sweep periods
+    assert np.all((rows[:, 6] >= rows[:, 0]) & (rows[:, 6] <= rows[:, 1]))  # the velocity phase ends inside
+    assert np.all((rows[:, 7] >= 1) & (rows[:, 7] <= 3))  # position iterations run (early exit)
     dyn = int(np.count_nonzero(batch.body_i[:, L.BI_TYPE] == L.DYNAMIC))
     assert rows[:, 3].sum() <= dyn and np.all(rows[:, 3] >= 1)
