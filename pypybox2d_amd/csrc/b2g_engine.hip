@@ -179,15 +179,23 @@ struct VC {
   double k11, k12, k22, n11, n12, n21, n22;  // K and K^-1 (n11 = c1x, n21 = c1y, n12 = c2x, n22 = c2y)
   int ia, ib;
   int rka, rkb;  // team path: rank | degree << 16 of the contact in body A's (B's) island sequence
-  int npts, pad;
+  int npts;
+  uint32_t nxt;  // team path: the level-order entry a team size after this contact's (its lane's next)
   VCP<T> p[2];
 };
+// A point's accumulated normal and tangent impulses, stored together by the team solver.
+struct ImpPair {
+  double ni, ti;
+};
+static_assert(offsetof(VCP<float>, ti) == offsetof(VCP<float>, ni) + 8 && sizeof(VCP<float>) % 16 == 0, "VCP");
+static_assert(offsetof(VCP<double>, ti) == offsetof(VCP<double>, ni) + 8 && sizeof(VCP<double>) % 16 == 0, "VCP");
 template <typename T>
 struct PC {
   T lp0x, lp0y, lp1x, lp1y, lnx, lny, lpx, lpy;
   T ra, rb, ma, mb, Ia, Ib, lcax, lcay, lcbx, lcby;
   int type, npts, ia, ib;
   int rka, rkb;         // as in VC
+  uint32_t nxt, pad;    // as in VC
   double scx, scy, sa;  // position of the contact's static body (static bodies never move)
 };
 
@@ -1659,7 +1667,7 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
   vc.Ia = d.b_iI[ba];
   vc.Ib = d.b_iI[bb];
   vc.npts = pc;
-  vc.pad = 0;
+  vc.nxt = 0u;
   vc.rka = rka;
   vc.rkb = rkb;
   const bool warm = c.warm;
@@ -1681,6 +1689,7 @@ __device__ void init_contact(const Dev<T>& d, const IslandK& c, int bstart, int 
   pcn.ib = ib;
   pcn.rka = rka;
   pcn.rkb = rkb;
+  pcn.nxt = pcn.pad = 0u;
   double cax, cay, aa, cbx, cby, ab;
   load_pos(d, bstart, b0, ia, cax, cay, aa);
   load_pos(d, bstart, b0, ib, cbx, cby, ab);
@@ -2070,33 +2079,35 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
     sx[3 * i + 2] = p.w;
     cnt[i] = 0;
   }
-  team_sync();
   // 3 + 4. warm start and velocity sweeps as one dataflow sequence
   const int s0 = c.warm ? 0 : 1;
   const int nsw = d.vel_iters + 1 - s0;
   const int mine = t < nci ? (nci - 1 - t) / TS + 1 : 0;  // contacts owned by this lane
-  // Lane t owns entries t, t + TS, ... of the island's level order (i_order) and takes its
-  // solves (sweep s, owned entry j) in order of s * period + level (sweep_period): a merge of
-  // two cursors, A on sweep sa and B on sweep sa + 1, whose entries are read one step ahead.
+  // Lane t owns entries t, t + TS, ... of the island's level order (i_order).  Each of its
+  // records learns the lane's entry after it (nxt), so a lane reads its next entry with the
+  // record it solves and no pass waits on a separate read of the order.  Only this lane reads
+  // these records later, and its reads follow its own writes.
   const uint32_t* order = d.i_order + cstart;
+  for (int p = t; p < nci; p += TS) {
+    const uint32_t e = order[p], n = p + TS < nci ? order[p + TS] : 0u;
+    d.vc[cstart + (e & 0xFFFFu)].nxt = n;
+    d.pc[cstart + (e & 0xFFFFu)].nxt = n;
+  }
+  team_sync();
+  // The lane takes its solves (sweep s, owned entry j) in order of s * period + level
+  // (sweep_period): a merge of two cursors, A on sweep sa and B on sweep sa + 1.
   {
     const uint32_t e0 = mine > 0 ? order[t] : 0u;
     bool a_on = nsw > 0 && mine > 0, b_on = nsw > 1 && mine > 0;
-    int sa = 0, ja = 0, jb = 0;
+    int sa = 0, ja = 0, jb = 0;  // each cursor's next entry (ea, eb)
     uint32_t ea = e0, eb = e0;
-    // the lane's next solve: B when its key (sa + 1) * period + level is below A's
-    auto next_solve = [&](int& s_out, int& k_out) -> bool {
-      if (!a_on) return false;
-      const bool take_b = b_on && jb < mine && (int)(eb >> 16) + period < (int)(ea >> 16);
-      if (take_b) {
-        s_out = sa + 1;
-        k_out = (int)(eb & 0xFFFFu);
-        if (++jb < mine) eb = order[t + jb * TS];
-      } else {
-        s_out = sa;
-        k_out = (int)(ea & 0xFFFFu);
-        if (++ja < mine) {
-          ea = order[t + ja * TS];
+    int took = 0;  // the cursor whose entry the record just solved was: 1 A, 2 B
+    // the lane's next solve: B when its key (sa + 1) * period + level is below A's; nxt is the
+    // entry after the one just solved (the solved record's nxt)
+    auto next_solve = [&](int& s_out, int& k_out, uint32_t nxt) -> bool {
+      if (took == 1) {
+        if (ja < mine) {
+          ea = nxt;
         } else if (b_on) {  // sweep sa done (B is never done first): B becomes A, B restarts
           sa += 1;
           ja = jb;
@@ -2107,7 +2118,15 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
         } else {
           a_on = false;
         }
+      } else if (took == 2) {
+        eb = nxt;  // (unused once jb reaches mine)
       }
+      if (!a_on) return false;
+      const bool take_b = b_on && jb < mine && (int)(eb >> 16) + period < (int)(ea >> 16);
+      s_out = take_b ? sa + 1 : sa;
+      k_out = (int)((take_b ? eb : ea) & 0xFFFFu);
+      if (take_b) ++jb; else ++ja;
+      took = take_b ? 2 : 1;
       return true;
     };
     // A lane holds one record in registers, read when it advances: a second, prefetched
@@ -2115,8 +2134,12 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
     // read latency it hides (a lane rarely takes two solves in consecutive passes).
     int s = 0, k = 0;
     VC<T> cur;
-    bool active = next_solve(s, k);
+    bool active = next_solve(s, k, 0u);
     if (active) cur = d.vc[cstart + k];
+    // The impulses a solve stores go out after the next record's read is issued (a read issued
+    // after a store would wait for the store to complete), from registers of their own that
+    // stay live into the next pass.
+    ImpPair st0{0.0, 0.0}, st1{0.0, 0.0};
     const int cap = 2 * nsw * nci + 64;
     // The pass test sits at the bottom and readiness is computed without branches, so the
     // compiler keeps the record where it is (inactive lanes and static sides read entry 0 and
@@ -2124,6 +2147,7 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
     int pass = 0;
     for (bool more = wave_any(active); more; more = wave_any(active && pass < cap)) {
       ++pass;
+      asm volatile("" ::"v"(st0.ni), "v"(st0.ti), "v"(st1.ni), "v"(st1.ti));
       {
         const int ia = cur.ia, ib = cur.ib;
         const int ra = s * (int)((unsigned)cur.rka >> 16) + (cur.rka & 0xFFFF);
@@ -2140,26 +2164,30 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
           V2d va = v2<double>(ia >= 0 ? pax : 0.0, ia >= 0 ? pay : 0.0);
           V2d vb = v2<double>(ib >= 0 ? pbx : 0.0, ib >= 0 ? pby : 0.0);
           double wa = ia >= 0 ? paw : 0.0, wb = ib >= 0 ? pbw : 0.0;
-          if (s + s0 == 0) {
-            warm_contact(cur, va, wa, vb, wb);
-          } else {
+          const bool solved = s + s0 != 0;
+          if (solved) {
             solve_contact_velocity(cur, va, wa, vb, wb);
-            VCP<T>* dst = d.vc[cstart + k].p;
-            dst[0].ni = cur.p[0].ni;
-            dst[0].ti = cur.p[0].ti;
-            if (cur.npts == 2) {
-              dst[1].ni = cur.p[1].ni;
-              dst[1].ti = cur.p[1].ti;
-            }
+          } else {
+            warm_contact(cur, va, wa, vb, wb);
           }
           if (ia >= 0) { sx[3 * ia] = va.x; sx[3 * ia + 1] = va.y; sx[3 * ia + 2] = wa; lds_st(&cnt[ia], ra + 1); }
           if (ib >= 0) { sx[3 * ib] = vb.x; sx[3 * ib + 1] = vb.y; sx[3 * ib + 2] = wb; lds_st(&cnt[ib], rb + 1); }
-          // the read follows this lane's own impulse stores, so a later sweep of the same
-          // contact sees them
-          if (next_solve(s, k)) {
-            cur = d.vc[cstart + k];
+          // (a one-point contact stores its unused second point unchanged)
+          st0 = ImpPair{cur.p[0].ni, cur.p[0].ti};
+          st1 = ImpPair{cur.p[1].ni, cur.p[1].ti};
+          asm volatile("" : "+v"(st0.ni), "+v"(st0.ti), "+v"(st1.ni), "+v"(st1.ti));
+          ImpPair* dst = reinterpret_cast<ImpPair*>(&d.vc[cstart + k].p[0].ni);
+          const int k_done = k;
+          if (next_solve(s, k, cur.nxt)) {
+            // the same contact again (its next sweep) continues from the registers, which hold
+            // its new impulses; any other record's read goes out before the stores below
+            if (k != k_done) cur = d.vc[cstart + k];
           } else {
             active = false;
+          }
+          if (solved) {
+            dst[0] = st0;
+            dst[sizeof(VCP<T>) / sizeof(ImpPair)] = st1;
           }
         }
       }
@@ -2198,6 +2226,9 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
       const int pcap = 2 * nci + 64;
       for (bool more = wave_any(active); more; more = wave_any(active && pass < pcap)) {
         ++pass;
+        // the record's ranks and next entry are taken here, a pass after its read: derived
+        // right after the read (where the compiler would move them), they would wait for it
+        asm volatile("" : "+v"(cur.rka), "+v"(cur.rkb), "+v"(cur.nxt));
         // readiness without branches (a branchy test makes the compiler copy the record at
         // every join); inactive lanes and static sides read entry 0 and ignore it
         const int ia = cur.ia, ib = cur.ib;
@@ -2217,7 +2248,7 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
         if (adv) {
           j += TS;
           active = j < nci;
-          if (active) cur = d.pc[cstart + (order[j] & 0xFFFFu)];
+          if (active) cur = d.pc[cstart + (cur.nxt & 0xFFFFu)];
         }
         pass_fence();
       }
@@ -4162,7 +4193,8 @@ __device__ void world_solve_toi(const Dev<T>& d, int w, ToiScratch& S) {
       vc.Ia = (double)d.b_iI[ga];
       vc.Ib = (double)d.b_iI[gbd];
       vc.npts = (cst >> CS_PC_SHIFT) & 3;
-      vc.rka = vc.rkb = vc.pad = 0;
+      vc.rka = vc.rkb = 0;
+      vc.nxt = 0u;
       vc.p[0].ni = imp.x; vc.p[0].ti = imp.y;  // the stored impulses (quirk Q8)
       vc.p[1].ni = imp.z; vc.p[1].ti = imp.w;
       vc.p[0].pad = vc.p[1].pad = 0.0;
