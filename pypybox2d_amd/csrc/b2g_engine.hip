@@ -2105,29 +2105,25 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
     // the lane's next solve: B when its key (sa + 1) * period + level is below A's; nxt is the
     // entry after the one just solved (the solved record's nxt)
     auto next_solve = [&](int& s_out, int& k_out, uint32_t nxt) -> bool {
-      if (took == 1) {
-        if (ja < mine) {
-          ea = nxt;
-        } else if (b_on) {  // sweep sa done (B is never done first): B becomes A, B restarts
-          sa += 1;
-          ja = jb;
-          ea = eb;
-          jb = 0;
-          eb = e0;
-          b_on = sa + 1 < nsw;
-        } else {
-          a_on = false;
-        }
-      } else if (took == 2) {
-        eb = nxt;  // (unused once jb reaches mine)
-      }
-      if (!a_on) return false;
-      const bool take_b = b_on && jb < mine && (int)(eb >> 16) + period < (int)(ea >> 16);
-      s_out = take_b ? sa + 1 : sa;
+      // selects only: a branch per case costs more than the few operations it skips
+      const bool ta = took == 1, tb = took == 2;
+      const bool a_done = ta & (ja >= mine);  // A finished sweep sa (B is never done first)
+      const bool shift = a_done & b_on;       // B becomes A, B restarts
+      a_on = a_on & !(a_done & !b_on);
+      const uint32_t eb1 = tb ? nxt : eb;  // (unused once jb reaches mine)
+      ea = shift ? eb1 : (ta ? nxt : ea);
+      ja = shift ? jb : ja;
+      eb = shift ? e0 : eb1;
+      jb = shift ? 0 : jb;
+      sa += shift ? 1 : 0;
+      b_on = shift ? sa + 1 < nsw : b_on;
+      const bool take_b = b_on & (jb < mine) & ((int)(eb >> 16) + period < (int)(ea >> 16));
+      s_out = sa + (take_b ? 1 : 0);
       k_out = (int)((take_b ? eb : ea) & 0xFFFFu);
-      if (take_b) ++jb; else ++ja;
+      jb += take_b ? 1 : 0;
+      ja += take_b ? 0 : 1;
       took = take_b ? 2 : 1;
-      return true;
+      return a_on;
     };
     // A lane holds one record in registers, read when it advances: a second, prefetched
     // record would have to be copied into place at every advance, which costs more than the
@@ -2138,8 +2134,11 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
     if (active) cur = d.vc[cstart + k];
     // The impulses a solve stores go out after the next record's read is issued (a read issued
     // after a store would wait for the store to complete), from registers of their own that
-    // stay live into the next pass.
+    // stay live into the next pass.  When the next solve is the same contact (its next sweep),
+    // the read returns the impulses from before the store: `fresh` has the solve take them from
+    // those registers instead.
     ImpPair st0{0.0, 0.0}, st1{0.0, 0.0};
+    bool fresh = false;
     const int cap = 2 * nsw * nci + 64;
     // The pass test sits at the bottom and readiness is computed without branches, so the
     // compiler keeps the record where it is (inactive lanes and static sides read entry 0 and
@@ -2164,6 +2163,10 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
           V2d va = v2<double>(ia >= 0 ? pax : 0.0, ia >= 0 ? pay : 0.0);
           V2d vb = v2<double>(ib >= 0 ? pbx : 0.0, ib >= 0 ? pby : 0.0);
           double wa = ia >= 0 ? paw : 0.0, wb = ib >= 0 ? pbw : 0.0;
+          cur.p[0].ni = fresh ? st0.ni : cur.p[0].ni;
+          cur.p[0].ti = fresh ? st0.ti : cur.p[0].ti;
+          cur.p[1].ni = fresh ? st1.ni : cur.p[1].ni;
+          cur.p[1].ti = fresh ? st1.ti : cur.p[1].ti;
           const bool solved = s + s0 != 0;
           if (solved) {
             solve_contact_velocity(cur, va, wa, vb, wb);
@@ -2178,13 +2181,9 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
           asm volatile("" : "+v"(st0.ni), "+v"(st0.ti), "+v"(st1.ni), "+v"(st1.ti));
           ImpPair* dst = reinterpret_cast<ImpPair*>(&d.vc[cstart + k].p[0].ni);
           const int k_done = k;
-          if (next_solve(s, k, cur.nxt)) {
-            // the same contact again (its next sweep) continues from the registers, which hold
-            // its new impulses; any other record's read goes out before the stores below
-            if (k != k_done) cur = d.vc[cstart + k];
-          } else {
-            active = false;
-          }
+          active = next_solve(s, k, cur.nxt);
+          fresh = k == k_done;
+          cur = d.vc[cstart + k];  // (a lane that is done reads its last record again)
           if (solved) {
             dst[0] = st0;
             dst[sizeof(VCP<T>) / sizeof(ImpPair)] = st1;
