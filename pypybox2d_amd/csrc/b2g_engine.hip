@@ -1487,29 +1487,24 @@ __device__ __forceinline__ void solve_point_position(const PC<T>& pc, int j, dou
     xfa.p = -rmul(qa, v2<T>(pc.lcax, pc.lcay));
     xfb.q = qb;
     xfb.p = dcb - rmul(qb, v2<T>(pc.lcbx, pc.lcby));
-    V2<T> normal, point;
-    T sep;
-    V2<T> lpj = j == 0 ? v2<T>(pc.lp0x, pc.lp0y) : v2<T>(pc.lp1x, pc.lp1y);
-    V2<T> ln = v2<T>(pc.lnx, pc.lny), lpp = v2<T>(pc.lpx, pc.lpy);
-    if (pc.type == M_CIRCLES) {
-      V2<T> pa = xmul(xfa, lpp), pb = xmul(xfb, v2<T>(pc.lp0x, pc.lp0y));
-      normal = normalized(pb - pa);
-      point = scale((T)0.5, pa + pb);
-      sep = dot(pb - pa, normal) - pc.ra - pc.rb;
-    } else if (pc.type == M_FACE_A) {
-      normal = rmul(qa, ln);
-      V2<T> plane = xmul(xfa, lpp);
-      V2<T> clip = xmul(xfb, lpj);
-      sep = dot(clip - plane, normal) - pc.ra - pc.rb;
-      point = clip;
-    } else {
-      normal = rmul(qb, ln);
-      V2<T> plane = xmul(xfb, lpp);
-      V2<T> clip = xmul(xfa, lpj);
-      sep = dot(clip - plane, normal) - pc.ra - pc.rb;
-      point = clip;
-      normal = -normal;
-    }
+    // PositionSolverManifold (contact_util.py): circles, face A and face B share one sequence of
+    // operations with the roles of the two transforms as selects (the same arithmetic per type
+    // as three branches, without a divergent path per type).  The reference point lpp goes
+    // through X, the clip point through Y: circles and face A: X = A, Y = B; face B swapped.
+    const bool circ = pc.type == M_CIRCLES, face_b = !circ && pc.type != M_FACE_A;
+    const V2<T> lpj = j == 0 ? v2<T>(pc.lp0x, pc.lp0y) : v2<T>(pc.lp1x, pc.lp1y);
+    const V2<T> ln = v2<T>(pc.lnx, pc.lny), lpp = v2<T>(pc.lpx, pc.lpy);
+    XF<T> X, Y;
+    X.q.c = face_b ? xfb.q.c : xfa.q.c; X.q.s = face_b ? xfb.q.s : xfa.q.s;
+    X.p.x = face_b ? xfb.p.x : xfa.p.x; X.p.y = face_b ? xfb.p.y : xfa.p.y;
+    Y.q.c = face_b ? xfa.q.c : xfb.q.c; Y.q.s = face_b ? xfa.q.s : xfb.q.s;
+    Y.p.x = face_b ? xfa.p.x : xfb.p.x; Y.p.y = face_b ? xfa.p.y : xfb.p.y;
+    const V2<T> plane = xmul(X, lpp), clip = xmul(Y, lpj);  // (a circle manifold has one point)
+    const V2<T> dd = clip - plane;
+    const V2<T> nrm = circ ? normalized(dd) : rmul(X.q, ln);
+    const T sep = dot(dd, nrm) - pc.ra - pc.rb;
+    const V2<T> point = circ ? scale((T)0.5, plane + clip) : clip;
+    const V2<T> normal = face_b ? -nrm : nrm;
     V2<T> ra = point, rb = point - dcb;
     min_sep = pmin(min_sep, sep);
     T C = pclamp((T)BAUMGARTE * (sep + (T)LINEAR_SLOP), -(T)MAX_LINEAR_CORRECTION, (T)0);
