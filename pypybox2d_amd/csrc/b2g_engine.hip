@@ -230,6 +230,7 @@ struct Dev {
   int* w_asleep;
   int* w_nisl;
   int2* w_itot;        // per world: bodies and contacts of its islands (the island lists' lengths)
+  int* w_ipre;         // exclusive prefixes of w_itot over worlds: bodies [nw + 1], then contacts [nw + 1]
   // bodies
   int* b_type;
   int* b_flags;
@@ -2274,34 +2275,96 @@ __device__ void solve_island_team(const Dev<T>& d, int w, int bstart, int nbi, i
 // 1. integrate velocities (island.py:71-95) of every island body, and 2. the constraint records
 // of every island contact (contact.py:376-496), as flat launches ahead of k_solve: the work
 // has no dependencies inside an island, so it need not sit at the head of each island's
-// solve.  Grid: (chunks of 256) x worlds; w_itot bounds each world's island lists.
+// solve.  k_island_prefix lays the worlds' island lists end to end (exclusive prefixes of
+// w_itot); the two kernels then stride over the flat index, each block finding the world of
+// its first entry by binary search and its threads stepping on from there, so no block is
+// launched for a world's unused capacity.
+constexpr int kPrefixBlock = 1024;
+constexpr int kPrepGrid = 4096;  // blocks of the flat setup kernels (each strides over the entries)
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_prep_bodies(Dev<T> d, int chunks) {
-  const int w = blockIdx.x / chunks, i = (blockIdx.x % chunks) * kBlock + threadIdx.x;
-  if (i == 0) {  // the world's island bodies and contacts, for the work counters
+__global__ void __launch_bounds__(kPrefixBlock) k_island_prefix(Dev<T> d) {
+  __shared__ int scratch[kPrefixBlock];
+  const int nw = d.nw;
+  int* pb = d.w_ipre;
+  int* pc = d.w_ipre + nw + 1;
+  for (int w = threadIdx.x; w < nw; w += kPrefixBlock) {
     const int2 t = d.w_itot[w];
-    if (t.x > 0) atomicAdd(&d.stat[S_IBODIES], (unsigned long long)t.x);
-    if (t.y > 0) atomicAdd(&d.stat[S_ICONTACTS], (unsigned long long)t.y);
+    pb[w] = t.x;
+    pc[w] = t.y;
   }
-  if (i >= d.w_itot[w].x) return;
-  const int b0 = d.w_body_off[w];
-  const IslandK c = island_k(d, w, b0);
-  d.pv[b0 + i] = integrate_velocity(d, c, b0, i);
+  __syncthreads();
+  const int tb = block_exclusive_scan<kPrefixBlock>(pb, nw, scratch);
+  const int tc = block_exclusive_scan<kPrefixBlock>(pc, nw, scratch);
+  if (threadIdx.x == 0) {
+    pb[nw] = tb;
+    pc[nw] = tc;
+    // the island bodies and contacts of the step, for the work counters
+    if (tb > 0) atomicAdd(&d.stat[S_IBODIES], (unsigned long long)tb);
+    if (tc > 0) atomicAdd(&d.stat[S_ICONTACTS], (unsigned long long)tc);
+  }
+}
+
+// The world of flat entry i: w with pre[w] <= i < pre[w + 1], from w_first (the world of the
+// block's first entry) on; worlds without island entries have pre[w] == pre[w + 1].
+__device__ __forceinline__ int world_of(const int* pre, int w_first, int i) {
+  int w = w_first;
+  while (pre[w + 1] <= i) ++w;
+  return w;
+}
+// The last world w with pre[w] <= i (block-uniform, by thread 0 into LDS).
+__device__ __forceinline__ int first_world(const int* pre, int nw, int i) {
+  int lo = 0, hi = nw;  // pre[lo] <= i < pre[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= i) lo = mid; else hi = mid;
+  }
+  return lo;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_prep_contacts(Dev<T> d, int chunks) {
-  const int w = blockIdx.x / chunks, k = (blockIdx.x % chunks) * kBlock + threadIdx.x;
-  if (k >= d.w_itot[w].y) return;
-  const int b0 = d.w_body_off[w], cb = d.w_con_base[w];
-  const int bstart = b0 + d.i_isb[cb + k];  // the contact's island
-  const IslandK c = island_k(d, w, bstart);
-  // each contact's rank in its bodies' sequences and their degrees: rank | degree << 16
-  const uint2 r = d.i_rank[cb + k];
-  const int rka = (int)((r.x & 0xFFFFu) | (r.y & 0xFFFFu) << 16);
-  const int rkb = (int)((r.x >> 16) | (r.y >> 16) << 16);
-  int ia, ib;
-  init_contact(d, c, bstart, cb, k, ia, ib, rka, rkb);
+__global__ void __launch_bounds__(kBlock) k_prep_bodies(Dev<T> d) {
+  __shared__ int w_first;
+  const int* pre = d.w_ipre;
+  const int total = pre[d.nw];
+  for (int base = blockIdx.x * kBlock; base < total; base += gridDim.x * kBlock) {
+    if (threadIdx.x == 0) w_first = first_world(pre, d.nw, base);
+    __syncthreads();
+    const int i = base + threadIdx.x;
+    if (i < total) {
+      const int w = world_of(pre, w_first, i);
+      const int b0 = d.w_body_off[w];
+      const IslandK c = island_k(d, w, b0);
+      const int k = i - pre[w];
+      d.pv[b0 + k] = integrate_velocity(d, c, b0, k);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_prep_contacts(Dev<T> d) {
+  __shared__ int w_first;
+  const int* pre = d.w_ipre + d.nw + 1;
+  const int total = pre[d.nw];
+  for (int base = blockIdx.x * kBlock; base < total; base += gridDim.x * kBlock) {
+    if (threadIdx.x == 0) w_first = first_world(pre, d.nw, base);
+    __syncthreads();
+    const int i = base + threadIdx.x;
+    if (i < total) {
+      const int w = world_of(pre, w_first, i);
+      const int k = i - pre[w];
+      const int b0 = d.w_body_off[w], cb = d.w_con_base[w];
+      const int bstart = b0 + d.i_isb[cb + k];  // the contact's island
+      const IslandK c = island_k(d, w, bstart);
+      // each contact's rank in its bodies' sequences and their degrees: rank | degree << 16
+      const uint2 r = d.i_rank[cb + k];
+      const int rka = (int)((r.x & 0xFFFFu) | (r.y & 0xFFFFu) << 16);
+      const int rkb = (int)((r.x >> 16) | (r.y >> 16) << 16);
+      int ia, ib;
+      init_contact(d, c, bstart, cb, k, ia, ib, rka, rkb);
+    }
+    __syncthreads();
+  }
 }
 
 // Island timing (b2g_island_times): wall-clock ticks around one island's solve, its size and
@@ -5052,6 +5115,7 @@ static int upload_batch(b2g_ctx* ctx, const b2g_batch* b) {
   TRY(dalloc(ctx, &d.w_asleep, nw));
   TRY(dalloc(ctx, &d.w_nisl, nw));
   TRY(dalloc(ctx, &d.w_itot, nw));
+  TRY(dalloc(ctx, &d.w_ipre, 2 * (nw + 1)));
   TRY(dalloc(ctx, &d.b_type, nb));
   TRY(dalloc(ctx, &d.b_flags, nb));
   TRY(dalloc(ctx, &d.b_island, nb));
@@ -5468,10 +5532,10 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[2], st));
     // the island solve: per-body and per-contact setup over all islands, then the islands
-    const int bch = std::max(1, (ctx->max_nb + kBlock - 1) / kBlock);
-    const int cch = std::max(1, (ctx->max_ccap + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_prep_bodies<T>, dim3((unsigned)(bch * nw)), dim3(kBlock), 0, st, d, bch);
-    hipLaunchKernelGGL(k_prep_contacts<T>, dim3((unsigned)(cch * nw)), dim3(kBlock), 0, st, d, cch);
+    hipLaunchKernelGGL(k_island_prefix<T>, dim3(1), dim3(kPrefixBlock), 0, st, d);
+    const int pgrid = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, kPrepGrid));
+    hipLaunchKernelGGL(k_prep_bodies<T>, dim3((unsigned)pgrid), dim3(kBlock), 0, st, d);
+    hipLaunchKernelGGL(k_prep_contacts<T>, dim3((unsigned)kPrepGrid), dim3(kBlock), 0, st, d);
     const int swaves = std::max(1, std::min((ctx->nb + kSolveBlock - 1) / kSolveBlock, ctx->solve_grid));
     hipLaunchKernelGGL(k_solve<T>, dim3(swaves), dim3(kSolveBlock), ctx->team_lds, st, d);
     if (ctx->nj > 0)
