@@ -2283,19 +2283,40 @@ constexpr int kPrefixBlock = 1024;
 constexpr int kPrepGrid = 4096;  // blocks of the flat setup kernels (each strides over the entries)
 template <typename T>
 __global__ void __launch_bounds__(kPrefixBlock) k_island_prefix(Dev<T> d) {
-  __shared__ int scratch[kPrefixBlock];
-  const int nw = d.nw;
+  // both prefixes in one pass: each thread sums a contiguous run of worlds, the runs' sums are
+  // scanned in LDS, and each thread writes its run's prefixes
+  __shared__ int sb[kPrefixBlock], sc[kPrefixBlock];
+  const int nw = d.nw, t = threadIdx.x;
+  const int per = (nw + kPrefixBlock - 1) / kPrefixBlock;
+  const int beg = min(nw, t * per), end = min(nw, beg + per);
+  int xb = 0, xc = 0;
+  for (int w = beg; w < end; ++w) {
+    const int2 v = d.w_itot[w];
+    xb += v.x;
+    xc += v.y;
+  }
+  sb[t] = xb;
+  sc[t] = xc;
+  __syncthreads();
+  for (int off = 1; off < kPrefixBlock; off <<= 1) {
+    const int vb = t >= off ? sb[t - off] : 0, vc = t >= off ? sc[t - off] : 0;
+    __syncthreads();
+    sb[t] += vb;
+    sc[t] += vc;
+    __syncthreads();
+  }
   int* pb = d.w_ipre;
   int* pc = d.w_ipre + nw + 1;
-  for (int w = threadIdx.x; w < nw; w += kPrefixBlock) {
-    const int2 t = d.w_itot[w];
-    pb[w] = t.x;
-    pc[w] = t.y;
+  int rb = sb[t] - xb, rc = sc[t] - xc;
+  for (int w = beg; w < end; ++w) {
+    const int2 v = d.w_itot[w];
+    pb[w] = rb;
+    pc[w] = rc;
+    rb += v.x;
+    rc += v.y;
   }
-  __syncthreads();
-  const int tb = block_exclusive_scan<kPrefixBlock>(pb, nw, scratch);
-  const int tc = block_exclusive_scan<kPrefixBlock>(pc, nw, scratch);
-  if (threadIdx.x == 0) {
+  if (t == kPrefixBlock - 1) {
+    const int tb = sb[t], tc = sc[t];
     pb[nw] = tb;
     pc[nw] = tc;
     // the island bodies and contacts of the step, for the work counters
