@@ -2406,10 +2406,12 @@ __device__ __forceinline__ void record_island_time(const Dev<T>& d, int q, long 
 // One wavefront per block.  The queue is ordered largest class first.  Its
 // entries of classes >= heavy_class are dealt to wavefronts 64 / T islands at
 // a time (T = team_size of the class), each team with its share of the LDS
-// budget; an island whose state does not fit that share is solved by the
-// team's first lane alone.  The remaining islands are dealt 64 per wavefront,
-// one lane each.  Wavefronts are taken in queue order, so the longest islands
-// start first and are spread over the whole chip.
+// budget (queue_class has moved an island whose state does not fit its class's
+// share to the top class, a whole wavefront and its budget; one that does not
+// fit even that is solved by the team's first lane alone).  The remaining
+// islands are dealt 64 per wavefront, one lane each.  Islands with joints are
+// left to k_solve_joints.  Wavefronts are taken in queue order, so the longest
+// islands start first and are spread over the whole chip.
 // Two wavefronts per SIMD at least: the FP64 records would otherwise take
 // more than 256 VGPRs and halve the resident wavefronts.
 template <typename T>
@@ -2437,7 +2439,6 @@ __global__ void __launch_bounds__(kSolveBlock) __attribute__((amdgpu_waves_per_e
         const int nci = nc & 0x3FFFFFFF, last_allow = (nc >> 30) & 1;
         const int share = d.team_lds / per;
         const long long t0 = d.i_time ? (long long)wall_clock64() : 0;
-        // an island with joints is solved by the team's first lane (the joints run in sequence)
         const bool fits = team_lds_bytes(nbi) <= share;
         SolveMarks mk{0, 0};
         if (island_has_joints(d, in.x, in.y)) {
