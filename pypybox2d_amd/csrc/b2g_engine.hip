@@ -2940,48 +2940,34 @@ __host__ __device__ inline size_t walk_lds_bytes(int nb, int ccap) {
   return ((size_t)nb * 4 + 15) / 16 * 16 * 2 + ((size_t)nb * 2 + 15) / 16 * 16 + ((size_t)(ccap + 31) / 32 * 4 + 15) / 16 * 16;
 }
 // Union-find over a large world's bodies (lock-free hooking of the larger root under the
-// smaller, path halving), on 16-bit parents packed two to a word in LDS (a world has fewer
-// than 65535 bodies, so any world's parents fit 128 KB): a half is replaced by a
-// compare-and-swap of its word, so a concurrent change to the other half only retries.  Once
-// the unions are done, roots are found without halving: a halving write could replace a root
-// that another thread has just stored with a non-root ancestor.
-__device__ __forceinline__ int ufl_get(const uint32_t* P, int x) {
-  const uint32_t w = __hip_atomic_load(&P[x >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  return (int)((x & 1) ? (w >> 16) : (w & 0xFFFFu));
+// smaller, path halving) on the world's parent array in global memory, shared by the
+// workgroups of k_islands_big_ends (device-scope atomics: the XCDs' L2s are not coherent for
+// plain accesses).  The unions end with that kernel; k_islands_big then finds the roots
+// read-only.
+__device__ __forceinline__ int gu_get(int* P, int x) {
+  return __hip_atomic_load(&P[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// parent[x] := v if it is still `expect`; false if it was not
-__device__ __forceinline__ bool ufl_cas(uint32_t* P, int x, int expect, int v) {
-  const int sh = (x & 1) * 16;
-  uint32_t w = __hip_atomic_load(&P[x >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+__device__ int gu_find(int* P, int x) {
   for (;;) {
-    if ((int)((w >> sh) & 0xFFFFu) != expect) return false;
-    const uint32_t nw = (w & ~(0xFFFFu << sh)) | ((uint32_t)v << sh);
-    const uint32_t old = atomicCAS(&P[x >> 1], w, nw);
-    if (old == w) return true;
-    w = old;
-  }
-}
-__device__ int ufl_find(uint32_t* P, int x) {
-  for (;;) {
-    const int p = ufl_get(P, x);
+    const int p = gu_get(P, x);
     if (p == x) return x;
-    const int gp = ufl_get(P, p);
-    if (gp != p) ufl_cas(P, x, p, gp);  // path halving (skipped if x moved meanwhile)
+    const int gp = gu_get(P, p);
+    if (gp != p) atomicCAS(&P[x], p, gp);  // path halving (skipped if x moved meanwhile)
     x = gp;
   }
 }
-__device__ int ufl_root(const uint32_t* P, int x) {
-  for (int p = ufl_get(P, x); p != x; p = ufl_get(P, x)) x = p;
-  return x;
-}
-__device__ void ufl_unite(uint32_t* P, int a, int b) {
+__device__ void gu_unite(int* P, int a, int b) {
   for (;;) {
-    a = ufl_find(P, a);
-    b = ufl_find(P, b);
+    a = gu_find(P, a);
+    b = gu_find(P, b);
     if (a == b) return;
     if (a < b) { const int t = a; a = b; b = t; }
-    if (ufl_cas(P, a, a, b)) return;  // the root with the larger index joins the smaller
+    if (atomicCAS(&P[a], a, b) == a) return;  // the root with the larger index joins the smaller
   }
+}
+__device__ __forceinline__ int gu_root(const int* P, int x) {
+  for (int p = P[x]; p != x; p = P[x]) x = p;
+  return x;
 }
 
 // The touching contacts among kEndsBatch slots base, base + stride, ... of a large world, with
@@ -3017,33 +3003,29 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// The large-world island search, in six launches: k_islands_big_head (one workgroup per
+// world) sets up the per-body state and runs the awake gating; k_islands_big_ends (many
+// workgroups per world) takes the touching contacts' ends, the bodies' degrees and the
+// union-find over them; k_islands_big_scan (one) the adjacency offsets; k_islands_big_fill
+// and k_islands_big_sort (many) the adjacency in slot order and the component roots; and
+// k_islands_big (one workgroup per world) the rest.
 template <typename T>
-__global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_cap, int smem_dyn) {
-  __shared__ int scan[kBigBlock];
+__global__ void __launch_bounds__(kBigBlock) k_islands_big_head(Dev<T> d) {
   __shared__ int sh[8];
-  // the union-find parents, 16 bits each; once the components are stored, each wavefront's
-  // 2 * sort_cap words for the level sort of islands of up to sort_cap contacts
-  extern __shared__ uint32_t uf_lds[];
   const int w = d.big_list[blockIdx.x], tid = threadIdx.x;
   const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
   const int f0 = d.w_fix_off[w];
-  const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
+  const int cb = d.w_con_base[w];
   const int used = d.w_con_used[w];
   int* bst = d.g_bst + b0;
   int* wake = d.g_wake + b0;
   int* off = d.g_off + b0 + w;
-  int* cur = d.g_cur + b0;
   int* lev = d.g_lev + b0;
   int* parent = d.g_parent + b0;
   int* seed = d.g_seed + b0;
   int* cntb = d.g_cntb + b0;
   int* cntc = d.g_cntc + b0;
-  int* bidx = d.g_bidx + b0;
-  int2* adj = d.g_adj + 2 * (size_t)cb;
   int* cflag = d.g_cflag + cb;
-  int4* isl = d.g_isl + b0;
-  int* isl_b = d.i_bodies + b0;
-  int* isl_c = d.i_contacts + cb;
   for (int i = tid; i < nb; i += kBigBlock) {
     const int fl = d.b_flags[b0 + i], ty = d.b_type[b0 + i];
     bst[i] = (ty == BT_STATIC ? 1 : 0) | ((fl & BF_AWAKE) ? 2 : 0) | ((fl & BF_ALLOW_SLEEP) ? 4 : 0) |
@@ -3056,7 +3038,6 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
     cntc[i] = 0;
   }
   for (int sl = tid; sl < used; sl += kBigBlock) cflag[sl] = 0;
-  for (int i = tid; i < (nb + 1) / 2; i += kBigBlock) uf_lds[i] = (uint32_t)(2 * i) | (uint32_t)(2 * i + 1) << 16;
   if (tid < 8) sh[tid] = 0;
   __syncthreads();
 
@@ -3097,11 +3078,23 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
     __syncthreads();
   }
 
-  // --- touching-contact graph in ContactManager order ---
-  for (int base = tid; base < used; base += kEndsBatch * kBigBlock) {
+}
+
+// The touching-contact graph's first pass: each touching contact adds to its bodies' degrees
+// and unites them (two non-static bodies); grid (kBigEndsGrid, big worlds) x kBlock.
+constexpr int kBigEndsGrid = 64;
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_islands_big_ends(Dev<T> d) {
+  const int w = d.big_list[blockIdx.y];
+  const int b0 = d.w_body_off[w], f0 = d.w_fix_off[w], cb = d.w_con_base[w], used = d.w_con_used[w];
+  const int* bst = d.g_bst + b0;
+  int* off = d.g_off + b0 + w;
+  int* parent = d.g_parent + b0;
+  const int stride = gridDim.x * kBlock;
+  for (int base = blockIdx.x * kBlock + threadIdx.x; base < used; base += kEndsBatch * stride) {
     bool ok[kEndsBatch];
     int la[kEndsBatch], lb[kEndsBatch];
-    touching_ends(d, cb, f0, used, base, ok, la, lb);
+    touching_ends(d, cb, f0, used, base, ok, la, lb, stride);
     int sa[kEndsBatch], sb[kEndsBatch];
 #pragma unroll
     for (int u = 0; u < kEndsBatch; ++u) {
@@ -3113,18 +3106,37 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
       if (!ok[u]) continue;
       atomicAdd(&off[la[u]], 1);
       atomicAdd(&off[lb[u]], 1);
-      if (!(sa[u] & 1) && !(sb[u] & 1)) ufl_unite(uf_lds, la[u], lb[u]);
+      if (!(sa[u] & 1) && !(sb[u] & 1)) gu_unite(parent, la[u], lb[u]);
     }
   }
-  __syncthreads();
+}
+
+// The bodies' adjacency offsets (exclusive scan of the degrees), one workgroup per world.
+template <typename T>
+__global__ void __launch_bounds__(kBigBlock) k_islands_big_scan(Dev<T> d) {
+  __shared__ int scan[kBigBlock];
+  const int w = d.big_list[blockIdx.x], tid = threadIdx.x;
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
+  int* off = d.g_off + b0 + w;
+  int* cur = d.g_cur + b0;
   const int total = block_exclusive_scan<kBigBlock>(off, nb, scan);
   if (tid == 0) off[nb] = total;
   for (int i = tid; i < nb; i += kBigBlock) cur[i] = off[i];
-  __syncthreads();
-  for (int base = tid; base < used; base += kEndsBatch * kBigBlock) {
+}
+
+// The adjacency entries (contact slot, other body) of every touching contact at its two bodies;
+// grid (kBigEndsGrid, big worlds) x kBlock.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_islands_big_fill(Dev<T> d) {
+  const int w = d.big_list[blockIdx.y];
+  const int b0 = d.w_body_off[w], f0 = d.w_fix_off[w], cb = d.w_con_base[w], used = d.w_con_used[w];
+  int* cur = d.g_cur + b0;
+  int2* adj = d.g_adj + 2 * (size_t)cb;
+  const int stride = gridDim.x * kBlock;
+  for (int base = blockIdx.x * kBlock + threadIdx.x; base < used; base += kEndsBatch * stride) {
     bool ok[kEndsBatch];
     int la[kEndsBatch], lb[kEndsBatch];
-    touching_ends(d, cb, f0, used, base, ok, la, lb);
+    touching_ends(d, cb, f0, used, base, ok, la, lb, stride);
     int pa[kEndsBatch], pb[kEndsBatch];
 #pragma unroll
     for (int u = 0; u < kEndsBatch; ++u) {
@@ -3134,13 +3146,27 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
 #pragma unroll
     for (int u = 0; u < kEndsBatch; ++u) {
       if (!ok[u]) continue;
-      const int sl = base + u * kBigBlock;
+      const int sl = base + u * stride;
       adj[pa[u]] = make_int2(sl, lb[u]);
       adj[pb[u]] = make_int2(sl, la[u]);
     }
   }
-  __syncthreads();
-  for (int i = tid; i < nb; i += kBigBlock) {  // ascending slot order per body; component roots
+}
+
+// Each body's entries in ascending slot order (= body._contacts order) and, for a non-static
+// body, its component's root into the wake slots (free again); grid (kBigEndsGrid, big worlds)
+// x kBlock.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_islands_big_sort(Dev<T> d) {
+  const int w = d.big_list[blockIdx.y];
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0, cb = d.w_con_base[w];
+  const int* bst = d.g_bst + b0;
+  const int* off = d.g_off + b0 + w;
+  const int* parent = d.g_parent + b0;
+  int* cur = d.g_cur + b0;
+  int* wake = d.g_wake + b0;
+  int2* adj = d.g_adj + 2 * (size_t)cb;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < nb; i += gridDim.x * kBlock) {
     cur[i] = 0;  // from here on: contacts of the body seen by the island search
     const int beg = off[i], end = off[i + 1];
     for (int k = beg + 1; k < end; ++k) {
@@ -3152,8 +3178,38 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
       }
       adj[j + 1] = e;
     }
-    if (!(bst[i] & 1)) wake[i] = ufl_root(uf_lds, i);  // the wake slots are free again
+    if (!(bst[i] & 1)) wake[i] = gu_root(parent, i);
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_cap, int smem_dyn) {
+  __shared__ int scan[kBigBlock];
+  __shared__ int sh[8];
+  // each wavefront's 2 * sort_cap words for the level sort of islands of up to sort_cap
+  // contacts, or the walks' state
+  extern __shared__ uint32_t uf_lds[];
+  const int w = d.big_list[blockIdx.x], tid = threadIdx.x;
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
+  const int f0 = d.w_fix_off[w];
+  const int cb = d.w_con_base[w], ccap = d.w_con_cap[w];
+  const int used = d.w_con_used[w];
+  int* bst = d.g_bst + b0;
+  int* wake = d.g_wake + b0;
+  int* off = d.g_off + b0 + w;
+  int* cur = d.g_cur + b0;
+  int* lev = d.g_lev + b0;
+  int* parent = d.g_parent + b0;
+  int* seed = d.g_seed + b0;
+  int* cntb = d.g_cntb + b0;
+  int* cntc = d.g_cntc + b0;
+  int* bidx = d.g_bidx + b0;
+  int2* adj = d.g_adj + 2 * (size_t)cb;
+  int* cflag = d.g_cflag + cb;
+  int4* isl = d.g_isl + b0;
+  int* isl_b = d.i_bodies + b0;
+  int* isl_c = d.i_contacts + cb;
+  if (tid < 8) sh[tid] = 0;
   __syncthreads();
   for (int i = tid; i < nb; i += kBigBlock)
     if (!(bst[i] & 1)) parent[i] = wake[i];
@@ -5547,9 +5603,15 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
       else
         hipLaunchKernelGGL((k_islands<T, kBlock / 2>), dim3(r.count), dim3(kBlock / 2), r.smem, s, d, r.first);
     }));
-    if (nbig > 0)
+    if (nbig > 0) {
+      hipLaunchKernelGGL(k_islands_big_head<T>, dim3(nbig), dim3(kBigBlock), 0, st, d);
+      hipLaunchKernelGGL(k_islands_big_ends<T>, dim3(kBigEndsGrid, nbig), dim3(kBlock), 0, st, d);
+      hipLaunchKernelGGL(k_islands_big_scan<T>, dim3(nbig), dim3(kBigBlock), 0, st, d);
+      hipLaunchKernelGGL(k_islands_big_fill<T>, dim3(kBigEndsGrid, nbig), dim3(kBlock), 0, st, d);
+      hipLaunchKernelGGL(k_islands_big_sort<T>, dim3(kBigEndsGrid, nbig), dim3(kBlock), 0, st, d);
       hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), ctx->smem_big, st, d,
                          (int)(ctx->smem_big / (kBigBlock / 64 * 2 * sizeof(int))), (int)ctx->smem_big);
+    }
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[2], st));
