@@ -3003,12 +3003,13 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// The large-world island search, in six launches: k_islands_big_head (one workgroup per
+// The large-world island search, in eight launches: k_islands_big_head (one workgroup per
 // world) sets up the per-body state and runs the awake gating; k_islands_big_ends (many
 // workgroups per world) takes the touching contacts' ends, the bodies' degrees and the
 // union-find over them; k_islands_big_scan (one) the adjacency offsets; k_islands_big_fill
-// and k_islands_big_sort (many) the adjacency in slot order and the component roots; and
-// k_islands_big (one workgroup per world) the rest.
+// and k_islands_big_sort (many) the adjacency in slot order and the component roots;
+// k_islands_big_seeds and k_islands_big_counts (many) the islands' seeds and sizes; and
+// k_islands_big (one workgroup per world) the islands' ranges, the walks and the rest.
 template <typename T>
 __global__ void __launch_bounds__(kBigBlock) k_islands_big_head(Dev<T> d) {
   __shared__ int sh[8];
@@ -3182,6 +3183,97 @@ __global__ void __launch_bounds__(kBlock) k_islands_big_sort(Dev<T> d) {
   }
 }
 
+// Islands are the components with an awake, active, non-static body; seed = the first such
+// body.  Each thread takes a contiguous run of bodies (then of slots) and adds to a
+// component's seed and counts once per run of equal roots: the bodies and contacts of one
+// component mostly sit together, and one atomic per body on a few hot roots serialises in L2.
+// Grid (kBigEndsGrid, big worlds) x kBlock.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_islands_big_seeds(Dev<T> d) {
+  const int w = d.big_list[blockIdx.y];
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
+  const int* bst = d.g_bst + b0;
+  const int* wake = d.g_wake + b0;  // the roots (k_islands_big_sort)
+  int* parent = d.g_parent + b0;
+  int* seed = d.g_seed + b0;
+  const int nt = gridDim.x * kBlock, t = blockIdx.x * kBlock + threadIdx.x;
+  const int bper = (nb + nt - 1) / nt, bbeg = min(nb, t * bper), bend = min(nb, bbeg + bper);
+  int root = -1, first = kIntMax;
+  for (int i = bbeg; i < bend; ++i) {
+    const int bs = bst[i];
+    if (bs & 1) continue;
+    const int r = wake[i];
+    parent[i] = r;
+    if (!(bs & 2) || !(bs & 8)) continue;
+    if (r != root) {
+      if (root >= 0) atomicMin(&seed[root], first);
+      root = r;
+      first = i;
+    }
+  }
+  if (root >= 0) atomicMin(&seed[root], first);
+}
+
+// Each island's bodies and contacts (components with a seed); grid (kBigEndsGrid, big worlds)
+// x kBlock.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_islands_big_counts(Dev<T> d) {
+  const int w = d.big_list[blockIdx.y];
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
+  const int f0 = d.w_fix_off[w], cb = d.w_con_base[w], used = d.w_con_used[w];
+  const int* bst = d.g_bst + b0;
+  const int* parent = d.g_parent + b0;
+  const int* seed = d.g_seed + b0;
+  int* cntb = d.g_cntb + b0;
+  int* cntc = d.g_cntc + b0;
+  const int nt = gridDim.x * kBlock, t = blockIdx.x * kBlock + threadIdx.x;
+  {
+    const int bper = (nb + nt - 1) / nt, bbeg = min(nb, t * bper), bend = min(nb, bbeg + bper);
+    int root = -1, n = 0;
+    for (int i = bbeg; i < bend; ++i) {
+      if (bst[i] & 1) continue;
+      const int r = parent[i];
+      if (r != root) {
+        if (root >= 0 && n > 0) atomicAdd(&cntb[root], n);
+        root = r;
+        n = seed[r] != kIntMax ? 1 : 0;
+      } else if (n > 0) {
+        ++n;
+      }
+    }
+    if (root >= 0 && n > 0) atomicAdd(&cntb[root], n);
+  }
+  {
+    const int sper = (used + nt - 1) / nt, sbeg = min(used, t * sper), send = min(used, sbeg + sper);
+    int root = -1, n = 0;
+    for (int base = sbeg; base < send; base += kEndsBatch) {
+      bool ok[kEndsBatch];
+      int la[kEndsBatch], lb[kEndsBatch];
+      touching_ends(d, cb, f0, send, base, ok, la, lb, 1);
+      int x[kEndsBatch];
+#pragma unroll
+      for (int u = 0; u < kEndsBatch; ++u) x[u] = ok[u] && (bst[la[u]] & 1) ? lb[u] : la[u];
+      int px[kEndsBatch];
+#pragma unroll
+      for (int u = 0; u < kEndsBatch; ++u) px[u] = ok[u] ? parent[x[u]] : 0;
+      int sd[kEndsBatch];
+#pragma unroll
+      for (int u = 0; u < kEndsBatch; ++u) sd[u] = ok[u] ? seed[px[u]] : kIntMax;
+#pragma unroll
+      for (int u = 0; u < kEndsBatch; ++u) {
+        if (!ok[u] || sd[u] == kIntMax) continue;
+        if (px[u] != root) {
+          if (root >= 0) atomicAdd(&cntc[root], n);
+          root = px[u];
+          n = 0;
+        }
+        ++n;
+      }
+    }
+    if (root >= 0 && n > 0) atomicAdd(&cntc[root], n);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_cap, int smem_dyn) {
   __shared__ int scan[kBigBlock];
@@ -3211,74 +3303,8 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
   int* isl_c = d.i_contacts + cb;
   if (tid < 8) sh[tid] = 0;
   __syncthreads();
-  for (int i = tid; i < nb; i += kBigBlock)
-    if (!(bst[i] & 1)) parent[i] = wake[i];
-  __syncthreads();
-  // --- islands: components with an awake, active, non-static body; seed = the first such body ---
-  // Each thread takes a contiguous run of bodies (then of slots) and adds to a component's
-  // seed and counts once per run of equal roots: the bodies and contacts of one component
-  // mostly sit together, and one atomic per body on a few hot roots serialises in L2.
-  const int bper = (nb + kBigBlock - 1) / kBigBlock, bbeg = tid * bper, bend = min(nb, bbeg + bper);
-  {
-    int root = -1, first = kIntMax;
-    for (int i = bbeg; i < bend; ++i) {
-      const int bs = bst[i];
-      if ((bs & 1) || !(bs & 2) || !(bs & 8)) continue;
-      const int r = parent[i];
-      if (r != root) {
-        if (root >= 0) atomicMin(&seed[root], first);
-        root = r;
-        first = i;
-      }
-    }
-    if (root >= 0) atomicMin(&seed[root], first);
-  }
-  __syncthreads();
-  {
-    int root = -1, n = 0;
-    for (int i = bbeg; i < bend; ++i) {
-      if (bst[i] & 1) continue;
-      const int r = parent[i];
-      if (r != root) {
-        if (root >= 0 && n > 0) atomicAdd(&cntb[root], n);
-        root = r;
-        n = seed[r] != kIntMax ? 1 : 0;
-      } else if (n > 0) {
-        ++n;
-      }
-    }
-    if (root >= 0 && n > 0) atomicAdd(&cntb[root], n);
-  }
-  {
-    const int sper = (used + kBigBlock - 1) / kBigBlock, sbeg = tid * sper, send = min(used, sbeg + sper);
-    int root = -1, n = 0;
-    for (int base = sbeg; base < send; base += kEndsBatch) {
-      bool ok[kEndsBatch];
-      int la[kEndsBatch], lb[kEndsBatch];
-      touching_ends(d, cb, f0, send, base, ok, la, lb, 1);
-      int x[kEndsBatch];
-#pragma unroll
-      for (int u = 0; u < kEndsBatch; ++u) x[u] = ok[u] && (bst[la[u]] & 1) ? lb[u] : la[u];
-      int px[kEndsBatch];
-#pragma unroll
-      for (int u = 0; u < kEndsBatch; ++u) px[u] = ok[u] ? parent[x[u]] : 0;
-      int sd[kEndsBatch];
-#pragma unroll
-      for (int u = 0; u < kEndsBatch; ++u) sd[u] = ok[u] ? seed[px[u]] : kIntMax;
-#pragma unroll
-      for (int u = 0; u < kEndsBatch; ++u) {
-        if (!ok[u] || sd[u] == kIntMax) continue;
-        if (px[u] != root) {
-          if (root >= 0) atomicAdd(&cntc[root], n);
-          root = px[u];
-          n = 0;
-        }
-        ++n;
-      }
-    }
-    if (root >= 0 && n > 0) atomicAdd(&cntc[root], n);
-  }
-  __syncthreads();
+  // --- islands: components with an awake, active, non-static body; seed = the first such body
+  // (k_islands_big_seeds and k_islands_big_counts) ---
   unsigned long long* alloc = (unsigned long long*)(d.g_wmisc + 16 * w);
   if (tid == 0) *alloc = 0ull;
   __syncthreads();
@@ -5609,6 +5635,8 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
       hipLaunchKernelGGL(k_islands_big_scan<T>, dim3(nbig), dim3(kBigBlock), 0, st, d);
       hipLaunchKernelGGL(k_islands_big_fill<T>, dim3(kBigEndsGrid, nbig), dim3(kBlock), 0, st, d);
       hipLaunchKernelGGL(k_islands_big_sort<T>, dim3(kBigEndsGrid, nbig), dim3(kBlock), 0, st, d);
+      hipLaunchKernelGGL(k_islands_big_seeds<T>, dim3(kBigEndsGrid, nbig), dim3(kBlock), 0, st, d);
+      hipLaunchKernelGGL(k_islands_big_counts<T>, dim3(kBigEndsGrid, nbig), dim3(kBlock), 0, st, d);
       hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), ctx->smem_big, st, d,
                          (int)(ctx->smem_big / (kBigBlock / 64 * 2 * sizeof(int))), (int)ctx->smem_big);
     }
