@@ -3003,13 +3003,14 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// The large-world island search, in eight launches: k_islands_big_head (one workgroup per
+// The large-world island search, in nine launches: k_islands_big_head (one workgroup per
 // world) sets up the per-body state and runs the awake gating; k_islands_big_ends (many
 // workgroups per world) takes the touching contacts' ends, the bodies' degrees and the
 // union-find over them; k_islands_big_scan (one) the adjacency offsets; k_islands_big_fill
 // and k_islands_big_sort (many) the adjacency in slot order and the component roots;
 // k_islands_big_seeds and k_islands_big_counts (many) the islands' seeds and sizes; and
-// k_islands_big (one workgroup per world) the islands' ranges, the walks and the rest.
+// k_islands_big (one workgroup per world) the islands' ranges, the walks, the level sorts and
+// the queue; and k_islands_big_tail (many) the island contacts' records and the bodies' flags.
 template <typename T>
 __global__ void __launch_bounds__(kBigBlock) k_islands_big_head(Dev<T> d) {
   __shared__ int sh[8];
@@ -3506,7 +3507,31 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
     d.q_nc[qbase + k] = in.w;
     atomicAdd(&scan[queue_class(d, in.w & 0x3FFFFFFF, in.y & 0xFFFF)], 1);
   }
-  for (int k = tid; k < nci_tot; k += kBigBlock) {
+  __syncthreads();
+  if (tid < kSizeClasses && scan[tid] > 0) atomicAdd(&d.q_cls[tid], scan[tid]);
+  if (tid == 0) {
+    d.w_nisl[w] = nisl;
+    d.w_itot[w] = make_int2(nbi_tot, nci_tot);
+  }
+  (void)ccap;
+}
+
+// The island contacts' ends (island-local bodies, or -1 - body for a static one), fixtures and
+// bodies, island body start, and ranks oriented as (body A, body B) with the bodies' degrees;
+// the bodies' woken and island flags.  Grid (kBigEndsGrid, big worlds) x kBlock.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_islands_big_tail(Dev<T> d) {
+  const int w = d.big_list[blockIdx.y];
+  const int b0 = d.w_body_off[w], nb = d.w_body_off[w + 1] - b0;
+  const int f0 = d.w_fix_off[w], cb = d.w_con_base[w];
+  const int* bst = d.g_bst + b0;
+  const int* wake = d.g_wake + b0;  // island body starts (the walks)
+  const int* cur = d.g_cur + b0;
+  const int* bidx = d.g_bidx + b0;
+  const int* isl_c = d.i_contacts + cb;
+  const int nci_tot = (int)(*(const unsigned long long*)(d.g_wmisc + 16 * w) & 0xffffffffu);
+  const int nt = gridDim.x * kBlock, t = blockIdx.x * kBlock + threadIdx.x;
+  for (int k = t; k < nci_tot; k += nt) {
     const int2 pr = d.c_pair[cb + isl_c[k]];
     const int la = d.f_info[f0 + pr.x].x, lb = d.f_info[f0 + pr.y].x;
     const int ia = (bst[la] & 1) ? -1 - la : bidx[la];
@@ -3520,9 +3545,7 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
     const uint32_t dgb = ib >= 0 ? (uint32_t)cur[lb] & 0xFFFFu : 0u;
     d.i_rank[cb + k] = make_uint2((int)r.x == la ? (lo | hi << 16) : (hi | lo << 16), dga | dgb << 16);
   }
-  __syncthreads();
-  if (tid < kSizeClasses && scan[tid] > 0) atomicAdd(&d.q_cls[tid], scan[tid]);
-  for (int i = tid; i < nb; i += kBigBlock) {
+  for (int i = t; i < nb; i += nt) {
     const int bs = bst[i];
     int fl = d.b_flags[b0 + i];
     if (bs & 32) {  // woken: Body.awake setter (body.py:800-812)
@@ -3532,11 +3555,6 @@ __global__ void __launch_bounds__(kBigBlock) k_islands_big(Dev<T> d, int sort_ca
     }
     d.b_island[b0 + i] = ((bs & 16) && !(bs & 1)) ? 1 : -1;
   }
-  if (tid == 0) {
-    d.w_nisl[w] = nisl;
-    d.w_itot[w] = make_int2(nbi_tot, nci_tot);
-  }
-  (void)ccap;
 }
 
 // Move buffer and the hash of existing contacts of each large world.
@@ -5639,6 +5657,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
       hipLaunchKernelGGL(k_islands_big_counts<T>, dim3(kBigEndsGrid, nbig), dim3(kBlock), 0, st, d);
       hipLaunchKernelGGL(k_islands_big<T>, dim3(nbig), dim3(kBigBlock), ctx->smem_big, st, d,
                          (int)(ctx->smem_big / (kBigBlock / 64 * 2 * sizeof(int))), (int)ctx->smem_big);
+      hipLaunchKernelGGL(k_islands_big_tail<T>, dim3(kBigEndsGrid, nbig), dim3(kBlock), 0, st, d);
     }
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
