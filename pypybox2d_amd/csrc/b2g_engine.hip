@@ -663,6 +663,12 @@ __global__ void __launch_bounds__(B) k_collide(Dev<T> d, int chunks) {
 // ---------------------------------------------------------------------------
 // per-world island search
 // ---------------------------------------------------------------------------
+// The 32-bit LDS address of a pointer into the workgroup's LDS (for LDS reads written in inline
+// assembly).
+template <typename P>
+__device__ __forceinline__ uint32_t lds_addr(P* p) {
+  return (uint32_t)(uintptr_t)((__attribute__((address_space(3))) P*)p);
+}
 struct IslandSmem {
   int* off;        // nb + 1
   int* cur;        // nb
@@ -941,15 +947,25 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(5))) k_i
         const int kend = L.off[b + 1];
         uint32_t e = L.off[b] < kend ? L.adj[L.off[b]] : 0u;
         for (int k = L.off[b]; k < kend; ++k) {
-          const uint32_t en = k + 1 < kend ? L.adj[k + 1] : 0u;
           const int s = (int)(e & 0xFFFFu), o = (int)(e >> 16);
-          e = en;
           const uint32_t bit = 1u << (s & 31);
-          // relaxed atomic loads are not moved below the test, so the four reads share one wait
-          const uint32_t fw = __hip_atomic_load(&L.cflag[s >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          const uint8_t so = __hip_atomic_load(&L.bst[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          const int co = __hip_atomic_load(&L.cur[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          const int lo_read = __hip_atomic_load(&L.lev[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          // the entry's contact flag word, the other body's state, rank counter and level, and
+          // the body's next entry: five LDS reads issued together and waited for once
+          uint32_t fw, so32, co32, lo32;
+          asm volatile(
+              "ds_read_b32 %0, %5\n\t"
+              "ds_read_u8 %1, %6\n\t"
+              "ds_read_b32 %2, %7\n\t"
+              "ds_read_u16 %3, %8\n\t"
+              "ds_read_b32 %4, %9\n\t"
+              "s_waitcnt lgkmcnt(0)"
+              : "=&v"(fw), "=&v"(so32), "=&v"(co32), "=&v"(lo32), "=&v"(e)
+              : "v"(lds_addr(&L.cflag[s >> 5])), "v"(lds_addr(&L.bst[o])), "v"(lds_addr(&L.cur[o])),
+                "v"(lds_addr(&L.lev[o])), "v"(lds_addr(&L.adj[k + 1 < kend ? k + 1 : k]))
+              : "memory");
+          const uint8_t so = (uint8_t)so32;
+          const int co = (int)co32;
+          const int lo_read = (int)lo32;
           if (fw & bit) continue;
           L.cflag[s >> 5] = fw | bit;
           const bool o_dyn = !(so & 1);
