@@ -365,7 +365,7 @@ def cpu_baseline(state: L.BatchArrays, steps: int, seconds: float, procs: int):
 
 
 # the kernels of each timed stage (the stage names are those of the stage's main kernel)
-STAGE_KERNELS = {"k_collide": ("k_reset_queue", "k_collide"), "k_islands": ("k_islands", "k_order_islands"),
+STAGE_KERNELS = {"k_collide": ("k_reset_queue", "k_collide"), "k_islands": ("k_islands", "k_order_islands", "k_order_team_islands"),
                  "k_solve": ("k_island_prefix", "k_prep_bodies", "k_prep_contacts", "k_solve"), "k_sync": ("k_sync",),
                  "k_pairs": ("k_pairs",)}
 

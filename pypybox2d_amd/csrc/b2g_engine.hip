@@ -4650,6 +4650,47 @@ __global__ void k_order_islands(Dev<T> d) {
   }
 }
 
+// Orders each team class's range of the queue by the passes its islands need,
+// longest first: a team lane walks its sweeps in steps of the island's sweep
+// period, and owns a team size's share of the contacts of each sweep, so the
+// larger of the two estimates the island's length.  When a class holds more
+// islands than the chip has wavefront slots, its last wavefronts to start
+// then carry its shortest islands.  One workgroup per class, a counting sort
+// through the unordered queue (free once k_order_islands has read it).
+// Scheduling only: results do not depend on it.
+constexpr int kTeamSortBins = 2048;
+template <typename T>
+__global__ void __launch_bounds__(kPrefixBlock) k_order_team_islands(Dev<T> d) {
+  __shared__ int bins[kTeamSortBins], scan[kPrefixBlock];
+  const int c = blockIdx.x;
+  if (c < d.heavy_class) return;
+  const int b = d.q_cls[kQBase + c], n = d.q_cls[c];
+  if (n < 2) return;
+  const int TS = team_size(c, d.team_full_class, d.team_max);
+  auto bin_of = [&](const int4& in, int nc) {
+    const int period = (int)((uint32_t)in.z >> 16), per_lane = ((nc & 0x3FFFFFFF) + TS - 1) / TS;
+    const int est = period > per_lane ? period : per_lane;
+    return kTeamSortBins - 1 - (est < kTeamSortBins - 1 ? est : kTeamSortBins - 1);
+  };
+  for (int i = threadIdx.x; i < kTeamSortBins; i += kPrefixBlock) bins[i] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kPrefixBlock) atomicAdd(&bins[bin_of(d.qs_isl[b + i], d.qs_nc[b + i])], 1);
+  __syncthreads();
+  block_exclusive_scan<kPrefixBlock>(bins, kTeamSortBins, scan);
+  for (int i = threadIdx.x; i < n; i += kPrefixBlock) {
+    const int4 in = d.qs_isl[b + i];
+    const int nc = d.qs_nc[b + i];
+    const int pos = b + atomicAdd(&bins[bin_of(in, nc)], 1);
+    d.q_isl[pos] = in;
+    d.q_nc[pos] = nc;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kPrefixBlock) {
+    d.qs_isl[b + i] = d.q_isl[b + i];
+    d.qs_nc[b + i] = d.q_nc[b + i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // parity seams
 // ---------------------------------------------------------------------------
@@ -5677,6 +5718,7 @@ static int launch_step(b2g_ctx* ctx, double dt, int vi, int pi, bool first_pairs
     }
     int nblk = std::max(1, std::min((ctx->nb + kBlock - 1) / kBlock, 1024));
     hipLaunchKernelGGL(k_order_islands<T>, dim3(nblk), dim3(kBlock), 0, st, d);
+    hipLaunchKernelGGL(k_order_team_islands<T>, dim3(kSizeClasses), dim3(kPrefixBlock), 0, st, d);
     if (timed) HIPCHK(hipEventRecord(ev[2], st));
     // the island solve: per-body and per-contact setup over all islands, then the islands
     hipLaunchKernelGGL(k_island_prefix<T>, dim3(1), dim3(kPrefixBlock), 0, st, d);
