@@ -115,6 +115,13 @@ __host__ __device__ inline int team_size(int cls, int full_cls, int tmax) {
   return cls == kSizeClasses - 1 ? kSolveBlock : (ts < tmax ? ts : tmax);
 }
 
+// The length of a team island in lane passes per sweep, roughly: a lane walks its sweeps in
+// steps of the island's sweep period and owns a team size's share of each sweep's contacts.
+__host__ __device__ inline int team_island_length(const int4& in, int nc, int TS) {
+  const int period = (int)((uint32_t)in.z >> 16), per_lane = ((nc & 0x3FFFFFFF) + TS - 1) / TS;
+  return period > per_lane ? period : per_lane;
+}
+
 __host__ __device__ inline int team_lds_bytes(int nbi) {
   return 24 * nbi + 4 * ((nbi + 1) & ~1);  // 3 doubles and an int counter per body
 }
@@ -4668,8 +4675,7 @@ __global__ void __launch_bounds__(kPrefixBlock) k_order_team_islands(Dev<T> d) {
   if (n < 2) return;
   const int TS = team_size(c, d.team_full_class, d.team_max);
   auto bin_of = [&](const int4& in, int nc) {
-    const int period = (int)((uint32_t)in.z >> 16), per_lane = ((nc & 0x3FFFFFFF) + TS - 1) / TS;
-    const int est = period > per_lane ? period : per_lane;
+    const int est = team_island_length(in, nc, TS);
     return kTeamSortBins - 1 - (est < kTeamSortBins - 1 ? est : kTeamSortBins - 1);
   };
   for (int i = threadIdx.x; i < kTeamSortBins; i += kPrefixBlock) bins[i] = 0;
